@@ -1,0 +1,128 @@
+/*
+ * uclbs_hip.h -- C ABI of libuclbs_hip.so, the MI355X (gfx950) ELBM
+ * collide-stream library that sits behind the UCLBS Fortran time loop.
+ *
+ * Boundary being replaced.  The reference advances one AMR level through six
+ * Fortran phase calls in looping/timestepping/loop.f95:104-139,
+ *     CALL streaming(level_step,fine,lvl)                 loop.f95:105
+ *     CALL calculate_flow_variables(level_step,fine,lvl)  loop.f95:112
+ *     CALL interlevel_interpolation(lvl,fine)             loop.f95:119
+ *     CALL fine_to_coarse(lvl,fine)                       loop.f95:126
+ *     CALL collision(level_step,fine,lvl)                 loop.f95:133
+ *     CALL damp_it(level_step,fine,lvl)                   loop.f95:138
+ * each of which sweeps the module-global AMReX MultiFabs of
+ * amr/modules/amr_info_holder.f95:50-76 (fi, fout, gi, gout, state, rho,
+ * u/v/w_vel, temp, alpha, the 13 Lagrange multipliers).  This library keeps
+ * that level state resident in HBM and advances it with fused kernels; the
+ * Fortran side reaches it through iso_c_binding (fortran/uclbs_hip_mod.f90).
+ *
+ * Conventions.
+ *  - Every call returns 0 on success or a negative UCLBS_E_* code;
+ *    uclbs_last_error() gives the message (the reference's error_out + STOP,
+ *    general/background_calcs/error_out.f95:11-14, becomes a returned code).
+ *  - Host arrays are FP64 in the reference layout: component-major,
+ *    x fastest, i.e. an AMReX FAB fi(a,b,c,q) restricted to the box's valid
+ *    nodes: f[q*n + (a-lo0) + nx*((b-lo1) + ny*(c-lo2))].
+ *  - Velocity index q follows linkwise.f95:30-37 for D3Q27 (0-based).
+ *  - The library owns one HIP stream per context; calls are synchronous
+ *    unless their name ends in _async.
+ *  - A context lives on the device that is current when it is created.
+ */
+#ifndef UCLBS_HIP_H
+#define UCLBS_HIP_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UCLBS_ABI_VERSION 1
+
+enum {
+  UCLBS_OK = 0,
+  UCLBS_E_ARG = -1,      /* bad argument */
+  UCLBS_E_HIP = -2,      /* HIP runtime error */
+  UCLBS_E_STATE = -3,    /* call out of order (e.g. level not defined) */
+  UCLBS_E_NOMEM = -4,    /* device allocation failed */
+  UCLBS_E_NUMERIC = -5,  /* NaN or non-positive density detected */
+  UCLBS_E_UNSUPPORTED = -6
+};
+
+enum { UCLBS_LATTICE_D3Q27 = 27, UCLBS_LATTICE_D3Q39 = 39 };
+enum { UCLBS_PREC_FP32_SHIFTED = 0, UCLBS_PREC_FP64 = 1 };
+enum {
+  UCLBS_ALPHA_NEWTON = 0,   /* entropic Newton root-find from alpha = 2 (plain) */
+  UCLBS_ALPHA_FIXED2 = 1,   /* alpha = 2 (LBGK), reference default turbulent=.false.
+                               (calculate_flow_variables.f95:306-311) */
+  UCLBS_ALPHA_LEARNED = 2   /* MLP initial guess, Newton, plain fallback */
+};
+
+/* Run configuration (the hot-path fields of the reference's &GLOBAL namelist,
+ * general/namelist/namelist_global_read.f95:30-85). */
+typedef struct uclbs_config {
+  int32_t lattice;      /* UCLBS_LATTICE_* */
+  int32_t precision;    /* UCLBS_PREC_* */
+  int32_t alpha_mode;   /* UCLBS_ALPHA_* */
+  int32_t alpha_maxit;  /* Newton cap (reference has none); 0 -> 30 */
+  double alpha_tol;     /* |g(alpha)| <= tol * sum f x^2; 0 -> 1e-10 */
+  double nu;            /* kinematic viscosity, lattice units: beta = 1/(6 nu + 1) */
+  int32_t reserved[8];
+} uclbs_config;
+
+/* Counters accumulated on the device by every step. */
+typedef struct uclbs_stats {
+  int64_t node_updates;    /* nodes collided */
+  int64_t newton_iters;    /* sum of Newton updates over nodes */
+  int64_t n_trivial;       /* nodes at equilibrium (alpha = 2, no solve) */
+  int64_t n_bound;         /* alpha clamped to the positivity bound */
+  int64_t n_nonpos;        /* nodes with a non-positive population (alpha = 2) */
+  int64_t n_noconv;        /* Newton hit the cap */
+  int64_t n_fallback;      /* learned guess rejected, plain root-find used */
+  int64_t n_logpath;       /* nodes whose alpha needed direct logarithms */
+} uclbs_stats;
+
+const char *uclbs_last_error(void);
+int uclbs_abi_version(void);
+
+/* Context: uclbs_init of the SURVEY-proposed ABI.  Replaces the setup that
+ * calcs_before_init/background_calcs perform (uclbs_main.f95:28-108). */
+int uclbs_create(const uclbs_config *cfg, void **ctx);
+int uclbs_destroy(void *ctx);
+
+/* One rectangular nodal box per level: valid nodes lo..hi inclusive (AMReX
+ * box convention, amr_processes.f95:383-466), periodic[d] != 0 wraps axis d.
+ * Allocates two population buffers (A/B) in the storage precision. */
+int uclbs_define_level(void *ctx, int lvl, const int32_t lo[3], const int32_t hi[3],
+                       const int32_t periodic[3]);
+
+/* Populations in and out (FP64 host, full f, reference layout). */
+int uclbs_upload_f(void *ctx, int lvl, const double *f, int64_t n_nodes);
+int uclbs_download_f(void *ctx, int lvl, double *f, int64_t n_nodes);
+/* rho, u, v, w of the current populations (moments are conserved by the
+ * collision, so these equal the fields calculate_flow_variables.f95:201-228
+ * computes before the next collision). */
+int uclbs_download_macro(void *ctx, int lvl, double *rho, double *u, double *v, double *w);
+/* alpha of the last collision per node (requires uclbs_set_keep_alpha(1)). */
+int uclbs_set_keep_alpha(void *ctx, int keep);
+int uclbs_download_alpha(void *ctx, int lvl, double *alpha);
+
+/* Initialisers (looping/initialization/init_fi.f95:41 fills fi/fout with an
+ * equilibrium; the Taylor-Green field is the benchmark's own). */
+int uclbs_init_tgv(void *ctx, int lvl, double u0, double noise_amp, uint64_t seed);
+
+/* Advance level lvl by nsteps fused stream+moments+alpha+collide steps:
+ * the streaming / calculate_flow_variables / collision calls of
+ * loop.f95:105,112,133 for one level, in that order. */
+int uclbs_step_level(void *ctx, int lvl, int nsteps);
+int uclbs_step_level_async(void *ctx, int lvl, int nsteps);
+/* Same as uclbs_step_level, and returns the elapsed time of the launches as
+ * measured by HIP events recorded on the library's stream. */
+int uclbs_time_steps(void *ctx, int lvl, int nsteps, double *elapsed_ms);
+int uclbs_sync(void *ctx);
+
+int uclbs_get_stats(void *ctx, uclbs_stats *out);
+int uclbs_reset_stats(void *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

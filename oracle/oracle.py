@@ -1,0 +1,176 @@
+"""ctypes binding of the FP64 CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker, never as the measured or
+shipped path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_DIR, "liboracle.so")
+
+
+def build(force: bool = False) -> None:
+    subprocess.run(["make", "-C", _DIR] + (["-B"] if force else []), check=True,
+                   stdout=subprocess.DEVNULL)
+
+
+class AlphaResult(ctypes.Structure):
+    _fields_ = [("alpha", ctypes.c_double), ("iters", ctypes.c_int), ("status", ctypes.c_int),
+                ("resid", ctypes.c_double)]
+
+
+class Params27(ctypes.Structure):
+    _fields_ = [("beta", ctypes.c_double), ("tol", ctypes.c_double), ("maxit", ctypes.c_int),
+                ("alpha_mode", ctypes.c_int), ("nthreads", ctypes.c_int)]
+
+
+class OStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in ("iters", "nodes", "n_bound", "n_nonpos",
+                                              "n_trivial", "n_noconv")]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        D, I = ctypes.c_double, ctypes.c_int
+        L.o27_feq.argtypes = [D, D, D, D, dp]
+        L.o27_moments.argtypes = [dp, dp, dp]
+        L.o27_alpha.argtypes = [dp, dp, D, D, I]
+        L.o27_alpha.restype = AlphaResult
+        L.o27_alpha_series.argtypes = [dp, dp, D, D, I, I]
+        L.o27_alpha_series.restype = AlphaResult
+        L.o27_entropy_g.argtypes = [dp, dp, D, dp, dp]
+        L.o27_collide_node.argtypes = [dp, D, D, D, I, I, dp, dp, dp]
+        L.o27_collide_node.restype = AlphaResult
+        L.o27_step_periodic.argtypes = [I, I, I, dp, dp, ctypes.POINTER(Params27), dp,
+                                        ctypes.POINTER(OStats)]
+        L.o27_init_tgv.argtypes = [I, I, I, D, D, ctypes.c_uint64, dp]
+        L.o27_macro.argtypes = [I, dp, dp, dp, dp, dp]
+        L.o_philox4x32_10.argtypes = [ctypes.POINTER(ctypes.c_uint32),
+                                      ctypes.POINTER(ctypes.c_uint32),
+                                      ctypes.POINTER(ctypes.c_uint32)]
+        L.o_uniform.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64]
+        L.o_uniform.restype = D
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _table(name, n):
+    return np.array((ctypes.c_int * n).in_dll(lib(), name)[:], dtype=np.int64)
+
+
+def lattice27():
+    L = lib()
+    w = np.array((ctypes.c_double * 27).in_dll(L, "O27_W")[:])
+    return (_table("O27_CX", 27), _table("O27_CY", 27), _table("O27_CZ", 27),
+            _table("O27_OPP", 27), w)
+
+
+def lattice39():
+    return (_table("O39_CX", 39), _table("O39_CY", 39), _table("O39_CZ", 39),
+            _table("O39_OPP", 39))
+
+
+def philox(ctr, key):
+    out = (ctypes.c_uint32 * 4)()
+    lib().o_philox4x32_10((ctypes.c_uint32 * 4)(*ctr), (ctypes.c_uint32 * 2)(*key), out)
+    return list(out)
+
+
+def uniform(node, comp, seed):
+    return lib().o_uniform(node, comp, seed)
+
+
+def feq27(rho, ux, uy, uz):
+    out = np.empty(27)
+    lib().o27_feq(rho, ux, uy, uz, _dp(out))
+    return out
+
+
+def moments27(f):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    r = ctypes.c_double()
+    u = np.empty(3)
+    lib().o27_moments(_dp(f), ctypes.byref(r), _dp(u))
+    return r.value, u
+
+
+def alpha27(f, feq, alpha0=0.0, tol=1e-10, maxit=30, series_order=0):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    feq = np.ascontiguousarray(feq, dtype=np.float64)
+    if series_order:
+        r = lib().o27_alpha_series(_dp(f), _dp(feq), alpha0, tol, maxit, series_order)
+    else:
+        r = lib().o27_alpha(_dp(f), _dp(feq), alpha0, tol, maxit)
+    return r.alpha, r.iters, r.status, r.resid
+
+
+def entropy_g27(f, feq, a):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    feq = np.ascontiguousarray(feq, dtype=np.float64)
+    g, gp = ctypes.c_double(), ctypes.c_double()
+    lib().o27_entropy_g(_dp(f), _dp(feq), a, ctypes.byref(g), ctypes.byref(gp))
+    return g.value, gp.value
+
+
+def collide27(f, beta, alpha0=0.0, tol=1e-10, maxit=30, fixed=False):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    out = np.empty(27)
+    rho = ctypes.c_double()
+    u = np.empty(3)
+    r = lib().o27_collide_node(_dp(f), beta, alpha0, tol, maxit, int(fixed), _dp(out),
+                               ctypes.byref(rho), _dp(u))
+    return out, r.alpha, r.iters, r.status
+
+
+def init_tgv27(shape, u0, noise_amp, seed):
+    nx, ny, nz = shape
+    f = np.empty((27, nx * ny * nz))
+    lib().o27_init_tgv(nx, ny, nz, u0, noise_amp, seed, _dp(f))
+    return f
+
+
+def step_periodic27(shape, f, beta, nsteps=1, tol=1e-10, maxit=30, fixed=False,
+                    nthreads=0, want_alpha=True):
+    """Advance f (27, n) by nsteps fused steps; returns (f, alpha, stats)."""
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    a = np.ascontiguousarray(f, dtype=np.float64).reshape(27, n).copy()
+    b = np.empty_like(a)
+    alpha = np.empty(n) if want_alpha else None
+    p = Params27(beta, tol, maxit, int(fixed), nthreads)
+    st = OStats()
+    for _ in range(nsteps):
+        lib().o27_step_periodic(nx, ny, nz, _dp(a), _dp(b), ctypes.byref(p),
+                                _dp(alpha) if want_alpha else None, ctypes.byref(st))
+        a, b = b, a
+    return a, alpha, st.as_dict()
+
+
+def macro27(f):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    n = f.shape[1]
+    out = [np.empty(n) for _ in range(4)]
+    lib().o27_macro(n, _dp(f), *[_dp(o) for o in out])
+    return out

@@ -1,0 +1,90 @@
+/*
+ * uclbs_oracle.h -- FP64 CPU oracle for the ELBM collide-stream path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP
+ * product path is compared against.  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg may load it; the product library never
+ * links or calls it.
+ *
+ * Every routine restates an algorithm of the UCLBS reference (Fortran,
+ * v0.08) or, where the reference has no implementation (D3Q27), the
+ * published method the build uses in its place.  Each definition below
+ * names the reference file:line it follows.  All arithmetic is FP64, as in
+ * the reference (general/modules/precise.f95:12).
+ *
+ * Population layout used by every grid routine: structure-of-arrays
+ * f[q * n_nodes + node], node = x + nx*(y + ny*z)  (x fastest), which is
+ * the component-major Fortran order of an AMReX FAB fi(a,b,c,q) without
+ * ghost cells.
+ */
+#ifndef UCLBS_ORACLE_H
+#define UCLBS_ORACLE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------- lattices (general/background_calcs/directions_and_weights.f95) */
+/* D3Q27 velocity order of linking/modules/linkwise.f95:30-37 (cx_27 .. opp_27),
+ * 0-based here.  Weights are the tensor products of {1/6, 2/3, 1/6}. */
+extern const int O27_CX[27], O27_CY[27], O27_CZ[27], O27_OPP[27];
+extern const double O27_W[27];
+/* D3Q39 (directions_and_weights.f95:436-444; opp :498-501; erp linkwise.f95:39-41) */
+extern const int O39_CX[39], O39_CY[39], O39_CZ[39], O39_OPP[39], O39_ERP[39];
+
+/* ---------------- Philox4x32-10 (Salmon et al., SC'11) ----------------------- */
+void o_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* uniform double in [0,1) from (node, component, seed): 53 bits of out[0..1] */
+double o_uniform(uint64_t node, uint32_t comp, uint64_t seed);
+
+/* ---------------- D3Q27 entropic LBM (no reference implementation) ------------ */
+/* Product-form entropic equilibrium (Ansumali, Karlin & Ottinger 2003). */
+void o27_feq(double rho, double ux, double uy, double uz, double feq[27]);
+void o27_moments(const double f[27], double *rho, double u[3]);
+
+/* Entropic alpha: nontrivial root of H(f + a*D) = H(f), H = sum f ln(f/w),
+ * D = feq - f.  Newton definition: calculate_alpha.f95:416-556 (commented-out
+ * in the reference; positivity bracket :424-445, "alpha_max < 2" rule :449-455).
+ * Status codes (also used by the GPU path):                                  */
+enum { O_ALPHA_NEWTON = 0, O_ALPHA_TRIVIAL = 1, O_ALPHA_BOUND = 2,
+       O_ALPHA_NONPOS = 3, O_ALPHA_NOCONV = 4 };
+typedef struct { double alpha; int iters; int status; double resid; } o_alpha_result;
+/* alpha0 <= 0 means the plain initial guess 2.  tol: |g(a)| <= tol * S2. */
+o_alpha_result o27_alpha(const double f[27], const double feq[27], double alpha0,
+                         double tol, int maxit);
+/* Same root via the power-sum form of g (used to check the GPU's evaluation). */
+o_alpha_result o27_alpha_series(const double f[27], const double feq[27], double alpha0,
+                                double tol, int maxit, int order);
+/* g(a) and g'(a) of the entropy equality in the conditioned form (log path). */
+void o27_entropy_g(const double f[27], const double feq[27], double a, double *g, double *gp);
+/* Direct H(f) = sum f ln(f/w) (used by tests in long double). */
+long double o27_H_ld(const long double f[27]);
+
+/* one node: stream result f -> post-collision fout; returns alpha result */
+o_alpha_result o27_collide_node(const double f[27], double beta, double alpha0, double tol,
+                                int maxit, int fixed_alpha_mode, double fout[27],
+                                double *rho, double u[3]);
+
+/* Periodic fused step on an nx*ny*nz grid: fout_new(x) = collide(pull(fin)).
+ * alpha_out, iters_out may be NULL.  Returns the sum of Newton iterations. */
+typedef struct {
+  double beta;          /* 1/(6 nu + 1) */
+  double tol;           /* alpha residual tolerance */
+  int maxit;            /* Newton cap */
+  int alpha_mode;       /* 0 = entropic Newton (plain), 1 = fixed alpha = 2 (LBGK) */
+  int nthreads;         /* OpenMP threads, 0 = default */
+} o27_params;
+typedef struct { int64_t iters; int64_t nodes; int64_t n_bound; int64_t n_nonpos;
+                 int64_t n_trivial; int64_t n_noconv; } o_stats;
+void o27_step_periodic(int nx, int ny, int nz, const double *fin, double *fout,
+                       const o27_params *p, double *alpha_out, o_stats *st);
+/* Taylor-Green vortex initial state: f = feq(1, u), u = U0*(sin x cos y cos z,
+ * -cos x sin y cos z, 0) + U(-amp, amp) noise per component from Philox. */
+void o27_init_tgv(int nx, int ny, int nz, double u0, double noise_amp, uint64_t seed,
+                  double *f);
+void o27_macro(int n, const double *f, double *rho, double *ux, double *uy, double *uz);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,126 @@
+"""D3Q27 on the GPU through the C ABI, against the FP64 oracle on the same seeded
+inputs.  Tolerances (relative L2 over the box) are stated per precision mode:
+  FP64 storage:         f, rho, u  <= 1e-12;  alpha max abs <= 1e-11
+  FP32 shifted storage: u <= 1e-5, rho - 1 <= 1e-5 (relative to the oracle's rho - 1),
+                        alpha max abs <= 1e-6 (C1: TGV 64^3, Ma 0.1, Re 1600, 200 steps)
+"""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SEED = 20260101
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def tgv_params(L, mach=0.1, re=1600.0):
+    u0 = mach / math.sqrt(3.0)
+    nu = u0 * (L / (2 * math.pi)) / re
+    return u0, nu
+
+
+def test_device_tgv_init_matches_oracle(pkg, orc):
+    shape = (16, 12, 8)
+    s = pkg.Solver(precision="fp64", nu=1e-3)
+    s.define_level(0, shape)
+    s.init_tgv(0, 0.05, 1e-4, SEED)
+    f_gpu = s.download_f(0)
+    f_orc = orc.init_tgv27(shape, 0.05, 1e-4, SEED)
+    assert np.abs(f_gpu - f_orc).max() < 1e-15
+
+
+def test_upload_download_roundtrip(pkg, orc):
+    shape = (10, 9, 8)
+    f = orc.init_tgv27(shape, 0.05, 1e-4, 5)
+    for prec, tol in (("fp64", 0.0), ("fp32s", 3e-9)):
+        s = pkg.Solver(precision=prec, nu=1e-3)
+        s.define_level(0, shape)
+        s.upload_f(0, f)
+        assert np.abs(s.download_f(0) - f).max() <= tol
+
+
+def test_fp64_parity_tgv(pkg, orc):
+    shape = (32, 24, 16)
+    u0, nu = 0.08, 2e-3
+    beta = 1 / (6 * nu + 1)
+    f0 = orc.init_tgv27(shape, u0, 1e-4, SEED)
+    steps = 20
+    f_o, a_o, st_o = orc.step_periodic27(shape, f0, beta, nsteps=steps, tol=1e-12)
+    s = pkg.Solver(precision="fp64", nu=nu, alpha_tol=1e-12)
+    s.define_level(0, shape)
+    s.keep_alpha(True)
+    s.upload_f(0, f0)
+    s.step(0, steps)
+    f_g = s.download_f(0)
+    a_g = s.alpha(0)
+    cx, cy, cz, _, w = orc.lattice27()
+    assert rel_l2(f_g - w[:, None], f_o - w[:, None]) < 1e-12
+    r_o, ux_o, uy_o, uz_o = orc.macro27(f_o)
+    r_g, ux_g, uy_g, uz_g = s.macro(0)
+    assert rel_l2(np.stack([ux_g, uy_g, uz_g]), np.stack([ux_o, uy_o, uz_o])) < 1e-12
+    assert np.abs(a_g - a_o).max() < 1e-11
+    st = s.stats()
+    n = shape[0] * shape[1] * shape[2]
+    assert st["node_updates"] == steps * n
+    assert st["n_noconv"] == 0 and st_o["n_noconv"] == 0
+    assert abs(st["newton_iters"] - st_o["iters"]) <= 0.005 * st_o["iters"] + 2
+    print("iters/node gpu", st["newton_iters"] / (steps * n), "oracle", st_o["iters"] / (steps * n))
+
+
+def test_fp32_shifted_c1_tgv(pkg, orc):
+    """C1: periodic TGV 64^3, Ma 0.1, Re 1600, noise 1e-4 (seed 20260101), 200 steps."""
+    L = 64
+    shape = (L, L, L)
+    u0, nu = tgv_params(L)
+    beta = 1 / (6 * nu + 1)
+    f0 = orc.init_tgv27(shape, u0, 1e-4, SEED)
+    steps = 200
+    f_o, a_o, st_o = orc.step_periodic27(shape, f0, beta, nsteps=steps, tol=1e-10)
+    s = pkg.Solver(precision="fp32s", nu=nu, alpha_tol=1e-10)
+    s.define_level(0, shape)
+    s.keep_alpha(True)
+    s.init_tgv(0, u0, 1e-4, SEED)
+    s.step(0, steps)
+    r_o, ux_o, uy_o, uz_o = orc.macro27(f_o)
+    r_g, ux_g, uy_g, uz_g = s.macro(0)
+    a_g = s.alpha(0)
+    e_u = rel_l2(np.stack([ux_g, uy_g, uz_g]), np.stack([ux_o, uy_o, uz_o]))
+    e_r = rel_l2(r_g - 1, r_o - 1)
+    e_a = float(np.abs(a_g - a_o).max())
+    print(f"C1 fp32s vs fp64 oracle: relL2 u {e_u:.3e}  rho-1 {e_r:.3e}  max|alpha| {e_a:.3e}")
+    assert e_u < 1e-5 and e_r < 1e-5 and e_a < 1e-6
+    assert rel_l2(r_g, r_o) < 1e-7
+
+
+def test_lbgk_mode_parity(pkg, orc):
+    shape = (16, 16, 16)
+    nu = 5e-3
+    f0 = orc.init_tgv27(shape, 0.05, 1e-4, 3)
+    f_o, _, _ = orc.step_periodic27(shape, f0, 1 / (6 * nu + 1), nsteps=10, fixed=True)
+    s = pkg.Solver(precision="fp64", nu=nu, alpha_mode="fixed2")
+    s.define_level(0, shape)
+    s.upload_f(0, f0)
+    s.step(0, 10)
+    assert np.abs(s.download_f(0) - f_o).max() < 1e-15
+
+
+def test_mass_momentum_conservation_256(pkg):
+    """Full-size property: 256^3 FP32-shifted, total mass and momentum conserved."""
+    L = 256
+    u0, nu = tgv_params(L)
+    s = pkg.Solver(precision="fp32s", nu=nu)
+    s.define_level(0, (L, L, L))
+    s.init_tgv(0, u0, 1e-4, SEED)
+    r0, ux0, uy0, uz0 = s.macro(0)
+    s.step(0, 20)
+    r1, ux1, uy1, uz1 = s.macro(0)
+    m0, m1 = r0.sum(), r1.sum()
+    assert abs(m1 - m0) / m0 < 1e-9
+    assert abs((r1 * ux1).sum() - (r0 * ux0).sum()) / np.abs(r0 * ux0).sum() < 1e-6
+    st = s.stats()
+    assert st["node_updates"] == 20 * L ** 3 and st["n_noconv"] == 0 and st["n_nonpos"] == 0

@@ -1,0 +1,198 @@
+"""Python host binding of libuclbs_hip.so (ctypes over the C ABI in include/uclbs_hip.h).
+
+The production host of this library is Fortran (fortran/uclbs_hip_mod.f90 binds the
+same symbols through iso_c_binding, as a drop-in for the phase calls of
+looping/timestepping/loop.f95:104-139).  This module binds the identical C ABI
+for tests and the benchmark.  There is no CPU fallback: if the shared library
+is missing or a call fails, an exception is raised.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+_PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_PKG_DIR, "libuclbs_hip.so")
+
+# exported symbols, in the order include/uclbs_hip.h declares them
+EXPORTS = (
+    "uclbs_last_error", "uclbs_abi_version", "uclbs_create", "uclbs_destroy",
+    "uclbs_define_level", "uclbs_upload_f", "uclbs_download_f", "uclbs_download_macro",
+    "uclbs_set_keep_alpha", "uclbs_download_alpha", "uclbs_init_tgv",
+    "uclbs_step_level", "uclbs_step_level_async", "uclbs_time_steps", "uclbs_sync",
+    "uclbs_get_stats", "uclbs_reset_stats",
+)
+
+LATTICE = {"d3q27": 27, "d3q39": 39}
+PRECISION = {"fp32s": 0, "fp64": 1}
+ALPHA_MODE = {"newton": 0, "fixed2": 1, "learned": 2}
+
+
+class UclbsError(RuntimeError):
+    pass
+
+
+class Config(ctypes.Structure):
+    _fields_ = [
+        ("lattice", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("alpha_mode", ctypes.c_int32),
+        ("alpha_maxit", ctypes.c_int32),
+        ("alpha_tol", ctypes.c_double),
+        ("nu", ctypes.c_double),
+        ("reserved", ctypes.c_int32 * 8),
+    ]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in (
+        "node_updates", "newton_iters", "n_trivial", "n_bound", "n_nonpos", "n_noconv",
+        "n_fallback", "n_logpath")]
+
+    def as_dict(self) -> dict:
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
+_lib: Optional[ctypes.CDLL] = None
+
+
+def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise UclbsError(f"{path} is missing: run __graft_entry__.build() (make -C <pkg>)")
+    lib = ctypes.CDLL(path)
+    P, I, I64, D, U64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_uint64
+    i3 = ctypes.POINTER(ctypes.c_int32)
+    dp = ctypes.POINTER(ctypes.c_double)
+    sig = {
+        "uclbs_last_error": (ctypes.c_char_p, []),
+        "uclbs_abi_version": (I, []),
+        "uclbs_create": (I, [ctypes.POINTER(Config), ctypes.POINTER(P)]),
+        "uclbs_destroy": (I, [P]),
+        "uclbs_define_level": (I, [P, I, i3, i3, i3]),
+        "uclbs_upload_f": (I, [P, I, dp, I64]),
+        "uclbs_download_f": (I, [P, I, dp, I64]),
+        "uclbs_download_macro": (I, [P, I, dp, dp, dp, dp]),
+        "uclbs_set_keep_alpha": (I, [P, I]),
+        "uclbs_download_alpha": (I, [P, I, dp]),
+        "uclbs_init_tgv": (I, [P, I, D, D, U64]),
+        "uclbs_step_level": (I, [P, I, I]),
+        "uclbs_step_level_async": (I, [P, I, I]),
+        "uclbs_time_steps": (I, [P, I, I, dp]),
+        "uclbs_sync": (I, [P]),
+        "uclbs_get_stats": (I, [P, ctypes.POINTER(Stats)]),
+        "uclbs_reset_stats": (I, [P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+class Solver:
+    """One library context (one device stream, resident level state)."""
+
+    def __init__(self, lattice: str = "d3q27", precision: str = "fp32s",
+                 alpha_mode: str = "newton", nu: float = 1e-3, alpha_tol: float = 1e-10,
+                 alpha_maxit: int = 30):
+        self.lib = load_library()
+        cfg = Config()
+        cfg.lattice = LATTICE[lattice]
+        cfg.precision = PRECISION[precision]
+        cfg.alpha_mode = ALPHA_MODE[alpha_mode]
+        cfg.alpha_maxit = alpha_maxit
+        cfg.alpha_tol = alpha_tol
+        cfg.nu = nu
+        self.q = cfg.lattice
+        self.ctx = ctypes.c_void_p()
+        self._check(self.lib.uclbs_create(ctypes.byref(cfg), ctypes.byref(self.ctx)))
+        self.shapes = {}
+
+    def _check(self, rc: int):
+        if rc != 0:
+            msg = self.lib.uclbs_last_error().decode(errors="replace")
+            raise UclbsError(f"uclbs call failed ({rc}): {msg}")
+
+    def close(self):
+        if self.ctx:
+            self.lib.uclbs_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def define_level(self, lvl: int, shape: Sequence[int], periodic=(1, 1, 1), lo=(0, 0, 0)):
+        lo3 = (ctypes.c_int32 * 3)(*lo)
+        hi3 = (ctypes.c_int32 * 3)(*[lo[d] + shape[d] - 1 for d in range(3)])
+        per = (ctypes.c_int32 * 3)(*[int(p) for p in periodic])
+        self._check(self.lib.uclbs_define_level(self.ctx, lvl, lo3, hi3, per))
+        self.shapes[lvl] = tuple(int(s) for s in shape)
+
+    def n_nodes(self, lvl: int) -> int:
+        s = self.shapes[lvl]
+        return s[0] * s[1] * s[2]
+
+    def upload_f(self, lvl: int, f: np.ndarray):
+        n = self.n_nodes(lvl)
+        f = np.ascontiguousarray(f, dtype=np.float64).reshape(self.q, n)
+        self._check(self.lib.uclbs_upload_f(self.ctx, lvl, _dptr(f), n))
+
+    def download_f(self, lvl: int) -> np.ndarray:
+        n = self.n_nodes(lvl)
+        out = np.empty((self.q, n), dtype=np.float64)
+        self._check(self.lib.uclbs_download_f(self.ctx, lvl, _dptr(out), n))
+        return out
+
+    def macro(self, lvl: int):
+        n = self.n_nodes(lvl)
+        arrs = [np.empty(n, dtype=np.float64) for _ in range(4)]
+        self._check(self.lib.uclbs_download_macro(self.ctx, lvl, *[_dptr(a) for a in arrs]))
+        return arrs
+
+    def keep_alpha(self, keep: bool = True):
+        self._check(self.lib.uclbs_set_keep_alpha(self.ctx, int(keep)))
+
+    def alpha(self, lvl: int) -> np.ndarray:
+        out = np.empty(self.n_nodes(lvl), dtype=np.float64)
+        self._check(self.lib.uclbs_download_alpha(self.ctx, lvl, _dptr(out)))
+        return out
+
+    def init_tgv(self, lvl: int, u0: float, noise_amp: float, seed: int):
+        self._check(self.lib.uclbs_init_tgv(self.ctx, lvl, u0, noise_amp, seed))
+
+    def step(self, lvl: int, nsteps: int = 1):
+        self._check(self.lib.uclbs_step_level(self.ctx, lvl, nsteps))
+
+    def step_async(self, lvl: int, nsteps: int = 1):
+        self._check(self.lib.uclbs_step_level_async(self.ctx, lvl, nsteps))
+
+    def time_steps(self, lvl: int, nsteps: int) -> float:
+        ms = ctypes.c_double()
+        self._check(self.lib.uclbs_time_steps(self.ctx, lvl, nsteps, ctypes.byref(ms)))
+        return ms.value
+
+    def sync(self):
+        self._check(self.lib.uclbs_sync(self.ctx))
+
+    def stats(self) -> dict:
+        s = Stats()
+        self._check(self.lib.uclbs_get_stats(self.ctx, ctypes.byref(s)))
+        return s.as_dict()
+
+    def reset_stats(self):
+        self._check(self.lib.uclbs_reset_stats(self.ctx))

@@ -1,0 +1,166 @@
+// Entropic over-relaxation parameter alpha: device-side root-finder.
+//
+// alpha is the nontrivial root of the entropy equality H(f + alpha*D) = H(f),
+// D = feq - f, following the Newton definition of the reference
+// (looping/timestepping/calculate_alpha.f95:416-556, commented out there):
+// positivity bracket alpha_max = min_{D_i<0} -f_i/D_i (:424-445), the rule
+// "alpha_max < 2 => alpha = alpha_max" (:449-455) and the clamp to alpha_max
+// (:529-536).  Two changes make it safe on a GPU: a bracket with bisection
+// fallback and an iteration cap, and the entropy function is evaluated in the
+// conditioned form
+//     g(a)  = sum_i f_i [(1 + a x_i) ln(1 + a x_i) - a x_i - a x_i ln(1 + x_i)],
+//     g'(a) = sum_i D_i [ln(1 + a x_i) - ln(1 + x_i)],     x_i = D_i / f_i,
+// which equals H(f + aD) - H(f) when D conserves mass and momentum and
+// ln(feq_i / w_i) is affine in c_i (true for the D3Q27 product-form
+// equilibrium).  The term -a x_i sums to -a sum(D) = 0 in exact arithmetic;
+// keeping it inside phi(y) = (1+y) ln(1+y) - y stops the FP64 rounding of the
+// conserved moments of feq from biasing the root.  For max|x_i| small the same g is evaluated from the power
+// sums S_n = sum_i f_i x_i^n:
+//     g(a)  = sum_{n>=2} (-1)^n S_n (a^n/(n(n-1)) - a/(n-1)),
+//     g'(a) = sum_{n>=2} (-1)^n S_n (a^(n-1) - 1)/(n-1),
+// truncated at an order whose remainder is below FP64 round-off for the
+// node's max|x_i|; each Newton iteration then costs O(order) scalar FP64 work
+// instead of 27 logarithms.  All of it runs in FP64.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace uclbs {
+
+enum : int { ALPHA_NEWTON = 0, ALPHA_TRIVIAL = 1, ALPHA_BOUND = 2, ALPHA_NONPOS = 3,
+             ALPHA_NOCONV = 4 };
+
+struct AlphaOut {
+  double alpha;
+  int iters;
+  int status;
+  bool logpath = false;  // g needed direct logarithms (max|x| beyond the series range)
+};
+
+// below this max|x| the node is at equilibrium and alpha = 2 (the "fi = fieq"
+// branch, calculate_alpha.f95:430-431 and :551-556)
+constexpr double kTrivialX = 1e-12;
+// series orders and the max|x| each is exact for (remainder (2.2 m)^(N-1)/(N(N-1)) < 1e-16)
+constexpr double kSeriesM5 = 8e-5, kSeriesM7 = 1.5e-3, kSeriesM10 = 1.2e-2, kSeriesM16 = 5.5e-2;
+
+// phi(y) = (1 + y) ln(1 + y) - y, series for small |y|, direct otherwise
+__device__ __forceinline__ double phi1p(double y) {
+  if (fabs(y) < 0.1) {
+    double s = 0.0, p = y * y;
+#pragma unroll
+    for (int n = 2; n <= 20; ++n) {
+      s += ((n & 1) ? -p : p) * (1.0 / (n * (n - 1.0)));
+      p *= y;
+    }
+    return s;
+  }
+  return (1.0 + y) * log1p(y) - y;
+}
+
+template <int N>
+__device__ __forceinline__ void series_g(const double* S, double a, double& g, double& gp) {
+  double an1 = a, sg = 0.0, sgp = 0.0;
+#pragma unroll
+  for (int n = 2; n <= N; ++n) {
+    const double an = an1 * a;
+    const double c1 = 1.0 / (n * (n - 1.0)), c2 = 1.0 / (n - 1.0);
+    const double tg = S[n] * (an * c1 - a * c2);
+    const double tgp = S[n] * ((an1 - 1.0) * c2);
+    if (n & 1) { sg -= tg; sgp -= tgp; } else { sg += tg; sgp += tgp; }
+    an1 = an;
+  }
+  g = sg;
+  gp = sgp;
+}
+
+// Safeguarded Newton shared by both evaluations of g.  Iterations count Newton
+// updates taken before |g(a)| <= tol * S2 (S2 = sum f x^2 ~ g'(root), so the
+// test bounds |a - root| by about tol).
+template <class Eval>
+__device__ __forceinline__ AlphaOut newton_alpha(Eval&& ev, double S2, double amax,
+                                                 double alpha0, double tol, int maxit) {
+  double lo = 1.0, hi = amax < 1e6 ? amax : 1e6;
+  bool hi_is_bound = amax < 1e6;
+  double a = (alpha0 > lo && alpha0 < hi) ? alpha0 : 2.0;
+  AlphaOut r{2.0, 0, ALPHA_NEWTON, false};
+  for (int k = 0;; ++k) {
+    double g, gp;
+    ev(a, g, gp);
+    if (fabs(g) <= tol * S2) { r.alpha = a; r.iters = k; return r; }
+    if (g < 0.0 && a >= amax) { r.alpha = amax; r.iters = k; r.status = ALPHA_BOUND; return r; }
+    if (k >= maxit) { r.alpha = a; r.iters = k; r.status = ALPHA_NOCONV; return r; }
+    if (g < 0.0) lo = a; else { hi = a; hi_is_bound = false; }
+    double an = a - g / gp;
+    if (!(an > lo && an < hi)) an = (hi_is_bound && an >= hi) ? amax : 0.5 * (lo + hi);
+    if (an == amax) hi_is_bound = false;
+    a = an;
+  }
+}
+
+template <int N, class XF>
+__device__ __forceinline__ void power_sums(XF&& dx, int q, double* S) {
+#pragma unroll
+  for (int n = 2; n <= N; ++n) S[n] = 0.0;
+#pragma unroll
+  for (int i = 0; i < q; ++i) {
+    double d, x;
+    dx(i, d, x);
+    double t = d * x;  // f x^2
+    S[2] += t;
+#pragma unroll
+    for (int n = 3; n <= N; ++n) { t *= x; S[n] += t; }
+  }
+}
+
+// dx(i, d, x) yields D_i and x_i in FP64; fx(i) yields f_i in FP64.
+template <int Q, class DX, class FX>
+__device__ AlphaOut solve_alpha(DX&& dx, FX&& fx, double m, double xmin, double S2,
+                                double alpha0, double tol, int maxit) {
+  const double amax = xmin < 0.0 ? -1.0 / xmin : 1e300;
+  if (m < kTrivialX) return AlphaOut{2.0, 0, ALPHA_TRIVIAL, false};
+  if (amax < 2.0) return AlphaOut{amax, 0, ALPHA_BOUND, false};
+  double S[17];
+  if (m <= kSeriesM7) {
+    if (m <= kSeriesM5) {
+      power_sums<5>(dx, Q, S);
+      return newton_alpha([&](double a, double& g, double& gp) { series_g<5>(S, a, g, gp); },
+                          S2, amax, alpha0, tol, maxit);
+    }
+    power_sums<7>(dx, Q, S);
+    return newton_alpha([&](double a, double& g, double& gp) { series_g<7>(S, a, g, gp); }, S2,
+                        amax, alpha0, tol, maxit);
+  }
+  if (m <= kSeriesM10) {
+    power_sums<10>(dx, Q, S);
+    return newton_alpha([&](double a, double& g, double& gp) { series_g<10>(S, a, g, gp); },
+                        S2, amax, alpha0, tol, maxit);
+  }
+  if (m <= kSeriesM16) {
+    power_sums<16>(dx, Q, S);
+    return newton_alpha([&](double a, double& g, double& gp) { series_g<16>(S, a, g, gp); },
+                        S2, amax, alpha0, tol, maxit);
+  }
+  // large non-equilibrium: direct logarithms
+  AlphaOut r = newton_alpha(
+      [&](double a, double& g, double& gp) {
+        double sg = 0.0, sgp = 0.0;
+        bool inf_gp = false;
+        for (int i = 0; i < Q; ++i) {
+          double d, x;
+          dx(i, d, x);
+          const double f = fx(i);
+          const double l1 = log1p(x);
+          const double y = a * x;
+          if (1.0 + y <= 0.0) { sg += f * (-y - y * l1); inf_gp = true; continue; }
+          const double la = log1p(y);
+          sg += f * (phi1p(y) - y * l1);
+          sgp += d * (la - l1);
+        }
+        g = sg;
+        gp = inf_gp ? __builtin_huge_val() : sgp;
+      },
+      S2, amax, alpha0, tol, maxit);
+  r.logpath = true;
+  return r;
+}
+
+}  // namespace uclbs

@@ -1,0 +1,177 @@
+// D3Q27 entropic collide for one node, two storage forms.
+//
+// The reference has no D3Q27 collision (collision.f95:211-222 handles only
+// dir 19 and 39; directions_and_weights.f95:226-237 marks dir=27 as "XXXX"), so
+// the node update follows the published entropic LBM the oracle states:
+//   rho = sum f, rho u = sum c f          (calculate_flow_variables.f95:201-207)
+//   feq = product-form entropic equilibrium (Ansumali, Karlin, Ottinger 2003)
+//   fout = f + alpha beta (feq - f)        (collision.f95:289 with q_bar = 0)
+//   alpha from alpha_solve.h; beta = 1/(6 nu + 1).
+//
+// Storage forms:
+//   Fp32Shifted: populations stored as f~ = f - w_i in FP32.  Moment sums run
+//     in FP64; the equilibrium is built directly in shifted form,
+//     feq~_i = w_i [(1+drho)(1+ex)(1+ey)(1+ez) - 1] with the 1-D factors
+//     written as 1 + e so no O(1) quantity is ever rounded in FP32; x_i, the
+//     power sums and the Newton for alpha run in FP64.
+//   Fp64: full populations in FP64, every step in FP64 (oracle mode).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "alpha_solve.h"
+#include "lattice.h"
+
+namespace uclbs {
+
+struct AlphaCfg {
+  double tol;
+  int maxit;
+  int mode;  // 0 = entropic Newton from 2, 1 = fixed alpha = 2 (LBGK), 2 = given initial guess
+};
+
+// 1-D factors Psi(c) = 1 + e(c) of the product-form equilibrium, c = -1, 0, +1.
+// With s = sqrt(1 + 3u^2) and d = s - 1 = 3u^2 / (1 + s):
+//   Psi(0) = 2 - s               -> e0 = -d
+//   Psi(+1) = (2-s)(2u+s)/(1-u)  -> ep = (3u - 2ud - d^2) / (1 - u)
+//   Psi(-1) = (2-s)(1-u)/(2u+s)  -> em = (-3u - 2d + ud) / (1 + 2u + d)
+template <typename R>
+__device__ __forceinline__ void axis_eps(R u, R& em, R& e0, R& ep) {
+  const R s = sqrt(R(1) + R(3) * u * u);
+  const R d = R(3) * u * u / (R(1) + s);
+  e0 = -d;
+  ep = (R(3) * u - R(2) * u * d - d * d) / (R(1) - u);
+  em = (R(-3) * u - R(2) * d + u * d) / (R(1) + R(2) * u + d);
+}
+
+// (1+a)(1+b) - 1 without forming 1+a
+template <typename R>
+__device__ __forceinline__ R mul1(R a, R b) { return a + b + a * b; }
+
+// ---------------------------------------------------------------- FP32 shifted
+// f[i] holds f~_i on entry and fout~_i on return.
+__device__ __forceinline__ AlphaOut collide27_shifted(float (&f)[27], double beta,
+                                                      const AlphaCfg& ac, double alpha0,
+                                                      double& rho_out, double (&u_out)[3]) {
+  double r = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
+#pragma unroll
+  for (int i = 0; i < 27; ++i) {
+    const double fi = f[i];
+    r += fi;
+    if (CX27[i] > 0) jx += fi; else if (CX27[i] < 0) jx -= fi;
+    if (CY27[i] > 0) jy += fi; else if (CY27[i] < 0) jy -= fi;
+    if (CZ27[i] > 0) jz += fi; else if (CZ27[i] < 0) jz -= fi;
+  }
+  const double rho = 1.0 + r;
+  const double inv = 1.0 / rho;
+  const double ux = jx * inv, uy = jy * inv, uz = jz * inv;
+  rho_out = rho;
+  u_out[0] = ux; u_out[1] = uy; u_out[2] = uz;
+
+  float ex[3], ey[3], ez[3];
+  axis_eps<float>((float)ux, ex[0], ex[1], ex[2]);
+  axis_eps<float>((float)uy, ey[0], ey[1], ey[2]);
+  axis_eps<float>((float)uz, ez[0], ez[1], ez[2]);
+  const float dr = (float)r;
+  float qx[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) qx[c] = mul1(dr, ex[c]);
+  float pyz[9];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) pyz[a * 3 + b] = mul1(ey[a], ez[b]);
+
+  float D[27];
+  double m = 0.0, xmin = 0.0, S2 = 0.0;
+  bool nonpos = false;
+#pragma unroll
+  for (int i = 0; i < 27; ++i) {
+    const float e = mul1(qx[CX27[i] + 1], pyz[(CY27[i] + 1) * 3 + CZ27[i] + 1]);
+    D[i] = (float)W27[i] * e - f[i];
+  }
+  AlphaOut ao{2.0, 0, ALPHA_TRIVIAL, false};
+  if (ac.mode != 1) {
+    // x_i = D_i / f_i in FP64
+#pragma unroll
+    for (int i = 0; i < 27; ++i) {
+      const double fi = W27[i] + (double)f[i];
+      nonpos |= !(fi > 0.0);
+      const double x = (double)D[i] / fi;
+      m = fmax(m, fabs(x));
+      xmin = fmin(xmin, x);
+      S2 += (double)D[i] * x;
+    }
+    if (nonpos) {
+      ao = AlphaOut{2.0, 0, ALPHA_NONPOS, false};
+    } else {
+      auto dx = [&](int i, double& d, double& x) {
+        d = (double)D[i];
+        x = d / (W27[i] + (double)f[i]);
+      };
+      auto fx = [&](int i) { return W27[i] + (double)f[i]; };
+      ao = solve_alpha<27>(dx, fx, m, xmin, S2, ac.mode == 2 ? alpha0 : 2.0, ac.tol, ac.maxit);
+    }
+  }
+  const float ab = (float)(ao.alpha * beta);
+#pragma unroll
+  for (int i = 0; i < 27; ++i) f[i] = fmaf(ab, D[i], f[i]);
+  return ao;
+}
+
+// ---------------------------------------------------------------- FP64 full
+__device__ __forceinline__ AlphaOut collide27_fp64(double (&f)[27], double beta,
+                                                   const AlphaCfg& ac, double alpha0,
+                                                   double& rho_out, double (&u_out)[3]) {
+  double r = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
+#pragma unroll
+  for (int i = 0; i < 27; ++i) {
+    r += f[i];
+    if (CX27[i] > 0) jx += f[i]; else if (CX27[i] < 0) jx -= f[i];
+    if (CY27[i] > 0) jy += f[i]; else if (CY27[i] < 0) jy -= f[i];
+    if (CZ27[i] > 0) jz += f[i]; else if (CZ27[i] < 0) jz -= f[i];
+  }
+  const double ux = jx / r, uy = jy / r, uz = jz / r;
+  rho_out = r;
+  u_out[0] = ux; u_out[1] = uy; u_out[2] = uz;
+  // 1-D factors in the oracle's direct form
+  double px[3], py[3], pz[3];
+  auto psi = [](double u, double* p) {
+    const double s = sqrt(1.0 + 3.0 * u * u);
+    const double b = 2.0 - s;
+    p[1] = b;
+    p[2] = b * (2.0 * u + s) / (1.0 - u);
+    p[0] = b * (1.0 - u) / (2.0 * u + s);
+  };
+  psi(ux, px); psi(uy, py); psi(uz, pz);
+  double D[27];
+  double m = 0.0, xmin = 0.0, S2 = 0.0;
+  bool nonpos = false;
+#pragma unroll
+  for (int i = 0; i < 27; ++i) {
+    const double feq = r * W27[i] * px[CX27[i] + 1] * py[CY27[i] + 1] * pz[CZ27[i] + 1];
+    D[i] = feq - f[i];
+  }
+  AlphaOut ao{2.0, 0, ALPHA_TRIVIAL, false};
+  if (ac.mode != 1) {
+#pragma unroll
+    for (int i = 0; i < 27; ++i) {
+      nonpos |= !(f[i] > 0.0);
+      const double x = D[i] / f[i];
+      m = fmax(m, fabs(x));
+      xmin = fmin(xmin, x);
+      S2 += D[i] * x;
+    }
+    if (nonpos) {
+      ao = AlphaOut{2.0, 0, ALPHA_NONPOS, false};
+    } else {
+      auto dx = [&](int i, double& d, double& x) { d = D[i]; x = D[i] / f[i]; };
+      auto fx = [&](int i) { return f[i]; };
+      ao = solve_alpha<27>(dx, fx, m, xmin, S2, ac.mode == 2 ? alpha0 : 2.0, ac.tol, ac.maxit);
+    }
+  }
+  const double ab = ao.alpha * beta;
+#pragma unroll
+  for (int i = 0; i < 27; ++i) f[i] = f[i] + ab * D[i];
+  return ao;
+}
+
+}  // namespace uclbs

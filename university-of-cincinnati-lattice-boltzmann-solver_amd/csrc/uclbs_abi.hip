@@ -1,0 +1,389 @@
+// C-ABI layer of libuclbs_hip.so: context, level state resident in HBM,
+// host<->device population transfer and step dispatch.  The exported entry
+// points are declared in include/uclbs_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/uclbs_hip.h"
+#include "kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int set_err(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return set_err(UCLBS_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),  \
+                     __FILE__, __LINE__);                                                 \
+  } while (0)
+
+struct Level {
+  bool defined = false;
+  int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  uclbs::Box box{};
+  void* f[2] = {nullptr, nullptr};
+  int cur = 0;  // index of the buffer holding the current post-collision state
+  float* alpha = nullptr;
+};
+
+struct Ctx {
+  uclbs_config cfg{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<Level> levels;
+  unsigned long long* d_stats = nullptr;
+  bool keep_alpha = false;
+  int64_t steps_done = 0;
+};
+
+size_t elem_size(const Ctx* c) {
+  return c->cfg.precision == UCLBS_PREC_FP64 ? sizeof(double) : sizeof(float);
+}
+
+int q_of(const Ctx* c) { return c->cfg.lattice; }
+
+int check_level(Ctx* c, int lvl, Level** out) {
+  if (!c) return set_err(UCLBS_E_ARG, "null context");
+  if (lvl < 0 || lvl >= (int)c->levels.size() || !c->levels[lvl].defined)
+    return set_err(UCLBS_E_STATE, "level %d is not defined", lvl);
+  *out = &c->levels[lvl];
+  return UCLBS_OK;
+}
+
+int set_device(Ctx* c) {
+  HIP_TRY(hipSetDevice(c->device));
+  return UCLBS_OK;
+}
+
+void free_level(Level& L) {
+  for (auto& p : L.f)
+    if (p) { (void)hipFree(p); p = nullptr; }
+  if (L.alpha) { (void)hipFree(L.alpha); L.alpha = nullptr; }
+  L.defined = false;
+}
+
+uclbs::StepArgs step_args(const Ctx* c) {
+  uclbs::StepArgs a{};
+  a.beta = 1.0 / (6.0 * c->cfg.nu + 1.0);
+  a.tol = c->cfg.alpha_tol;
+  a.maxit = c->cfg.alpha_maxit;
+  a.alpha_mode = c->cfg.alpha_mode;
+  a.shifted = c->cfg.precision == UCLBS_PREC_FP32_SHIFTED;
+  return a;
+}
+
+int launch_steps(Ctx* c, Level* L, int nsteps) {
+  const uclbs::StepArgs a = step_args(c);
+  for (int s = 0; s < nsteps; ++s) {
+    if (c->cfg.lattice != UCLBS_LATTICE_D3Q27)
+      return set_err(UCLBS_E_UNSUPPORTED, "lattice %d not supported by step", c->cfg.lattice);
+    if (!(L->box.per[0] && L->box.per[1] && L->box.per[2]))
+      return set_err(UCLBS_E_UNSUPPORTED,
+                     "level without boundary tables must be periodic on all axes");
+    HIP_TRY(uclbs::launch_d3q27_step_periodic(L->box, L->f[L->cur], L->f[1 - L->cur],
+                                              c->keep_alpha ? L->alpha : nullptr, c->d_stats,
+                                              a, c->stream));
+    L->cur = 1 - L->cur;
+    c->steps_done++;
+  }
+  return UCLBS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* uclbs_last_error(void) { return g_err.c_str(); }
+
+int uclbs_abi_version(void) { return UCLBS_ABI_VERSION; }
+
+int uclbs_create(const uclbs_config* cfg, void** out) {
+  if (!cfg || !out) return set_err(UCLBS_E_ARG, "null argument");
+  *out = nullptr;
+  if (cfg->lattice != UCLBS_LATTICE_D3Q27 && cfg->lattice != UCLBS_LATTICE_D3Q39)
+    return set_err(UCLBS_E_ARG, "lattice must be 27 or 39, got %d", cfg->lattice);
+  if (cfg->precision != UCLBS_PREC_FP32_SHIFTED && cfg->precision != UCLBS_PREC_FP64)
+    return set_err(UCLBS_E_ARG, "bad precision %d", cfg->precision);
+  if (cfg->alpha_mode < 0 || cfg->alpha_mode > 2)
+    return set_err(UCLBS_E_ARG, "bad alpha_mode %d", cfg->alpha_mode);
+  if (!(cfg->nu > 0.0)) return set_err(UCLBS_E_ARG, "nu must be > 0");
+  Ctx* c = new Ctx();
+  c->cfg = *cfg;
+  if (c->cfg.alpha_maxit <= 0) c->cfg.alpha_maxit = 30;
+  if (c->cfg.alpha_maxit > 63) c->cfg.alpha_maxit = 63;
+  if (!(c->cfg.alpha_tol > 0.0)) c->cfg.alpha_tol = 1e-10;
+  hipError_t e = hipGetDevice(&c->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev0);
+  if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+  if (e == hipSuccess)
+    e = hipMalloc(&c->d_stats, sizeof(unsigned long long) * uclbs::kStatSlots * uclbs::kStatFields);
+  if (e == hipSuccess)
+    e = hipMemset(c->d_stats, 0,
+                  sizeof(unsigned long long) * uclbs::kStatSlots * uclbs::kStatFields);
+  if (e != hipSuccess) {
+    int rc = set_err(UCLBS_E_HIP, "context setup failed: %s", hipGetErrorString(e));
+    uclbs_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return UCLBS_OK;
+}
+
+int uclbs_destroy(void* ctx) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c) return UCLBS_OK;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& L : c->levels) free_level(L);
+  if (c->d_stats) (void)hipFree(c->d_stats);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return UCLBS_OK;
+}
+
+int uclbs_define_level(void* ctx, int lvl, const int32_t lo[3], const int32_t hi[3],
+                       const int32_t periodic[3]) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || !lo || !hi || !periodic) return set_err(UCLBS_E_ARG, "null argument");
+  if (lvl < 0 || lvl > 31) return set_err(UCLBS_E_ARG, "level %d out of range", lvl);
+  int rc = set_device(c);
+  if (rc) return rc;
+  if ((int)c->levels.size() <= lvl) c->levels.resize(lvl + 1);
+  Level& L = c->levels[lvl];
+  free_level(L);
+  int n[3];
+  for (int d = 0; d < 3; ++d) {
+    n[d] = hi[d] - lo[d] + 1;
+    if (n[d] < 3) return set_err(UCLBS_E_ARG, "box extent %d on axis %d is below 3", n[d], d);
+    L.lo[d] = lo[d];
+    L.hi[d] = hi[d];
+    L.box.per[d] = periodic[d] != 0;
+  }
+  L.box.nx = n[0];
+  L.box.ny = n[1];
+  L.box.nz = n[2];
+  L.box.n = (int64_t)n[0] * n[1] * n[2];
+  const size_t bytes = (size_t)L.box.n * q_of(c) * elem_size(c);
+  for (auto& p : L.f) {
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      free_level(L);
+      return set_err(UCLBS_E_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+  }
+  HIP_TRY(hipMalloc(&L.alpha, (size_t)L.box.n * sizeof(float)));
+  L.cur = 0;
+  L.defined = true;
+  return UCLBS_OK;
+}
+
+int uclbs_upload_f(void* ctx, int lvl, const double* f, int64_t n_nodes) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  Level* L;
+  int rc = check_level(c, lvl, &L);
+  if (rc) return rc;
+  if (!f || n_nodes != L->box.n)
+    return set_err(UCLBS_E_ARG, "upload: expected %lld nodes, got %lld", (long long)L->box.n,
+                   (long long)n_nodes);
+  if ((rc = set_device(c))) return rc;
+  const size_t bytes = (size_t)L->box.n * q_of(c) * sizeof(double);
+  double* tmp = nullptr;
+  HIP_TRY(hipMalloc(&tmp, bytes));
+  hipError_t e = hipMemcpyAsync(tmp, f, bytes, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess)
+    e = uclbs::launch_d3q27_from_fp64(L->box, tmp, L->f[L->cur],
+                                       c->cfg.precision == UCLBS_PREC_FP32_SHIFTED, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return set_err(UCLBS_E_HIP, "upload failed: %s", hipGetErrorString(e));
+  return UCLBS_OK;
+}
+
+int uclbs_download_f(void* ctx, int lvl, double* f, int64_t n_nodes) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  Level* L;
+  int rc = check_level(c, lvl, &L);
+  if (rc) return rc;
+  if (!f || n_nodes != L->box.n)
+    return set_err(UCLBS_E_ARG, "download: expected %lld nodes, got %lld", (long long)L->box.n,
+                   (long long)n_nodes);
+  if ((rc = set_device(c))) return rc;
+  const size_t bytes = (size_t)L->box.n * q_of(c) * sizeof(double);
+  double* tmp = nullptr;
+  HIP_TRY(hipMalloc(&tmp, bytes));
+  hipError_t e = uclbs::launch_d3q27_to_fp64(L->box, L->f[L->cur], tmp,
+                                             c->cfg.precision == UCLBS_PREC_FP32_SHIFTED,
+                                             c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(f, tmp, bytes, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return set_err(UCLBS_E_HIP, "download failed: %s", hipGetErrorString(e));
+  return UCLBS_OK;
+}
+
+int uclbs_download_macro(void* ctx, int lvl, double* rho, double* u, double* v, double* w) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  Level* L;
+  int rc = check_level(c, lvl, &L);
+  if (rc) return rc;
+  if (!rho || !u || !v || !w) return set_err(UCLBS_E_ARG, "null output array");
+  if ((rc = set_device(c))) return rc;
+  const size_t bytes = (size_t)L->box.n * sizeof(double);
+  double* tmp = nullptr;
+  HIP_TRY(hipMalloc(&tmp, 4 * bytes));
+  const int64_t n = L->box.n;
+  hipError_t e = uclbs::launch_d3q27_macro(L->box, L->f[L->cur],
+                                           c->cfg.precision == UCLBS_PREC_FP32_SHIFTED, tmp,
+                                           tmp + n, tmp + 2 * n, tmp + 3 * n, c->stream);
+  double* outs[4] = {rho, u, v, w};
+  for (int k = 0; k < 4 && e == hipSuccess; ++k)
+    e = hipMemcpyAsync(outs[k], tmp + k * n, bytes, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return set_err(UCLBS_E_HIP, "macro failed: %s", hipGetErrorString(e));
+  return UCLBS_OK;
+}
+
+int uclbs_set_keep_alpha(void* ctx, int keep) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c) return set_err(UCLBS_E_ARG, "null context");
+  c->keep_alpha = keep != 0;
+  return UCLBS_OK;
+}
+
+int uclbs_download_alpha(void* ctx, int lvl, double* alpha) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  Level* L;
+  int rc = check_level(c, lvl, &L);
+  if (rc) return rc;
+  if (!alpha) return set_err(UCLBS_E_ARG, "null output array");
+  if (!c->keep_alpha) return set_err(UCLBS_E_STATE, "alpha is not kept (uclbs_set_keep_alpha)");
+  if ((rc = set_device(c))) return rc;
+  const size_t bytes = (size_t)L->box.n * sizeof(double);
+  double* tmp = nullptr;
+  HIP_TRY(hipMalloc(&tmp, bytes));
+  hipError_t e = uclbs::launch_f32_to_f64(L->alpha, tmp, L->box.n, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(alpha, tmp, bytes, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return set_err(UCLBS_E_HIP, "alpha download failed: %s", hipGetErrorString(e));
+  return UCLBS_OK;
+}
+
+int uclbs_init_tgv(void* ctx, int lvl, double u0, double noise_amp, uint64_t seed) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  Level* L;
+  int rc = check_level(c, lvl, &L);
+  if (rc) return rc;
+  if (c->cfg.lattice != UCLBS_LATTICE_D3Q27)
+    return set_err(UCLBS_E_UNSUPPORTED, "init_tgv is a D3Q27 initialiser");
+  if (!(fabs(u0) < 0.5) || !(noise_amp >= 0.0))
+    return set_err(UCLBS_E_ARG, "init_tgv: |u0| must be < 0.5 and noise_amp >= 0");
+  if ((rc = set_device(c))) return rc;
+  HIP_TRY(uclbs::launch_d3q27_init_tgv(L->box, L->f[L->cur],
+                                       c->cfg.precision == UCLBS_PREC_FP32_SHIFTED, u0,
+                                       noise_amp, seed, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return UCLBS_OK;
+}
+
+int uclbs_step_level_async(void* ctx, int lvl, int nsteps) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  Level* L;
+  int rc = check_level(c, lvl, &L);
+  if (rc) return rc;
+  if (nsteps < 0) return set_err(UCLBS_E_ARG, "nsteps < 0");
+  if ((rc = set_device(c))) return rc;
+  return launch_steps(c, L, nsteps);
+}
+
+int uclbs_step_level(void* ctx, int lvl, int nsteps) {
+  int rc = uclbs_step_level_async(ctx, lvl, nsteps);
+  if (rc) return rc;
+  return uclbs_sync(ctx);
+}
+
+int uclbs_time_steps(void* ctx, int lvl, int nsteps, double* elapsed_ms) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  Level* L;
+  int rc = check_level(c, lvl, &L);
+  if (rc) return rc;
+  if (!elapsed_ms || nsteps <= 0) return set_err(UCLBS_E_ARG, "bad timing arguments");
+  if ((rc = set_device(c))) return rc;
+  HIP_TRY(hipEventRecord(c->ev0, c->stream));
+  if ((rc = launch_steps(c, L, nsteps))) return rc;
+  HIP_TRY(hipEventRecord(c->ev1, c->stream));
+  HIP_TRY(hipEventSynchronize(c->ev1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  *elapsed_ms = ms;
+  return UCLBS_OK;
+}
+
+int uclbs_sync(void* ctx) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c) return set_err(UCLBS_E_ARG, "null context");
+  int rc = set_device(c);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return UCLBS_OK;
+}
+
+int uclbs_get_stats(void* ctx, uclbs_stats* out) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || !out) return set_err(UCLBS_E_ARG, "null argument");
+  int rc = set_device(c);
+  if (rc) return rc;
+  std::vector<unsigned long long> h(uclbs::kStatSlots * uclbs::kStatFields);
+  HIP_TRY(hipMemcpyAsync(h.data(), c->d_stats, h.size() * sizeof(h[0]), hipMemcpyDeviceToHost,
+                         c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  int64_t acc[uclbs::kStatFields] = {0};
+  for (int s = 0; s < uclbs::kStatSlots; ++s)
+    for (int k = 0; k < uclbs::kStatFields; ++k) acc[k] += (int64_t)h[s * uclbs::kStatFields + k];
+  out->node_updates = acc[uclbs::ST_NODES];
+  out->newton_iters = acc[uclbs::ST_ITERS];
+  out->n_trivial = acc[uclbs::ST_TRIVIAL];
+  out->n_bound = acc[uclbs::ST_BOUND];
+  out->n_nonpos = acc[uclbs::ST_NONPOS];
+  out->n_noconv = acc[uclbs::ST_NOCONV];
+  out->n_fallback = acc[uclbs::ST_FALLBACK];
+  out->n_logpath = acc[uclbs::ST_LOGPATH];
+  return UCLBS_OK;
+}
+
+int uclbs_reset_stats(void* ctx) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c) return set_err(UCLBS_E_ARG, "null context");
+  int rc = set_device(c);
+  if (rc) return rc;
+  HIP_TRY(hipMemsetAsync(c->d_stats, 0,
+                         sizeof(unsigned long long) * uclbs::kStatSlots * uclbs::kStatFields,
+                         c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return UCLBS_OK;
+}
+
+}  // extern "C"
