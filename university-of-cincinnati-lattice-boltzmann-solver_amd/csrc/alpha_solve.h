@@ -27,7 +27,7 @@
 namespace uclbs {
 
 enum : int { ALPHA_NEWTON = 0, ALPHA_TRIVIAL = 1, ALPHA_BOUND = 2, ALPHA_NONPOS = 3,
-             ALPHA_NOCONV = 4 };
+             ALPHA_NOCONV = 4, ALPHA_DEFER = 5 /* left to the general-path fixup kernel */ };
 
 struct AlphaOut {
   double alpha;
@@ -39,8 +39,24 @@ struct AlphaOut {
 // below this max|x| the node is at equilibrium and alpha = 2 (the "fi = fieq"
 // branch, calculate_alpha.f95:430-431 and :551-556)
 constexpr double kTrivialX = 1e-12;
-// series orders and the max|x| each is exact for (remainder (2.2 m)^(N-1)/(N(N-1)) < 1e-16)
-constexpr double kSeriesM5 = 8e-5, kSeriesM7 = 1.5e-3, kSeriesM10 = 1.2e-2, kSeriesM16 = 5.5e-2;
+
+// Largest max|x| for which the order-N truncation of the power-sum series is
+// used.  The remainder of g relative to S2 is about (2.2 m)^(N-1) / (N(N-1))
+// for Newton iterates a <= 2.2; the host sets the bound from a target eps
+// (1e-16 in FP64 mode, i.e. exact to round-off; 0.1 * tol in FP32 mode, so
+// truncation moves the root by less than a tenth of the Newton tolerance).
+struct SeriesThr {
+  double m5, m7, m10, m16;  // general path orders
+  float m_fast;             // order kFastOrder of the fused FP32-storage pass
+};
+constexpr int kFastOrder = 6;
+inline double series_limit(int N, double eps) {
+  return __builtin_pow(eps * N * (N - 1.0), 1.0 / (N - 1.0)) / 2.2;
+}
+inline SeriesThr make_series_thr(double eps) {
+  return SeriesThr{series_limit(5, eps), series_limit(7, eps), series_limit(10, eps),
+                   series_limit(16, eps), (float)series_limit(kFastOrder, eps)};
+}
 
 // phi(y) = (1 + y) ln(1 + y) - y, series for small |y|, direct otherwise
 __device__ __forceinline__ double phi1p(double y) {
@@ -114,13 +130,13 @@ __device__ __forceinline__ void power_sums(XF&& dx, int q, double* S) {
 // dx(i, d, x) yields D_i and x_i in FP64; fx(i) yields f_i in FP64.
 template <int Q, class DX, class FX>
 __device__ AlphaOut solve_alpha(DX&& dx, FX&& fx, double m, double xmin, double S2,
-                                double alpha0, double tol, int maxit) {
+                                double alpha0, double tol, int maxit, const SeriesThr& thr) {
   const double amax = xmin < 0.0 ? -1.0 / xmin : 1e300;
   if (m < kTrivialX) return AlphaOut{2.0, 0, ALPHA_TRIVIAL, false};
   if (amax < 2.0) return AlphaOut{amax, 0, ALPHA_BOUND, false};
   double S[17];
-  if (m <= kSeriesM7) {
-    if (m <= kSeriesM5) {
+  if (m <= thr.m7) {
+    if (m <= thr.m5) {
       power_sums<5>(dx, Q, S);
       return newton_alpha([&](double a, double& g, double& gp) { series_g<5>(S, a, g, gp); },
                           S2, amax, alpha0, tol, maxit);
@@ -129,12 +145,12 @@ __device__ AlphaOut solve_alpha(DX&& dx, FX&& fx, double m, double xmin, double 
     return newton_alpha([&](double a, double& g, double& gp) { series_g<7>(S, a, g, gp); }, S2,
                         amax, alpha0, tol, maxit);
   }
-  if (m <= kSeriesM10) {
+  if (m <= thr.m10) {
     power_sums<10>(dx, Q, S);
     return newton_alpha([&](double a, double& g, double& gp) { series_g<10>(S, a, g, gp); },
                         S2, amax, alpha0, tol, maxit);
   }
-  if (m <= kSeriesM16) {
+  if (m <= thr.m16) {
     power_sums<16>(dx, Q, S);
     return newton_alpha([&](double a, double& g, double& gp) { series_g<16>(S, a, g, gp); },
                         S2, amax, alpha0, tol, maxit);

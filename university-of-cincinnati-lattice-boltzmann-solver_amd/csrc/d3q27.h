@@ -12,8 +12,9 @@
 //   Fp32Shifted: populations stored as f~ = f - w_i in FP32.  Moment sums run
 //     in FP64; the equilibrium is built directly in shifted form,
 //     feq~_i = w_i [(1+drho)(1+ex)(1+ey)(1+ez) - 1] with the 1-D factors
-//     written as 1 + e so no O(1) quantity is ever rounded in FP32; x_i, the
-//     power sums and the Newton for alpha run in FP64.
+//     written as 1 + e so no O(1) quantity is ever rounded in FP32; x_i and
+//     the products f x^n are FP32 (they carry the FP32 inputs' precision),
+//     the entropy power sums, the mass residual and the alpha Newton are FP64.
 //   Fp64: full populations in FP64, every step in FP64 (oracle mode).
 #pragma once
 #include <hip/hip_runtime.h>
@@ -26,7 +27,10 @@ struct AlphaCfg {
   double tol;
   int maxit;
   int mode;  // 0 = entropic Newton from 2, 1 = fixed alpha = 2 (LBGK), 2 = given initial guess
+  SeriesThr thr;
 };
+
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // 1-D factors Psi(c) = 1 + e(c) of the product-form equilibrium, c = -1, 0, +1.
 // With s = sqrt(1 + 3u^2) and d = s - 1 = 3u^2 / (1 + s):
@@ -41,13 +45,27 @@ __device__ __forceinline__ void axis_eps(R u, R& em, R& e0, R& ep) {
   ep = (R(3) * u - R(2) * u * d - d * d) / (R(1) - u);
   em = (R(-3) * u - R(2) * d + u * d) / (R(1) + R(2) * u + d);
 }
+// FP32 form with hardware reciprocal / square root (about 1 ulp each; the
+// factors only need relative accuracy ~1e-7 since they are O(u) deviations).
+__device__ __forceinline__ void axis_eps_fast(float u, float& em, float& e0, float& ep) {
+  const float uu = 3.0f * u * u;
+  const float s = __builtin_amdgcn_sqrtf(1.0f + uu);
+  const float d = uu * frcp(1.0f + s);
+  e0 = -d;
+  ep = (3.0f * u - 2.0f * u * d - d * d) * frcp(1.0f - u);
+  em = (-3.0f * u - 2.0f * d + u * d) * frcp(1.0f + 2.0f * u + d);
+}
 
 // (1+a)(1+b) - 1 without forming 1+a
 template <typename R>
 __device__ __forceinline__ R mul1(R a, R b) { return a + b + a * b; }
 
 // ---------------------------------------------------------------- FP32 shifted
-// f[i] holds f~_i on entry and fout~_i on return.
+// f[i] holds f~_i on entry and fout~_i on return.  With kInlineGeneral false a
+// node whose max|x| lies beyond the fused series order returns ALPHA_DEFER
+// with f[] untouched (the caller stores the pre-collision state and queues the
+// node); this keeps the logarithm path's registers out of the streaming kernel.
+template <bool kInlineGeneral>
 __device__ __forceinline__ AlphaOut collide27_shifted(float (&f)[27], double beta,
                                                       const AlphaCfg& ac, double alpha0,
                                                       double& rho_out, double (&u_out)[3]) {
@@ -67,9 +85,9 @@ __device__ __forceinline__ AlphaOut collide27_shifted(float (&f)[27], double bet
   u_out[0] = ux; u_out[1] = uy; u_out[2] = uz;
 
   float ex[3], ey[3], ez[3];
-  axis_eps<float>((float)ux, ex[0], ex[1], ex[2]);
-  axis_eps<float>((float)uy, ey[0], ey[1], ey[2]);
-  axis_eps<float>((float)uz, ez[0], ez[1], ez[2]);
+  axis_eps_fast((float)ux, ex[0], ex[1], ex[2]);
+  axis_eps_fast((float)uy, ey[0], ey[1], ey[2]);
+  axis_eps_fast((float)uz, ez[0], ez[1], ez[2]);
   const float dr = (float)r;
   float qx[3];
 #pragma unroll
@@ -80,35 +98,65 @@ __device__ __forceinline__ AlphaOut collide27_shifted(float (&f)[27], double bet
 #pragma unroll
     for (int b = 0; b < 3; ++b) pyz[a * 3 + b] = mul1(ey[a], ez[b]);
 
+  // D = feq~ - f~.  Mass projection: in exact arithmetic sum D = 0; the FP32
+  // rounding of the 1-D factors leaves a small, partly systematic residual
+  // (it drifted the total mass by ~1e-10 per node and step), so the residual
+  // is summed in FP64 and removed through the rest population.
   float D[27];
-  double m = 0.0, xmin = 0.0, S2 = 0.0;
-  bool nonpos = false;
+  double sd = 0.0;
 #pragma unroll
   for (int i = 0; i < 27; ++i) {
     const float e = mul1(qx[CX27[i] + 1], pyz[(CY27[i] + 1) * 3 + CZ27[i] + 1]);
     D[i] = (float)W27[i] * e - f[i];
+    sd += (double)D[i];
   }
+  D[0] -= (float)sd;
+
   AlphaOut ao{2.0, 0, ALPHA_TRIVIAL, false};
   if (ac.mode != 1) {
-    // x_i = D_i / f_i in FP64
+    // One fused pass: x_i = D_i / f_i (FP32, hardware reciprocal), max|x|,
+    // min x, and the power sums S_n = sum f x^n = sum D x^(n-1), n = 2..6,
+    // with FP32 products accumulated in FP64.
+    float m = 0.0f, xmin = 0.0f;
+    bool nonpos = false;
+    double S[kFastOrder + 1];
+#pragma unroll
+    for (int n = 2; n <= kFastOrder; ++n) S[n] = 0.0;
 #pragma unroll
     for (int i = 0; i < 27; ++i) {
-      const double fi = W27[i] + (double)f[i];
-      nonpos |= !(fi > 0.0);
-      const double x = (double)D[i] / fi;
-      m = fmax(m, fabs(x));
-      xmin = fmin(xmin, x);
-      S2 += (double)D[i] * x;
+      const float fi = (float)W27[i] + f[i];
+      nonpos |= !(fi > 0.0f);
+      const float x = D[i] * frcp(fi);
+      m = fmaxf(m, fabsf(x));
+      xmin = fminf(xmin, x);
+      float t = D[i] * x;
+      S[2] += (double)t;
+#pragma unroll
+      for (int n = 3; n <= kFastOrder; ++n) { t *= x; S[n] += (double)t; }
     }
+    const double amax = xmin < 0.0f ? -1.0 / (double)xmin : 1e300;
     if (nonpos) {
       ao = AlphaOut{2.0, 0, ALPHA_NONPOS, false};
+    } else if ((double)m < kTrivialX) {
+      ao = AlphaOut{2.0, 0, ALPHA_TRIVIAL, false};
+    } else if (amax < 2.0) {
+      ao = AlphaOut{amax, 0, ALPHA_BOUND, false};
+    } else if (m <= ac.thr.m_fast) {
+      // within the fused order: Newton on the scalar series
+      const double a0 = ac.mode == 2 ? alpha0 : 2.0;
+      ao = newton_alpha([&](double a, double& g, double& gp) { series_g<kFastOrder>(S, a, g, gp); },
+                        S[2], amax, a0, ac.tol, ac.maxit);
+    } else if constexpr (!kInlineGeneral) {
+      return AlphaOut{2.0, 0, ALPHA_DEFER, false};
     } else {
+      // some lane of the wavefront is further from equilibrium: general path
       auto dx = [&](int i, double& d, double& x) {
         d = (double)D[i];
         x = d / (W27[i] + (double)f[i]);
       };
       auto fx = [&](int i) { return W27[i] + (double)f[i]; };
-      ao = solve_alpha<27>(dx, fx, m, xmin, S2, ac.mode == 2 ? alpha0 : 2.0, ac.tol, ac.maxit);
+      ao = solve_alpha<27>(dx, fx, (double)m, (double)xmin, S[2], ac.mode == 2 ? alpha0 : 2.0,
+                           ac.tol, ac.maxit, ac.thr);
     }
   }
   const float ab = (float)(ao.alpha * beta);
@@ -165,7 +213,8 @@ __device__ __forceinline__ AlphaOut collide27_fp64(double (&f)[27], double beta,
     } else {
       auto dx = [&](int i, double& d, double& x) { d = D[i]; x = D[i] / f[i]; };
       auto fx = [&](int i) { return f[i]; };
-      ao = solve_alpha<27>(dx, fx, m, xmin, S2, ac.mode == 2 ? alpha0 : 2.0, ac.tol, ac.maxit);
+      ao = solve_alpha<27>(dx, fx, m, xmin, S2, ac.mode == 2 ? alpha0 : 2.0, ac.tol, ac.maxit,
+                           ac.thr);
     }
   }
   const double ab = ao.alpha * beta;

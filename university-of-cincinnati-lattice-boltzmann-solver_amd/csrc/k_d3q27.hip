@@ -55,7 +55,7 @@ __device__ __forceinline__ void wave_stats(unsigned long long* __restrict__ stat
 template <bool kShifted, bool kKeepAlpha>
 __global__ __launch_bounds__(256) void k27_step_periodic(const void* __restrict__ fin_,
                                                          void* __restrict__ fout_,
-                                                         float* __restrict__ alpha_out,
+                                                         double* __restrict__ alpha_out,
                                                          unsigned long long* __restrict__ stats,
                                                          Box b, StepArgs a) {
   using T = typename std::conditional<kShifted, float, double>::type;
@@ -86,18 +86,60 @@ __global__ __launch_bounds__(256) void k27_step_periodic(const void* __restrict_
   for (int i = 0; i < 27; ++i)
     f[i] = fin[(int64_t)i * b.n + ox[CX27[i] + 1] + oy[CY27[i] + 1] + oz[CZ27[i] + 1]];
 
-  AlphaCfg ac{a.tol, a.maxit, a.alpha_mode};
+  AlphaCfg ac{a.tol, a.maxit, a.alpha_mode, a.thr};
   double rho, u[3];
   AlphaOut ao;
-  if constexpr (kShifted) ao = collide27_shifted(f, a.beta, ac, 2.0, rho, u);
+  if constexpr (kShifted) ao = collide27_shifted<false>(f, a.beta, ac, 2.0, rho, u);
   else ao = collide27_fp64(f, a.beta, ac, 2.0, rho, u);
 
   if (active) {
+    // a deferred node stores its pre-collision state; the fixup kernel collides it in place
 #pragma unroll
     for (int i = 0; i < 27; ++i) fout[(int64_t)i * b.n + node] = f[i];
-    if constexpr (kKeepAlpha) alpha_out[node] = (float)ao.alpha;
+    if constexpr (kKeepAlpha) alpha_out[node] = ao.alpha;
   }
-  wave_stats(stats, active, ao.iters, ao.status, false, ao.logpath);
+  if constexpr (kShifted) {
+    const bool defer = active && ao.status == ALPHA_DEFER;
+    const unsigned long long dm = __ballot(defer);
+    if (dm) {
+      unsigned base = 0;
+      const int leader = __ffsll((long long)dm) - 1;
+      if ((int)__lane_id() == leader) base = atomicAdd(a.defer_count + a.parity, (unsigned)__popcll(dm));
+      base = __shfl(base, leader);
+      if (defer) a.defer_list[base + __popcll(dm & ((1ull << __lane_id()) - 1ull))] = (unsigned)node;
+    }
+  }
+  wave_stats(stats, active && ao.status != ALPHA_DEFER, ao.iters, ao.status, false, ao.logpath);
+}
+
+// Collides, in place, the nodes the streaming kernel queued (max|x| beyond the
+// fused series order): full FP64 x_i, higher series orders or logarithms.
+// Grid-stride over the device-side count; also clears the other parity's
+// counter for the next step.
+__global__ __launch_bounds__(256) void k27_fixup(float* __restrict__ f_, double* __restrict__ alpha_out,
+                                                 unsigned long long* __restrict__ stats, Box b,
+                                                 StepArgs a) {
+  const unsigned count = a.defer_count[a.parity];
+  if (blockIdx.x == 0 && threadIdx.x == 0) a.defer_count[1 - a.parity] = 0u;
+  AlphaCfg ac{a.tol, a.maxit, a.alpha_mode, a.thr};
+  const unsigned stride = gridDim.x * blockDim.x;
+  const unsigned rounds = (count + stride - 1) / stride;
+  for (unsigned r = 0; r < rounds; ++r) {
+    const unsigned k = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = k < count;
+    const int64_t node = active ? (int64_t)a.defer_list[k] : 0;
+    float f[27];
+#pragma unroll
+    for (int i = 0; i < 27; ++i) f[i] = f_[(int64_t)i * b.n + node];
+    double rho, u[3];
+    AlphaOut ao = collide27_shifted<true>(f, a.beta, ac, 2.0, rho, u);
+    if (active) {
+#pragma unroll
+      for (int i = 0; i < 27; ++i) f_[(int64_t)i * b.n + node] = f[i];
+      if (alpha_out) alpha_out[node] = ao.alpha;
+    }
+    wave_stats(stats, active, ao.iters, ao.status, false, ao.logpath);
+  }
 }
 
 template <bool kShifted>
@@ -184,16 +226,11 @@ __global__ void k27_init_tgv(void* __restrict__ dst_, Box b, double u0, double a
   }
 }
 
-__global__ void k_f32_to_f64(const float* __restrict__ s, double* __restrict__ d, int64_t n) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) d[k] = s[k];
-}
-
 inline unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
 }  // namespace
 
-hipError_t launch_d3q27_step_periodic(const Box& b, const void* fin, void* fout, float* alpha,
+hipError_t launch_d3q27_step_periodic(const Box& b, const void* fin, void* fout, double* alpha,
                                       unsigned long long* stats, const StepArgs& a,
                                       hipStream_t s) {
   const unsigned g = blocks_for(b.n, 256);
@@ -204,6 +241,12 @@ hipError_t launch_d3q27_step_periodic(const Box& b, const void* fin, void* fout,
     if (alpha) k27_step_periodic<false, true><<<g, 256, 0, s>>>(fin, fout, alpha, stats, b, a);
     else k27_step_periodic<false, false><<<g, 256, 0, s>>>(fin, fout, alpha, stats, b, a);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_d3q27_fixup(const Box& b, void* fout, double* alpha,
+                              unsigned long long* stats, const StepArgs& a, hipStream_t s) {
+  k27_fixup<<<512, 256, 0, s>>>(static_cast<float*>(fout), alpha, stats, b, a);
   return hipGetLastError();
 }
 
@@ -232,11 +275,6 @@ hipError_t launch_d3q27_init_tgv(const Box& b, void* f, int shifted, double u0,
                                  double noise_amp, uint64_t seed, hipStream_t s) {
   if (shifted) k27_init_tgv<true><<<blocks_for(b.n, 256), 256, 0, s>>>(f, b, u0, noise_amp, seed);
   else k27_init_tgv<false><<<blocks_for(b.n, 256), 256, 0, s>>>(f, b, u0, noise_amp, seed);
-  return hipGetLastError();
-}
-
-hipError_t launch_f32_to_f64(const float* src, double* dst, int64_t n, hipStream_t s) {
-  k_f32_to_f64<<<blocks_for(n, 256), 256, 0, s>>>(src, dst, n);
   return hipGetLastError();
 }
 

@@ -41,7 +41,10 @@ struct Level {
   uclbs::Box box{};
   void* f[2] = {nullptr, nullptr};
   int cur = 0;  // index of the buffer holding the current post-collision state
-  float* alpha = nullptr;
+  double* alpha = nullptr;
+  unsigned int* defer_count = nullptr;  // [2], ping-pong by step parity
+  unsigned int* defer_list = nullptr;   // n entries
+  int parity = 0;
 };
 
 struct Ctx {
@@ -78,6 +81,8 @@ void free_level(Level& L) {
   for (auto& p : L.f)
     if (p) { (void)hipFree(p); p = nullptr; }
   if (L.alpha) { (void)hipFree(L.alpha); L.alpha = nullptr; }
+  if (L.defer_count) { (void)hipFree(L.defer_count); L.defer_count = nullptr; }
+  if (L.defer_list) { (void)hipFree(L.defer_list); L.defer_list = nullptr; }
   L.defined = false;
 }
 
@@ -88,20 +93,31 @@ uclbs::StepArgs step_args(const Ctx* c) {
   a.maxit = c->cfg.alpha_maxit;
   a.alpha_mode = c->cfg.alpha_mode;
   a.shifted = c->cfg.precision == UCLBS_PREC_FP32_SHIFTED;
+  // FP64 storage: series exact to round-off; FP32 storage: truncation below
+  // a tenth of the Newton tolerance
+  a.thr = uclbs::make_series_thr(a.shifted ? 0.1 * a.tol : 1e-16);
   return a;
 }
 
 int launch_steps(Ctx* c, Level* L, int nsteps) {
-  const uclbs::StepArgs a = step_args(c);
+  uclbs::StepArgs a = step_args(c);
+  a.defer_count = L->defer_count;
+  a.defer_list = L->defer_list;
   for (int s = 0; s < nsteps; ++s) {
     if (c->cfg.lattice != UCLBS_LATTICE_D3Q27)
       return set_err(UCLBS_E_UNSUPPORTED, "lattice %d not supported by step", c->cfg.lattice);
     if (!(L->box.per[0] && L->box.per[1] && L->box.per[2]))
       return set_err(UCLBS_E_UNSUPPORTED,
                      "level without boundary tables must be periodic on all axes");
+    a.parity = L->parity;
     HIP_TRY(uclbs::launch_d3q27_step_periodic(L->box, L->f[L->cur], L->f[1 - L->cur],
                                               c->keep_alpha ? L->alpha : nullptr, c->d_stats,
                                               a, c->stream));
+    if (a.shifted)
+      HIP_TRY(uclbs::launch_d3q27_fixup(L->box, L->f[1 - L->cur],
+                                        c->keep_alpha ? L->alpha : nullptr, c->d_stats, a,
+                                        c->stream));
+    L->parity ^= 1;
     L->cur = 1 - L->cur;
     c->steps_done++;
   }
@@ -193,7 +209,12 @@ int uclbs_define_level(void* ctx, int lvl, const int32_t lo[3], const int32_t hi
       return set_err(UCLBS_E_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
     }
   }
-  HIP_TRY(hipMalloc(&L.alpha, (size_t)L.box.n * sizeof(float)));
+  HIP_TRY(hipMalloc(&L.alpha, (size_t)L.box.n * sizeof(double)));
+  if (L.box.n > 0xffffffffll) return set_err(UCLBS_E_ARG, "box exceeds 2^32 nodes");
+  HIP_TRY(hipMalloc(&L.defer_count, 2 * sizeof(unsigned int)));
+  HIP_TRY(hipMemset(L.defer_count, 0, 2 * sizeof(unsigned int)));
+  HIP_TRY(hipMalloc(&L.defer_list, (size_t)L.box.n * sizeof(unsigned int)));
+  L.parity = 0;
   L.cur = 0;
   L.defined = true;
   return UCLBS_OK;
@@ -282,12 +303,8 @@ int uclbs_download_alpha(void* ctx, int lvl, double* alpha) {
   if (!c->keep_alpha) return set_err(UCLBS_E_STATE, "alpha is not kept (uclbs_set_keep_alpha)");
   if ((rc = set_device(c))) return rc;
   const size_t bytes = (size_t)L->box.n * sizeof(double);
-  double* tmp = nullptr;
-  HIP_TRY(hipMalloc(&tmp, bytes));
-  hipError_t e = uclbs::launch_f32_to_f64(L->alpha, tmp, L->box.n, c->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(alpha, tmp, bytes, hipMemcpyDeviceToHost, c->stream);
+  hipError_t e = hipMemcpyAsync(alpha, L->alpha, bytes, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  (void)hipFree(tmp);
   if (e != hipSuccess) return set_err(UCLBS_E_HIP, "alpha download failed: %s", hipGetErrorString(e));
   return UCLBS_OK;
 }
