@@ -80,6 +80,27 @@ def cpu_baseline(budget_s: float):
                       f"{threads} OpenMP threads on {cpu_model()}"}
 
 
+def copy_ceiling(nbytes: int, reps: int = 10) -> float:
+    """Device-to-device copy bandwidth (read + write bytes / time) of a plain
+    copy of nbytes, torch on the library's device: the measured ceiling that is
+    reported next to the vendor peak."""
+    import torch
+    n = nbytes // 4
+    a = torch.empty(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    dt = e0.elapsed_time(e1) / 1e3 / reps
+    return 2 * nbytes / dt / 1e9
+
+
 def read_traffic(name):
     p = os.path.join(ROOT, "profiles", name)
     try:
@@ -98,7 +119,13 @@ def main():
     ap.add_argument("--alpha", default="newton", choices=["newton", "fixed2", "learned"])
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run the plain copy of one population buffer (counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        nb = 27 * 4 * args.size ** 3
+        print(json.dumps({"copy_bytes_each_way": nb, "copy_GBps": round(copy_ceiling(nb), 1)}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding is a design note only)"}))
@@ -123,6 +150,7 @@ def main():
     avg_launch_s = kernel_ms / 1e3 / args.steps
     achieved = BYTES_PER_NODE * n / avg_launch_s / 1e9
     traffic = read_traffic(f"traffic_d3q27_tgv{L}.json")
+    ceiling = copy_ceiling(27 * 4 * n)
     out = {
         "metric": "MLUPS & %HBM roofline, D3Q27 256^3; learned vs plain alpha iters at equal err",
         "value": round(mlups, 1),
@@ -143,6 +171,8 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic.get("bytes_per_launch") if traffic else None,
                      "kernel": "k27_step_periodic<fp32 shifted>",
+                     "measured_copy_GBps": round(ceiling, 1),
+                     "frac_of_copy": round(achieved / ceiling, 4),
                      "avg_launch_ms": round(avg_launch_s * 1e3, 4)},
         "alpha_stats": {"newton_iters_per_node": round(st["newton_iters"] / max(1, st["node_updates"]), 4),
                         "n_logpath": st["n_logpath"], "n_bound": st["n_bound"],

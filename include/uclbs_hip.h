@@ -62,7 +62,8 @@ typedef struct uclbs_config {
   int32_t lattice;      /* UCLBS_LATTICE_* */
   int32_t precision;    /* UCLBS_PREC_* */
   int32_t alpha_mode;   /* UCLBS_ALPHA_* */
-  int32_t alpha_maxit;  /* Newton cap (reference has none); 0 -> 30 */
+  int32_t alpha_maxit;  /* Newton cap (the reference has none,
+                           calculate_lagrangian_mults.f95:341); 0 -> 30 */
   double alpha_tol;     /* |g(alpha)| <= tol * sum f x^2; 0 -> 1e-10 */
   double nu;            /* kinematic viscosity, lattice units: beta = 1/(6 nu + 1) */
   int32_t reserved[8];
@@ -78,12 +79,15 @@ typedef struct uclbs_stats {
   int64_t n_noconv;        /* Newton hit the cap */
   int64_t n_fallback;      /* learned guess rejected, plain root-find used */
   int64_t n_logpath;       /* nodes whose alpha needed direct logarithms */
+  int64_t n_ibb_links;     /* populations rebuilt by interpolated bounce-back */
+  int64_t n_repair;        /* negative rebuilt populations replaced by |f|
+                              (interpolative_bounceback_bc.f95:2969-2973) */
 } uclbs_stats;
 
 const char *uclbs_last_error(void);
 int uclbs_abi_version(void);
 
-/* Context: uclbs_init of the SURVEY-proposed ABI.  Replaces the setup that
+/* Context: configuration and device stream.  Replaces the setup that
  * calcs_before_init/background_calcs perform (uclbs_main.f95:28-108). */
 int uclbs_create(const uclbs_config *cfg, void **ctx);
 int uclbs_destroy(void *ctx);
@@ -108,6 +112,45 @@ int uclbs_download_alpha(void *ctx, int lvl, double *alpha);
 /* Initialisers (looping/initialization/init_fi.f95:41 fills fi/fout with an
  * equilibrium; the Taylor-Green field is the benchmark's own). */
 int uclbs_init_tgv(void *ctx, int lvl, double u0, double noise_amp, uint64_t seed);
+
+/* ---- boundaries and link tables ------------------------------------------ */
+/* Face codes x-, y-, z-, x+, y+, z+ of a level's box, in the reference's
+ * encoding of edge links (lbgrid/state_directions.f95:194-684): inlet j is
+ * -100-j, outlet j is -200-j, -10 a bounce-back wall, -1000 freestream;
+ * UCLBS_FACE_PERIODIC marks an axis declared periodic in uclbs_define_level.
+ * Faces of non-periodic axes default to -10. */
+#define UCLBS_FACE_PERIODIC 2147483647
+int uclbs_set_faces(void *ctx, int lvl, const int32_t face_code[6]);
+/* Inlet / outlet / freestream states (rho, u, v, w) per boundary number; the
+ * library tabulates their equilibria, as inlet_outlet_basis_calcs.f90:90-125
+ * and freestream_basis_calcs.f90:35-194 tabulate inl_fieq / outl_fieq /
+ * fs_fieq.  fs_state may be NULL. */
+int uclbs_set_bc_states(void *ctx, int n_in, const double *in_states, int n_out,
+                        const double *out_states, const double *fs_state);
+/* Builds the level's link table on the device from a triangle soup in the
+ * layout of geometry/modules/geom_data.f95:9-12 (tri[t*9 + vertex*3 + axis],
+ * nrm[t*3 + axis]): node codes as lbgrid/fluid_or_solid.f95:86-442, wall-link
+ * q codes as lbgrid/inter_bb_setup.f95:44-246 and linkwise.f95:45-57, face and
+ * interior codes as lbgrid/state_directions.f95.  Node (a,b,c) of the box sits
+ * at origin + (a,b,c)*dx (amr_processes.f95:1155-1168).  The table stays
+ * resident; *n_tie counts rays re-cast because of a tie. */
+int uclbs_build_links(void *ctx, int lvl, const double origin[3], double dx, int64_t ntri,
+                      const double *tri, const double *nrm, int64_t *n_solid, int64_t *n_tie);
+/* Takes a reference-form table state[q*n + node] (int32, the `state` imultifab
+ * of lbgrid/state_directions.f95:716 restricted to the box) instead. */
+int uclbs_set_links(void *ctx, int lvl, const int32_t *state, int64_t n_nodes);
+int uclbs_download_links(void *ctx, int lvl, int32_t *state, int64_t n_nodes);
+/* fluid nodes with a boundary link, and solid/null nodes, of a level's table */
+int uclbs_link_counts(void *ctx, int lvl, int64_t *n_boundary, int64_t *n_solid);
+/* Uniform initial state: fluid nodes feq(rho, u), solid nodes the rest state
+ * (looping/initialization/multilevel_initialization.f90:185-284). */
+int uclbs_init_uniform(void *ctx, int lvl, double rho, double u, double v, double w);
+/* Level-`level` icosphere (20*4^level triangles) of the given radius and
+ * centre, rotated by a uniform random rotation drawn from Philox4x32-10 with
+ * the seed; outward normals.  With tri or nrm NULL only *ntri is set.  A
+ * generated stand-in for the STL import of geometry/geom_import.f95:48-133. */
+int uclbs_geom_icosphere(int level, double radius, const double center[3], uint64_t seed,
+                         double *tri, double *nrm, int64_t capacity, int64_t *ntri);
 
 /* Advance level lvl by nsteps fused stream+moments+alpha+collide steps:
  * the streaming / calculate_flow_variables / collision calls of

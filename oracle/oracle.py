@@ -39,6 +39,26 @@ class OStats(ctypes.Structure):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
 
 
+class OBox(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("dx", ctypes.c_double), ("origin", ctypes.c_double * 3),
+                ("face_code", ctypes.c_int32 * 6)]
+
+
+class BoxParams27(ctypes.Structure):
+    _fields_ = [("beta", ctypes.c_double), ("tol", ctypes.c_double), ("maxit", ctypes.c_int),
+                ("alpha_mode", ctypes.c_int), ("nthreads", ctypes.c_int),
+                ("n_in", ctypes.c_int), ("n_out", ctypes.c_int),
+                ("feq_in", ctypes.POINTER(ctypes.c_double)),
+                ("feq_out", ctypes.POINTER(ctypes.c_double)),
+                ("feq_fs", ctypes.POINTER(ctypes.c_double))]
+
+
+class BcStats(ctypes.Structure):
+    _fields_ = [("n_ibb_links", ctypes.c_int64), ("n_repair", ctypes.c_int64)]
+
+
+FACE_PERIODIC = 2147483647
 _lib = None
 
 
@@ -68,6 +88,12 @@ def lib() -> ctypes.CDLL:
                                       ctypes.POINTER(ctypes.c_uint32)]
         L.o_uniform.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64]
         L.o_uniform.restype = D
+        L.o_build_links27.argtypes = [ctypes.POINTER(OBox), I, dp, dp,
+                                      ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int64)]
+        L.o_build_links27.restype = ctypes.c_int64
+        L.o27_step_box.argtypes = [ctypes.POINTER(OBox), ctypes.POINTER(ctypes.c_int32), dp, dp,
+                                   ctypes.POINTER(BoxParams27), dp, ctypes.POINTER(OStats),
+                                   ctypes.POINTER(BcStats)]
         _lib = L
     return _lib
 
@@ -174,3 +200,63 @@ def macro27(f):
     out = [np.empty(n) for _ in range(4)]
     lib().o27_macro(n, _dp(f), *[_dp(o) for o in out])
     return out
+
+
+def make_box(shape, faces, dx=1.0, origin=(0.0, 0.0, 0.0)):
+    b = OBox()
+    b.nx, b.ny, b.nz = shape
+    b.dx = dx
+    b.origin = (ctypes.c_double * 3)(*origin)
+    b.face_code = (ctypes.c_int32 * 6)(*faces)
+    return b
+
+
+def build_links27(shape, faces, tri, nrm, dx=1.0, origin=(0.0, 0.0, 0.0)):
+    """Reference-form link table state (27, n) int32; returns (state, n_solid, n_tie)."""
+    b = make_box(shape, faces, dx, origin)
+    n = shape[0] * shape[1] * shape[2]
+    st = np.empty((27, n), dtype=np.int32)
+    tri = np.ascontiguousarray(tri, dtype=np.float64)
+    nrm = np.ascontiguousarray(nrm, dtype=np.float64)
+    ties = ctypes.c_int64()
+    ns = lib().o_build_links27(ctypes.byref(b), len(tri), _dp(tri) if len(tri) else None,
+                               _dp(nrm) if len(nrm) else None,
+                               st.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                               ctypes.byref(ties))
+    return st, int(ns), int(ties.value)
+
+
+def step_box27(shape, faces, state, f, beta, nsteps=1, inlets=(), outlets=(), freestream=None,
+               tol=1e-10, maxit=30, fixed=False, nthreads=0, want_alpha=True):
+    """Advance f (27, n) on a box with link table `state`; returns (f, alpha, stats, bcstats).
+    inlets / outlets / freestream are (rho, u, v, w) states tabulated with feq27."""
+    b = make_box(shape, faces)
+    n = shape[0] * shape[1] * shape[2]
+    fin = np.array([feq27(*s_) for s_ in inlets]).reshape(-1) if len(inlets) else np.zeros(1)
+    fou = np.array([feq27(*s_) for s_ in outlets]).reshape(-1) if len(outlets) else np.zeros(1)
+    ffs = feq27(*freestream) if freestream is not None else None
+    p = BoxParams27(beta, tol, maxit, int(fixed), nthreads, len(inlets), len(outlets),
+                    _dp(fin), _dp(fou), _dp(ffs) if ffs is not None else None)
+    st = np.ascontiguousarray(state, dtype=np.int32)
+    a = np.ascontiguousarray(f, dtype=np.float64).reshape(27, n).copy()
+    bb = np.empty_like(a)
+    alpha = np.empty(n) if want_alpha else None
+    ost, bst = OStats(), BcStats()
+    for _ in range(nsteps):
+        lib().o27_step_box(ctypes.byref(b), st.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                           _dp(a), _dp(bb), ctypes.byref(p),
+                           _dp(alpha) if want_alpha else None, ctypes.byref(ost),
+                           ctypes.byref(bst))
+        a, bb = bb, a
+    return a, alpha, ost.as_dict(), {"n_ibb_links": int(bst.n_ibb_links),
+                                     "n_repair": int(bst.n_repair)}
+
+
+def uniform27(state, rho, u):
+    """feq(rho, u) on fluid nodes, rest weights on solid / null nodes."""
+    n = state.shape[1]
+    f = np.repeat(feq27(rho, *u)[:, None], n, axis=1)
+    w = lattice27()[4]
+    solid = state[0] < 0
+    f[:, solid] = w[:, None]
+    return f

@@ -84,6 +84,41 @@ void o27_init_tgv(int nx, int ny, int nz, double u0, double noise_amp, uint64_t 
                   double *f);
 void o27_macro(int n, const double *f, double *rho, double *ux, double *uy, double *uz);
 
+/* ---------------- link tables (lbgrid/, linking/modules/linkwise.f95) --------- */
+/* Face codes of a box: the reference's edge classification (state_directions.f95
+ * :194-684): inlet j -> -100-j, outlet j -> -200-j, bounce-back wall -10,
+ * freestream -1000; O_FACE_PERIODIC wraps the axis instead. */
+#define O_FACE_PERIODIC 2147483647
+typedef struct {
+  int nx, ny, nz;      /* nodes per axis (nodal box, level 0) */
+  double dx;           /* node spacing */
+  double origin[3];    /* coordinate of node (0,0,0) (amrex_problo) */
+  int32_t face_code[6];/* x-, y-, z-, x+, y+, z+ */
+} o_box;
+/* state[q*n + node] in the reference encoding (component 0: node code 0 fluid /
+ * -1001 solid; q>0: link code).  Triangles: tri[t*9 + vertex*3 + axis],
+ * nrm[t*3 + axis].  Returns the number of solid nodes; *n_tie counts ties
+ * resolved by a deterministic re-cast. */
+int64_t o_build_links27(const o_box *bx, int ntri, const double *tri, const double *nrm,
+                        int32_t *state, int64_t *n_tie);
+int32_t o_wall_code(const o_box *bx, int ntri, const double *tri, const double *nrm,
+                    const double *tbb, int a, int b, int c, int cx, int cy, int cz, int *tie);
+int o_classify_node(const o_box *bx, int ntri, const double *tri, const double *nrm,
+                    const double *tbb, const double gbb[6], int a, int b, int c, int *tie);
+
+/* ---------------- D3Q27 step on a box with boundary links ---------------------- */
+typedef struct {
+  double beta, tol;
+  int maxit, alpha_mode, nthreads;
+  int n_in, n_out;
+  const double *feq_in;  /* [n_in][27]: inlet equilibria (inlet_outlet_basis_calcs.f90:113-119) */
+  const double *feq_out; /* [n_out][27] */
+  const double *feq_fs;  /* [27] freestream equilibrium (freestream_basis_calcs.f90:66-71) */
+} o27_box_params;
+typedef struct { int64_t n_ibb_links; int64_t n_repair; } o_bc_stats;
+void o27_step_box(const o_box *bx, const int32_t *state, const double *fin, double *fout,
+                  const o27_box_params *p, double *alpha_out, o_stats *st, o_bc_stats *bst);
+
 #ifdef __cplusplus
 }
 #endif

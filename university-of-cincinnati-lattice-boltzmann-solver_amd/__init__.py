@@ -23,8 +23,11 @@ EXPORTS = (
     "uclbs_define_level", "uclbs_upload_f", "uclbs_download_f", "uclbs_download_macro",
     "uclbs_set_keep_alpha", "uclbs_download_alpha", "uclbs_init_tgv",
     "uclbs_step_level", "uclbs_step_level_async", "uclbs_time_steps", "uclbs_sync",
-    "uclbs_get_stats", "uclbs_reset_stats",
+    "uclbs_get_stats", "uclbs_reset_stats", "uclbs_set_faces", "uclbs_set_bc_states",
+    "uclbs_build_links", "uclbs_set_links", "uclbs_download_links", "uclbs_link_counts",
+    "uclbs_init_uniform", "uclbs_geom_icosphere",
 )
+FACE_PERIODIC = 2147483647
 
 LATTICE = {"d3q27": 27, "d3q39": 39}
 PRECISION = {"fp32s": 0, "fp64": 1}
@@ -50,7 +53,7 @@ class Config(ctypes.Structure):
 class Stats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in (
         "node_updates", "newton_iters", "n_trivial", "n_bound", "n_nonpos", "n_noconv",
-        "n_fallback", "n_logpath")]
+        "n_fallback", "n_logpath", "n_ibb_links", "n_repair")]
 
     def as_dict(self) -> dict:
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
@@ -88,6 +91,14 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "uclbs_sync": (I, [P]),
         "uclbs_get_stats": (I, [P, ctypes.POINTER(Stats)]),
         "uclbs_reset_stats": (I, [P]),
+        "uclbs_set_faces": (I, [P, I, i3]),
+        "uclbs_set_bc_states": (I, [P, I, dp, I, dp, dp]),
+        "uclbs_build_links": (I, [P, I, dp, D, I64, dp, dp, ctypes.POINTER(I64), ctypes.POINTER(I64)]),
+        "uclbs_set_links": (I, [P, I, i3, I64]),
+        "uclbs_download_links": (I, [P, I, i3, I64]),
+        "uclbs_link_counts": (I, [P, I, ctypes.POINTER(I64), ctypes.POINTER(I64)]),
+        "uclbs_init_uniform": (I, [P, I, D, D, D, D]),
+        "uclbs_geom_icosphere": (I, [I, D, dp, U64, dp, dp, I64, ctypes.POINTER(I64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -99,6 +110,27 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
 
 def _dptr(a: np.ndarray):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _iptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+def icosphere(level: int, radius: float, center, seed: int):
+    """Triangle soup (tri (T,9), nrm (T,3)) of a randomly rotated icosphere."""
+    lib = load_library()
+    nt = ctypes.c_int64()
+    c3 = np.ascontiguousarray(center, dtype=np.float64)
+    rc = lib.uclbs_geom_icosphere(level, radius, _dptr(c3), seed, None, None, 0, ctypes.byref(nt))
+    if rc:
+        raise UclbsError(f"icosphere size query failed ({rc})")
+    tri = np.empty((nt.value, 9))
+    nrm = np.empty((nt.value, 3))
+    rc = lib.uclbs_geom_icosphere(level, radius, _dptr(c3), seed, _dptr(tri), _dptr(nrm),
+                                  nt.value, ctypes.byref(nt))
+    if rc:
+        raise UclbsError(f"icosphere failed ({rc})")
+    return tri, nrm
 
 
 class Solver:
@@ -151,6 +183,48 @@ class Solver:
         n = self.n_nodes(lvl)
         f = np.ascontiguousarray(f, dtype=np.float64).reshape(self.q, n)
         self._check(self.lib.uclbs_upload_f(self.ctx, lvl, _dptr(f), n))
+
+    def set_faces(self, lvl: int, codes):
+        c6 = (ctypes.c_int32 * 6)(*[int(v) for v in codes])
+        self._check(self.lib.uclbs_set_faces(self.ctx, lvl, c6))
+
+    def set_bc_states(self, inlets=(), outlets=(), freestream=None):
+        ins = np.ascontiguousarray(np.array(inlets, dtype=np.float64).reshape(-1, 4))
+        outs = np.ascontiguousarray(np.array(outlets, dtype=np.float64).reshape(-1, 4))
+        fs = None if freestream is None else np.ascontiguousarray(freestream, dtype=np.float64)
+        self._check(self.lib.uclbs_set_bc_states(
+            self.ctx, len(ins), _dptr(ins) if len(ins) else None, len(outs),
+            _dptr(outs) if len(outs) else None, _dptr(fs) if fs is not None else None))
+
+    def build_links(self, lvl: int, tri: np.ndarray, nrm: np.ndarray, origin=(0.0, 0.0, 0.0),
+                    dx: float = 1.0):
+        tri = np.ascontiguousarray(tri, dtype=np.float64)
+        nrm = np.ascontiguousarray(nrm, dtype=np.float64)
+        o3 = np.ascontiguousarray(origin, dtype=np.float64)
+        ns, nt = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self.lib.uclbs_build_links(
+            self.ctx, lvl, _dptr(o3), dx, len(tri), _dptr(tri) if len(tri) else None,
+            _dptr(nrm) if len(nrm) else None, ctypes.byref(ns), ctypes.byref(nt)))
+        return ns.value, nt.value
+
+    def set_links(self, lvl: int, state: np.ndarray):
+        n = self.n_nodes(lvl)
+        st = np.ascontiguousarray(state, dtype=np.int32).reshape(27, n)
+        self._check(self.lib.uclbs_set_links(self.ctx, lvl, _iptr(st), n))
+
+    def download_links(self, lvl: int) -> np.ndarray:
+        n = self.n_nodes(lvl)
+        out = np.empty((27, n), dtype=np.int32)
+        self._check(self.lib.uclbs_download_links(self.ctx, lvl, _iptr(out), n))
+        return out
+
+    def link_counts(self, lvl: int):
+        nb, ns = ctypes.c_int64(), ctypes.c_int64()
+        self._check(self.lib.uclbs_link_counts(self.ctx, lvl, ctypes.byref(nb), ctypes.byref(ns)))
+        return nb.value, ns.value
+
+    def init_uniform(self, lvl: int, rho: float, u):
+        self._check(self.lib.uclbs_init_uniform(self.ctx, lvl, rho, *[float(v) for v in u]))
 
     def download_f(self, lvl: int) -> np.ndarray:
         n = self.n_nodes(lvl)

@@ -52,18 +52,22 @@ __device__ __forceinline__ void wave_stats(unsigned long long* __restrict__ stat
   }
 }
 
-template <bool kShifted, bool kKeepAlpha>
+// kFlags: the level has link tables; only interior fluid nodes (flag 0) are
+// updated here, the boundary kernel updates the rest.
+template <bool kShifted, bool kKeepAlpha, bool kFlags>
 __global__ __launch_bounds__(256) void k27_step_periodic(const void* __restrict__ fin_,
                                                          void* __restrict__ fout_,
                                                          double* __restrict__ alpha_out,
                                                          unsigned long long* __restrict__ stats,
-                                                         Box b, StepArgs a) {
+                                                         Box b, StepArgs a,
+                                                         const uint8_t* __restrict__ flags) {
   using T = typename std::conditional<kShifted, float, double>::type;
   const T* __restrict__ fin = static_cast<const T*>(fin_);
   T* __restrict__ fout = static_cast<T*>(fout_);
   const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool active = gid < b.n;
-  const int64_t node = active ? gid : 0;
+  const int64_t node = gid < b.n ? gid : 0;
+  bool active = gid < b.n;
+  if constexpr (kFlags) active = active && flags[node] == 0;
   const int x = (int)(node % b.nx);
   const int64_t yz = node / b.nx;
   const int y = (int)(yz % b.ny);
@@ -226,6 +230,227 @@ __global__ void k27_init_tgv(void* __restrict__ dst_, Box b, double u0, double a
   }
 }
 
+
+// ------------------------------------------------------------------ boundaries
+// One thread per fluid node that has a boundary link.  The incoming population
+// f_i(x) (velocity c_i, from x - c_i) is chosen by the code s of the link from
+// x towards x - c_i, state[opp(i)][x] -- the dispatch of streaming.f95:365-1133
+// seen from the receiving node:
+//   s >= 0         pull fout_i(x - c_i)
+//   s = -10        halfway bounce-back fout_opp(i)(x) (the rule bounceback.f95
+//                  aims at; its in-place swap is not reproduced)
+//   -1xx / -2xx    inlet / outlet equilibrium table (inlet_bc.f95, outlet_bc.f95,
+//                  rho_v_bc.f95: fi(x,opp(k)) = inl_fieq(opp(k)))
+//   -1000          freestream equilibrium (freestream_edge_bc.f95:18-19)
+//   <= -1001       interpolated bounce-back:
+//                  q_k = |s_k + 1001|/1e9 (target_finder.f95:40); target
+//                  u_t = mean_k q_k u(x - c_k)/(q_k + 1) (target_finder.f95:36-83);
+//                  rho_t from the known populations, an unknown f_k replaced by
+//                  f_opp(k) (target_finder.f95:56-60); f_i = feq_opp(i)(rho_t,u_t)
+//                  (interpolative_bounceback_bc.f95:2917 + streaming.f95:1132)
+//                  + f_neq_i, f_neq_i = -3 tau rho_t w_i (c_ia c_ib - d_ab/3) du_b/dx_a
+//                  with du/dx_a from u(x +- e_a), a wall side at distance q with
+//                  u = 0; a negative value is replaced by its magnitude
+//                  (interpolative_bounceback_bc.f95:2969-2973) and counted
+//   other          fout_opp(i)(x)
+// u(y) are the moments of fout at y (conserved by the collision).  The node
+// then collides with the general-path collide.  All BC arithmetic is FP64.
+template <bool kShifted>
+__device__ __forceinline__ double ldf(const void* f, int i, int64_t n, int64_t k) {
+  if constexpr (kShifted) return W27[i] + (double)static_cast<const float*>(f)[(int64_t)i * n + k];
+  else return static_cast<const double*>(f)[(int64_t)i * n + k];
+}
+
+template <bool kShifted>
+__device__ void vel_at(const void* f, int64_t n, int64_t k, double u[3]) {
+  double r = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
+#pragma unroll
+  for (int i = 0; i < 27; ++i) {
+    const double v = ldf<kShifted>(f, i, n, k);
+    r += v;
+    jx += CX27[i] * v;
+    jy += CY27[i] * v;
+    jz += CZ27[i] * v;
+  }
+  u[0] = jx / r;
+  u[1] = jy / r;
+  u[2] = jz / r;
+}
+
+__device__ __forceinline__ int wrapc(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+__device__ void feq27_fp64(double rho, const double u[3], double* feq) {
+  double p[3][3];
+  for (int d = 0; d < 3; ++d) {
+    const double s = sqrt(1.0 + 3.0 * u[d] * u[d]);
+    const double bb = 2.0 - s;
+    p[d][1] = bb;
+    p[d][2] = bb * (2.0 * u[d] + s) / (1.0 - u[d]);
+    p[d][0] = bb * (1.0 - u[d]) / (2.0 * u[d] + s);
+  }
+#pragma unroll
+  for (int i = 0; i < 27; ++i)
+    feq[i] = rho * W27[i] * p[0][CX27[i] + 1] * p[1][CY27[i] + 1] * p[2][CZ27[i] + 1];
+}
+
+template <bool kShifted>
+__global__ __launch_bounds__(256) void k27_boundary(const void* __restrict__ fin,
+                                                    void* __restrict__ fout_,
+                                                    double* __restrict__ alpha_out,
+                                                    unsigned long long* __restrict__ stats,
+                                                    Box b, StepArgs a, BcArgs bc) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = j < bc.nlist;
+  const int64_t k = bc.list[active ? j : 0];
+  const int64_t n = b.n;
+  const int x = (int)(k % b.nx);
+  const int64_t yz = k / b.nx;
+  const int y = (int)(yz % b.ny), z = (int)(yz / b.ny);
+  double f[27];
+  bool unknown[27];
+  bool any_wall = false;
+#pragma unroll
+  for (int i = 0; i < 27; ++i) {
+    unknown[i] = false;
+    if (i == 0) { f[0] = ldf<kShifted>(fin, 0, n, k); continue; }
+    const int32_t s = bc.state[(int64_t)OPP27[i] * n + k];
+    if (s >= 0) {
+      const int64_t src = wrapc(x - CX27[i], b.nx) +
+                          (int64_t)b.nx * (wrapc(y - CY27[i], b.ny) + (int64_t)b.ny * wrapc(z - CZ27[i], b.nz));
+      f[i] = ldf<kShifted>(fin, i, n, src);
+    } else if (s == -10) {
+      f[i] = ldf<kShifted>(fin, OPP27[i], n, k);
+    } else if (s <= -100 && s >= -199 && -s - 100 <= bc.n_in) {
+      f[i] = bc.feq_in[(int64_t)(-s - 101) * 27 + i];
+    } else if (s <= -200 && s >= -299 && -s - 200 <= bc.n_out) {
+      f[i] = bc.feq_out[(int64_t)(-s - 201) * 27 + i];
+    } else if (s == -1000 && bc.feq_fs) {
+      f[i] = bc.feq_fs[i];
+    } else if (s <= -1001) {
+      unknown[i] = true;
+      any_wall = true;
+      f[i] = 0.0;
+    } else {
+      f[i] = ldf<kShifted>(fin, OPP27[i], n, k);
+    }
+  }
+  int nibb = 0, nrep = 0;
+  if (any_wall && active) {
+    double ut[3] = {0.0, 0.0, 0.0}, ux[3];
+    int cnt = 0;
+    vel_at<kShifted>(fin, n, k, ux);
+    for (int kk = 1; kk < 27; ++kk) {
+      const int32_t s = bc.state[(int64_t)kk * n + k];
+      if (s > -1001) continue;
+      const double q = fabs((double)s + 1001.0) / 1.0e9;
+      double uu[3];
+      if (bc.state[(int64_t)OPP27[kk] * n + k] >= 0) {
+        const int64_t src = wrapc(x - CX27[kk], b.nx) +
+                            (int64_t)b.nx * (wrapc(y - CY27[kk], b.ny) + (int64_t)b.ny * wrapc(z - CZ27[kk], b.nz));
+        vel_at<kShifted>(fin, n, src, uu);
+      } else {
+        uu[0] = ux[0]; uu[1] = ux[1]; uu[2] = ux[2];
+      }
+      for (int d = 0; d < 3; ++d) ut[d] += q * uu[d] / (q + 1.0);
+      cnt++;
+    }
+    for (int d = 0; d < 3; ++d) ut[d] /= cnt;
+    double rt = 0.0;
+    for (int kk = 0; kk < 27; ++kk) {
+      if (!unknown[kk]) rt += f[kk];
+      else if (!unknown[OPP27[kk]]) rt += f[OPP27[kk]];
+      else rt += ldf<kShifted>(fin, OPP27[kk], n, k);
+    }
+    double G[3][3];
+    for (int ax = 0; ax < 3; ++ax) {
+      const int dp = 1 + ax, dm = 4 + ax;
+      const int32_t sp = bc.state[(int64_t)dp * n + k], sm = bc.state[(int64_t)dm * n + k];
+      const int ex = ax == 0, ey = ax == 1, ez = ax == 2;
+      double hp, hm, up[3], um[3];
+      if (sp >= 0) {
+        hp = 1.0;
+        vel_at<kShifted>(fin, n, wrapc(x + ex, b.nx) + (int64_t)b.nx * (wrapc(y + ey, b.ny) + (int64_t)b.ny * wrapc(z + ez, b.nz)), up);
+      } else if (sp <= -1001) {
+        hp = fabs((double)sp + 1001.0) / 1.0e9;
+        up[0] = up[1] = up[2] = 0.0;
+      } else {
+        hp = 1.0;
+        up[0] = ut[0]; up[1] = ut[1]; up[2] = ut[2];
+      }
+      if (sm >= 0) {
+        hm = 1.0;
+        vel_at<kShifted>(fin, n, wrapc(x - ex, b.nx) + (int64_t)b.nx * (wrapc(y - ey, b.ny) + (int64_t)b.ny * wrapc(z - ez, b.nz)), um);
+      } else if (sm <= -1001) {
+        hm = fabs((double)sm + 1001.0) / 1.0e9;
+        um[0] = um[1] = um[2] = 0.0;
+      } else {
+        hm = 1.0;
+        um[0] = ut[0]; um[1] = ut[1]; um[2] = ut[2];
+      }
+      for (int c2 = 0; c2 < 3; ++c2) G[ax][c2] = (up[c2] - um[c2]) / (hp + hm);
+    }
+    double feq_t[27];
+    feq27_fp64(rt, ut, feq_t);
+    const double tau = 1.0 / (2.0 * a.beta);
+    for (int i = 1; i < 27; ++i) {
+      if (!unknown[i]) continue;
+      const int c[3] = {CX27[i], CY27[i], CZ27[i]};
+      double qs = 0.0;
+      for (int p = 0; p < 3; ++p)
+        for (int r = 0; r < 3; ++r)
+          qs += ((double)(c[p] * c[r]) - (p == r ? 1.0 / 3.0 : 0.0)) * G[p][r];
+      double v = feq_t[OPP27[i]] - 3.0 * tau * rt * W27[i] * qs;
+      if (v < 0.0) { v = -v; nrep++; }
+      f[i] = v;
+      nibb++;
+    }
+  }
+  AlphaCfg ac{a.tol, a.maxit, a.alpha_mode, a.thr};
+  double rho, u[3];
+  AlphaOut ao;
+  if constexpr (kShifted) {
+    float fs[27];
+#pragma unroll
+    for (int i = 0; i < 27; ++i) fs[i] = (float)(f[i] - W27[i]);
+    ao = collide27_shifted<true>(fs, a.beta, ac, 2.0, rho, u);
+    if (active) {
+#pragma unroll
+      for (int i = 0; i < 27; ++i) static_cast<float*>(fout_)[(int64_t)i * n + k] = fs[i];
+    }
+  } else {
+    ao = collide27_fp64(f, a.beta, ac, 2.0, rho, u);
+    if (active) {
+#pragma unroll
+      for (int i = 0; i < 27; ++i) static_cast<double*>(fout_)[(int64_t)i * n + k] = f[i];
+    }
+  }
+  if (active && alpha_out) alpha_out[k] = ao.alpha;
+  wave_stats(stats, active, ao.iters, ao.status, false, ao.logpath);
+  if (active && (nibb || nrep)) {
+    unsigned long long* sl = stats + (size_t)((blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64) &
+                                              (kStatSlots - 1)) * kStatFields;
+    if (nibb) atomicAdd(sl + ST_IBB, (unsigned long long)nibb);
+    if (nrep) atomicAdd(sl + ST_REPAIR, (unsigned long long)nrep);
+  }
+}
+
+// uniform state: fluid nodes feq(rho, u), solid / null nodes the rest state w
+template <bool kShifted>
+__global__ void k27_init_uniform(void* __restrict__ dst, Box b, const uint8_t* __restrict__ flags,
+                                 double rho, double ux, double uy, double uz) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= b.n) return;
+  const bool solid = flags && flags[k] == 2;
+  double u[3] = {solid ? 0.0 : ux, solid ? 0.0 : uy, solid ? 0.0 : uz};
+  double feq[27];
+  feq27_fp64(solid ? 1.0 : rho, u, feq);
+#pragma unroll
+  for (int i = 0; i < 27; ++i) {
+    if constexpr (kShifted) static_cast<float*>(dst)[(int64_t)i * b.n + k] = (float)(feq[i] - W27[i]);
+    else static_cast<double*>(dst)[(int64_t)i * b.n + k] = feq[i];
+  }
+}
+
 inline unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
 }  // namespace
@@ -234,13 +459,40 @@ hipError_t launch_d3q27_step_periodic(const Box& b, const void* fin, void* fout,
                                       unsigned long long* stats, const StepArgs& a,
                                       hipStream_t s) {
   const unsigned g = blocks_for(b.n, 256);
+  const uint8_t* nf = nullptr;
   if (a.shifted) {
-    if (alpha) k27_step_periodic<true, true><<<g, 256, 0, s>>>(fin, fout, alpha, stats, b, a);
-    else k27_step_periodic<true, false><<<g, 256, 0, s>>>(fin, fout, alpha, stats, b, a);
+    if (alpha) k27_step_periodic<true, true, false><<<g, 256, 0, s>>>(fin, fout, alpha, stats, b, a, nf);
+    else k27_step_periodic<true, false, false><<<g, 256, 0, s>>>(fin, fout, alpha, stats, b, a, nf);
   } else {
-    if (alpha) k27_step_periodic<false, true><<<g, 256, 0, s>>>(fin, fout, alpha, stats, b, a);
-    else k27_step_periodic<false, false><<<g, 256, 0, s>>>(fin, fout, alpha, stats, b, a);
+    if (alpha) k27_step_periodic<false, true, false><<<g, 256, 0, s>>>(fin, fout, alpha, stats, b, a, nf);
+    else k27_step_periodic<false, false, false><<<g, 256, 0, s>>>(fin, fout, alpha, stats, b, a, nf);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_d3q27_step_links(const Box& b, const void* fin, void* fout, double* alpha,
+                                   unsigned long long* stats, const StepArgs& a,
+                                   const BcArgs& bc, hipStream_t s) {
+  const unsigned g = blocks_for(b.n, 256);
+  if (a.shifted) {
+    if (alpha) k27_step_periodic<true, true, true><<<g, 256, 0, s>>>(fin, fout, alpha, stats, b, a, bc.flags);
+    else k27_step_periodic<true, false, true><<<g, 256, 0, s>>>(fin, fout, alpha, stats, b, a, bc.flags);
+  } else {
+    if (alpha) k27_step_periodic<false, true, true><<<g, 256, 0, s>>>(fin, fout, alpha, stats, b, a, bc.flags);
+    else k27_step_periodic<false, false, true><<<g, 256, 0, s>>>(fin, fout, alpha, stats, b, a, bc.flags);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || bc.nlist == 0) return e;
+  const unsigned gb = blocks_for(bc.nlist, 256);
+  if (a.shifted) k27_boundary<true><<<gb, 256, 0, s>>>(fin, fout, alpha, stats, b, a, bc);
+  else k27_boundary<false><<<gb, 256, 0, s>>>(fin, fout, alpha, stats, b, a, bc);
+  return hipGetLastError();
+}
+
+hipError_t launch_d3q27_init_uniform(const Box& b, void* f, int shifted, const uint8_t* flags,
+                                     double rho, double ux, double uy, double uz, hipStream_t s) {
+  if (shifted) k27_init_uniform<true><<<blocks_for(b.n, 256), 256, 0, s>>>(f, b, flags, rho, ux, uy, uz);
+  else k27_init_uniform<false><<<blocks_for(b.n, 256), 256, 0, s>>>(f, b, flags, rho, ux, uy, uz);
   return hipGetLastError();
 }
 

@@ -1,0 +1,259 @@
+// Device-side link / neighbour table builder (compiled with -ffp-contract=off).
+//
+// Builds the reference's `state` table, state[q*n + node] in int32, for one
+// nodal box directly in HBM:
+//   component 0, node code: lbgrid/fluid_or_solid.f95:86-442.  A ray from the
+//     node away from the grid centre (utilities/ray_casting_away.f95), length
+//     100 x the largest extent, box-tested (utilities/box_test.f95), cut with
+//     each triangle plane (fluid_or_solid.f95:176-196) and sign-tested
+//     (utilities/tri_volume.f95); odd parity = solid (-1001), even = fluid (0);
+//     one odd edge hit with even crossings counts as a crossing (:393-399).
+//   components 1..26, link codes: lbgrid/state_directions.f95 -- a link to a
+//     solid node gets the inter_bb_setup.f95:44-246 code
+//     -1001 - floor(|crossing - node| / (|c| dx) * 1e9) of the first triangle
+//     (index order) the segment crosses, or 0 when none is crossed; a link
+//     leaving the box takes the face code (faces in the order x-, y-, z-, x+,
+//     y+, z+, :194-684) unless the face is periodic; any other link takes the
+//     neighbour's node code (:687-716); a solid node gets -666 everywhere (:87).
+// Ties that make the reference re-cast with a time-seeded random ray
+// (utilities/ray_casting.f95) are re-cast here along three fixed directions
+// and counted.  One thread per node; the triangle soup and its bounding boxes
+// are read in index order by every thread (the same addresses across a
+// wavefront, so they stream through the cache).  The arithmetic is written in
+// the reference's expression order with contraction off, so the table is
+// bit-identical to the CPU restatement.
+#pragma clang fp contract(off)
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernels.h"
+#include "lattice.h"
+
+namespace uclbs {
+namespace {
+
+constexpr double kPlusEps = 1.24e-16;  // general/modules/precise.f95:13
+constexpr double kMinusEps = -1.24e-16;
+constexpr double kVSmall = 1.0e-14;    // precise.f95:16
+
+__device__ __forceinline__ bool box_out(const double* p1, const double* p2, const double* bb) {
+  for (int d = 0; d < 3; ++d) {
+    const double mx = bb[2 * d], mn = bb[2 * d + 1];
+    if (p1[d] >= mx + kPlusEps && p2[d] >= mx + kPlusEps) return true;
+    if (p1[d] <= mn + kMinusEps && p2[d] <= mn + kMinusEps) return true;
+  }
+  return false;
+}
+
+__device__ __forceinline__ int tri_volume(const double* xc, const double* e, const double* t) {
+  const double x1 = t[0] - e[0], x2 = t[3] - e[0], x3 = t[6] - e[0];
+  const double y1 = t[1] - e[1], y2 = t[4] - e[1], y3 = t[7] - e[1];
+  const double z1 = t[2] - e[2], z2 = t[5] - e[2], z3 = t[8] - e[2];
+  const double px = xc[0] - e[0], py = xc[1] - e[1], pz = xc[2] - e[2];
+  double v[3];
+  v[0] = px * (y1 * z2 - z1 * y2) + py * (z1 * x2 - x1 * z2) + pz * (x1 * y2 - y1 * x2);
+  v[1] = -px * (y1 * z3 - z1 * y3) - py * (z1 * x3 - x1 * z3) - pz * (x1 * y3 - y1 * x3);
+  v[2] = px * (y2 * z3 - z2 * y3) + py * (z2 * x3 - x2 * z3) + pz * (x2 * y3 - y2 * x3);
+  int zc = 0;
+  for (int i = 0; i < 3; ++i)
+    if (v[i] >= kMinusEps && v[i] <= kPlusEps) zc++;
+  if (v[0] > 0.0 && v[1] > 0.0 && v[2] > 0.0) return 1;
+  if (v[0] < 0.0 && v[1] < 0.0 && v[2] < 0.0) return 1;
+  if (zc == 1) return -1;
+  if (zc == 2) return -2;
+  if (zc == 3) return -3;
+  return 0;
+}
+
+// parity of one cast: 0 even, 1 odd, -1 tie
+__device__ int cast_parity(const double* e1, const double* e2, int ntri,
+                           const double* __restrict__ tri, const double* __restrict__ nrm,
+                           const double* __restrict__ tbb) {
+  int cross = 0, edge = 0;
+  for (int i = 0; i < ntri; ++i) {
+    if (box_out(e1, e2, tbb + 6 * (size_t)i)) continue;
+    const double* t = tri + 9 * (size_t)i;
+    const double* n = nrm + 3 * (size_t)i;
+    const double dot = n[0] * t[0] + n[1] * t[1] + n[2] * t[2];
+    const double upper = dot - (e1[0] * n[0] + e1[1] * n[1] + e1[2] * n[2]);
+    const double lower =
+        (e2[0] - e1[0]) * n[0] + (e2[1] - e1[1]) * n[1] + (e2[2] - e1[2]) * n[2];
+    if (lower >= kMinusEps && lower <= kPlusEps) return -1;
+    const double pc = upper / lower;
+    if (pc > 0.0 && pc < 1.0) {
+      const double xc[3] = {e1[0] + pc * (e2[0] - e1[0]), e1[1] + pc * (e2[1] - e1[1]),
+                            e1[2] + pc * (e2[2] - e1[2])};
+      if (upper >= lower + kMinusEps && upper <= lower + kPlusEps) return -1;
+      const int v = tri_volume(xc, e1, t);
+      if (v == 1) cross++;
+      else if (v == -1) edge++;
+      else if (v <= -2) return -1;
+    } else if (upper >= -kVSmall && upper <= kVSmall) {
+      return -1;
+    }
+  }
+  if (edge >= 2) return -1;
+  if ((edge % 2) == 1 && (cross % 2) == 0) { edge = 0; cross++; }
+  edge /= 2;
+  return (cross + edge) % 2;
+}
+
+__constant__ double kRecast[3][3] = {{0.5773502691896258, 0.6311775151936213, 0.5179036655461733},
+                                     {-0.7071067811865476, 0.3826834323650898, 0.5946035575013605},
+                                     {0.2672612419124244, -0.5345224838248488, 0.8017837257372732}};
+
+__global__ __launch_bounds__(256) void k_classify(LinkGeom g, int32_t* __restrict__ state,
+                                                  unsigned long long* __restrict__ counts) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= g.box.n) return;
+  const int a = (int)(k % g.box.nx);
+  const int64_t yz = k / g.box.nx;
+  const int b = (int)(yz % g.box.ny), c = (int)(yz / g.box.ny);
+  const double e1[3] = {g.origin[0] + (double)a * g.dx, g.origin[1] + (double)b * g.dx,
+                        g.origin[2] + (double)c * g.dx};
+  double cen[3], ext = 0.0;
+  const int nd[3] = {g.box.nx, g.box.ny, g.box.nz};
+  for (int d = 0; d < 3; ++d) {
+    const double lo = g.origin[d], hi = g.origin[d] + (double)(nd[d] - 1) * g.dx;
+    cen[d] = (lo + hi) / 2.0;
+    if (hi - lo > ext) ext = hi - lo;
+  }
+  const double L = fabs(100.0 * ext);
+  const double dl[3] = {e1[0] - cen[0], e1[1] - cen[1], e1[2] - cen[2]};
+  const double dist = sqrt(dl[0] * dl[0] + dl[1] * dl[1] + dl[2] * dl[2]);
+  int solid = 0, tie = 0;
+  if (g.ntri > 0) {
+    for (int attempt = 0; attempt < 4; ++attempt) {
+      double e2[3];
+      if (attempt == 0 && dist > 0.0) {
+        for (int d = 0; d < 3; ++d) e2[d] = e1[d] + L * dl[d] / dist;
+      } else {
+        const int r = attempt == 0 ? 0 : attempt - 1;
+        for (int d = 0; d < 3; ++d) e2[d] = e1[d] + L * kRecast[r][d];
+        if (attempt == 0) tie = 1;
+      }
+      if (box_out(e1, e2, g.gbb)) { solid = 0; break; }
+      const int p = cast_parity(e1, e2, g.ntri, g.tri, g.nrm, g.tbb);
+      if (p >= 0) { solid = p; break; }
+      tie = 1;
+    }
+  }
+  state[k] = solid ? -1001 : 0;
+  if (solid) atomicAdd(counts + 0, 1ull);
+  if (tie) atomicAdd(counts + 1, 1ull);
+}
+
+__device__ int32_t wall_code(const LinkGeom& g, int a, int b, int c, int cx, int cy, int cz,
+                             int* tie) {
+  const double e1[3] = {g.origin[0] + (double)a * g.dx, g.origin[1] + (double)b * g.dx,
+                        g.origin[2] + (double)c * g.dx};
+  const double e2[3] = {e1[0] + (double)cx * g.dx, e1[1] + (double)cy * g.dx,
+                        e1[2] + (double)cz * g.dx};
+  const double link = sqrt((double)(cx * cx + cy * cy + cz * cz)) * g.dx;
+  for (int i = 0; i < g.ntri; ++i) {
+    if (box_out(e1, e2, g.tbb + 6 * (size_t)i)) continue;
+    const double* t = g.tri + 9 * (size_t)i;
+    const double* n = g.nrm + 3 * (size_t)i;
+    const double dot = n[0] * t[0] + n[1] * t[1] + n[2] * t[2];
+    const double upper = dot - (e1[0] * n[0] + e1[1] * n[1] + e1[2] * n[2]);
+    const double lower =
+        (e2[0] - e1[0]) * n[0] + (e2[1] - e1[1]) * n[1] + (e2[2] - e1[2]) * n[2];
+    if (lower >= kMinusEps && lower <= kPlusEps) continue;
+    const double pc = upper / lower;
+    if (pc > 0.0 && pc < 1.0 + kPlusEps) {
+      const double xc[3] = {e1[0] + pc * (e2[0] - e1[0]), e1[1] + pc * (e2[1] - e1[1]),
+                            e1[2] + pc * (e2[2] - e1[2])};
+      bool hit;
+      if (upper >= lower - kVSmall && upper <= lower + kVSmall) { hit = true; *tie = 1; }
+      else hit = tri_volume(xc, e1, t) != 0;
+      if (hit) {
+        const double d = sqrt((e1[0] - xc[0]) * (e1[0] - xc[0]) +
+                              (e1[1] - xc[1]) * (e1[1] - xc[1]) +
+                              (e1[2] - xc[2]) * (e1[2] - xc[2]));
+        return (int32_t)(-1001 - (int64_t)floor(d / link * 1e9));
+      }
+    } else if (upper >= -kVSmall && upper <= kVSmall) {
+      *tie = 1;
+    }
+  }
+  return 0;
+}
+
+__global__ __launch_bounds__(256) void k_links(LinkGeom g, int32_t* __restrict__ state,
+                                               unsigned long long* __restrict__ counts) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= g.box.n) return;
+  const int a = (int)(k % g.box.nx);
+  const int64_t yz = k / g.box.nx;
+  const int b = (int)(yz % g.box.ny), c = (int)(yz / g.box.ny);
+  const int64_t n = g.box.n;
+  const int32_t own = state[k];
+  int ties = 0;
+  for (int i = 1; i < 27; ++i) {
+    int32_t code;
+    if (own == -1001 || own == -666) {
+      code = -666;
+    } else {
+      const int cx = CX27[i], cy = CY27[i], cz = CZ27[i];
+      int na = a + cx, nb = b + cy, nc = c + cz;
+      const bool out[6] = {na < 0, nb < 0, nc < 0, na > g.box.nx - 1, nb > g.box.ny - 1,
+                           nc > g.box.nz - 1};
+      int face = -1;
+      for (int f = 0; f < 6; ++f)
+        if (out[f] && g.face_code[f] != kFacePeriodic) { face = f; break; }
+      if (face >= 0) {
+        code = g.face_code[face];
+      } else {
+        na = (na + g.box.nx) % g.box.nx;
+        nb = (nb + g.box.ny) % g.box.ny;
+        nc = (nc + g.box.nz) % g.box.nz;
+        const int32_t nbr = state[na + (int64_t)g.box.nx * (nb + (int64_t)g.box.ny * nc)];
+        if (nbr == -1001) {
+          int tie = 0;
+          code = wall_code(g, a, b, c, cx, cy, cz, &tie);
+          ties += tie;
+        } else {
+          code = nbr;
+        }
+      }
+    }
+    state[(int64_t)i * n + k] = code;
+  }
+  if (ties) atomicAdd(counts + 1, (unsigned long long)ties);
+}
+
+// node flags: 0 interior fluid (every link a fluid code), 1 fluid with a
+// boundary link, 2 solid or null
+__global__ void k_flags(const int32_t* __restrict__ state, int64_t n, uint8_t* __restrict__ flags) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  uint8_t fl = 0;
+  if (state[k] < 0) {
+    fl = 2;
+  } else {
+    for (int i = 1; i < 27; ++i)
+      if (state[(int64_t)i * n + k] < 0) fl = 1;
+  }
+  flags[k] = fl;
+}
+
+inline unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+}  // namespace
+
+hipError_t launch_build_links27(const LinkGeom& g, int32_t* state, unsigned long long* counts,
+                                hipStream_t s) {
+  k_classify<<<blocks_for(g.box.n, 256), 256, 0, s>>>(g, state, counts);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  k_links<<<blocks_for(g.box.n, 256), 256, 0, s>>>(g, state, counts);
+  return hipGetLastError();
+}
+
+hipError_t launch_flags27(const int32_t* state, int64_t n, uint8_t* flags, hipStream_t s) {
+  k_flags<<<blocks_for(n, 256), 256, 0, s>>>(state, n, flags);
+  return hipGetLastError();
+}
+
+}  // namespace uclbs

@@ -9,9 +9,11 @@
 namespace uclbs {
 
 constexpr int kStatSlots = 64;   // per-wave counters are spread over this many slots
-constexpr int kStatFields = 8;   // order of uclbs_stats
+constexpr int kStatFields = 10;  // order of uclbs_stats
 enum StatField : int { ST_NODES = 0, ST_ITERS, ST_TRIVIAL, ST_BOUND, ST_NONPOS, ST_NOCONV,
-                       ST_FALLBACK, ST_LOGPATH };
+                       ST_FALLBACK, ST_LOGPATH, ST_IBB, ST_REPAIR };
+
+constexpr int32_t kFacePeriodic = 2147483647;  // face code: wrap instead of a boundary
 
 struct Box {
   int nx, ny, nz;
@@ -35,6 +37,41 @@ struct StepArgs {
 hipError_t launch_d3q27_step_periodic(const Box& b, const void* fin, void* fout, double* alpha,
                                       unsigned long long* stats, const StepArgs& a,
                                       hipStream_t s);
+// Boundary data of a level whose links are defined: the reference-form link
+// table, the list of fluid nodes with a boundary link, and the equilibrium
+// populations of the inlets / outlets / freestream (full FP64 values).
+struct BcArgs {
+  const int32_t* state;   // [27][n]
+  const uint8_t* flags;   // [n]: 0 interior fluid, 1 boundary fluid, 2 solid/null
+  const uint32_t* list;   // boundary fluid nodes
+  int64_t nlist;
+  int n_in, n_out;
+  const double* feq_in;   // [n_in][27]
+  const double* feq_out;  // [n_out][27]
+  const double* feq_fs;   // [27] or null
+};
+
+struct LinkGeom {
+  Box box;
+  double origin[3];
+  double dx;
+  int32_t face_code[6];  // x-, y-, z-, x+, y+, z+
+  int ntri;
+  const double* tri;  // [ntri][9]
+  const double* nrm;  // [ntri][3]
+  const double* tbb;  // [ntri][6]: xmax, xmin, ymax, ymin, zmax, zmin
+  double gbb[6];      // geometry bounding box, same order
+};
+
+hipError_t launch_build_links27(const LinkGeom& g, int32_t* state, unsigned long long* counts,
+                                hipStream_t s);
+hipError_t launch_flags27(const int32_t* state, int64_t n, uint8_t* flags, hipStream_t s);
+hipError_t launch_d3q27_step_links(const Box& b, const void* fin, void* fout, double* alpha,
+                                   unsigned long long* stats, const StepArgs& a,
+                                   const BcArgs& bc, hipStream_t s);
+hipError_t launch_d3q27_init_uniform(const Box& b, void* f, int shifted, const uint8_t* flags,
+                                     double rho, double ux, double uy, double uz, hipStream_t s);
+
 // general-path fixup for the nodes the step queued (FP32 storage only)
 hipError_t launch_d3q27_fixup(const Box& b, void* fout, double* alpha,
                               unsigned long long* stats, const StepArgs& a, hipStream_t s);

@@ -12,6 +12,7 @@
 
 #include "../../include/uclbs_hip.h"
 #include "kernels.h"
+#include "lattice.h"
 
 namespace {
 
@@ -45,6 +46,13 @@ struct Level {
   unsigned int* defer_count = nullptr;  // [2], ping-pong by step parity
   unsigned int* defer_list = nullptr;   // n entries
   int parity = 0;
+  // link tables (reference form) and the compact form the kernels use
+  int32_t face_code[6] = {0, 0, 0, 0, 0, 0};
+  int32_t* state = nullptr;   // [27][n]
+  uint8_t* flags = nullptr;   // [n]
+  uint32_t* blist = nullptr;  // boundary fluid nodes, ascending
+  int64_t nb = 0, nsolid = 0;
+  bool has_links = false;
 };
 
 struct Ctx {
@@ -54,6 +62,11 @@ struct Ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   std::vector<Level> levels;
   unsigned long long* d_stats = nullptr;
+  // inlet / outlet / freestream equilibria (full FP64 populations)
+  int n_in = 0, n_out = 0;
+  double* d_feq_in = nullptr;
+  double* d_feq_out = nullptr;
+  double* d_feq_fs = nullptr;
   bool keep_alpha = false;
   int64_t steps_done = 0;
 };
@@ -83,6 +96,11 @@ void free_level(Level& L) {
   if (L.alpha) { (void)hipFree(L.alpha); L.alpha = nullptr; }
   if (L.defer_count) { (void)hipFree(L.defer_count); L.defer_count = nullptr; }
   if (L.defer_list) { (void)hipFree(L.defer_list); L.defer_list = nullptr; }
+  if (L.state) { (void)hipFree(L.state); L.state = nullptr; }
+  if (L.flags) { (void)hipFree(L.flags); L.flags = nullptr; }
+  if (L.blist) { (void)hipFree(L.blist); L.blist = nullptr; }
+  L.has_links = false;
+  L.nb = L.nsolid = 0;
   L.defined = false;
 }
 
@@ -99,6 +117,73 @@ uclbs::StepArgs step_args(const Ctx* c) {
   return a;
 }
 
+uclbs::BcArgs bc_args(const Ctx* c, const Level* L) {
+  uclbs::BcArgs bc{};
+  bc.state = L->state;
+  bc.flags = L->flags;
+  bc.list = L->blist;
+  bc.nlist = L->nb;
+  bc.n_in = c->n_in;
+  bc.n_out = c->n_out;
+  bc.feq_in = c->d_feq_in;
+  bc.feq_out = c->d_feq_out;
+  bc.feq_fs = c->d_feq_fs;
+  return bc;
+}
+
+// product-form D3Q27 equilibrium on the host (FP64), for the boundary tables
+void feq27_host(double rho, const double u[3], double* feq) {
+  double p[3][3];
+  for (int d = 0; d < 3; ++d) {
+    const double s = std::sqrt(1.0 + 3.0 * u[d] * u[d]);
+    const double bb = 2.0 - s;
+    p[d][1] = bb;
+    p[d][2] = bb * (2.0 * u[d] + s) / (1.0 - u[d]);
+    p[d][0] = bb * (1.0 - u[d]) / (2.0 * u[d] + s);
+  }
+  for (int i = 0; i < 27; ++i)
+    feq[i] = rho * uclbs::W27[i] * p[0][uclbs::CX27[i] + 1] * p[1][uclbs::CY27[i] + 1] *
+             p[2][uclbs::CZ27[i] + 1];
+}
+
+size_t level_bytes(const Ctx* c, const Level* L) {
+  return (size_t)L->box.n * q_of(c) * elem_size(c);
+}
+
+// the other population buffer mirrors the current one after a state change, so
+// nodes no kernel writes (solid, null) hold valid values in both
+int mirror_buffers(Ctx* c, Level* L) {
+  HIP_TRY(hipMemcpyAsync(L->f[1 - L->cur], L->f[L->cur], level_bytes(c, L),
+                         hipMemcpyDeviceToDevice, c->stream));
+  return UCLBS_OK;
+}
+
+// flags + ascending boundary list from the link table
+int finalize_links(Ctx* c, Level* L) {
+  const int64_t n = L->box.n;
+  if (!L->flags) HIP_TRY(hipMalloc(&L->flags, (size_t)n));
+  HIP_TRY(uclbs::launch_flags27(L->state, n, L->flags, c->stream));
+  std::vector<uint8_t> hf((size_t)n);
+  HIP_TRY(hipMemcpyAsync(hf.data(), L->flags, (size_t)n, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  std::vector<uint32_t> list;
+  int64_t nsolid = 0;
+  for (int64_t k = 0; k < n; ++k) {
+    if (hf[k] == 1) list.push_back((uint32_t)k);
+    nsolid += hf[k] == 2;
+  }
+  if (L->blist) { (void)hipFree(L->blist); L->blist = nullptr; }
+  L->nb = (int64_t)list.size();
+  L->nsolid = nsolid;
+  if (L->nb) {
+    HIP_TRY(hipMalloc(&L->blist, list.size() * sizeof(uint32_t)));
+    HIP_TRY(hipMemcpy(L->blist, list.data(), list.size() * sizeof(uint32_t),
+                      hipMemcpyHostToDevice));
+  }
+  L->has_links = true;
+  return UCLBS_OK;
+}
+
 int launch_steps(Ctx* c, Level* L, int nsteps) {
   uclbs::StepArgs a = step_args(c);
   a.defer_count = L->defer_count;
@@ -106,13 +191,18 @@ int launch_steps(Ctx* c, Level* L, int nsteps) {
   for (int s = 0; s < nsteps; ++s) {
     if (c->cfg.lattice != UCLBS_LATTICE_D3Q27)
       return set_err(UCLBS_E_UNSUPPORTED, "lattice %d not supported by step", c->cfg.lattice);
-    if (!(L->box.per[0] && L->box.per[1] && L->box.per[2]))
+    if (!L->has_links && !(L->box.per[0] && L->box.per[1] && L->box.per[2]))
       return set_err(UCLBS_E_UNSUPPORTED,
-                     "level without boundary tables must be periodic on all axes");
+                     "level without link tables must be periodic on all axes");
     a.parity = L->parity;
-    HIP_TRY(uclbs::launch_d3q27_step_periodic(L->box, L->f[L->cur], L->f[1 - L->cur],
-                                              c->keep_alpha ? L->alpha : nullptr, c->d_stats,
-                                              a, c->stream));
+    if (L->has_links)
+      HIP_TRY(uclbs::launch_d3q27_step_links(L->box, L->f[L->cur], L->f[1 - L->cur],
+                                             c->keep_alpha ? L->alpha : nullptr, c->d_stats, a,
+                                             bc_args(c, L), c->stream));
+    else
+      HIP_TRY(uclbs::launch_d3q27_step_periodic(L->box, L->f[L->cur], L->f[1 - L->cur],
+                                                c->keep_alpha ? L->alpha : nullptr, c->d_stats,
+                                                a, c->stream));
     if (a.shifted)
       HIP_TRY(uclbs::launch_d3q27_fixup(L->box, L->f[1 - L->cur],
                                         c->keep_alpha ? L->alpha : nullptr, c->d_stats, a,
@@ -172,6 +262,9 @@ int uclbs_destroy(void* ctx) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto& L : c->levels) free_level(L);
   if (c->d_stats) (void)hipFree(c->d_stats);
+  if (c->d_feq_in) (void)hipFree(c->d_feq_in);
+  if (c->d_feq_out) (void)hipFree(c->d_feq_out);
+  if (c->d_feq_fs) (void)hipFree(c->d_feq_fs);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -196,6 +289,8 @@ int uclbs_define_level(void* ctx, int lvl, const int32_t lo[3], const int32_t hi
     L.lo[d] = lo[d];
     L.hi[d] = hi[d];
     L.box.per[d] = periodic[d] != 0;
+    // faces default to periodic or to a bounce-back wall (-10)
+    L.face_code[d] = L.face_code[d + 3] = L.box.per[d] ? uclbs::kFacePeriodic : -10;
   }
   L.box.nx = n[0];
   L.box.ny = n[1];
@@ -236,6 +331,9 @@ int uclbs_upload_f(void* ctx, int lvl, const double* f, int64_t n_nodes) {
   if (e == hipSuccess)
     e = uclbs::launch_d3q27_from_fp64(L->box, tmp, L->f[L->cur],
                                        c->cfg.precision == UCLBS_PREC_FP32_SHIFTED, c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(L->f[1 - L->cur], L->f[L->cur], level_bytes(c, L),
+                       hipMemcpyDeviceToDevice, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   (void)hipFree(tmp);
   if (e != hipSuccess) return set_err(UCLBS_E_HIP, "upload failed: %s", hipGetErrorString(e));
@@ -322,7 +420,175 @@ int uclbs_init_tgv(void* ctx, int lvl, double u0, double noise_amp, uint64_t see
   HIP_TRY(uclbs::launch_d3q27_init_tgv(L->box, L->f[L->cur],
                                        c->cfg.precision == UCLBS_PREC_FP32_SHIFTED, u0,
                                        noise_amp, seed, c->stream));
+  if ((rc = mirror_buffers(c, L))) return rc;
   HIP_TRY(hipStreamSynchronize(c->stream));
+  return UCLBS_OK;
+}
+
+int uclbs_init_uniform(void* ctx, int lvl, double rho, double u, double v, double w) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  Level* L;
+  int rc = check_level(c, lvl, &L);
+  if (rc) return rc;
+  if (c->cfg.lattice != UCLBS_LATTICE_D3Q27)
+    return set_err(UCLBS_E_UNSUPPORTED, "init_uniform is a D3Q27 initialiser");
+  if (!(rho > 0.0) || !(fabs(u) < 0.5 && fabs(v) < 0.5 && fabs(w) < 0.5))
+    return set_err(UCLBS_E_ARG, "init_uniform: rho must be > 0 and |u| < 0.5");
+  if ((rc = set_device(c))) return rc;
+  HIP_TRY(uclbs::launch_d3q27_init_uniform(L->box, L->f[L->cur],
+                                           c->cfg.precision == UCLBS_PREC_FP32_SHIFTED,
+                                           L->has_links ? L->flags : nullptr, rho, u, v, w,
+                                           c->stream));
+  if ((rc = mirror_buffers(c, L))) return rc;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return UCLBS_OK;
+}
+
+int uclbs_set_faces(void* ctx, int lvl, const int32_t face_code[6]) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  Level* L;
+  int rc = check_level(c, lvl, &L);
+  if (rc) return rc;
+  if (!face_code) return set_err(UCLBS_E_ARG, "null face_code");
+  for (int f = 0; f < 6; ++f) {
+    const int32_t v = face_code[f];
+    const bool per = v == uclbs::kFacePeriodic;
+    const bool ok = per || v == -10 || v == -1000 || (v <= -101 && v >= -199) ||
+                    (v <= -201 && v >= -299);
+    if (!ok) return set_err(UCLBS_E_ARG, "face %d: code %d is not a boundary code", f, v);
+    if (per != (bool)L->box.per[f % 3])
+      return set_err(UCLBS_E_ARG, "face %d: periodic code must match the level's periodic axes", f);
+  }
+  for (int f = 0; f < 6; ++f) L->face_code[f] = face_code[f];
+  return UCLBS_OK;
+}
+
+int uclbs_set_bc_states(void* ctx, int n_in, const double* in_states, int n_out,
+                        const double* out_states, const double* fs_state) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c) return set_err(UCLBS_E_ARG, "null context");
+  if (n_in < 0 || n_in > 99 || n_out < 0 || n_out > 99 || (n_in && !in_states) ||
+      (n_out && !out_states))
+    return set_err(UCLBS_E_ARG, "bad inlet/outlet counts");
+  int rc = set_device(c);
+  if (rc) return rc;
+  auto table = [&](int m, const double* st, double** dptr) -> int {
+    if (*dptr) { (void)hipFree(*dptr); *dptr = nullptr; }
+    if (m == 0) return UCLBS_OK;
+    std::vector<double> h((size_t)m * 27);
+    for (int j = 0; j < m; ++j) {
+      const double* s4 = st + 4 * j;
+      if (!(s4[0] > 0.0) || !(fabs(s4[1]) < 0.5 && fabs(s4[2]) < 0.5 && fabs(s4[3]) < 0.5))
+        return set_err(UCLBS_E_ARG, "boundary state %d: rho must be > 0 and |u| < 0.5", j);
+      feq27_host(s4[0], s4 + 1, h.data() + 27 * j);
+    }
+    HIP_TRY(hipMalloc(dptr, h.size() * sizeof(double)));
+    HIP_TRY(hipMemcpy(*dptr, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+    return UCLBS_OK;
+  };
+  if ((rc = table(n_in, in_states, &c->d_feq_in))) return rc;
+  if ((rc = table(n_out, out_states, &c->d_feq_out))) return rc;
+  if ((rc = table(fs_state ? 1 : 0, fs_state, &c->d_feq_fs))) return rc;
+  c->n_in = n_in;
+  c->n_out = n_out;
+  return UCLBS_OK;
+}
+
+int uclbs_build_links(void* ctx, int lvl, const double origin[3], double dx, int64_t ntri,
+                      const double* tri, const double* nrm, int64_t* n_solid, int64_t* n_tie) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  Level* L;
+  int rc = check_level(c, lvl, &L);
+  if (rc) return rc;
+  if (c->cfg.lattice != UCLBS_LATTICE_D3Q27)
+    return set_err(UCLBS_E_UNSUPPORTED, "build_links: D3Q27 tables only");
+  if (!origin || !(dx > 0.0) || ntri < 0 || ntri > (1 << 26) || (ntri && (!tri || !nrm)))
+    return set_err(UCLBS_E_ARG, "build_links: bad geometry arguments");
+  if ((rc = set_device(c))) return rc;
+  const int64_t n = L->box.n;
+  uclbs::LinkGeom g{};
+  g.box = L->box;
+  for (int d = 0; d < 3; ++d) g.origin[d] = origin[d];
+  g.dx = dx;
+  for (int f = 0; f < 6; ++f) g.face_code[f] = L->face_code[f];
+  g.ntri = (int)ntri;
+  std::vector<double> tbb((size_t)(ntri > 0 ? ntri : 1) * 6);
+  double gbb[6] = {-1e300, 1e300, -1e300, 1e300, -1e300, 1e300};
+  for (int64_t t = 0; t < ntri; ++t)
+    for (int d = 0; d < 3; ++d) {
+      const double a = tri[t * 9 + d], b = tri[t * 9 + 3 + d], e = tri[t * 9 + 6 + d];
+      double mx = a > b ? a : b; mx = mx > e ? mx : e;
+      double mn = a < b ? a : b; mn = mn < e ? mn : e;
+      tbb[t * 6 + 2 * d] = mx;
+      tbb[t * 6 + 2 * d + 1] = mn;
+      if (mx > gbb[2 * d]) gbb[2 * d] = mx;
+      if (mn < gbb[2 * d + 1]) gbb[2 * d + 1] = mn;
+    }
+  for (int i = 0; i < 6; ++i) g.gbb[i] = gbb[i];
+  double *dtri = nullptr, *dnrm = nullptr, *dtbb = nullptr;
+  unsigned long long* dcnt = nullptr;
+  const size_t nt = (size_t)(ntri > 0 ? ntri : 1);
+  hipError_t e = hipMalloc(&dtri, nt * 9 * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&dnrm, nt * 3 * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&dtbb, nt * 6 * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&dcnt, 2 * sizeof(unsigned long long));
+  if (e == hipSuccess && ntri) e = hipMemcpy(dtri, tri, (size_t)ntri * 9 * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess && ntri) e = hipMemcpy(dnrm, nrm, (size_t)ntri * 3 * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dtbb, tbb.data(), nt * 6 * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(dcnt, 0, 2 * sizeof(unsigned long long));
+  if (e == hipSuccess && !L->state) e = hipMalloc(&L->state, (size_t)n * 27 * sizeof(int32_t));
+  unsigned long long hc[2] = {0, 0};
+  if (e == hipSuccess) {
+    g.tri = dtri;
+    g.nrm = dnrm;
+    g.tbb = dtbb;
+    e = uclbs::launch_build_links27(g, L->state, dcnt, c->stream);
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(hc, dcnt, sizeof hc, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(dtri);
+  (void)hipFree(dnrm);
+  (void)hipFree(dtbb);
+  (void)hipFree(dcnt);
+  if (e != hipSuccess) return set_err(UCLBS_E_HIP, "build_links failed: %s", hipGetErrorString(e));
+  if ((rc = finalize_links(c, L))) return rc;
+  if (n_solid) *n_solid = (int64_t)hc[0];
+  if (n_tie) *n_tie = (int64_t)hc[1];
+  return UCLBS_OK;
+}
+
+int uclbs_set_links(void* ctx, int lvl, const int32_t* state, int64_t n_nodes) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  Level* L;
+  int rc = check_level(c, lvl, &L);
+  if (rc) return rc;
+  if (!state || n_nodes != L->box.n) return set_err(UCLBS_E_ARG, "set_links: wrong node count");
+  if ((rc = set_device(c))) return rc;
+  if (!L->state) HIP_TRY(hipMalloc(&L->state, (size_t)n_nodes * 27 * sizeof(int32_t)));
+  HIP_TRY(hipMemcpy(L->state, state, (size_t)n_nodes * 27 * sizeof(int32_t), hipMemcpyHostToDevice));
+  return finalize_links(c, L);
+}
+
+int uclbs_download_links(void* ctx, int lvl, int32_t* state, int64_t n_nodes) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  Level* L;
+  int rc = check_level(c, lvl, &L);
+  if (rc) return rc;
+  if (!L->state) return set_err(UCLBS_E_STATE, "level %d has no link table", lvl);
+  if (!state || n_nodes != L->box.n) return set_err(UCLBS_E_ARG, "download_links: wrong node count");
+  if ((rc = set_device(c))) return rc;
+  HIP_TRY(hipMemcpy(state, L->state, (size_t)n_nodes * 27 * sizeof(int32_t), hipMemcpyDeviceToHost));
+  return UCLBS_OK;
+}
+
+int uclbs_link_counts(void* ctx, int lvl, int64_t* n_boundary, int64_t* n_solid) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  Level* L;
+  int rc = check_level(c, lvl, &L);
+  if (rc) return rc;
+  if (!L->has_links) return set_err(UCLBS_E_STATE, "level %d has no link table", lvl);
+  if (n_boundary) *n_boundary = L->nb;
+  if (n_solid) *n_solid = L->nsolid;
   return UCLBS_OK;
 }
 
@@ -388,6 +654,8 @@ int uclbs_get_stats(void* ctx, uclbs_stats* out) {
   out->n_noconv = acc[uclbs::ST_NOCONV];
   out->n_fallback = acc[uclbs::ST_FALLBACK];
   out->n_logpath = acc[uclbs::ST_LOGPATH];
+  out->n_ibb_links = acc[uclbs::ST_IBB];
+  out->n_repair = acc[uclbs::ST_REPAIR];
   return UCLBS_OK;
 }
 
