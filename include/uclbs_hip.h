@@ -160,7 +160,15 @@ int uclbs_step_level_async(void *ctx, int lvl, int nsteps);
 /* Same as uclbs_step_level, and returns the elapsed time of the launches as
  * measured by HIP events recorded on the library's stream. */
 int uclbs_time_steps(void *ctx, int lvl, int nsteps, double *elapsed_ms);
+/* As uclbs_time_steps; also records HIP events around each step's streaming
+ * kernel (the dominant launch) and returns their summed duration. */
+int uclbs_time_kernels(void *ctx, int lvl, int nsteps, double *step_ms, double *main_ms);
 int uclbs_sync(void *ctx);
+
+/* Measurement helper with no counterpart in the reference: bandwidth of a
+ * plain copy of nbytes on the current device (read + write bytes / time), the
+ * achievable ceiling reported next to the vendor HBM peak. */
+int uclbs_copy_bandwidth(int64_t nbytes, int reps, double *gbps);
 
 int uclbs_get_stats(void *ctx, uclbs_stats *out);
 int uclbs_reset_stats(void *ctx);

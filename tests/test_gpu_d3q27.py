@@ -125,3 +125,37 @@ def test_mass_momentum_conservation_256(pkg):
     assert abs((r1 * ux1).sum() - (r0 * ux0).sum()) / np.abs(r0 * ux0).sum() < 1e-6
     st = s.stats()
     assert st["node_updates"] == 20 * L ** 3 and st["n_noconv"] == 0 and st["n_nonpos"] == 0
+
+
+def _strong_noneq(orc, shape, amp, seed):
+    """Periodic box state far from equilibrium: feq(1, u) times (1 + amp * U(-1, 1))."""
+    rng = np.random.default_rng(seed)
+    n = shape[0] * shape[1] * shape[2]
+    f = orc.init_tgv27(shape, 0.05, 0.0, 1)
+    return f * (1.0 + amp * rng.uniform(-1.0, 1.0, f.shape))
+
+
+@pytest.mark.parametrize("prec,tol_a", [("fp64", 1e-11), ("fp32s", 1e-6)])
+def test_far_from_equilibrium_alpha_log_path(pkg, orc, prec, tol_a):
+    """Large |x| drives the series tiers out of range: the logarithmic Newton
+    (FP32 storage: the deferred fixup kernel) must still match the oracle."""
+    shape = (16, 12, 10)
+    nu = 1e-3
+    f0 = _strong_noneq(orc, shape, 0.4, 5)
+    f_o, a_o, st_o = orc.step_periodic27(shape, f0, 1 / (6 * nu + 1), nsteps=1, tol=1e-12)
+    s = pkg.Solver(precision=prec, nu=nu, alpha_tol=1e-12)
+    s.define_level(0, shape)
+    s.keep_alpha(True)
+    s.upload_f(0, f0)
+    s.step(0, 1)
+    a_g = s.alpha(0)
+    st = s.stats()
+    assert st["n_logpath"] > 100
+    assert st["n_bound"] == st_o["n_bound"] and st["n_noconv"] == 0
+    assert np.abs(a_g - a_o).max() < tol_a
+    if prec == "fp64":
+        # same definition as the oracle: updates of the entropic Newton from 2
+        assert abs(st["newton_iters"] - st_o["iters"]) <= 0.01 * st_o["iters"] + 5
+    else:
+        # FP32 storage also counts the series-seed updates of queued far nodes
+        assert st_o["iters"] - 5 <= st["newton_iters"] <= st_o["iters"] + 9 * st["n_logpath"]

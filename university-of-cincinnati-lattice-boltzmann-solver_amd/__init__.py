@@ -25,7 +25,8 @@ EXPORTS = (
     "uclbs_step_level", "uclbs_step_level_async", "uclbs_time_steps", "uclbs_sync",
     "uclbs_get_stats", "uclbs_reset_stats", "uclbs_set_faces", "uclbs_set_bc_states",
     "uclbs_build_links", "uclbs_set_links", "uclbs_download_links", "uclbs_link_counts",
-    "uclbs_init_uniform", "uclbs_geom_icosphere",
+    "uclbs_init_uniform", "uclbs_geom_icosphere", "uclbs_time_kernels",
+    "uclbs_copy_bandwidth",
 )
 FACE_PERIODIC = 2147483647
 
@@ -99,6 +100,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "uclbs_link_counts": (I, [P, I, ctypes.POINTER(I64), ctypes.POINTER(I64)]),
         "uclbs_init_uniform": (I, [P, I, D, D, D, D]),
         "uclbs_geom_icosphere": (I, [I, D, dp, U64, dp, dp, I64, ctypes.POINTER(I64)]),
+        "uclbs_time_kernels": (I, [P, I, I, dp, dp]),
+        "uclbs_copy_bandwidth": (I, [I64, I, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -131,6 +134,16 @@ def icosphere(level: int, radius: float, center, seed: int):
     if rc:
         raise UclbsError(f"icosphere failed ({rc})")
     return tri, nrm
+
+
+def copy_bandwidth(nbytes: int, reps: int = 10) -> float:
+    """GB/s of a plain device copy of nbytes (read + write bytes / time)."""
+    lib = load_library()
+    g = ctypes.c_double()
+    rc = lib.uclbs_copy_bandwidth(int(nbytes), reps, ctypes.byref(g))
+    if rc:
+        raise UclbsError(f"copy_bandwidth failed ({rc})")
+    return g.value
 
 
 class Solver:
@@ -259,6 +272,13 @@ class Solver:
         ms = ctypes.c_double()
         self._check(self.lib.uclbs_time_steps(self.ctx, lvl, nsteps, ctypes.byref(ms)))
         return ms.value
+
+    def time_kernels(self, lvl: int, nsteps: int):
+        """(total ms of nsteps steps, summed ms of the streaming kernel launches)"""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.uclbs_time_kernels(self.ctx, lvl, nsteps, ctypes.byref(a),
+                                                ctypes.byref(b)))
+        return a.value, b.value
 
     def sync(self):
         self._check(self.lib.uclbs_sync(self.ctx))

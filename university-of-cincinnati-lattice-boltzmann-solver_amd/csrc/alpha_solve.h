@@ -27,7 +27,7 @@
 namespace uclbs {
 
 enum : int { ALPHA_NEWTON = 0, ALPHA_TRIVIAL = 1, ALPHA_BOUND = 2, ALPHA_NONPOS = 3,
-             ALPHA_NOCONV = 4, ALPHA_DEFER = 5 /* left to the general-path fixup kernel */ };
+             ALPHA_NOCONV = 4, ALPHA_DEFER = 5 /* left to the far-from-equilibrium kernel */ };
 
 struct AlphaOut {
   double alpha;
@@ -48,14 +48,52 @@ constexpr double kTrivialX = 1e-12;
 struct SeriesThr {
   double m5, m7, m10, m16;  // general path orders
   float m_fast;             // order kFastOrder of the fused FP32-storage pass
+  float m_mid;              // order kMidOrder, the second in-register tier
 };
 constexpr int kFastOrder = 6;
+constexpr int kMidOrder = 10;
 inline double series_limit(int N, double eps) {
   return __builtin_pow(eps * N * (N - 1.0), 1.0 / (N - 1.0)) / 2.2;
 }
 inline SeriesThr make_series_thr(double eps) {
   return SeriesThr{series_limit(5, eps), series_limit(7, eps), series_limit(10, eps),
-                   series_limit(16, eps), (float)series_limit(kFastOrder, eps)};
+                   series_limit(16, eps), (float)series_limit(kFastOrder, eps),
+                   (float)series_limit(kMidOrder, eps)};
+}
+
+// ln(1 + x) for x > -1 in FP64, about 35 VALU operations: u = 1 + x = 2^e m
+// with m in [sqrt(1/2), sqrt(2)), ln m = 2 atanh(s), s = (m-1)/(m+1), |s| <=
+// 0.1716, summed to s^23 (remainder < 1e-18); ln 2 split in a high part exact
+// in e*ln2_hi and a low part; the rounding error of 1 + x is removed to first
+// order, ln(1+x) = ln(u) - ((u - 1) - x)/u.
+__host__ __device__ __forceinline__ double log1p_fast(double x) {
+  const double u = 1.0 + x;
+  const double c = (u - 1.0) - x;
+  int e;
+  double m = __builtin_frexp(u, &e);  // u = m 2^e, m in [0.5, 1)
+  if (m < 0.70710678118654752440) {
+    m *= 2.0;
+    e -= 1;
+  }
+  const double s = (m - 1.0) / (m + 1.0);
+  const double s2 = s * s;
+  // 2 * sum_{k=0..11} s2^k / (2k+1)
+  double p = 2.0 / 23.0;
+  p = p * s2 + 2.0 / 21.0;
+  p = p * s2 + 2.0 / 19.0;
+  p = p * s2 + 2.0 / 17.0;
+  p = p * s2 + 2.0 / 15.0;
+  p = p * s2 + 2.0 / 13.0;
+  p = p * s2 + 2.0 / 11.0;
+  p = p * s2 + 2.0 / 9.0;
+  p = p * s2 + 2.0 / 7.0;
+  p = p * s2 + 2.0 / 5.0;
+  p = p * s2 + 2.0 / 3.0;
+  p = p * s2 + 2.0;
+  const double ln2_hi = 6.93147180369123816490e-01;  // 0x3FE62E42FEE00000, 32 significant bits
+  const double ln2_lo = 1.90821492927058770002e-10;
+  const double de = (double)e;
+  return de * ln2_hi + (de * ln2_lo + s * p) - c / u;
 }
 
 // phi(y) = (1 + y) ln(1 + y) - y, series for small |y|, direct otherwise
@@ -81,6 +119,26 @@ __device__ __forceinline__ void series_g(const double* S, double a, double& g, d
     const double c1 = 1.0 / (n * (n - 1.0)), c2 = 1.0 / (n - 1.0);
     const double tg = S[n] * (an * c1 - a * c2);
     const double tgp = S[n] * ((an1 - 1.0) * c2);
+    if (n & 1) { sg -= tg; sgp -= tgp; } else { sg += tg; sgp += tgp; }
+    an1 = an;
+  }
+  g = sg;
+  gp = sgp;
+}
+
+// Same series with the orders split over two arrays: S[2..N1] and Sh[0..N2-N1-1]
+// holding orders N1+1..N2 (keeps the second tier's registers small).
+template <int N1, int N2>
+__device__ __forceinline__ void series_g2(const double* S, const double* Sh, double a, double& g,
+                                          double& gp) {
+  double an1 = a, sg = 0.0, sgp = 0.0;
+#pragma unroll
+  for (int n = 2; n <= N2; ++n) {
+    const double an = an1 * a;
+    const double c1 = 1.0 / (n * (n - 1.0)), c2 = 1.0 / (n - 1.0);
+    const double Sn = n <= N1 ? S[n] : Sh[n - N1 - 1];
+    const double tg = Sn * (an * c1 - a * c2);
+    const double tgp = Sn * ((an1 - 1.0) * c2);
     if (n & 1) { sg -= tg; sgp -= tgp; } else { sg += tg; sgp += tgp; }
     an1 = an;
   }
@@ -138,38 +196,47 @@ __device__ AlphaOut solve_alpha(DX&& dx, FX&& fx, double m, double xmin, double 
   if (m <= thr.m7) {
     if (m <= thr.m5) {
       power_sums<5>(dx, Q, S);
-      return newton_alpha([&](double a, double& g, double& gp) { series_g<5>(S, a, g, gp); },
+      return newton_alpha([&](double a, double& g, double& gp) __attribute__((always_inline)) { series_g<5>(S, a, g, gp); },
                           S2, amax, alpha0, tol, maxit);
     }
     power_sums<7>(dx, Q, S);
-    return newton_alpha([&](double a, double& g, double& gp) { series_g<7>(S, a, g, gp); }, S2,
+    return newton_alpha([&](double a, double& g, double& gp) __attribute__((always_inline)) { series_g<7>(S, a, g, gp); }, S2,
                         amax, alpha0, tol, maxit);
   }
   if (m <= thr.m10) {
     power_sums<10>(dx, Q, S);
-    return newton_alpha([&](double a, double& g, double& gp) { series_g<10>(S, a, g, gp); },
+    return newton_alpha([&](double a, double& g, double& gp) __attribute__((always_inline)) { series_g<10>(S, a, g, gp); },
                         S2, amax, alpha0, tol, maxit);
   }
   if (m <= thr.m16) {
     power_sums<16>(dx, Q, S);
-    return newton_alpha([&](double a, double& g, double& gp) { series_g<16>(S, a, g, gp); },
+    return newton_alpha([&](double a, double& g, double& gp) __attribute__((always_inline)) { series_g<16>(S, a, g, gp); },
                         S2, amax, alpha0, tol, maxit);
   }
-  // large non-equilibrium: direct logarithms
+  // large non-equilibrium: direct logarithms, x_i and ln(1 + x_i) formed once
+  // (with log1p_fast each evaluation costs 27 logarithms of ~35 operations)
+  double xs[Q], l1[Q];
+#pragma unroll
+  for (int i = 0; i < Q; ++i) {
+    double d;
+    dx(i, d, xs[i]);
+    l1[i] = log1p_fast(xs[i]);
+  }
   AlphaOut r = newton_alpha(
-      [&](double a, double& g, double& gp) {
+      [&](double a, double& g, double& gp) __attribute__((always_inline)) {
         double sg = 0.0, sgp = 0.0;
         bool inf_gp = false;
+#pragma unroll
         for (int i = 0; i < Q; ++i) {
-          double d, x;
-          dx(i, d, x);
+          const double x = xs[i];
           const double f = fx(i);
-          const double l1 = log1p(x);
           const double y = a * x;
-          if (1.0 + y <= 0.0) { sg += f * (-y - y * l1); inf_gp = true; continue; }
-          const double la = log1p(y);
-          sg += f * (phi1p(y) - y * l1);
-          sgp += d * (la - l1);
+          if (1.0 + y <= 0.0) { sg += f * (-y - y * l1[i]); inf_gp = true; continue; }
+          const double la = log1p_fast(y);
+          // (1+y) ln(1+y) - y loses only absolute accuracy O(eps*|y|), far
+          // below tol * S2
+          sg += f * ((1.0 + y) * la - y - y * l1[i]);
+          sgp += f * x * (la - l1[i]);
         }
         g = sg;
         gp = inf_gp ? __builtin_huge_val() : sgp;

@@ -60,15 +60,10 @@ __device__ __forceinline__ void axis_eps_fast(float u, float& em, float& e0, flo
 template <typename R>
 __device__ __forceinline__ R mul1(R a, R b) { return a + b + a * b; }
 
-// ---------------------------------------------------------------- FP32 shifted
-// f[i] holds f~_i on entry and fout~_i on return.  With kInlineGeneral false a
-// node whose max|x| lies beyond the fused series order returns ALPHA_DEFER
-// with f[] untouched (the caller stores the pre-collision state and queues the
-// node); this keeps the logarithm path's registers out of the streaming kernel.
-template <bool kInlineGeneral>
-__device__ __forceinline__ AlphaOut collide27_shifted(float (&f)[27], double beta,
-                                                      const AlphaCfg& ac, double alpha0,
-                                                      double& rho_out, double (&u_out)[3]) {
+// moments (FP64 sums), shifted equilibrium and D = feq~ - f~ with the mass
+// projection; shared by the streaming and far-from-equilibrium collides
+__device__ __forceinline__ void prep27_shifted(const float (&f)[27], float (&D)[27],
+                                               double& rho_out, double (&u_out)[3]) {
   double r = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
 #pragma unroll
   for (int i = 0; i < 27; ++i) {
@@ -102,7 +97,6 @@ __device__ __forceinline__ AlphaOut collide27_shifted(float (&f)[27], double bet
   // rounding of the 1-D factors leaves a small, partly systematic residual
   // (it drifted the total mass by ~1e-10 per node and step), so the residual
   // is summed in FP64 and removed through the rest population.
-  float D[27];
   double sd = 0.0;
 #pragma unroll
   for (int i = 0; i < 27; ++i) {
@@ -111,9 +105,52 @@ __device__ __forceinline__ AlphaOut collide27_shifted(float (&f)[27], double bet
     sd += (double)D[i];
   }
   D[0] -= (float)sd;
+}
+
+// ---------------------------------------------------------------- FP32 shifted
+// f[i] holds f~_i on entry and fout~_i on return.  Nodes whose max|x| lies
+// beyond the in-register series tiers need the logarithmic Newton: with
+// kInlineGeneral false (streaming kernel) the node returns ALPHA_DEFER with
+// f[] untouched and, in `alpha`, the root of the order-kMidOrder truncated
+// series as a starting point; the caller stores the pre-collision state and
+// queues the node, so one far lane does not stall its wavefront.  With
+// kInlineGeneral true the general path of alpha_solve.h runs in place.
+// kGeneralOnly skips the fused tiers (for nodes already known to be far from
+// equilibrium) and keeps only the general path's registers live.
+template <bool kInlineGeneral, bool kGeneralOnly = false>
+__device__ __forceinline__ AlphaOut collide27_shifted(float (&f)[27], double beta,
+                                                      const AlphaCfg& ac, double alpha0,
+                                                      double& rho_out, double (&u_out)[3]) {
+  float D[27];
+  prep27_shifted(f, D, rho_out, u_out);
 
   AlphaOut ao{2.0, 0, ALPHA_TRIVIAL, false};
-  if (ac.mode != 1) {
+  if constexpr (kGeneralOnly) {
+    if (ac.mode != 1) {
+      double m = 0.0, xmin = 0.0, S2 = 0.0;
+      bool nonpos = false;
+#pragma unroll
+      for (int i = 0; i < 27; ++i) {
+        const double fi = W27[i] + (double)f[i];
+        nonpos |= !(fi > 0.0);
+        const double x = (double)D[i] / fi;
+        m = fmax(m, fabs(x));
+        xmin = fmin(xmin, x);
+        S2 += (double)D[i] * x;
+      }
+      if (nonpos) {
+        ao = AlphaOut{2.0, 0, ALPHA_NONPOS, false};
+      } else {
+        auto dx = [&](int i, double& d, double& x) __attribute__((always_inline)) {
+          d = (double)D[i];
+          x = d / (W27[i] + (double)f[i]);
+        };
+        auto fx = [&](int i) __attribute__((always_inline)) { return W27[i] + (double)f[i]; };
+        ao = solve_alpha<27>(dx, fx, m, xmin, S2, ac.mode == 2 ? alpha0 : 2.0, ac.tol, ac.maxit,
+                             ac.thr);
+      }
+    }
+  } else if (ac.mode != 1) {
     // One fused pass: x_i = D_i / f_i (FP32, hardware reciprocal), max|x|,
     // min x, and the power sums S_n = sum f x^n = sum D x^(n-1), n = 2..6,
     // with FP32 products accumulated in FP64.
@@ -141,23 +178,106 @@ __device__ __forceinline__ AlphaOut collide27_shifted(float (&f)[27], double bet
       ao = AlphaOut{2.0, 0, ALPHA_TRIVIAL, false};
     } else if (amax < 2.0) {
       ao = AlphaOut{amax, 0, ALPHA_BOUND, false};
-    } else if (m <= ac.thr.m_fast) {
-      // within the fused order: Newton on the scalar series
+    } else if (!__any(m > ac.thr.m_fast)) {
+      // the whole wavefront is within the fused order: Newton on the scalar series
       const double a0 = ac.mode == 2 ? alpha0 : 2.0;
-      ao = newton_alpha([&](double a, double& g, double& gp) { series_g<kFastOrder>(S, a, g, gp); },
+      ao = newton_alpha([&](double a, double& g, double& gp) __attribute__((always_inline)) { series_g<kFastOrder>(S, a, g, gp); },
                         S[2], amax, a0, ac.tol, ac.maxit);
-    } else if constexpr (!kInlineGeneral) {
-      return AlphaOut{2.0, 0, ALPHA_DEFER, false};
     } else {
-      // some lane of the wavefront is further from equilibrium: general path
-      auto dx = [&](int i, double& d, double& x) {
-        d = (double)D[i];
-        x = d / (W27[i] + (double)f[i]);
-      };
-      auto fx = [&](int i) { return W27[i] + (double)f[i]; };
-      ao = solve_alpha<27>(dx, fx, (double)m, (double)xmin, S[2], ac.mode == 2 ? alpha0 : 2.0,
-                           ac.tol, ac.maxit, ac.thr);
+      // some lane needs more terms: the wavefront extends its sums to order
+      // kMidOrder from the populations still in registers
+      double Sh[kMidOrder - kFastOrder];
+#pragma unroll
+      for (int q = 0; q < kMidOrder - kFastOrder; ++q) Sh[q] = 0.0;
+#pragma unroll
+      for (int i = 0; i < 27; ++i) {
+        const float x = D[i] * frcp((float)W27[i] + f[i]);
+        float t = D[i] * x;
+#pragma unroll
+        for (int q = 3; q <= kFastOrder; ++q) t *= x;
+#pragma unroll
+        for (int q = 0; q < kMidOrder - kFastOrder; ++q) { t *= x; Sh[q] += (double)t; }
+      }
+      const double a0 = ac.mode == 2 ? alpha0 : 2.0;
+      if (m <= ac.thr.m_mid) {
+        ao = newton_alpha(
+            [&](double a, double& g, double& gp) __attribute__((always_inline)) { series_g2<kFastOrder, kMidOrder>(S, Sh, a, g, gp); },
+            S[2], amax, a0, ac.tol, ac.maxit);
+      } else if constexpr (!kInlineGeneral) {
+        // seed for the deferred logarithmic Newton: root of the truncated series
+        AlphaOut sd = newton_alpha(
+            [&](double a, double& g, double& gp) __attribute__((always_inline)) {
+              series_g2<kFastOrder, kMidOrder>(S, Sh, a, g, gp);
+            },
+            S[2], amax, a0, 1e-6, 8);
+        return AlphaOut{sd.alpha, sd.iters, ALPHA_DEFER, false};
+        ao.logpath = true;
+      } else {
+        auto dx = [&](int i, double& d, double& x) __attribute__((always_inline)) {
+          d = (double)D[i];
+          x = d / (W27[i] + (double)f[i]);
+        };
+        auto fx = [&](int i) __attribute__((always_inline)) { return W27[i] + (double)f[i]; };
+        ao = solve_alpha<27>(dx, fx, (double)m, (double)xmin, S[2], a0, ac.tol, ac.maxit, ac.thr);
+      }
     }
+  }
+  const float ab = (float)(ao.alpha * beta);
+#pragma unroll
+  for (int i = 0; i < 27; ++i) f[i] = fmaf(ab, D[i], f[i]);
+  return ao;
+}
+
+// Far-from-equilibrium node (queued by the streaming kernel): logarithmic
+// Newton from the series seed, then relaxation.  Same moments, equilibrium
+// and D as the streaming kernel formed for it.
+__device__ __forceinline__ AlphaOut collide27_far(float (&f)[27], double beta, const AlphaCfg& ac,
+                                                  double seed, double& rho_out,
+                                                  double (&u_out)[3]) {
+  float D[27];
+  prep27_shifted(f, D, rho_out, u_out);
+  float xmin = 0.0f;
+  double S2 = 0.0;
+  bool nonpos = false;
+#pragma unroll
+  for (int i = 0; i < 27; ++i) {
+    const float fi = (float)W27[i] + f[i];
+    nonpos |= !(fi > 0.0f);
+    const float x = D[i] * frcp(fi);
+    xmin = fminf(xmin, x);
+    S2 += (double)(D[i] * x);
+  }
+  const double amax = xmin < 0.0f ? -1.0 / (double)xmin : 1e300;
+  AlphaOut ao;
+  if (nonpos) {
+    ao = AlphaOut{2.0, 0, ALPHA_NONPOS, false};
+  } else if (amax < 2.0) {
+    ao = AlphaOut{amax, 0, ALPHA_BOUND, false};
+  } else {
+    double K = 0.0;
+#pragma clang loop unroll(full)
+    for (int i = 0; i < 27; ++i) {
+      const double fi = W27[i] + (double)f[i];
+      K += (double)D[i] * log1p_fast((double)D[i] / fi);
+    }
+    ao = newton_alpha(
+        [&](double a, double& g, double& gp) __attribute__((always_inline)) {
+          double sg = 0.0, sgp = 0.0;
+          bool inf_gp = false;
+#pragma clang loop unroll(full)
+          for (int i = 0; i < 27; ++i) {
+            const double fi = W27[i] + (double)f[i];
+            const double y = a * ((double)D[i] / fi);
+            if (1.0 + y <= 0.0) { sg -= fi * y; inf_gp = true; continue; }
+            const double la = log1p_fast(y);
+            sg += fi * ((1.0 + y) * la - y);
+            sgp += (double)D[i] * la;
+          }
+          g = sg - a * K;
+          gp = inf_gp ? __builtin_huge_val() : sgp - K;
+        },
+        S2, amax, seed, ac.tol, ac.maxit);
+    ao.logpath = true;
   }
   const float ab = (float)(ao.alpha * beta);
 #pragma unroll
@@ -211,8 +331,8 @@ __device__ __forceinline__ AlphaOut collide27_fp64(double (&f)[27], double beta,
     if (nonpos) {
       ao = AlphaOut{2.0, 0, ALPHA_NONPOS, false};
     } else {
-      auto dx = [&](int i, double& d, double& x) { d = D[i]; x = D[i] / f[i]; };
-      auto fx = [&](int i) { return f[i]; };
+      auto dx = [&](int i, double& d, double& x) __attribute__((always_inline)) { d = D[i]; x = D[i] / f[i]; };
+      auto fx = [&](int i) __attribute__((always_inline)) { return f[i]; };
       ao = solve_alpha<27>(dx, fx, m, xmin, S2, ac.mode == 2 ? alpha0 : 2.0, ac.tol, ac.maxit,
                            ac.thr);
     }

@@ -19,6 +19,8 @@
 #include "kernels.h"
 #include "philox.h"
 
+#include <algorithm>
+
 namespace uclbs {
 
 namespace {
@@ -28,6 +30,8 @@ __device__ __forceinline__ void wave_stats(unsigned long long* __restrict__ stat
                                            bool logpath) {
   const unsigned long long act = __ballot(active);
   if (act == 0ull) return;
+  // status -1: the node is finished elsewhere; only its iterations count here
+  const unsigned long long done = __ballot(active && status >= 0);
   unsigned long long it = 0;
 #pragma unroll
   for (int b = 0; b < 6; ++b)
@@ -41,7 +45,7 @@ __device__ __forceinline__ void wave_stats(unsigned long long* __restrict__ stat
   if (__lane_id() == 0) {
     unsigned long long* s = stats + (size_t)((blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64) &
                                              (kStatSlots - 1)) * kStatFields;
-    atomicAdd(s + ST_NODES, (unsigned long long)__popcll(act));
+    if (done) atomicAdd(s + ST_NODES, (unsigned long long)__popcll(done));
     if (it) atomicAdd(s + ST_ITERS, it);
     if (nt) atomicAdd(s + ST_TRIVIAL, nt);
     if (nb) atomicAdd(s + ST_BOUND, nb);
@@ -55,40 +59,89 @@ __device__ __forceinline__ void wave_stats(unsigned long long* __restrict__ stat
 // kFlags: the level has link tables; only interior fluid nodes (flag 0) are
 // updated here, the boundary kernel updates the rest.
 template <bool kShifted, bool kKeepAlpha, bool kFlags>
-__global__ __launch_bounds__(256) void k27_step_periodic(const void* __restrict__ fin_,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k27_step_periodic(const void* __restrict__ fin_,
                                                          void* __restrict__ fout_,
                                                          double* __restrict__ alpha_out,
                                                          unsigned long long* __restrict__ stats,
                                                          Box b, StepArgs a,
-                                                         const uint8_t* __restrict__ flags) {
+                                                         const uint8_t* __restrict__ flags,
+                                                         BcTables bcf) {
   using T = typename std::conditional<kShifted, float, double>::type;
   const T* __restrict__ fin = static_cast<const T*>(fin_);
   T* __restrict__ fout = static_cast<T*>(fout_);
-  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t node = gid < b.n ? gid : 0;
-  bool active = gid < b.n;
-  if constexpr (kFlags) active = active && flags[node] == 0;
-  const int x = (int)(node % b.nx);
-  const int64_t yz = node / b.nx;
-  const int y = (int)(yz % b.ny);
-  const int z = (int)(yz / b.ny);
-  const int64_t sx = 1, sy = b.nx, sz = (int64_t)b.nx * b.ny;
+  // 32-bit node offsets against a per-population scalar base (boxes are
+  // limited to 2^30 nodes), so each of the 27 loads needs one address VGPR
+  const unsigned gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned nn = (unsigned)b.n;
+  const unsigned node = gid < nn ? gid : 0u;
+  bool active = gid < nn;
+  uint8_t fl = 0;
+  if constexpr (kFlags) {
+    fl = flags[node];
+    active = active && (fl == kFlagInterior || fl == kFlagFace);
+  }
+  const unsigned nx = (unsigned)b.nx, ny = (unsigned)b.ny, nz = (unsigned)b.nz;
+  const unsigned x = node % nx;
+  const unsigned yz = node / nx;
+  const unsigned y = yz % ny;
+  const unsigned z = yz / ny;
+  const unsigned sy = nx, sz = nx * ny;
   // source coordinate x - c for c = -1, 0, +1 (periodic wrap)
-  int64_t ox[3], oy[3], oz[3];
-  ox[0] = (x + 1 == b.nx ? 0 : x + 1) * sx;
-  ox[1] = (int64_t)x * sx;
-  ox[2] = (x == 0 ? b.nx - 1 : x - 1) * sx;
-  oy[0] = (y + 1 == b.ny ? 0 : y + 1) * sy;
-  oy[1] = (int64_t)y * sy;
-  oy[2] = (y == 0 ? b.ny - 1 : y - 1) * sy;
-  oz[0] = (z + 1 == b.nz ? 0 : z + 1) * sz;
-  oz[1] = (int64_t)z * sz;
-  oz[2] = (z == 0 ? b.nz - 1 : z - 1) * sz;
+  unsigned ox[3], oy[3], oz[3];
+  ox[0] = x + 1 == nx ? 0u : x + 1;
+  ox[1] = x;
+  ox[2] = x == 0 ? nx - 1 : x - 1;
+  oy[0] = (y + 1 == ny ? 0u : y + 1) * sy;
+  oy[1] = y * sy;
+  oy[2] = (y == 0 ? ny - 1 : y - 1) * sy;
+  oz[0] = (z + 1 == nz ? 0u : z + 1) * sz;
+  oz[1] = z * sz;
+  oz[2] = (z == 0 ? nz - 1 : z - 1) * sz;
 
   T f[27];
 #pragma unroll
-  for (int i = 0; i < 27; ++i)
-    f[i] = fin[(int64_t)i * b.n + ox[CX27[i] + 1] + oy[CY27[i] + 1] + oz[CZ27[i] + 1]];
+  for (int i = 0; i < 27; ++i) {
+    const T* __restrict__ base = fin + (size_t)i * nn;
+    f[i] = base[ox[CX27[i] + 1] + oy[CY27[i] + 1] + oz[CZ27[i] + 1]];
+  }
+  // Nodes whose only boundary links leave the box through a face: the
+  // population arriving from outside takes the face's code, the same code the
+  // link table holds (first non-periodic face in the order x-, y-, z-, x+, y+,
+  // z+, so faces are applied in reverse order and the first one wins).
+  // -10: halfway bounce-back of the node's own opposite population; inlet /
+  // outlet / freestream: their equilibrium.  Wave-uniform guards keep the
+  // interior wavefronts out of this code.
+  if constexpr (kFlags) {
+    if (__any(fl == kFlagFace)) {
+      const bool isf = fl == kFlagFace;
+      const bool on[6] = {isf && x == 0u && !b.per[0], isf && y == 0u && !b.per[1],
+                          isf && z == 0u && !b.per[2], isf && x == nx - 1 && !b.per[0],
+                          isf && y == ny - 1 && !b.per[1], isf && z == nz - 1 && !b.per[2]};
+#pragma unroll
+      for (int q = 5; q >= 0; --q) {
+        if (!__any(on[q])) continue;
+        const int32_t code = a.face_code[q];
+        const double* tab = code == -10 ? nullptr
+                            : (code <= -101 && code >= -199) ? bcf.feq_in + (-code - 101) * 27
+                            : (code <= -201 && code >= -299) ? bcf.feq_out + (-code - 201) * 27
+                                                             : bcf.feq_fs;
+#pragma unroll
+        for (int i = 1; i < 27; ++i) {
+          // the source x - c_i lies beyond face q
+          const int cq = q % 3 == 0 ? CX27[i] : (q % 3 == 1 ? CY27[i] : CZ27[i]);
+          if (!(q < 3 ? cq > 0 : cq < 0)) continue;
+          T v;
+          if (tab) {
+            if constexpr (kShifted) v = (T)(tab[i] - W27[i]);
+            else v = (T)tab[i];
+          } else {
+            v = on[q] ? (fin + (size_t)OPP27[i] * nn)[node] : f[i];
+          }
+          if (on[q]) f[i] = v;
+        }
+      }
+    }
+  }
 
   AlphaCfg ac{a.tol, a.maxit, a.alpha_mode, a.thr};
   double rho, u[3];
@@ -97,10 +150,11 @@ __global__ __launch_bounds__(256) void k27_step_periodic(const void* __restrict_
   else ao = collide27_fp64(f, a.beta, ac, 2.0, rho, u);
 
   if (active) {
-    // a deferred node stores its pre-collision state; the fixup kernel collides it in place
+    // a deferred node stores its pre-collision state; the far-from-equilibrium
+    // kernel collides it in place
 #pragma unroll
-    for (int i = 0; i < 27; ++i) fout[(int64_t)i * b.n + node] = f[i];
-    if constexpr (kKeepAlpha) alpha_out[node] = ao.alpha;
+    for (int i = 0; i < 27; ++i) (fout + (size_t)i * nn)[node] = f[i];
+    if (kKeepAlpha || ao.status == ALPHA_DEFER) alpha_out[node] = ao.alpha;
   }
   if constexpr (kShifted) {
     const bool defer = active && ao.status == ALPHA_DEFER;
@@ -108,43 +162,52 @@ __global__ __launch_bounds__(256) void k27_step_periodic(const void* __restrict_
     if (dm) {
       unsigned base = 0;
       const int leader = __ffsll((long long)dm) - 1;
-      if ((int)__lane_id() == leader) base = atomicAdd(a.defer_count + a.parity, (unsigned)__popcll(dm));
+      if ((int)__lane_id() == leader) base = atomicAdd(a.defer_count, (unsigned)__popcll(dm));
       base = __shfl(base, leader);
-      if (defer) a.defer_list[base + __popcll(dm & ((1ull << __lane_id()) - 1ull))] = (unsigned)node;
+      if (defer) a.defer_list[base + __popcll(dm & ((1ull << __lane_id()) - 1ull))] = node;
     }
   }
-  wave_stats(stats, active && ao.status != ALPHA_DEFER, ao.iters, ao.status, false, ao.logpath);
+  // deferred nodes report their seed iterations here and the rest in the far kernel
+  wave_stats(stats, active, ao.iters, ao.status == ALPHA_DEFER ? -1 : ao.status, false,
+             ao.logpath);
 }
 
-// Collides, in place, the nodes the streaming kernel queued (max|x| beyond the
-// fused series order): full FP64 x_i, higher series orders or logarithms.
-// Grid-stride over the device-side count; also clears the other parity's
-// counter for the next step.
-__global__ __launch_bounds__(256) void k27_fixup(float* __restrict__ f_, double* __restrict__ alpha_out,
-                                                 unsigned long long* __restrict__ stats, Box b,
-                                                 StepArgs a) {
-  const unsigned count = a.defer_count[a.parity];
-  if (blockIdx.x == 0 && threadIdx.x == 0) a.defer_count[1 - a.parity] = 0u;
-  AlphaCfg ac{a.tol, a.maxit, a.alpha_mode, a.thr};
+// Far-from-equilibrium nodes queued by the streaming kernel (max|x| beyond
+// the in-register series): one thread per queued node reads the
+// pre-collision state stored in place, re-forms D, and runs the logarithmic
+// Newton from the series seed left in alpha[]; then relaxes and stores.
+// The queue length stays on the device; the grid covers the box and blocks
+// past the length exit at once.  The last block to finish clears the length.
+__global__ __launch_bounds__(256) void k27_far(float* __restrict__ f_, double* __restrict__ alpha,
+                                               unsigned long long* __restrict__ stats, Box b,
+                                               StepArgs a, int keep_alpha) {
+  const unsigned count = *(volatile unsigned*)a.defer_count;
   const unsigned stride = gridDim.x * blockDim.x;
-  const unsigned rounds = (count + stride - 1) / stride;
-  for (unsigned r = 0; r < rounds; ++r) {
-    const unsigned k = r * stride + blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned first = blockIdx.x * blockDim.x;
+  const unsigned nn = (unsigned)b.n;
+  AlphaCfg ac{a.tol, a.maxit, a.alpha_mode, a.thr};
+  // grid-stride over the queue; whole blocks past its end leave at once
+  for (unsigned base = first; base < count; base += stride) {
+    const unsigned k = base + threadIdx.x;
     const bool active = k < count;
-    const int64_t node = active ? (int64_t)a.defer_list[k] : 0;
+    const unsigned node = a.defer_list[active ? k : base];
     float f[27];
 #pragma unroll
-    for (int i = 0; i < 27; ++i) f[i] = f_[(int64_t)i * b.n + node];
+    for (int i = 0; i < 27; ++i) f[i] = (f_ + (size_t)i * nn)[node];
+    const double seed = alpha[node];
     double rho, u[3];
-    AlphaOut ao = collide27_shifted<true>(f, a.beta, ac, 2.0, rho, u);
+    AlphaOut ao = collide27_far(f, a.beta, ac, seed, rho, u);
     if (active) {
 #pragma unroll
-      for (int i = 0; i < 27; ++i) f_[(int64_t)i * b.n + node] = f[i];
-      if (alpha_out) alpha_out[node] = ao.alpha;
+      for (int i = 0; i < 27; ++i) (f_ + (size_t)i * nn)[node] = f[i];
+      if (keep_alpha) alpha[node] = ao.alpha;
     }
     wave_stats(stats, active, ao.iters, ao.status, false, ao.logpath);
   }
 }
+
+// resets the queue length for the next step (after the far kernel has read it)
+__global__ void k_clear_count(unsigned* c) { *c = 0u; }
 
 template <bool kShifted>
 __global__ void k27_from_fp64(const double* __restrict__ src, void* __restrict__ dst_, Box b) {
@@ -456,36 +519,42 @@ inline unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t - 1) / t
 }  // namespace
 
 hipError_t launch_d3q27_step_periodic(const Box& b, const void* fin, void* fout, double* alpha,
-                                      unsigned long long* stats, const StepArgs& a,
-                                      hipStream_t s) {
+                                      int keep_alpha, unsigned long long* stats,
+                                      const StepArgs& a, hipStream_t s) {
   const unsigned g = blocks_for(b.n, 256);
   const uint8_t* nf = nullptr;
   if (a.shifted) {
-    if (alpha) k27_step_periodic<true, true, false><<<g, 256, 0, s>>>(fin, fout, alpha, stats, b, a, nf);
-    else k27_step_periodic<true, false, false><<<g, 256, 0, s>>>(fin, fout, alpha, stats, b, a, nf);
+    if (keep_alpha) k27_step_periodic<true, true, false><<<g, 256, 0, s>>>(fin, fout, alpha, stats, b, a, nf, BcTables{});
+    else k27_step_periodic<true, false, false><<<g, 256, 0, s>>>(fin, fout, alpha, stats, b, a, nf, BcTables{});
   } else {
-    if (alpha) k27_step_periodic<false, true, false><<<g, 256, 0, s>>>(fin, fout, alpha, stats, b, a, nf);
-    else k27_step_periodic<false, false, false><<<g, 256, 0, s>>>(fin, fout, alpha, stats, b, a, nf);
+    if (keep_alpha) k27_step_periodic<false, true, false><<<g, 256, 0, s>>>(fin, fout, alpha, stats, b, a, nf, BcTables{});
+    else k27_step_periodic<false, false, false><<<g, 256, 0, s>>>(fin, fout, alpha, stats, b, a, nf, BcTables{});
   }
   return hipGetLastError();
 }
 
 hipError_t launch_d3q27_step_links(const Box& b, const void* fin, void* fout, double* alpha,
-                                   unsigned long long* stats, const StepArgs& a,
+                                   int keep_alpha, unsigned long long* stats, const StepArgs& a,
                                    const BcArgs& bc, hipStream_t s) {
   const unsigned g = blocks_for(b.n, 256);
+  const BcTables t{bc.feq_in, bc.feq_out, bc.feq_fs};
   if (a.shifted) {
-    if (alpha) k27_step_periodic<true, true, true><<<g, 256, 0, s>>>(fin, fout, alpha, stats, b, a, bc.flags);
-    else k27_step_periodic<true, false, true><<<g, 256, 0, s>>>(fin, fout, alpha, stats, b, a, bc.flags);
+    if (keep_alpha) k27_step_periodic<true, true, true><<<g, 256, 0, s>>>(fin, fout, alpha, stats, b, a, bc.flags, t);
+    else k27_step_periodic<true, false, true><<<g, 256, 0, s>>>(fin, fout, alpha, stats, b, a, bc.flags, t);
   } else {
-    if (alpha) k27_step_periodic<false, true, true><<<g, 256, 0, s>>>(fin, fout, alpha, stats, b, a, bc.flags);
-    else k27_step_periodic<false, false, true><<<g, 256, 0, s>>>(fin, fout, alpha, stats, b, a, bc.flags);
+    if (keep_alpha) k27_step_periodic<false, true, true><<<g, 256, 0, s>>>(fin, fout, alpha, stats, b, a, bc.flags, t);
+    else k27_step_periodic<false, false, true><<<g, 256, 0, s>>>(fin, fout, alpha, stats, b, a, bc.flags, t);
   }
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess || bc.nlist == 0) return e;
-  const unsigned gb = blocks_for(bc.nlist, 256);
-  if (a.shifted) k27_boundary<true><<<gb, 256, 0, s>>>(fin, fout, alpha, stats, b, a, bc);
-  else k27_boundary<false><<<gb, 256, 0, s>>>(fin, fout, alpha, stats, b, a, bc);
+  return hipGetLastError();
+}
+
+hipError_t launch_d3q27_boundary(const Box& b, const void* fin, void* fout, double* alpha,
+                                 unsigned long long* stats, const StepArgs& a, const BcArgs& bc,
+                                 hipStream_t s) {
+  if (bc.nlist == 0) return hipSuccess;
+  const unsigned gb = blocks_for(bc.nlist, 64);
+  if (a.shifted) k27_boundary<true><<<gb, 64, 0, s>>>(fin, fout, alpha, stats, b, a, bc);
+  else k27_boundary<false><<<gb, 64, 0, s>>>(fin, fout, alpha, stats, b, a, bc);
   return hipGetLastError();
 }
 
@@ -496,9 +565,14 @@ hipError_t launch_d3q27_init_uniform(const Box& b, void* f, int shifted, const u
   return hipGetLastError();
 }
 
-hipError_t launch_d3q27_fixup(const Box& b, void* fout, double* alpha,
-                              unsigned long long* stats, const StepArgs& a, hipStream_t s) {
-  k27_fixup<<<512, 256, 0, s>>>(static_cast<float*>(fout), alpha, stats, b, a);
+hipError_t launch_d3q27_far(const Box& b, void* fout, double* alpha, int keep_alpha,
+                            unsigned long long* stats, const StepArgs& a, hipStream_t s) {
+  // persistent-style grid: 8 blocks per CU of a 256-CU device, grid-stride over the queue
+  const unsigned g = (unsigned)std::min<int64_t>(blocks_for(b.n, 256), 2048);
+  k27_far<<<g, 256, 0, s>>>(static_cast<float*>(fout), alpha, stats, b, a, keep_alpha);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  k_clear_count<<<1, 1, 0, s>>>(a.defer_count);
   return hipGetLastError();
 }
 

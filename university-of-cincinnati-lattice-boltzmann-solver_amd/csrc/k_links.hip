@@ -33,6 +33,10 @@
 namespace uclbs {
 namespace {
 
+struct FaceCodes {
+  int32_t code[6];
+};
+
 constexpr double kPlusEps = 1.24e-16;  // general/modules/precise.f95:13
 constexpr double kMinusEps = -1.24e-16;
 constexpr double kVSmall = 1.0e-14;    // precise.f95:16
@@ -223,17 +227,38 @@ __global__ __launch_bounds__(256) void k_links(LinkGeom g, int32_t* __restrict__
   if (ties) atomicAdd(counts + 1, (unsigned long long)ties);
 }
 
-// node flags: 0 interior fluid (every link a fluid code), 1 fluid with a
-// boundary link, 2 solid or null
-__global__ void k_flags(const int32_t* __restrict__ state, int64_t n, uint8_t* __restrict__ flags) {
+// node flags: interior fluid (every link a fluid code), face (fluid whose
+// negative links all leave the box with exactly the face's code, so the
+// streaming kernel can rebuild them from coordinates), boundary (any other
+// negative link: wall q codes, null links -- the boundary kernel), solid/null
+__global__ void k_flags(const int32_t* __restrict__ state, Box b, FaceCodes fc,
+                        uint8_t* __restrict__ flags) {
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  uint8_t fl = 0;
+  if (k >= b.n) return;
+  const int64_t n = b.n;
+  uint8_t fl = kFlagInterior;
   if (state[k] < 0) {
-    fl = 2;
+    fl = kFlagSolid;
   } else {
-    for (int i = 1; i < 27; ++i)
-      if (state[(int64_t)i * n + k] < 0) fl = 1;
+    const int a = (int)(k % b.nx);
+    const int64_t yz = k / b.nx;
+    const int bb = (int)(yz % b.ny), c = (int)(yz / b.ny);
+    for (int i = 1; i < 27; ++i) {
+      const int32_t s = state[(int64_t)i * n + k];
+      if (s >= 0) continue;
+      const int na = a + CX27[i], nb = bb + CY27[i], nc = c + CZ27[i];
+      const bool out[6] = {na < 0 && !b.per[0], nb < 0 && !b.per[1], nc < 0 && !b.per[2],
+                           na > b.nx - 1 && !b.per[0], nb > b.ny - 1 && !b.per[1],
+                           nc > b.nz - 1 && !b.per[2]};
+      int32_t want = 1;
+      for (int f = 5; f >= 0; --f)
+        if (out[f]) want = fc.code[f];
+      if (want == s) {
+        if (fl == kFlagInterior) fl = kFlagFace;
+      } else {
+        fl = kFlagBoundary;
+      }
+    }
   }
   flags[k] = fl;
 }
@@ -251,8 +276,11 @@ hipError_t launch_build_links27(const LinkGeom& g, int32_t* state, unsigned long
   return hipGetLastError();
 }
 
-hipError_t launch_flags27(const int32_t* state, int64_t n, uint8_t* flags, hipStream_t s) {
-  k_flags<<<blocks_for(n, 256), 256, 0, s>>>(state, n, flags);
+hipError_t launch_flags27(const int32_t* state, const Box& b, const int32_t face_code[6],
+                          uint8_t* flags, hipStream_t s) {
+  FaceCodes fc;
+  for (int f = 0; f < 6; ++f) fc.code[f] = face_code[f];
+  k_flags<<<blocks_for(b.n, 256), 256, 0, s>>>(state, b, fc, flags);
   return hipGetLastError();
 }
 

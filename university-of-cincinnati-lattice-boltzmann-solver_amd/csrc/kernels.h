@@ -28,21 +28,29 @@ struct StepArgs {
   int alpha_mode;
   int shifted;  // 1: FP32 f - w storage, 0: FP64 full
   SeriesThr thr;
-  // deferred-node queue (FP32 storage): counters[2] ping-pong by step parity
-  unsigned int* defer_count;
-  unsigned int* defer_list;
-  int parity;
+  int32_t face_code[6];  // x-, y-, z-, x+, y+, z+ (links leaving the box)
+  unsigned* defer_count;  // queue of far-from-equilibrium nodes (FP32 storage)
+  unsigned* defer_list;
+};
+
+// node flags of a level with link tables
+enum : uint8_t { kFlagInterior = 0, kFlagBoundary = 1, kFlagSolid = 2, kFlagFace = 3 };
+
+struct BcTables {
+  const double* feq_in;
+  const double* feq_out;
+  const double* feq_fs;
 };
 
 hipError_t launch_d3q27_step_periodic(const Box& b, const void* fin, void* fout, double* alpha,
-                                      unsigned long long* stats, const StepArgs& a,
-                                      hipStream_t s);
+                                      int keep_alpha, unsigned long long* stats,
+                                      const StepArgs& a, hipStream_t s);
 // Boundary data of a level whose links are defined: the reference-form link
 // table, the list of fluid nodes with a boundary link, and the equilibrium
 // populations of the inlets / outlets / freestream (full FP64 values).
 struct BcArgs {
   const int32_t* state;   // [27][n]
-  const uint8_t* flags;   // [n]: 0 interior fluid, 1 boundary fluid, 2 solid/null
+  const uint8_t* flags;   // [n]: kFlag* values
   const uint32_t* list;   // boundary fluid nodes
   int64_t nlist;
   int n_in, n_out;
@@ -65,16 +73,22 @@ struct LinkGeom {
 
 hipError_t launch_build_links27(const LinkGeom& g, int32_t* state, unsigned long long* counts,
                                 hipStream_t s);
-hipError_t launch_flags27(const int32_t* state, int64_t n, uint8_t* flags, hipStream_t s);
+hipError_t launch_flags27(const int32_t* state, const Box& b, const int32_t face_code[6],
+                          uint8_t* flags, hipStream_t s);
 hipError_t launch_d3q27_step_links(const Box& b, const void* fin, void* fout, double* alpha,
-                                   unsigned long long* stats, const StepArgs& a,
+                                   int keep_alpha, unsigned long long* stats, const StepArgs& a,
                                    const BcArgs& bc, hipStream_t s);
+// interior + face nodes (streaming kernel) and the boundary-list kernel; the
+// two write disjoint nodes and may run concurrently
+hipError_t launch_d3q27_boundary(const Box& b, const void* fin, void* fout, double* alpha,
+                                 unsigned long long* stats, const StepArgs& a, const BcArgs& bc,
+                                 hipStream_t s);
+// queued far-from-equilibrium nodes, after the streaming kernel
+hipError_t launch_d3q27_far(const Box& b, void* fout, double* alpha, int keep_alpha,
+                            unsigned long long* stats, const StepArgs& a, hipStream_t s);
 hipError_t launch_d3q27_init_uniform(const Box& b, void* f, int shifted, const uint8_t* flags,
                                      double rho, double ux, double uy, double uz, hipStream_t s);
 
-// general-path fixup for the nodes the step queued (FP32 storage only)
-hipError_t launch_d3q27_fixup(const Box& b, void* fout, double* alpha,
-                              unsigned long long* stats, const StepArgs& a, hipStream_t s);
 hipError_t launch_d3q27_from_fp64(const Box& b, const double* src, void* dst, int shifted,
                                   hipStream_t s);
 hipError_t launch_d3q27_to_fp64(const Box& b, const void* src, double* dst, int shifted,

@@ -43,9 +43,8 @@ struct Level {
   void* f[2] = {nullptr, nullptr};
   int cur = 0;  // index of the buffer holding the current post-collision state
   double* alpha = nullptr;
-  unsigned int* defer_count = nullptr;  // [2], ping-pong by step parity
-  unsigned int* defer_list = nullptr;   // n entries
-  int parity = 0;
+  unsigned* defer_count = nullptr;  // far-from-equilibrium queue
+  unsigned* defer_list = nullptr;
   // link tables (reference form) and the compact form the kernels use
   int32_t face_code[6] = {0, 0, 0, 0, 0, 0};
   int32_t* state = nullptr;   // [27][n]
@@ -59,7 +58,8 @@ struct Ctx {
   uclbs_config cfg{};
   int device = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipStream_t side = nullptr;  // boundary-list kernel, concurrent with the streaming kernel
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_fork = nullptr, ev_join = nullptr;
   std::vector<Level> levels;
   unsigned long long* d_stats = nullptr;
   // inlet / outlet / freestream equilibria (full FP64 populations)
@@ -162,15 +162,15 @@ int mirror_buffers(Ctx* c, Level* L) {
 int finalize_links(Ctx* c, Level* L) {
   const int64_t n = L->box.n;
   if (!L->flags) HIP_TRY(hipMalloc(&L->flags, (size_t)n));
-  HIP_TRY(uclbs::launch_flags27(L->state, n, L->flags, c->stream));
+  HIP_TRY(uclbs::launch_flags27(L->state, L->box, L->face_code, L->flags, c->stream));
   std::vector<uint8_t> hf((size_t)n);
   HIP_TRY(hipMemcpyAsync(hf.data(), L->flags, (size_t)n, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   std::vector<uint32_t> list;
   int64_t nsolid = 0;
   for (int64_t k = 0; k < n; ++k) {
-    if (hf[k] == 1) list.push_back((uint32_t)k);
-    nsolid += hf[k] == 2;
+    if (hf[k] == uclbs::kFlagBoundary) list.push_back((uint32_t)k);
+    nsolid += hf[k] == uclbs::kFlagSolid;
   }
   if (L->blist) { (void)hipFree(L->blist); L->blist = nullptr; }
   L->nb = (int64_t)list.size();
@@ -184,30 +184,60 @@ int finalize_links(Ctx* c, Level* L) {
   return UCLBS_OK;
 }
 
-int launch_steps(Ctx* c, Level* L, int nsteps) {
+// ev: optional [2*nsteps] events recorded around each step's streaming kernel
+// every face code must have its table before a kernel reads it
+int check_face_tables(const Ctx* c, const Level* L) {
+  for (int f = 0; f < 6; ++f) {
+    const int32_t v = L->face_code[f];
+    if (v <= -101 && v >= -199 && (-v - 100) > c->n_in)
+      return set_err(UCLBS_E_STATE, "face %d uses inlet %d but %d inlet states are set", f,
+                     -v - 100, c->n_in);
+    if (v <= -201 && v >= -299 && (-v - 200) > c->n_out)
+      return set_err(UCLBS_E_STATE, "face %d uses outlet %d but %d outlet states are set", f,
+                     -v - 200, c->n_out);
+    if (v == -1000 && !c->d_feq_fs)
+      return set_err(UCLBS_E_STATE, "face %d is a freestream face but no freestream state is set", f);
+  }
+  return UCLBS_OK;
+}
+
+int launch_steps(Ctx* c, Level* L, int nsteps, hipEvent_t* ev = nullptr) {
   uclbs::StepArgs a = step_args(c);
+  if (L->has_links) {
+    int rc = check_face_tables(c, L);
+    if (rc) return rc;
+  }
   a.defer_count = L->defer_count;
   a.defer_list = L->defer_list;
+  for (int f = 0; f < 6; ++f) a.face_code[f] = L->face_code[f];
   for (int s = 0; s < nsteps; ++s) {
     if (c->cfg.lattice != UCLBS_LATTICE_D3Q27)
       return set_err(UCLBS_E_UNSUPPORTED, "lattice %d not supported by step", c->cfg.lattice);
     if (!L->has_links && !(L->box.per[0] && L->box.per[1] && L->box.per[2]))
       return set_err(UCLBS_E_UNSUPPORTED,
                      "level without link tables must be periodic on all axes");
-    a.parity = L->parity;
+    double* al = c->keep_alpha ? L->alpha : nullptr;
+    if (L->has_links && L->nb) {
+      // boundary-list nodes on the side stream, concurrent with the streaming kernel
+      HIP_TRY(hipEventRecord(c->ev_fork, c->stream));
+      HIP_TRY(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+      HIP_TRY(uclbs::launch_d3q27_boundary(L->box, L->f[L->cur], L->f[1 - L->cur], al, c->d_stats,
+                                           a, bc_args(c, L), c->side));
+      HIP_TRY(hipEventRecord(c->ev_join, c->side));
+    }
+    if (ev) HIP_TRY(hipEventRecord(ev[2 * s], c->stream));
     if (L->has_links)
-      HIP_TRY(uclbs::launch_d3q27_step_links(L->box, L->f[L->cur], L->f[1 - L->cur],
-                                             c->keep_alpha ? L->alpha : nullptr, c->d_stats, a,
-                                             bc_args(c, L), c->stream));
+      HIP_TRY(uclbs::launch_d3q27_step_links(L->box, L->f[L->cur], L->f[1 - L->cur], L->alpha,
+                                             c->keep_alpha, c->d_stats, a, bc_args(c, L),
+                                             c->stream));
     else
-      HIP_TRY(uclbs::launch_d3q27_step_periodic(L->box, L->f[L->cur], L->f[1 - L->cur],
-                                                c->keep_alpha ? L->alpha : nullptr, c->d_stats,
-                                                a, c->stream));
+      HIP_TRY(uclbs::launch_d3q27_step_periodic(L->box, L->f[L->cur], L->f[1 - L->cur], L->alpha,
+                                                c->keep_alpha, c->d_stats, a, c->stream));
+    if (ev) HIP_TRY(hipEventRecord(ev[2 * s + 1], c->stream));
     if (a.shifted)
-      HIP_TRY(uclbs::launch_d3q27_fixup(L->box, L->f[1 - L->cur],
-                                        c->keep_alpha ? L->alpha : nullptr, c->d_stats, a,
-                                        c->stream));
-    L->parity ^= 1;
+      HIP_TRY(uclbs::launch_d3q27_far(L->box, L->f[1 - L->cur], L->alpha, c->keep_alpha,
+                                      c->d_stats, a, c->stream));
+    if (L->has_links && L->nb) HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_join, 0));
     L->cur = 1 - L->cur;
     c->steps_done++;
   }
@@ -241,6 +271,9 @@ int uclbs_create(const uclbs_config* cfg, void** out) {
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipEventCreate(&c->ev0);
   if (e == hipSuccess) e = hipEventCreate(&c->ev1);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming);
   if (e == hipSuccess)
     e = hipMalloc(&c->d_stats, sizeof(unsigned long long) * uclbs::kStatSlots * uclbs::kStatFields);
   if (e == hipSuccess)
@@ -267,6 +300,9 @@ int uclbs_destroy(void* ctx) {
   if (c->d_feq_fs) (void)hipFree(c->d_feq_fs);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->side) { (void)hipStreamSynchronize(c->side); (void)hipStreamDestroy(c->side); }
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return UCLBS_OK;
@@ -305,11 +341,10 @@ int uclbs_define_level(void* ctx, int lvl, const int32_t lo[3], const int32_t hi
     }
   }
   HIP_TRY(hipMalloc(&L.alpha, (size_t)L.box.n * sizeof(double)));
-  if (L.box.n > 0xffffffffll) return set_err(UCLBS_E_ARG, "box exceeds 2^32 nodes");
-  HIP_TRY(hipMalloc(&L.defer_count, 2 * sizeof(unsigned int)));
-  HIP_TRY(hipMemset(L.defer_count, 0, 2 * sizeof(unsigned int)));
-  HIP_TRY(hipMalloc(&L.defer_list, (size_t)L.box.n * sizeof(unsigned int)));
-  L.parity = 0;
+  HIP_TRY(hipMalloc(&L.defer_count, sizeof(unsigned)));
+  HIP_TRY(hipMemset(L.defer_count, 0, sizeof(unsigned)));
+  HIP_TRY(hipMalloc(&L.defer_list, (size_t)L.box.n * sizeof(unsigned)));
+  if (L.box.n > (1ll << 30)) return set_err(UCLBS_E_ARG, "box exceeds 2^30 nodes");
   L.cur = 0;
   L.defined = true;
   return UCLBS_OK;
@@ -622,6 +657,40 @@ int uclbs_time_steps(void* ctx, int lvl, int nsteps, double* elapsed_ms) {
   float ms = 0.f;
   HIP_TRY(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   *elapsed_ms = ms;
+  return UCLBS_OK;
+}
+
+int uclbs_time_kernels(void* ctx, int lvl, int nsteps, double* step_ms, double* main_ms) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  Level* L;
+  int rc = check_level(c, lvl, &L);
+  if (rc) return rc;
+  if (!step_ms || !main_ms || nsteps <= 0 || nsteps > 100000)
+    return set_err(UCLBS_E_ARG, "bad timing arguments");
+  if ((rc = set_device(c))) return rc;
+  std::vector<hipEvent_t> ev((size_t)2 * nsteps, nullptr);
+  hipError_t e = hipSuccess;
+  for (auto& x : ev)
+    if (e == hipSuccess) e = hipEventCreate(&x);
+  if (e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
+  if (e == hipSuccess) rc = launch_steps(c, L, nsteps, ev.data());
+  if (e == hipSuccess && rc == UCLBS_OK) e = hipEventRecord(c->ev1, c->stream);
+  if (e == hipSuccess && rc == UCLBS_OK) e = hipEventSynchronize(c->ev1);
+  double tot = 0.0;
+  if (e == hipSuccess && rc == UCLBS_OK) {
+    float ms = 0.f;
+    e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    *step_ms = ms;
+    for (int s = 0; s < nsteps && e == hipSuccess; ++s) {
+      e = hipEventElapsedTime(&ms, ev[2 * s], ev[2 * s + 1]);
+      tot += ms;
+    }
+    *main_ms = tot;
+  }
+  for (auto& x : ev)
+    if (x) (void)hipEventDestroy(x);
+  if (rc) return rc;
+  if (e != hipSuccess) return set_err(UCLBS_E_HIP, "kernel timing failed: %s", hipGetErrorString(e));
   return UCLBS_OK;
 }
 
