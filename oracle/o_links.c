@@ -144,6 +144,21 @@ int o_classify_node(const o_box *bx, int ntri, const double *tri, const double *
   return 0;
 }
 
+/* linkwise.f95:45-57: code of a crossing xc on a link of length link from e1 */
+int32_t o_link_code(const double e1[3], const double xc[3], double link) {
+  double d = sqrt((e1[0] - xc[0]) * (e1[0] - xc[0]) + (e1[1] - xc[1]) * (e1[1] - xc[1]) +
+                  (e1[2] - xc[2]) * (e1[2] - xc[2]));
+  return (int32_t)(-1001 - (int64_t)floor(d / link * 1e9));
+}
+
+int o_tri_volume(const double xc[3], const double e[3], const double t[9]) {
+  return tri_volume(xc, e, t);
+}
+
+int o_box_test(const double p1[3], const double p2[3], const double bb[6]) {
+  return box_test(p1, p2, bb);
+}
+
 /* inter_bb_setup for the segment node -> node + c dx; returns the link code */
 int32_t o_wall_code(const o_box *bx, int ntri, const double *tri, const double *nrm,
                     const double *tbb, int a, int b, int c, int cx, int cy, int cz, int *tie) {
@@ -168,9 +183,7 @@ int32_t o_wall_code(const o_box *bx, int ntri, const double *tri, const double *
       if (upper >= lower - VSMALL && upper <= lower + VSMALL) { hit = 1; *tie = 1; }
       else hit = tri_volume(xc, e1, t) != 0;
       if (hit) {
-        double d = sqrt((e1[0] - xc[0]) * (e1[0] - xc[0]) + (e1[1] - xc[1]) * (e1[1] - xc[1]) +
-                        (e1[2] - xc[2]) * (e1[2] - xc[2]));
-        return (int32_t)(-1001 - (int64_t)floor(d / link * 1e9));
+        return o_link_code(e1, xc, link);
       }
     } else if (upper >= -VSMALL && upper <= VSMALL) {
       *tie = 1; /* the reference nudges the node off the surface and retries */

@@ -94,8 +94,39 @@ def lib() -> ctypes.CDLL:
         L.o27_step_box.argtypes = [ctypes.POINTER(OBox), ctypes.POINTER(ctypes.c_int32), dp, dp,
                                    ctypes.POINTER(BoxParams27), dp, ctypes.POINTER(OStats),
                                    ctypes.POINTER(BcStats)]
+        for nm in ("o_tri_volume", "o_box_test"):
+            getattr(L, nm).argtypes = [dp, dp, dp]
+        L.o_link_code.argtypes = [dp, dp, D]
+        L.o_link_code.restype = ctypes.c_int32
         _lib = L
     return _lib
+
+
+REF_PATH = os.path.join(_DIR, "_ref", "libuclbs_ref.so")
+_ref = None
+
+
+def ref_lib():
+    """The reference's own FP64 routines compiled from its sources (built by
+    oracle/ref/Makefile where the reference tree is present), or None."""
+    global _ref
+    if _ref is None and os.path.exists(REF_PATH):
+        R = ctypes.CDLL(REF_PATH)
+        dp = ctypes.POINTER(ctypes.c_double)
+        D, I = ctypes.c_double, ctypes.c_int
+        R.ref_tri_volume.argtypes = [dp, dp, dp]
+        R.ref_box_test.argtypes = [dp, dp, dp]
+        R.ref_link_code.argtypes = [dp, dp, I, D]
+        R.ref_link_code.restype = ctypes.c_int32
+        R.ref_set_constants.argtypes = [D] * 6
+        R.ref_sutherland.argtypes = [D]
+        R.ref_sutherland.restype = D
+        R.ref_therm_cond.argtypes = [D, D]
+        R.ref_therm_cond.restype = D
+        R.ref_lu_decomp.argtypes = [dp, dp, dp]
+        R.ref_lu_decomp.restype = D
+        _ref = R
+    return _ref
 
 
 def _dp(a):
@@ -260,3 +291,20 @@ def uniform27(state, rho, u):
     solid = state[0] < 0
     f[:, solid] = w[:, None]
     return f
+
+
+def tri_volume(xc, e, t):
+    return lib().o_tri_volume(_dp(np.ascontiguousarray(xc, np.float64)),
+                              _dp(np.ascontiguousarray(e, np.float64)),
+                              _dp(np.ascontiguousarray(t, np.float64)))
+
+
+def box_test(p1, p2, bb):
+    return lib().o_box_test(_dp(np.ascontiguousarray(p1, np.float64)),
+                            _dp(np.ascontiguousarray(p2, np.float64)),
+                            _dp(np.ascontiguousarray(bb, np.float64)))
+
+
+def link_code(e1, xc, link):
+    return int(lib().o_link_code(_dp(np.ascontiguousarray(e1, np.float64)),
+                                 _dp(np.ascontiguousarray(xc, np.float64)), float(link)))

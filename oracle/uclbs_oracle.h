@@ -101,6 +101,12 @@ typedef struct {
  * resolved by a deterministic re-cast. */
 int64_t o_build_links27(const o_box *bx, int ntri, const double *tri, const double *nrm,
                         int32_t *state, int64_t *n_tie);
+/* geometry helpers of the table builder, exported for checks against the
+ * reference's own routines: tri_volume.f95, box_test.f95 (bb = xmax, xmin,
+ * ymax, ymin, zmax, zmin), linkwise.f95:45-57 */
+int o_tri_volume(const double xc[3], const double e[3], const double t[9]);
+int o_box_test(const double p1[3], const double p2[3], const double bb[6]);
+int32_t o_link_code(const double e1[3], const double xc[3], double link);
 int32_t o_wall_code(const o_box *bx, int ntri, const double *tri, const double *nrm,
                     const double *tbb, int a, int b, int c, int cx, int cy, int cz, int *tie);
 int o_classify_node(const o_box *bx, int ntri, const double *tri, const double *nrm,
