@@ -1,0 +1,536 @@
+/* D3Q39 compressible entropic step, FP64.  TEST INFRASTRUCTURE ONLY (see uclbs_oracle.h).
+ *
+ * Restates the reference's two-population (f, g) D3Q39 ELBM for one periodic
+ * box at level lvl:
+ *   constants     general/background_calcs/basic_fluids_calcs.f95:17-86,
+ *                 utilities/viscosity_and_timing.f95:29-73,
+ *                 general/background_calcs/timing_calcs.f95:24-26,
+ *                 general/modules/constants.f95:14-17
+ *   lattice       general/background_calcs/directions_and_weights.f95:436-536
+ *                 (shifted: c_x + 1 for primary flow +x when Mach >= 1.2)
+ *   moments       looping/timestepping/calculate_flow_variables.f95:201-228
+ *   LM Newton     lagrangian_multipliers/calculate_lagrangian_mults.f95:337-1098
+ *                 with lagrangian_multipliers/lu_decomp.f95
+ *   equilibrium   amr/modules/amr_processes.f95:1592-1747 (fieq_comp_39[_shifted])
+ *   alpha         looping/timestepping/calculate_alpha.f95:118-289 (the live
+ *                 closed-form estimate)
+ *   collision     looping/timestepping/collision.f95:206-293, find_q_bar.f95:33-220,
+ *                 find_little_q_bar.f95:23-52, amr_processes.f95:1751-1814,
+ *                 utilities/modules/quick_calcs.f95:24-215
+ *   repair        looping/timestepping/damp_it.f95:64-177
+ * Literals the reference writes without a kind suffix are default (single)
+ * REAL widened to REAL(8); they are reproduced as (double)<float literal>.
+ */
+#include "uclbs_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define NQ 39
+#define NM 13
+
+/* constants.f95:14-17 */
+#define C_PI 3.1415926535
+#define C_MOLEC_MASS_AIR ((double)28.97f)
+#define C_AVOGADRO 6.022e23
+#define C_K_BOLTZ 1.38064852e-23
+#define C_KAPPA_CONVERT 418.1
+#define C_PLUS_EPS 1.24e-16
+#define C_VSMALL 1.0e-14
+
+void o39_velocities(int shifted, int cx[39], int cy[39], int cz[39]) {
+  for (int i = 0; i < NQ; i++) {
+    cx[i] = O39_CX[i] + (shifted ? 1 : 0);  /* directions_and_weights.f95:477-481 */
+    cy[i] = O39_CY[i];
+    cz[i] = O39_CZ[i];
+  }
+}
+
+/* moment row of direction i in the Lagrange-multiplier order
+ * chi, zeta(3), pi_xx, pi_yy, pi_zz, pi_xy, pi_xz, pi_yz, lambda(3) */
+static void mrow(int x, int y, int z, int m[NM]) {
+  int c2 = x * x + y * y + z * z;
+  m[0] = 1; m[1] = x; m[2] = y; m[3] = z;
+  m[4] = x * x; m[5] = y * y; m[6] = z * z;
+  m[7] = x * y; m[8] = x * z; m[9] = y * z;
+  m[10] = x * c2; m[11] = y * c2; m[12] = z * c2;
+}
+
+void o39_consts_make(o39_consts *c, double gama, double temperature_k, double pressure_pa,
+                     double mach, double dx_m, int startup, int shifted, int turbulent, int lvl) {
+  memset(c, 0, sizeof(*c));
+  c->gama = gama;
+  c->shifted = shifted;
+  c->turbulent = turbulent;
+  c->lvl = lvl;
+  c->startup = startup;
+  /* basic_fluids_calcs.f95:17-27 */
+  double R = 287.052874;
+  c->R = R;
+  c->rho_ref = pressure_pa / (R * temperature_k);
+  c->t_ref = temperature_k;
+  c->p_ref = pressure_pa;
+  c->cv = 1.0 / (gama - 1.0);
+  c->cp = gama / (gama - 1.0);
+  c->lead_g = 2.0 * c->cv - 3.0;                 /* collision.f95:73 */
+  double temperature = 1.0 / gama, rho_inf = 1.0;  /* :31-33 */
+  double sos = sqrt(gama * R * c->t_ref);          /* :37-38 */
+  c->sos = sos;
+  c->mach = mach;
+  c->molec_weight = C_MOLEC_MASS_AIR / (C_AVOGADRO * 1000);  /* :86 */
+  /* viscosity_and_timing.f95:29-68 */
+  double S = (double)110.4f, visc_ref = (double)1.716E-5f, temp_ref = (double)273.15f;
+  double viscosity = visc_ref * pow(c->t_ref / temp_ref, 1.5) * ((temp_ref + S) / (c->t_ref + S));
+  c->mu_ref = c->rho_ref * sos * mach * 1;
+  c->kappa_ref = 1.0;  /* kappa_bottom = 1 for the start-up conductivity */
+  double kappa_base = 0.0;
+  o39_therm_cond(rho_inf, temperature, c, &kappa_base);
+  c->kappa_ref = rho_inf * sos * sos * sos / temperature;
+  c->init_beta1 = 0.5001;
+  c->init_beta2 = 0.5001;
+  double b1 = 1.0 / ((2.0 * viscosity) / (rho_inf * temperature) + 1.0);
+  double b2 = 1.0 / ((2.0 * kappa_base / c->kappa_ref) / (rho_inf * c->cp * temperature) + 1.0);
+  c->beta1_scale = (b1 - c->init_beta1) / (double)startup;
+  c->beta2_scale = (b2 - c->init_beta2) / (double)startup;
+  /* timing_calcs.f95:24-26, level spacing dx / 2^lvl */
+  double dx = dx_m / ldexp(1.0, lvl);
+  c->dt = dx / sos;
+  c->eps = c->dt / 1.0;
+  c->maxit = 50;
+}
+
+/* quick_calcs.f95:24-58 (inter_lvl = .false.) */
+double o39_sutherland(double T, const o39_consts *c, double *dim_mu, double *dim_temp) {
+  const double const_s = (double)110.15f, temp_ref = (double)273.15f, visc_ref = 1.716e-5;
+  double unit_temp = T * c->t_ref * c->gama;
+  double unit_mu = visc_ref * pow(unit_temp / temp_ref, 1.5) * ((temp_ref + const_s) / (unit_temp + const_s));
+  if (dim_mu) *dim_mu = unit_mu;
+  if (dim_temp) *dim_temp = unit_temp;
+  return unit_mu / c->mu_ref;
+}
+
+static double kfit(double ak, double bk, double ck, double dk, double ek, double unit_temp) {
+  /* integer powers by repeated multiplication, as the Fortran ** with an
+   * integer exponent evaluates them */
+  double kek = log(unit_temp / 10000.0);
+  double k2 = kek * kek, k3 = k2 * kek, k4 = k3 * kek;
+  return exp(ak * k4 + bk * k3 + ck * k2 + dk * kek + ek);
+}
+
+/* quick_calcs.f95:70-215; kappa_bottom = c->kappa_ref.  Returns 0, or -1
+ * where the reference calls error_out. */
+int o39_therm_cond(double rho, double T, const o39_consts *c, double *kappa) {
+  double kb = c->kappa_ref;
+  double unit_temp = T * c->t_ref * c->gama;
+  double unit_rho = rho * c->rho_ref;
+  double unit_p = unit_temp * unit_rho * c->R;  /* Pa: the 2250-4250 K bands compare this */
+  if (unit_temp <= 500.0) {
+    *kappa = C_KAPPA_CONVERT * (5.9776e-6 * pow(unit_temp, 1.5) / (unit_temp + (double)194.4f)) / kb;
+    return 0;
+  }
+  if (unit_temp <= 2250.0) {
+    unit_p = unit_rho * c->R * unit_temp / 101300.0;  /* atm (quick_calcs.f95:101-103) */
+    if (unit_p < 10.0 && unit_p > 0.1) { *kappa = kfit(0.334316, 3.28202, 11.9939, 20.0944, 4.62882, unit_temp) / kb; return 0; }
+    if (unit_p > 10.0 && unit_p < 100.0) { *kappa = kfit(0.413573, 3.83393, 13.1885, 20.7305, 4.27728, unit_temp) / kb; return 0; }
+    if (unit_p > 100.0 && unit_p < 1000.0) { *kappa = kfit(0.208749, 1.92122, 6.58813, 10.7630, -1.27699, unit_temp) / kb; return 0; }
+    return -1;  /* the 0.01..0.1 atm band can never be selected (quick_calcs.f95:139) */
+  }
+  if (unit_temp <= 4250.0) {
+    if (unit_p < 10.0 && unit_p > 0.1) { *kappa = kfit(10.9992, 387.106, 387.282, 5.48304, -1.20106, unit_temp) / kb; return 0; }
+    if (unit_p > 10.0 && unit_p < 100.0) { *kappa = kfit(82.1184, 308.927, 423.174, 250.668, 47.5889, unit_temp) / kb; return 0; }
+    if (unit_p > 100.0 && unit_p < 1000.0) { *kappa = kfit(38.8677, 123.284, 144.224, 72.8083, 0.684807, unit_temp) / kb; return 0; }
+    return -1;
+  }
+  return -1;
+}
+
+/* fieq_comp_39_shifted (amr_processes.f95:1729-1747); the un-shifted
+ * fieq_comp_39 (:1592-1723) is the same exponent written out per direction */
+void o39_feq(const double lm[13], double rho, int shifted, double feq[39]) {
+  int cx[NQ], cy[NQ], cz[NQ], m[NM];
+  o39_velocities(shifted, cx, cy, cz);
+  for (int i = 0; i < NQ; i++) {
+    mrow(cx[i], cy[i], cz[i], m);
+    double e = lm[0];
+    for (int k = 1; k < NM; k++) e += (double)m[k] * lm[k];
+    feq[i] = rho * exp(e);
+  }
+}
+
+/* One Newton update through the LU factorisation of lu_decomp.f95: Doolittle
+ * without pivoting (rows top-down, lower then upper), forward substitution
+ * of J(1,k) - target_k, back substitution from the last unknown, lm -= delta.
+ * ref_mode reproduces the two index slips of the reference,
+ * lower(5,4) using lower(5,2)*upper(3,4) (:105) and z9 using lower(9,7)*z8
+ * (:441); ref_mode = 0 is the exact factorisation.  Returns max|delta|. */
+double o39_lu_step(const double *J, const double *mom, double *lm, int ref_mode) {
+  /* mom = rho, u, v, w, Pxx, Pyy, Pzz, Pxy, Pxz, Pyz, qx, qy, qz; the
+   * momentum targets are formed as rho * u (lu_decomp.f95:426-430) */
+  double tgt[NM];
+  for (int k = 0; k < NM; k++) tgt[k] = mom[k];
+  for (int k = 1; k < 4; k++) tgt[k] = mom[0] * mom[k];
+  double L[NM][NM], U[NM][NM], z[NM], d[NM];
+  memset(L, 0, sizeof(L));
+  memset(U, 0, sizeof(U));
+  for (int j = 0; j < NM; j++) U[0][j] = J[j];
+  for (int k = 1; k < NM; k++) {
+    for (int j = 0; j < k; j++) {
+      double s = J[k * NM + j];
+      for (int m = 0; m < j; m++) {
+        double lk = L[k][m];
+        if (ref_mode && k == 4 && j == 3 && m == 2) lk = L[4][1];
+        s -= lk * U[m][j];
+      }
+      L[k][j] = s / U[j][j];
+    }
+    for (int j = k; j < NM; j++) {
+      double s = J[k * NM + j];
+      for (int m = 0; m < k; m++) s -= L[k][m] * U[m][j];
+      U[k][j] = s;
+    }
+  }
+  for (int k = 0; k < NM; k++) {
+    double s = J[k] - tgt[k];
+    for (int m = 0; m < k; m++) {
+      double zm = z[m];
+      if (ref_mode && k == 8 && m == 6) zm = z[7];
+      s -= L[k][m] * zm;
+    }
+    z[k] = s;
+  }
+  double mx = 0.0;
+  for (int k = NM - 1; k >= 0; k--) {
+    double s = z[k];
+    for (int m = NM - 1; m > k; m--) s -= U[k][m] * d[m];
+    d[k] = s / U[k][k];
+  }
+  for (int k = 0; k < NM; k++) {
+    double a = fabs(d[k]);
+    if (a > mx) mx = a;
+    lm[k] -= d[k];
+  }
+  return mx;
+}
+
+/* Jacobian J(k,l) = sum_i m_k(c_i) m_l(c_i) expo_i (row-major) */
+static void jacobian(const int *cx, const int *cy, const int *cz, const double *expo, double *J) {
+  for (int k = 0; k < NM * NM; k++) J[k] = 0.0;
+  for (int i = 0; i < NQ; i++) {
+    int m[NM];
+    mrow(cx[i], cy[i], cz[i], m);
+    for (int k = 0; k < NM; k++)
+      for (int l = 0; l < NM; l++) J[k * NM + l] += (double)(m[k] * m[l]) * expo[i];
+  }
+}
+
+/* calculate_lagrangian_mults.f95:43-49, 337-1098: targets rho, rho u, P,
+ * q = (rho |u|^2 + 5 rho T) u; Newton from lm until max|delta| <= 1e-7.
+ * expo_last = the equilibrium of the last iteration's multipliers (before
+ * its update: :1098).  Returns the iteration count, or -1 at the cap. */
+int o39_lm_solve(double rho, const double u[3], double T, const double P[6], double lm[13],
+                 double expo_last[39], int shifted, int ref_mode, int maxit) {
+  int cx[NQ], cy[NQ], cz[NQ];
+  o39_velocities(shifted, cx, cy, cz);
+  double vm = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  double qf = rho * vm + 5 * rho * T;
+  double tgt[NM] = {rho, u[0], u[1], u[2], P[0], P[1], P[2], P[3], P[4], P[5],
+                    qf * u[0], qf * u[1], qf * u[2]};
+  double J[NM * NM];
+  double max_diff = 1.0;
+  int it = 0;
+  while (max_diff > 1.0e-7) {
+    if (it >= maxit) return -1;
+    it++;
+    o39_feq(lm, rho, shifted, expo_last);
+    jacobian(cx, cy, cz, expo_last, J);
+    max_diff = o39_lu_step(J, tgt, lm, ref_mode);
+    if (!(max_diff == max_diff)) return -1;
+  }
+  return it;
+}
+
+/* calculate_alpha.f95:118-289 */
+double o39_alpha_estimate(const double f[39], const double diff[39], double T, int64_t step,
+                          const o39_consts *c, int *status) {
+  const double alpha_hi = 2.125, alpha_start = 1.0e-6;
+  double dim_mu, dim_temp;
+  *status = 0;
+  o39_sutherland(T, c, &dim_mu, &dim_temp);
+  double mfp = dim_mu / c->p_ref * sqrt(C_PI * C_K_BOLTZ * dim_temp / (2 * c->molec_weight));
+  double mft = mfp / sqrt(3 * C_K_BOLTZ * dim_temp / c->molec_weight);
+  double beta_mfp = c->dt / (2 * mft + c->dt);
+  double ft1 = 0, ft2 = 0, ft3 = 0, ft4 = 0, st1 = 0, st2 = 0, third = 0;
+  double cs1 = 0, cs2 = 0, cs3 = 0;
+  for (int i = 0; i < NQ; i++) {
+    double df = f[i] < 0.0 ? fabs(f[i]) : f[i];
+    double x = diff[i] / df;
+    cs1 = cs1 + df * (2.0 * (x * x) / (2 + x));
+    cs2 = cs2 + df * (x * x);
+    if (x <= 0.0 && x >= -1.0) cs3 = cs3 + df * (x * x * x);
+  }
+  if (fabs(cs3) < C_PLUS_EPS) { *status = 1; return 2.0; }
+  double asm_ = (cs2 - sqrt(cs2 * cs2 - 8.0 * cs3 * cs1)) / (2.0 * cs3);
+  if (asm_ > alpha_hi) asm_ = alpha_hi - 1.0e-9;
+  if (step < c->startup) asm_ = (asm_ - alpha_start) / (double)c->startup * (double)step;
+  for (int i = 0; i < NQ; i++) {
+    double df = f[i] < 0.0 ? fabs(f[i]) : f[i];
+    double x1 = diff[i] / df, x2 = x1 * x1, x3 = x1 * x1 * x1;
+    double f2 = x2 * df, f3 = x3 * df, f4 = (x2 * x2) * df, f5 = x2 * x3 * df, f6 = (x3 * x3) * df;
+    if (x1 <= 0.0 && x1 >= -1.0) { ft1 += f3; ft2 += f4; ft3 += f5; ft4 += f6; }
+    st1 += f2;
+    if (x1 >= 0.0)
+      st2 += f3 * (2.0 / (4 + asm_ * x1) + 1.0 / (4 + 2 * asm_ * x1) + 2.0 / (4 + 3 * asm_ * x1));
+    third = third - (60 * f2 + 60 * f3 + 11 * f4) / (60 + 90 * x1 + 36 * x2 + 3 * x3);
+  }
+  double bm2 = beta_mfp * beta_mfp;
+  double a2h = alpha_hi * alpha_hi, a3h = alpha_hi * alpha_hi * alpha_hi;
+  double b3 = beta_mfp * beta_mfp * beta_mfp;
+  double first = -bm2 * (ft1 / 6 - alpha_hi * beta_mfp * ft2 / 12 + a2h * bm2 * ft3 / 20 -
+                         a3h * b3 * ft4 / 5);
+  double second = st1 / 2 - 2 * asm_ * bm2 * st2 / 15;
+  double na = 0.0, a;
+  if (fabs(first) < C_VSMALL) {
+    a = 2.0;
+    *status = 1;
+  } else {
+    na = (-second + sqrt(second * second - 4 * first * third)) / (2 * first);
+    if (na < asm_) { a = asm_; *status = 2; }
+    else if (na > alpha_hi) { a = alpha_hi; *status = 3; }
+    else a = na;
+  }
+  if (na != na) *status = -1;  /* error_out (:303-313) */
+  return a;
+}
+
+/* collision.f95:206-293 for one node; feq from the stored multipliers */
+int o39_collide(const double f[39], const double g[39], const double lm[13], double rho,
+                const double u[3], double T, double alpha, int64_t step, const o39_consts *c,
+                double fout[39], double gout[39]) {
+  int cx[NQ], cy[NQ], cz[NQ];
+  o39_velocities(c->shifted, cx, cy, cz);
+  double feq[NQ], df[NQ], dg[NQ];
+  o39_feq(lm, rho, c->shifted, feq);
+  for (int i = 0; i < NQ; i++) {
+    df[i] = feq[i] - f[i];
+    dg[i] = c->lead_g * T * feq[i] - g[i];
+  }
+  /* epsilon_calculator (amr_processes.f95:1789-1814) */
+  double eps = step < c->startup ? (((double)step + 1.0) / (double)c->startup) * c->eps : c->eps;
+  /* find_q_bar.f95:52-220: central third moments Q_abc of the deviation */
+  double Q[3][3][3];
+  memset(Q, 0, sizeof(Q));
+  double xxx = 0, yyy = 0, zzz = 0, xxy = 0, xxz = 0, xyy = 0, xzz = 0, yyz = 0, yzz = 0, xyz = 0;
+  for (int i = 0; i < NQ; i++) {
+    double a = cx[i] - u[0], b = cy[i] - u[1], e = cz[i] - u[2];
+    xxx += a * a * a * df[i];
+    yyy += b * b * b * df[i];
+    zzz += e * e * e * df[i];
+    xxy += a * a * b * df[i];
+    xxz += a * a * e * df[i];
+    xyy += b * b * a * df[i];
+    xzz += e * e * a * df[i];
+    yyz += b * b * e * df[i];
+    yzz += e * e * b * df[i];
+    xyz += a * b * e * df[i];
+  }
+  double qabc[27] = {xxx, xxy, xxz, xxy, xyy, xyz, xxz, xyz, xzz,
+                     xxy, xyy, xyz, xyy, yyy, yyz, xyz, yyz, yzz,
+                     xxz, xyz, xzz, xyz, yyz, yzz, xzz, yzz, zzz};
+  double tc = 6.0 * T * T * T;
+  double qbar[NQ];
+  for (int i = 0; i < NQ; i++) {
+    int cv[3] = {cx[i], cy[i], cz[i]};
+    double sy = 0.0, sk = 0.0;
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++)
+        for (int e = 0; e < 3; e++) {
+          double q = qabc[9 * a + 3 * b + e];
+          sy += q * (double)(cv[a] * cv[b] * cv[e]);
+          sk += q * (double)(a == b ? cv[e] : 0);   /* kroen_val: delta_ab c_c */
+        }
+    qbar[i] = eps * (sy - 3 * T * sk) / tc;
+  }
+  /* find_little_q_bar.f95:23-52 */
+  double lq[3] = {0, 0, 0};
+  for (int i = 0; i < NQ; i++) {
+    lq[0] += (cx[i] - u[0]) * dg[i];
+    lq[1] += (cy[i] - u[1]) * dg[i];
+    lq[2] += (cz[i] - u[2]) * dg[i];
+  }
+  double ql = eps / T;
+  /* quick_sutherlands / quick_therm_cond, beta_1/2_calculator */
+  double mu = o39_sutherland(T, c, NULL, NULL), kappa = 0.0;
+  int rc = o39_therm_cond(rho, T, c, &kappa);
+  double b1, b2;
+  if (step < c->startup) {
+    b1 = c->init_beta1 + c->beta1_scale * (double)step;
+    b2 = c->init_beta2 + c->beta2_scale * (double)step;
+  } else {
+    b1 = 1.0 / ((ldexp(1.0, c->lvl + 1) * mu) / (rho * T) + 1.0);
+    b2 = 1.0 / ((2.0 * kappa) / (rho * c->cp * T) + 1.0);
+  }
+  for (int i = 0; i < NQ; i++) {
+    double hq = ql * (cx[i] * lq[0] + cy[i] * lq[1] + cz[i] * lq[2]);
+    fout[i] = f[i] + alpha * b1 * (df[i] - qbar[i]) + b2 * qbar[i];
+    gout[i] = g[i] + alpha * b1 * (dg[i] - hq) + b2 * hq;
+  }
+  return rc;
+}
+
+/* calculate_flow_variables.f95:201-228 (P_zz from the current w) */
+void o39_moments(const double f[39], const double g[39], int shifted, double *rho, double u[3],
+                 double *T, double P[6], double cv) {
+  int cx[NQ], cy[NQ], cz[NQ];
+  o39_velocities(shifted, cx, cy, cz);
+  double r = 0, ux = 0, uy = 0, uz = 0, en = 0;
+  for (int i = 0; i < NQ; i++) {
+    double cm = (double)(cx[i] * cx[i] + cy[i] * cy[i] + cz[i] * cz[i]);
+    r += f[i];
+    ux += cx[i] * f[i];
+    uy += cy[i] * f[i];
+    en = en + cm * f[i] + g[i];
+  }
+  for (int i = 0; i < NQ; i++) uz += cz[i] * f[i];
+  u[0] = ux / r; u[1] = uy / r; u[2] = uz / r;
+  double vm = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  *rho = r;
+  *T = (en - r * vm) / (2 * cv * r);
+  P[0] = r * (*T + u[0] * u[0]);
+  P[1] = r * (*T + u[1] * u[1]);
+  P[2] = r * (*T + u[2] * u[2]);
+  P[3] = r * u[0] * u[1];
+  P[4] = r * u[0] * u[2];
+  P[5] = r * u[1] * u[2];
+}
+
+static inline int64_t wrap(int64_t a, int64_t n) { a %= n; return a < 0 ? a + n : a; }
+
+/* damp_it.f95:64-92 for an all-fluid periodic box: a negative post-collision
+ * population is replaced by the mean of its positive neighbours along +-c_i
+ * (or their mean with |f|, or |f|).  jacobi = 1 reads the neighbours' values
+ * from before any repair (the order-free form the GPU uses); jacobi = 0 sweeps
+ * in the reference order (direction, then z, y, x) and sees earlier repairs. */
+int64_t o39_damp_periodic(int nx, int ny, int nz, double *fo, int shifted, int jacobi) {
+  int cx[NQ], cy[NQ], cz[NQ];
+  o39_velocities(shifted, cx, cy, cz);
+  const int64_t n = (int64_t)nx * ny * nz;
+  double *src = fo;
+  if (jacobi) {
+    src = (double *)malloc(sizeof(double) * (size_t)(n * NQ));
+    memcpy(src, fo, sizeof(double) * (size_t)(n * NQ));
+  }
+  int64_t nrep = 0;
+  for (int i = 0; i < NQ; i++) {
+    for (int64_t k = 0; k < n; k++) {
+      double v = src[(int64_t)i * n + k];
+      if (!(v < 0.0)) continue;
+      int64_t a = k % nx, b = (k / nx) % ny, cc = k / ((int64_t)nx * ny);
+      int64_t kp = wrap(a + cx[i], nx) + nx * (wrap(b + cy[i], ny) + ny * wrap(cc + cz[i], nz));
+      int64_t km = wrap(a - cx[i], nx) + nx * (wrap(b - cy[i], ny) + ny * wrap(cc - cz[i], nz));
+      double fp = src[(int64_t)i * n + kp], fm = src[(int64_t)i * n + km];
+      double r;
+      if (fp > 0.0 && fm > 0.0) r = (fp + fm) / 2.0;
+      else if (fp > 0.0 && fm < 0.0) r = (fp + fabs(v)) / 2;
+      else if (fp < 0.0 && fm > 0.0) r = (fm + fabs(v)) / 2;
+      else r = fabs(v);
+      fo[(int64_t)i * n + k] = r;
+      nrep++;
+    }
+  }
+  if (jacobi) free(src);
+  return nrep;
+}
+
+void o39_init_fields(int nx, int ny, int nz, const double *rho, const double *ux, const double *uy,
+                     const double *uz, const double *T, const o39_consts *c, double *f, double *g,
+                     double *lm, int64_t *bad) {
+  const int64_t n = (int64_t)nx * ny * nz;
+  int64_t nb = 0;
+#pragma omp parallel for schedule(static) reduction(+ : nb)
+  for (int64_t k = 0; k < n; k++) {
+    double u[3] = {ux[k], uy[k], uz[k]}, P[6], l[NM], e[NQ];
+    P[0] = rho[k] * (T[k] + u[0] * u[0]);
+    P[1] = rho[k] * (T[k] + u[1] * u[1]);
+    P[2] = rho[k] * (T[k] + u[2] * u[2]);
+    P[3] = rho[k] * u[0] * u[1];
+    P[4] = rho[k] * u[0] * u[2];
+    P[5] = rho[k] * u[1] * u[2];
+    /* start from the Maxwellian's multipliers for rho = 1 at (u, T) */
+    memset(l, 0, sizeof(l));
+    l[0] = -1.5 * log(2 * C_PI * T[k]) - (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]) / (2 * T[k]);
+    l[1] = u[0] / T[k]; l[2] = u[1] / T[k]; l[3] = u[2] / T[k];
+    l[4] = l[5] = l[6] = -1.0 / (2 * T[k]);
+    int it = o39_lm_solve(rho[k], u, T[k], P, l, e, c->shifted, 0, 200);
+    if (it < 0) nb++;
+    o39_feq(l, rho[k], c->shifted, e);
+    for (int i = 0; i < NQ; i++) {
+      f[(int64_t)i * n + k] = e[i];
+      g[(int64_t)i * n + k] = c->lead_g * T[k] * e[i];
+    }
+    for (int j = 0; j < NM; j++) lm[(int64_t)j * n + k] = l[j];
+  }
+  *bad = nb;
+}
+
+void o39_step_periodic(int nx, int ny, int nz, double *f, double *g, double *lm, double *alpha_out,
+                       const o39_consts *c, int64_t step0, int nsteps, int ref_lu, int jacobi,
+                       o39_stats *st) {
+  int cx[NQ], cy[NQ], cz[NQ];
+  o39_velocities(c->shifted, cx, cy, cz);
+  const int64_t n = (int64_t)nx * ny * nz;
+  double *fo = (double *)malloc(sizeof(double) * (size_t)(n * NQ));
+  double *go = (double *)malloc(sizeof(double) * (size_t)(n * NQ));
+  memset(st, 0, sizeof(*st));
+  for (int s = 0; s < nsteps; s++) {
+    int64_t step = step0 + s;
+    int64_t it_sum = 0, noconv = 0, a2 = 0, alo = 0, ahi = 0, nan = 0, kerr = 0;
+    /* post-collision state of the previous step lives in f, g: stream it */
+#pragma omp parallel for schedule(static) reduction(+ : it_sum, noconv, a2, alo, ahi, nan, kerr)
+    for (int64_t k = 0; k < n; k++) {
+      int64_t a = k % nx, b = (k / nx) % ny, cc = k / ((int64_t)nx * ny);
+      double fi[NQ], gi[NQ], l[NM], e[NQ], u[3], P[6], rho, T;
+      for (int i = 0; i < NQ; i++) {
+        int64_t src = wrap(a - cx[i], nx) + nx * (wrap(b - cy[i], ny) + ny * wrap(cc - cz[i], nz));
+        fi[i] = f[(int64_t)i * n + src];
+        gi[i] = g[(int64_t)i * n + src];
+      }
+      o39_moments(fi, gi, c->shifted, &rho, u, &T, P, c->cv);
+      for (int j = 0; j < NM; j++) l[j] = lm[(int64_t)j * n + k];
+      int it = o39_lm_solve(rho, u, T, P, l, e, c->shifted, ref_lu, c->maxit);
+      if (it < 0) { noconv++; it = c->maxit; }
+      it_sum += it;
+      for (int j = 0; j < NM; j++) lm[(int64_t)j * n + k] = l[j];
+      double diff[NQ];
+      for (int i = 0; i < NQ; i++) diff[i] = e[i] - fi[i];
+      double alpha = 2.0;
+      if (c->turbulent) {
+        int stt;
+        alpha = o39_alpha_estimate(fi, diff, T, step, c, &stt);
+        if (stt == 1) a2++;
+        else if (stt == 2) alo++;
+        else if (stt == 3) ahi++;
+        else if (stt < 0) nan++;
+      }
+      if (alpha_out) alpha_out[k] = alpha;
+      double fq[NQ], gq[NQ];
+      if (o39_collide(fi, gi, l, rho, u, T, alpha, step, c, fq, gq)) kerr++;
+      for (int i = 0; i < NQ; i++) {
+        fo[(int64_t)i * n + k] = fq[i];
+        go[(int64_t)i * n + k] = gq[i];
+      }
+    }
+    st->nodes += n;
+    st->lm_iters += it_sum;
+    st->lm_noconv += noconv;
+    st->n_alpha2 += a2;
+    st->n_alpha_lo += alo;
+    st->n_alpha_hi += ahi;
+    st->n_nan += nan;
+    st->n_kappa_err += kerr;
+    st->n_repair += o39_damp_periodic(nx, ny, nz, fo, c->shifted, jacobi);
+    memcpy(f, fo, sizeof(double) * (size_t)(n * NQ));
+    memcpy(g, go, sizeof(double) * (size_t)(n * NQ));
+  }
+  free(fo);
+  free(go);
+}

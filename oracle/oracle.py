@@ -58,6 +58,23 @@ class BcStats(ctypes.Structure):
     _fields_ = [("n_ibb_links", ctypes.c_int64), ("n_repair", ctypes.c_int64)]
 
 
+class Consts39(ctypes.Structure):
+    _fields_ = ([(n, ctypes.c_int) for n in ("shifted", "turbulent", "lvl", "startup", "maxit")] +
+                [(n, ctypes.c_double) for n in (
+                    "gama", "R", "cv", "cp", "lead_g", "t_ref", "p_ref", "rho_ref", "mu_ref",
+                    "kappa_ref", "sos", "mach", "molec_weight", "dt", "eps", "init_beta1",
+                    "init_beta2", "beta1_scale", "beta2_scale")])
+
+
+class Stats39(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in ("nodes", "lm_iters", "lm_noconv", "n_alpha2",
+                                              "n_alpha_lo", "n_alpha_hi", "n_nan",
+                                              "n_kappa_err", "n_repair")]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
 FACE_PERIODIC = 2147483647
 _lib = None
 
@@ -94,6 +111,25 @@ def lib() -> ctypes.CDLL:
         L.o27_step_box.argtypes = [ctypes.POINTER(OBox), ctypes.POINTER(ctypes.c_int32), dp, dp,
                                    ctypes.POINTER(BoxParams27), dp, ctypes.POINTER(OStats),
                                    ctypes.POINTER(BcStats)]
+        c39 = ctypes.POINTER(Consts39)
+        L.o39_consts_make.argtypes = [c39, D, D, D, D, D, I, I, I, I]
+        L.o39_sutherland.argtypes = [D, c39, dp, dp]
+        L.o39_sutherland.restype = D
+        L.o39_therm_cond.argtypes = [D, D, c39, dp]
+        L.o39_feq.argtypes = [dp, D, I, dp]
+        L.o39_lu_step.argtypes = [dp, dp, dp, I]
+        L.o39_lu_step.restype = D
+        L.o39_lm_solve.argtypes = [D, dp, D, dp, dp, dp, I, I, I]
+        L.o39_alpha_estimate.argtypes = [dp, dp, D, ctypes.c_int64, c39, ctypes.POINTER(I)]
+        L.o39_alpha_estimate.restype = D
+        L.o39_collide.argtypes = [dp, dp, dp, D, dp, D, D, ctypes.c_int64, c39, dp, dp]
+        L.o39_moments.argtypes = [dp, dp, I, dp, dp, dp, dp, D]
+        L.o39_damp_periodic.argtypes = [I, I, I, dp, I, I]
+        L.o39_damp_periodic.restype = ctypes.c_int64
+        L.o39_init_fields.argtypes = [I, I, I, dp, dp, dp, dp, dp, c39, dp, dp, dp,
+                                      ctypes.POINTER(ctypes.c_int64)]
+        L.o39_step_periodic.argtypes = [I, I, I, dp, dp, dp, dp, c39, ctypes.c_int64, I, I, I,
+                                        ctypes.POINTER(Stats39)]
         for nm in ("o_tri_volume", "o_box_test"):
             getattr(L, nm).argtypes = [dp, dp, dp]
         L.o_link_code.argtypes = [dp, dp, D]
@@ -308,3 +344,117 @@ def box_test(p1, p2, bb):
 def link_code(e1, xc, link):
     return int(lib().o_link_code(_dp(np.ascontiguousarray(e1, np.float64)),
                                  _dp(np.ascontiguousarray(xc, np.float64)), float(link)))
+
+
+# ---- D3Q39 -----------------------------------------------------------------
+def consts39(gama=1.4, temperature=300.0, pressure=101325.0, mach=0.3, dx=1e-3, startup=25,
+             shifted=False, turbulent=True, lvl=0):
+    c = Consts39()
+    lib().o39_consts_make(ctypes.byref(c), gama, temperature, pressure, mach, dx, startup,
+                          int(shifted), int(turbulent), lvl)
+    return c
+
+
+def velocities39(shifted=False):
+    cx, cy, cz = lattice39()[:3]
+    return cx + (1 if shifted else 0), cy, cz
+
+
+def feq39(lm, rho, shifted=False):
+    out = np.zeros(39)
+    lib().o39_feq(_dp(np.ascontiguousarray(lm, np.float64)), float(rho), int(shifted), _dp(out))
+    return out
+
+
+def lu_step39(J, tgt, lm, ref_mode=False):
+    """J: 13x13; tgt: rho, u, v, w, P(6), q(3); returns (new lm, max|delta|)."""
+    J = np.ascontiguousarray(J, np.float64)
+    tgt = np.ascontiguousarray(tgt, np.float64)
+    lm = np.array(lm, np.float64)
+    mx = lib().o39_lu_step(_dp(J), _dp(tgt), _dp(lm), int(ref_mode))
+    return lm, mx
+
+
+def lm_solve39(rho, u, T, P, lm0, shifted=False, ref_mode=False, maxit=50):
+    lm = np.array(lm0, np.float64)
+    e = np.zeros(39)
+    it = lib().o39_lm_solve(float(rho), _dp(np.ascontiguousarray(u, np.float64)), float(T),
+                            _dp(np.ascontiguousarray(P, np.float64)), _dp(lm), _dp(e),
+                            int(shifted), int(ref_mode), maxit)
+    return lm, e, it
+
+
+def sutherland39(T, c):
+    dm, dt_ = ctypes.c_double(), ctypes.c_double()
+    mu = lib().o39_sutherland(float(T), ctypes.byref(c), ctypes.byref(dm), ctypes.byref(dt_))
+    return mu, dm.value, dt_.value
+
+
+def therm_cond39(rho, T, c):
+    k = ctypes.c_double()
+    rc = lib().o39_therm_cond(float(rho), float(T), ctypes.byref(c), ctypes.byref(k))
+    return (k.value if rc == 0 else None)
+
+
+def alpha_est39(f, diff, T, step, c):
+    st = ctypes.c_int()
+    a = lib().o39_alpha_estimate(_dp(np.ascontiguousarray(f, np.float64)),
+                                 _dp(np.ascontiguousarray(diff, np.float64)), float(T), int(step),
+                                 ctypes.byref(c), ctypes.byref(st))
+    return a, st.value
+
+
+def moments39(f, g, c):
+    r, T = ctypes.c_double(), ctypes.c_double()
+    u = np.zeros(3)
+    P = np.zeros(6)
+    lib().o39_moments(_dp(np.ascontiguousarray(f, np.float64)), _dp(np.ascontiguousarray(g, np.float64)),
+                      c.shifted, ctypes.byref(r), _dp(u), ctypes.byref(T), _dp(P), c.cv)
+    return r.value, u, T.value, P
+
+
+def collide39(f, g, lm, rho, u, T, alpha, step, c):
+    fo, go = np.zeros(39), np.zeros(39)
+    lib().o39_collide(_dp(np.ascontiguousarray(f, np.float64)), _dp(np.ascontiguousarray(g, np.float64)),
+                      _dp(np.ascontiguousarray(lm, np.float64)), float(rho),
+                      _dp(np.ascontiguousarray(u, np.float64)), float(T), float(alpha), int(step),
+                      ctypes.byref(c), _dp(fo), _dp(go))
+    return fo, go
+
+
+def init39(shape, rho, ux, uy, uz, T, c):
+    nx, ny, nz = shape
+    n = nx * ny * nz
+    f = np.zeros((39, n))
+    g = np.zeros((39, n))
+    lm = np.zeros((13, n))
+    bad = ctypes.c_int64()
+    arrs = [np.ascontiguousarray(np.broadcast_to(a, (n,)), np.float64) for a in (rho, ux, uy, uz, T)]
+    lib().o39_init_fields(nx, ny, nz, *[_dp(a) for a in arrs], ctypes.byref(c), _dp(f), _dp(g),
+                          _dp(lm), ctypes.byref(bad))
+    assert bad.value == 0
+    return f, g, lm
+
+
+def step_periodic39(shape, f, g, lm, c, step0=0, nsteps=1, ref_lu=False, jacobi=True, keep_alpha=False):
+    nx, ny, nz = shape
+    f = np.array(f, np.float64)
+    g = np.array(g, np.float64)
+    lm = np.array(lm, np.float64)
+    alpha = np.zeros(nx * ny * nz) if keep_alpha else None
+    st = Stats39()
+    lib().o39_step_periodic(nx, ny, nz, _dp(f), _dp(g), _dp(lm),
+                            _dp(alpha) if alpha is not None else None, ctypes.byref(c), step0,
+                            nsteps, int(ref_lu), int(jacobi), ctypes.byref(st))
+    return f, g, lm, alpha, st.as_dict()
+
+
+def macro39(f, g, c):
+    cx, cy, cz = velocities39(bool(c.shifted))
+    rho = f.sum(0)
+    ux = (cx[:, None] * f).sum(0) / rho
+    uy = (cy[:, None] * f).sum(0) / rho
+    uz = (cz[:, None] * f).sum(0) / rho
+    cm = (cx ** 2 + cy ** 2 + cz ** 2)[:, None]
+    T = ((cm * f).sum(0) + g.sum(0) - rho * (ux ** 2 + uy ** 2 + uz ** 2)) / (2 * c.cv * rho)
+    return rho, ux, uy, uz, T

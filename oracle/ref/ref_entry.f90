@@ -81,8 +81,8 @@ contains
     call quick_therm_cond(kappa, rho, temp)
   end function
 
-  ! jac: 13x13 column-major; tgt: the 13 target moments rho, u, v, w, Pxx,
-  ! Pyy, Pzz, Pxy, Pxz, Pyz, qx, qy, qz; lm: chi, zeta(3), pi_xx, pi_yy,
+  ! jac: 13x13 column-major; tgt: rho, u, v, w, Pxx, Pyy, Pzz, Pxy, Pxz,
+  ! Pyz, qx, qy, qz (velocities, not momenta); lm: chi, zeta(3), pi_xx, pi_yy,
   ! pi_zz, pi_xy, pi_xz, pi_yz, lambda(3), updated in place.
   function ref_lu_decomp(jac, tgt, lm) bind(c, name='ref_lu_decomp') result(max_diff)
     real(c_double), intent(in) :: jac(13, 13), tgt(13)

@@ -125,6 +125,41 @@ typedef struct { int64_t n_ibb_links; int64_t n_repair; } o_bc_stats;
 void o27_step_box(const o_box *bx, const int32_t *state, const double *fin, double *fout,
                   const o27_box_params *p, double *alpha_out, o_stats *st, o_bc_stats *bst);
 
+
+/* ---- D3Q39 two-population compressible step (o_d3q39.c) ---------------- */
+typedef struct o39_consts {
+  int shifted, turbulent, lvl, startup, maxit;
+  double gama, R, cv, cp, lead_g, t_ref, p_ref, rho_ref, mu_ref, kappa_ref, sos, mach;
+  double molec_weight, dt, eps, init_beta1, init_beta2, beta1_scale, beta2_scale;
+} o39_consts;
+typedef struct o39_stats {
+  int64_t nodes, lm_iters, lm_noconv, n_alpha2, n_alpha_lo, n_alpha_hi, n_nan, n_kappa_err,
+      n_repair;
+} o39_stats;
+void o39_velocities(int shifted, int cx[39], int cy[39], int cz[39]);
+void o39_consts_make(o39_consts *c, double gama, double temperature_k, double pressure_pa,
+                     double mach, double dx_m, int startup, int shifted, int turbulent, int lvl);
+double o39_sutherland(double T, const o39_consts *c, double *dim_mu, double *dim_temp);
+int o39_therm_cond(double rho, double T, const o39_consts *c, double *kappa);
+void o39_feq(const double lm[13], double rho, int shifted, double feq[39]);
+double o39_lu_step(const double *J, const double *mom, double *lm, int ref_mode);
+int o39_lm_solve(double rho, const double u[3], double T, const double P[6], double lm[13],
+                 double expo_last[39], int shifted, int ref_mode, int maxit);
+double o39_alpha_estimate(const double f[39], const double diff[39], double T, int64_t step,
+                          const o39_consts *c, int *status);
+int o39_collide(const double f[39], const double g[39], const double lm[13], double rho,
+                const double u[3], double T, double alpha, int64_t step, const o39_consts *c,
+                double fout[39], double gout[39]);
+void o39_moments(const double f[39], const double g[39], int shifted, double *rho, double u[3],
+                 double *T, double P[6], double cv);
+int64_t o39_damp_periodic(int nx, int ny, int nz, double *fo, int shifted, int jacobi);
+void o39_init_fields(int nx, int ny, int nz, const double *rho, const double *ux, const double *uy,
+                     const double *uz, const double *T, const o39_consts *c, double *f, double *g,
+                     double *lm, int64_t *bad);
+void o39_step_periodic(int nx, int ny, int nz, double *f, double *g, double *lm, double *alpha_out,
+                       const o39_consts *c, int64_t step0, int nsteps, int ref_lu, int jacobi,
+                       o39_stats *st);
+
 #ifdef __cplusplus
 }
 #endif
