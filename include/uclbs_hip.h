@@ -152,6 +152,48 @@ int uclbs_init_uniform(void *ctx, int lvl, double rho, double u, double v, doubl
 int uclbs_geom_icosphere(int level, double radius, const double center[3], uint64_t seed,
                          double *tri, double *nrm, int64_t capacity, int64_t *ntri);
 
+/* ---- D3Q39 two-population compressible model ----------------------------
+ * A context created with lattice = UCLBS_LATTICE_D3Q39 advances f and g
+ * (reference mffi/mfgi, amr_info_holder.f95:50-76) and the 13 Lagrange
+ * multipliers per node (the warm start of calculate_lagrangian_mults).
+ * Storage: FP32 deviations from the freestream equilibrium, or FP64.  Stats
+ * for this lattice: newton_iters = multiplier Newton iterations, n_noconv =
+ * Newton at its cap, n_trivial = alpha estimate 2 (calculate_alpha.f95:171-175,
+ * 277-280), n_bound = estimate clamped (:281-289), n_nonpos = no real root
+ * (the reference's error_out at :303-313), n_fallback = conductivity outside
+ * the fitted bands (quick_calcs.f95:149-153, 207-212), n_repair =
+ * populations repaired by damp_it.f95:64-92.  uclbs_step_level returns
+ * UCLBS_E_NUMERIC when a step hit either error case. */
+typedef struct uclbs_gas {
+  double gama;          /* ratio of specific heats (namelist gama) */
+  double temperature;   /* freestream temperature, K */
+  double pressure;      /* freestream pressure, Pa */
+  double mach;          /* freestream Mach number, primary flow along +x */
+  double dx;            /* level-0 grid spacing, m (amrex_geom(0)%dx(1)) */
+  int32_t startup_timesteps;  /* namelist startup_timesteps (reference default 25) */
+  int32_t turbulent;    /* 1: alpha from calculate_alpha's estimate, 0: alpha = 2 */
+  int32_t shifted;      /* -1: shift c_x by +1 when mach >= 1.2 (directions_and_weights.f95:477),
+                           0 / 1: force */
+  int32_t lm_maxit;     /* multiplier Newton cap per node (the reference has none); 0 -> 50 */
+  int32_t reserved[4];
+} uclbs_gas;
+/* Derives the gas constants as basic_fluids_calcs.f95:17-86,
+ * viscosity_and_timing.f95:29-73 and timing_calcs.f95:24-26 do.  Call before
+ * uclbs_define_level on a D3Q39 context. */
+int uclbs_set_gas(void *ctx, const uclbs_gas *gas);
+/* Equilibrium state of the given fields (n_nodes FP64 each, layout as f):
+ * multipliers by Newton from the Maxwellian's, f = f_eq, g = (2 c_v - 3) T f_eq. */
+int uclbs_init_fields(void *ctx, int lvl, const double *rho, const double *u, const double *v,
+                      const double *w, const double *temp);
+/* f, g: [39][n] FP64 full values; lm: [13][n] multipliers (chi, zeta_x..z,
+ * pi_xx, pi_yy, pi_zz, pi_xy, pi_xz, pi_yz, lambda_x..z), any may be NULL on download */
+int uclbs_upload_state(void *ctx, int lvl, const double *f, const double *g, const double *lm,
+                       int64_t n_nodes);
+int uclbs_download_state(void *ctx, int lvl, double *f, double *g, double *lm, int64_t n_nodes);
+/* The level's step counter (reference timestep(lvl)), which drives the
+ * start-up ramps of beta, epsilon and alpha; advanced by every step. */
+int uclbs_set_timestep(void *ctx, int lvl, int64_t step);
+
 /* Advance level lvl by nsteps fused stream+moments+alpha+collide steps:
  * the streaming / calculate_flow_variables / collision calls of
  * loop.f95:105,112,133 for one level, in that order. */

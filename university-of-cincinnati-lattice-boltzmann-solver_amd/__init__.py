@@ -26,7 +26,8 @@ EXPORTS = (
     "uclbs_get_stats", "uclbs_reset_stats", "uclbs_set_faces", "uclbs_set_bc_states",
     "uclbs_build_links", "uclbs_set_links", "uclbs_download_links", "uclbs_link_counts",
     "uclbs_init_uniform", "uclbs_geom_icosphere", "uclbs_time_kernels",
-    "uclbs_copy_bandwidth",
+    "uclbs_copy_bandwidth", "uclbs_set_gas", "uclbs_init_fields", "uclbs_upload_state",
+    "uclbs_download_state", "uclbs_set_timestep",
 )
 FACE_PERIODIC = 2147483647
 
@@ -37,6 +38,16 @@ ALPHA_MODE = {"newton": 0, "fixed2": 1, "learned": 2}
 
 class UclbsError(RuntimeError):
     pass
+
+
+class Gas(ctypes.Structure):
+    """uclbs_gas: the D3Q39 gas and freestream (namelist gama, temperature,
+    pressure, Mach, dx, startup_timesteps, turbulent)."""
+    _fields_ = [("gama", ctypes.c_double), ("temperature", ctypes.c_double),
+                ("pressure", ctypes.c_double), ("mach", ctypes.c_double), ("dx", ctypes.c_double),
+                ("startup_timesteps", ctypes.c_int32), ("turbulent", ctypes.c_int32),
+                ("shifted", ctypes.c_int32), ("lm_maxit", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 4)]
 
 
 class Config(ctypes.Structure):
@@ -102,6 +113,11 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "uclbs_geom_icosphere": (I, [I, D, dp, U64, dp, dp, I64, ctypes.POINTER(I64)]),
         "uclbs_time_kernels": (I, [P, I, I, dp, dp]),
         "uclbs_copy_bandwidth": (I, [I64, I, dp]),
+        "uclbs_set_gas": (I, [P, ctypes.POINTER(Gas)]),
+        "uclbs_init_fields": (I, [P, I, dp, dp, dp, dp, dp]),
+        "uclbs_upload_state": (I, [P, I, dp, dp, dp, I64]),
+        "uclbs_download_state": (I, [P, I, dp, dp, dp, I64]),
+        "uclbs_set_timestep": (I, [P, I, I64]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -196,6 +212,38 @@ class Solver:
         n = self.n_nodes(lvl)
         f = np.ascontiguousarray(f, dtype=np.float64).reshape(self.q, n)
         self._check(self.lib.uclbs_upload_f(self.ctx, lvl, _dptr(f), n))
+
+    # ---- D3Q39 ---------------------------------------------------------------
+    def set_gas(self, gama=1.4, temperature=300.0, pressure=101325.0, mach=0.3, dx=1e-3,
+                startup=25, turbulent=True, shifted=-1, lm_maxit=50):
+        g = Gas(gama=gama, temperature=temperature, pressure=pressure, mach=mach, dx=dx,
+                startup_timesteps=startup, turbulent=int(turbulent), shifted=shifted,
+                lm_maxit=lm_maxit)
+        self._check(self.lib.uclbs_set_gas(self.ctx, ctypes.byref(g)))
+
+    def init_fields(self, lvl: int, rho, u, v, w, temp):
+        n = self.n_nodes(lvl)
+        arrs = [np.ascontiguousarray(np.broadcast_to(np.asarray(a, np.float64), (n,)))
+                for a in (rho, u, v, w, temp)]
+        self._check(self.lib.uclbs_init_fields(self.ctx, lvl, *[_dptr(a) for a in arrs]))
+
+    def upload_state(self, lvl: int, f, g, lm):
+        n = self.n_nodes(lvl)
+        f = np.ascontiguousarray(f, np.float64).reshape(39, n)
+        g = np.ascontiguousarray(g, np.float64).reshape(39, n)
+        lm = np.ascontiguousarray(lm, np.float64).reshape(13, n)
+        self._check(self.lib.uclbs_upload_state(self.ctx, lvl, _dptr(f), _dptr(g), _dptr(lm), n))
+
+    def download_state(self, lvl: int):
+        n = self.n_nodes(lvl)
+        f = np.empty((39, n))
+        g = np.empty((39, n))
+        lm = np.empty((13, n))
+        self._check(self.lib.uclbs_download_state(self.ctx, lvl, _dptr(f), _dptr(g), _dptr(lm), n))
+        return f, g, lm
+
+    def set_timestep(self, lvl: int, step: int):
+        self._check(self.lib.uclbs_set_timestep(self.ctx, lvl, int(step)))
 
     def set_faces(self, lvl: int, codes):
         c6 = (ctypes.c_int32 * 6)(*[int(v) for v in codes])

@@ -1,0 +1,488 @@
+// D3Q39 kernels: fused pull-stream + moments + multiplier Newton + alpha
+// estimate + collision for the two-population compressible model, the
+// positivity repair, initialisation from macroscopic fields and layout
+// conversion.
+//
+// Layout in HBM: f[q][node], g[q][node] (q = 0..38, node = x + nx(y + ny z)),
+// FP32 deviations from the reference-state equilibrium (f - f_ref, g - g_ref)
+// or FP64 full values; the 13 Lagrange multipliers lm[k][node] in FP64 (the
+// warm start of the next step's Newton).  One thread per node, one wavefront
+// per block: the node's pulled populations sit in LDS (39 + 39 values), the
+// 91-entry packed Jacobian and the collision temporaries in registers.  The
+// step is FP64-bound: per node and Newton iteration 39 exponentials, a
+// Jacobian of ~1.4k FMAs (integer weights folded at compile time) and a
+// 13 x 13 L D L^T solve.
+//
+// Reference correspondence: streaming.f95:365-368 (pull form), then
+// calculate_flow_variables.f95:201-311, calculate_alpha.f95:118-289,
+// collision.f95:206-293 with find_q_bar.f95 / find_little_q_bar.f95 /
+// quick_calcs.f95 / amr_processes.f95:1751-1814, and damp_it.f95:64-92.
+#include "d3q39.h"
+#include "kernels.h"
+
+#include <type_traits>
+
+namespace uclbs {
+
+namespace {
+
+__device__ __forceinline__ int wrapm(int s, int n) {
+  s += (s < 0) ? n : 0;
+  return s - ((s >= n) ? n : 0);
+}
+
+__device__ __forceinline__ void wave_stats39(unsigned long long* __restrict__ stats, bool active,
+                                             int iters, bool noconv, int ast, bool kerr) {
+  const unsigned long long act = __ballot(active);
+  if (act == 0ull) return;
+  unsigned long long it = 0;
+#pragma unroll
+  for (int bt = 0; bt < 8; ++bt)
+    it += (unsigned long long)__popcll(__ballot(active && ((iters >> bt) & 1))) << bt;
+  const unsigned long long nc = __popcll(__ballot(active && noconv));
+  const unsigned long long nt = __popcll(__ballot(active && ast == 1));
+  const unsigned long long nb = __popcll(__ballot(active && (ast == 2 || ast == 3)));
+  const unsigned long long nn = __popcll(__ballot(active && ast < 0));
+  const unsigned long long nk = __popcll(__ballot(active && kerr));
+  if (__lane_id() == 0) {
+    unsigned long long* s = stats + (size_t)(blockIdx.x & (kStatSlots - 1)) * kStatFields;
+    atomicAdd(s + ST_NODES, (unsigned long long)__popcll(act));
+    if (it) atomicAdd(s + ST_ITERS, it);
+    if (nc) atomicAdd(s + ST_NOCONV, nc);
+    if (nt) atomicAdd(s + ST_TRIVIAL, nt);
+    if (nb) atomicAdd(s + ST_BOUND, nb);
+    if (nn) atomicAdd(s + ST_NONPOS, nn);
+    if (nk) atomicAdd(s + ST_FALLBACK, nk);
+  }
+}
+
+// calculate_alpha.f95:118-289 over the populations f (LDS, full value f_i =
+// sf + fref) and the last iteration's equilibrium from lm_prev; sd is LDS
+// scratch for the deviation.  Returns alpha; *st = 0 (root), 1 (alpha = 2),
+// 2 / 3 (clamped low / high), -1 (no real root: the reference stops).
+template <bool kShifted, typename S>
+__device__ __forceinline__ double alpha_estimate39(const S (*sf)[64], double (*sd)[64], int t,
+                                                   const Ref39& rf, const double lmp[kM39],
+                                                   double rho, double T, long long step,
+                                                   const Gas39& gs, int* st) {
+  const double alpha_hi = 2.125, alpha_start = 1.0e-6;
+  double dim_mu, dim_temp;
+  sutherland39(T, gs, &dim_mu, &dim_temp);
+  const double kb = 1.38064852e-23, pi = 3.1415926535;
+  const double mfp = dim_mu / gs.p_ref * sqrt(pi * kb * dim_temp / (2 * gs.molec_weight));
+  const double mft = mfp / sqrt(3 * kb * dim_temp / gs.molec_weight);
+  const double beta_mfp = gs.dt / (2 * mft + gs.dt);
+  double cs1 = 0.0, cs2 = 0.0, cs3 = 0.0;
+#pragma unroll 1
+  for (int i = 0; i < Q39; ++i) {
+    const double f = (double)sf[i][t] + rf.f[i];
+    const double d = expo39<kShifted>(lmp, rho, i) - f;
+    sd[i][t] = d;
+    const double df = f < 0.0 ? fabs(f) : f;
+    const double x = d / df;
+    cs1 = cs1 + df * (2.0 * (x * x) / (2 + x));
+    cs2 = cs2 + df * (x * x);
+    if (x <= 0.0 && x >= -1.0) cs3 = cs3 + df * (x * x * x);
+  }
+  *st = 0;
+  if (fabs(cs3) < 1.24e-16) {
+    *st = 1;
+    return 2.0;
+  }
+  double asm_ = (cs2 - sqrt(cs2 * cs2 - 8.0 * cs3 * cs1)) / (2.0 * cs3);
+  if (asm_ > alpha_hi) asm_ = alpha_hi - 1.0e-9;
+  if (step < gs.startup) asm_ = (asm_ - alpha_start) / (double)gs.startup * (double)step;
+  double ft1 = 0, ft2 = 0, ft3 = 0, ft4 = 0, st1 = 0, st2 = 0, third = 0;
+#pragma unroll 1
+  for (int i = 0; i < Q39; ++i) {
+    const double f = (double)sf[i][t] + rf.f[i];
+    const double df = f < 0.0 ? fabs(f) : f;
+    const double x1 = sd[i][t] / df, x2 = x1 * x1, x3 = x1 * x1 * x1;
+    const double f2 = x2 * df, f3 = x3 * df, f4 = (x2 * x2) * df, f5 = x2 * x3 * df,
+                 f6 = (x3 * x3) * df;
+    if (x1 <= 0.0 && x1 >= -1.0) {
+      ft1 += f3;
+      ft2 += f4;
+      ft3 += f5;
+      ft4 += f6;
+    }
+    st1 += f2;
+    if (x1 >= 0.0)
+      st2 += f3 * (2.0 / (4 + asm_ * x1) + 1.0 / (4 + 2 * asm_ * x1) + 2.0 / (4 + 3 * asm_ * x1));
+    third = third - (60 * f2 + 60 * f3 + 11 * f4) / (60 + 90 * x1 + 36 * x2 + 3 * x3);
+  }
+  const double bm2 = beta_mfp * beta_mfp, b3 = bm2 * beta_mfp;
+  const double a2h = alpha_hi * alpha_hi, a3h = a2h * alpha_hi;
+  const double first = -bm2 * (ft1 / 6 - alpha_hi * beta_mfp * ft2 / 12 + a2h * bm2 * ft3 / 20 -
+                               a3h * b3 * ft4 / 5);
+  const double second = st1 / 2 - 2 * asm_ * bm2 * st2 / 15;
+  if (fabs(first) < 1.0e-14) {
+    *st = 1;
+    return 2.0;
+  }
+  const double na = (-second + sqrt(second * second - 4 * first * third)) / (2 * first);
+  if (!(na == na)) {
+    *st = -1;
+    return 2.0;
+  }
+  if (na < asm_) {
+    *st = 2;
+    return asm_;
+  }
+  if (na > alpha_hi) {
+    *st = 3;
+    return alpha_hi;
+  }
+  return na;
+}
+
+template <bool kFp32, bool kShifted, bool kTurb, bool kKeepAlpha>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void k39_step(
+    const void* __restrict__ fA_, const void* __restrict__ gA_, void* __restrict__ fB_,
+    void* __restrict__ gB_, double* __restrict__ lmv, double* __restrict__ alpha, Box b,
+    Gas39 gs, Ref39 rf, long long step, unsigned long long* __restrict__ stats,
+    unsigned* __restrict__ neg_count, unsigned* __restrict__ neg_list) {
+  using S = typename std::conditional<kFp32, float, double>::type;
+  const S* __restrict__ fA = static_cast<const S*>(fA_);
+  const S* __restrict__ gA = static_cast<const S*>(gA_);
+  S* __restrict__ fB = static_cast<S*>(fB_);
+  S* __restrict__ gB = static_cast<S*>(gB_);
+  __shared__ S sf[Q39][64];
+  __shared__ S sg[Q39][64];
+  __shared__ double sd[Q39][64];
+  const int t = threadIdx.x;
+  const unsigned n = (unsigned)b.n;
+  const unsigned node = blockIdx.x * 64u + (unsigned)t;
+  const bool active = node < n;
+  int iters = 0, ast = 0;
+  bool noconv = false, kerr = false;
+  if (active) {
+    const int nx = b.nx, ny = b.ny, nz = b.nz;
+    const int x = (int)(node % (unsigned)nx);
+    const int y = (int)((node / (unsigned)nx) % (unsigned)ny);
+    const int z = (int)(node / ((unsigned)nx * (unsigned)ny));
+    double rho = 0.0, mx = 0.0, my = 0.0, mz = 0.0, en = 0.0;
+#pragma unroll
+    for (int i = 0; i < Q39; ++i) {
+      const int cx = cx39<kShifted>(i), cy = CY39[i], cz = CZ39[i];
+      const unsigned src = (unsigned)wrapm(x - cx, nx) +
+                           (unsigned)nx * ((unsigned)wrapm(y - cy, ny) +
+                                           (unsigned)ny * (unsigned)wrapm(z - cz, nz));
+      const S fs = fA[(size_t)i * n + src];
+      const S gsv = gA[(size_t)i * n + src];
+      sf[i][t] = fs;
+      sg[i][t] = gsv;
+      const double f = (double)fs + rf.f[i], g = (double)gsv + rf.g[i];
+      const int cm = cx * cx + cy * cy + cz * cz;
+      rho += f;
+      if (cx) mx = fma((double)cx, f, mx);
+      if (cy) my = fma((double)cy, f, my);
+      if (cz) mz = fma((double)cz, f, mz);
+      en += (double)cm * f + g;
+    }
+    double u[3] = {mx / rho, my / rho, mz / rho};
+    const double vm = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+    const double T = (en - rho * vm) / (2 * gs.cv * rho);
+    const double qf = rho * vm + 5 * rho * T;
+    const double tg[kM39] = {rho, rho * u[0], rho * u[1], rho * u[2], rho * (T + u[0] * u[0]),
+                             rho * (T + u[1] * u[1]), rho * (T + u[2] * u[2]), rho * u[0] * u[1],
+                             rho * u[0] * u[2], rho * u[1] * u[2], qf * u[0], qf * u[1],
+                             qf * u[2]};
+    double lm[kM39], lmp[kM39];
+#pragma unroll
+    for (int k = 0; k < kM39; ++k) lm[k] = lmv[(size_t)k * n + node];
+    const int it = lm_newton39<kShifted>(rho, tg, lm, lmp, gs.maxit);
+    noconv = it < 0;
+    iters = noconv ? gs.maxit : it;
+#pragma unroll
+    for (int k = 0; k < kM39; ++k) lmv[(size_t)k * n + node] = lm[k];
+
+    double al = 2.0;
+    if (kTurb) al = alpha_estimate39<kShifted>(sf, sd, t, rf, lmp, rho, T, step, gs, &ast);
+    if (kKeepAlpha) alpha[node] = al;
+
+    // collision: deviation from the stored multipliers' equilibrium (D in
+    // LDS; the g deviation is rebuilt from f + D where it is needed)
+    double qxxx = 0, qyyy = 0, qzzz = 0, qxxy = 0, qxxz = 0, qxyy = 0, qxzz = 0, qyyz = 0,
+           qyzz = 0, qxyz = 0, lq0 = 0, lq1 = 0, lq2 = 0;
+    const double gT = gs.lead_g * T;
+#pragma unroll
+    for (int i = 0; i < Q39; ++i) {
+      const double feq = expo39<kShifted>(lm, rho, i);
+      const double f = (double)sf[i][t] + rf.f[i], g = (double)sg[i][t] + rf.g[i];
+      const double d = feq - f;
+      sd[i][t] = d;
+      const double dgi = gT * feq - g;
+      const double a = cx39<kShifted>(i) - u[0], bb = CY39[i] - u[1], e = CZ39[i] - u[2];
+      qxxx += a * a * a * d;
+      qyyy += bb * bb * bb * d;
+      qzzz += e * e * e * d;
+      qxxy += a * a * bb * d;
+      qxxz += a * a * e * d;
+      qxyy += bb * bb * a * d;
+      qxzz += e * e * a * d;
+      qyyz += bb * bb * e * d;
+      qyzz += e * e * bb * d;
+      qxyz += a * bb * e * d;
+      lq0 += a * dgi;
+      lq1 += bb * dgi;
+      lq2 += e * dgi;
+    }
+    const double eps =
+        step < gs.startup ? (((double)step + 1.0) / (double)gs.startup) * gs.eps : gs.eps;
+    const double qs = eps / (6.0 * T * T * T), ql = eps / T;
+    // contraction terms of qbar: sum_abc Q_abc c_a c_b c_c and sum_c (Q_aac) c_c
+    const double sx = qxxx + qxyy + qxzz, sy = qxxy + qyyy + qyzz, sz = qxxz + qyyz + qzzz;
+    double dmu, dte;
+    const double mu = sutherland39(T, gs, &dmu, &dte);
+    double kappa = 0.0;
+    kerr = !therm_cond39(rho, T, gs, &kappa);
+    double b1, b2;
+    if (step < gs.startup) {
+      b1 = gs.init_beta1 + gs.beta1_scale * (double)step;
+      b2 = gs.init_beta2 + gs.beta2_scale * (double)step;
+    } else {
+      b1 = 1.0 / ((ldexp(1.0, gs.lvl + 1) * mu) / (rho * T) + 1.0);
+      b2 = 1.0 / ((2.0 * kappa) / (rho * gs.cp * T) + 1.0);
+    }
+    const double ab1 = al * b1;
+    bool neg = false;
+#pragma unroll
+    for (int i = 0; i < Q39; ++i) {
+      const double cx = cx39<kShifted>(i), cy = CY39[i], cz = CZ39[i];
+      const double cub = qxxx * cx * cx * cx + qyyy * cy * cy * cy + qzzz * cz * cz * cz +
+                         3.0 * (qxxy * cx * cx * cy + qxxz * cx * cx * cz + qxyy * cx * cy * cy +
+                                qxzz * cx * cz * cz + qyyz * cy * cy * cz + qyzz * cy * cz * cz) +
+                         6.0 * qxyz * cx * cy * cz;
+      const double qbar = qs * (cub - 3 * T * (sx * cx + sy * cy + sz * cz));
+      const double hq = ql * (cx * lq0 + cy * lq1 + cz * lq2);
+      const double f = (double)sf[i][t] + rf.f[i], g = (double)sg[i][t] + rf.g[i];
+      const double d = sd[i][t];
+      const double dgi = gT * (d + f) - g;
+      const double fo = f + ab1 * (d - qbar) + b2 * qbar;
+      const double go = g + ab1 * (dgi - hq) + b2 * hq;
+      neg |= fo < 0.0;
+      fB[(size_t)i * n + node] = (S)(fo - rf.f[i]);
+      gB[(size_t)i * n + node] = (S)(go - rf.g[i]);
+    }
+    if (neg) {
+      const unsigned slot = atomicAdd(neg_count, 1u);
+      neg_list[slot] = node;
+    }
+  }
+  wave_stats39(stats, active, iters, noconv, ast, kerr);
+}
+
+// damp_it.f95:64-92 for the listed nodes: each negative post-collision
+// population takes the mean of its positive neighbours along +-c_i (or the
+// mean of one with |f|, or |f|), all read before any repair is written;
+// the new values go to rep[j][q] (NaN where nothing changes).
+template <bool kFp32, bool kShifted>
+__global__ __launch_bounds__(256) void k39_damp_compute(const void* __restrict__ f_, Box b,
+                                                        Ref39 rf, const unsigned* __restrict__ count,
+                                                        const unsigned* __restrict__ list,
+                                                        double* __restrict__ rep,
+                                                        unsigned long long* __restrict__ stats) {
+  using S = typename std::conditional<kFp32, float, double>::type;
+  const S* __restrict__ f = static_cast<const S*>(f_);
+  const unsigned cnt = *count;
+  const unsigned n = (unsigned)b.n;
+  unsigned long long nrep = 0;
+  for (unsigned j = blockIdx.x * blockDim.x + threadIdx.x; j < cnt; j += gridDim.x * blockDim.x) {
+    const unsigned node = list[j];
+    const int x = (int)(node % (unsigned)b.nx);
+    const int y = (int)((node / (unsigned)b.nx) % (unsigned)b.ny);
+    const int z = (int)(node / ((unsigned)b.nx * (unsigned)b.ny));
+    for (int i = 0; i < Q39; ++i) {
+      const double v = (double)f[(size_t)i * n + node] + rf.f[i];
+      double r = __builtin_nan("");
+      if (v < 0.0) {
+        const int cx = cx39<kShifted>(i), cy = CY39[i], cz = CZ39[i];
+        const unsigned kp = (unsigned)wrapm(x + cx, b.nx) +
+                            (unsigned)b.nx * ((unsigned)wrapm(y + cy, b.ny) +
+                                              (unsigned)b.ny * (unsigned)wrapm(z + cz, b.nz));
+        const unsigned km = (unsigned)wrapm(x - cx, b.nx) +
+                            (unsigned)b.nx * ((unsigned)wrapm(y - cy, b.ny) +
+                                              (unsigned)b.ny * (unsigned)wrapm(z - cz, b.nz));
+        const double fp = (double)f[(size_t)i * n + kp] + rf.f[i];
+        const double fm = (double)f[(size_t)i * n + km] + rf.f[i];
+        if (fp > 0.0 && fm > 0.0) r = (fp + fm) / 2.0;
+        else if (fp > 0.0 && fm < 0.0) r = (fp + fabs(v)) / 2;
+        else if (fp < 0.0 && fm > 0.0) r = (fm + fabs(v)) / 2;
+        else r = fabs(v);
+        ++nrep;
+      }
+      rep[(size_t)j * Q39 + i] = r;
+    }
+  }
+  if (nrep) atomicAdd(stats + (size_t)(blockIdx.x & (kStatSlots - 1)) * kStatFields + ST_REPAIR, nrep);
+}
+
+template <bool kFp32>
+__global__ __launch_bounds__(256) void k39_damp_apply(void* __restrict__ f_, Box b, Ref39 rf,
+                                                      unsigned* __restrict__ count,
+                                                      const unsigned* __restrict__ list,
+                                                      const double* __restrict__ rep) {
+  using S = typename std::conditional<kFp32, float, double>::type;
+  S* __restrict__ f = static_cast<S*>(f_);
+  const unsigned cnt = *count;
+  const unsigned n = (unsigned)b.n;
+  for (unsigned j = blockIdx.x * blockDim.x + threadIdx.x; j < cnt; j += gridDim.x * blockDim.x) {
+    const unsigned node = list[j];
+    for (int i = 0; i < Q39; ++i) {
+      const double r = rep[(size_t)j * Q39 + i];
+      if (r == r) f[(size_t)i * n + node] = (S)(r - rf.f[i]);
+    }
+  }
+}
+
+__global__ void k39_clear(unsigned* c) { *c = 0u; }
+
+// Equilibrium state of given rho, u, T per node: Newton from the Maxwellian's
+// multipliers, f = f_eq, g = (2 cv - 3) T f_eq (init_fi.f95:41 sets the
+// populations to the equilibrium of the initial state the same way).
+template <bool kFp32, bool kShifted>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void k39_init(
+    const double* __restrict__ fields, void* __restrict__ f_, void* __restrict__ g_,
+    double* __restrict__ lmv, Box b, Gas39 gs, Ref39 rf, unsigned long long* __restrict__ bad) {
+  using S = typename std::conditional<kFp32, float, double>::type;
+  S* __restrict__ fo = static_cast<S*>(f_);
+  S* __restrict__ go = static_cast<S*>(g_);
+  const unsigned n = (unsigned)b.n;
+  const unsigned node = blockIdx.x * 64u + threadIdx.x;
+  if (node >= n) return;
+  const double rho = fields[node], u0 = fields[(size_t)n + node], u1 = fields[2 * (size_t)n + node],
+               u2 = fields[3 * (size_t)n + node], T = fields[4 * (size_t)n + node];
+  const double vm = u0 * u0 + u1 * u1 + u2 * u2, qf = rho * vm + 5 * rho * T;
+  const double tg[kM39] = {rho, rho * u0, rho * u1, rho * u2, rho * (T + u0 * u0),
+                           rho * (T + u1 * u1), rho * (T + u2 * u2), rho * u0 * u1, rho * u0 * u2,
+                           rho * u1 * u2, qf * u0, qf * u1, qf * u2};
+  double lm[kM39] = {-1.5 * log(2 * 3.141592653589793 * T) - vm / (2 * T), u0 / T, u1 / T, u2 / T,
+                     -1 / (2 * T), -1 / (2 * T), -1 / (2 * T), 0, 0, 0, 0, 0, 0};
+  double lmp[kM39];
+  const int it = lm_newton39<kShifted>(rho, tg, lm, lmp, 200);
+  if (it < 0) atomicAdd(bad, 1ull);
+#pragma unroll
+  for (int k = 0; k < kM39; ++k) lmv[(size_t)k * n + node] = lm[k];
+#pragma unroll
+  for (int i = 0; i < Q39; ++i) {
+    const double e = expo39<kShifted>(lm, rho, i);
+    fo[(size_t)i * n + node] = (S)(e - rf.f[i]);
+    go[(size_t)i * n + node] = (S)(gs.lead_g * T * e - rf.g[i]);
+  }
+}
+
+// host layout (FP64 full values) <-> device storage
+template <bool kFp32>
+__global__ void k39_from_fp64(const double* __restrict__ src, void* __restrict__ dst_,
+                              const double* __restrict__ ref, size_t n) {
+  using S = typename std::conditional<kFp32, float, double>::type;
+  S* dst = static_cast<S*>(dst_);
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n * Q39;
+       k += (size_t)gridDim.x * blockDim.x)
+    dst[k] = (S)(src[k] - ref[k / n]);
+}
+
+template <bool kFp32>
+__global__ void k39_to_fp64(const void* __restrict__ src_, double* __restrict__ dst,
+                            const double* __restrict__ ref, size_t n) {
+  using S = typename std::conditional<kFp32, float, double>::type;
+  const S* src = static_cast<const S*>(src_);
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n * Q39;
+       k += (size_t)gridDim.x * blockDim.x)
+    dst[k] = (double)src[k] + ref[k / n];
+}
+
+}  // namespace
+
+hipError_t launch_d3q39_step(const Box& b, const Step39& a, hipStream_t s) {
+  const unsigned blocks = (unsigned)((b.n + 63) / 64);
+  hipLaunchKernelGGL(k39_clear, dim3(1), dim3(1), 0, s, a.neg_count);
+#define UCLBS_K39(F, SH, TU, KA)                                                                  \
+  hipLaunchKernelGGL((k39_step<F, SH, TU, KA>), dim3(blocks), dim3(64), 0, s, a.fin, a.gin,     \
+                     a.fout, a.gout, a.lm, a.alpha, b, a.gas, a.ref, (long long)a.step, a.stats, \
+                     a.neg_count, a.neg_list)
+#define UCLBS_K39_KA(F, SH, TU) \
+  do {                          \
+    if (a.alpha) UCLBS_K39(F, SH, TU, true); else UCLBS_K39(F, SH, TU, false); \
+  } while (0)
+#define UCLBS_K39_TU(F, SH) \
+  do {                      \
+    if (a.gas.turbulent) UCLBS_K39_KA(F, SH, true); else UCLBS_K39_KA(F, SH, false); \
+  } while (0)
+  if (a.fp32) {
+    if (a.shifted) UCLBS_K39_TU(true, true); else UCLBS_K39_TU(true, false);
+  } else {
+    if (a.shifted) UCLBS_K39_TU(false, true); else UCLBS_K39_TU(false, false);
+  }
+#undef UCLBS_K39_TU
+#undef UCLBS_K39_KA
+#undef UCLBS_K39
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  // repair pass over the nodes that produced a negative population; the
+  // grids read the count on the device, so no host synchronisation is needed
+  const dim3 rg(256), rb(256);
+  if (a.fp32) {
+    if (a.shifted)
+      hipLaunchKernelGGL((k39_damp_compute<true, true>), rg, rb, 0, s, a.fout, b, a.ref,
+                         a.neg_count, a.neg_list, a.rep, a.stats);
+    else
+      hipLaunchKernelGGL((k39_damp_compute<true, false>), rg, rb, 0, s, a.fout, b, a.ref,
+                         a.neg_count, a.neg_list, a.rep, a.stats);
+    hipLaunchKernelGGL((k39_damp_apply<true>), rg, rb, 0, s, a.fout, b, a.ref, a.neg_count,
+                       a.neg_list, a.rep);
+  } else {
+    if (a.shifted)
+      hipLaunchKernelGGL((k39_damp_compute<false, true>), rg, rb, 0, s, a.fout, b, a.ref,
+                         a.neg_count, a.neg_list, a.rep, a.stats);
+    else
+      hipLaunchKernelGGL((k39_damp_compute<false, false>), rg, rb, 0, s, a.fout, b, a.ref,
+                         a.neg_count, a.neg_list, a.rep, a.stats);
+    hipLaunchKernelGGL((k39_damp_apply<false>), rg, rb, 0, s, a.fout, b, a.ref, a.neg_count,
+                       a.neg_list, a.rep);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_d3q39_init(const Box& b, const double* fields, void* f, void* g, double* lm,
+                             int fp32, int shifted, const Gas39& gs, const Ref39& rf,
+                             unsigned long long* bad, hipStream_t s) {
+  const unsigned blocks = (unsigned)((b.n + 63) / 64);
+  if (fp32) {
+    if (shifted)
+      hipLaunchKernelGGL((k39_init<true, true>), dim3(blocks), dim3(64), 0, s, fields, f, g, lm, b,
+                         gs, rf, bad);
+    else
+      hipLaunchKernelGGL((k39_init<true, false>), dim3(blocks), dim3(64), 0, s, fields, f, g, lm,
+                         b, gs, rf, bad);
+  } else {
+    if (shifted)
+      hipLaunchKernelGGL((k39_init<false, true>), dim3(blocks), dim3(64), 0, s, fields, f, g, lm,
+                         b, gs, rf, bad);
+    else
+      hipLaunchKernelGGL((k39_init<false, false>), dim3(blocks), dim3(64), 0, s, fields, f, g, lm,
+                         b, gs, rf, bad);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_d3q39_from_fp64(const Box& b, const double* src, void* dst, int fp32,
+                                  const double* ref, hipStream_t s) {
+  const size_t tot = (size_t)b.n * Q39;
+  const unsigned blocks = (unsigned)std::min<size_t>((tot + 255) / 256, 65536);
+  if (fp32) hipLaunchKernelGGL((k39_from_fp64<true>), dim3(blocks), dim3(256), 0, s, src, dst, ref, (size_t)b.n);
+  else hipLaunchKernelGGL((k39_from_fp64<false>), dim3(blocks), dim3(256), 0, s, src, dst, ref, (size_t)b.n);
+  return hipGetLastError();
+}
+
+hipError_t launch_d3q39_to_fp64(const Box& b, const void* src, double* dst, int fp32,
+                                const double* ref, hipStream_t s) {
+  const size_t tot = (size_t)b.n * Q39;
+  const unsigned blocks = (unsigned)std::min<size_t>((tot + 255) / 256, 65536);
+  if (fp32) hipLaunchKernelGGL((k39_to_fp64<true>), dim3(blocks), dim3(256), 0, s, src, dst, ref, (size_t)b.n);
+  else hipLaunchKernelGGL((k39_to_fp64<false>), dim3(blocks), dim3(256), 0, s, src, dst, ref, (size_t)b.n);
+  return hipGetLastError();
+}
+
+}  // namespace uclbs

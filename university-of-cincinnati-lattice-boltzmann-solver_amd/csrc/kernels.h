@@ -5,6 +5,7 @@
 #include <cstdint>
 
 #include "alpha_solve.h"
+#include "d3q39.h"
 
 namespace uclbs {
 
@@ -97,5 +98,36 @@ hipError_t launch_d3q27_macro(const Box& b, const void* f, int shifted, double* 
                               double* ux, double* uy, double* uz, hipStream_t s);
 hipError_t launch_d3q27_init_tgv(const Box& b, void* f, int shifted, double u0,
                                  double noise_amp, uint64_t seed, hipStream_t s);
+
+// ---- D3Q39 ------------------------------------------------------------------
+// reference-state populations subtracted from the FP32 storage (zero for FP64)
+struct Ref39 {
+  double f[39];
+  double g[39];
+};
+struct Step39 {
+  const void* fin;
+  const void* gin;
+  void* fout;
+  void* gout;
+  double* lm;      // [13][n] multipliers, updated in place
+  double* alpha;   // [n] or null
+  Gas39 gas;
+  Ref39 ref;
+  long long step;  // reference timestep(amr_max_lvl) seen by the start-up ramps
+  int fp32, shifted;
+  unsigned long long* stats;
+  unsigned* neg_count;  // nodes with a negative post-collision population
+  unsigned* neg_list;
+  double* rep;          // [n][39] repair values
+};
+hipError_t launch_d3q39_step(const Box& b, const Step39& a, hipStream_t s);
+hipError_t launch_d3q39_init(const Box& b, const double* fields, void* f, void* g, double* lm,
+                             int fp32, int shifted, const Gas39& gs, const Ref39& rf,
+                             unsigned long long* bad, hipStream_t s);
+hipError_t launch_d3q39_from_fp64(const Box& b, const double* src, void* dst, int fp32,
+                                  const double* ref, hipStream_t s);
+hipError_t launch_d3q39_to_fp64(const Box& b, const void* src, double* dst, int fp32,
+                                const double* ref, hipStream_t s);
 
 }  // namespace uclbs

@@ -52,6 +52,13 @@ struct Level {
   uint32_t* blist = nullptr;  // boundary fluid nodes, ascending
   int64_t nb = 0, nsolid = 0;
   bool has_links = false;
+  // D3Q39: second population, multipliers, repair list, step counter
+  void* g[2] = {nullptr, nullptr};
+  double* lm = nullptr;
+  unsigned* neg_count = nullptr;
+  unsigned* neg_list = nullptr;
+  double* rep = nullptr;
+  int64_t step = 0;
 };
 
 struct Ctx {
@@ -69,6 +76,13 @@ struct Ctx {
   double* d_feq_fs = nullptr;
   bool keep_alpha = false;
   int64_t steps_done = 0;
+  // D3Q39 gas constants and the reference-state populations of the FP32 storage
+  bool gas_set = false;
+  int shifted39 = 0;
+  uclbs::Gas39 gas39{};
+  uclbs::Ref39 ref39{};
+  double* d_ref39 = nullptr;  // f_ref[39] then g_ref[39]
+  int64_t err_seen = 0;       // D3Q39 error-path counts already reported
 };
 
 size_t elem_size(const Ctx* c) {
@@ -99,6 +113,13 @@ void free_level(Level& L) {
   if (L.state) { (void)hipFree(L.state); L.state = nullptr; }
   if (L.flags) { (void)hipFree(L.flags); L.flags = nullptr; }
   if (L.blist) { (void)hipFree(L.blist); L.blist = nullptr; }
+  for (auto& p : L.g)
+    if (p) { (void)hipFree(p); p = nullptr; }
+  if (L.lm) { (void)hipFree(L.lm); L.lm = nullptr; }
+  if (L.neg_count) { (void)hipFree(L.neg_count); L.neg_count = nullptr; }
+  if (L.neg_list) { (void)hipFree(L.neg_list); L.neg_list = nullptr; }
+  if (L.rep) { (void)hipFree(L.rep); L.rep = nullptr; }
+  L.step = 0;
   L.has_links = false;
   L.nb = L.nsolid = 0;
   L.defined = false;
@@ -201,7 +222,36 @@ int check_face_tables(const Ctx* c, const Level* L) {
   return UCLBS_OK;
 }
 
+int launch_steps39(Ctx* c, Level* L, int nsteps, hipEvent_t* ev) {
+  uclbs::Step39 a{};
+  a.gas = c->gas39;
+  a.ref = c->ref39;
+  a.fp32 = c->cfg.precision == UCLBS_PREC_FP32_SHIFTED;
+  a.shifted = c->shifted39;
+  a.stats = c->d_stats;
+  a.neg_count = L->neg_count;
+  a.neg_list = L->neg_list;
+  a.rep = L->rep;
+  a.lm = L->lm;
+  a.alpha = c->keep_alpha ? L->alpha : nullptr;
+  for (int s = 0; s < nsteps; ++s) {
+    a.fin = L->f[L->cur];
+    a.gin = L->g[L->cur];
+    a.fout = L->f[1 - L->cur];
+    a.gout = L->g[1 - L->cur];
+    a.step = (long long)L->step;
+    if (ev) HIP_TRY(hipEventRecord(ev[2 * s], c->stream));
+    HIP_TRY(uclbs::launch_d3q39_step(L->box, a, c->stream));
+    if (ev) HIP_TRY(hipEventRecord(ev[2 * s + 1], c->stream));
+    L->cur = 1 - L->cur;
+    L->step++;
+    c->steps_done++;
+  }
+  return UCLBS_OK;
+}
+
 int launch_steps(Ctx* c, Level* L, int nsteps, hipEvent_t* ev = nullptr) {
+  if (c->cfg.lattice == UCLBS_LATTICE_D3Q39) return launch_steps39(c, L, nsteps, ev);
   uclbs::StepArgs a = step_args(c);
   if (L->has_links) {
     int rc = check_face_tables(c, L);
@@ -298,6 +348,7 @@ int uclbs_destroy(void* ctx) {
   if (c->d_feq_in) (void)hipFree(c->d_feq_in);
   if (c->d_feq_out) (void)hipFree(c->d_feq_out);
   if (c->d_feq_fs) (void)hipFree(c->d_feq_fs);
+  if (c->d_ref39) (void)hipFree(c->d_ref39);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->side) { (void)hipStreamSynchronize(c->side); (void)hipStreamDestroy(c->side); }
@@ -332,6 +383,13 @@ int uclbs_define_level(void* ctx, int lvl, const int32_t lo[3], const int32_t hi
   L.box.ny = n[1];
   L.box.nz = n[2];
   L.box.n = (int64_t)n[0] * n[1] * n[2];
+  if (c->cfg.lattice == UCLBS_LATTICE_D3Q39) {
+    if (!c->gas_set) return set_err(UCLBS_E_STATE, "D3Q39: call uclbs_set_gas before defining levels");
+    if (!(L.box.per[0] && L.box.per[1] && L.box.per[2]))
+      return set_err(UCLBS_E_UNSUPPORTED, "D3Q39 levels are periodic boxes in this build");
+    for (int d = 0; d < 3; ++d)
+      if (n[d] < 5) return set_err(UCLBS_E_ARG, "D3Q39 box extent %d on axis %d is below 5", n[d], d);
+  }
   const size_t bytes = (size_t)L.box.n * q_of(c) * elem_size(c);
   for (auto& p : L.f) {
     hipError_t e = hipMalloc(&p, bytes);
@@ -345,6 +403,15 @@ int uclbs_define_level(void* ctx, int lvl, const int32_t lo[3], const int32_t hi
   HIP_TRY(hipMemset(L.defer_count, 0, sizeof(unsigned)));
   HIP_TRY(hipMalloc(&L.defer_list, (size_t)L.box.n * sizeof(unsigned)));
   if (L.box.n > (1ll << 30)) return set_err(UCLBS_E_ARG, "box exceeds 2^30 nodes");
+  if (c->cfg.lattice == UCLBS_LATTICE_D3Q39) {
+    for (auto& p : L.g) HIP_TRY(hipMalloc(&p, bytes));
+    HIP_TRY(hipMalloc(&L.lm, (size_t)L.box.n * uclbs::kM39 * sizeof(double)));
+    HIP_TRY(hipMalloc(&L.neg_count, sizeof(unsigned)));
+    HIP_TRY(hipMemset(L.neg_count, 0, sizeof(unsigned)));
+    HIP_TRY(hipMalloc(&L.neg_list, (size_t)L.box.n * sizeof(unsigned)));
+    HIP_TRY(hipMalloc(&L.rep, (size_t)L.box.n * uclbs::Q39 * sizeof(double)));
+  }
+  L.step = 0;
   L.cur = 0;
   L.defined = true;
   return UCLBS_OK;
@@ -355,6 +422,9 @@ int uclbs_upload_f(void* ctx, int lvl, const double* f, int64_t n_nodes) {
   Level* L;
   int rc = check_level(c, lvl, &L);
   if (rc) return rc;
+  if (c->cfg.lattice != UCLBS_LATTICE_D3Q27)
+    return set_err(UCLBS_E_UNSUPPORTED, "D3Q39 state moves through uclbs_upload_state / "
+                                        "uclbs_download_state");
   if (!f || n_nodes != L->box.n)
     return set_err(UCLBS_E_ARG, "upload: expected %lld nodes, got %lld", (long long)L->box.n,
                    (long long)n_nodes);
@@ -380,6 +450,9 @@ int uclbs_download_f(void* ctx, int lvl, double* f, int64_t n_nodes) {
   Level* L;
   int rc = check_level(c, lvl, &L);
   if (rc) return rc;
+  if (c->cfg.lattice != UCLBS_LATTICE_D3Q27)
+    return set_err(UCLBS_E_UNSUPPORTED, "D3Q39 state moves through uclbs_upload_state / "
+                                        "uclbs_download_state");
   if (!f || n_nodes != L->box.n)
     return set_err(UCLBS_E_ARG, "download: expected %lld nodes, got %lld", (long long)L->box.n,
                    (long long)n_nodes);
@@ -402,6 +475,9 @@ int uclbs_download_macro(void* ctx, int lvl, double* rho, double* u, double* v, 
   Level* L;
   int rc = check_level(c, lvl, &L);
   if (rc) return rc;
+  if (c->cfg.lattice != UCLBS_LATTICE_D3Q27)
+    return set_err(UCLBS_E_UNSUPPORTED, "D3Q39 state moves through uclbs_upload_state / "
+                                        "uclbs_download_state");
   if (!rho || !u || !v || !w) return set_err(UCLBS_E_ARG, "null output array");
   if ((rc = set_device(c))) return rc;
   const size_t bytes = (size_t)L->box.n * sizeof(double);
@@ -640,7 +716,23 @@ int uclbs_step_level_async(void* ctx, int lvl, int nsteps) {
 int uclbs_step_level(void* ctx, int lvl, int nsteps) {
   int rc = uclbs_step_level_async(ctx, lvl, nsteps);
   if (rc) return rc;
-  return uclbs_sync(ctx);
+  if ((rc = uclbs_sync(ctx))) return rc;
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (c->cfg.lattice == UCLBS_LATTICE_D3Q39) {
+    // the reference stops in error_out on these (calculate_alpha.f95:303-313,
+    // quick_calcs.f95:149-153, 207-212)
+    uclbs_stats st;
+    if ((rc = uclbs_get_stats(ctx, &st))) return rc;
+    const int64_t errs = st.n_nonpos + st.n_fallback;
+    if (errs > c->err_seen) {
+      c->err_seen = errs;
+      return set_err(UCLBS_E_NUMERIC,
+                     "D3Q39 step: %lld nodes without a real alpha root, %lld outside the "
+                     "conductivity fit",
+                     (long long)st.n_nonpos, (long long)st.n_fallback);
+    }
+  }
+  return UCLBS_OK;
 }
 
 int uclbs_time_steps(void* ctx, int lvl, int nsteps, double* elapsed_ms) {
@@ -737,6 +829,203 @@ int uclbs_reset_stats(void* ctx) {
                          sizeof(unsigned long long) * uclbs::kStatSlots * uclbs::kStatFields,
                          c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  c->err_seen = 0;
+  return UCLBS_OK;
+}
+
+// ---- D3Q39 ----------------------------------------------------------------
+
+int uclbs_set_gas(void* ctx, const uclbs_gas* gas) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || !gas) return set_err(UCLBS_E_ARG, "null argument");
+  if (c->cfg.lattice != UCLBS_LATTICE_D3Q39)
+    return set_err(UCLBS_E_STATE, "uclbs_set_gas needs a D3Q39 context");
+  if (!(gas->gama > 1.0) || !(gas->temperature > 0.0) || !(gas->pressure > 0.0) ||
+      !(gas->mach >= 0.0) || !(gas->dx > 0.0) || gas->startup_timesteps <= 0)
+    return set_err(UCLBS_E_ARG, "bad gas parameters");
+  for (auto& L : c->levels)
+    if (L.defined) return set_err(UCLBS_E_STATE, "uclbs_set_gas after levels were defined");
+  uclbs::Gas39& g = c->gas39;
+  g = uclbs::Gas39{};
+  // basic_fluids_calcs.f95:17-27, 31-38, 86
+  const double R = 287.052874;
+  g.gama = gas->gama;
+  g.R = R;
+  g.rho_ref = gas->pressure / (R * gas->temperature);
+  g.t_ref = gas->temperature;
+  g.p_ref = gas->pressure;
+  g.cv = 1.0 / (gas->gama - 1.0);
+  g.cp = gas->gama / (gas->gama - 1.0);
+  g.lead_g = 2.0 * g.cv - 3.0;
+  const double temperature = 1.0 / gas->gama, rho_inf = 1.0;
+  const double sos = std::sqrt(gas->gama * R * g.t_ref);
+  g.molec_weight = (double)28.97f / (6.022e23 * 1000);
+  // viscosity_and_timing.f95:29-68
+  const double S = (double)110.4f, visc_ref = (double)1.716E-5f, temp_ref = (double)273.15f;
+  const double viscosity =
+      visc_ref * std::pow(g.t_ref / temp_ref, 1.5) * ((temp_ref + S) / (g.t_ref + S));
+  g.mu_ref = g.rho_ref * sos * gas->mach * 1;
+  g.kappa_ref = 1.0;  // kappa_bottom = 1 for the start-up conductivity
+  double kappa_base = 0.0;
+  if (!uclbs::therm_cond39(rho_inf, temperature, g, &kappa_base))
+    return set_err(UCLBS_E_ARG, "freestream outside the conductivity fit");
+  g.kappa_ref = rho_inf * sos * sos * sos / temperature;
+  g.init_beta1 = 0.5001;
+  g.init_beta2 = 0.5001;
+  const double b1 = 1.0 / ((2.0 * viscosity) / (rho_inf * temperature) + 1.0);
+  const double b2 = 1.0 / ((2.0 * kappa_base / g.kappa_ref) / (rho_inf * g.cp * temperature) + 1.0);
+  g.startup = gas->startup_timesteps;
+  g.beta1_scale = (b1 - g.init_beta1) / (double)g.startup;
+  g.beta2_scale = (b2 - g.init_beta2) / (double)g.startup;
+  // timing_calcs.f95:24-26 (level 0)
+  g.lvl = 0;
+  g.dt = gas->dx / sos;
+  g.eps = g.dt / 1.0;
+  g.turbulent = gas->turbulent != 0;
+  g.maxit = gas->lm_maxit > 0 ? gas->lm_maxit : 50;
+  c->shifted39 = gas->shifted < 0 ? (gas->mach >= 1.2 ? 1 : 0) : (gas->shifted != 0);
+  // FP32 storage keeps deviations from the freestream equilibrium
+  c->ref39 = uclbs::Ref39{};
+  if (c->cfg.precision == UCLBS_PREC_FP32_SHIFTED) {
+    const double u0 = gas->mach, T = temperature, rho = 1.0;
+    const double qf = rho * u0 * u0 + 5 * rho * T;
+    const double tg[uclbs::kM39] = {rho, rho * u0, 0, 0, rho * (T + u0 * u0), rho * T, rho * T,
+                                    0, 0, 0, qf * u0, 0, 0};
+    double lm[uclbs::kM39] = {-1.5 * std::log(2 * 3.141592653589793 * T) - u0 * u0 / (2 * T),
+                              u0 / T, 0, 0, -1 / (2 * T), -1 / (2 * T), -1 / (2 * T), 0, 0, 0, 0, 0, 0};
+    double lmp[uclbs::kM39];
+    const int it = c->shifted39 ? uclbs::lm_newton39<true>(rho, tg, lm, lmp, 200)
+                                : uclbs::lm_newton39<false>(rho, tg, lm, lmp, 200);
+    if (it < 0) return set_err(UCLBS_E_NUMERIC, "freestream multipliers did not converge");
+    for (int i = 0; i < uclbs::Q39; ++i) {
+      const double e = c->shifted39 ? uclbs::expo39<true>(lm, rho, i) : uclbs::expo39<false>(lm, rho, i);
+      c->ref39.f[i] = e;
+      c->ref39.g[i] = g.lead_g * T * e;
+    }
+  }
+  int rc = set_device(c);
+  if (rc) return rc;
+  if (!c->d_ref39) HIP_TRY(hipMalloc(&c->d_ref39, 2 * uclbs::Q39 * sizeof(double)));
+  HIP_TRY(hipMemcpy(c->d_ref39, &c->ref39, 2 * uclbs::Q39 * sizeof(double), hipMemcpyHostToDevice));
+  c->gas_set = true;
+  return UCLBS_OK;
+}
+
+static int check39(Ctx* c, int lvl, Level** L) {
+  int rc = check_level(c, lvl, L);
+  if (rc) return rc;
+  if (c->cfg.lattice != UCLBS_LATTICE_D3Q39)
+    return set_err(UCLBS_E_STATE, "needs a D3Q39 context");
+  return set_device(c);
+}
+
+int uclbs_init_fields(void* ctx, int lvl, const double* rho, const double* u, const double* v,
+                      const double* w, const double* temp) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  Level* L;
+  int rc = check39(c, lvl, &L);
+  if (rc) return rc;
+  if (!rho || !u || !v || !w || !temp) return set_err(UCLBS_E_ARG, "null field");
+  const size_t n = (size_t)L->box.n;
+  double* d = nullptr;
+  unsigned long long* bad = nullptr;
+  HIP_TRY(hipMalloc(&d, 5 * n * sizeof(double)));
+  hipError_t e = hipMalloc(&bad, sizeof(unsigned long long));
+  const double* src[5] = {rho, u, v, w, temp};
+  if (e == hipSuccess) e = hipMemsetAsync(bad, 0, sizeof(unsigned long long), c->stream);
+  for (int k = 0; k < 5 && e == hipSuccess; ++k)
+    e = hipMemcpyAsync(d + k * n, src[k], n * sizeof(double), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess)
+    e = uclbs::launch_d3q39_init(L->box, d, L->f[L->cur], L->g[L->cur], L->lm,
+                                 c->cfg.precision == UCLBS_PREC_FP32_SHIFTED, c->shifted39,
+                                 c->gas39, c->ref39, bad, c->stream);
+  const size_t bytes = n * uclbs::Q39 * elem_size(c);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(L->f[1 - L->cur], L->f[L->cur], bytes, hipMemcpyDeviceToDevice, c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(L->g[1 - L->cur], L->g[L->cur], bytes, hipMemcpyDeviceToDevice, c->stream);
+  unsigned long long nbad = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&nbad, bad, sizeof(nbad), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d);
+  if (bad) (void)hipFree(bad);
+  if (e != hipSuccess) return set_err(UCLBS_E_HIP, "init_fields failed: %s", hipGetErrorString(e));
+  if (nbad) return set_err(UCLBS_E_NUMERIC, "multiplier Newton failed at %llu nodes", nbad);
+  return UCLBS_OK;
+}
+
+int uclbs_upload_state(void* ctx, int lvl, const double* f, const double* g, const double* lm,
+                       int64_t n_nodes) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  Level* L;
+  int rc = check39(c, lvl, &L);
+  if (rc) return rc;
+  if (!f || !g || !lm || n_nodes != L->box.n)
+    return set_err(UCLBS_E_ARG, "upload_state: null array or %lld nodes (expected %lld)",
+                   (long long)n_nodes, (long long)L->box.n);
+  const size_t n = (size_t)L->box.n, pb = n * uclbs::Q39 * sizeof(double);
+  const int fp32 = c->cfg.precision == UCLBS_PREC_FP32_SHIFTED;
+  double* tmp = nullptr;
+  HIP_TRY(hipMalloc(&tmp, pb));
+  hipError_t e = hipSuccess;
+  const double* src[2] = {f, g};
+  void* dst[2] = {L->f[L->cur], L->g[L->cur]};
+  void* dst2[2] = {L->f[1 - L->cur], L->g[1 - L->cur]};
+  for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+    e = hipMemcpyAsync(tmp, src[k], pb, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess)
+      e = uclbs::launch_d3q39_from_fp64(L->box, tmp, dst[k], fp32, c->d_ref39 + k * uclbs::Q39,
+                                        c->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(dst2[k], dst[k], n * uclbs::Q39 * elem_size(c), hipMemcpyDeviceToDevice,
+                         c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  }
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(L->lm, lm, n * uclbs::kM39 * sizeof(double), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return set_err(UCLBS_E_HIP, "upload_state failed: %s", hipGetErrorString(e));
+  return UCLBS_OK;
+}
+
+int uclbs_download_state(void* ctx, int lvl, double* f, double* g, double* lm, int64_t n_nodes) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  Level* L;
+  int rc = check39(c, lvl, &L);
+  if (rc) return rc;
+  if (n_nodes != L->box.n)
+    return set_err(UCLBS_E_ARG, "download_state: %lld nodes (expected %lld)", (long long)n_nodes,
+                   (long long)L->box.n);
+  const size_t n = (size_t)L->box.n, pb = n * uclbs::Q39 * sizeof(double);
+  const int fp32 = c->cfg.precision == UCLBS_PREC_FP32_SHIFTED;
+  double* tmp = nullptr;
+  HIP_TRY(hipMalloc(&tmp, pb));
+  hipError_t e = hipSuccess;
+  double* out[2] = {f, g};
+  const void* src[2] = {L->f[L->cur], L->g[L->cur]};
+  for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+    if (!out[k]) continue;
+    e = uclbs::launch_d3q39_to_fp64(L->box, src[k], tmp, fp32, c->d_ref39 + k * uclbs::Q39,
+                                    c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(out[k], tmp, pb, hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  }
+  if (e == hipSuccess && lm)
+    e = hipMemcpyAsync(lm, L->lm, n * uclbs::kM39 * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return set_err(UCLBS_E_HIP, "download_state failed: %s", hipGetErrorString(e));
+  return UCLBS_OK;
+}
+
+int uclbs_set_timestep(void* ctx, int lvl, int64_t step) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  Level* L;
+  int rc = check_level(c, lvl, &L);
+  if (rc) return rc;
+  if (step < 0) return set_err(UCLBS_E_ARG, "negative timestep");
+  L->step = step;
   return UCLBS_OK;
 }
 
