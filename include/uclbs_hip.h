@@ -194,6 +194,39 @@ int uclbs_download_state(void *ctx, int lvl, double *f, double *g, double *lm, i
  * start-up ramps of beta, epsilon and alpha; advanced by every step. */
 int uclbs_set_timestep(void *ctx, int lvl, int64_t step);
 
+/* ---- learned alpha (D3Q27, FP32-shifted storage) --------------------------
+ * No counterpart in the reference (whose closest precedent, the fitted
+ * multiplier guesses of amr_processes.f95:1871-2039, is disabled at
+ * calculate_lagrangian_mults.f95:729-741).  A 12-64-64-64-1 GELU network maps
+ * 12 invariants of a node's non-equilibrium part to y = (alpha - 2) / m,
+ * m = max|D_i / f_i|; with alpha_mode = UCLBS_ALPHA_LEARNED every step first
+ * evaluates it on BF16 MFMA and the Newton root-find of H(f + alpha D) = H(f)
+ * starts from alpha0 = 2 + m (y_mean + y_std y); the plain start alpha = 2
+ * takes over when alpha0 lies outside the bracket or does not converge
+ * (counted in uclbs_stats.n_fallback).  Parameters: W1 (12x64), b1, W2
+ * (64x64), b2, W3 (64x64), b3, W4 (64), b4, FP32, 9,217 values. */
+#define UCLBS_MLP_PARAMS 9217
+/* uniform(+-sqrt(6/fan_in)) weights from Philox4x32-10 with the seed, zero biases */
+int uclbs_mlp_init(void *ctx, uint64_t seed);
+int uclbs_mlp_set_params(void *ctx, const float *params, float y_mean, float y_std);
+int uclbs_mlp_get_params(void *ctx, float *params, float *y_mean, float *y_std);
+/* loss 0.5 mean (net(X) - y)^2 and its gradient for a host batch X (B x 12), y (B) */
+int uclbs_mlp_loss_grad(void *ctx, const float *X, const float *y, int B, float *grad,
+                        double *loss);
+/* network output for a host batch; use_mfma 1: BF16 MFMA path, 0: FP32 path */
+int uclbs_mlp_forward(void *ctx, const float *X, int B, int use_mfma, float *yhat);
+/* device sample buffer for (features, y) training pairs */
+int uclbs_mlp_reserve(void *ctx, int64_t capacity);
+/* logs every stride-th interior node of the current state, advances the level
+ * one plain step (alpha_mode must be UCLBS_ALPHA_NEWTON) and stores the
+ * resulting alpha as the target; *n_samples = pairs held */
+int uclbs_mlp_collect(void *ctx, int lvl, int stride, int64_t *n_samples);
+int uclbs_mlp_download_samples(void *ctx, float *X, float *y, int64_t capacity, int64_t *n);
+/* Adam (lr, beta 0.9 / 0.999) on random batches of the held pairs (targets
+ * normalised to zero mean, unit variance on first use) */
+int uclbs_mlp_train(void *ctx, int64_t steps, int batch, double lr, uint64_t seed,
+                    double *loss_first, double *loss_last);
+
 /* Advance level lvl by nsteps fused stream+moments+alpha+collide steps:
  * the streaming / calculate_flow_variables / collision calls of
  * loop.f95:105,112,133 for one level, in that order. */

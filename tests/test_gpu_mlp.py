@@ -1,0 +1,101 @@
+"""Learned alpha on the GPU through the C ABI.
+  gradient (FP32 kernels) vs FP64 oracle: relative L2 <= 1e-5, loss rel <= 1e-6
+  forward FP32 path vs oracle: max abs <= 1e-5 * (1 + max|y|)
+  forward BF16-MFMA path vs oracle: max abs <= 0.03 max|y| + 0.01 (BF16 inputs
+    and weights, FP32 accumulation)
+  trained network as Newton start on the TGV box: fewer Newton updates than the
+    plain start, fields equal to the plain run within 1e-7 relative L2."""
+import math
+
+import numpy as np
+import pytest
+
+import o_mlp
+
+pytestmark = pytest.mark.gpu
+
+
+def _solver(pkg, **kw):
+    s = pkg.Solver(**kw)
+    s.define_level(0, (16, 16, 16))
+    return s
+
+
+def test_mlp_gradient_matches_oracle(pkg):
+    s = _solver(pkg)
+    s.mlp_init(11)
+    p, _, _ = s.mlp_get_params()
+    rng = np.random.default_rng(2)
+    X = rng.uniform(-1, 1, (300, 12)).astype(np.float32)
+    y = rng.normal(0, 1, 300).astype(np.float32)
+    loss_g, g_g = s.mlp_loss_grad(X, y)
+    loss_o, g_o = o_mlp.loss_and_grad(p.astype(np.float64), X.astype(np.float64), y.astype(np.float64))
+    e = float(np.linalg.norm(g_g - g_o) / np.linalg.norm(g_o))
+    print(f"MLP gradient GPU vs oracle: relL2 {e:.2e}, loss {loss_g:.6e} vs {loss_o:.6e}")
+    assert e < 1e-5 and abs(loss_g - loss_o) <= 1e-6 * loss_o
+
+
+def test_mlp_forward_paths_match_oracle(pkg):
+    s = _solver(pkg)
+    s.mlp_init(12)
+    p, _, _ = s.mlp_get_params()
+    rng = np.random.default_rng(3)
+    X = rng.uniform(-1, 1, (1000, 12)).astype(np.float32)
+    ref, _ = o_mlp.forward(p.astype(np.float64), X.astype(np.float64))
+    y32 = s.mlp_forward(X, mfma=False)
+    y16 = s.mlp_forward(X, mfma=True)
+    e32 = float(np.abs(y32 - ref).max())
+    e16 = float(np.abs(y16 - ref).max())
+    print(f"MLP forward: FP32 max abs {e32:.2e}, BF16-MFMA max abs {e16:.2e}, max|y| {np.abs(ref).max():.3f}")
+    assert e32 <= 1e-5 * (1 + np.abs(ref).max())
+    assert e16 <= 0.03 * np.abs(ref).max() + 0.01
+
+
+def _tgv(pkg, mode, L=48):
+    u0 = 0.1 / math.sqrt(3.0)
+    nu = u0 * (L / (2 * math.pi)) / 1600.0
+    s = pkg.Solver(precision="fp32s", nu=nu, alpha_mode=mode, alpha_tol=1e-10)
+    s.define_level(0, (L, L, L))
+    s.init_tgv(0, u0, 1e-3, 20260101)
+    return s
+
+
+def test_learned_start_reduces_newton_updates(pkg):
+    """Train on pairs logged from the plain solver, then run the same state
+    with the plain start, an untrained network and the trained network."""
+    L = 48
+    plain = _tgv(pkg, "newton", L)
+    plain.step(0, 10)
+    plain.mlp_reserve(4_000_000)
+    for _ in range(8):
+        n = plain.mlp_collect(0, 1)
+    assert n > 200_000
+    params0 = None
+    plain.mlp_init(5)
+    params0 = plain.mlp_get_params()
+    l0, l1 = plain.mlp_train(3000, 4096, 2e-3, 9)
+    print(f"training: {n} pairs, loss {l0:.3e} -> {l1:.3e}")
+    assert l1 < 0.2 * l0
+    trained = plain.mlp_get_params()
+    f_state = plain.download_f(0)
+    results = {}
+    for name, mode, params in (("plain", "newton", None), ("untrained", "learned", params0),
+                               ("trained", "learned", trained)):
+        s = _tgv(pkg, mode, L)
+        if params is not None:
+            s.mlp_set_params(*params)
+        s.upload_f(0, f_state)
+        s.reset_stats()
+        s.step(0, 10)
+        st = s.stats()
+        results[name] = (st["newton_iters"] / st["node_updates"], st["n_fallback"] / st["node_updates"],
+                         s.macro(0))
+    ip, _, mp = results["plain"]
+    it, ft, mt = results["trained"]
+    iu, fu, _ = results["untrained"]
+    e = max(float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(mt[1:], mp[1:]))
+    print(f"Newton updates/node: plain {ip:.3f}, untrained {iu:.3f} (fallback {fu:.3f}), "
+          f"trained {it:.3f} (fallback {ft:.4f}); ratio plain/trained {ip / it:.2f}; "
+          f"u relL2 trained vs plain {e:.2e}")
+    assert it < 0.8 * ip and it < iu
+    assert e < 1e-7

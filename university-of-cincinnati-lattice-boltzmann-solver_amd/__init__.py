@@ -27,8 +27,11 @@ EXPORTS = (
     "uclbs_build_links", "uclbs_set_links", "uclbs_download_links", "uclbs_link_counts",
     "uclbs_init_uniform", "uclbs_geom_icosphere", "uclbs_time_kernels",
     "uclbs_copy_bandwidth", "uclbs_set_gas", "uclbs_init_fields", "uclbs_upload_state",
-    "uclbs_download_state", "uclbs_set_timestep",
+    "uclbs_download_state", "uclbs_set_timestep", "uclbs_mlp_init", "uclbs_mlp_set_params",
+    "uclbs_mlp_get_params", "uclbs_mlp_loss_grad", "uclbs_mlp_forward", "uclbs_mlp_reserve",
+    "uclbs_mlp_collect", "uclbs_mlp_download_samples", "uclbs_mlp_train",
 )
+MLP_PARAMS = 9217
 FACE_PERIODIC = 2147483647
 
 LATTICE = {"d3q27": 27, "d3q39": 39}
@@ -85,6 +88,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
     P, I, I64, D, U64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_uint64
     i3 = ctypes.POINTER(ctypes.c_int32)
     dp = ctypes.POINTER(ctypes.c_double)
+    fp = ctypes.POINTER(ctypes.c_float)
     sig = {
         "uclbs_last_error": (ctypes.c_char_p, []),
         "uclbs_abi_version": (I, []),
@@ -118,6 +122,15 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "uclbs_upload_state": (I, [P, I, dp, dp, dp, I64]),
         "uclbs_download_state": (I, [P, I, dp, dp, dp, I64]),
         "uclbs_set_timestep": (I, [P, I, I64]),
+        "uclbs_mlp_init": (I, [P, U64]),
+        "uclbs_mlp_set_params": (I, [P, fp, ctypes.c_float, ctypes.c_float]),
+        "uclbs_mlp_get_params": (I, [P, fp, fp, fp]),
+        "uclbs_mlp_loss_grad": (I, [P, fp, fp, I, fp, dp]),
+        "uclbs_mlp_forward": (I, [P, fp, I, I, fp]),
+        "uclbs_mlp_reserve": (I, [P, I64]),
+        "uclbs_mlp_collect": (I, [P, I, I, ctypes.POINTER(I64)]),
+        "uclbs_mlp_download_samples": (I, [P, fp, fp, I64, ctypes.POINTER(I64)]),
+        "uclbs_mlp_train": (I, [P, I64, I, D, U64, dp, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -129,6 +142,10 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
 
 def _dptr(a: np.ndarray):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
 
 
 def _iptr(a: np.ndarray):
@@ -241,6 +258,58 @@ class Solver:
         lm = np.empty((13, n))
         self._check(self.lib.uclbs_download_state(self.ctx, lvl, _dptr(f), _dptr(g), _dptr(lm), n))
         return f, g, lm
+
+    # ---- learned alpha ---------------------------------------------------------
+    def mlp_init(self, seed: int):
+        self._check(self.lib.uclbs_mlp_init(self.ctx, seed))
+
+    def mlp_set_params(self, params, y_mean: float, y_std: float):
+        p = np.ascontiguousarray(params, np.float32)
+        self._check(self.lib.uclbs_mlp_set_params(self.ctx, _fptr(p), y_mean, y_std))
+
+    def mlp_get_params(self):
+        p = np.empty(MLP_PARAMS, np.float32)
+        m, s = ctypes.c_float(), ctypes.c_float()
+        self._check(self.lib.uclbs_mlp_get_params(self.ctx, _fptr(p), ctypes.byref(m), ctypes.byref(s)))
+        return p, m.value, s.value
+
+    def mlp_loss_grad(self, X, y):
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        g = np.empty(MLP_PARAMS, np.float32)
+        loss = ctypes.c_double()
+        self._check(self.lib.uclbs_mlp_loss_grad(self.ctx, _fptr(X), _fptr(y), len(y), _fptr(g),
+                                                 ctypes.byref(loss)))
+        return loss.value, g
+
+    def mlp_forward(self, X, mfma: bool):
+        X = np.ascontiguousarray(X, np.float32)
+        out = np.empty(X.shape[0], np.float32)
+        self._check(self.lib.uclbs_mlp_forward(self.ctx, _fptr(X), X.shape[0], int(mfma), _fptr(out)))
+        return out
+
+    def mlp_reserve(self, capacity: int):
+        self._check(self.lib.uclbs_mlp_reserve(self.ctx, int(capacity)))
+
+    def mlp_collect(self, lvl: int, stride: int) -> int:
+        n = ctypes.c_int64()
+        self._check(self.lib.uclbs_mlp_collect(self.ctx, lvl, stride, ctypes.byref(n)))
+        return n.value
+
+    def mlp_samples(self, capacity: int):
+        X = np.empty((capacity, 12), np.float32)
+        y = np.empty(capacity, np.float32)
+        n = ctypes.c_int64()
+        self._check(self.lib.uclbs_mlp_download_samples(self.ctx, _fptr(X), _fptr(y), capacity,
+                                                        ctypes.byref(n)))
+        k = min(n.value, capacity)
+        return X[:k], y[:k]
+
+    def mlp_train(self, steps: int, batch: int, lr: float, seed: int):
+        l0, l1 = ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.uclbs_mlp_train(self.ctx, int(steps), batch, lr, seed,
+                                             ctypes.byref(l0), ctypes.byref(l1)))
+        return l0.value, l1.value
 
     def set_timestep(self, lvl: int, step: int):
         self._check(self.lib.uclbs_set_timestep(self.ctx, lvl, int(step)))

@@ -34,6 +34,7 @@ struct AlphaOut {
   int iters;
   int status;
   bool logpath = false;  // g needed direct logarithms (max|x| beyond the series range)
+  bool fallback = false; // the learned start failed and the plain start (alpha = 2) ran
 };
 
 // below this max|x| the node is at equilibrium and alpha = 2 (the "fi = fieq"

@@ -180,9 +180,23 @@ __device__ __forceinline__ AlphaOut collide27_shifted(float (&f)[27], double bet
       ao = AlphaOut{amax, 0, ALPHA_BOUND, false};
     } else if (!__any(m > ac.thr.m_fast)) {
       // the whole wavefront is within the fused order: Newton on the scalar series
-      const double a0 = ac.mode == 2 ? alpha0 : 2.0;
-      ao = newton_alpha([&](double a, double& g, double& gp) __attribute__((always_inline)) { series_g<kFastOrder>(S, a, g, gp); },
-                        S[2], amax, a0, ac.tol, ac.maxit);
+      auto ev = [&](double a, double& g, double& gp) __attribute__((always_inline)) {
+        series_g<kFastOrder>(S, a, g, gp);
+      };
+      if (ac.mode == 2) {
+        // learned start; the plain start takes over when it is outside the
+        // bracket or does not converge
+        const bool inside = alpha0 > 1.0 && alpha0 < fmin(amax, 1e6);
+        ao = newton_alpha(ev, S[2], amax, inside ? alpha0 : 2.0, ac.tol, ac.maxit);
+        if (!inside || ao.status == ALPHA_NOCONV) {
+          const int spent = inside ? ao.iters : 0;
+          ao = newton_alpha(ev, S[2], amax, 2.0, ac.tol, ac.maxit);
+          ao.iters += spent;
+          ao.fallback = true;
+        }
+      } else {
+        ao = newton_alpha(ev, S[2], amax, 2.0, ac.tol, ac.maxit);
+      }
     } else {
       // some lane needs more terms: the wavefront extends its sums to order
       // kMidOrder from the populations still in registers

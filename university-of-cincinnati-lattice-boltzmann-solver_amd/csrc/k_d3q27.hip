@@ -146,7 +146,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   AlphaCfg ac{a.tol, a.maxit, a.alpha_mode, a.thr};
   double rho, u[3];
   AlphaOut ao;
-  if constexpr (kShifted) ao = collide27_shifted<false>(f, a.beta, ac, 2.0, rho, u);
+  // learned mode: the initial guess left in alpha[] by the guess kernel
+  const double a0 = (kShifted && a.alpha_mode == 2 && active) ? alpha_out[node] : 2.0;
+  if constexpr (kShifted) ao = collide27_shifted<false>(f, a.beta, ac, a0, rho, u);
   else ao = collide27_fp64(f, a.beta, ac, 2.0, rho, u);
 
   if (active) {
@@ -168,7 +170,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     }
   }
   // deferred nodes report their seed iterations here and the rest in the far kernel
-  wave_stats(stats, active, ao.iters, ao.status == ALPHA_DEFER ? -1 : ao.status, false,
+  wave_stats(stats, active, ao.iters, ao.status == ALPHA_DEFER ? -1 : ao.status, ao.fallback,
              ao.logpath);
 }
 

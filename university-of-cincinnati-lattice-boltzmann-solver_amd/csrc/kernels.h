@@ -99,6 +99,40 @@ hipError_t launch_d3q27_macro(const Box& b, const void* f, int shifted, double* 
 hipError_t launch_d3q27_init_tgv(const Box& b, void* f, int shifted, double u0,
                                  double noise_amp, uint64_t seed, hipStream_t s);
 
+// ---- learned alpha -------------------------------------------------------------
+constexpr int kMlpParams = 9217;  // 12-64-64-64-1
+struct MlpSamples {               // device-resident training pairs
+  float* X = nullptr;             // [cap][12]
+  float* y = nullptr;             // [cap] (alpha - 2) / m
+  unsigned* node_of = nullptr;    // node of a pending sample
+  float* m_of = nullptr;          // its m
+  unsigned long long* count = nullptr;
+  int64_t cap = 0;
+};
+struct MlpWork {                  // activations of one training batch
+  float *Z1, *A1, *Z2, *A2, *Z3, *A3, *Z4, *A4, *D1, *D2, *D3, *D4;
+};
+hipError_t launch_mlp_collect(const Box& b, const float* fin, const uint8_t* flags, int stride,
+                              int phase, MlpSamples& s, hipStream_t st);
+hipError_t launch_mlp_targets(const double* alpha, MlpSamples& s, unsigned long long first,
+                              unsigned long long last, hipStream_t st);
+hipError_t launch_mlp_loss_grad(const float* P, const float* X, const float* y,
+                                const unsigned* idx, int B, MlpWork& w, float* g, double* loss,
+                                hipStream_t st);
+hipError_t launch_mlp_forward_fp32(const float* P, const float* X, int B, MlpWork& w, float* yhat,
+                                   hipStream_t st);
+hipError_t launch_mlp_adam(float* P, const float* g, float* m, float* v, int t, float lr,
+                           hipStream_t st);
+hipError_t launch_mlp_gather(const float* X, unsigned long long n, uint64_t seed, uint64_t step,
+                             int B, float* Xb, unsigned* idx, hipStream_t st);
+hipError_t launch_mlp_normalise(float* y, unsigned long long n, double* ms, hipStream_t st);
+hipError_t launch_mlp_pack(const float* P, short* Wbf, hipStream_t st);
+hipError_t launch_mlp_infer(const float* X, int B, const short* Wbf, const float* P, float* yhat,
+                            hipStream_t st);
+hipError_t launch_alpha_guess27(const Box& b, const float* fin, const uint8_t* flags,
+                                const short* Wbf, const float* P, float ymean, float ystd,
+                                double* alpha, hipStream_t st);
+
 // ---- D3Q39 ------------------------------------------------------------------
 // reference-state populations subtracted from the FP32 storage (zero for FP64)
 struct Ref39 {

@@ -8,11 +8,13 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <algorithm>
 #include <vector>
 
 #include "../../include/uclbs_hip.h"
 #include "kernels.h"
 #include "lattice.h"
+#include "philox.h"
 
 namespace {
 
@@ -83,6 +85,23 @@ struct Ctx {
   uclbs::Ref39 ref39{};
   double* d_ref39 = nullptr;  // f_ref[39] then g_ref[39]
   int64_t err_seen = 0;       // D3Q39 error-path counts already reported
+  // learned alpha: FP32 master parameters, Adam moments, bf16 inference copy
+  float* mlp_p = nullptr;
+  float* mlp_g = nullptr;
+  float* mlp_m = nullptr;
+  float* mlp_v = nullptr;
+  short* mlp_wbf = nullptr;
+  float mlp_ymean = 0.0f, mlp_ystd = 1.0f;
+  bool mlp_ready = false;
+  int mlp_t = 0;
+  uclbs::MlpSamples samples;
+  bool samples_normalised = false;
+  uclbs::MlpWork work{};
+  int work_batch = 0;
+  float* work_buf = nullptr;
+  float* batch_x = nullptr;
+  unsigned* batch_idx = nullptr;
+  double* d_scalar = nullptr;  // loss, mean, std
 };
 
 size_t elem_size(const Ctx* c) {
@@ -260,13 +279,25 @@ int launch_steps(Ctx* c, Level* L, int nsteps, hipEvent_t* ev = nullptr) {
   a.defer_count = L->defer_count;
   a.defer_list = L->defer_list;
   for (int f = 0; f < 6; ++f) a.face_code[f] = L->face_code[f];
+  const bool learned = c->cfg.alpha_mode == UCLBS_ALPHA_LEARNED;
+  if (learned && !c->mlp_ready)
+    return set_err(UCLBS_E_STATE, "learned alpha: no trained network (uclbs_mlp_train or "
+                                  "uclbs_mlp_set_params)");
+  if (learned && c->cfg.precision != UCLBS_PREC_FP32_SHIFTED)
+    return set_err(UCLBS_E_UNSUPPORTED, "learned alpha runs on FP32-shifted storage");
   for (int s = 0; s < nsteps; ++s) {
     if (c->cfg.lattice != UCLBS_LATTICE_D3Q27)
       return set_err(UCLBS_E_UNSUPPORTED, "lattice %d not supported by step", c->cfg.lattice);
+    if (learned)
+      HIP_TRY(uclbs::launch_alpha_guess27(L->box, static_cast<const float*>(L->f[L->cur]),
+                                          L->has_links ? L->flags : nullptr, c->mlp_wbf, c->mlp_p,
+                                          c->mlp_ymean, c->mlp_ystd, L->alpha, c->stream));
     if (!L->has_links && !(L->box.per[0] && L->box.per[1] && L->box.per[2]))
       return set_err(UCLBS_E_UNSUPPORTED,
                      "level without link tables must be periodic on all axes");
     double* al = c->keep_alpha ? L->alpha : nullptr;
+    // learned mode: the streaming kernel reads its start from alpha[]
+    const int keep = c->keep_alpha || learned;
     if (L->has_links && L->nb) {
       // boundary-list nodes on the side stream, concurrent with the streaming kernel
       HIP_TRY(hipEventRecord(c->ev_fork, c->stream));
@@ -278,11 +309,11 @@ int launch_steps(Ctx* c, Level* L, int nsteps, hipEvent_t* ev = nullptr) {
     if (ev) HIP_TRY(hipEventRecord(ev[2 * s], c->stream));
     if (L->has_links)
       HIP_TRY(uclbs::launch_d3q27_step_links(L->box, L->f[L->cur], L->f[1 - L->cur], L->alpha,
-                                             c->keep_alpha, c->d_stats, a, bc_args(c, L),
+                                             keep, c->d_stats, a, bc_args(c, L),
                                              c->stream));
     else
       HIP_TRY(uclbs::launch_d3q27_step_periodic(L->box, L->f[L->cur], L->f[1 - L->cur], L->alpha,
-                                                c->keep_alpha, c->d_stats, a, c->stream));
+                                                keep, c->d_stats, a, c->stream));
     if (ev) HIP_TRY(hipEventRecord(ev[2 * s + 1], c->stream));
     if (a.shifted)
       HIP_TRY(uclbs::launch_d3q27_far(L->box, L->f[1 - L->cur], L->alpha, c->keep_alpha,
@@ -349,6 +380,11 @@ int uclbs_destroy(void* ctx) {
   if (c->d_feq_out) (void)hipFree(c->d_feq_out);
   if (c->d_feq_fs) (void)hipFree(c->d_feq_fs);
   if (c->d_ref39) (void)hipFree(c->d_ref39);
+  for (void* p : {(void*)c->mlp_p, (void*)c->mlp_g, (void*)c->mlp_m, (void*)c->mlp_v,
+                  (void*)c->mlp_wbf, (void*)c->samples.X, (void*)c->samples.y,
+                  (void*)c->samples.node_of, (void*)c->samples.m_of, (void*)c->samples.count,
+                  (void*)c->work_buf, (void*)c->batch_x, (void*)c->batch_idx, (void*)c->d_scalar})
+    if (p) (void)hipFree(p);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->side) { (void)hipStreamSynchronize(c->side); (void)hipStreamDestroy(c->side); }
@@ -1027,6 +1063,258 @@ int uclbs_set_timestep(void* ctx, int lvl, int64_t step) {
   if (step < 0) return set_err(UCLBS_E_ARG, "negative timestep");
   L->step = step;
   return UCLBS_OK;
+}
+
+// ---- learned alpha ------------------------------------------------------------
+
+static int mlp_alloc(Ctx* c) {
+  if (c->mlp_p) return UCLBS_OK;
+  const size_t pb = uclbs::kMlpParams * sizeof(float);
+  HIP_TRY(hipMalloc(&c->mlp_p, pb));
+  HIP_TRY(hipMalloc(&c->mlp_g, pb));
+  HIP_TRY(hipMalloc(&c->mlp_m, pb));
+  HIP_TRY(hipMalloc(&c->mlp_v, pb));
+  HIP_TRY(hipMalloc(&c->mlp_wbf, (64 * 16 + 2 * 64 * 64) * sizeof(short)));
+  HIP_TRY(hipMalloc(&c->d_scalar, 4 * sizeof(double)));
+  HIP_TRY(hipMemset(c->mlp_m, 0, pb));
+  HIP_TRY(hipMemset(c->mlp_v, 0, pb));
+  c->mlp_t = 0;
+  return UCLBS_OK;
+}
+
+static int mlp_work(Ctx* c, int B) {
+  if (c->work_batch >= B) return UCLBS_OK;
+  if (c->work_buf) (void)hipFree(c->work_buf);
+  if (c->batch_x) (void)hipFree(c->batch_x);
+  if (c->batch_idx) (void)hipFree(c->batch_idx);
+  c->work_buf = nullptr;
+  c->batch_x = nullptr;
+  c->batch_idx = nullptr;
+  c->work_batch = 0;
+  const size_t per = (size_t)B * 64;
+  HIP_TRY(hipMalloc(&c->work_buf, (9 * per + 3 * (size_t)B) * sizeof(float)));
+  float* q = c->work_buf;
+  uclbs::MlpWork& w = c->work;
+  w.Z1 = q; q += per; w.A1 = q; q += per; w.Z2 = q; q += per; w.A2 = q; q += per;
+  w.Z3 = q; q += per; w.A3 = q; q += per; w.D1 = q; q += per; w.D2 = q; q += per;
+  w.D3 = q; q += per; w.Z4 = q; q += B; w.A4 = q; q += B; w.D4 = q; q += B;
+  HIP_TRY(hipMalloc(&c->batch_x, (size_t)B * 12 * sizeof(float)));
+  HIP_TRY(hipMalloc(&c->batch_idx, (size_t)B * sizeof(unsigned)));
+  c->work_batch = B;
+  return UCLBS_OK;
+}
+
+static int mlp_publish(Ctx* c) {
+  HIP_TRY(uclbs::launch_mlp_pack(c->mlp_p, c->mlp_wbf, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->mlp_ready = true;
+  return UCLBS_OK;
+}
+
+int uclbs_mlp_init(void* ctx, uint64_t seed) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c) return set_err(UCLBS_E_ARG, "null context");
+  int rc = set_device(c);
+  if (rc || (rc = mlp_alloc(c))) return rc;
+  // uniform(-sqrt(6/fan_in), +sqrt(6/fan_in)) weights from Philox4x32-10, zero biases
+  std::vector<float> p(uclbs::kMlpParams, 0.0f);
+  const int fan[4] = {12, 64, 64, 64}, outs[4] = {64, 64, 64, 1};
+  size_t k = 0;
+  for (int l = 0; l < 4; ++l) {
+    const double lim = std::sqrt(6.0 / fan[l]);
+    for (int t = 0; t < fan[l] * outs[l]; ++t, ++k)
+      p[k] = (float)((2.0 * uclbs::philox_uniform(k, 77u, seed) - 1.0) * lim);
+    k += outs[l];
+  }
+  HIP_TRY(hipMemcpy(c->mlp_p, p.data(), p.size() * sizeof(float), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemset(c->mlp_m, 0, p.size() * sizeof(float)));
+  HIP_TRY(hipMemset(c->mlp_v, 0, p.size() * sizeof(float)));
+  c->mlp_t = 0;
+  c->mlp_ymean = 0.0f;
+  c->mlp_ystd = 1.0f;
+  return mlp_publish(c);
+}
+
+int uclbs_mlp_set_params(void* ctx, const float* params, float y_mean, float y_std) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || !params) return set_err(UCLBS_E_ARG, "null argument");
+  if (!(y_std > 0.0f)) return set_err(UCLBS_E_ARG, "y_std must be > 0");
+  int rc = set_device(c);
+  if (rc || (rc = mlp_alloc(c))) return rc;
+  HIP_TRY(hipMemcpy(c->mlp_p, params, uclbs::kMlpParams * sizeof(float), hipMemcpyHostToDevice));
+  c->mlp_ymean = y_mean;
+  c->mlp_ystd = y_std;
+  return mlp_publish(c);
+}
+
+int uclbs_mlp_get_params(void* ctx, float* params, float* y_mean, float* y_std) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || !params || !y_mean || !y_std) return set_err(UCLBS_E_ARG, "null argument");
+  if (!c->mlp_p) return set_err(UCLBS_E_STATE, "no network");
+  int rc = set_device(c);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpy(params, c->mlp_p, uclbs::kMlpParams * sizeof(float), hipMemcpyDeviceToHost));
+  *y_mean = c->mlp_ymean;
+  *y_std = c->mlp_ystd;
+  return UCLBS_OK;
+}
+
+int uclbs_mlp_loss_grad(void* ctx, const float* X, const float* y, int B, float* grad,
+                        double* loss) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || !X || !y || !grad || !loss || B <= 0) return set_err(UCLBS_E_ARG, "bad argument");
+  if (!c->mlp_p) return set_err(UCLBS_E_STATE, "no network (uclbs_mlp_init)");
+  int rc = set_device(c);
+  if (rc || (rc = mlp_work(c, B))) return rc;
+  float *dX = nullptr, *dy = nullptr;
+  HIP_TRY(hipMalloc(&dX, (size_t)B * 12 * sizeof(float)));
+  hipError_t e = hipMalloc(&dy, (size_t)B * sizeof(float));
+  if (e == hipSuccess) e = hipMemcpy(dX, X, (size_t)B * 12 * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dy, y, (size_t)B * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = uclbs::launch_mlp_loss_grad(c->mlp_p, dX, dy, nullptr, B, c->work, c->mlp_g, c->d_scalar,
+                                    c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess)
+    e = hipMemcpy(grad, c->mlp_g, uclbs::kMlpParams * sizeof(float), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(loss, c->d_scalar, sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(dX);
+  if (dy) (void)hipFree(dy);
+  if (e != hipSuccess) return set_err(UCLBS_E_HIP, "mlp_loss_grad: %s", hipGetErrorString(e));
+  return UCLBS_OK;
+}
+
+int uclbs_mlp_forward(void* ctx, const float* X, int B, int use_mfma, float* yhat) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || !X || !yhat || B <= 0) return set_err(UCLBS_E_ARG, "bad argument");
+  if (!c->mlp_ready) return set_err(UCLBS_E_STATE, "no network");
+  int rc = set_device(c);
+  if (rc || (rc = mlp_work(c, B))) return rc;
+  float *dX = nullptr, *dy = nullptr;
+  HIP_TRY(hipMalloc(&dX, (size_t)B * 12 * sizeof(float)));
+  hipError_t e = hipMalloc(&dy, (size_t)B * sizeof(float));
+  if (e == hipSuccess) e = hipMemcpy(dX, X, (size_t)B * 12 * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = use_mfma ? uclbs::launch_mlp_infer(dX, B, c->mlp_wbf, c->mlp_p, dy, c->stream)
+                 : uclbs::launch_mlp_forward_fp32(c->mlp_p, dX, B, c->work, dy, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess) e = hipMemcpy(yhat, dy, (size_t)B * sizeof(float), hipMemcpyDeviceToHost);
+  (void)hipFree(dX);
+  if (dy) (void)hipFree(dy);
+  if (e != hipSuccess) return set_err(UCLBS_E_HIP, "mlp_forward: %s", hipGetErrorString(e));
+  return UCLBS_OK;
+}
+
+int uclbs_mlp_reserve(void* ctx, int64_t capacity) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || capacity <= 0 || capacity > (1ll << 31)) return set_err(UCLBS_E_ARG, "bad capacity");
+  int rc = set_device(c);
+  if (rc) return rc;
+  uclbs::MlpSamples& m = c->samples;
+  for (void* p : {(void*)m.X, (void*)m.y, (void*)m.node_of, (void*)m.m_of, (void*)m.count})
+    if (p) (void)hipFree(p);
+  m = uclbs::MlpSamples{};
+  HIP_TRY(hipMalloc(&m.X, (size_t)capacity * 12 * sizeof(float)));
+  HIP_TRY(hipMalloc(&m.y, (size_t)capacity * sizeof(float)));
+  HIP_TRY(hipMalloc(&m.node_of, (size_t)capacity * sizeof(unsigned)));
+  HIP_TRY(hipMalloc(&m.m_of, (size_t)capacity * sizeof(float)));
+  HIP_TRY(hipMalloc(&m.count, sizeof(unsigned long long)));
+  HIP_TRY(hipMemset(m.count, 0, sizeof(unsigned long long)));
+  m.cap = capacity;
+  c->samples_normalised = false;
+  return UCLBS_OK;
+}
+
+static int sample_count(Ctx* c, unsigned long long* n) {
+  HIP_TRY(hipMemcpyAsync(n, c->samples.count, sizeof(*n), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (*n > (unsigned long long)c->samples.cap) *n = (unsigned long long)c->samples.cap;
+  return UCLBS_OK;
+}
+
+int uclbs_mlp_collect(void* ctx, int lvl, int stride, int64_t* n_samples) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  Level* L;
+  int rc = check_level(c, lvl, &L);
+  if (rc) return rc;
+  if (!n_samples || stride < 1) return set_err(UCLBS_E_ARG, "bad argument");
+  if (!c->samples.cap) return set_err(UCLBS_E_STATE, "no sample buffer (uclbs_mlp_reserve)");
+  if (c->cfg.lattice != UCLBS_LATTICE_D3Q27 || c->cfg.precision != UCLBS_PREC_FP32_SHIFTED ||
+      c->cfg.alpha_mode != UCLBS_ALPHA_NEWTON)
+    return set_err(UCLBS_E_UNSUPPORTED, "samples come from the plain D3Q27 FP32-shifted solver");
+  if (c->samples_normalised) return set_err(UCLBS_E_STATE, "samples already used for training");
+  if ((rc = set_device(c))) return rc;
+  unsigned long long first = 0, last = 0;
+  if ((rc = sample_count(c, &first))) return rc;
+  HIP_TRY(uclbs::launch_mlp_collect(L->box, static_cast<const float*>(L->f[L->cur]),
+                                    L->has_links ? L->flags : nullptr, stride,
+                                    (int)(c->steps_done % stride), c->samples, c->stream));
+  // the plain step produces the targets
+  const bool keep = c->keep_alpha;
+  c->keep_alpha = true;
+  rc = launch_steps(c, L, 1);
+  c->keep_alpha = keep;
+  if (rc) return rc;
+  if ((rc = sample_count(c, &last))) return rc;
+  HIP_TRY(uclbs::launch_mlp_targets(L->alpha, c->samples, first, last, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  *n_samples = (int64_t)last;
+  return UCLBS_OK;
+}
+
+int uclbs_mlp_download_samples(void* ctx, float* X, float* y, int64_t capacity, int64_t* n) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || !n) return set_err(UCLBS_E_ARG, "null argument");
+  if (!c->samples.cap) return set_err(UCLBS_E_STATE, "no sample buffer");
+  int rc = set_device(c);
+  if (rc) return rc;
+  unsigned long long cnt = 0;
+  if ((rc = sample_count(c, &cnt))) return rc;
+  *n = (int64_t)cnt;
+  const size_t k = (size_t)std::min<int64_t>((int64_t)cnt, capacity);
+  if (X && k) HIP_TRY(hipMemcpy(X, c->samples.X, k * 12 * sizeof(float), hipMemcpyDeviceToHost));
+  if (y && k) HIP_TRY(hipMemcpy(y, c->samples.y, k * sizeof(float), hipMemcpyDeviceToHost));
+  return UCLBS_OK;
+}
+
+int uclbs_mlp_train(void* ctx, int64_t steps, int batch, double lr, uint64_t seed,
+                    double* loss_first, double* loss_last) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || steps <= 0 || batch <= 0 || batch > (1 << 20) || !(lr > 0.0))
+    return set_err(UCLBS_E_ARG, "bad training arguments");
+  if (!c->mlp_p) return set_err(UCLBS_E_STATE, "no network (uclbs_mlp_init)");
+  int rc = set_device(c);
+  if (rc || (rc = mlp_work(c, batch))) return rc;
+  unsigned long long n = 0;
+  if ((rc = sample_count(c, &n))) return rc;
+  if (n < 2) return set_err(UCLBS_E_STATE, "no training samples (uclbs_mlp_collect)");
+  if (!c->samples_normalised) {
+    HIP_TRY(uclbs::launch_mlp_normalise(c->samples.y, n, c->d_scalar + 2, c->stream));
+    double ms[2];
+    HIP_TRY(hipMemcpyAsync(ms, c->d_scalar + 2, sizeof(ms), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->mlp_ymean = (float)ms[0];
+    c->mlp_ystd = (float)ms[1];
+    c->samples_normalised = true;
+  }
+  double l0 = 0.0, l1 = 0.0;
+  for (int64_t t = 0; t < steps; ++t) {
+    HIP_TRY(uclbs::launch_mlp_gather(c->samples.X, n, seed, (uint64_t)t, batch, c->batch_x,
+                                     c->batch_idx, c->stream));
+    HIP_TRY(uclbs::launch_mlp_loss_grad(c->mlp_p, c->batch_x, c->samples.y, c->batch_idx, batch,
+                                        c->work, c->mlp_g, c->d_scalar, c->stream));
+    HIP_TRY(uclbs::launch_mlp_adam(c->mlp_p, c->mlp_g, c->mlp_m, c->mlp_v, ++c->mlp_t, (float)lr,
+                                   c->stream));
+    if (t == 0 || t == steps - 1) {
+      double l;
+      HIP_TRY(hipMemcpyAsync(&l, c->d_scalar, sizeof(l), hipMemcpyDeviceToHost, c->stream));
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      (t == 0 ? l0 : l1) = l;
+    }
+  }
+  if (loss_first) *loss_first = l0;
+  if (loss_last) *loss_last = steps == 1 ? l0 : l1;
+  return mlp_publish(c);
 }
 
 }  // extern "C"
