@@ -200,7 +200,7 @@ __device__ __forceinline__ AlphaOut collide27_shifted(float (&f)[27], double bet
     } else {
       // some lane needs more terms: the wavefront extends its sums to order
       // kMidOrder from the populations still in registers
-      double Sh[kMidOrder - kFastOrder];
+      double Sh[kMidOrder > kFastOrder ? kMidOrder - kFastOrder : 1];
 #pragma unroll
       for (int q = 0; q < kMidOrder - kFastOrder; ++q) Sh[q] = 0.0;
 #pragma unroll
