@@ -190,6 +190,23 @@ int uclbs_init_fields(void *ctx, int lvl, const double *rho, const double *u, co
 int uclbs_upload_state(void *ctx, int lvl, const double *f, const double *g, const double *lm,
                        int64_t n_nodes);
 int uclbs_download_state(void *ctx, int lvl, double *f, double *g, double *lm, int64_t n_nodes);
+/* ---- refined levels (D3Q27) ------------------------------------------------
+ * uclbs_define_level with lvl >= 1 defines a box nested in level lvl-1 at
+ * refinement ratio 2 (lo/hi in level lvl's own node indices; node G of level
+ * lvl sits at coordinate G/2 of level lvl-1), stored with a one-node ghost
+ * shell: its arrays (upload/download) hold (nx+2)(ny+2)(nz+2) nodes.  Lattice
+ * viscosity doubles per level (acoustic scaling).  The ghost shell is filled
+ * from the coarser level before each of the level's steps: */
+/* theta in [0, 1]: time between level lvl-1's previous and current states
+ * (interlevel_interpolation.f95:114-493: amrex_fillcoarsepatch with
+ * amrex_interp_node_bilinear; negative populations -> coarse equilibrium) */
+int uclbs_interlevel(void *ctx, int lvl, double theta);
+/* every node of level lvl from level lvl-1's current state (start-up / regrid) */
+int uclbs_init_from_coarse(void *ctx, int lvl);
+/* covered nodes of level lvl-1 take the coincident level-lvl populations
+ * (fine_to_coarse.f95:25-65, amrex_average_down_nodes at ratio 2) */
+int uclbs_fine_to_coarse(void *ctx, int lvl);
+
 /* The level's step counter (reference timestep(lvl)), which drives the
  * start-up ramps of beta, epsilon and alpha; advanced by every step. */
 int uclbs_set_timestep(void *ctx, int lvl, int64_t step);

@@ -29,7 +29,8 @@ EXPORTS = (
     "uclbs_copy_bandwidth", "uclbs_set_gas", "uclbs_init_fields", "uclbs_upload_state",
     "uclbs_download_state", "uclbs_set_timestep", "uclbs_mlp_init", "uclbs_mlp_set_params",
     "uclbs_mlp_get_params", "uclbs_mlp_loss_grad", "uclbs_mlp_forward", "uclbs_mlp_reserve",
-    "uclbs_mlp_collect", "uclbs_mlp_download_samples", "uclbs_mlp_train",
+    "uclbs_mlp_collect", "uclbs_mlp_download_samples", "uclbs_mlp_train", "uclbs_interlevel",
+    "uclbs_init_from_coarse", "uclbs_fine_to_coarse",
 )
 MLP_PARAMS = 9217
 FACE_PERIODIC = 2147483647
@@ -131,6 +132,9 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "uclbs_mlp_collect": (I, [P, I, I, ctypes.POINTER(I64)]),
         "uclbs_mlp_download_samples": (I, [P, fp, fp, I64, ctypes.POINTER(I64)]),
         "uclbs_mlp_train": (I, [P, I64, I, D, U64, dp, dp]),
+        "uclbs_interlevel": (I, [P, I, D]),
+        "uclbs_init_from_coarse": (I, [P, I]),
+        "uclbs_fine_to_coarse": (I, [P, I]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -215,11 +219,35 @@ class Solver:
             pass
 
     def define_level(self, lvl: int, shape: Sequence[int], periodic=(1, 1, 1), lo=(0, 0, 0)):
+        """shape: valid nodes; a refined level (lvl >= 1, periodic (0,0,0)) is
+        stored with a one-node ghost shell, so its arrays have shape + 2."""
         lo3 = (ctypes.c_int32 * 3)(*lo)
         hi3 = (ctypes.c_int32 * 3)(*[lo[d] + shape[d] - 1 for d in range(3)])
         per = (ctypes.c_int32 * 3)(*[int(p) for p in periodic])
         self._check(self.lib.uclbs_define_level(self.ctx, lvl, lo3, hi3, per))
-        self.shapes[lvl] = tuple(int(s) for s in shape)
+        g = 2 if lvl > 0 else 0
+        self.shapes[lvl] = tuple(int(s) + g for s in shape)
+
+    def interlevel(self, lvl: int, theta: float):
+        self._check(self.lib.uclbs_interlevel(self.ctx, lvl, theta))
+
+    def init_from_coarse(self, lvl: int):
+        self._check(self.lib.uclbs_init_from_coarse(self.ctx, lvl))
+
+    def fine_to_coarse(self, lvl: int):
+        self._check(self.lib.uclbs_fine_to_coarse(self.ctx, lvl))
+
+    def advance(self, nsteps: int, lvl: int = 0):
+        """Level lvl one step, each finer level two substeps per step (ghosts
+        at the coarse level's previous state, then half way), finer levels
+        injected back after their second substep; repeated nsteps times."""
+        for _ in range(nsteps):
+            self.step(lvl, 1)
+            if lvl + 1 in self.shapes:
+                for theta in (0.0, 0.5):
+                    self.interlevel(lvl + 1, theta)
+                    self.advance(1, lvl + 1)
+                self.fine_to_coarse(lvl + 1)
 
     def n_nodes(self, lvl: int) -> int:
         s = self.shapes[lvl]

@@ -35,7 +35,8 @@ struct StepArgs {
 };
 
 // node flags of a level with link tables
-enum : uint8_t { kFlagInterior = 0, kFlagBoundary = 1, kFlagSolid = 2, kFlagFace = 3 };
+enum : uint8_t { kFlagInterior = 0, kFlagBoundary = 1, kFlagSolid = 2, kFlagFace = 3,
+                 kFlagGhost = 4 /* ghost shell of a refined level, filled from the coarser one */ };
 
 struct BcTables {
   const double* feq_in;
@@ -98,6 +99,22 @@ hipError_t launch_d3q27_macro(const Box& b, const void* f, int shifted, double* 
                               double* ux, double* uy, double* uz, hipStream_t s);
 hipError_t launch_d3q27_init_tgv(const Box& b, void* f, int shifted, double u0,
                                  double noise_amp, uint64_t seed, hipStream_t s);
+
+// ---- coarse-fine exchange -------------------------------------------------------
+struct LevelMap {   // a level's stored box in its global (level) index space
+  Box box;
+  int origin[3];    // global index of stored node (0, 0, 0)
+  int per[3];
+};
+struct Range3 {
+  int lo[3], hi[3];
+};
+hipError_t launch_coarse_to_fine27(const void* cprev, const void* cnew, const LevelMap& cm, void* ff,
+                                   const LevelMap& fm, const unsigned* ghosts, unsigned nghost,
+                                   double theta, int shifted, unsigned long long* stats,
+                                   hipStream_t s);
+hipError_t launch_fine_to_coarse27(const void* ff, const LevelMap& fm, void* fc, const LevelMap& cm,
+                                   const Range3& cover, int shifted, hipStream_t s);
 
 // ---- learned alpha -------------------------------------------------------------
 constexpr int kMlpParams = 9217;  // 12-64-64-64-1

@@ -61,6 +61,12 @@ struct Level {
   unsigned* neg_list = nullptr;
   double* rep = nullptr;
   int64_t step = 0;
+  // placement in the level's global index space; levels >= 1 carry a
+  // one-node ghost shell filled from the coarser level
+  int origin[3] = {0, 0, 0};
+  uint32_t* ghosts = nullptr;
+  unsigned nghost = 0;
+  bool refined = false;
 };
 
 struct Ctx {
@@ -138,15 +144,20 @@ void free_level(Level& L) {
   if (L.neg_count) { (void)hipFree(L.neg_count); L.neg_count = nullptr; }
   if (L.neg_list) { (void)hipFree(L.neg_list); L.neg_list = nullptr; }
   if (L.rep) { (void)hipFree(L.rep); L.rep = nullptr; }
+  if (L.ghosts) { (void)hipFree(L.ghosts); L.ghosts = nullptr; }
+  L.nghost = 0;
+  L.refined = false;
   L.step = 0;
   L.has_links = false;
   L.nb = L.nsolid = 0;
   L.defined = false;
 }
 
-uclbs::StepArgs step_args(const Ctx* c) {
+uclbs::StepArgs step_args(const Ctx* c, int lvl = 0) {
   uclbs::StepArgs a{};
-  a.beta = 1.0 / (6.0 * c->cfg.nu + 1.0);
+  // acoustic scaling (dx, dt halved per level): the lattice viscosity doubles
+  // per level, as beta_1_calculator's 2^(lvl+1) mu does (amr_processes.f95:1763)
+  a.beta = 1.0 / (6.0 * c->cfg.nu * std::ldexp(1.0, lvl) + 1.0);
   a.tol = c->cfg.alpha_tol;
   a.maxit = c->cfg.alpha_maxit;
   a.alpha_mode = c->cfg.alpha_mode;
@@ -271,7 +282,7 @@ int launch_steps39(Ctx* c, Level* L, int nsteps, hipEvent_t* ev) {
 
 int launch_steps(Ctx* c, Level* L, int nsteps, hipEvent_t* ev = nullptr) {
   if (c->cfg.lattice == UCLBS_LATTICE_D3Q39) return launch_steps39(c, L, nsteps, ev);
-  uclbs::StepArgs a = step_args(c);
+  uclbs::StepArgs a = step_args(c, (int)(L - c->levels.data()));
   if (L->has_links) {
     int rc = check_face_tables(c, L);
     if (rc) return rc;
@@ -405,10 +416,32 @@ int uclbs_define_level(void* ctx, int lvl, const int32_t lo[3], const int32_t hi
   if ((int)c->levels.size() <= lvl) c->levels.resize(lvl + 1);
   Level& L = c->levels[lvl];
   free_level(L);
+  if (lvl > 0) {
+    // a refined level: nested in level lvl-1 with ratio 2, not periodic, and
+    // the coarse nodes its ghost shell interpolates from inside the coarser
+    // level's valid nodes (no wrap across a periodic coarse face)
+    if (c->cfg.lattice != UCLBS_LATTICE_D3Q27)
+      return set_err(UCLBS_E_UNSUPPORTED, "refined levels are D3Q27 in this build");
+    if (lvl - 1 >= (int)c->levels.size() || !c->levels[lvl - 1].defined)
+      return set_err(UCLBS_E_STATE, "define level %d before level %d", lvl - 1, lvl);
+    const Level& P = c->levels[lvl - 1];
+    if (P.has_links && !P.refined)
+      return set_err(UCLBS_E_UNSUPPORTED, "refined levels nest in a level without link tables");
+    for (int d = 0; d < 3; ++d) {
+      if (periodic[d]) return set_err(UCLBS_E_ARG, "a refined level is not periodic");
+      const int cv_lo = P.lo[d], cv_hi = P.hi[d];
+      const int g_lo = (lo[d] - 1) >> 1, g_hi = (hi[d] + 2) >> 1;
+      if (g_lo < cv_lo || g_hi > cv_hi)
+        return set_err(UCLBS_E_ARG, "level %d box [%d,%d] on axis %d needs coarse nodes "
+                                    "[%d,%d] outside level %d's [%d,%d]",
+                       lvl, lo[d], hi[d], d, g_lo, g_hi, lvl - 1, cv_lo, cv_hi);
+    }
+  }
   int n[3];
   for (int d = 0; d < 3; ++d) {
-    n[d] = hi[d] - lo[d] + 1;
+    n[d] = hi[d] - lo[d] + 1 + (lvl > 0 ? 2 : 0);
     if (n[d] < 3) return set_err(UCLBS_E_ARG, "box extent %d on axis %d is below 3", n[d], d);
+    L.origin[d] = lo[d] - (lvl > 0 ? 1 : 0);
     L.lo[d] = lo[d];
     L.hi[d] = hi[d];
     L.box.per[d] = periodic[d] != 0;
@@ -450,6 +483,28 @@ int uclbs_define_level(void* ctx, int lvl, const int32_t lo[3], const int32_t hi
   L.step = 0;
   L.cur = 0;
   L.defined = true;
+  if (lvl > 0) {
+    // flags (ghost shell / interior) and the ghost list
+    L.refined = true;
+    std::vector<uint8_t> hf((size_t)L.box.n, uclbs::kFlagInterior);
+    std::vector<uint32_t> gl;
+    for (int cz = 0; cz < n[2]; ++cz)
+      for (int cy = 0; cy < n[1]; ++cy)
+        for (int cx = 0; cx < n[0]; ++cx)
+          if (cx == 0 || cy == 0 || cz == 0 || cx == n[0] - 1 || cy == n[1] - 1 || cz == n[2] - 1) {
+            const size_t k = (size_t)cx + (size_t)n[0] * ((size_t)cy + (size_t)n[1] * cz);
+            hf[k] = uclbs::kFlagGhost;
+            gl.push_back((uint32_t)k);
+          }
+    HIP_TRY(hipMalloc(&L.flags, (size_t)L.box.n));
+    HIP_TRY(hipMemcpy(L.flags, hf.data(), hf.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&L.ghosts, gl.size() * sizeof(uint32_t)));
+    HIP_TRY(hipMemcpy(L.ghosts, gl.data(), gl.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    L.nghost = (unsigned)gl.size();
+    L.has_links = true;  // the streaming kernel skips the ghost shell by its flag
+    L.nb = 0;
+    for (int f = 0; f < 6; ++f) L.face_code[f] = -10;
+  }
   return UCLBS_OK;
 }
 
@@ -596,6 +651,7 @@ int uclbs_set_faces(void* ctx, int lvl, const int32_t face_code[6]) {
   Level* L;
   int rc = check_level(c, lvl, &L);
   if (rc) return rc;
+  if (L->refined) return set_err(UCLBS_E_UNSUPPORTED, "level %d is a refined level (no link tables)", lvl);
   if (!face_code) return set_err(UCLBS_E_ARG, "null face_code");
   for (int f = 0; f < 6; ++f) {
     const int32_t v = face_code[f];
@@ -647,6 +703,7 @@ int uclbs_build_links(void* ctx, int lvl, const double origin[3], double dx, int
   Level* L;
   int rc = check_level(c, lvl, &L);
   if (rc) return rc;
+  if (L->refined) return set_err(UCLBS_E_UNSUPPORTED, "level %d is a refined level (no link tables)", lvl);
   if (c->cfg.lattice != UCLBS_LATTICE_D3Q27)
     return set_err(UCLBS_E_UNSUPPORTED, "build_links: D3Q27 tables only");
   if (!origin || !(dx > 0.0) || ntri < 0 || ntri > (1 << 26) || (ntri && (!tri || !nrm)))
@@ -709,6 +766,7 @@ int uclbs_set_links(void* ctx, int lvl, const int32_t* state, int64_t n_nodes) {
   Level* L;
   int rc = check_level(c, lvl, &L);
   if (rc) return rc;
+  if (L->refined) return set_err(UCLBS_E_UNSUPPORTED, "level %d is a refined level (no link tables)", lvl);
   if (!state || n_nodes != L->box.n) return set_err(UCLBS_E_ARG, "set_links: wrong node count");
   if ((rc = set_device(c))) return rc;
   if (!L->state) HIP_TRY(hipMalloc(&L->state, (size_t)n_nodes * 27 * sizeof(int32_t)));
@@ -1052,6 +1110,71 @@ int uclbs_download_state(void* ctx, int lvl, double* f, double* g, double* lm, i
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   (void)hipFree(tmp);
   if (e != hipSuccess) return set_err(UCLBS_E_HIP, "download_state failed: %s", hipGetErrorString(e));
+  return UCLBS_OK;
+}
+
+static uclbs::LevelMap level_map(const Level& L) {
+  uclbs::LevelMap m{};
+  m.box = L.box;
+  for (int d = 0; d < 3; ++d) {
+    m.origin[d] = L.origin[d];
+    m.per[d] = L.box.per[d];
+  }
+  return m;
+}
+
+static int interlevel_fill(Ctx* c, int lvl, double theta, bool all_nodes) {
+  Level* F;
+  int rc = check_level(c, lvl, &F);
+  if (rc) return rc;
+  if (!F->refined) return set_err(UCLBS_E_ARG, "level %d is not a refined level", lvl);
+  if (!(theta >= 0.0 && theta <= 1.0)) return set_err(UCLBS_E_ARG, "theta outside [0, 1]");
+  if ((rc = set_device(c))) return rc;
+  const Level& C = c->levels[lvl - 1];
+  const uclbs::LevelMap cm = level_map(C), fm = level_map(*F);
+  const int sh = c->cfg.precision == UCLBS_PREC_FP32_SHIFTED;
+  if (all_nodes) {
+    HIP_TRY(uclbs::launch_coarse_to_fine27(C.f[1 - C.cur], C.f[C.cur], cm, F->f[F->cur], fm, nullptr,
+                                           (unsigned)F->box.n, theta, sh, c->d_stats, c->stream));
+    HIP_TRY(hipMemcpyAsync(F->f[1 - F->cur], F->f[F->cur], level_bytes(c, F),
+                           hipMemcpyDeviceToDevice, c->stream));
+  } else {
+    HIP_TRY(uclbs::launch_coarse_to_fine27(C.f[1 - C.cur], C.f[C.cur], cm, F->f[F->cur], fm,
+                                           F->ghosts, F->nghost, theta, sh, c->d_stats, c->stream));
+  }
+  return UCLBS_OK;
+}
+
+int uclbs_interlevel(void* ctx, int lvl, double theta) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c) return set_err(UCLBS_E_ARG, "null context");
+  return interlevel_fill(c, lvl, theta, false);
+}
+
+int uclbs_init_from_coarse(void* ctx, int lvl) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c) return set_err(UCLBS_E_ARG, "null context");
+  int rc = interlevel_fill(c, lvl, 1.0, true);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return UCLBS_OK;
+}
+
+int uclbs_fine_to_coarse(void* ctx, int lvl) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  Level* F;
+  int rc = check_level(c, lvl, &F);
+  if (rc) return rc;
+  if (!F->refined) return set_err(UCLBS_E_ARG, "level %d is not a refined level", lvl);
+  if ((rc = set_device(c))) return rc;
+  Level& C = c->levels[lvl - 1];
+  uclbs::Range3 cover{};
+  for (int d = 0; d < 3; ++d) {
+    cover.lo[d] = (F->lo[d] + 1) >> 1;  // coarse nodes whose fine image 2C is valid
+    cover.hi[d] = F->hi[d] >> 1;
+  }
+  HIP_TRY(uclbs::launch_fine_to_coarse27(F->f[F->cur], level_map(*F), C.f[C.cur], level_map(C), cover,
+                                         c->cfg.precision == UCLBS_PREC_FP32_SHIFTED, c->stream));
   return UCLBS_OK;
 }
 
