@@ -1426,7 +1426,13 @@ int uclbs_mlp_train(void* ctx, int64_t steps, int batch, double lr, uint64_t see
                                      c->batch_idx, c->stream));
     HIP_TRY(uclbs::launch_mlp_loss_grad(c->mlp_p, c->batch_x, c->samples.y, c->batch_idx, batch,
                                         c->work, c->mlp_g, c->d_scalar, c->stream));
-    HIP_TRY(uclbs::launch_mlp_adam(c->mlp_p, c->mlp_g, c->mlp_m, c->mlp_v, ++c->mlp_t, (float)lr,
+    // learning rate: linear warm-up over the first 2% of the steps, then a
+    // cosine decay to 5% of lr (keeps the late steps from undoing the fit)
+    const double warm = std::max<double>(1.0, 0.02 * (double)steps);
+    const double prog = (double)t / (double)steps;
+    const double lrt = t < warm ? lr * (t + 1) / warm
+                                : lr * (0.05 + 0.95 * 0.5 * (1.0 + std::cos(3.141592653589793 * prog)));
+    HIP_TRY(uclbs::launch_mlp_adam(c->mlp_p, c->mlp_g, c->mlp_m, c->mlp_v, ++c->mlp_t, (float)lrt,
                                    c->stream));
     if (t == 0 || t == steps - 1) {
       double l;
