@@ -18,6 +18,7 @@ def test_fortran_driver_binds_c_abi():
     out = subprocess.run(["nm", "-D", "--undefined-only", DRIVER], capture_output=True,
                          text=True, check=True).stdout
     for sym in ("uclbs_create", "uclbs_define_level", "uclbs_init_tgv", "uclbs_step_level",
+                "uclbs_interlevel", "uclbs_fine_to_coarse", "uclbs_init_from_coarse",
                 "uclbs_download_macro", "uclbs_get_stats", "uclbs_last_error", "uclbs_destroy"):
         assert sym in out, sym
 
@@ -45,3 +46,26 @@ def test_fortran_loop_c1_matches_oracle(orc, tmp_path):
     e_r = np.linalg.norm(rho_g - rho_o) / np.linalg.norm(rho_o)
     print(f"Fortran loop C1: relL2 u {e_u:.2e}, rho {e_r:.2e}")
     assert e_u < 1e-5 and e_r < 1e-7
+
+
+@pytest.mark.gpu
+def test_fortran_two_level_loop_matches_python_schedule(pkg, tmp_path):
+    """The Fortran recursive level advance (amr_advance) and the Python
+    Solver.advance issue the same library calls: identical coarse fields."""
+    L, steps = 32, 12
+    out = tmp_path / "amr.bin"
+    r = subprocess.run([DRIVER, str(out), str(L), str(steps), "0", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    data = np.fromfile(out, dtype=np.float64).reshape(4, L ** 3)
+    u0 = 0.1 / math.sqrt(3.0)
+    nu = u0 * (L / (2 * math.pi)) / 1600.0
+    s = pkg.Solver(precision="fp32s", nu=nu, alpha_tol=1e-10)
+    s.define_level(0, (L, L, L))
+    s.init_tgv(0, u0, 1e-4, 20260101)
+    s.define_level(1, (L + 1, L + 1, L + 1), periodic=(0, 0, 0), lo=(L // 2, L // 2, L // 2))
+    s.init_from_coarse(1)
+    s.advance(steps)
+    ref = np.stack(s.macro(0))
+    assert np.abs(data - ref).max() < 1e-12
+    assert np.all(np.isfinite(data))

@@ -28,6 +28,14 @@ module uclbs_hip
     integer(c_int32_t) :: reserved(8) = 0
   end type
 
+  ! D3Q39 gas and freestream (include/uclbs_hip.h uclbs_gas)
+  type, bind(c), public :: uclbs_gas
+    real(c_double) :: gama = 1.4_c_double, temperature = 300.0_c_double, &
+                      pressure = 101325.0_c_double, mach = 0.3_c_double, dx = 1.0e-3_c_double
+    integer(c_int32_t) :: startup_timesteps = 25, turbulent = 1, shifted = -1, lm_maxit = 0
+    integer(c_int32_t) :: reserved(4) = 0
+  end type
+
   type, bind(c), public :: uclbs_stats
     integer(c_int64_t) :: node_updates, newton_iters, n_trivial, n_bound, n_nonpos, &
                           n_noconv, n_fallback, n_logpath, n_ibb_links, n_repair
@@ -38,7 +46,10 @@ module uclbs_hip
             uclbs_download_alpha, uclbs_init_tgv, uclbs_init_uniform, uclbs_set_faces, &
             uclbs_set_bc_states, uclbs_build_links, uclbs_set_links, uclbs_download_links, &
             uclbs_step_level, uclbs_sync, uclbs_get_stats, uclbs_reset_stats, &
-            uclbs_geom_icosphere, uclbs_check
+            uclbs_geom_icosphere, uclbs_check, uclbs_set_gas, uclbs_init_fields, &
+            uclbs_upload_state, uclbs_download_state, uclbs_set_timestep, uclbs_interlevel, &
+            uclbs_init_from_coarse, uclbs_fine_to_coarse, uclbs_mlp_init, uclbs_mlp_set_params, &
+            uclbs_mlp_get_params, uclbs_mlp_reserve, uclbs_mlp_collect, uclbs_mlp_train
 
   interface
     function c_last_error() bind(c, name='uclbs_last_error') result(p)
@@ -190,6 +201,102 @@ module uclbs_hip
       real(c_double), intent(out) :: tri(*), nrm(*)
       integer(c_int64_t), value :: cap
       integer(c_int64_t), intent(out) :: ntri
+      integer(c_int) :: rc
+    end function
+    function uclbs_set_gas(ctx, gas) bind(c, name='uclbs_set_gas') result(rc)
+      import :: c_ptr, c_int, uclbs_gas
+      type(c_ptr), value :: ctx
+      type(uclbs_gas), intent(in) :: gas
+      integer(c_int) :: rc
+    end function
+    function uclbs_init_fields(ctx, lvl, rho, u, v, w, temp) bind(c, name='uclbs_init_fields') result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: lvl
+      real(c_double), intent(in) :: rho(*), u(*), v(*), w(*), temp(*)
+      integer(c_int) :: rc
+    end function
+    function uclbs_upload_state(ctx, lvl, f, g, lm, n) bind(c, name='uclbs_upload_state') result(rc)
+      import :: c_ptr, c_int, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: lvl
+      real(c_double), intent(in) :: f(*), g(*), lm(*)
+      integer(c_int64_t), value :: n
+      integer(c_int) :: rc
+    end function
+    function uclbs_download_state(ctx, lvl, f, g, lm, n) bind(c, name='uclbs_download_state') result(rc)
+      import :: c_ptr, c_int, c_double, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: lvl
+      real(c_double), intent(out) :: f(*), g(*), lm(*)
+      integer(c_int64_t), value :: n
+      integer(c_int) :: rc
+    end function
+    function uclbs_set_timestep(ctx, lvl, step) bind(c, name='uclbs_set_timestep') result(rc)
+      import :: c_ptr, c_int, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: lvl
+      integer(c_int64_t), value :: step
+      integer(c_int) :: rc
+    end function
+    function uclbs_interlevel(ctx, lvl, theta) bind(c, name='uclbs_interlevel') result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: lvl
+      real(c_double), value :: theta
+      integer(c_int) :: rc
+    end function
+    function uclbs_init_from_coarse(ctx, lvl) bind(c, name='uclbs_init_from_coarse') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: lvl
+      integer(c_int) :: rc
+    end function
+    function uclbs_fine_to_coarse(ctx, lvl) bind(c, name='uclbs_fine_to_coarse') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: lvl
+      integer(c_int) :: rc
+    end function
+    function uclbs_mlp_init(ctx, seed) bind(c, name='uclbs_mlp_init') result(rc)
+      import :: c_ptr, c_int, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: seed
+      integer(c_int) :: rc
+    end function
+    function uclbs_mlp_set_params(ctx, p, ymean, ystd) bind(c, name='uclbs_mlp_set_params') result(rc)
+      import :: c_ptr, c_int, c_float
+      type(c_ptr), value :: ctx
+      real(c_float), intent(in) :: p(*)
+      real(c_float), value :: ymean, ystd
+      integer(c_int) :: rc
+    end function
+    function uclbs_mlp_get_params(ctx, p, ymean, ystd) bind(c, name='uclbs_mlp_get_params') result(rc)
+      import :: c_ptr, c_int, c_float
+      type(c_ptr), value :: ctx
+      real(c_float), intent(out) :: p(*), ymean, ystd
+      integer(c_int) :: rc
+    end function
+    function uclbs_mlp_reserve(ctx, cap) bind(c, name='uclbs_mlp_reserve') result(rc)
+      import :: c_ptr, c_int, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: cap
+      integer(c_int) :: rc
+    end function
+    function uclbs_mlp_collect(ctx, lvl, stride, n) bind(c, name='uclbs_mlp_collect') result(rc)
+      import :: c_ptr, c_int, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: lvl, stride
+      integer(c_int64_t), intent(out) :: n
+      integer(c_int) :: rc
+    end function
+    function uclbs_mlp_train(ctx, steps, batch, lr, seed, l0, l1) bind(c, name='uclbs_mlp_train') result(rc)
+      import :: c_ptr, c_int, c_int64_t, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int64_t), value :: steps, seed
+      integer(c_int), value :: batch
+      real(c_double), value :: lr
+      real(c_double), intent(out) :: l0, l1
       integer(c_int) :: rc
     end function
     function c_strlen(p) bind(c, name='strlen') result(n)

@@ -12,7 +12,7 @@ module uclbs_timestep
   use uclbs_hip
   implicit none
   private
-  public :: valid_levels, steps_counter, level_advance, run_loop
+  public :: valid_levels, steps_counter, level_advance, run_loop, amr_advance
 
 contains
 
@@ -88,6 +88,27 @@ contains
       call steps_counter(level_step, fine, timestep, time, time_prev, dt)
     end do
     call uclbs_check(uclbs_sync(ctx), 'uclbs_sync')
+  end subroutine
+
+  ! One step of level lvl with the levels above it subcycled at ratio 2: the
+  ! finer level's ghost shell is filled from this level's previous state
+  ! (theta = 0) and then half way (theta = 1/2) before its two substeps, and
+  ! the finer level is injected back once it has caught up
+  ! (interlevel_interpolation.f95 / fine_to_coarse.f95 between the level
+  ! steps of loop.f95:104-139).
+  recursive subroutine amr_advance(ctx, lvl, finest)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: lvl, finest
+    integer :: k
+    call uclbs_check(uclbs_step_level(ctx, int(lvl, c_int), 1_c_int), 'uclbs_step_level')
+    if (lvl < finest) then
+      do k = 0, 1
+        call uclbs_check(uclbs_interlevel(ctx, int(lvl + 1, c_int), 0.5_c_double * k), &
+                         'uclbs_interlevel')
+        call amr_advance(ctx, lvl + 1, finest)
+      end do
+      call uclbs_check(uclbs_fine_to_coarse(ctx, int(lvl + 1, c_int)), 'uclbs_fine_to_coarse')
+    end if
   end subroutine
 
 end module uclbs_timestep
