@@ -65,11 +65,34 @@ __host__ __device__ inline double expo39(const double* __restrict__ lm, double r
   return rho * exp(e);
 }
 
+// Monomials of every direction for both lattices as a constant table: the
+// Newton loop walks the 39 directions without unrolling them (the fully
+// unrolled Jacobian, with its weights folded into ~2k specialised FMAs, made
+// the kernel six times slower, measured: its code no longer fits the
+// instruction cache), reading each direction's 13 weights as scalars.
+struct Mono39Table {
+  signed char v[2][39][13];
+  constexpr Mono39Table() : v() {
+    for (int sh = 0; sh < 2; ++sh)
+      for (int i = 0; i < 39; ++i)
+        for (int k = 0; k < 13; ++k)
+          v[sh][i][k] = (signed char)(sh ? mono39<true>(i, k) : mono39<false>(i, k));
+  }
+};
+__constant__ constexpr Mono39Table kMonoT{};
+#define kMono39 kMonoT.v
+
+struct NoSink39 {
+  __host__ __device__ void operator()(int, double) const {}
+};
+
 // Newton on the moment constraints; lm in/out, lm_prev = multipliers of the
-// last iteration before its update.  Returns iterations, or -1 at the cap.
-template <bool kShifted>
+// last iteration before its update; sink(i, e) receives each iteration's
+// equilibrium (so the caller keeps the last one).  Returns iterations, or -1
+// at the cap.
+template <bool kShifted, class Sink = NoSink39>
 __host__ __device__ inline int lm_newton39(double rho, const double tg[kM39], double lm[kM39],
-                                           double lm_prev[kM39], int maxit) {
+                                           double lm_prev[kM39], int maxit, Sink sink = Sink()) {
   double mx = 1.0;
   int it = 0;
   while (mx > 1.0e-7) {
@@ -78,18 +101,20 @@ __host__ __device__ inline int lm_newton39(double rho, const double tg[kM39], do
     double J[kJ39];
 #pragma unroll
     for (int p = 0; p < kJ39; ++p) J[p] = 0.0;
-#pragma unroll
+#pragma unroll 1
     for (int i = 0; i < Q39; ++i) {
-      const double e = expo39<kShifted>(lm, rho, i);
+      double ex = lm[0];
+#pragma unroll
+      for (int k = 1; k < kM39; ++k) ex = fma((double)kMono39[kShifted][i][k], lm[k], ex);
+      const double e = rho * exp(ex);
+      sink(i, e);
+      double me[kM39];
+#pragma unroll
+      for (int k = 0; k < kM39; ++k) me[k] = (double)kMono39[kShifted][i][k] * e;
 #pragma unroll
       for (int k = 0; k < kM39; ++k)
 #pragma unroll
-        for (int l = 0; l <= k; ++l) {
-          const int w = mono39<kShifted>(i, k) * mono39<kShifted>(i, l);
-          if (w == 1) J[jidx(k, l)] += e;
-          else if (w == -1) J[jidx(k, l)] -= e;
-          else if (w != 0) J[jidx(k, l)] = fma((double)w, e, J[jidx(k, l)]);
-        }
+        for (int l = 0; l <= k; ++l) J[jidx(k, l)] = fma((double)kMono39[kShifted][i][l], me[k], J[jidx(k, l)]);
     }
     // residual: current moments (row 0 of J) minus targets
     double r[kM39];
@@ -133,6 +158,13 @@ __host__ __device__ inline int lm_newton39(double rho, const double tg[kM39], do
     if (!(mx == mx)) return -1;
   }
   return it;
+}
+
+// exp(-s) for the small argument left by the last Newton update (|s| below
+// 1e-3: five Taylor terms, error below 2e-21), else exp
+__host__ __device__ inline double expm_small(double s) {
+  if (fabs(s) < 1e-3) return 1.0 - s * (1.0 - s * (0.5 - s * (1.0 / 6.0 - s * (1.0 / 24.0 - s / 120.0))));
+  return exp(-s);
 }
 
 // quick_calcs.f95:24-58 (non-dimensional viscosity; dim_* the SI values)

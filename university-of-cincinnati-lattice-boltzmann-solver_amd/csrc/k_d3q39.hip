@@ -7,11 +7,12 @@
 // FP32 deviations from the reference-state equilibrium (f - f_ref, g - g_ref)
 // or FP64 full values; the 13 Lagrange multipliers lm[k][node] in FP64 (the
 // warm start of the next step's Newton).  One thread per node, one wavefront
-// per block: the node's pulled populations sit in LDS (39 + 39 values), the
-// 91-entry packed Jacobian and the collision temporaries in registers.  The
+// per block: the node's pulled populations and the last Newton iteration's
+// equilibrium sit in LDS, the 91-entry packed Jacobian in registers.  The
 // step is FP64-bound: per node and Newton iteration 39 exponentials, a
-// Jacobian of ~1.4k FMAs (integer weights folded at compile time) and a
-// 13 x 13 L D L^T solve.
+// Jacobian of 39 x 91 FMAs (weights from a constant table, read as scalars)
+// and a 13 x 13 L D L^T solve; alpha and the collision reuse the last
+// iteration's equilibrium instead of recomputing 78 exponentials.
 //
 // Reference correspondence: streaming.f95:365-368 (pull form), then
 // calculate_flow_variables.f95:201-311, calculate_alpha.f95:118-289,
@@ -57,14 +58,13 @@ __device__ __forceinline__ void wave_stats39(unsigned long long* __restrict__ st
 }
 
 // calculate_alpha.f95:118-289 over the populations f (LDS, full value f_i =
-// sf + fref) and the last iteration's equilibrium from lm_prev; sd is LDS
-// scratch for the deviation.  Returns alpha; *st = 0 (root), 1 (alpha = 2),
-// 2 / 3 (clamped low / high), -1 (no real root: the reference stops).
+// sf + fref) and the last Newton iteration's equilibrium (LDS, sd).  Returns
+// alpha; *st = 0 (root), 1 (alpha = 2), 2 / 3 (clamped low / high), -1 (no
+// real root: the reference stops).
 template <bool kShifted, typename S>
-__device__ __forceinline__ double alpha_estimate39(const S (*sf)[64], double (*sd)[64], int t,
-                                                   const Ref39& rf, const double lmp[kM39],
-                                                   double rho, double T, long long step,
-                                                   const Gas39& gs, int* st) {
+__device__ __forceinline__ double alpha_estimate39(const S (*sf)[64], const double (*sd)[64], int t,
+                                                   const Ref39& rf, double rho, double T,
+                                                   long long step, const Gas39& gs, int* st) {
   const double alpha_hi = 2.125, alpha_start = 1.0e-6;
   double dim_mu, dim_temp;
   sutherland39(T, gs, &dim_mu, &dim_temp);
@@ -76,8 +76,7 @@ __device__ __forceinline__ double alpha_estimate39(const S (*sf)[64], double (*s
 #pragma unroll 1
   for (int i = 0; i < Q39; ++i) {
     const double f = (double)sf[i][t] + rf.f[i];
-    const double d = expo39<kShifted>(lmp, rho, i) - f;
-    sd[i][t] = d;
+    const double d = sd[i][t] - f;
     const double df = f < 0.0 ? fabs(f) : f;
     const double x = d / df;
     cs1 = cs1 + df * (2.0 * (x * x) / (2 + x));
@@ -97,7 +96,7 @@ __device__ __forceinline__ double alpha_estimate39(const S (*sf)[64], double (*s
   for (int i = 0; i < Q39; ++i) {
     const double f = (double)sf[i][t] + rf.f[i];
     const double df = f < 0.0 ? fabs(f) : f;
-    const double x1 = sd[i][t] / df, x2 = x1 * x1, x3 = x1 * x1 * x1;
+    const double x1 = (sd[i][t] - f) / df, x2 = x1 * x1, x3 = x1 * x1 * x1;
     const double f2 = x2 * df, f3 = x3 * df, f4 = (x2 * x2) * df, f5 = x2 * x3 * df,
                  f6 = (x3 * x3) * df;
     if (x1 <= 0.0 && x1 >= -1.0) {
@@ -191,24 +190,37 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
     double lm[kM39], lmp[kM39];
 #pragma unroll
     for (int k = 0; k < kM39; ++k) lm[k] = lmv[(size_t)k * n + node];
-    const int it = lm_newton39<kShifted>(rho, tg, lm, lmp, gs.maxit);
+    // the last iteration's equilibrium stays in LDS for alpha and collision
+    const int it = lm_newton39<kShifted>(rho, tg, lm, lmp, gs.maxit,
+                                         [&](int i, double e) { sd[i][t] = e; });
     noconv = it < 0;
     iters = noconv ? gs.maxit : it;
 #pragma unroll
     for (int k = 0; k < kM39; ++k) lmv[(size_t)k * n + node] = lm[k];
 
     double al = 2.0;
-    if (kTurb) al = alpha_estimate39<kShifted>(sf, sd, t, rf, lmp, rho, T, step, gs, &ast);
+    if (kTurb) al = alpha_estimate39<kShifted>(sf, sd, t, rf, rho, T, step, gs, &ast);
     if (kKeepAlpha) alpha[node] = al;
 
-    // collision: deviation from the stored multipliers' equilibrium (D in
-    // LDS; the g deviation is rebuilt from f + D where it is needed)
+    // collision: deviation from the updated multipliers' equilibrium,
+    // exp(lm . m_i) = e_i exp(-(lm_prev - lm) . m_i) with e_i the last
+    // iteration's value (D replaces it in LDS; the g deviation is rebuilt
+    // from f + D where it is needed)
+    double dl[kM39];
+#pragma unroll
+    for (int k = 0; k < kM39; ++k) dl[k] = lmp[k] - lm[k];
     double qxxx = 0, qyyy = 0, qzzz = 0, qxxy = 0, qxxz = 0, qxyy = 0, qxzz = 0, qyyz = 0,
            qyzz = 0, qxyz = 0, lq0 = 0, lq1 = 0, lq2 = 0;
     const double gT = gs.lead_g * T;
 #pragma unroll
     for (int i = 0; i < Q39; ++i) {
-      const double feq = expo39<kShifted>(lm, rho, i);
+      double sdl = dl[0];
+#pragma unroll
+      for (int k = 1; k < kM39; ++k) {
+        const int m = mono39<kShifted>(i, k);
+        if (m != 0) sdl += (double)m * dl[k];
+      }
+      const double feq = sd[i][t] * expm_small(sdl);
       const double f = (double)sf[i][t] + rf.f[i], g = (double)sg[i][t] + rf.g[i];
       const double d = feq - f;
       sd[i][t] = d;
