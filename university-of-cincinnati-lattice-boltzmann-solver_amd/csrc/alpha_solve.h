@@ -97,6 +97,47 @@ __host__ __device__ __forceinline__ double log1p_fast(double x) {
   return de * ln2_hi + (de * ln2_lo + s * p) - c / u;
 }
 
+// 1/v for a normal v: the hardware estimate and two Newton steps (each at
+// least doubles the correct bits), within an ulp or so of the quotient
+__device__ __forceinline__ double drcp(double v) {
+  double r = __builtin_amdgcn_rcp(v);
+  r = fma(fma(-v, r, 1.0), r, r);
+  r = fma(fma(-v, r, 1.0), r, r);
+  return r;
+}
+
+// log1p_fast with both divisions replaced by reciprocals (device only): the
+// quotient s only needs relative accuracy (its error enters ln as 2 ds with
+// |s| <= 0.1716), and c/u is a first-order correction of size eps
+__device__ __forceinline__ double log1p_dev(double x) {
+  const double u = 1.0 + x;
+  const double c = (u - 1.0) - x;
+  int e;
+  double m = __builtin_frexp(u, &e);
+  if (m < 0.70710678118654752440) {
+    m *= 2.0;
+    e -= 1;
+  }
+  const double s = (m - 1.0) * drcp(m + 1.0);
+  const double s2 = s * s;
+  double p = 2.0 / 23.0;
+  p = p * s2 + 2.0 / 21.0;
+  p = p * s2 + 2.0 / 19.0;
+  p = p * s2 + 2.0 / 17.0;
+  p = p * s2 + 2.0 / 15.0;
+  p = p * s2 + 2.0 / 13.0;
+  p = p * s2 + 2.0 / 11.0;
+  p = p * s2 + 2.0 / 9.0;
+  p = p * s2 + 2.0 / 7.0;
+  p = p * s2 + 2.0 / 5.0;
+  p = p * s2 + 2.0 / 3.0;
+  p = p * s2 + 2.0;
+  const double ln2_hi = 6.93147180369123816490e-01;
+  const double ln2_lo = 1.90821492927058770002e-10;
+  const double de = (double)e;
+  return de * ln2_hi + (de * ln2_lo + s * p) - c * __builtin_amdgcn_rcp(u);
+}
+
 // phi(y) = (1 + y) ln(1 + y) - y, series for small |y|, direct otherwise
 __device__ __forceinline__ double phi1p(double y) {
   if (fabs(y) < 0.1) {
@@ -165,6 +206,109 @@ __device__ __forceinline__ AlphaOut newton_alpha(Eval&& ev, double S2, double am
     if (k >= maxit) { r.alpha = a; r.iters = k; r.status = ALPHA_NOCONV; return r; }
     if (g < 0.0) lo = a; else { hi = a; hi_is_bound = false; }
     double an = a - g / gp;
+    if (!(an > lo && an < hi)) an = (hi_is_bound && an >= hi) ? amax : 0.5 * (lo + hi);
+    if (an == amax) hi_is_bound = false;
+    a = an;
+  }
+}
+
+// Root of g for a node far from equilibrium, from sweeps of logarithms at a
+// centre a0 and the exact re-expansion of g about it.  With d = a - a0,
+// A_i = 1 + a0 x_i and z_i = x_i / A_i, 1 + a x_i = A_i (1 + d z_i) and
+//   g(a0 + d) = g(a0) + g'(a0) d + sum_{n>=2} (-1)^n T_n d^n / (n(n-1)),
+//   T_n = sum_i D_i z_i^(n-1),
+// from (1+t) ln(1+t) = t + sum_{n>=2} (-1)^n t^n / (n(n-1)) at t = d z_i and
+// f_i A_i z_i^n = D_i z_i^(n-1).  Since D_i z_i = f_i x_i^2 / A_i >= 0,
+// |T_n| <= zmax^(n-2) T_2 and the orders beyond N add at most
+//   R = T_2 d^2 (|d| zmax)^(N-1) / (N (N+1) (1 - |d| zmax)).
+// One sweep (Q logarithms) gives g(a0), g'(a0) and T_2..T_N; Newton on the
+// order-N polynomial (scalar work) gives d, and a0 + d is the root when the
+// polynomial's residual plus R is within tol * S2.  Otherwise the next sweep
+// is centred at a0 + d, kept inside the Newton bracket and within
+// |d| zmax <= 1/2.  From a seed near the root one sweep suffices where Newton
+// on direct logarithms evaluates g two to four times.  Bracket, bound and
+// iteration cap follow newton_alpha; iters counts the updates of the centre
+// or of the accepted root, as Newton counts its updates.
+// dx(i, d, x) yields D_i and x_i in FP64, fx(i) yields f_i; K = sum D ln(1+x).
+// kUnroll: unroll factor of the sweep over i (1 when dx / fx read memory).
+template <int Q, int N, int kUnroll, class DX, class FX>
+__device__ __forceinline__ AlphaOut recentred_alpha(DX&& dx, FX&& fx, double K, double S2,
+                                                    double amax, double alpha0, double tol,
+                                                    int maxit) {
+  double lo = 1.0, hi = amax < 1e6 ? amax : 1e6;
+  bool hi_is_bound = amax < 1e6;
+  double a = (alpha0 > lo && alpha0 < hi) ? alpha0 : 2.0;
+  AlphaOut r{2.0, 0, ALPHA_NEWTON, true};
+  for (int k = 0;; ++k) {
+    double g = 0.0, gp = 0.0, zmax = 0.0;
+    double T[N + 1];
+#pragma unroll
+    for (int n = 2; n <= N; ++n) T[n] = 0.0;
+    bool edge = false;  // a at the positivity bound: some 1 + a x_i <= 0
+#pragma unroll kUnroll
+    for (int i = 0; i < Q; ++i) {
+      double d, x;
+      dx(i, d, x);
+      const double f = fx(i);
+      const double y = a * x;
+      const double A = 1.0 + y;
+      if (A <= 0.0) { g -= f * y; edge = true; continue; }  // limit of (1+y) ln(1+y) at 0
+      const double la = log1p_dev(y);
+      g += f * (A * la - y);
+      gp += d * la;
+      const double z = x * drcp(A);
+      zmax = fmax(zmax, fabs(z));
+      double t = d * z;
+      T[2] += t;
+#pragma unroll
+      for (int n = 3; n <= N; ++n) { t *= z; T[n] += t; }
+    }
+    g -= a * K;
+    gp -= K;
+    if (fabs(g) <= tol * S2) { r.alpha = a; r.iters = k; return r; }
+    if (g < 0.0 && a >= amax) { r.alpha = amax; r.iters = k; r.status = ALPHA_BOUND; return r; }
+    if (k >= maxit) { r.alpha = a; r.iters = k; r.status = ALPHA_NOCONV; return r; }
+    if (g < 0.0) lo = a; else { hi = a; hi_is_bound = false; }
+    if (edge) { a = 0.5 * (lo + hi); continue; }
+    // Newton on the polynomial from d = 0, |d| within the trust radius
+    const double rad = zmax > 0.0 ? 0.5 / zmax : 1e300;
+    double dl = lo - a, dh = hi - a;
+    double dd = 0.0, p = g;
+    bool ok = true;
+    for (int j = 0; j < 8; ++j) {
+      double pd = gp, dn = dd;  // p'(d): gp + sum (-1)^n T_n d^(n-1) / (n-1)
+#pragma unroll
+      for (int n = 2; n <= N; ++n) {
+        const double tn = T[n] * dn / (n - 1.0);
+        pd += (n & 1) ? -tn : tn;
+        dn *= dd;
+      }
+      double st = dd - p / pd;
+      if (!(st > dl && st < dh) || fabs(st) > rad) { ok = false; dd = st; break; }
+      dd = st;
+      // p(d) = g + gp d + sum (-1)^n T_n d^n / (n(n-1))
+      double pv = g + gp * dd, dn2 = dd * dd;
+#pragma unroll
+      for (int n = 2; n <= N; ++n) {
+        const double tn = T[n] * dn2 / (n * (n - 1.0));
+        pv += (n & 1) ? -tn : tn;
+        dn2 *= dd;
+      }
+      p = pv;
+      if (fabs(p) <= 0.25 * tol * S2) break;
+    }
+    const double q = fabs(dd) * zmax;
+    double rem = 1e300;
+    if (ok && q < 0.5) {
+      double qp = 1.0;
+#pragma unroll
+      for (int n = 0; n < N - 1; ++n) qp *= q;
+      rem = T[2] * dd * dd * qp / (N * (N + 1.0) * (1.0 - q));
+    }
+    if (ok && fabs(p) + rem <= tol * S2) { r.alpha = a + dd; r.iters = k + 1; return r; }
+    // next centre: the polynomial's step where it stays in the bracket and the
+    // trust radius, else the bound or the middle of the bracket
+    double an = a + (fabs(dd) > rad ? (dd > 0.0 ? rad : -rad) : dd);
     if (!(an > lo && an < hi)) an = (hi_is_bound && an >= hi) ? amax : 0.5 * (lo + hi);
     if (an == amax) hi_is_bound = false;
     a = an;

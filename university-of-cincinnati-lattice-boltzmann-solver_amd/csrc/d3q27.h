@@ -108,16 +108,12 @@ __device__ __forceinline__ void prep27_shifted(const float (&f)[27], float (&D)[
 }
 
 // ---------------------------------------------------------------- FP32 shifted
-// f[i] holds f~_i on entry and fout~_i on return.  Nodes whose max|x| lies
-// beyond the in-register series tiers need the logarithmic Newton: with
-// kInlineGeneral false (streaming kernel) the node returns ALPHA_DEFER with
-// f[] untouched and, in `alpha`, the root of the order-kMidOrder truncated
-// series as a starting point; the caller stores the pre-collision state and
-// queues the node, so one far lane does not stall its wavefront.  With
-// kInlineGeneral true the general path of alpha_solve.h runs in place.
-// kGeneralOnly skips the fused tiers (for nodes already known to be far from
-// equilibrium) and keeps only the general path's registers live.
-template <bool kInlineGeneral, bool kGeneralOnly = false>
+// f[i] holds f~_i on entry and fout~_i on return.  A node whose max|x| lies
+// beyond the in-register series tiers returns ALPHA_DEFER with f[] untouched
+// and, in `alpha`, the root of the order-kMidOrder truncated series as a
+// starting point; the caller stores the pre-collision state and queues the
+// node for the far-from-equilibrium kernel, so one far lane does not stall
+// its wavefront.
 __device__ __forceinline__ AlphaOut collide27_shifted(float (&f)[27], double beta,
                                                       const AlphaCfg& ac, double alpha0,
                                                       double& rho_out, double (&u_out)[3]) {
@@ -125,32 +121,7 @@ __device__ __forceinline__ AlphaOut collide27_shifted(float (&f)[27], double bet
   prep27_shifted(f, D, rho_out, u_out);
 
   AlphaOut ao{2.0, 0, ALPHA_TRIVIAL, false};
-  if constexpr (kGeneralOnly) {
-    if (ac.mode != 1) {
-      double m = 0.0, xmin = 0.0, S2 = 0.0;
-      bool nonpos = false;
-#pragma unroll
-      for (int i = 0; i < 27; ++i) {
-        const double fi = W27[i] + (double)f[i];
-        nonpos |= !(fi > 0.0);
-        const double x = (double)D[i] / fi;
-        m = fmax(m, fabs(x));
-        xmin = fmin(xmin, x);
-        S2 += (double)D[i] * x;
-      }
-      if (nonpos) {
-        ao = AlphaOut{2.0, 0, ALPHA_NONPOS, false};
-      } else {
-        auto dx = [&](int i, double& d, double& x) __attribute__((always_inline)) {
-          d = (double)D[i];
-          x = d / (W27[i] + (double)f[i]);
-        };
-        auto fx = [&](int i) __attribute__((always_inline)) { return W27[i] + (double)f[i]; };
-        ao = solve_alpha<27>(dx, fx, m, xmin, S2, ac.mode == 2 ? alpha0 : 2.0, ac.tol, ac.maxit,
-                             ac.thr);
-      }
-    }
-  } else if (ac.mode != 1) {
+  if (ac.mode != 1) {
     // One fused pass: x_i = D_i / f_i (FP32, hardware reciprocal), max|x|,
     // min x, and the power sums S_n = sum f x^n = sum D x^(n-1), n = 2..6,
     // with FP32 products accumulated in FP64.
@@ -199,41 +170,33 @@ __device__ __forceinline__ AlphaOut collide27_shifted(float (&f)[27], double bet
       }
     } else {
       // some lane needs more terms: the wavefront extends its sums to order
-      // kMidOrder from the populations still in registers
-      double Sh[kMidOrder > kFastOrder ? kMidOrder - kFastOrder : 1];
+      // kMidOrder from the populations still in registers.  Orders above 6
+      // are accumulated in FP32: at max|x| <= m_mid their terms of g are below
+      // 1e-6 S2, so FP32 sums leave errors far below tol * S2.
+      float Sh[kMidOrder > kFastOrder ? kMidOrder - kFastOrder : 1];
 #pragma unroll
-      for (int q = 0; q < kMidOrder - kFastOrder; ++q) Sh[q] = 0.0;
+      for (int q = 0; q < kMidOrder - kFastOrder; ++q) Sh[q] = 0.0f;
 #pragma unroll
       for (int i = 0; i < 27; ++i) {
         const float x = D[i] * frcp((float)W27[i] + f[i]);
-        float t = D[i] * x;
+        const float x2 = x * x;
+        float t = D[i] * x * (x2 * x2);  // D x^5, the order-6 term
 #pragma unroll
-        for (int q = 3; q <= kFastOrder; ++q) t *= x;
-#pragma unroll
-        for (int q = 0; q < kMidOrder - kFastOrder; ++q) { t *= x; Sh[q] += (double)t; }
+        for (int q = 0; q < kMidOrder - kFastOrder; ++q) { t *= x; Sh[q] += t; }
       }
+      double Shd[kMidOrder > kFastOrder ? kMidOrder - kFastOrder : 1];
+#pragma unroll
+      for (int q = 0; q < kMidOrder - kFastOrder; ++q) Shd[q] = (double)Sh[q];
+      // one Newton for both kinds of lane: to the tolerance within the tier,
+      // to a seed (1e-6, at most 8 updates) for a lane beyond it
+      const bool far = m > ac.thr.m_mid;
       const double a0 = ac.mode == 2 ? alpha0 : 2.0;
-      if (m <= ac.thr.m_mid) {
-        ao = newton_alpha(
-            [&](double a, double& g, double& gp) __attribute__((always_inline)) { series_g2<kFastOrder, kMidOrder>(S, Sh, a, g, gp); },
-            S[2], amax, a0, ac.tol, ac.maxit);
-      } else if constexpr (!kInlineGeneral) {
-        // seed for the deferred logarithmic Newton: root of the truncated series
-        AlphaOut sd = newton_alpha(
-            [&](double a, double& g, double& gp) __attribute__((always_inline)) {
-              series_g2<kFastOrder, kMidOrder>(S, Sh, a, g, gp);
-            },
-            S[2], amax, a0, 1e-6, 8);
-        return AlphaOut{sd.alpha, sd.iters, ALPHA_DEFER, false};
-        ao.logpath = true;
-      } else {
-        auto dx = [&](int i, double& d, double& x) __attribute__((always_inline)) {
-          d = (double)D[i];
-          x = d / (W27[i] + (double)f[i]);
-        };
-        auto fx = [&](int i) __attribute__((always_inline)) { return W27[i] + (double)f[i]; };
-        ao = solve_alpha<27>(dx, fx, (double)m, (double)xmin, S[2], a0, ac.tol, ac.maxit, ac.thr);
-      }
+      ao = newton_alpha(
+          [&](double a, double& g, double& gp) __attribute__((always_inline)) {
+            series_g2<kFastOrder, kMidOrder>(S, Shd, a, g, gp);
+          },
+          S[2], amax, a0, far ? 1e-6 : ac.tol, far ? 8 : ac.maxit);
+      if (far) return AlphaOut{ao.alpha, ao.iters, ALPHA_DEFER, false};
     }
   }
   const float ab = (float)(ao.alpha * beta);
@@ -242,60 +205,62 @@ __device__ __forceinline__ AlphaOut collide27_shifted(float (&f)[27], double bet
   return ao;
 }
 
-// Far-from-equilibrium node (queued by the streaming kernel): logarithmic
-// Newton from the series seed, then relaxation.  Same moments, equilibrium
-// and D as the streaming kernel formed for it.
+// Far-from-equilibrium node (queued by the streaming kernel): the re-centred
+// logarithmic solve of alpha_solve.h from the series seed, then relaxation.
+// Same moments, equilibrium and D as the streaming kernel formed for it.
+// f~ and D are parked in the block's LDS columns (stride 256 = blockDim) so the
+// sweeps over the 27 directions run as rolled loops with few live registers;
+// the relaxed populations are written to fout~ at node.
+constexpr int kRecentreOrder = 6;
 __device__ __forceinline__ AlphaOut collide27_far(float (&f)[27], double beta, const AlphaCfg& ac,
-                                                  double seed, double& rho_out,
-                                                  double (&u_out)[3]) {
-  float D[27];
-  prep27_shifted(f, D, rho_out, u_out);
+                                                  double seed, float* __restrict__ sf,
+                                                  float* __restrict__ sD, bool active,
+                                                  float* __restrict__ fout, unsigned nn,
+                                                  unsigned node) {
+  double rho_unused, u_unused[3];
+  AlphaOut ao;
   float xmin = 0.0f;
   double S2 = 0.0;
   bool nonpos = false;
+  {
+    float D[27];
+    prep27_shifted(f, D, rho_unused, u_unused);
 #pragma unroll
-  for (int i = 0; i < 27; ++i) {
-    const float fi = (float)W27[i] + f[i];
-    nonpos |= !(fi > 0.0f);
-    const float x = D[i] * frcp(fi);
-    xmin = fminf(xmin, x);
-    S2 += (double)(D[i] * x);
+    for (int i = 0; i < 27; ++i) {
+      const float fi = (float)W27[i] + f[i];
+      nonpos |= !(fi > 0.0f);
+      const float x = D[i] * frcp(fi);
+      xmin = fminf(xmin, x);
+      S2 += (double)(D[i] * x);
+      sf[i * 256] = f[i];
+      sD[i * 256] = D[i];
+    }
   }
   const double amax = xmin < 0.0f ? -1.0 / (double)xmin : 1e300;
-  AlphaOut ao;
   if (nonpos) {
     ao = AlphaOut{2.0, 0, ALPHA_NONPOS, false};
   } else if (amax < 2.0) {
     ao = AlphaOut{amax, 0, ALPHA_BOUND, false};
   } else {
+    auto dx = [&](int i, double& d, double& x) __attribute__((always_inline)) {
+      d = (double)sD[i * 256];
+      x = d * drcp(W27[i] + (double)sf[i * 256]);
+    };
+    auto fx = [&](int i) __attribute__((always_inline)) { return W27[i] + (double)sf[i * 256]; };
     double K = 0.0;
-#pragma clang loop unroll(full)
+#pragma unroll 1
     for (int i = 0; i < 27; ++i) {
-      const double fi = W27[i] + (double)f[i];
-      K += (double)D[i] * log1p_fast((double)D[i] / fi);
+      double d, x;
+      dx(i, d, x);
+      K += d * log1p_dev(x);
     }
-    ao = newton_alpha(
-        [&](double a, double& g, double& gp) __attribute__((always_inline)) {
-          double sg = 0.0, sgp = 0.0;
-          bool inf_gp = false;
-#pragma clang loop unroll(full)
-          for (int i = 0; i < 27; ++i) {
-            const double fi = W27[i] + (double)f[i];
-            const double y = a * ((double)D[i] / fi);
-            if (1.0 + y <= 0.0) { sg -= fi * y; inf_gp = true; continue; }
-            const double la = log1p_fast(y);
-            sg += fi * ((1.0 + y) * la - y);
-            sgp += (double)D[i] * la;
-          }
-          g = sg - a * K;
-          gp = inf_gp ? __builtin_huge_val() : sgp - K;
-        },
-        S2, amax, seed, ac.tol, ac.maxit);
-    ao.logpath = true;
+    ao = recentred_alpha<27, kRecentreOrder, 1>(dx, fx, K, S2, amax, seed, ac.tol, ac.maxit);
   }
   const float ab = (float)(ao.alpha * beta);
+  if (active) {
 #pragma unroll
-  for (int i = 0; i < 27; ++i) f[i] = fmaf(ab, D[i], f[i]);
+    for (int i = 0; i < 27; ++i) (fout + (size_t)i * nn)[node] = fmaf(ab, sD[i * 256], sf[i * 256]);
+  }
   return ao;
 }
 

@@ -56,6 +56,19 @@ __device__ __forceinline__ void wave_stats(unsigned long long* __restrict__ stat
   }
 }
 
+// appends the wavefront's deferred nodes to the far-from-equilibrium queue
+// (one atomic per wavefront)
+__device__ __forceinline__ void queue_far(const StepArgs& a, bool defer, unsigned node) {
+  const unsigned long long dm = __ballot(defer);
+  if (dm) {
+    unsigned base = 0;
+    const int leader = __ffsll((long long)dm) - 1;
+    if ((int)__lane_id() == leader) base = atomicAdd(a.defer_count, (unsigned)__popcll(dm));
+    base = __shfl(base, leader);
+    if (defer) a.defer_list[base + __popcll(dm & ((1ull << __lane_id()) - 1ull))] = node;
+  }
+}
+
 // kFlags: the level has link tables; only interior fluid nodes (flag 0) are
 // updated here, the boundary kernel updates the rest.
 template <bool kShifted, bool kKeepAlpha, bool kFlags>
@@ -148,7 +161,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   AlphaOut ao;
   // learned mode: the initial guess left in alpha[] by the guess kernel
   const double a0 = (kShifted && a.alpha_mode == 2 && active) ? alpha_out[node] : 2.0;
-  if constexpr (kShifted) ao = collide27_shifted<false>(f, a.beta, ac, a0, rho, u);
+  if constexpr (kShifted) ao = collide27_shifted(f, a.beta, ac, a0, rho, u);
   else ao = collide27_fp64(f, a.beta, ac, 2.0, rho, u);
 
   if (active) {
@@ -158,17 +171,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     for (int i = 0; i < 27; ++i) (fout + (size_t)i * nn)[node] = f[i];
     if (kKeepAlpha || ao.status == ALPHA_DEFER) alpha_out[node] = ao.alpha;
   }
-  if constexpr (kShifted) {
-    const bool defer = active && ao.status == ALPHA_DEFER;
-    const unsigned long long dm = __ballot(defer);
-    if (dm) {
-      unsigned base = 0;
-      const int leader = __ffsll((long long)dm) - 1;
-      if ((int)__lane_id() == leader) base = atomicAdd(a.defer_count, (unsigned)__popcll(dm));
-      base = __shfl(base, leader);
-      if (defer) a.defer_list[base + __popcll(dm & ((1ull << __lane_id()) - 1ull))] = node;
-    }
-  }
+  if constexpr (kShifted) queue_far(a, active && ao.status == ALPHA_DEFER, node);
   // deferred nodes report their seed iterations here and the rest in the far kernel
   wave_stats(stats, active, ao.iters, ao.status == ALPHA_DEFER ? -1 : ao.status, ao.fallback,
              ao.logpath);
@@ -191,6 +194,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   const unsigned first = blockIdx.x * blockDim.x;
   const unsigned nn = (unsigned)b.n;
   AlphaCfg ac{a.tol, a.maxit, a.alpha_mode, a.thr};
+  __shared__ float sf[27 * 256], sD[27 * 256];
   // grid-stride over the queue; whole blocks past its end leave at once
   for (unsigned base = first; base < count; base += stride) {
     const unsigned k = base + threadIdx.x;
@@ -200,13 +204,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
     for (int i = 0; i < 27; ++i) f[i] = (f_ + (size_t)i * nn)[node];
     const double seed = alpha[node];
-    double rho, u[3];
-    AlphaOut ao = collide27_far(f, a.beta, ac, seed, rho, u);
-    if (active) {
-#pragma unroll
-      for (int i = 0; i < 27; ++i) (f_ + (size_t)i * nn)[node] = f[i];
-      if (keep_alpha) alpha[node] = ao.alpha;
-    }
+    AlphaOut ao = collide27_far(f, a.beta, ac, seed, sf + threadIdx.x, sD + threadIdx.x, active,
+                                f_, nn, node);
+    if (active && keep_alpha) alpha[node] = ao.alpha;
     wave_stats(stats, active, ao.iters, ao.status, false, ao.logpath);
   }
 }
@@ -361,10 +361,33 @@ __device__ void feq27_fp64(double rho, const double u[3], double* feq) {
     feq[i] = rho * W27[i] * p[0][CX27[i] + 1] * p[1][CY27[i] + 1] * p[2][CZ27[i] + 1];
 }
 
+// u(x - c_q) for every boundary-list node x and direction q (q = 0: x itself),
+// one thread per pair, so the boundary collide reads its interpolation and
+// gradient velocities instead of forming them one neighbour after another.
+// Sources beyond a non-periodic face wrap; their codes never ask for them.
 template <bool kShifted>
-__global__ __launch_bounds__(256) void k27_boundary(const void* __restrict__ fin,
+__global__ __launch_bounds__(256) void k27_bnd_vel(const void* __restrict__ fin, Box b, BcArgs bc) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t np = bc.nlist * 27;
+  if (t >= np) return;
+  const int q = (int)(t % 27);
+  const int64_t k = bc.list[t / 27];
+  const int x = (int)(k % b.nx);
+  const int64_t yz = k / b.nx;
+  const int y = (int)(yz % b.ny), z = (int)(yz / b.ny);
+  const int64_t src = wrapc(x - CX27[q], b.nx) +
+                      (int64_t)b.nx * (wrapc(y - CY27[q], b.ny) + (int64_t)b.ny * wrapc(z - CZ27[q], b.nz));
+  double u[3];
+  vel_at<kShifted>(fin, b.n, src, u);
+  bc.vel[t] = u[0];
+  bc.vel[np + t] = u[1];
+  bc.vel[2 * np + t] = u[2];
+}
+
+template <bool kShifted>
+__global__ __launch_bounds__(64) void k27_boundary(const void* __restrict__ fin,
                                                     void* __restrict__ fout_,
-                                                    double* __restrict__ alpha_out,
+                                                    double* __restrict__ alpha, int keep_alpha,
                                                     unsigned long long* __restrict__ stats,
                                                     Box b, StepArgs a, BcArgs bc) {
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -404,21 +427,23 @@ __global__ __launch_bounds__(256) void k27_boundary(const void* __restrict__ fin
   }
   int nibb = 0, nrep = 0;
   if (any_wall && active) {
-    double ut[3] = {0.0, 0.0, 0.0}, ux[3];
+    // u(x - c_q) of this node, formed by k27_bnd_vel
+    const int64_t np = bc.nlist * 27;
+    auto vel = [&](int q, double u[3]) __attribute__((always_inline)) {
+      const int64_t t = j * 27 + q;
+      u[0] = bc.vel[t];
+      u[1] = bc.vel[np + t];
+      u[2] = bc.vel[2 * np + t];
+    };
+    double ut[3] = {0.0, 0.0, 0.0};
     int cnt = 0;
-    vel_at<kShifted>(fin, n, k, ux);
+#pragma unroll 1
     for (int kk = 1; kk < 27; ++kk) {
       const int32_t s = bc.state[(int64_t)kk * n + k];
       if (s > -1001) continue;
       const double q = fabs((double)s + 1001.0) / 1.0e9;
       double uu[3];
-      if (bc.state[(int64_t)OPP27[kk] * n + k] >= 0) {
-        const int64_t src = wrapc(x - CX27[kk], b.nx) +
-                            (int64_t)b.nx * (wrapc(y - CY27[kk], b.ny) + (int64_t)b.ny * wrapc(z - CZ27[kk], b.nz));
-        vel_at<kShifted>(fin, n, src, uu);
-      } else {
-        uu[0] = ux[0]; uu[1] = ux[1]; uu[2] = ux[2];
-      }
+      vel(bc.state[(int64_t)OPP27[kk] * n + k] >= 0 ? kk : 0, uu);
       for (int d = 0; d < 3; ++d) ut[d] += q * uu[d] / (q + 1.0);
       cnt++;
     }
@@ -433,11 +458,11 @@ __global__ __launch_bounds__(256) void k27_boundary(const void* __restrict__ fin
     for (int ax = 0; ax < 3; ++ax) {
       const int dp = 1 + ax, dm = 4 + ax;
       const int32_t sp = bc.state[(int64_t)dp * n + k], sm = bc.state[(int64_t)dm * n + k];
-      const int ex = ax == 0, ey = ax == 1, ez = ax == 2;
       double hp, hm, up[3], um[3];
+      // x + e_a = x - c_dm, x - e_a = x - c_dp
       if (sp >= 0) {
         hp = 1.0;
-        vel_at<kShifted>(fin, n, wrapc(x + ex, b.nx) + (int64_t)b.nx * (wrapc(y + ey, b.ny) + (int64_t)b.ny * wrapc(z + ez, b.nz)), up);
+        vel(dm, up);
       } else if (sp <= -1001) {
         hp = fabs((double)sp + 1001.0) / 1.0e9;
         up[0] = up[1] = up[2] = 0.0;
@@ -447,7 +472,7 @@ __global__ __launch_bounds__(256) void k27_boundary(const void* __restrict__ fin
       }
       if (sm >= 0) {
         hm = 1.0;
-        vel_at<kShifted>(fin, n, wrapc(x - ex, b.nx) + (int64_t)b.nx * (wrapc(y - ey, b.ny) + (int64_t)b.ny * wrapc(z - ez, b.nz)), um);
+        vel(dp, um);
       } else if (sm <= -1001) {
         hm = fabs((double)sm + 1001.0) / 1.0e9;
         um[0] = um[1] = um[2] = 0.0;
@@ -457,9 +482,16 @@ __global__ __launch_bounds__(256) void k27_boundary(const void* __restrict__ fin
       }
       for (int c2 = 0; c2 < 3; ++c2) G[ax][c2] = (up[c2] - um[c2]) / (hp + hm);
     }
-    double feq_t[27];
-    feq27_fp64(rt, ut, feq_t);
+    double pt[3][3];  // 1-D factors of feq(rt, ut)
+    for (int d = 0; d < 3; ++d) {
+      const double sq = sqrt(1.0 + 3.0 * ut[d] * ut[d]);
+      const double bb = 2.0 - sq;
+      pt[d][1] = bb;
+      pt[d][2] = bb * (2.0 * ut[d] + sq) / (1.0 - ut[d]);
+      pt[d][0] = bb * (1.0 - ut[d]) / (2.0 * ut[d] + sq);
+    }
     const double tau = 1.0 / (2.0 * a.beta);
+#pragma unroll
     for (int i = 1; i < 27; ++i) {
       if (!unknown[i]) continue;
       const int c[3] = {CX27[i], CY27[i], CZ27[i]};
@@ -467,7 +499,9 @@ __global__ __launch_bounds__(256) void k27_boundary(const void* __restrict__ fin
       for (int p = 0; p < 3; ++p)
         for (int r = 0; r < 3; ++r)
           qs += ((double)(c[p] * c[r]) - (p == r ? 1.0 / 3.0 : 0.0)) * G[p][r];
-      double v = feq_t[OPP27[i]] - 3.0 * tau * rt * W27[i] * qs;
+      const int o = OPP27[i];
+      const double feq_o = rt * W27[o] * pt[0][CX27[o] + 1] * pt[1][CY27[o] + 1] * pt[2][CZ27[o] + 1];
+      double v = feq_o - 3.0 * tau * rt * W27[i] * qs;
       if (v < 0.0) { v = -v; nrep++; }
       f[i] = v;
       nibb++;
@@ -477,23 +511,29 @@ __global__ __launch_bounds__(256) void k27_boundary(const void* __restrict__ fin
   double rho, u[3];
   AlphaOut ao;
   if constexpr (kShifted) {
+    // the streaming kernel's collide: a node beyond the series tiers stores
+    // its pre-collision state and joins the far-from-equilibrium queue
     float fs[27];
 #pragma unroll
     for (int i = 0; i < 27; ++i) fs[i] = (float)(f[i] - W27[i]);
-    ao = collide27_shifted<true>(fs, a.beta, ac, 2.0, rho, u);
+    ao = collide27_shifted(fs, a.beta, ac, 2.0, rho, u);
     if (active) {
 #pragma unroll
       for (int i = 0; i < 27; ++i) static_cast<float*>(fout_)[(int64_t)i * n + k] = fs[i];
     }
+    const bool defer = active && ao.status == ALPHA_DEFER;
+    if (active && (keep_alpha || defer)) alpha[k] = ao.alpha;
+    queue_far(a, defer, (unsigned)k);
   } else {
     ao = collide27_fp64(f, a.beta, ac, 2.0, rho, u);
     if (active) {
 #pragma unroll
       for (int i = 0; i < 27; ++i) static_cast<double*>(fout_)[(int64_t)i * n + k] = f[i];
     }
+    if (active && keep_alpha) alpha[k] = ao.alpha;
   }
-  if (active && alpha_out) alpha_out[k] = ao.alpha;
-  wave_stats(stats, active, ao.iters, ao.status, false, ao.logpath);
+  wave_stats(stats, active, ao.iters, ao.status == ALPHA_DEFER ? -1 : ao.status, false,
+             ao.logpath);
   if (active && (nibb || nrep)) {
     unsigned long long* sl = stats + (size_t)((blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64) &
                                               (kStatSlots - 1)) * kStatFields;
@@ -554,12 +594,16 @@ hipError_t launch_d3q27_step_links(const Box& b, const void* fin, void* fout, do
 }
 
 hipError_t launch_d3q27_boundary(const Box& b, const void* fin, void* fout, double* alpha,
-                                 unsigned long long* stats, const StepArgs& a, const BcArgs& bc,
-                                 hipStream_t s) {
+                                 int keep_alpha, unsigned long long* stats, const StepArgs& a,
+                                 const BcArgs& bc, hipStream_t s) {
   if (bc.nlist == 0) return hipSuccess;
+  if (!bc.vel) return hipErrorInvalidValue;
+  const unsigned gv = blocks_for(bc.nlist * 27, 256);
+  if (a.shifted) k27_bnd_vel<true><<<gv, 256, 0, s>>>(fin, b, bc);
+  else k27_bnd_vel<false><<<gv, 256, 0, s>>>(fin, b, bc);
   const unsigned gb = blocks_for(bc.nlist, 64);
-  if (a.shifted) k27_boundary<true><<<gb, 64, 0, s>>>(fin, fout, alpha, stats, b, a, bc);
-  else k27_boundary<false><<<gb, 64, 0, s>>>(fin, fout, alpha, stats, b, a, bc);
+  if (a.shifted) k27_boundary<true><<<gb, 64, 0, s>>>(fin, fout, alpha, keep_alpha, stats, b, a, bc);
+  else k27_boundary<false><<<gb, 64, 0, s>>>(fin, fout, alpha, keep_alpha, stats, b, a, bc);
   return hipGetLastError();
 }
 

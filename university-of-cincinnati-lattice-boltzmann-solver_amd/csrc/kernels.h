@@ -59,6 +59,7 @@ struct BcArgs {
   const double* feq_in;   // [n_in][27]
   const double* feq_out;  // [n_out][27]
   const double* feq_fs;   // [27] or null
+  double* vel;            // [3][nlist * 27]: u(x - c_q) of list node j at j * 27 + q
 };
 
 struct LinkGeom {
@@ -83,8 +84,8 @@ hipError_t launch_d3q27_step_links(const Box& b, const void* fin, void* fout, do
 // interior + face nodes (streaming kernel) and the boundary-list kernel; the
 // two write disjoint nodes and may run concurrently
 hipError_t launch_d3q27_boundary(const Box& b, const void* fin, void* fout, double* alpha,
-                                 unsigned long long* stats, const StepArgs& a, const BcArgs& bc,
-                                 hipStream_t s);
+                                 int keep_alpha, unsigned long long* stats, const StepArgs& a,
+                                 const BcArgs& bc, hipStream_t s);
 // queued far-from-equilibrium nodes, after the streaming kernel
 hipError_t launch_d3q27_far(const Box& b, void* fout, double* alpha, int keep_alpha,
                             unsigned long long* stats, const StepArgs& a, hipStream_t s);

@@ -52,6 +52,7 @@ struct Level {
   int32_t* state = nullptr;   // [27][n]
   uint8_t* flags = nullptr;   // [n]
   uint32_t* blist = nullptr;  // boundary fluid nodes, ascending
+  double* bvel = nullptr;     // neighbour velocities of the boundary nodes, [3][nb * 27]
   int64_t nb = 0, nsolid = 0;
   bool has_links = false;
   // D3Q39: second population, multipliers, repair list, step counter
@@ -138,6 +139,7 @@ void free_level(Level& L) {
   if (L.state) { (void)hipFree(L.state); L.state = nullptr; }
   if (L.flags) { (void)hipFree(L.flags); L.flags = nullptr; }
   if (L.blist) { (void)hipFree(L.blist); L.blist = nullptr; }
+  if (L.bvel) { (void)hipFree(L.bvel); L.bvel = nullptr; }
   for (auto& p : L.g)
     if (p) { (void)hipFree(p); p = nullptr; }
   if (L.lm) { (void)hipFree(L.lm); L.lm = nullptr; }
@@ -179,6 +181,7 @@ uclbs::BcArgs bc_args(const Ctx* c, const Level* L) {
   bc.feq_in = c->d_feq_in;
   bc.feq_out = c->d_feq_out;
   bc.feq_fs = c->d_feq_fs;
+  bc.vel = L->bvel;
   return bc;
 }
 
@@ -224,9 +227,11 @@ int finalize_links(Ctx* c, Level* L) {
     nsolid += hf[k] == uclbs::kFlagSolid;
   }
   if (L->blist) { (void)hipFree(L->blist); L->blist = nullptr; }
+  if (L->bvel) { (void)hipFree(L->bvel); L->bvel = nullptr; }
   L->nb = (int64_t)list.size();
   L->nsolid = nsolid;
   if (L->nb) {
+    HIP_TRY(hipMalloc(&L->bvel, (size_t)L->nb * 27 * 3 * sizeof(double)));
     HIP_TRY(hipMalloc(&L->blist, list.size() * sizeof(uint32_t)));
     HIP_TRY(hipMemcpy(L->blist, list.data(), list.size() * sizeof(uint32_t),
                       hipMemcpyHostToDevice));
@@ -306,15 +311,15 @@ int launch_steps(Ctx* c, Level* L, int nsteps, hipEvent_t* ev = nullptr) {
     if (!L->has_links && !(L->box.per[0] && L->box.per[1] && L->box.per[2]))
       return set_err(UCLBS_E_UNSUPPORTED,
                      "level without link tables must be periodic on all axes");
-    double* al = c->keep_alpha ? L->alpha : nullptr;
     // learned mode: the streaming kernel reads its start from alpha[]
     const int keep = c->keep_alpha || learned;
     if (L->has_links && L->nb) {
-      // boundary-list nodes on the side stream, concurrent with the streaming kernel
+      // boundary-list nodes on the side stream, concurrent with the streaming
+      // kernel: neighbour velocities, then the boundary collide
       HIP_TRY(hipEventRecord(c->ev_fork, c->stream));
       HIP_TRY(hipStreamWaitEvent(c->side, c->ev_fork, 0));
-      HIP_TRY(uclbs::launch_d3q27_boundary(L->box, L->f[L->cur], L->f[1 - L->cur], al, c->d_stats,
-                                           a, bc_args(c, L), c->side));
+      HIP_TRY(uclbs::launch_d3q27_boundary(L->box, L->f[L->cur], L->f[1 - L->cur], L->alpha,
+                                           keep, c->d_stats, a, bc_args(c, L), c->side));
       HIP_TRY(hipEventRecord(c->ev_join, c->side));
     }
     if (ev) HIP_TRY(hipEventRecord(ev[2 * s], c->stream));
@@ -326,10 +331,11 @@ int launch_steps(Ctx* c, Level* L, int nsteps, hipEvent_t* ev = nullptr) {
       HIP_TRY(uclbs::launch_d3q27_step_periodic(L->box, L->f[L->cur], L->f[1 - L->cur], L->alpha,
                                                 keep, c->d_stats, a, c->stream));
     if (ev) HIP_TRY(hipEventRecord(ev[2 * s + 1], c->stream));
+    // the far kernel takes the queue of both kernels
+    if (L->has_links && L->nb) HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_join, 0));
     if (a.shifted)
       HIP_TRY(uclbs::launch_d3q27_far(L->box, L->f[1 - L->cur], L->alpha, c->keep_alpha,
                                       c->d_stats, a, c->stream));
-    if (L->has_links && L->nb) HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_join, 0));
     L->cur = 1 - L->cur;
     c->steps_done++;
   }
