@@ -44,8 +44,8 @@ constexpr double kTrivialX = 1e-12;
 // Largest max|x| for which the order-N truncation of the power-sum series is
 // used.  The remainder of g relative to S2 is about (2.2 m)^(N-1) / (N(N-1))
 // for Newton iterates a <= 2.2; the host sets the bound from a target eps
-// (1e-16 in FP64 mode, i.e. exact to round-off; 0.1 * tol in FP32 mode, so
-// truncation moves the root by less than a tenth of the Newton tolerance).
+// (1e-16 in FP64 mode, i.e. exact to round-off; tol / 2 in FP32 mode, so
+// truncation adds at most half the Newton tolerance to the residual of g).
 struct SeriesThr {
   double m5, m7, m10, m16;  // general path orders
   float m_fast;             // order kFastOrder of the fused FP32-storage pass

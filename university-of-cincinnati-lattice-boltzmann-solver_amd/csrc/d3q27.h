@@ -123,24 +123,34 @@ __device__ __forceinline__ AlphaOut collide27_shifted(float (&f)[27], double bet
   AlphaOut ao{2.0, 0, ALPHA_TRIVIAL, false};
   if (ac.mode != 1) {
     // One fused pass: x_i = D_i / f_i (FP32, hardware reciprocal), max|x|,
-    // min x, and the power sums S_n = sum f x^n = sum D x^(n-1), n = 2..6,
-    // with FP32 products accumulated in FP64.
+    // min x, and the power sums S_n = sum f x^n = sum D x^(n-1), n = 2..6:
+    // FP32 products summed in FP32 over three groups of nine directions, the
+    // group sums in FP64 (the summation adds errors of the size the FP32
+    // terms already carry, and no O(1) quantity is summed in FP32).
     float m = 0.0f, xmin = 0.0f;
     bool nonpos = false;
     double S[kFastOrder + 1];
 #pragma unroll
     for (int n = 2; n <= kFastOrder; ++n) S[n] = 0.0;
 #pragma unroll
-    for (int i = 0; i < 27; ++i) {
-      const float fi = (float)W27[i] + f[i];
-      nonpos |= !(fi > 0.0f);
-      const float x = D[i] * frcp(fi);
-      m = fmaxf(m, fabsf(x));
-      xmin = fminf(xmin, x);
-      float t = D[i] * x;
-      S[2] += (double)t;
+    for (int g = 0; g < 3; ++g) {
+      float P[kFastOrder + 1];
 #pragma unroll
-      for (int n = 3; n <= kFastOrder; ++n) { t *= x; S[n] += (double)t; }
+      for (int n = 2; n <= kFastOrder; ++n) P[n] = 0.0f;
+#pragma unroll
+      for (int i = 9 * g; i < 9 * g + 9; ++i) {
+        const float fi = (float)W27[i] + f[i];
+        nonpos |= !(fi > 0.0f);
+        const float x = D[i] * frcp(fi);
+        m = fmaxf(m, fabsf(x));
+        xmin = fminf(xmin, x);
+        float t = D[i] * x;
+        P[2] += t;
+#pragma unroll
+        for (int n = 3; n <= kFastOrder; ++n) { t *= x; P[n] += t; }
+      }
+#pragma unroll
+      for (int n = 2; n <= kFastOrder; ++n) S[n] += (double)P[n];
     }
     const double amax = xmin < 0.0f ? -1.0 / (double)xmin : 1e300;
     if (nonpos) {
@@ -208,10 +218,12 @@ __device__ __forceinline__ AlphaOut collide27_shifted(float (&f)[27], double bet
 // Far-from-equilibrium node (queued by the streaming kernel): the re-centred
 // logarithmic solve of alpha_solve.h from the series seed, then relaxation.
 // Same moments, equilibrium and D as the streaming kernel formed for it.
-// f~ and D are parked in the block's LDS columns (stride 256 = blockDim) so the
-// sweeps over the 27 directions run as rolled loops with few live registers;
-// the relaxed populations are written to fout~ at node.
+// f~ and D are parked in the block's LDS columns (stride kFarBlock = blockDim)
+// so the sweeps over the 27 directions run as loops unrolled by three (three
+// independent logarithms in flight) with few live registers; the relaxed
+// populations are written to fout~ at node.
 constexpr int kRecentreOrder = 6;
+constexpr int kFarBlock = 64;
 __device__ __forceinline__ AlphaOut collide27_far(float (&f)[27], double beta, const AlphaCfg& ac,
                                                   double seed, float* __restrict__ sf,
                                                   float* __restrict__ sD, bool active,
@@ -232,8 +244,8 @@ __device__ __forceinline__ AlphaOut collide27_far(float (&f)[27], double beta, c
       const float x = D[i] * frcp(fi);
       xmin = fminf(xmin, x);
       S2 += (double)(D[i] * x);
-      sf[i * 256] = f[i];
-      sD[i * 256] = D[i];
+      sf[i * kFarBlock] = f[i];
+      sD[i * kFarBlock] = D[i];
     }
   }
   const double amax = xmin < 0.0f ? -1.0 / (double)xmin : 1e300;
@@ -243,23 +255,23 @@ __device__ __forceinline__ AlphaOut collide27_far(float (&f)[27], double beta, c
     ao = AlphaOut{amax, 0, ALPHA_BOUND, false};
   } else {
     auto dx = [&](int i, double& d, double& x) __attribute__((always_inline)) {
-      d = (double)sD[i * 256];
-      x = d * drcp(W27[i] + (double)sf[i * 256]);
+      d = (double)sD[i * kFarBlock];
+      x = d * drcp(W27[i] + (double)sf[i * kFarBlock]);
     };
-    auto fx = [&](int i) __attribute__((always_inline)) { return W27[i] + (double)sf[i * 256]; };
+    auto fx = [&](int i) __attribute__((always_inline)) { return W27[i] + (double)sf[i * kFarBlock]; };
     double K = 0.0;
-#pragma unroll 1
+#pragma unroll 3
     for (int i = 0; i < 27; ++i) {
       double d, x;
       dx(i, d, x);
       K += d * log1p_dev(x);
     }
-    ao = recentred_alpha<27, kRecentreOrder, 1>(dx, fx, K, S2, amax, seed, ac.tol, ac.maxit);
+    ao = recentred_alpha<27, kRecentreOrder, 3>(dx, fx, K, S2, amax, seed, ac.tol, ac.maxit);
   }
   const float ab = (float)(ao.alpha * beta);
   if (active) {
 #pragma unroll
-    for (int i = 0; i < 27; ++i) (fout + (size_t)i * nn)[node] = fmaf(ab, sD[i * 256], sf[i * 256]);
+    for (int i = 0; i < 27; ++i) (fout + (size_t)i * nn)[node] = fmaf(ab, sD[i * kFarBlock], sf[i * kFarBlock]);
   }
   return ao;
 }

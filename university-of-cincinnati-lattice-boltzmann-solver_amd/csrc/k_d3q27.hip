@@ -56,6 +56,24 @@ __device__ __forceinline__ void wave_stats(unsigned long long* __restrict__ stat
   }
 }
 
+// f[i * nn + k] through a wave-uniform population base and a 32-bit byte
+// offset (FP32: boxes hold under 2^30 nodes), which the compiler issues as one
+// global load / store with a scalar base and no per-lane 64-bit address
+template <typename T>
+__device__ __forceinline__ const T& pop(const T* __restrict__ f, unsigned i, unsigned nn,
+                                        unsigned k) {
+  if constexpr (sizeof(T) == 4) {
+    const char* base = reinterpret_cast<const char*>(f + (size_t)i * nn);
+    return *reinterpret_cast<const T*>(base + k * 4u);
+  } else {
+    return f[(size_t)i * nn + k];
+  }
+}
+template <typename T>
+__device__ __forceinline__ T& pop(T* __restrict__ f, unsigned i, unsigned nn, unsigned k) {
+  return const_cast<T&>(pop(const_cast<const T*>(f), i, nn, k));
+}
+
 // appends the wavefront's deferred nodes to the far-from-equilibrium queue
 // (one atomic per wavefront)
 __device__ __forceinline__ void queue_far(const StepArgs& a, bool defer, unsigned node) {
@@ -113,10 +131,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 
   T f[27];
 #pragma unroll
-  for (int i = 0; i < 27; ++i) {
-    const T* __restrict__ base = fin + (size_t)i * nn;
-    f[i] = base[ox[CX27[i] + 1] + oy[CY27[i] + 1] + oz[CZ27[i] + 1]];
-  }
+  for (int i = 0; i < 27; ++i) f[i] = pop(fin, i, nn, ox[CX27[i] + 1] + oy[CY27[i] + 1] + oz[CZ27[i] + 1]);
   // Nodes whose only boundary links leave the box through a face: the
   // population arriving from outside takes the face's code, the same code the
   // link table holds (first non-periodic face in the order x-, y-, z-, x+, y+,
@@ -148,7 +163,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             if constexpr (kShifted) v = (T)(tab[i] - W27[i]);
             else v = (T)tab[i];
           } else {
-            v = on[q] ? (fin + (size_t)OPP27[i] * nn)[node] : f[i];
+            v = on[q] ? pop(fin, OPP27[i], nn, node) : f[i];
           }
           if (on[q]) f[i] = v;
         }
@@ -168,7 +183,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     // a deferred node stores its pre-collision state; the far-from-equilibrium
     // kernel collides it in place
 #pragma unroll
-    for (int i = 0; i < 27; ++i) (fout + (size_t)i * nn)[node] = f[i];
+    for (int i = 0; i < 27; ++i) pop(fout, i, nn, node) = f[i];
     if (kKeepAlpha || ao.status == ALPHA_DEFER) alpha_out[node] = ao.alpha;
   }
   if constexpr (kShifted) queue_far(a, active && ao.status == ALPHA_DEFER, node);
@@ -177,16 +192,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
              ao.logpath);
 }
 
-// Far-from-equilibrium nodes queued by the streaming kernel (max|x| beyond
-// the in-register series): one thread per queued node reads the
-// pre-collision state stored in place, re-forms D, and runs the logarithmic
-// Newton from the series seed left in alpha[]; then relaxes and stores.
-// The queue length stays on the device; the grid covers the box and blocks
-// past the length exit at once.  The last block to finish clears the length.
-// Two waves per SIMD (256 registers each, a few spilled) run faster here than
-// one wave with the fully unrolled logarithm loop in registers (measured on
-// the C2 duct: 0.05 ms less per step).
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k27_far(float* __restrict__ f_, double* __restrict__ alpha,
+// Far-from-equilibrium nodes queued by the streaming and boundary kernels
+// (max|x| beyond the in-register series): one thread per queued node reads
+// the pre-collision state stored in place, re-forms D, and solves for alpha by
+// the re-centred logarithmic sweeps from the series seed left in alpha[]; then
+// relaxes and stores.  The queue length stays on the device; the grid is
+// capped and strides over the queue, and blocks past its end leave at once.
+// k_clear_count resets the length afterwards on the same stream.
+// One-wave blocks: the 13.8 KB of LDS per block lets three waves share a SIMD.
+__global__ __launch_bounds__(kFarBlock) void k27_far(float* __restrict__ f_, double* __restrict__ alpha,
                                                unsigned long long* __restrict__ stats, Box b,
                                                StepArgs a, int keep_alpha) {
   const unsigned count = *(volatile unsigned*)a.defer_count;
@@ -194,7 +208,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
   const unsigned first = blockIdx.x * blockDim.x;
   const unsigned nn = (unsigned)b.n;
   AlphaCfg ac{a.tol, a.maxit, a.alpha_mode, a.thr};
-  __shared__ float sf[27 * 256], sD[27 * 256];
+  __shared__ float sf[27 * kFarBlock], sD[27 * kFarBlock];
   // grid-stride over the queue; whole blocks past its end leave at once
   for (unsigned base = first; base < count; base += stride) {
     const unsigned k = base + threadIdx.x;
@@ -616,9 +630,9 @@ hipError_t launch_d3q27_init_uniform(const Box& b, void* f, int shifted, const u
 
 hipError_t launch_d3q27_far(const Box& b, void* fout, double* alpha, int keep_alpha,
                             unsigned long long* stats, const StepArgs& a, hipStream_t s) {
-  // persistent-style grid: 8 blocks per CU of a 256-CU device, grid-stride over the queue
-  const unsigned g = (unsigned)std::min<int64_t>(blocks_for(b.n, 256), 2048);
-  k27_far<<<g, 256, 0, s>>>(static_cast<float*>(fout), alpha, stats, b, a, keep_alpha);
+  // persistent-style grid: 32 waves per CU of a 256-CU device, grid-stride over the queue
+  const unsigned g = (unsigned)std::min<int64_t>(blocks_for(b.n, kFarBlock), 8192);
+  k27_far<<<g, kFarBlock, 0, s>>>(static_cast<float*>(fout), alpha, stats, b, a, keep_alpha);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   k_clear_count<<<1, 1, 0, s>>>(a.defer_count);

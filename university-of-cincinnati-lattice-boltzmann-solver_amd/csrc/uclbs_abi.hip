@@ -164,9 +164,9 @@ uclbs::StepArgs step_args(const Ctx* c, int lvl = 0) {
   a.maxit = c->cfg.alpha_maxit;
   a.alpha_mode = c->cfg.alpha_mode;
   a.shifted = c->cfg.precision == UCLBS_PREC_FP32_SHIFTED;
-  // FP64 storage: series exact to round-off; FP32 storage: truncation below
-  // a tenth of the Newton tolerance
-  a.thr = uclbs::make_series_thr(a.shifted ? 0.1 * a.tol : 1e-16);
+  // FP64 storage: series exact to round-off.  FP32 storage: the truncation of
+  // the series tiers adds at most tol / 2 to the Newton residual bound tol
+  a.thr = uclbs::make_series_thr(a.shifted ? 0.5 * a.tol : 1e-16);
   return a;
 }
 
