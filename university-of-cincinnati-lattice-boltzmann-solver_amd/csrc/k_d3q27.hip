@@ -74,16 +74,20 @@ __device__ __forceinline__ T& pop(T* __restrict__ f, unsigned i, unsigned nn, un
   return const_cast<T&>(pop(const_cast<const T*>(f), i, nn, k));
 }
 
-// appends the wavefront's deferred nodes to the far-from-equilibrium queue
-// (one atomic per wavefront)
-__device__ __forceinline__ void queue_far(const StepArgs& a, bool defer, unsigned node) {
+// appends the wavefront's deferred nodes, with their alpha seeds, to shard
+// `shard` of the far-from-equilibrium queue (one atomic per wavefront; the
+// shards keep the streaming kernel's atomics off a single address)
+__device__ __forceinline__ void queue_far(const StepArgs& a, unsigned shard, bool defer,
+                                          unsigned node, double seed) {
   const unsigned long long dm = __ballot(defer);
   if (dm) {
     unsigned base = 0;
     const int leader = __ffsll((long long)dm) - 1;
-    if ((int)__lane_id() == leader) base = atomicAdd(a.defer_count, (unsigned)__popcll(dm));
+    if ((int)__lane_id() == leader) base = atomicAdd(a.defer_count + shard, (unsigned)__popcll(dm));
     base = __shfl(base, leader);
-    if (defer) a.defer_list[base + __popcll(dm & ((1ull << __lane_id()) - 1ull))] = node;
+    if (defer)
+      a.defer_list[(size_t)shard * a.defer_cap + base + __popcll(dm & ((1ull << __lane_id()) - 1ull))] =
+          make_uint2(node, __float_as_uint((float)seed));
   }
 }
 
@@ -184,9 +188,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     // kernel collides it in place
 #pragma unroll
     for (int i = 0; i < 27; ++i) pop(fout, i, nn, node) = f[i];
-    if (kKeepAlpha || ao.status == ALPHA_DEFER) alpha_out[node] = ao.alpha;
+    if (kKeepAlpha) alpha_out[node] = ao.alpha;
   }
-  if constexpr (kShifted) queue_far(a, active && ao.status == ALPHA_DEFER, node);
+  if constexpr (kShifted)
+    queue_far(a, blockIdx.x % kFarShards, active && ao.status == ALPHA_DEFER, node, ao.alpha);
   // deferred nodes report their seed iterations here and the rest in the far kernel
   wave_stats(stats, active, ao.iters, ao.status == ALPHA_DEFER ? -1 : ao.status, ao.fallback,
              ao.logpath);
@@ -203,21 +208,57 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 __global__ __launch_bounds__(kFarBlock) void k27_far(float* __restrict__ f_, double* __restrict__ alpha,
                                                unsigned long long* __restrict__ stats, Box b,
                                                StepArgs a, int keep_alpha) {
-  const unsigned count = *(volatile unsigned*)a.defer_count;
+  // exclusive prefix of the shard lengths: lane l sums shards 5l .. 5l+4
+  __shared__ unsigned pre[kFarShards + 2];
+  {
+    constexpr int kPer = (kFarShards + 1 + 63) / 64;
+    static_assert(kFarBlock == 64 && kPer * 64 >= kFarShards + 1, "one wave scans the shards");
+    const int l = (int)threadIdx.x;
+    unsigned c[kPer], sum = 0;
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int s = l * kPer + q;
+      c[q] = s <= kFarShards ? a.defer_count[s] : 0u;
+      sum += c[q];
+    }
+    unsigned inc = sum;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const unsigned v = __shfl_up(inc, d);
+      if (l >= d) inc += v;
+    }
+    unsigned run = inc - sum;
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int s = l * kPer + q;
+      if (s <= kFarShards + 1) pre[s] = run;
+      run += c[q];
+    }
+    __syncthreads();
+  }
+  const unsigned count = pre[kFarShards + 1];
   const unsigned stride = gridDim.x * blockDim.x;
   const unsigned first = blockIdx.x * blockDim.x;
   const unsigned nn = (unsigned)b.n;
   AlphaCfg ac{a.tol, a.maxit, a.alpha_mode, a.thr};
   __shared__ float sf[27 * kFarBlock], sD[27 * kFarBlock];
-  // grid-stride over the queue; whole blocks past its end leave at once
+  // grid-stride over the concatenated shards; whole blocks past the end leave at once
   for (unsigned base = first; base < count; base += stride) {
     const unsigned k = base + threadIdx.x;
     const bool active = k < count;
-    const unsigned node = a.defer_list[active ? k : base];
+    const unsigned kk = active ? k : base;
+    // shard holding entry kk: the last s with pre[s] <= kk
+    int lo = 0, hi = kFarShards;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (pre[mid] <= kk) lo = mid; else hi = mid - 1;
+    }
+    const uint2 e = a.defer_list[(size_t)lo * a.defer_cap + (kk - pre[lo])];
+    const unsigned node = e.x;
     float f[27];
 #pragma unroll
     for (int i = 0; i < 27; ++i) f[i] = (f_ + (size_t)i * nn)[node];
-    const double seed = alpha[node];
+    const double seed = (double)__uint_as_float(e.y);
     AlphaOut ao = collide27_far(f, a.beta, ac, seed, sf + threadIdx.x, sD + threadIdx.x, active,
                                 f_, nn, node);
     if (active && keep_alpha) alpha[node] = ao.alpha;
@@ -225,8 +266,10 @@ __global__ __launch_bounds__(kFarBlock) void k27_far(float* __restrict__ f_, dou
   }
 }
 
-// resets the queue length for the next step (after the far kernel has read it)
-__global__ void k_clear_count(unsigned* c) { *c = 0u; }
+// resets the shard lengths for the next step (after the far kernel has read them)
+__global__ void k_clear_count(unsigned* c) {
+  if (threadIdx.x <= (unsigned)kFarShards) c[threadIdx.x] = 0u;
+}
 
 template <bool kShifted>
 __global__ void k27_from_fp64(const double* __restrict__ src, void* __restrict__ dst_, Box b) {
@@ -535,9 +578,8 @@ __global__ __launch_bounds__(64) void k27_boundary(const void* __restrict__ fin,
 #pragma unroll
       for (int i = 0; i < 27; ++i) static_cast<float*>(fout_)[(int64_t)i * n + k] = fs[i];
     }
-    const bool defer = active && ao.status == ALPHA_DEFER;
-    if (active && (keep_alpha || defer)) alpha[k] = ao.alpha;
-    queue_far(a, defer, (unsigned)k);
+    if (active && keep_alpha) alpha[k] = ao.alpha;
+    queue_far(a, kFarShards, active && ao.status == ALPHA_DEFER, (unsigned)k, ao.alpha);
   } else {
     ao = collide27_fp64(f, a.beta, ac, 2.0, rho, u);
     if (active) {
@@ -635,7 +677,7 @@ hipError_t launch_d3q27_far(const Box& b, void* fout, double* alpha, int keep_al
   k27_far<<<g, kFarBlock, 0, s>>>(static_cast<float*>(fout), alpha, stats, b, a, keep_alpha);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  k_clear_count<<<1, 1, 0, s>>>(a.defer_count);
+  k_clear_count<<<1, kFarShards + 64, 0, s>>>(a.defer_count);
   return hipGetLastError();
 }
 

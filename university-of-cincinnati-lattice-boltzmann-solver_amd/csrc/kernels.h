@@ -30,9 +30,15 @@ struct StepArgs {
   int shifted;  // 1: FP32 f - w storage, 0: FP64 full
   SeriesThr thr;
   int32_t face_code[6];  // x-, y-, z-, x+, y+, z+ (links leaving the box)
-  unsigned* defer_count;  // queue of far-from-equilibrium nodes (FP32 storage)
-  unsigned* defer_list;
+  // queue of far-from-equilibrium nodes (FP32 storage), in kFarShards + 1
+  // shards: shard s holds defer_count[s] entries from defer_list[s * defer_cap];
+  // streaming-kernel block b appends to shard b % kFarShards, the boundary
+  // kernel to shard kFarShards.  Entry: {node, FP32 bits of the alpha seed}.
+  unsigned* defer_count;
+  uint2* defer_list;
+  unsigned defer_cap;
 };
+constexpr int kFarShards = 256;
 
 // node flags of a level with link tables
 enum : uint8_t { kFlagInterior = 0, kFlagBoundary = 1, kFlagSolid = 2, kFlagFace = 3,

@@ -45,8 +45,9 @@ struct Level {
   void* f[2] = {nullptr, nullptr};
   int cur = 0;  // index of the buffer holding the current post-collision state
   double* alpha = nullptr;
-  unsigned* defer_count = nullptr;  // far-from-equilibrium queue
-  unsigned* defer_list = nullptr;
+  unsigned* defer_count = nullptr;  // far-from-equilibrium queue, kFarShards + 1 shards
+  uint2* defer_list = nullptr;
+  unsigned defer_cap = 0;
   // link tables (reference form) and the compact form the kernels use
   int32_t face_code[6] = {0, 0, 0, 0, 0, 0};
   int32_t* state = nullptr;   // [27][n]
@@ -212,6 +213,17 @@ int mirror_buffers(Ctx* c, Level* L) {
   return UCLBS_OK;
 }
 
+// far-from-equilibrium queue: kFarShards shards of the streaming kernel's
+// blocks (ceil(blocks / kFarShards) blocks of 256 nodes each) and one shard of
+// nb entries for the boundary kernel
+hipError_t alloc_far_queue(Level& L, int64_t nb) {
+  const int64_t blocks = (L.box.n + 255) / 256;
+  const int64_t cap = (blocks + uclbs::kFarShards - 1) / uclbs::kFarShards * 256;
+  if (L.defer_list) { (void)hipFree(L.defer_list); L.defer_list = nullptr; }
+  L.defer_cap = (unsigned)cap;
+  return hipMalloc(&L.defer_list, (size_t)(cap * uclbs::kFarShards + nb + 1) * sizeof(uint2));
+}
+
 // flags + ascending boundary list from the link table
 int finalize_links(Ctx* c, Level* L) {
   const int64_t n = L->box.n;
@@ -230,6 +242,7 @@ int finalize_links(Ctx* c, Level* L) {
   if (L->bvel) { (void)hipFree(L->bvel); L->bvel = nullptr; }
   L->nb = (int64_t)list.size();
   L->nsolid = nsolid;
+  HIP_TRY(alloc_far_queue(*L, L->nb));
   if (L->nb) {
     HIP_TRY(hipMalloc(&L->bvel, (size_t)L->nb * 27 * 3 * sizeof(double)));
     HIP_TRY(hipMalloc(&L->blist, list.size() * sizeof(uint32_t)));
@@ -294,6 +307,7 @@ int launch_steps(Ctx* c, Level* L, int nsteps, hipEvent_t* ev = nullptr) {
   }
   a.defer_count = L->defer_count;
   a.defer_list = L->defer_list;
+  a.defer_cap = L->defer_cap;
   for (int f = 0; f < 6; ++f) a.face_code[f] = L->face_code[f];
   const bool learned = c->cfg.alpha_mode == UCLBS_ALPHA_LEARNED;
   if (learned && !c->mlp_ready)
@@ -474,10 +488,10 @@ int uclbs_define_level(void* ctx, int lvl, const int32_t lo[3], const int32_t hi
     }
   }
   HIP_TRY(hipMalloc(&L.alpha, (size_t)L.box.n * sizeof(double)));
-  HIP_TRY(hipMalloc(&L.defer_count, sizeof(unsigned)));
-  HIP_TRY(hipMemset(L.defer_count, 0, sizeof(unsigned)));
-  HIP_TRY(hipMalloc(&L.defer_list, (size_t)L.box.n * sizeof(unsigned)));
   if (L.box.n > (1ll << 30)) return set_err(UCLBS_E_ARG, "box exceeds 2^30 nodes");
+  HIP_TRY(hipMalloc(&L.defer_count, (uclbs::kFarShards + 1) * sizeof(unsigned)));
+  HIP_TRY(hipMemset(L.defer_count, 0, (uclbs::kFarShards + 1) * sizeof(unsigned)));
+  HIP_TRY(alloc_far_queue(L, 0));
   if (c->cfg.lattice == UCLBS_LATTICE_D3Q39) {
     for (auto& p : L.g) HIP_TRY(hipMalloc(&p, bytes));
     HIP_TRY(hipMalloc(&L.lm, (size_t)L.box.n * uclbs::kM39 * sizeof(double)));
