@@ -203,11 +203,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 // the re-centred logarithmic sweeps from the series seed left in alpha[]; then
 // relaxes and stores.  The queue length stays on the device; the grid is
 // capped and strides over the queue, and blocks past its end leave at once.
-// k_clear_count resets the length afterwards on the same stream.
+// Block 0 zeroes the other set of lengths, which the next step appends to.
 // One-wave blocks: the 13.8 KB of LDS per block lets three waves share a SIMD.
 __global__ __launch_bounds__(kFarBlock) void k27_far(float* __restrict__ f_, double* __restrict__ alpha,
                                                unsigned long long* __restrict__ stats, Box b,
                                                StepArgs a, int keep_alpha) {
+  // the next step's lengths start from zero (the previous far kernel read them)
+  if (blockIdx.x == 0)
+    for (unsigned s = threadIdx.x; s <= (unsigned)kFarShards; s += blockDim.x) a.defer_next[s] = 0u;
   // exclusive prefix of the shard lengths: lane l sums shards 5l .. 5l+4
   __shared__ unsigned pre[kFarShards + 2];
   {
@@ -264,11 +267,6 @@ __global__ __launch_bounds__(kFarBlock) void k27_far(float* __restrict__ f_, dou
     if (active && keep_alpha) alpha[node] = ao.alpha;
     wave_stats(stats, active, ao.iters, ao.status, false, ao.logpath);
   }
-}
-
-// resets the shard lengths for the next step (after the far kernel has read them)
-__global__ void k_clear_count(unsigned* c) {
-  if (threadIdx.x <= (unsigned)kFarShards) c[threadIdx.x] = 0u;
 }
 
 template <bool kShifted>
@@ -675,9 +673,6 @@ hipError_t launch_d3q27_far(const Box& b, void* fout, double* alpha, int keep_al
   // persistent-style grid: 32 waves per CU of a 256-CU device, grid-stride over the queue
   const unsigned g = (unsigned)std::min<int64_t>(blocks_for(b.n, kFarBlock), 8192);
   k27_far<<<g, kFarBlock, 0, s>>>(static_cast<float*>(fout), alpha, stats, b, a, keep_alpha);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  k_clear_count<<<1, kFarShards + 64, 0, s>>>(a.defer_count);
   return hipGetLastError();
 }
 

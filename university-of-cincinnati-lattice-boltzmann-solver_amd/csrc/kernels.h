@@ -34,7 +34,10 @@ struct StepArgs {
   // shards: shard s holds defer_count[s] entries from defer_list[s * defer_cap];
   // streaming-kernel block b appends to shard b % kFarShards, the boundary
   // kernel to shard kFarShards.  Entry: {node, FP32 bits of the alpha seed}.
+  // The lengths are double-buffered per step: the far kernel of a step reads
+  // defer_count and zeroes defer_next, which the next step appends to.
   unsigned* defer_count;
+  unsigned* defer_next;
   uint2* defer_list;
   unsigned defer_cap;
 };

@@ -45,9 +45,10 @@ struct Level {
   void* f[2] = {nullptr, nullptr};
   int cur = 0;  // index of the buffer holding the current post-collision state
   double* alpha = nullptr;
-  unsigned* defer_count = nullptr;  // far-from-equilibrium queue, kFarShards + 1 shards
+  unsigned* defer_count = nullptr;  // far-from-equilibrium queue: 2 x (kFarShards + 1) lengths
   uint2* defer_list = nullptr;
   unsigned defer_cap = 0;
+  int far_parity = 0;  // which half of defer_count this step appends to
   // link tables (reference form) and the compact form the kernels use
   int32_t face_code[6] = {0, 0, 0, 0, 0, 0};
   int32_t* state = nullptr;   // [27][n]
@@ -305,7 +306,6 @@ int launch_steps(Ctx* c, Level* L, int nsteps, hipEvent_t* ev = nullptr) {
     int rc = check_face_tables(c, L);
     if (rc) return rc;
   }
-  a.defer_count = L->defer_count;
   a.defer_list = L->defer_list;
   a.defer_cap = L->defer_cap;
   for (int f = 0; f < 6; ++f) a.face_code[f] = L->face_code[f];
@@ -327,6 +327,8 @@ int launch_steps(Ctx* c, Level* L, int nsteps, hipEvent_t* ev = nullptr) {
                      "level without link tables must be periodic on all axes");
     // learned mode: the streaming kernel reads its start from alpha[]
     const int keep = c->keep_alpha || learned;
+    a.defer_count = L->defer_count + L->far_parity * (uclbs::kFarShards + 1);
+    a.defer_next = L->defer_count + (1 - L->far_parity) * (uclbs::kFarShards + 1);
     if (L->has_links && L->nb) {
       // boundary-list nodes on the side stream, concurrent with the streaming
       // kernel: neighbour velocities, then the boundary collide
@@ -347,9 +349,11 @@ int launch_steps(Ctx* c, Level* L, int nsteps, hipEvent_t* ev = nullptr) {
     if (ev) HIP_TRY(hipEventRecord(ev[2 * s + 1], c->stream));
     // the far kernel takes the queue of both kernels
     if (L->has_links && L->nb) HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_join, 0));
-    if (a.shifted)
+    if (a.shifted) {
       HIP_TRY(uclbs::launch_d3q27_far(L->box, L->f[1 - L->cur], L->alpha, c->keep_alpha,
                                       c->d_stats, a, c->stream));
+      L->far_parity ^= 1;
+    }
     L->cur = 1 - L->cur;
     c->steps_done++;
   }
@@ -489,8 +493,9 @@ int uclbs_define_level(void* ctx, int lvl, const int32_t lo[3], const int32_t hi
   }
   HIP_TRY(hipMalloc(&L.alpha, (size_t)L.box.n * sizeof(double)));
   if (L.box.n > (1ll << 30)) return set_err(UCLBS_E_ARG, "box exceeds 2^30 nodes");
-  HIP_TRY(hipMalloc(&L.defer_count, (uclbs::kFarShards + 1) * sizeof(unsigned)));
-  HIP_TRY(hipMemset(L.defer_count, 0, (uclbs::kFarShards + 1) * sizeof(unsigned)));
+  HIP_TRY(hipMalloc(&L.defer_count, 2 * (uclbs::kFarShards + 1) * sizeof(unsigned)));
+  HIP_TRY(hipMemset(L.defer_count, 0, 2 * (uclbs::kFarShards + 1) * sizeof(unsigned)));
+  L.far_parity = 0;
   HIP_TRY(alloc_far_queue(L, 0));
   if (c->cfg.lattice == UCLBS_LATTICE_D3Q39) {
     for (auto& p : L.g) HIP_TRY(hipMalloc(&p, bytes));
