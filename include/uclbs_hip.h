@@ -239,6 +239,9 @@ int uclbs_mlp_reserve(void *ctx, int64_t capacity);
  * resulting alpha as the target; *n_samples = pairs held */
 int uclbs_mlp_collect(void *ctx, int lvl, int stride, int64_t *n_samples);
 int uclbs_mlp_download_samples(void *ctx, float *X, float *y, int64_t capacity, int64_t *n);
+/* node index (in the level's FAB order) of each held pair, in the order of
+ * uclbs_mlp_download_samples -- for checking the logged features */
+int uclbs_mlp_download_sample_nodes(void *ctx, uint32_t *node, int64_t capacity, int64_t *n);
 /* Adam (beta 0.9 / 0.999; peak rate lr after a linear warm-up over 2% of the
  * steps, cosine decay to 0.05 lr) on random batches of the held pairs
  * (targets normalised to zero mean, unit variance on first use) */

@@ -4,7 +4,10 @@
   forward BF16-MFMA path vs oracle: max abs <= 0.03 max|y| + 0.01 (BF16 inputs
     and weights, FP32 accumulation)
   trained network as Newton start on the TGV box: fewer Newton updates than the
-    plain start, fields equal to the plain run within 1e-7 relative L2."""
+    plain start, fields equal to the plain run within 1e-7 relative L2.
+  logged features (FP32 on the device) vs the oracle's FP64 definition at the
+    same nodes: max abs <= 1e-4 per feature (the features are O(1); measured
+    <= 4e-6)."""
 import math
 
 import numpy as np
@@ -110,3 +113,25 @@ def test_learned_start_reduces_newton_updates(pkg, orc):
     assert it < 0.8 * ip and it < iu
     # equal field error against the FP64 oracle
     assert et < 1e-5 and et < 2 * ep + 1e-7
+
+
+def test_logged_features_match_oracle(pkg):
+    L = 24
+    s = _tgv(pkg, "newton", L)
+    s.step(0, 5)
+    f_post = s.download_f(0).reshape(27, L, L, L)  # (q, z, y, x)
+    s.mlp_reserve(L ** 3)
+    n = s.mlp_collect(0, 1)
+    assert n > L ** 3 // 2
+    X, _ = s.mlp_samples(n)
+    nodes = s.mlp_sample_nodes(n)
+    assert len(np.unique(nodes)) == n and nodes.max() < L ** 3
+    import oracle as orc_mod
+    cx, cy, cz, _, _ = orc_mod.lattice27()
+    # pre-collision populations: periodic pull f_i(x) <- f_i(x - c_i)
+    f_pre = np.stack([np.roll(f_post[i], (cz[i], cy[i], cx[i]), axis=(0, 1, 2)) for i in range(27)])
+    f_pre = f_pre.reshape(27, -1)[:, nodes]
+    phi, m = o_mlp.features27(f_pre)
+    err = np.abs(X.astype(np.float64) - phi).max(0)
+    print("logged features vs oracle, max abs per feature:", " ".join(f"{e:.1e}" for e in err))
+    assert np.all(err <= 1e-4)

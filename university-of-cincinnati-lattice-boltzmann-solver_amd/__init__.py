@@ -29,7 +29,8 @@ EXPORTS = (
     "uclbs_copy_bandwidth", "uclbs_set_gas", "uclbs_init_fields", "uclbs_upload_state",
     "uclbs_download_state", "uclbs_set_timestep", "uclbs_mlp_init", "uclbs_mlp_set_params",
     "uclbs_mlp_get_params", "uclbs_mlp_loss_grad", "uclbs_mlp_forward", "uclbs_mlp_reserve",
-    "uclbs_mlp_collect", "uclbs_mlp_download_samples", "uclbs_mlp_train", "uclbs_interlevel",
+    "uclbs_mlp_collect", "uclbs_mlp_download_samples", "uclbs_mlp_download_sample_nodes",
+    "uclbs_mlp_train", "uclbs_interlevel",
     "uclbs_init_from_coarse", "uclbs_fine_to_coarse",
 )
 MLP_PARAMS = 9217
@@ -131,6 +132,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "uclbs_mlp_reserve": (I, [P, I64]),
         "uclbs_mlp_collect": (I, [P, I, I, ctypes.POINTER(I64)]),
         "uclbs_mlp_download_samples": (I, [P, fp, fp, I64, ctypes.POINTER(I64)]),
+        "uclbs_mlp_download_sample_nodes": (I, [P, ctypes.POINTER(ctypes.c_uint32), I64,
+                                                ctypes.POINTER(I64)]),
         "uclbs_mlp_train": (I, [P, I64, I, D, U64, dp, dp]),
         "uclbs_interlevel": (I, [P, I, D]),
         "uclbs_init_from_coarse": (I, [P, I]),
@@ -332,6 +335,15 @@ class Solver:
                                                         ctypes.byref(n)))
         k = min(n.value, capacity)
         return X[:k], y[:k]
+
+    def mlp_sample_nodes(self, capacity: int) -> np.ndarray:
+        """Node index of each held pair, in the order of mlp_samples."""
+        node = np.empty(capacity, np.uint32)
+        n = ctypes.c_int64()
+        self._check(self.lib.uclbs_mlp_download_sample_nodes(
+            self.ctx, node.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), capacity,
+            ctypes.byref(n)))
+        return node[:min(n.value, capacity)]
 
     def mlp_train(self, steps: int, batch: int, lr: float, seed: int):
         l0, l1 = ctypes.c_double(), ctypes.c_double()

@@ -19,6 +19,8 @@
 
 #include <hip/hip_bf16.h>
 
+#include <algorithm>
+
 namespace uclbs {
 
 namespace {
@@ -38,51 +40,54 @@ __device__ __forceinline__ float gelu_grad(float x) {
   return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * c * (1.0f + 3.0f * 0.044715f * x * x);
 }
 
-// The 12 features of a node from its pre-collision shifted populations.
-// Returns m = max|x| (0 for a node at equilibrium or with f <= 0).
-__device__ __forceinline__ double features27(const float (&f)[27], float (&phi)[kIn]) {
+// The 12 features of a node from its pre-collision shifted populations, in
+// FP32 (they feed a BF16 network; the moments and equilibrium behind D are the
+// streaming kernel's FP64-summed ones).  Returns m = max|x| (0 for a node at
+// equilibrium or with f <= 0).
+__device__ __forceinline__ float features27(const float (&f)[27], float (&phi)[kIn]) {
   float D[27];
   double rho, u[3];
   prep27_shifted(f, D, rho, u);
   float m = 0.0f, xmin = 0.0f;
   bool nonpos = false;
-  double S[7] = {0, 0, 0, 0, 0, 0, 0};
+  float S[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int i = 0; i < 27; ++i) {
     const float fi = (float)W27[i] + f[i];
     nonpos |= !(fi > 0.0f);
-    const float x = D[i] / fi;
+    const float x = D[i] * __builtin_amdgcn_rcpf(fi);
     m = fmaxf(m, fabsf(x));
     xmin = fminf(xmin, x);
     float t = D[i] * x;
-    S[2] += (double)t;
+    S[2] += t;
 #pragma unroll
     for (int n = 3; n <= 6; ++n) {
       t *= x;
-      S[n] += (double)t;
+      S[n] += t;
     }
   }
-  if (nonpos || !(m > 1e-30f) || !(S[2] > 0.0)) {
+  if (nonpos || !(m > 1e-30f) || !(S[2] > 0.0f)) {
 #pragma unroll
     for (int j = 0; j < kIn; ++j) phi[j] = 0.0f;
-    return 0.0;
+    return 0.0f;
   }
-  const double md = m;
-  phi[0] = (float)(log10(md) * 0.25 + 1.0);
-  double mp = 1.0;
+  constexpr float kLog10_2 = 0.30102999566398120f;
+  const float mi = __builtin_amdgcn_rcpf(m), r2 = __builtin_amdgcn_rcpf(S[2]);
+  phi[0] = __builtin_amdgcn_logf(m) * (kLog10_2 * 0.25f) + 1.0f;  // log10(m) / 4 + 1
+  float q = r2;
 #pragma unroll
   for (int n = 3; n <= 6; ++n) {
-    mp *= md;
-    phi[n - 2] = (float)(S[n] / S[2] / mp);
+    q *= mi;
+    phi[n - 2] = S[n] * q;  // (S_n / S_2) / m^(n-2)
   }
-  phi[5] = xmin / m;
-  phi[6] = (float)(S[2] / (rho * md * md));
+  phi[5] = xmin * mi;
+  phi[6] = S[2] * mi * mi * __builtin_amdgcn_rcpf((float)rho);
   phi[7] = (float)(5.0 * u[0]);
   phi[8] = (float)(5.0 * u[1]);
   phi[9] = (float)(5.0 * u[2]);
   phi[10] = (float)(25.0 * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
-  phi[11] = (float)(log10(S[2]) * 0.125 + 1.0);
-  return md;
+  phi[11] = __builtin_amdgcn_logf(S[2]) * (kLog10_2 * 0.125f) + 1.0f;  // log10(S2) / 8 + 1
+  return m;
 }
 
 // pull of an interior node of a periodic box or of a level whose flag is
@@ -121,11 +126,11 @@ __global__ __launch_bounds__(256) void k27_collect(const float* __restrict__ fin
   bool take = gid < nn && (int)(gid % (unsigned)stride) == phase;
   if (take && flags) take = flags[gid] == kFlagInterior;
   float f[27], phi[kIn];
-  double m = 0.0;
+  float m = 0.0f;
   if (take) {
     pull27(fin, gid, b, f);
     m = features27(f, phi);
-    take = m > 1e-8;
+    take = m > 1e-8f;
   }
   const unsigned long long bm = __ballot(take);
   if (!bm) return;
@@ -185,21 +190,37 @@ __global__ void k_out_grad(const float* __restrict__ yhat, const float* __restri
   if (threadIdx.x == 0) *loss = red[0] / B;
 }
 
-// dW[k][o] = sum_b A[b][k] dZ[b][o], db[o] = sum_b dZ[b][o]
-__global__ void k_bwd_weights(const float* __restrict__ A, int K, const float* __restrict__ dZ,
-                              int O, int B, float* __restrict__ gW, float* __restrict__ gb) {
+// Weight gradients gW[k][o] = sum_r A[r][k] dZ[r][o] and gb[o] = sum_r dZ[r][o]
+// in two deterministic passes: each block of the first sums one chunk of
+// kRowChunk rows for 256 weights (one weight per thread), the second adds the
+// chunk sums in chunk order.
+constexpr int kRowChunk = 256;
+__global__ void k_bwd_weights_part(const float* __restrict__ A, int K, const float* __restrict__ dZ,
+                                   int O, int B, float* __restrict__ part) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = K * O + O;
+  if (t >= n) return;
+  const int r0 = (int)blockIdx.y * kRowChunk, r1 = min(B, r0 + kRowChunk);
+  float s = 0.0f;
   if (t < K * O) {
     const int k = t / O, o = t % O;
-    float s = 0.0f;
-    for (int r = 0; r < B; ++r) s = fmaf(A[r * K + k], dZ[r * O + o], s);
-    gW[t] = s;
-  } else if (t < K * O + O) {
+    for (int r = r0; r < r1; ++r) s = fmaf(A[r * K + k], dZ[r * O + o], s);
+  } else {
     const int o = t - K * O;
-    float s = 0.0f;
-    for (int r = 0; r < B; ++r) s += dZ[r * O + o];
-    gb[o] = s;
+    for (int r = r0; r < r1; ++r) s += dZ[r * O + o];
   }
+  part[(size_t)blockIdx.y * n + t] = s;
+}
+
+__global__ void k_bwd_weights_sum(const float* __restrict__ part, int K, int O, int nc,
+                                  float* __restrict__ gW, float* __restrict__ gb) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = K * O + O;
+  if (t >= n) return;
+  float s = 0.0f;
+  for (int c = 0; c < nc; ++c) s += part[(size_t)c * n + t];
+  if (t < K * O) gW[t] = s;
+  else gb[t - K * O] = s;
 }
 
 // dZprev[b][k] = (sum_o dZ[b][o] W[k][o]) * gelu'(Zprev[b][k])
@@ -282,110 +303,173 @@ __device__ __forceinline__ short bf16_bits(float x) {
   return (short)__bfloat16_as_ushort(__float2bfloat16(x));
 }
 
-// one dense layer for the wave's 64 rows: out[64][64] = act(in[64][K] W[K][64] + b),
-// in/out in LDS as bf16 bits with a row pitch of 72 (bank spread)
-constexpr int kPitch = 72;
-template <int K>
-__device__ __forceinline__ void mfma_layer(const short* __restrict__ in, const short* __restrict__ Wt,
-                                           const float* __restrict__ bias, short* __restrict__ out,
-                                           bool act, float* __restrict__ out_f) {
-  const int l = __lane_id();
-  const int r16 = l % 16, kq = 4 * (l / 16);
-#pragma unroll
-  for (int rt = 0; rt < 4; ++rt) {
-#pragma unroll
-    for (int ct = 0; ct < 4; ++ct) {
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int kk = 0; kk < K / 16; ++kk) {
-        bf16x4 a, bv;
-        const short* ap = in + (rt * 16 + r16) * kPitch + kk * 16 + kq;
-        // Wt is W transposed: Wt[o][k], so a lane's 4 k-values are contiguous
-        const short* bp = Wt + (ct * 16 + r16) * K + kk * 16 + kq;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          a[j] = ap[j];
-          bv[j] = bp[j];
-        }
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bv, acc, 0, 0, 0);
-      }
-      const int col = ct * 16 + r16;
-      const float bb = bias[col];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int row = rt * 16 + kq + j;
-        const float z = acc[j] + bb;
-        const float v = act ? gelu(z) : z;
-        if (out) out[row * kPitch + col] = bf16_bits(v);
-        if (out_f) out_f[row * kPitch + col] = v;
-      }
-    }
+// GELU for the BF16 inference path: x sigmoid(2 sqrt(2/pi)(x + 0.044715 x^3)),
+// the tanh form rewritten as x / (1 + 2^(x (a + b x^2))) with
+// a = -2 sqrt(2/pi) log2(e), b = 0.044715 a, on the hardware exp2 and
+// reciprocal (its ~1e-6 relative error is far below the BF16 rounding of the
+// activations)
+__device__ __forceinline__ float gelu_fast(float x) {
+  const float e = __builtin_amdgcn_exp2f(x * fmaf(-0.10294324f, x * x, -2.3022082f));
+  return x * __builtin_amdgcn_rcpf(1.0f + e);
+}
+
+// Inference on v_mfma_f32_16x16x16_bf16, one wavefront per 64 nodes.  Each
+// layer is computed transposed, O^T = W^T X^T: the weights are the A operand
+// (lane l: W^T[row l%16][k 4(l/16)..+3]), the activations the B operand
+// (lane l: X^T[k 4(l/16)..+3][node l%16]), and the result tile D holds
+// O^T[out 4(l/16)..+3][node l%16] -- exactly the B operand of the next layer
+// for the same lane.  So for each group of 16 nodes the three hidden layers
+// chain in registers; only the input features pass through LDS (one transpose)
+// and the 64 -> 1 output layer is a dot product reduced across the four lane
+// groups.  LDS weight layout (bf16 bits, row = output, pitch kWp):
+// W1t 64 x 16 (K padded), W2t and W3t 64 x 64.
+constexpr int kWp1 = 24, kWp = 72;
+constexpr int kWsmem = 64 * kWp1 + 2 * 64 * kWp;  // shorts
+struct MlpSmem {
+  short w[kWsmem];
+  float bias[3][64];
+  float w4[64];
+};
+
+// stage the bf16 weights (global layout of k_pack_bf16: W1t 64 x 16, W2t, W3t
+// 64 x 64, rows = outputs) and the FP32 biases / output weights in LDS
+__device__ __forceinline__ void mlp_stage(MlpSmem& sm, const short* __restrict__ Wbf,
+                                          const float* __restrict__ P) {
+  for (int t = threadIdx.x; t < 64 * 16; t += blockDim.x) sm.w[(t / 16) * kWp1 + t % 16] = Wbf[t];
+  for (int t = threadIdx.x; t < 2 * 64 * 64; t += blockDim.x) {
+    const int layer = t / 4096, v = t % 4096;
+    sm.w[64 * kWp1 + layer * 64 * kWp + (v / 64) * kWp + v % 64] = Wbf[64 * 16 + t];
+  }
+  for (int t = threadIdx.x; t < 64; t += blockDim.x) {
+    sm.bias[0][t] = P[oB1 + t];
+    sm.bias[1][t] = P[oB2 + t];
+    sm.bias[2][t] = P[oB3 + t];
+    sm.w4[t] = P[oW4 + t];
   }
 }
 
-// Network output for the wave's 64 rows of phi (one per lane); returns the
-// lane's y (normalised).  Wt1: 64 x 16 (K padded), Wt2, Wt3: 64 x 64, all
-// bf16 bits, transposed; bias / W4 / b4 FP32 from the master copy.
-__device__ __forceinline__ float mlp_wave(const float (&phi)[kIn], const short* __restrict__ Wt1,
-                                          const short* __restrict__ Wt2, const short* __restrict__ Wt3,
-                                          const float* __restrict__ P, short* __restrict__ bufA,
-                                          short* __restrict__ bufB, float* __restrict__ bufF) {
-  const int l = __lane_id();
+__device__ __forceinline__ bf16x4 pack_gelu(const f32x4& z) {
+  bf16x4 r;
 #pragma unroll
-  for (int j = 0; j < 16; ++j) bufA[l * kPitch + j] = bf16_bits(j < kIn ? phi[j] : 0.0f);
+  for (int j = 0; j < 4; ++j) r[j] = bf16_bits(gelu_fast(z[j]));
+  return r;
+}
+
+// Network output y (normalised) for the wavefront's 64 nodes; lane l passes
+// the features of node l and receives y of node l.  feat: this wavefront's
+// 64 x 16 bf16 transpose buffer (pitch kWp1).
+__device__ __forceinline__ float mlp_wave(const float (&phi)[kIn], const MlpSmem& sm,
+                                          short* __restrict__ feat, float b4) {
+  const int l = __lane_id();
+  const int r16 = l % 16, g4 = 4 * (l / 16);
+#pragma unroll
+  for (int j = 0; j < 16; ++j) feat[l * kWp1 + j] = bf16_bits(j < kIn ? phi[j] : 0.0f);
   __builtin_amdgcn_wave_barrier();
-  mfma_layer<16>(bufA, Wt1, P + oB1, bufB, true, nullptr);
+  // A operands for this lane (same for every node group)
+  bf16x4 A1[4], A2[4][4], A3[4][4];
+  const short* w2 = sm.w + 64 * kWp1;
+  const short* w3 = w2 + 64 * kWp;
+#pragma unroll
+  for (int ot = 0; ot < 4; ++ot) {
+    A1[ot] = *reinterpret_cast<const bf16x4*>(sm.w + (ot * 16 + r16) * kWp1 + g4);
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      A2[ot][kk] = *reinterpret_cast<const bf16x4*>(w2 + (ot * 16 + r16) * kWp + kk * 16 + g4);
+      A3[ot][kk] = *reinterpret_cast<const bf16x4*>(w3 + (ot * 16 + r16) * kWp + kk * 16 + g4);
+    }
+  }
+  // biases and output weights of this lane's rows 4(l/16)..+3 of each output tile
+  auto bias = [&](int layer, int ot) __attribute__((always_inline)) {
+    return *reinterpret_cast<const f32x4*>(&sm.bias[layer][ot * 16 + g4]);
+  };
+  float mine = 0.0f;
+#pragma unroll 1
+  for (int ct = 0; ct < 4; ++ct) {
+    // features 4(l/16)..+3 of node ct*16 + l%16
+    const bf16x4 x = *reinterpret_cast<const bf16x4*>(feat + (ct * 16 + r16) * kWp1 + g4);
+    bf16x4 h1[4], h2[4];
+#pragma unroll
+    for (int ot = 0; ot < 4; ++ot)
+      h1[ot] = pack_gelu(__builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A1[ot], x, bias(0, ot), 0, 0, 0));
+#pragma unroll
+    for (int ot = 0; ot < 4; ++ot) {
+      f32x4 acc = bias(1, ot);
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk)
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A2[ot][kk], h1[kk], acc, 0, 0, 0);
+      h2[ot] = pack_gelu(acc);
+    }
+    float y = 0.0f;
+#pragma unroll
+    for (int ot = 0; ot < 4; ++ot) {
+      f32x4 acc = bias(2, ot);
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk)
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A3[ot][kk], h2[kk], acc, 0, 0, 0);
+      const f32x4 w4 = *reinterpret_cast<const f32x4*>(&sm.w4[ot * 16 + g4]);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) y = fmaf(gelu_fast(acc[j]), w4[j], y);
+    }
+    // sum over the four output groups held by lanes l%16 + 16q
+    y += __shfl_xor(y, 16);
+    y += __shfl_xor(y, 32);
+    if (l / 16 == ct) mine = y;
+  }
   __builtin_amdgcn_wave_barrier();
-  mfma_layer<64>(bufB, Wt2, P + oB2, bufA, true, nullptr);
-  __builtin_amdgcn_wave_barrier();
-  mfma_layer<64>(bufA, Wt3, P + oB3, nullptr, true, bufF);
-  __builtin_amdgcn_wave_barrier();
-  float y = P[oB4];
-#pragma unroll 8
-  for (int k = 0; k < kH; ++k) y = fmaf(bufF[l * kPitch + k], P[oW4 + k], y);
-  return y;
+  return mine + b4;
 }
 
 // forward of a host-given batch through the MFMA path (checks of the layout)
-__global__ __launch_bounds__(64) void k_mlp_infer(const float* __restrict__ X, int B,
-                                                  const short* __restrict__ Wbf,
-                                                  const float* __restrict__ P,
-                                                  float* __restrict__ yhat) {
-  __shared__ short bufA[64 * kPitch], bufB[64 * kPitch];
-  __shared__ float bufF[64 * kPitch];
-  const int r = blockIdx.x * 64 + threadIdx.x;
+__global__ __launch_bounds__(256) void k_mlp_infer(const float* __restrict__ X, int B,
+                                                   const short* __restrict__ Wbf,
+                                                   const float* __restrict__ P,
+                                                   float* __restrict__ yhat) {
+  __shared__ MlpSmem sm;
+  __shared__ short feat[4][64 * kWp1];
+  mlp_stage(sm, Wbf, P);
+  __syncthreads();
+  const int r = blockIdx.x * 256 + threadIdx.x;
   float phi[kIn];
 #pragma unroll
   for (int j = 0; j < kIn; ++j) phi[j] = r < B ? X[r * kIn + j] : 0.0f;
-  const float y = mlp_wave(phi, Wbf, Wbf + 64 * 16, Wbf + 64 * 16 + 64 * 64, P, bufA, bufB, bufF);
+  const float y = mlp_wave(phi, sm, feat[threadIdx.x / 64], P[oB4]);
   if (r < B) yhat[r] = y;
 }
 
 // Initial guesses for the step: interior nodes get alpha0 = 2 + m (mean +
-// std * net(phi)); every other node 2.  Written to alpha[].
-__global__ __launch_bounds__(64) void k27_alpha_guess(const float* __restrict__ fin, Box b,
-                                                      const uint8_t* __restrict__ flags,
-                                                      const short* __restrict__ Wbf,
-                                                      const float* __restrict__ P, float ymean,
-                                                      float ystd, double* __restrict__ alpha) {
-  __shared__ short bufA[64 * kPitch], bufB[64 * kPitch];
-  __shared__ float bufF[64 * kPitch];
-  const unsigned gid = blockIdx.x * 64u + threadIdx.x;
+// std * net(phi)); every other node 2.  Written to alpha[].  Blocks of four
+// wavefronts stage the weights once and stride over 64-node tiles.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k27_alpha_guess(const float* __restrict__ fin, Box b,
+                                                       const uint8_t* __restrict__ flags,
+                                                       const short* __restrict__ Wbf,
+                                                       const float* __restrict__ P, float ymean,
+                                                       float ystd, double* __restrict__ alpha) {
+  __shared__ MlpSmem sm;
+  __shared__ short feat[4][64 * kWp1];
+  mlp_stage(sm, Wbf, P);
+  __syncthreads();
+  const float b4 = P[oB4];
   const unsigned nn = (unsigned)b.n;
-  bool ok = gid < nn;
-  if (ok && flags) ok = flags[gid] == kFlagInterior;
-  float f[27], phi[kIn];
-  double m = 0.0;
-  if (ok) {
-    pull27(fin, gid, b, f);
-    m = features27(f, phi);
-  } else {
+  const unsigned ntiles = (nn + 63u) / 64u;
+  short* fw = feat[threadIdx.x / 64];
+  for (unsigned tile = blockIdx.x * 4u + threadIdx.x / 64u; tile < ntiles; tile += gridDim.x * 4u) {
+    const unsigned gid = tile * 64u + __lane_id();
+    bool ok = gid < nn;
+    if (ok && flags) ok = flags[gid] == kFlagInterior;
+    float f[27], phi[kIn];
+    float m = 0.0f;
+    if (ok) {
+      pull27(fin, gid, b, f);
+      m = features27(f, phi);
+    } else {
 #pragma unroll
-    for (int j = 0; j < kIn; ++j) phi[j] = 0.0f;
+      for (int j = 0; j < kIn; ++j) phi[j] = 0.0f;
+    }
+    // the whole wave evaluates the network (MFMA needs all lanes)
+    const float y = mlp_wave(phi, sm, fw, b4);
+    if (gid < nn)
+      alpha[gid] = m > 1e-12f ? 2.0 + (double)m * ((double)ymean + (double)ystd * (double)y) : 2.0;
   }
-  // the whole wave evaluates the network (MFMA needs all lanes)
-  const float y = mlp_wave(phi, Wbf, Wbf + 64 * 16, Wbf + 64 * 16 + 64 * 64, P, bufA, bufB, bufF);
-  if (gid < nn) alpha[gid] = m > 1e-12 ? 2.0 + m * ((double)ymean + (double)ystd * (double)y) : 2.0;
 }
 
 // bf16 transposed copies of W1 (padded to K = 16), W2, W3 for the MFMA path
@@ -428,18 +512,25 @@ hipError_t launch_mlp_loss_grad(const float* P, const float* X, const float* y,
                                 hipStream_t st) {
   const int T = 256;
   auto nb = [&](int n) { return dim3((unsigned)((n + T - 1) / T)); };
+  const int nc = (B + kRowChunk - 1) / kRowChunk;
+  auto bwd_weights = [&](const float* A, int K, const float* dZ, int O, int Bn, float* gW, float* gb) {
+    const int n = K * O + O;
+    hipLaunchKernelGGL(k_bwd_weights_part, dim3((unsigned)((n + T - 1) / T), (unsigned)nc), dim3(T), 0,
+                       st, A, K, dZ, O, Bn, w.part);
+    hipLaunchKernelGGL(k_bwd_weights_sum, nb(n), dim3(T), 0, st, w.part, K, O, nc, gW, gb);
+  };
   hipLaunchKernelGGL(k_fwd_layer, nb(B * kH), dim3(T), 0, st, X, kIn, P + oW1, P + oB1, kH, B, 1, w.Z1, w.A1);
   hipLaunchKernelGGL(k_fwd_layer, nb(B * kH), dim3(T), 0, st, w.A1, kH, P + oW2, P + oB2, kH, B, 1, w.Z2, w.A2);
   hipLaunchKernelGGL(k_fwd_layer, nb(B * kH), dim3(T), 0, st, w.A2, kH, P + oW3, P + oB3, kH, B, 1, w.Z3, w.A3);
   hipLaunchKernelGGL(k_fwd_layer, nb(B), dim3(T), 0, st, w.A3, kH, P + oW4, P + oB4, 1, B, 0, w.Z4, w.A4);
   hipLaunchKernelGGL(k_out_grad, dim3(1), dim3(256), 0, st, w.A4, y, idx, B, w.D4, loss);
-  hipLaunchKernelGGL(k_bwd_weights, nb(kH + 1), dim3(T), 0, st, w.A3, kH, w.D4, 1, B, g + oW4, g + oB4);
+  bwd_weights(w.A3, kH, w.D4, 1, B, g + oW4, g + oB4);
   hipLaunchKernelGGL(k_bwd_input, nb(B * kH), dim3(T), 0, st, w.D4, 1, P + oW4, w.Z3, kH, B, w.D3);
-  hipLaunchKernelGGL(k_bwd_weights, nb(kH * kH + kH), dim3(T), 0, st, w.A2, kH, w.D3, kH, B, g + oW3, g + oB3);
+  bwd_weights(w.A2, kH, w.D3, kH, B, g + oW3, g + oB3);
   hipLaunchKernelGGL(k_bwd_input, nb(B * kH), dim3(T), 0, st, w.D3, kH, P + oW3, w.Z2, kH, B, w.D2);
-  hipLaunchKernelGGL(k_bwd_weights, nb(kH * kH + kH), dim3(T), 0, st, w.A1, kH, w.D2, kH, B, g + oW2, g + oB2);
+  bwd_weights(w.A1, kH, w.D2, kH, B, g + oW2, g + oB2);
   hipLaunchKernelGGL(k_bwd_input, nb(B * kH), dim3(T), 0, st, w.D2, kH, P + oW2, w.Z1, kH, B, w.D1);
-  hipLaunchKernelGGL(k_bwd_weights, nb(kIn * kH + kH), dim3(T), 0, st, X, kIn, w.D1, kH, B, g + oW1, g + oB1);
+  bwd_weights(X, kIn, w.D1, kH, B, g + oW1, g + oB1);
   return hipGetLastError();
 }
 
@@ -481,15 +572,18 @@ hipError_t launch_mlp_pack(const float* P, short* Wbf, hipStream_t st) {
 
 hipError_t launch_mlp_infer(const float* X, int B, const short* Wbf, const float* P, float* yhat,
                             hipStream_t st) {
-  hipLaunchKernelGGL(k_mlp_infer, dim3((B + 63) / 64), dim3(64), 0, st, X, B, Wbf, P, yhat);
+  hipLaunchKernelGGL(k_mlp_infer, dim3((B + 255) / 256), dim3(256), 0, st, X, B, Wbf, P, yhat);
   return hipGetLastError();
 }
 
 hipError_t launch_alpha_guess27(const Box& b, const float* fin, const uint8_t* flags,
                                 const short* Wbf, const float* P, float ymean, float ystd,
                                 double* alpha, hipStream_t st) {
-  hipLaunchKernelGGL(k27_alpha_guess, dim3((unsigned)((b.n + 63) / 64)), dim3(64), 0, st, fin, b,
-                     flags, Wbf, P, ymean, ystd, alpha);
+  // persistent-style grid: 8 blocks of four wavefronts per CU of a 256-CU device
+  const unsigned tiles = (unsigned)((b.n + 63) / 64);
+  const unsigned blocks = std::min<unsigned>((tiles + 3) / 4, 2048u);
+  hipLaunchKernelGGL(k27_alpha_guess, dim3(blocks), dim3(256), 0, st, fin, b, flags, Wbf, P, ymean,
+                     ystd, alpha);
   return hipGetLastError();
 }
 

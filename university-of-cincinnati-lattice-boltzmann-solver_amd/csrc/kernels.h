@@ -138,6 +138,7 @@ struct MlpSamples {               // device-resident training pairs
 };
 struct MlpWork {                  // activations of one training batch
   float *Z1, *A1, *Z2, *A2, *Z3, *A3, *Z4, *A4, *D1, *D2, *D3, *D4;
+  float* part;                    // per-chunk weight-gradient sums: ceil(B / 256) x 4160
 };
 hipError_t launch_mlp_collect(const Box& b, const float* fin, const uint8_t* flags, int stride,
                               int phase, MlpSamples& s, hipStream_t st);

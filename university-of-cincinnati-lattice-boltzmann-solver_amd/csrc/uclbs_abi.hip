@@ -1240,12 +1240,14 @@ static int mlp_work(Ctx* c, int B) {
   c->batch_idx = nullptr;
   c->work_batch = 0;
   const size_t per = (size_t)B * 64;
-  HIP_TRY(hipMalloc(&c->work_buf, (9 * per + 3 * (size_t)B) * sizeof(float)));
+  const size_t npart = (size_t)((B + 255) / 256) * (64 * 64 + 64);
+  HIP_TRY(hipMalloc(&c->work_buf, (9 * per + 3 * (size_t)B + npart) * sizeof(float)));
   float* q = c->work_buf;
   uclbs::MlpWork& w = c->work;
   w.Z1 = q; q += per; w.A1 = q; q += per; w.Z2 = q; q += per; w.A2 = q; q += per;
   w.Z3 = q; q += per; w.A3 = q; q += per; w.D1 = q; q += per; w.D2 = q; q += per;
   w.D3 = q; q += per; w.Z4 = q; q += B; w.A4 = q; q += B; w.D4 = q; q += B;
+  w.part = q;
   HIP_TRY(hipMalloc(&c->batch_x, (size_t)B * 12 * sizeof(float)));
   HIP_TRY(hipMalloc(&c->batch_idx, (size_t)B * sizeof(unsigned)));
   c->work_batch = B;
@@ -1422,6 +1424,21 @@ int uclbs_mlp_download_samples(void* ctx, float* X, float* y, int64_t capacity, 
   const size_t k = (size_t)std::min<int64_t>((int64_t)cnt, capacity);
   if (X && k) HIP_TRY(hipMemcpy(X, c->samples.X, k * 12 * sizeof(float), hipMemcpyDeviceToHost));
   if (y && k) HIP_TRY(hipMemcpy(y, c->samples.y, k * sizeof(float), hipMemcpyDeviceToHost));
+  return UCLBS_OK;
+}
+
+int uclbs_mlp_download_sample_nodes(void* ctx, uint32_t* node, int64_t capacity, int64_t* n) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || !n) return set_err(UCLBS_E_ARG, "null argument");
+  if (!c->samples.cap) return set_err(UCLBS_E_STATE, "no sample buffer");
+  int rc = set_device(c);
+  if (rc) return rc;
+  unsigned long long cnt = 0;
+  if ((rc = sample_count(c, &cnt))) return rc;
+  *n = (int64_t)cnt;
+  const size_t k = (size_t)std::min<int64_t>((int64_t)cnt, capacity);
+  if (node && k)
+    HIP_TRY(hipMemcpy(node, c->samples.node_of, k * sizeof(uint32_t), hipMemcpyDeviceToHost));
   return UCLBS_OK;
 }
 
