@@ -207,6 +207,36 @@ int uclbs_init_from_coarse(void *ctx, int lvl);
  * (fine_to_coarse.f95:25-65, amrex_average_down_nodes at ratio 2) */
 int uclbs_fine_to_coarse(void *ctx, int lvl);
 
+/* A level made of many disjoint boxes (the AMReX box array of a level,
+ * amr_processes.f95:383-466): box b's valid nodes are lo[3b..3b+2] ..
+ * hi[3b..3b+2] inclusive in the level's index space.  Each box is stored with
+ * a one-node ghost shell, boxes one after another in the order given (box b
+ * from node sum_{a<b} prod(hi_a - lo_a + 3), x fastest), and the whole level
+ * advances in one launch of the batched streaming kernel.  Level 0: the boxes
+ * must tile their bounding box, which is periodic on every axis.  Level l >= 1
+ * nests in level l-1 (ratio 2): a ghost node that is no valid node of its own
+ * level is interpolated from level l-1 by uclbs_interlevel (every such ghost
+ * needs its coarse stencil inside level l-1's valid nodes), and
+ * uclbs_fine_to_coarse injects the level's even nodes.  Host arrays of the
+ * level (uclbs_upload_f, uclbs_download_f, uclbs_download_macro) are in this
+ * stored order.  D3Q27, plain or fixed alpha; no link tables on such levels. */
+int uclbs_define_level_boxes(void *ctx, int lvl, int nbox, const int32_t *lo, const int32_t *hi);
+
+/* Advances level 0 by ncoarse steps and every finer level by two substeps per
+ * step of its parent -- loop.f95:104-139 under the level schedule of
+ * valid_levels.f95:21-59 -- with the phase order: step, ghosts of the finer
+ * level at theta 0, its substep, ghosts at theta 1/2, its substep, injection.
+ * use_graph != 0: the launches of two coarse steps are captured once as a
+ * hipGraph and replayed (re-captured when the levels, kernel variants or
+ * buffer parities change; an odd remainder runs eagerly). */
+int uclbs_advance(void *ctx, int ncoarse, int use_graph);
+
+/* Picks the tile (block) size of the batched streaming kernel of a level of
+ * many boxes by timing 64, 128 and 256 nodes per block on the live state, then
+ * restores the state; *tile_nodes receives the choice.  No counterpart in the
+ * reference. */
+int uclbs_autotune(void *ctx, int lvl, int *tile_nodes);
+
 /* The level's step counter (reference timestep(lvl)), which drives the
  * start-up ramps of beta, epsilon and alpha; advanced by every step. */
 int uclbs_set_timestep(void *ctx, int lvl, int64_t step);

@@ -31,7 +31,8 @@ EXPORTS = (
     "uclbs_mlp_get_params", "uclbs_mlp_loss_grad", "uclbs_mlp_forward", "uclbs_mlp_reserve",
     "uclbs_mlp_collect", "uclbs_mlp_download_samples", "uclbs_mlp_download_sample_nodes",
     "uclbs_mlp_train", "uclbs_interlevel",
-    "uclbs_init_from_coarse", "uclbs_fine_to_coarse",
+    "uclbs_init_from_coarse", "uclbs_fine_to_coarse", "uclbs_define_level_boxes",
+    "uclbs_advance", "uclbs_autotune",
 )
 MLP_PARAMS = 9217
 FACE_PERIODIC = 2147483647
@@ -132,6 +133,10 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "uclbs_mlp_reserve": (I, [P, I64]),
         "uclbs_mlp_collect": (I, [P, I, I, ctypes.POINTER(I64)]),
         "uclbs_mlp_download_samples": (I, [P, fp, fp, I64, ctypes.POINTER(I64)]),
+        "uclbs_define_level_boxes": (I, [P, I, I, ctypes.POINTER(ctypes.c_int32),
+                                         ctypes.POINTER(ctypes.c_int32)]),
+        "uclbs_advance": (I, [P, I, I]),
+        "uclbs_autotune": (I, [P, I, ctypes.POINTER(I)]),
         "uclbs_mlp_download_sample_nodes": (I, [P, ctypes.POINTER(ctypes.c_uint32), I64,
                                                 ctypes.POINTER(I64)]),
         "uclbs_mlp_train": (I, [P, I64, I, D, U64, dp, dp]),
@@ -230,6 +235,67 @@ class Solver:
         self._check(self.lib.uclbs_define_level(self.ctx, lvl, lo3, hi3, per))
         g = 2 if lvl > 0 else 0
         self.shapes[lvl] = tuple(int(s) + g for s in shape)
+
+    def define_level_boxes(self, lvl: int, boxes):
+        """boxes: [(lo, hi), ...] valid nodes inclusive in the level's index
+        space.  The level's arrays are in stored order: each box with a one-node
+        ghost shell, boxes one after another (see box_layout)."""
+        lo = np.ascontiguousarray([b[0] for b in boxes], np.int32).reshape(-1)
+        hi = np.ascontiguousarray([b[1] for b in boxes], np.int32).reshape(-1)
+        pi = ctypes.POINTER(ctypes.c_int32)
+        self._check(self.lib.uclbs_define_level_boxes(self.ctx, lvl, len(boxes),
+                                                      lo.ctypes.data_as(pi), hi.ctypes.data_as(pi)))
+        self.boxes = getattr(self, "boxes", {})
+        self.boxes[lvl] = [(tuple(int(v) for v in b[0]), tuple(int(v) for v in b[1])) for b in boxes]
+        n = sum(int(np.prod([h - l + 3 for l, h in zip(*b)])) for b in self.boxes[lvl])
+        self.shapes[lvl] = (n, 1, 1)
+
+    def box_layout(self, lvl: int):
+        """(offset, stored extents (x, y, z), lo of the valid region) per box."""
+        out, off = [], 0
+        for lo, hi in self.boxes[lvl]:
+            e = tuple(h - l + 3 for l, h in zip(lo, hi))
+            out.append((off, e, lo))
+            off += e[0] * e[1] * e[2]
+        return out
+
+    def dense_to_stored(self, lvl: int, a: np.ndarray, origin=(0, 0, 0)):
+        """a[..., z, y, x] over level positions origin + (x, y, z) -> stored
+        order (valid nodes taken from a, ghost-shell nodes from the wrapped
+        position, so a periodic level-0 array fills every stored node)."""
+        a = np.asarray(a)
+        lead = a.shape[:-3]
+        out = np.zeros(lead + (self.shapes[lvl][0],), a.dtype)
+        nz, ny, nx = a.shape[-3:]
+        for off, e, lo in self.box_layout(lvl):
+            zz = (np.arange(e[2]) + lo[2] - 1 - origin[2]) % nz
+            yy = (np.arange(e[1]) + lo[1] - 1 - origin[1]) % ny
+            xx = (np.arange(e[0]) + lo[0] - 1 - origin[0]) % nx
+            blk = a[..., zz[:, None, None], yy[None, :, None], xx[None, None, :]]
+            out[..., off:off + e[0] * e[1] * e[2]] = blk.reshape(lead + (-1,))
+        return out
+
+    def stored_to_dense(self, lvl: int, v: np.ndarray, shape, origin=(0, 0, 0), fill=np.nan):
+        """Valid nodes of stored-order v into a dense [..., z, y, x] array of
+        the given shape over positions origin + (x, y, z); other entries fill."""
+        v = np.asarray(v)
+        lead = v.shape[:-1]
+        out = np.full(lead + tuple(shape), fill, dtype=np.result_type(v.dtype, np.float64))
+        for off, e, lo in self.box_layout(lvl):
+            blk = v[..., off:off + e[0] * e[1] * e[2]].reshape(lead + (e[2], e[1], e[0]))
+            z0, y0, x0 = lo[2] - origin[2], lo[1] - origin[1], lo[0] - origin[0]
+            out[..., z0:z0 + e[2] - 2, y0:y0 + e[1] - 2, x0:x0 + e[0] - 2] = blk[..., 1:-1, 1:-1, 1:-1]
+        return out
+
+    def advance_native(self, ncoarse: int, graph: bool = False):
+        """uclbs_advance: the same schedule as advance(), in the library, with
+        the launches of two coarse steps replayed as a hipGraph when graph."""
+        self._check(self.lib.uclbs_advance(self.ctx, ncoarse, int(graph)))
+
+    def autotune(self, lvl: int) -> int:
+        t = ctypes.c_int()
+        self._check(self.lib.uclbs_autotune(self.ctx, lvl, ctypes.byref(t)))
+        return t.value
 
     def interlevel(self, lvl: int, theta: float):
         self._check(self.lib.uclbs_interlevel(self.ctx, lvl, theta))

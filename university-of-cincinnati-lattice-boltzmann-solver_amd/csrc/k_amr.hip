@@ -28,6 +28,12 @@ namespace uclbs {
 namespace {
 
 template <bool kShifted>
+__device__ __forceinline__ void repair_store27(double (&f)[27],
+                                               typename std::conditional<kShifted, float, double>::type* __restrict__ ff,
+                                               unsigned fnn, unsigned node,
+                                               unsigned long long* __restrict__ stats);
+
+template <bool kShifted>
 __global__ __launch_bounds__(256) void k27_coarse_to_fine(
     const void* __restrict__ cprev_, const void* __restrict__ cnew_, LevelMap cm,
     void* __restrict__ ff_, LevelMap fm, const unsigned* __restrict__ ghosts, unsigned nghost,
@@ -76,7 +82,17 @@ __global__ __launch_bounds__(256) void k27_coarse_to_fine(
           f[i] += w * ((1.0 - theta) * a + theta * b);
         }
       }
-  // negative populations take the equilibrium of the interpolated moments
+  repair_store27<kShifted>(f, ff, fnn, node, stats);
+}
+
+// negative (or NaN) interpolated populations take the equilibrium of the
+// interpolated moments; then the node is stored
+template <bool kShifted>
+__device__ __forceinline__ void repair_store27(double (&f)[27],
+                                               typename std::conditional<kShifted, float, double>::type* __restrict__ ff,
+                                               unsigned fnn, unsigned node,
+                                               unsigned long long* __restrict__ stats) {
+  using T = typename std::conditional<kShifted, float, double>::type;
   bool neg = false;
 #pragma unroll
   for (int i = 0; i < 27; ++i) neg |= !((kShifted ? W27[i] + f[i] : f[i]) > 0.0);
@@ -115,6 +131,50 @@ __global__ __launch_bounds__(256) void k27_coarse_to_fine(
   for (int i = 0; i < 27; ++i) ff[(size_t)i * fnn + node] = (T)f[i];
 }
 
+// ghost nodes of a level of many boxes from precomputed coarse stencils; the
+// same arithmetic and order as k27_coarse_to_fine
+template <bool kShifted>
+__global__ __launch_bounds__(256) void k27_c2f_stencils(const void* __restrict__ cprev_,
+                                                        const void* __restrict__ cnew_, unsigned cnn,
+                                                        void* __restrict__ ff_, unsigned fnn,
+                                                        const C2FStencil* __restrict__ st,
+                                                        unsigned nst, double theta,
+                                                        unsigned long long* __restrict__ stats) {
+  using T = typename std::conditional<kShifted, float, double>::type;
+  const T* __restrict__ cprev = static_cast<const T*>(cprev_);
+  const T* __restrict__ cnew = static_cast<const T*>(cnew_);
+  const unsigned j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nst) return;
+  const C2FStencil s = st[j];
+  double f[27];
+#pragma unroll
+  for (int i = 0; i < 27; ++i) f[i] = 0.0;
+  for (int q = 0; q < 8; ++q) {
+    const double w = (double)s.w[q];
+    if (w == 0.0) continue;
+    const unsigned k = s.src[q];
+#pragma unroll
+    for (int i = 0; i < 27; ++i) {
+      const double a = (double)cprev[(size_t)i * cnn + k], b = (double)cnew[(size_t)i * cnn + k];
+      f[i] += w * ((1.0 - theta) * a + theta * b);
+    }
+  }
+  repair_store27<kShifted>(f, static_cast<T*>(ff_), fnn, s.dst, stats);
+}
+
+// src and dst may be the same level (ghost copies): a pair's source is a valid
+// node, never a ghost, so no pair reads what another writes
+template <typename T>
+__global__ __launch_bounds__(256) void k27_copy_pairs(const T* src, unsigned snn, T* dst, unsigned dnn,
+                                                      const uint2* __restrict__ pairs,
+                                                      unsigned npairs) {
+  const unsigned j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= npairs) return;
+  const uint2 p = pairs[j];
+#pragma unroll
+  for (int i = 0; i < 27; ++i) dst[(size_t)i * dnn + p.x] = src[(size_t)i * snn + p.y];
+}
+
 // injection: covered coarse node C <- fine node 2C
 template <typename T>
 __global__ __launch_bounds__(256) void k27_fine_to_coarse(const T* __restrict__ ff, LevelMap fm,
@@ -150,6 +210,31 @@ hipError_t launch_coarse_to_fine27(const void* cprev, const void* cnew, const Le
     k27_coarse_to_fine<true><<<g, 256, 0, s>>>(cprev, cnew, cm, ff, fm, ghosts, nghost, theta, stats);
   else
     k27_coarse_to_fine<false><<<g, 256, 0, s>>>(cprev, cnew, cm, ff, fm, ghosts, nghost, theta, stats);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_pairs27(const void* src, unsigned nn_src, void* dst, unsigned nn_dst,
+                               const uint2* pairs, unsigned npairs, int shifted, hipStream_t s) {
+  if (!npairs) return hipSuccess;
+  const unsigned g = (npairs + 255) / 256;
+  if (shifted)
+    k27_copy_pairs<float><<<g, 256, 0, s>>>(static_cast<const float*>(src), nn_src,
+                                            static_cast<float*>(dst), nn_dst, pairs, npairs);
+  else
+    k27_copy_pairs<double><<<g, 256, 0, s>>>(static_cast<const double*>(src), nn_src,
+                                             static_cast<double*>(dst), nn_dst, pairs, npairs);
+  return hipGetLastError();
+}
+
+hipError_t launch_c2f_stencils27(const void* cprev, const void* cnew, unsigned nn_c, void* ff,
+                                 unsigned nn_f, const C2FStencil* st, unsigned nst, double theta,
+                                 int shifted, unsigned long long* stats, hipStream_t s) {
+  if (!nst) return hipSuccess;
+  const unsigned g = (nst + 255) / 256;
+  if (shifted)
+    k27_c2f_stencils<true><<<g, 256, 0, s>>>(cprev, cnew, nn_c, ff, nn_f, st, nst, theta, stats);
+  else
+    k27_c2f_stencils<false><<<g, 256, 0, s>>>(cprev, cnew, nn_c, ff, nn_f, st, nst, theta, stats);
   return hipGetLastError();
 }
 

@@ -197,6 +197,46 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
              ao.logpath);
 }
 
+// All boxes of a level in one launch: block b takes tile b (tile_nodes nodes
+// of one box), pulls over the box's strides -- a valid node's neighbours lie in
+// the box or its ghost shell -- and collides as k27_step_periodic does.
+// Ghost-shell nodes are filled by the copy / interpolation kernels beforehand.
+template <bool kShifted, bool kKeepAlpha>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k27_step_boxes(
+    const void* __restrict__ fin_, void* __restrict__ fout_, double* __restrict__ alpha_out,
+    unsigned long long* __restrict__ stats, unsigned nn, StepArgs a,
+    const uint8_t* __restrict__ flags, const BoxTile* __restrict__ tiles,
+    const BoxDesc* __restrict__ boxes, int tile_nodes) {
+  using T = typename std::conditional<kShifted, float, double>::type;
+  const T* __restrict__ fin = static_cast<const T*>(fin_);
+  T* __restrict__ fout = static_cast<T*>(fout_);
+  const BoxTile t = tiles[blockIdx.x];
+  const BoxDesc bx = boxes[t.box];
+  const unsigned local = t.start + threadIdx.x;
+  bool active = (int)threadIdx.x < tile_nodes && local < bx.n;
+  const unsigned node = bx.off + (active ? local : 0u);
+  active = active && flags[node] == kFlagInterior;
+  const unsigned sy = bx.nx, sz = bx.nx * bx.ny;
+  T f[27];
+#pragma unroll
+  for (int i = 0; i < 27; ++i)
+    f[i] = pop(fin, i, nn, active ? node - CX27[i] - CY27[i] * sy - CZ27[i] * sz : node);
+  AlphaCfg ac{a.tol, a.maxit, a.alpha_mode, a.thr};
+  double rho, u[3];
+  AlphaOut ao;
+  if constexpr (kShifted) ao = collide27_shifted(f, a.beta, ac, 2.0, rho, u);
+  else ao = collide27_fp64(f, a.beta, ac, 2.0, rho, u);
+  if (active) {
+#pragma unroll
+    for (int i = 0; i < 27; ++i) pop(fout, i, nn, node) = f[i];
+    if (kKeepAlpha) alpha_out[node] = ao.alpha;
+  }
+  if constexpr (kShifted)
+    queue_far(a, blockIdx.x % kFarShards, active && ao.status == ALPHA_DEFER, node, ao.alpha);
+  wave_stats(stats, active, ao.iters, ao.status == ALPHA_DEFER ? -1 : ao.status, ao.fallback,
+             ao.logpath);
+}
+
 // Far-from-equilibrium nodes queued by the streaming and boundary kernels
 // (max|x| beyond the in-register series): one thread per queued node reads
 // the pre-collision state stored in place, re-forms D, and solves for alpha by
@@ -643,6 +683,23 @@ hipError_t launch_d3q27_step_links(const Box& b, const void* fin, void* fout, do
   } else {
     if (keep_alpha) k27_step_periodic<false, true, true><<<g, 256, 0, s>>>(fin, fout, alpha, stats, b, a, bc.flags, t);
     else k27_step_periodic<false, false, true><<<g, 256, 0, s>>>(fin, fout, alpha, stats, b, a, bc.flags, t);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_d3q27_step_boxes(unsigned nn, const void* fin, void* fout, double* alpha,
+                                   int keep_alpha, unsigned long long* stats, const StepArgs& a,
+                                   const uint8_t* flags, const BoxTile* tiles, unsigned ntiles,
+                                   const BoxDesc* boxes, int tile_nodes, hipStream_t s) {
+  if (!ntiles) return hipSuccess;
+  if (tile_nodes < 64 || tile_nodes > 256 || tile_nodes % 64) return hipErrorInvalidValue;
+  const dim3 g(ntiles), blk((unsigned)tile_nodes);
+  if (a.shifted) {
+    if (keep_alpha) k27_step_boxes<true, true><<<g, blk, 0, s>>>(fin, fout, alpha, stats, nn, a, flags, tiles, boxes, tile_nodes);
+    else k27_step_boxes<true, false><<<g, blk, 0, s>>>(fin, fout, alpha, stats, nn, a, flags, tiles, boxes, tile_nodes);
+  } else {
+    if (keep_alpha) k27_step_boxes<false, true><<<g, blk, 0, s>>>(fin, fout, alpha, stats, nn, a, flags, tiles, boxes, tile_nodes);
+    else k27_step_boxes<false, false><<<g, blk, 0, s>>>(fin, fout, alpha, stats, nn, a, flags, tiles, boxes, tile_nodes);
   }
   return hipGetLastError();
 }

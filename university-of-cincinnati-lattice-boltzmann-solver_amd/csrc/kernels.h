@@ -126,6 +126,36 @@ hipError_t launch_coarse_to_fine27(const void* cprev, const void* cnew, const Le
 hipError_t launch_fine_to_coarse27(const void* ff, const LevelMap& fm, void* fc, const LevelMap& cm,
                                    const Range3& cover, int shifted, hipStream_t s);
 
+// ---- levels of many boxes ---------------------------------------------------------
+// A level may hold many disjoint boxes, each stored with a one-node ghost shell
+// in one stretch of the level's population arrays (SoA over the whole level).
+struct BoxDesc {
+  unsigned off;     // first stored node of the box
+  unsigned n;       // stored nodes (ghost shell included)
+  unsigned nx, ny;  // stored extents on x and y (nz = n / (nx ny))
+};
+struct BoxTile {    // one block of the batched streaming kernel
+  unsigned box;
+  unsigned start;   // first node of the tile within the box
+};
+struct C2FStencil { // a ghost node interpolated from the coarser level
+  unsigned dst;     // fine stored node
+  unsigned src[8];  // coarse stored nodes, tensor order (z outer, x inner)
+  float w[8];       // their weights (products of 1 and 1/2; 0 = unused)
+};
+hipError_t launch_d3q27_step_boxes(unsigned nn, const void* fin, void* fout, double* alpha,
+                                   int keep_alpha, unsigned long long* stats, const StepArgs& a,
+                                   const uint8_t* flags, const BoxTile* tiles, unsigned ntiles,
+                                   const BoxDesc* boxes, int tile_nodes, hipStream_t s);
+// dst[i][p.x] = src[i][p.y] for each pair p (ghost copies and injection)
+hipError_t launch_copy_pairs27(const void* src, unsigned nn_src, void* dst, unsigned nn_dst,
+                               const uint2* pairs, unsigned npairs, int shifted, hipStream_t s);
+// time-interpolated coarse stencils (theta between cprev and cnew), negative
+// results replaced by the equilibrium of the interpolated moments
+hipError_t launch_c2f_stencils27(const void* cprev, const void* cnew, unsigned nn_c, void* ff,
+                                 unsigned nn_f, const C2FStencil* st, unsigned nst, double theta,
+                                 int shifted, unsigned long long* stats, hipStream_t s);
+
 // ---- learned alpha -------------------------------------------------------------
 constexpr int kMlpParams = 9217;  // 12-64-64-64-1
 struct MlpSamples {               // device-resident training pairs

@@ -70,6 +70,25 @@ struct Level {
   uint32_t* ghosts = nullptr;
   unsigned nghost = 0;
   bool refined = false;
+  // a level of many boxes (uclbs_define_level_boxes)
+  bool multibox = false;
+  std::vector<int32_t> blo, bhi;  // valid region of each box, [nbox][3], level index space
+  std::vector<uclbs::BoxDesc> hbox;
+  uclbs::BoxDesc* d_boxes = nullptr;
+  uclbs::BoxTile* d_tiles = nullptr;
+  unsigned ntiles = 0;
+  int tile_nodes = 256;
+  uint2* d_copies = nullptr;  // same-level ghost copies {dst, src}
+  unsigned ncopies = 0;
+  uclbs::C2FStencil* d_c2f = nullptr;  // ghosts interpolated from the coarser level
+  unsigned nc2f = 0;
+  uint2* d_inject = nullptr;  // {coarse node, fine node} of fine_to_coarse
+  unsigned ninject = 0;
+  int dom_lo[3] = {0, 0, 0}, dom_hi[3] = {0, 0, 0};  // level 0: the periodic domain
+  // host lookup of the box holding a level position: buckets of edge bk_e over
+  // the bounding box of the valid regions, each listing the boxes it meets
+  int bk_lo[3] = {0, 0, 0}, bk_n[3] = {0, 0, 0}, bk_e = 8;
+  std::vector<std::vector<int>> buckets;
 };
 
 struct Ctx {
@@ -111,7 +130,15 @@ struct Ctx {
   float* batch_x = nullptr;
   unsigned* batch_idx = nullptr;
   double* d_scalar = nullptr;  // loss, mean, std
+  // captured multi-level advance (uclbs_advance with use_graph)
+  hipGraphExec_t graph = nullptr;
+  std::vector<int> graph_sig;
+  int64_t structure = 0;  // bumped by every change of levels or links
 };
+
+int hip_rc(hipError_t e) {
+  return e == hipSuccess ? UCLBS_OK : set_err(UCLBS_E_HIP, "HIP error: %s", hipGetErrorString(e));
+}
 
 size_t elem_size(const Ctx* c) {
   return c->cfg.precision == UCLBS_PREC_FP64 ? sizeof(double) : sizeof(float);
@@ -150,6 +177,19 @@ void free_level(Level& L) {
   if (L.rep) { (void)hipFree(L.rep); L.rep = nullptr; }
   if (L.ghosts) { (void)hipFree(L.ghosts); L.ghosts = nullptr; }
   L.nghost = 0;
+  for (void* p : {(void*)L.d_boxes, (void*)L.d_tiles, (void*)L.d_copies, (void*)L.d_c2f,
+                  (void*)L.d_inject})
+    if (p) (void)hipFree(p);
+  L.d_boxes = nullptr;
+  L.d_tiles = nullptr;
+  L.d_copies = nullptr;
+  L.d_c2f = nullptr;
+  L.d_inject = nullptr;
+  L.ntiles = L.ncopies = L.nc2f = L.ninject = 0;
+  L.multibox = false;
+  L.blo.clear();
+  L.bhi.clear();
+  L.hbox.clear();
   L.refined = false;
   L.step = 0;
   L.has_links = false;
@@ -227,6 +267,7 @@ hipError_t alloc_far_queue(Level& L, int64_t nb) {
 
 // flags + ascending boundary list from the link table
 int finalize_links(Ctx* c, Level* L) {
+  c->structure++;
   const int64_t n = L->box.n;
   if (!L->flags) HIP_TRY(hipMalloc(&L->flags, (size_t)n));
   HIP_TRY(uclbs::launch_flags27(L->state, L->box, L->face_code, L->flags, c->stream));
@@ -299,8 +340,40 @@ int launch_steps39(Ctx* c, Level* L, int nsteps, hipEvent_t* ev) {
   return UCLBS_OK;
 }
 
+// A level of many boxes: same-level ghost copies, the batched streaming kernel
+// over all boxes, the far-from-equilibrium kernel.  Ghosts that come from the
+// coarser level are filled by uclbs_interlevel before the step.
+int launch_steps_boxes(Ctx* c, Level* L, int nsteps, hipEvent_t* ev) {
+  if (c->cfg.alpha_mode == UCLBS_ALPHA_LEARNED)
+    return set_err(UCLBS_E_UNSUPPORTED, "learned alpha runs on single-box levels");
+  uclbs::StepArgs a = step_args(c, (int)(L - c->levels.data()));
+  a.defer_list = L->defer_list;
+  a.defer_cap = L->defer_cap;
+  const unsigned nn = (unsigned)L->box.n;
+  for (int s = 0; s < nsteps; ++s) {
+    a.defer_count = L->defer_count + L->far_parity * (uclbs::kFarShards + 1);
+    a.defer_next = L->defer_count + (1 - L->far_parity) * (uclbs::kFarShards + 1);
+    HIP_TRY(uclbs::launch_copy_pairs27(L->f[L->cur], nn, L->f[L->cur], nn, L->d_copies, L->ncopies,
+                                       a.shifted, c->stream));
+    if (ev) HIP_TRY(hipEventRecord(ev[2 * s], c->stream));
+    HIP_TRY(uclbs::launch_d3q27_step_boxes(nn, L->f[L->cur], L->f[1 - L->cur], L->alpha,
+                                           c->keep_alpha, c->d_stats, a, L->flags, L->d_tiles,
+                                           L->ntiles, L->d_boxes, L->tile_nodes, c->stream));
+    if (ev) HIP_TRY(hipEventRecord(ev[2 * s + 1], c->stream));
+    if (a.shifted) {
+      HIP_TRY(uclbs::launch_d3q27_far(L->box, L->f[1 - L->cur], L->alpha, c->keep_alpha,
+                                      c->d_stats, a, c->stream));
+      L->far_parity ^= 1;
+    }
+    L->cur = 1 - L->cur;
+    c->steps_done++;
+  }
+  return UCLBS_OK;
+}
+
 int launch_steps(Ctx* c, Level* L, int nsteps, hipEvent_t* ev = nullptr) {
   if (c->cfg.lattice == UCLBS_LATTICE_D3Q39) return launch_steps39(c, L, nsteps, ev);
+  if (L->multibox) return launch_steps_boxes(c, L, nsteps, ev);
   uclbs::StepArgs a = step_args(c, (int)(L - c->levels.data()));
   if (L->has_links) {
     int rc = check_face_tables(c, L);
@@ -409,6 +482,7 @@ int uclbs_destroy(void* ctx) {
   if (!c) return UCLBS_OK;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->graph) (void)hipGraphExecDestroy(c->graph);
   for (auto& L : c->levels) free_level(L);
   if (c->d_stats) (void)hipFree(c->d_stats);
   if (c->d_feq_in) (void)hipFree(c->d_feq_in);
@@ -440,6 +514,7 @@ int uclbs_define_level(void* ctx, int lvl, const int32_t lo[3], const int32_t hi
   if ((int)c->levels.size() <= lvl) c->levels.resize(lvl + 1);
   Level& L = c->levels[lvl];
   free_level(L);
+  c->structure++;
   if (lvl > 0) {
     // a refined level: nested in level lvl-1 with ratio 2, not periodic, and
     // the coarse nodes its ghost shell interpolates from inside the coarser
@@ -451,6 +526,9 @@ int uclbs_define_level(void* ctx, int lvl, const int32_t lo[3], const int32_t hi
     const Level& P = c->levels[lvl - 1];
     if (P.has_links && !P.refined)
       return set_err(UCLBS_E_UNSUPPORTED, "refined levels nest in a level without link tables");
+    if (P.multibox)
+      return set_err(UCLBS_E_UNSUPPORTED, "a single-box level does not nest in a level of many "
+                                          "boxes (use uclbs_define_level_boxes)");
     for (int d = 0; d < 3; ++d) {
       if (periodic[d]) return set_err(UCLBS_E_ARG, "a refined level is not periodic");
       const int cv_lo = P.lo[d], cv_hi = P.hi[d];
@@ -530,6 +608,310 @@ int uclbs_define_level(void* ctx, int lvl, const int32_t lo[3], const int32_t hi
     L.nb = 0;
     for (int f = 0; f < 6; ++f) L.face_code[f] = -10;
   }
+  return UCLBS_OK;
+}
+
+
+}  // extern "C"
+
+// ---- levels of many boxes: host tables -------------------------------------------
+
+// buckets of the valid regions (host lookup of the box that holds a position)
+static void build_buckets(Level& L) {
+  const int nb = (int)L.hbox.size();
+  int lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) {
+    lo[d] = INT32_MAX;
+    hi[d] = INT32_MIN;
+    for (int b = 0; b < nb; ++b) {
+      lo[d] = std::min(lo[d], L.blo[3 * b + d]);
+      hi[d] = std::max(hi[d], L.bhi[3 * b + d]);
+    }
+  }
+  int ext = 1;
+  for (int d = 0; d < 3; ++d) ext = std::max(ext, hi[d] - lo[d] + 1);
+  L.bk_e = std::max(8, (ext + 63) / 64);
+  size_t total = 1;
+  for (int d = 0; d < 3; ++d) {
+    L.bk_lo[d] = lo[d];
+    L.bk_n[d] = (hi[d] - lo[d]) / L.bk_e + 1;
+    total *= (size_t)L.bk_n[d];
+  }
+  L.buckets.assign(total, {});
+  for (int b = 0; b < nb; ++b) {
+    int c0[3], c1[3];
+    for (int d = 0; d < 3; ++d) {
+      c0[d] = (L.blo[3 * b + d] - lo[d]) / L.bk_e;
+      c1[d] = (L.bhi[3 * b + d] - lo[d]) / L.bk_e;
+    }
+    for (int z = c0[2]; z <= c1[2]; ++z)
+      for (int y = c0[1]; y <= c1[1]; ++y)
+        for (int x = c0[0]; x <= c1[0]; ++x)
+          L.buckets[(size_t)x + (size_t)L.bk_n[0] * ((size_t)y + (size_t)L.bk_n[1] * z)].push_back(b);
+  }
+}
+
+// stored index of the valid node at level position G (-1 if no valid node of
+// the level is there).  Periodic axes wrap: a single box over its extent,
+// level 0 of many boxes over its domain.
+static int64_t valid_index(const Level& L, const int G[3]) {
+  int g[3] = {G[0], G[1], G[2]};
+  if (!L.multibox) {
+    int loc[3];
+    const int n[3] = {L.box.nx, L.box.ny, L.box.nz};
+    for (int d = 0; d < 3; ++d) {
+      if (L.box.per[d]) g[d] = ((g[d] - L.origin[d]) % n[d] + n[d]) % n[d] + L.origin[d];
+      if (g[d] < L.lo[d] || g[d] > L.hi[d]) return -1;
+      loc[d] = g[d] - L.origin[d];
+    }
+    return (int64_t)loc[0] + (int64_t)n[0] * (loc[1] + (int64_t)n[1] * loc[2]);
+  }
+  if (L.box.per[0]) {
+    for (int d = 0; d < 3; ++d) {
+      const int n = L.dom_hi[d] - L.dom_lo[d] + 1;
+      g[d] = ((g[d] - L.dom_lo[d]) % n + n) % n + L.dom_lo[d];
+    }
+  }
+  int cell[3];
+  for (int d = 0; d < 3; ++d) {
+    const int r = g[d] - L.bk_lo[d];
+    if (r < 0) return -1;
+    cell[d] = r / L.bk_e;
+    if (cell[d] >= L.bk_n[d]) return -1;
+  }
+  for (int b : L.buckets[(size_t)cell[0] + (size_t)L.bk_n[0] * ((size_t)cell[1] + (size_t)L.bk_n[1] * cell[2])]) {
+    const int* lo = &L.blo[3 * b];
+    const int* hi = &L.bhi[3 * b];
+    if (g[0] < lo[0] || g[0] > hi[0] || g[1] < lo[1] || g[1] > hi[1] || g[2] < lo[2] || g[2] > hi[2])
+      continue;
+    const uclbs::BoxDesc& bx = L.hbox[b];
+    const unsigned nz = bx.n / (bx.nx * bx.ny);
+    (void)nz;
+    return (int64_t)bx.off + (g[0] - lo[0] + 1) +
+           (int64_t)bx.nx * ((g[1] - lo[1] + 1) + (int64_t)bx.ny * (g[2] - lo[2] + 1));
+  }
+  return -1;
+}
+
+// coarse stencil of fine position G: per axis the coincident coarse node
+// (G even) or the two neighbours with weight 1/2 (G odd), tensor order z, y, x
+// -- the order k27_coarse_to_fine sums in
+static bool c2f_stencil(const Level& C, const int G[3], uclbs::C2FStencil& st) {
+  int c0[3], odd[3];
+  for (int d = 0; d < 3; ++d) {
+    odd[d] = G[d] & 1;
+    c0[d] = G[d] >> 1;
+  }
+  int q = 0;
+  for (int cz = 0; cz < 2; ++cz)
+    for (int cy = 0; cy < 2; ++cy)
+      for (int cx = 0; cx < 2; ++cx) {
+        const double wx = odd[0] ? 0.5 : (cx ? 0.0 : 1.0), wy = odd[1] ? 0.5 : (cy ? 0.0 : 1.0),
+                     wz = odd[2] ? 0.5 : (cz ? 0.0 : 1.0);
+        const double w = wx * wy * wz;
+        st.src[q] = 0;
+        st.w[q] = 0.0f;
+        if (w != 0.0) {
+          const int P[3] = {c0[0] + cx * odd[0], c0[1] + cy * odd[1], c0[2] + cz * odd[2]};
+          const int64_t k = valid_index(C, P);
+          if (k < 0) return false;
+          st.src[q] = (unsigned)k;
+          st.w[q] = (float)w;
+        }
+        ++q;
+      }
+  return true;
+}
+
+template <typename T>
+static hipError_t upload_vec(T** d, const std::vector<T>& h) {
+  if (*d) (void)hipFree(*d);
+  *d = nullptr;
+  if (h.empty()) return hipSuccess;
+  hipError_t e = hipMalloc(d, h.size() * sizeof(T));
+  if (e == hipSuccess) e = hipMemcpy(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice);
+  return e;
+}
+
+static hipError_t build_tiles(Level& L, int tile_nodes) {
+  std::vector<uclbs::BoxTile> tiles;
+  for (unsigned b = 0; b < (unsigned)L.hbox.size(); ++b)
+    for (unsigned s = 0; s < L.hbox[b].n; s += (unsigned)tile_nodes) tiles.push_back({b, s});
+  L.tile_nodes = tile_nodes;
+  L.ntiles = (unsigned)tiles.size();
+  return upload_vec(&L.d_tiles, tiles);
+}
+
+extern "C" {
+
+int uclbs_define_level_boxes(void* ctx, int lvl, int nbox, const int32_t* lo, const int32_t* hi) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || !lo || !hi) return set_err(UCLBS_E_ARG, "null argument");
+  if (lvl < 0 || lvl > 31) return set_err(UCLBS_E_ARG, "level %d out of range", lvl);
+  if (nbox < 1) return set_err(UCLBS_E_ARG, "no boxes");
+  if (c->cfg.lattice != UCLBS_LATTICE_D3Q27)
+    return set_err(UCLBS_E_UNSUPPORTED, "levels of many boxes are D3Q27 in this build");
+  int rc = set_device(c);
+  if (rc) return rc;
+  if (lvl > 0 && (lvl - 1 >= (int)c->levels.size() || !c->levels[lvl - 1].defined))
+    return set_err(UCLBS_E_STATE, "define level %d before level %d", lvl - 1, lvl);
+  if (lvl > 0 && c->levels[lvl - 1].has_links && !c->levels[lvl - 1].refined)
+    return set_err(UCLBS_E_UNSUPPORTED, "refined levels nest in a level without link tables");
+  // validity and disjointness of the valid regions
+  int64_t vol = 0, stored = 0;
+  for (int b = 0; b < nbox; ++b)
+    for (int d = 0; d < 3; ++d)
+      if (hi[3 * b + d] < lo[3 * b + d])
+        return set_err(UCLBS_E_ARG, "box %d is empty on axis %d", b, d);
+  for (int b = 0; b < nbox; ++b) {
+    int64_t v = 1, sv = 1;
+    for (int d = 0; d < 3; ++d) {
+      v *= hi[3 * b + d] - lo[3 * b + d] + 1;
+      sv *= hi[3 * b + d] - lo[3 * b + d] + 3;
+    }
+    vol += v;
+    stored += sv;
+    for (int o = 0; o < b; ++o) {
+      bool meet = true;
+      for (int d = 0; d < 3; ++d)
+        meet = meet && lo[3 * b + d] <= hi[3 * o + d] && lo[3 * o + d] <= hi[3 * b + d];
+      if (meet) return set_err(UCLBS_E_ARG, "boxes %d and %d overlap", o, b);
+    }
+  }
+  if (stored > (1ll << 30)) return set_err(UCLBS_E_ARG, "level exceeds 2^30 stored nodes");
+  if ((int)c->levels.size() <= lvl) c->levels.resize(lvl + 1);
+  Level& L = c->levels[lvl];
+  free_level(L);
+  c->structure++;
+  L.multibox = true;
+  L.blo.assign(lo, lo + 3 * nbox);
+  L.bhi.assign(hi, hi + 3 * nbox);
+  unsigned off = 0;
+  for (int b = 0; b < nbox; ++b) {
+    uclbs::BoxDesc bx{};
+    const unsigned e[3] = {(unsigned)(hi[3 * b] - lo[3 * b] + 3), (unsigned)(hi[3 * b + 1] - lo[3 * b + 1] + 3),
+                           (unsigned)(hi[3 * b + 2] - lo[3 * b + 2] + 3)};
+    bx.off = off;
+    bx.nx = e[0];
+    bx.ny = e[1];
+    bx.n = e[0] * e[1] * e[2];
+    off += bx.n;
+    L.hbox.push_back(bx);
+  }
+  for (int d = 0; d < 3; ++d) {
+    L.dom_lo[d] = INT32_MAX;
+    L.dom_hi[d] = INT32_MIN;
+    for (int b = 0; b < nbox; ++b) {
+      L.dom_lo[d] = std::min(L.dom_lo[d], lo[3 * b + d]);
+      L.dom_hi[d] = std::max(L.dom_hi[d], hi[3 * b + d]);
+    }
+  }
+  if (lvl == 0) {
+    // level 0: the boxes tile their bounding box, periodic on every axis
+    int64_t dv = 1;
+    for (int d = 0; d < 3; ++d) dv *= L.dom_hi[d] - L.dom_lo[d] + 1;
+    if (dv != vol) {
+      free_level(L);
+      return set_err(UCLBS_E_ARG, "level 0 boxes cover %lld of the %lld nodes of their bounding "
+                                  "box (they must tile it)", (long long)vol, (long long)dv);
+    }
+  }
+  // a flat stored "box" for the element-wise kernels and the far queue
+  L.box.nx = (int)off;
+  L.box.ny = L.box.nz = 1;
+  L.box.n = off;
+  for (int d = 0; d < 3; ++d) {
+    L.box.per[d] = lvl == 0;
+    L.lo[d] = L.dom_lo[d];
+    L.hi[d] = L.dom_hi[d];
+    L.face_code[d] = L.face_code[d + 3] = -10;
+  }
+  build_buckets(L);
+  // ghost sources
+  std::vector<uint8_t> hf((size_t)off, uclbs::kFlagInterior);
+  std::vector<uint2> copies;
+  std::vector<uclbs::C2FStencil> c2f;
+  const Level* C = lvl > 0 ? &c->levels[lvl - 1] : nullptr;
+  for (int b = 0; b < nbox; ++b) {
+    const uclbs::BoxDesc& bx = L.hbox[b];
+    const unsigned nz = bx.n / (bx.nx * bx.ny);
+    for (unsigned z = 0; z < nz; ++z)
+      for (unsigned y = 0; y < bx.ny; ++y)
+        for (unsigned x = 0; x < bx.nx; ++x) {
+          if (x > 0 && y > 0 && z > 0 && x + 1 < bx.nx && y + 1 < bx.ny && z + 1 < nz) continue;
+          const unsigned k = bx.off + x + bx.nx * (y + bx.ny * z);
+          hf[k] = uclbs::kFlagGhost;
+          const int G[3] = {lo[3 * b] - 1 + (int)x, lo[3 * b + 1] - 1 + (int)y, lo[3 * b + 2] - 1 + (int)z};
+          const int64_t src = valid_index(L, G);
+          if (src >= 0) {
+            copies.push_back(make_uint2(k, (unsigned)src));
+            continue;
+          }
+          uclbs::C2FStencil st{};
+          st.dst = k;
+          if (!C || !c2f_stencil(*C, G, st)) {
+            free_level(L);
+            return set_err(UCLBS_E_ARG, "ghost node (%d,%d,%d) of box %d of level %d is neither a "
+                                        "valid node of the level nor interpolable from level %d",
+                           G[0], G[1], G[2], b, lvl, lvl - 1);
+          }
+          c2f.push_back(st);
+        }
+  }
+  // fine_to_coarse: coarse node C <- fine node 2C for every even valid fine node
+  std::vector<uint2> inj;
+  if (C) {
+    for (int b = 0; b < nbox; ++b) {
+      int c0[3], c1[3];
+      for (int d = 0; d < 3; ++d) {
+        c0[d] = (lo[3 * b + d] + 1) >> 1;
+        c1[d] = hi[3 * b + d] >> 1;
+      }
+      for (int z = c0[2]; z <= c1[2]; ++z)
+        for (int y = c0[1]; y <= c1[1]; ++y)
+          for (int x = c0[0]; x <= c1[0]; ++x) {
+            const int P[3] = {x, y, z}, Fp[3] = {2 * x, 2 * y, 2 * z};
+            const int64_t kc = valid_index(*C, P), kf = valid_index(L, Fp);
+            if (kc < 0 || kf < 0) {
+              free_level(L);
+              return set_err(UCLBS_E_ARG, "level %d box %d is not nested in level %d", lvl, b, lvl - 1);
+            }
+            inj.push_back(make_uint2((unsigned)kc, (unsigned)kf));
+          }
+    }
+  }
+  const size_t bytes = (size_t)off * q_of(c) * elem_size(c);
+  for (auto& p : L.f) {
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      free_level(L);
+      return set_err(UCLBS_E_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+  }
+  HIP_TRY(hipMalloc(&L.alpha, (size_t)off * sizeof(double)));
+  HIP_TRY(hipMalloc(&L.defer_count, 2 * (uclbs::kFarShards + 1) * sizeof(unsigned)));
+  HIP_TRY(hipMemset(L.defer_count, 0, 2 * (uclbs::kFarShards + 1) * sizeof(unsigned)));
+  L.far_parity = 0;
+  // queue shards sized for tiles of 64 nodes (the smallest tile shape)
+  {
+    const int64_t cap = (((int64_t)off + 63) / 64 + nbox + uclbs::kFarShards - 1) / uclbs::kFarShards * 256;
+    L.defer_cap = (unsigned)cap;
+    HIP_TRY(hipMalloc(&L.defer_list, (size_t)(cap * uclbs::kFarShards + 1) * sizeof(uint2)));
+  }
+  HIP_TRY(hipMalloc(&L.flags, (size_t)off));
+  HIP_TRY(hipMemcpy(L.flags, hf.data(), hf.size(), hipMemcpyHostToDevice));
+  HIP_TRY(upload_vec(&L.d_boxes, L.hbox));
+  HIP_TRY(build_tiles(L, 256));
+  HIP_TRY(upload_vec(&L.d_copies, copies));
+  L.ncopies = (unsigned)copies.size();
+  HIP_TRY(upload_vec(&L.d_c2f, c2f));
+  L.nc2f = (unsigned)c2f.size();
+  HIP_TRY(upload_vec(&L.d_inject, inj));
+  L.ninject = (unsigned)inj.size();
+  L.refined = lvl > 0;
+  L.step = 0;
+  L.cur = 0;
+  L.defined = true;
   return UCLBS_OK;
 }
 
@@ -641,6 +1023,7 @@ int uclbs_init_tgv(void* ctx, int lvl, double u0, double noise_amp, uint64_t see
   if (rc) return rc;
   if (c->cfg.lattice != UCLBS_LATTICE_D3Q27)
     return set_err(UCLBS_E_UNSUPPORTED, "init_tgv is a D3Q27 initialiser");
+  if (L->multibox) return set_err(UCLBS_E_UNSUPPORTED, "init_tgv fills a single box (upload the state)");
   if (!(fabs(u0) < 0.5) || !(noise_amp >= 0.0))
     return set_err(UCLBS_E_ARG, "init_tgv: |u0| must be < 0.5 and noise_amp >= 0");
   if ((rc = set_device(c))) return rc;
@@ -676,6 +1059,7 @@ int uclbs_set_faces(void* ctx, int lvl, const int32_t face_code[6]) {
   Level* L;
   int rc = check_level(c, lvl, &L);
   if (rc) return rc;
+  if (L->multibox) return set_err(UCLBS_E_UNSUPPORTED, "no link tables on levels of many boxes");
   if (L->refined) return set_err(UCLBS_E_UNSUPPORTED, "level %d is a refined level (no link tables)", lvl);
   if (!face_code) return set_err(UCLBS_E_ARG, "null face_code");
   for (int f = 0; f < 6; ++f) {
@@ -728,6 +1112,7 @@ int uclbs_build_links(void* ctx, int lvl, const double origin[3], double dx, int
   Level* L;
   int rc = check_level(c, lvl, &L);
   if (rc) return rc;
+  if (L->multibox) return set_err(UCLBS_E_UNSUPPORTED, "no link tables on levels of many boxes");
   if (L->refined) return set_err(UCLBS_E_UNSUPPORTED, "level %d is a refined level (no link tables)", lvl);
   if (c->cfg.lattice != UCLBS_LATTICE_D3Q27)
     return set_err(UCLBS_E_UNSUPPORTED, "build_links: D3Q27 tables only");
@@ -791,6 +1176,7 @@ int uclbs_set_links(void* ctx, int lvl, const int32_t* state, int64_t n_nodes) {
   Level* L;
   int rc = check_level(c, lvl, &L);
   if (rc) return rc;
+  if (L->multibox) return set_err(UCLBS_E_UNSUPPORTED, "no link tables on levels of many boxes");
   if (L->refined) return set_err(UCLBS_E_UNSUPPORTED, "level %d is a refined level (no link tables)", lvl);
   if (!state || n_nodes != L->box.n) return set_err(UCLBS_E_ARG, "set_links: wrong node count");
   if ((rc = set_device(c))) return rc;
@@ -1156,8 +1542,42 @@ static int interlevel_fill(Ctx* c, int lvl, double theta, bool all_nodes) {
   if (!(theta >= 0.0 && theta <= 1.0)) return set_err(UCLBS_E_ARG, "theta outside [0, 1]");
   if ((rc = set_device(c))) return rc;
   const Level& C = c->levels[lvl - 1];
-  const uclbs::LevelMap cm = level_map(C), fm = level_map(*F);
   const int sh = c->cfg.precision == UCLBS_PREC_FP32_SHIFTED;
+  if (F->multibox) {
+    if (!all_nodes)
+      return hip_rc(uclbs::launch_c2f_stencils27(C.f[1 - C.cur], C.f[C.cur], (unsigned)C.box.n,
+                                                 F->f[F->cur], (unsigned)F->box.n, F->d_c2f,
+                                                 F->nc2f, theta, sh, c->d_stats, c->stream));
+    // start-up: every valid node (and every ghost the coarse level reaches)
+    std::vector<uclbs::C2FStencil> all;
+    for (size_t b = 0; b < F->hbox.size(); ++b) {
+      const uclbs::BoxDesc& bx = F->hbox[b];
+      const unsigned nz = bx.n / (bx.nx * bx.ny);
+      for (unsigned z = 0; z < nz; ++z)
+        for (unsigned y = 0; y < bx.ny; ++y)
+          for (unsigned x = 0; x < bx.nx; ++x) {
+            const bool ghost = x == 0 || y == 0 || z == 0 || x + 1 == bx.nx || y + 1 == bx.ny || z + 1 == nz;
+            const int G[3] = {F->blo[3 * b] - 1 + (int)x, F->blo[3 * b + 1] - 1 + (int)y,
+                              F->blo[3 * b + 2] - 1 + (int)z};
+            uclbs::C2FStencil st{};
+            st.dst = bx.off + x + bx.nx * (y + bx.ny * z);
+            if (c2f_stencil(C, G, st)) all.push_back(st);
+            else if (!ghost) return set_err(UCLBS_E_STATE, "fine node outside the coarse level");
+          }
+    }
+    uclbs::C2FStencil* d = nullptr;
+    HIP_TRY(upload_vec(&d, all));
+    hipError_t e = uclbs::launch_c2f_stencils27(C.f[1 - C.cur], C.f[C.cur], (unsigned)C.box.n,
+                                                F->f[F->cur], (unsigned)F->box.n, d,
+                                                (unsigned)all.size(), theta, sh, c->d_stats, c->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(F->f[1 - F->cur], F->f[F->cur], level_bytes(c, F), hipMemcpyDeviceToDevice,
+                         c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    (void)hipFree(d);
+    return hip_rc(e);
+  }
+  const uclbs::LevelMap cm = level_map(C), fm = level_map(*F);
   if (all_nodes) {
     HIP_TRY(uclbs::launch_coarse_to_fine27(C.f[1 - C.cur], C.f[C.cur], cm, F->f[F->cur], fm, nullptr,
                                            (unsigned)F->box.n, theta, sh, c->d_stats, c->stream));
@@ -1193,6 +1613,10 @@ int uclbs_fine_to_coarse(void* ctx, int lvl) {
   if (!F->refined) return set_err(UCLBS_E_ARG, "level %d is not a refined level", lvl);
   if ((rc = set_device(c))) return rc;
   Level& C = c->levels[lvl - 1];
+  if (F->multibox)
+    return hip_rc(uclbs::launch_copy_pairs27(F->f[F->cur], (unsigned)F->box.n, C.f[C.cur],
+                                             (unsigned)C.box.n, F->d_inject, F->ninject,
+                                             c->cfg.precision == UCLBS_PREC_FP32_SHIFTED, c->stream));
   uclbs::Range3 cover{};
   for (int d = 0; d < 3; ++d) {
     cover.lo[d] = (F->lo[d] + 1) >> 1;  // coarse nodes whose fine image 2C is valid
@@ -1200,6 +1624,153 @@ int uclbs_fine_to_coarse(void* ctx, int lvl) {
   }
   HIP_TRY(uclbs::launch_fine_to_coarse27(F->f[F->cur], level_map(*F), C.f[C.cur], level_map(C), cover,
                                          c->cfg.precision == UCLBS_PREC_FP32_SHIFTED, c->stream));
+  return UCLBS_OK;
+}
+
+}  // extern "C"
+
+// One step of level lvl and, recursively, two substeps of each finer level:
+// ghosts from the coarser level at its previous state (theta 0) and half way
+// (theta 1/2), then the finer level injected back -- the phase order of
+// loop.f95:104-139 under the level schedule of valid_levels.f95:21-59.
+static int advance_rec(Ctx* c, int lvl) {
+  int rc = launch_steps(c, &c->levels[lvl], 1);
+  if (rc) return rc;
+  const int f = lvl + 1;
+  if (f < (int)c->levels.size() && c->levels[f].defined && c->levels[f].refined) {
+    for (double theta : {0.0, 0.5}) {
+      if ((rc = interlevel_fill(c, f, theta, false))) return rc;
+      if ((rc = advance_rec(c, f))) return rc;
+    }
+    rc = uclbs_fine_to_coarse(c, f);
+  }
+  return rc;
+}
+
+// what a captured advance depends on: buffer and queue parities of every
+// level, the kernel variants and the level structure
+static std::vector<int> graph_signature(const Ctx* c) {
+  std::vector<int> sig = {(int)c->structure, (int)c->keep_alpha, c->cfg.alpha_mode,
+                          (int)c->levels.size()};
+  for (const Level& L : c->levels) {
+    sig.push_back(L.defined);
+    sig.push_back(L.cur);
+    sig.push_back(L.far_parity);
+    sig.push_back(L.tile_nodes);
+  }
+  return sig;
+}
+
+static int levels_in_use(const Ctx* c) {
+  int n = 1;
+  while (n < (int)c->levels.size() && c->levels[n].defined && c->levels[n].refined) ++n;
+  return n;
+}
+
+extern "C" {
+
+int uclbs_advance(void* ctx, int ncoarse, int use_graph) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c) return set_err(UCLBS_E_ARG, "null context");
+  if (ncoarse < 0) return set_err(UCLBS_E_ARG, "ncoarse < 0");
+  if (c->levels.empty() || !c->levels[0].defined) return set_err(UCLBS_E_STATE, "level 0 undefined");
+  if (c->cfg.lattice != UCLBS_LATTICE_D3Q27)
+    return set_err(UCLBS_E_UNSUPPORTED, "uclbs_advance drives D3Q27 levels");
+  int rc = set_device(c);
+  if (rc) return rc;
+  const int nl = levels_in_use(c);
+  const int pairs = use_graph ? ncoarse / 2 : 0;
+  if (pairs > 0) {
+    const std::vector<int> sig = graph_signature(c);
+    if (!c->graph || c->graph_sig != sig) {
+      if (c->graph) (void)hipGraphExecDestroy(c->graph);
+      c->graph = nullptr;
+      // capture two coarse steps: every level's buffer and queue parities
+      // return to where they were, so the graph replays as it stands
+      const int64_t done0 = c->steps_done;
+      HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      rc = advance_rec(c, 0);
+      if (!rc) rc = advance_rec(c, 0);
+      hipGraph_t g = nullptr;
+      const hipError_t e = hipStreamEndCapture(c->stream, &g);
+      c->steps_done = done0;
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess) return set_err(UCLBS_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
+      hipGraphExec_t x = nullptr;
+      const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ei != hipSuccess)
+        return set_err(UCLBS_E_HIP, "graph instantiation failed: %s", hipGetErrorString(ei));
+      if (graph_signature(c) != sig) {
+        (void)hipGraphExecDestroy(x);
+        return set_err(UCLBS_E_STATE, "captured advance does not return to its start state");
+      }
+      c->graph = x;
+      c->graph_sig = sig;
+    }
+    for (int k = 0; k < pairs; ++k) HIP_TRY(hipGraphLaunch(c->graph, c->stream));
+    for (int l = 0; l < nl; ++l)
+      c->steps_done += (int64_t)pairs * (2ll << l);  // level l steps 2^l times per coarse step
+  }
+  for (int k = 2 * pairs; k < ncoarse; ++k)
+    if ((rc = advance_rec(c, 0))) return rc;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return UCLBS_OK;
+}
+
+int uclbs_autotune(void* ctx, int lvl, int* tile_nodes) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  Level* L;
+  int rc = check_level(c, lvl, &L);
+  if (rc) return rc;
+  if (!L->multibox) return set_err(UCLBS_E_UNSUPPORTED, "autotune picks the tile shape of a level of many boxes");
+  if ((rc = set_device(c))) return rc;
+  // keep the state: the trial steps run on a copy of it
+  const size_t bytes = level_bytes(c, L);
+  const size_t sbytes = sizeof(unsigned long long) * uclbs::kStatSlots * uclbs::kStatFields;
+  void* save = nullptr;
+  unsigned long long* ssave = nullptr;
+  HIP_TRY(hipMalloc(&save, bytes));
+  hipError_t e = hipMalloc(&ssave, sbytes);
+  if (e == hipSuccess) e = hipMemcpyAsync(save, L->f[L->cur], bytes, hipMemcpyDeviceToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(ssave, c->d_stats, sbytes, hipMemcpyDeviceToDevice, c->stream);
+  const int cur0 = L->cur, par0 = L->far_parity;
+  const int64_t done0 = c->steps_done, step0 = L->step;
+  const int shapes[3] = {64, 128, 256};
+  int best = L->tile_nodes;
+  float best_ms = 1e30f;
+  for (int t = 0; t < 3 && e == hipSuccess && !rc; ++t) {
+    e = build_tiles(*L, shapes[t]);
+    // one untimed and three timed steps per shape (an even total, so the
+    // buffer and queue parities end where they began)
+    if (e == hipSuccess) rc = launch_steps(c, L, 1);
+    if (!rc && e == hipSuccess) e = hipEventRecord(c->ev0, c->stream);
+    if (!rc && e == hipSuccess) rc = launch_steps(c, L, 3);
+    if (!rc && e == hipSuccess) e = hipEventRecord(c->ev1, c->stream);
+    if (!rc && e == hipSuccess) e = hipEventSynchronize(c->ev1);
+    float ms = 0.0f;
+    if (!rc && e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    if (!rc && e == hipSuccess && ms < best_ms) {
+      best_ms = ms;
+      best = shapes[t];
+    }
+  }
+  L->cur = cur0;
+  L->far_parity = par0;
+  c->steps_done = done0;
+  L->step = step0;
+  if (e == hipSuccess) e = hipMemcpyAsync(L->f[L->cur], save, bytes, hipMemcpyDeviceToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(c->d_stats, ssave, sbytes, hipMemcpyDeviceToDevice, c->stream);
+  if (e == hipSuccess) e = build_tiles(*L, best);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(save);
+  if (ssave) (void)hipFree(ssave);
+  if (rc) return rc;
+  if (e != hipSuccess) return set_err(UCLBS_E_HIP, "autotune failed: %s", hipGetErrorString(e));
+  if (tile_nodes) *tile_nodes = best;
   return UCLBS_OK;
 }
 
