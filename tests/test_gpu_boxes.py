@@ -161,3 +161,22 @@ def test_three_level_hierarchy_conserves_and_stays_finite(pkg, orc):
         assert np.isfinite(s.macro(lvl)[0]).all()
     st = s.stats()
     assert st["n_noconv"] == 0 and st["n_nonpos"] == 0
+
+
+def test_level0_boxes_tgv_init_matches_single_box(pkg):
+    shape = (32, 24, 20)
+    ref = pkg.Solver(precision="fp32s", nu=0.005, alpha_tol=1e-10)
+    ref.define_level(0, shape)
+    ref.init_tgv(0, 0.05, 1e-3, 99)
+    s = pkg.Solver(precision="fp32s", nu=0.005, alpha_tol=1e-10)
+    s.define_level_boxes(0, _tiling((0, 0, 0), tuple(n - 1 for n in shape), (5, 8, 12), 23))
+    s.init_tgv(0, 0.05, 1e-3, 99)
+    dshape = (shape[2], shape[1], shape[0])
+    fr = ref.download_f(0).reshape((27,) + dshape)
+    assert np.array_equal(fr, s.stored_to_dense(0, s.download_f(0), dshape))
+    # the ghost shell holds the wrapped domain values as well
+    assert np.array_equal(s.download_f(0), s.dense_to_stored(0, fr))
+    ref.step(0, 4)
+    s.step(0, 4)
+    assert np.array_equal(ref.download_f(0).reshape((27,) + dshape),
+                          s.stored_to_dense(0, s.download_f(0), dshape))

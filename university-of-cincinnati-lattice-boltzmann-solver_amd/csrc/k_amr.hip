@@ -31,7 +31,8 @@ template <bool kShifted>
 __device__ __forceinline__ void repair_store27(double (&f)[27],
                                                typename std::conditional<kShifted, float, double>::type* __restrict__ ff,
                                                unsigned fnn, unsigned node,
-                                               unsigned long long* __restrict__ stats);
+                                               unsigned long long* __restrict__ stats,
+                                               unsigned mask = kAllPops27);
 
 template <bool kShifted>
 __global__ __launch_bounds__(256) void k27_coarse_to_fine(
@@ -91,7 +92,8 @@ template <bool kShifted>
 __device__ __forceinline__ void repair_store27(double (&f)[27],
                                                typename std::conditional<kShifted, float, double>::type* __restrict__ ff,
                                                unsigned fnn, unsigned node,
-                                               unsigned long long* __restrict__ stats) {
+                                               unsigned long long* __restrict__ stats,
+                                               unsigned mask) {
   using T = typename std::conditional<kShifted, float, double>::type;
   bool neg = false;
 #pragma unroll
@@ -128,7 +130,8 @@ __device__ __forceinline__ void repair_store27(double (&f)[27],
     atomicAdd(stats + (size_t)(blockIdx.x & (kStatSlots - 1)) * kStatFields + ST_REPAIR, nrep);
   }
 #pragma unroll
-  for (int i = 0; i < 27; ++i) ff[(size_t)i * fnn + node] = (T)f[i];
+  for (int i = 0; i < 27; ++i)
+    if ((mask >> i) & 1u) ff[(size_t)i * fnn + node] = (T)f[i];
 }
 
 // ghost nodes of a level of many boxes from precomputed coarse stencils; the
@@ -159,20 +162,21 @@ __global__ __launch_bounds__(256) void k27_c2f_stencils(const void* __restrict__
       f[i] += w * ((1.0 - theta) * a + theta * b);
     }
   }
-  repair_store27<kShifted>(f, static_cast<T*>(ff_), fnn, s.dst, stats);
+  repair_store27<kShifted>(f, static_cast<T*>(ff_), fnn, s.dst, stats, s.mask);
 }
 
 // src and dst may be the same level (ghost copies): a pair's source is a valid
 // node, never a ghost, so no pair reads what another writes
 template <typename T>
 __global__ __launch_bounds__(256) void k27_copy_pairs(const T* src, unsigned snn, T* dst, unsigned dnn,
-                                                      const uint2* __restrict__ pairs,
+                                                      const uint3* __restrict__ pairs,
                                                       unsigned npairs) {
   const unsigned j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= npairs) return;
-  const uint2 p = pairs[j];
+  const uint3 p = pairs[j];
 #pragma unroll
-  for (int i = 0; i < 27; ++i) dst[(size_t)i * dnn + p.x] = src[(size_t)i * snn + p.y];
+  for (int i = 0; i < 27; ++i)
+    if ((p.z >> i) & 1u) dst[(size_t)i * dnn + p.x] = src[(size_t)i * snn + p.y];
 }
 
 // injection: covered coarse node C <- fine node 2C
@@ -214,7 +218,7 @@ hipError_t launch_coarse_to_fine27(const void* cprev, const void* cnew, const Le
 }
 
 hipError_t launch_copy_pairs27(const void* src, unsigned nn_src, void* dst, unsigned nn_dst,
-                               const uint2* pairs, unsigned npairs, int shifted, hipStream_t s) {
+                               const uint3* pairs, unsigned npairs, int shifted, hipStream_t s) {
   if (!npairs) return hipSuccess;
   const unsigned g = (npairs + 255) / 256;
   if (shifted)

@@ -199,40 +199,60 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 
 // All boxes of a level in one launch: block b takes tile b (tile_nodes nodes
 // of one box), pulls over the box's strides -- a valid node's neighbours lie in
-// the box or its ghost shell -- and collides as k27_step_periodic does.
-// Ghost-shell nodes are filled by the copy / interpolation kernels beforehand.
+// the box or its ghost shell -- and collides as k27_step_periodic does.  A
+// ghost node that shows a valid node s of the level (kFlagMirror) repeats s's
+// update -- the pull around s in s's box and the same collide, so the same
+// bits -- and stores it in place: the shell is current after the step with no
+// copy pass (a deferred mirror is queued too, marked, and solved by the far
+// kernel like s).  Ghosts at a coarse-fine interface are interpolated before
+// the step.
 template <bool kShifted, bool kKeepAlpha>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k27_step_boxes(
     const void* __restrict__ fin_, void* __restrict__ fout_, double* __restrict__ alpha_out,
     unsigned long long* __restrict__ stats, unsigned nn, StepArgs a,
-    const uint8_t* __restrict__ flags, const BoxTile* __restrict__ tiles,
-    const BoxDesc* __restrict__ boxes, int tile_nodes) {
+    const uint8_t* __restrict__ flags, const uint2* __restrict__ alias,
+    const BoxTile* __restrict__ tiles, const BoxDesc* __restrict__ boxes, int tile_nodes) {
   using T = typename std::conditional<kShifted, float, double>::type;
   const T* __restrict__ fin = static_cast<const T*>(fin_);
   T* __restrict__ fout = static_cast<T*>(fout_);
   const BoxTile t = tiles[blockIdx.x];
   const BoxDesc bx = boxes[t.box];
   const unsigned local = t.start + threadIdx.x;
-  bool active = (int)threadIdx.x < tile_nodes && local < bx.n;
-  const unsigned node = bx.off + (active ? local : 0u);
-  active = active && flags[node] == kFlagInterior;
-  const unsigned sy = bx.nx, sz = bx.nx * bx.ny;
+  const bool in_box = (int)threadIdx.x < tile_nodes && local < bx.n;
+  const unsigned node = bx.off + (in_box ? local : 0u);
+  const uint8_t fl = in_box ? flags[node] : kFlagGhost;
+  const bool active = fl == kFlagInterior;  // counted node updates
+  const bool mirror = fl == kFlagMirror;
+  // the node whose update this lane computes, and that node's box strides
+  unsigned src = node, sy = bx.nx, sz = bx.nx * bx.ny;
+  if (mirror) {
+    const uint2 al = alias[node];
+    const BoxDesc sb = boxes[al.y];
+    src = al.x;
+    sy = sb.nx;
+    sz = sb.nx * sb.ny;
+  }
+  const bool pull = active || mirror;
   T f[27];
 #pragma unroll
   for (int i = 0; i < 27; ++i)
-    f[i] = pop(fin, i, nn, active ? node - CX27[i] - CY27[i] * sy - CZ27[i] * sz : node);
+    f[i] = pop(fin, i, nn, pull ? src - CX27[i] - CY27[i] * sy - CZ27[i] * sz : node);
   AlphaCfg ac{a.tol, a.maxit, a.alpha_mode, a.thr};
   double rho, u[3];
   AlphaOut ao;
   if constexpr (kShifted) ao = collide27_shifted(f, a.beta, ac, 2.0, rho, u);
   else ao = collide27_fp64(f, a.beta, ac, 2.0, rho, u);
-  if (active) {
+  // ghost-shell lanes store too (values nobody reads: the shell is refilled
+  // before the next step), so every line of the box is written whole and the
+  // L2 does not fetch partly written lines
+  if (in_box) {
 #pragma unroll
     for (int i = 0; i < 27; ++i) pop(fout, i, nn, node) = f[i];
-    if (kKeepAlpha) alpha_out[node] = ao.alpha;
   }
+  if (kKeepAlpha && active) alpha_out[node] = ao.alpha;
   if constexpr (kShifted)
-    queue_far(a, blockIdx.x % kFarShards, active && ao.status == ALPHA_DEFER, node, ao.alpha);
+    queue_far(a, blockIdx.x % kFarShards, pull && ao.status == ALPHA_DEFER,
+              mirror ? (node | kMirrorBit) : node, ao.alpha);
   wave_stats(stats, active, ao.iters, ao.status == ALPHA_DEFER ? -1 : ao.status, ao.fallback,
              ao.logpath);
 }
@@ -297,15 +317,16 @@ __global__ __launch_bounds__(kFarBlock) void k27_far(float* __restrict__ f_, dou
       if (pre[mid] <= kk) lo = mid; else hi = mid - 1;
     }
     const uint2 e = a.defer_list[(size_t)lo * a.defer_cap + (kk - pre[lo])];
-    const unsigned node = e.x;
+    const unsigned node = e.x & ~kMirrorBit;
+    const bool counted = active && !(e.x & kMirrorBit);
     float f[27];
 #pragma unroll
     for (int i = 0; i < 27; ++i) f[i] = (f_ + (size_t)i * nn)[node];
     const double seed = (double)__uint_as_float(e.y);
     AlphaOut ao = collide27_far(f, a.beta, ac, seed, sf + threadIdx.x, sD + threadIdx.x, active,
                                 f_, nn, node);
-    if (active && keep_alpha) alpha[node] = ao.alpha;
-    wave_stats(stats, active, ao.iters, ao.status, false, ao.logpath);
+    if (counted && keep_alpha) alpha[node] = ao.alpha;
+    wave_stats(stats, counted, ao.iters, ao.status, false, ao.logpath);
   }
 }
 
@@ -361,21 +382,31 @@ __global__ void k27_macro(const void* __restrict__ src_, Box b, double* __restri
 // Taylor-Green vortex: u = U0 (sin X cos Y cos Z, -cos X sin Y cos Z, 0) plus
 // uniform noise in (-amp, amp) per component from Philox4x32-10 keyed by the
 // seed and counted by (node, component); f = feq(1, u).
+// Region form: the stored nodes of a box of extents ext whose node (0,0,0)
+// sits at domain position org (wrapped into the periodic domain dom), written
+// from node out_off of arrays of nn nodes; the field and the noise counter
+// follow the domain position, so a domain split into boxes starts from
+// exactly the state of the domain as one box.
 template <bool kShifted>
-__global__ void k27_init_tgv(void* __restrict__ dst_, Box b, double u0, double amp,
+__global__ void k27_init_tgv(void* __restrict__ dst_, Box dom, int64_t nn, int64_t out_off,
+                             int ex, int ey, int ez, int ox, int oy, int oz, double u0, double amp,
                              uint64_t seed) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= b.n) return;
-  const int x = (int)(k % b.nx);
-  const int64_t yz = k / b.nx;
-  const int y = (int)(yz % b.ny), z = (int)(yz / b.ny);
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= (int64_t)ex * ey * ez) return;
+  const int lx = (int)(r % ex);
+  const int64_t lyz = r / ex;
+  const int ly = (int)(lyz % ey), lz = (int)(lyz / ey);
+  const int x = ((ox + lx) % dom.nx + dom.nx) % dom.nx, y = ((oy + ly) % dom.ny + dom.ny) % dom.ny,
+            z = ((oz + lz) % dom.nz + dom.nz) % dom.nz;
+  const int64_t kd = x + (int64_t)dom.nx * (y + (int64_t)dom.ny * z);
+  const int64_t k = out_off + r;
   const double tp = 6.283185307179586476925286766559;
-  const double X = tp * x / b.nx, Y = tp * y / b.ny, Z = tp * z / b.nz;
+  const double X = tp * x / dom.nx, Y = tp * y / dom.ny, Z = tp * z / dom.nz;
   double u[3];
   u[0] = u0 * sin(X) * cos(Y) * cos(Z);
   u[1] = -u0 * cos(X) * sin(Y) * cos(Z);
   u[2] = 0.0;
-  for (int c = 0; c < 3; ++c) u[c] += amp * (2.0 * philox_uniform((uint64_t)k, (uint32_t)c, seed) - 1.0);
+  for (int c = 0; c < 3; ++c) u[c] += amp * (2.0 * philox_uniform((uint64_t)kd, (uint32_t)c, seed) - 1.0);
   double px[3], py[3], pz[3];
   auto psi = [](double uu, double* p) {
     const double s = sqrt(1.0 + 3.0 * uu * uu);
@@ -388,8 +419,8 @@ __global__ void k27_init_tgv(void* __restrict__ dst_, Box b, double u0, double a
 #pragma unroll
   for (int i = 0; i < 27; ++i) {
     const double feq = W27[i] * px[CX27[i] + 1] * py[CY27[i] + 1] * pz[CZ27[i] + 1];
-    if constexpr (kShifted) static_cast<float*>(dst_)[(int64_t)i * b.n + k] = (float)(feq - W27[i]);
-    else static_cast<double*>(dst_)[(int64_t)i * b.n + k] = feq;
+    if constexpr (kShifted) static_cast<float*>(dst_)[(int64_t)i * nn + k] = (float)(feq - W27[i]);
+    else static_cast<double*>(dst_)[(int64_t)i * nn + k] = feq;
   }
 }
 
@@ -689,17 +720,18 @@ hipError_t launch_d3q27_step_links(const Box& b, const void* fin, void* fout, do
 
 hipError_t launch_d3q27_step_boxes(unsigned nn, const void* fin, void* fout, double* alpha,
                                    int keep_alpha, unsigned long long* stats, const StepArgs& a,
-                                   const uint8_t* flags, const BoxTile* tiles, unsigned ntiles,
-                                   const BoxDesc* boxes, int tile_nodes, hipStream_t s) {
+                                   const uint8_t* flags, const uint2* alias, const BoxTile* tiles,
+                                   unsigned ntiles, const BoxDesc* boxes, int tile_nodes,
+                                   hipStream_t s) {
   if (!ntiles) return hipSuccess;
   if (tile_nodes < 64 || tile_nodes > 256 || tile_nodes % 64) return hipErrorInvalidValue;
   const dim3 g(ntiles), blk((unsigned)tile_nodes);
   if (a.shifted) {
-    if (keep_alpha) k27_step_boxes<true, true><<<g, blk, 0, s>>>(fin, fout, alpha, stats, nn, a, flags, tiles, boxes, tile_nodes);
-    else k27_step_boxes<true, false><<<g, blk, 0, s>>>(fin, fout, alpha, stats, nn, a, flags, tiles, boxes, tile_nodes);
+    if (keep_alpha) k27_step_boxes<true, true><<<g, blk, 0, s>>>(fin, fout, alpha, stats, nn, a, flags, alias, tiles, boxes, tile_nodes);
+    else k27_step_boxes<true, false><<<g, blk, 0, s>>>(fin, fout, alpha, stats, nn, a, flags, alias, tiles, boxes, tile_nodes);
   } else {
-    if (keep_alpha) k27_step_boxes<false, true><<<g, blk, 0, s>>>(fin, fout, alpha, stats, nn, a, flags, tiles, boxes, tile_nodes);
-    else k27_step_boxes<false, false><<<g, blk, 0, s>>>(fin, fout, alpha, stats, nn, a, flags, tiles, boxes, tile_nodes);
+    if (keep_alpha) k27_step_boxes<false, true><<<g, blk, 0, s>>>(fin, fout, alpha, stats, nn, a, flags, alias, tiles, boxes, tile_nodes);
+    else k27_step_boxes<false, false><<<g, blk, 0, s>>>(fin, fout, alpha, stats, nn, a, flags, alias, tiles, boxes, tile_nodes);
   }
   return hipGetLastError();
 }
@@ -754,10 +786,27 @@ hipError_t launch_d3q27_macro(const Box& b, const void* f, int shifted, double* 
   return hipGetLastError();
 }
 
+hipError_t launch_d3q27_init_tgv_region(const Box& dom, void* f, int64_t nn, int64_t off,
+                                        const int ext[3], const int org[3], int shifted, double u0,
+                                        double noise_amp, uint64_t seed, hipStream_t s) {
+  const int64_t n = (int64_t)ext[0] * ext[1] * ext[2];
+  if (shifted)
+    k27_init_tgv<true><<<blocks_for(n, 256), 256, 0, s>>>(f, dom, nn, off, ext[0], ext[1], ext[2],
+                                                           org[0], org[1], org[2], u0, noise_amp, seed);
+  else
+    k27_init_tgv<false><<<blocks_for(n, 256), 256, 0, s>>>(f, dom, nn, off, ext[0], ext[1], ext[2],
+                                                            org[0], org[1], org[2], u0, noise_amp, seed);
+  return hipGetLastError();
+}
+
 hipError_t launch_d3q27_init_tgv(const Box& b, void* f, int shifted, double u0,
                                  double noise_amp, uint64_t seed, hipStream_t s) {
-  if (shifted) k27_init_tgv<true><<<blocks_for(b.n, 256), 256, 0, s>>>(f, b, u0, noise_amp, seed);
-  else k27_init_tgv<false><<<blocks_for(b.n, 256), 256, 0, s>>>(f, b, u0, noise_amp, seed);
+  if (shifted)
+    k27_init_tgv<true><<<blocks_for(b.n, 256), 256, 0, s>>>(f, b, b.n, 0, b.nx, b.ny, b.nz, 0, 0, 0,
+                                                             u0, noise_amp, seed);
+  else
+    k27_init_tgv<false><<<blocks_for(b.n, 256), 256, 0, s>>>(f, b, b.n, 0, b.nx, b.ny, b.nz, 0, 0, 0,
+                                                              u0, noise_amp, seed);
   return hipGetLastError();
 }
 

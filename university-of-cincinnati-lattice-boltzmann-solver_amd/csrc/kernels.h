@@ -45,7 +45,11 @@ constexpr int kFarShards = 256;
 
 // node flags of a level with link tables
 enum : uint8_t { kFlagInterior = 0, kFlagBoundary = 1, kFlagSolid = 2, kFlagFace = 3,
-                 kFlagGhost = 4 /* ghost shell of a refined level, filled from the coarser one */ };
+                 kFlagGhost = 4 /* ghost shell of a refined level, filled from the coarser one */,
+                 kFlagMirror = 5 /* ghost node of a box that shows a valid node of the same
+                                    level: updated as a copy of that node's update */ };
+// far-queue entries of mirror nodes carry this bit (solved, not counted)
+constexpr unsigned kMirrorBit = 0x80000000u;
 
 struct BcTables {
   const double* feq_in;
@@ -107,6 +111,9 @@ hipError_t launch_d3q27_to_fp64(const Box& b, const void* src, double* dst, int 
                                 hipStream_t s);
 hipError_t launch_d3q27_macro(const Box& b, const void* f, int shifted, double* rho,
                               double* ux, double* uy, double* uz, hipStream_t s);
+hipError_t launch_d3q27_init_tgv_region(const Box& dom, void* f, int64_t nn, int64_t off,
+                                        const int ext[3], const int org[3], int shifted, double u0,
+                                        double noise_amp, uint64_t seed, hipStream_t s);
 hipError_t launch_d3q27_init_tgv(const Box& b, void* f, int shifted, double u0,
                                  double noise_amp, uint64_t seed, hipStream_t s);
 
@@ -140,16 +147,22 @@ struct BoxTile {    // one block of the batched streaming kernel
 };
 struct C2FStencil { // a ghost node interpolated from the coarser level
   unsigned dst;     // fine stored node
+  unsigned mask;    // populations stored (bit i; all 27 are interpolated)
   unsigned src[8];  // coarse stored nodes, tensor order (z outer, x inner)
   float w[8];       // their weights (products of 1 and 1/2; 0 = unused)
 };
+// alias[k] = {valid node mirrored by node k, its box} for kFlagMirror nodes
 hipError_t launch_d3q27_step_boxes(unsigned nn, const void* fin, void* fout, double* alpha,
                                    int keep_alpha, unsigned long long* stats, const StepArgs& a,
-                                   const uint8_t* flags, const BoxTile* tiles, unsigned ntiles,
-                                   const BoxDesc* boxes, int tile_nodes, hipStream_t s);
-// dst[i][p.x] = src[i][p.y] for each pair p (ghost copies and injection)
+                                   const uint8_t* flags, const uint2* alias, const BoxTile* tiles,
+                                   unsigned ntiles, const BoxDesc* boxes, int tile_nodes,
+                                   hipStream_t s);
+// dst[i][p.x] = src[i][p.y] for each pair p and each population i in p.z
+// (bit i; ghost copies carry only the populations the box pulls from a ghost
+// node, injections all 27)
+constexpr unsigned kAllPops27 = (1u << 27) - 1u;
 hipError_t launch_copy_pairs27(const void* src, unsigned nn_src, void* dst, unsigned nn_dst,
-                               const uint2* pairs, unsigned npairs, int shifted, hipStream_t s);
+                               const uint3* pairs, unsigned npairs, int shifted, hipStream_t s);
 // time-interpolated coarse stencils (theta between cprev and cnew), negative
 // results replaced by the equilibrium of the interpolated moments
 hipError_t launch_c2f_stencils27(const void* cprev, const void* cnew, unsigned nn_c, void* ff,

@@ -78,11 +78,12 @@ struct Level {
   uclbs::BoxTile* d_tiles = nullptr;
   unsigned ntiles = 0;
   int tile_nodes = 256;
-  uint2* d_copies = nullptr;  // same-level ghost copies {dst, src}
+  uint3* d_copies = nullptr;  // same-level ghost copies {dst, src, population mask}
+  uint2* d_alias = nullptr;   // {mirrored valid node, its box} of every kFlagMirror node
   unsigned ncopies = 0;
   uclbs::C2FStencil* d_c2f = nullptr;  // ghosts interpolated from the coarser level
   unsigned nc2f = 0;
-  uint2* d_inject = nullptr;  // {coarse node, fine node} of fine_to_coarse
+  uint3* d_inject = nullptr;  // {coarse node, fine node, all populations} of fine_to_coarse
   unsigned ninject = 0;
   int dom_lo[3] = {0, 0, 0}, dom_hi[3] = {0, 0, 0};  // level 0: the periodic domain
   // host lookup of the box holding a level position: buckets of edge bk_e over
@@ -178,8 +179,9 @@ void free_level(Level& L) {
   if (L.ghosts) { (void)hipFree(L.ghosts); L.ghosts = nullptr; }
   L.nghost = 0;
   for (void* p : {(void*)L.d_boxes, (void*)L.d_tiles, (void*)L.d_copies, (void*)L.d_c2f,
-                  (void*)L.d_inject})
+                  (void*)L.d_inject, (void*)L.d_alias})
     if (p) (void)hipFree(p);
+  L.d_alias = nullptr;
   L.d_boxes = nullptr;
   L.d_tiles = nullptr;
   L.d_copies = nullptr;
@@ -357,8 +359,9 @@ int launch_steps_boxes(Ctx* c, Level* L, int nsteps, hipEvent_t* ev) {
                                        a.shifted, c->stream));
     if (ev) HIP_TRY(hipEventRecord(ev[2 * s], c->stream));
     HIP_TRY(uclbs::launch_d3q27_step_boxes(nn, L->f[L->cur], L->f[1 - L->cur], L->alpha,
-                                           c->keep_alpha, c->d_stats, a, L->flags, L->d_tiles,
-                                           L->ntiles, L->d_boxes, L->tile_nodes, c->stream));
+                                           c->keep_alpha, c->d_stats, a, L->flags, L->d_alias,
+                                           L->d_tiles, L->ntiles, L->d_boxes, L->tile_nodes,
+                                           c->stream));
     if (ev) HIP_TRY(hipEventRecord(ev[2 * s + 1], c->stream));
     if (a.shifted) {
       HIP_TRY(uclbs::launch_d3q27_far(L->box, L->f[1 - L->cur], L->alpha, c->keep_alpha,
@@ -654,7 +657,7 @@ static void build_buckets(Level& L) {
 // stored index of the valid node at level position G (-1 if no valid node of
 // the level is there).  Periodic axes wrap: a single box over its extent,
 // level 0 of many boxes over its domain.
-static int64_t valid_index(const Level& L, const int G[3]) {
+static int64_t valid_index(const Level& L, const int G[3], int* box_out = nullptr) {
   int g[3] = {G[0], G[1], G[2]};
   if (!L.multibox) {
     int loc[3];
@@ -685,8 +688,7 @@ static int64_t valid_index(const Level& L, const int G[3]) {
     if (g[0] < lo[0] || g[0] > hi[0] || g[1] < lo[1] || g[1] > hi[1] || g[2] < lo[2] || g[2] > hi[2])
       continue;
     const uclbs::BoxDesc& bx = L.hbox[b];
-    const unsigned nz = bx.n / (bx.nx * bx.ny);
-    (void)nz;
+    if (box_out) *box_out = b;
     return (int64_t)bx.off + (g[0] - lo[0] + 1) +
            (int64_t)bx.nx * ((g[1] - lo[1] + 1) + (int64_t)bx.ny * (g[2] - lo[2] + 1));
   }
@@ -829,7 +831,8 @@ int uclbs_define_level_boxes(void* ctx, int lvl, int nbox, const int32_t* lo, co
   build_buckets(L);
   // ghost sources
   std::vector<uint8_t> hf((size_t)off, uclbs::kFlagInterior);
-  std::vector<uint2> copies;
+  std::vector<uint2> alias((size_t)off, make_uint2(0u, 0u));
+  std::vector<uint3> copies;  // none: mirrors are updated by the streaming kernel
   std::vector<uclbs::C2FStencil> c2f;
   const Level* C = lvl > 0 ? &c->levels[lvl - 1] : nullptr;
   for (int b = 0; b < nbox; ++b) {
@@ -841,14 +844,30 @@ int uclbs_define_level_boxes(void* ctx, int lvl, int nbox, const int32_t* lo, co
           if (x > 0 && y > 0 && z > 0 && x + 1 < bx.nx && y + 1 < bx.ny && z + 1 < nz) continue;
           const unsigned k = bx.off + x + bx.nx * (y + bx.ny * z);
           hf[k] = uclbs::kFlagGhost;
+          // populations a valid node pulls from this ghost: c_d = +1 across a
+          // low face, -1 across a high face
+          const unsigned loc[3] = {x, y, z}, ext[3] = {bx.nx, bx.ny, nz};
+          unsigned mask = 0;
+          for (int i = 0; i < 27; ++i) {
+            const int ci[3] = {uclbs::CX27[i], uclbs::CY27[i], uclbs::CZ27[i]};
+            bool need = true;
+            for (int d = 0; d < 3; ++d) {
+              if (loc[d] == 0) need = need && ci[d] == 1;
+              if (loc[d] + 1 == ext[d]) need = need && ci[d] == -1;
+            }
+            if (need) mask |= 1u << i;
+          }
           const int G[3] = {lo[3 * b] - 1 + (int)x, lo[3 * b + 1] - 1 + (int)y, lo[3 * b + 2] - 1 + (int)z};
-          const int64_t src = valid_index(L, G);
+          int sbox = -1;
+          const int64_t src = valid_index(L, G, &sbox);
           if (src >= 0) {
-            copies.push_back(make_uint2(k, (unsigned)src));
+            hf[k] = uclbs::kFlagMirror;
+            alias[k] = make_uint2((unsigned)src, (unsigned)sbox);
             continue;
           }
           uclbs::C2FStencil st{};
           st.dst = k;
+          st.mask = mask;
           if (!C || !c2f_stencil(*C, G, st)) {
             free_level(L);
             return set_err(UCLBS_E_ARG, "ghost node (%d,%d,%d) of box %d of level %d is neither a "
@@ -859,7 +878,7 @@ int uclbs_define_level_boxes(void* ctx, int lvl, int nbox, const int32_t* lo, co
         }
   }
   // fine_to_coarse: coarse node C <- fine node 2C for every even valid fine node
-  std::vector<uint2> inj;
+  std::vector<uint3> inj;
   if (C) {
     for (int b = 0; b < nbox; ++b) {
       int c0[3], c1[3];
@@ -876,7 +895,7 @@ int uclbs_define_level_boxes(void* ctx, int lvl, int nbox, const int32_t* lo, co
               free_level(L);
               return set_err(UCLBS_E_ARG, "level %d box %d is not nested in level %d", lvl, b, lvl - 1);
             }
-            inj.push_back(make_uint2((unsigned)kc, (unsigned)kf));
+            inj.push_back(make_uint3((unsigned)kc, (unsigned)kf, uclbs::kAllPops27));
           }
     }
   }
@@ -904,6 +923,7 @@ int uclbs_define_level_boxes(void* ctx, int lvl, int nbox, const int32_t* lo, co
   HIP_TRY(build_tiles(L, 256));
   HIP_TRY(upload_vec(&L.d_copies, copies));
   L.ncopies = (unsigned)copies.size();
+  HIP_TRY(upload_vec(&L.d_alias, alias));
   HIP_TRY(upload_vec(&L.d_c2f, c2f));
   L.nc2f = (unsigned)c2f.size();
   HIP_TRY(upload_vec(&L.d_inject, inj));
@@ -1023,10 +1043,31 @@ int uclbs_init_tgv(void* ctx, int lvl, double u0, double noise_amp, uint64_t see
   if (rc) return rc;
   if (c->cfg.lattice != UCLBS_LATTICE_D3Q27)
     return set_err(UCLBS_E_UNSUPPORTED, "init_tgv is a D3Q27 initialiser");
-  if (L->multibox) return set_err(UCLBS_E_UNSUPPORTED, "init_tgv fills a single box (upload the state)");
+  if (L->multibox && lvl > 0)
+    return set_err(UCLBS_E_UNSUPPORTED, "init_tgv fills level 0 (refined levels: uclbs_init_from_coarse)");
   if (!(fabs(u0) < 0.5) || !(noise_amp >= 0.0))
     return set_err(UCLBS_E_ARG, "init_tgv: |u0| must be < 0.5 and noise_amp >= 0");
   if ((rc = set_device(c))) return rc;
+  if (L->multibox) {
+    // every stored node of every box from its domain position (ghosts wrap)
+    uclbs::Box dom{};
+    dom.nx = L->dom_hi[0] - L->dom_lo[0] + 1;
+    dom.ny = L->dom_hi[1] - L->dom_lo[1] + 1;
+    dom.nz = L->dom_hi[2] - L->dom_lo[2] + 1;
+    dom.n = (int64_t)dom.nx * dom.ny * dom.nz;
+    for (size_t b = 0; b < L->hbox.size(); ++b) {
+      const uclbs::BoxDesc& bx = L->hbox[b];
+      const int ext[3] = {(int)bx.nx, (int)bx.ny, (int)(bx.n / (bx.nx * bx.ny))};
+      const int org[3] = {L->blo[3 * b] - 1 - L->dom_lo[0], L->blo[3 * b + 1] - 1 - L->dom_lo[1],
+                          L->blo[3 * b + 2] - 1 - L->dom_lo[2]};
+      HIP_TRY(uclbs::launch_d3q27_init_tgv_region(dom, L->f[L->cur], L->box.n, bx.off, ext, org,
+                                                  c->cfg.precision == UCLBS_PREC_FP32_SHIFTED, u0,
+                                                  noise_amp, seed, c->stream));
+    }
+    if ((rc = mirror_buffers(c, L))) return rc;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return UCLBS_OK;
+  }
   HIP_TRY(uclbs::launch_d3q27_init_tgv(L->box, L->f[L->cur],
                                        c->cfg.precision == UCLBS_PREC_FP32_SHIFTED, u0,
                                        noise_amp, seed, c->stream));
@@ -1561,6 +1602,7 @@ static int interlevel_fill(Ctx* c, int lvl, double theta, bool all_nodes) {
                               F->blo[3 * b + 2] - 1 + (int)z};
             uclbs::C2FStencil st{};
             st.dst = bx.off + x + bx.nx * (y + bx.ny * z);
+            st.mask = uclbs::kAllPops27;
             if (c2f_stencil(C, G, st)) all.push_back(st);
             else if (!ghost) return set_err(UCLBS_E_STATE, "fine node outside the coarse level");
           }
