@@ -118,8 +118,20 @@ int uclbs_init_tgv(void *ctx, int lvl, double u0, double noise_amp, uint64_t see
  * encoding of edge links (lbgrid/state_directions.f95:194-684): inlet j is
  * -100-j, outlet j is -200-j, -10 a bounce-back wall, -1000 freestream;
  * UCLBS_FACE_PERIODIC marks an axis declared periodic in uclbs_define_level.
- * Faces of non-periodic axes default to -10. */
+ * Faces of non-periodic axes default to -10.
+ * D3Q39 contexts (level 0) take -1000 (freestream_edge_bc.f95:18-19: the
+ * missing population is the freestream equilibrium fs_fieq / fs_gieq),
+ * UCLBS_FACE_OUTFLOW (an outlet of type 6, pure_outflow_bc.f95:17-20: the
+ * node's own post-collision population of that direction), -10
+ * (bounceback.f95:13-19: its opposite; the identity on the shifted lattice,
+ * as the reference's opp) and UCLBS_FACE_INVISCID (-11, inviscid_edge.f95:
+ * specular reflection about the half-way plane; not on the x faces of the
+ * shifted lattice).  Where a population's upstream node lies across several
+ * faces the first in the order x-, y-, z-, x+, y+, z+ decides, the order of
+ * amr/amr_code/new_node_values/shifted_state_directions_lvl.f95:194-686. */
 #define UCLBS_FACE_PERIODIC 2147483647
+#define UCLBS_FACE_OUTFLOW (-6)
+#define UCLBS_FACE_INVISCID (-11)
 int uclbs_set_faces(void *ctx, int lvl, const int32_t face_code[6]);
 /* Inlet / outlet / freestream states (rho, u, v, w) per boundary number; the
  * library tabulates their equilibria, as inlet_outlet_basis_calcs.f90:90-125

@@ -405,12 +405,42 @@ void o39_moments(const double f[39], const double g[39], int shifted, double *rh
 
 static inline int64_t wrap(int64_t a, int64_t n) { a %= n; return a < 0 ? a + n : a; }
 
-/* damp_it.f95:64-92 for an all-fluid periodic box: a negative post-collision
- * population is replaced by the mean of its positive neighbours along +-c_i
- * (or their mean with |f|, or |f|).  jacobi = 1 reads the neighbours' values
+/* Face codes of a box (x-, y-, z-, x+, y+, z+), the reference's edge-link
+ * codes (amr/amr_code/new_node_values/state_directions_lvl.f95:94-420 and
+ * shifted_state_directions_lvl.f95:194-686 assign them per face, in that
+ * chain order): O39_FACE_PERIODIC wraps, -1000 freestream_edge_bc, -6 the
+ * pure outflow of outlet type 6, -10 bounceback, -11 inviscid_edge. */
+#define O39_FACE_PERIODIC 2147483647
+
+static int axis_out(int64_t s, int64_t n, int f) { return f < 3 ? s < 0 : s >= n; }
+
+/* Neighbour x +- c_i of damp_it.f95:64-123 on a box with faces: -1 when it
+ * lies across a non-periodic face (a link whose code is negative, which the
+ * reference repairs from the other side only), else its flat index. */
+static int64_t damp_neighbour(int64_t a, int64_t b, int64_t cc, int dx, int dy, int dz, int nx,
+                              int ny, int nz, const int32_t *fc) {
+  int64_t s[3] = {a + dx, b + dy, cc + dz};
+  const int64_t n[3] = {nx, ny, nz};
+  for (int d = 0; d < 3; d++) {
+    if (s[d] < 0) {
+      if (fc[d] != O39_FACE_PERIODIC) return -1;
+      s[d] += n[d];
+    } else if (s[d] >= n[d]) {
+      if (fc[d + 3] != O39_FACE_PERIODIC) return -1;
+      s[d] -= n[d];
+    }
+  }
+  return s[0] + n[0] * (s[1] + n[1] * s[2]);
+}
+
+/* damp_it.f95:64-123: a negative post-collision population is replaced by the
+ * mean of its positive neighbours along +-c_i (or their mean with |f|, or
+ * |f|); a neighbour across a non-periodic face counts as not positive, which
+ * is the one-sided form of :109-123.  jacobi = 1 reads the neighbours' values
  * from before any repair (the order-free form the GPU uses); jacobi = 0 sweeps
  * in the reference order (direction, then z, y, x) and sees earlier repairs. */
-int64_t o39_damp_periodic(int nx, int ny, int nz, double *fo, int shifted, int jacobi) {
+int64_t o39_damp_faces(int nx, int ny, int nz, const int32_t fc[6], double *fo, int shifted,
+                       int jacobi) {
   int cx[NQ], cy[NQ], cz[NQ];
   o39_velocities(shifted, cx, cy, cz);
   const int64_t n = (int64_t)nx * ny * nz;
@@ -425,9 +455,10 @@ int64_t o39_damp_periodic(int nx, int ny, int nz, double *fo, int shifted, int j
       double v = src[(int64_t)i * n + k];
       if (!(v < 0.0)) continue;
       int64_t a = k % nx, b = (k / nx) % ny, cc = k / ((int64_t)nx * ny);
-      int64_t kp = wrap(a + cx[i], nx) + nx * (wrap(b + cy[i], ny) + ny * wrap(cc + cz[i], nz));
-      int64_t km = wrap(a - cx[i], nx) + nx * (wrap(b - cy[i], ny) + ny * wrap(cc - cz[i], nz));
-      double fp = src[(int64_t)i * n + kp], fm = src[(int64_t)i * n + km];
+      int64_t kp = damp_neighbour(a, b, cc, cx[i], cy[i], cz[i], nx, ny, nz, fc);
+      int64_t km = damp_neighbour(a, b, cc, -cx[i], -cy[i], -cz[i], nx, ny, nz, fc);
+      double fp = kp >= 0 ? src[(int64_t)i * n + kp] : -1.0;
+      double fm = km >= 0 ? src[(int64_t)i * n + km] : -1.0;
       double r;
       if (fp > 0.0 && fm > 0.0) r = (fp + fm) / 2.0;
       else if (fp > 0.0 && fm < 0.0) r = (fp + fabs(v)) / 2;
@@ -439,6 +470,63 @@ int64_t o39_damp_periodic(int nx, int ny, int nz, double *fo, int shifted, int j
   }
   if (jacobi) free(src);
   return nrep;
+}
+
+int64_t o39_damp_periodic(int nx, int ny, int nz, double *fo, int shifted, int jacobi) {
+  const int32_t fc[6] = {O39_FACE_PERIODIC, O39_FACE_PERIODIC, O39_FACE_PERIODIC,
+                         O39_FACE_PERIODIC, O39_FACE_PERIODIC, O39_FACE_PERIODIC};
+  return o39_damp_faces(nx, ny, nz, fc, fo, shifted, jacobi);
+}
+
+/* Source of population k pulled into node (a, b, cc): streaming.f95:365-474
+ * in pull form.  A population whose upstream node x - c_k lies across a
+ * non-periodic face is the one the reference's edge link fills (the link i
+ * with x + c_i outside and k = opp(i) on the plain lattice, the link i = k
+ * with x - c_i outside on the shifted one, where opp is the identity).  The
+ * first face crossed in the chain order -x, -y, -z, +x, +y, +z decides:
+ *   -1000  fi = fs_fieq(k), gi = fs_gieq(k)     freestream_edge_bc.f95:18-19
+ *   -6     fi = fout(x, k)                      pure_outflow_bc.f95:17-20
+ *   -10    fi = fout(x, opp(k))                 bounceback.f95:13-19
+ *   -11    specular: the population leaves x' = x - c_k reflected about the
+ *          half-way plane of every crossed inviscid face, with the mirrored
+ *          velocity (inviscid_edge.f95:28-1243 tabulates the same remap as
+ *          direction inviscid_dir written at x + loc; a face crossed after the
+ *          reflection that is not inviscid then decides as above).
+ * Returns 1 for the freestream value, else 0 with (*src, *dir) to read. */
+static int source39(int64_t a, int64_t b, int64_t cc, int k, int nx, int ny, int nz,
+                    const int32_t *fc, const int *cx, const int *cy, const int *cz, int shifted,
+                    int64_t *src, int *dir) {
+  const int64_t n[3] = {nx, ny, nz};
+  int64_t s[3] = {a - cx[k], b - cy[k], cc - cz[k]};
+  int first = -1;
+  for (int f = 0; f < 6 && first < 0; f++)
+    if (axis_out(s[f % 3], n[f % 3], f) && fc[f] != O39_FACE_PERIODIC) first = f;
+  for (int d = 0; d < 3; d++) {
+    if (s[d] < 0 && fc[d] == O39_FACE_PERIODIC) s[d] += n[d];
+    if (s[d] >= n[d] && fc[d + 3] == O39_FACE_PERIODIC) s[d] -= n[d];
+  }
+  int m[3] = {cx[k], cy[k], cz[k]};
+  if (first >= 0 && fc[first] == -11) {
+    for (int d = 0; d < 3; d++) {
+      if (s[d] < 0 && fc[d] == -11) { s[d] = -1 - s[d]; m[d] = -m[d]; }
+      else if (s[d] >= n[d] && fc[d + 3] == -11) { s[d] = 2 * n[d] - 1 - s[d]; m[d] = -m[d]; }
+    }
+    first = -1;
+    for (int f = 0; f < 6 && first < 0; f++)
+      if (axis_out(s[f % 3], n[f % 3], f)) first = f;
+  }
+  const int64_t own = a + n[0] * (b + n[1] * cc);
+  if (first < 0) {
+    int j = 0;
+    while (j < NQ && !(cx[j] == m[0] && cy[j] == m[1] && cz[j] == m[2])) j++;
+    *src = s[0] + n[0] * (s[1] + n[1] * s[2]);
+    *dir = j;
+    return 0;
+  }
+  if (fc[first] == -1000) return 1;
+  *src = own;
+  *dir = (fc[first] == -10 && !shifted) ? O39_OPP[k] : k;
+  return 0;
 }
 
 void o39_init_fields(int nx, int ny, int nz, const double *rho, const double *ux, const double *uy,
@@ -472,9 +560,21 @@ void o39_init_fields(int nx, int ny, int nz, const double *rho, const double *ux
   *bad = nb;
 }
 
-void o39_step_periodic(int nx, int ny, int nz, double *f, double *g, double *lm, double *alpha_out,
-                       const o39_consts *c, int64_t step0, int nsteps, int ref_lu, int jacobi,
-                       o39_stats *st) {
+/* source39 on the lattice of `shifted`, for tests */
+int o39_source(int64_t a, int64_t b, int64_t cc, int k, int nx, int ny, int nz, const int32_t fc[6],
+               int shifted, int64_t *src, int *dir) {
+  int cx[NQ], cy[NQ], cz[NQ];
+  o39_velocities(shifted, cx, cy, cz);
+  return source39(a, b, cc, k, nx, ny, nz, fc, cx, cy, cz, shifted, src, dir);
+}
+
+/* One or more steps of a box whose faces carry the codes fc (see source39);
+ * fs_f / fs_g are the freestream equilibria (freestream_basis_calcs.f90:66-71)
+ * and may be NULL when no face is -1000. */
+void o39_step_faces(int nx, int ny, int nz, const int32_t fc[6], const double *fs_f,
+                    const double *fs_g, double *f, double *g, double *lm, double *alpha_out,
+                    const o39_consts *c, int64_t step0, int nsteps, int ref_lu, int jacobi,
+                    o39_stats *st) {
   int cx[NQ], cy[NQ], cz[NQ];
   o39_velocities(c->shifted, cx, cy, cz);
   const int64_t n = (int64_t)nx * ny * nz;
@@ -490,9 +590,15 @@ void o39_step_periodic(int nx, int ny, int nz, double *f, double *g, double *lm,
       int64_t a = k % nx, b = (k / nx) % ny, cc = k / ((int64_t)nx * ny);
       double fi[NQ], gi[NQ], l[NM], e[NQ], u[3], P[6], rho, T;
       for (int i = 0; i < NQ; i++) {
-        int64_t src = wrap(a - cx[i], nx) + nx * (wrap(b - cy[i], ny) + ny * wrap(cc - cz[i], nz));
-        fi[i] = f[(int64_t)i * n + src];
-        gi[i] = g[(int64_t)i * n + src];
+        int64_t src;
+        int dir;
+        if (source39(a, b, cc, i, nx, ny, nz, fc, cx, cy, cz, c->shifted, &src, &dir)) {
+          fi[i] = fs_f[i];
+          gi[i] = fs_g[i];
+        } else {
+          fi[i] = f[(int64_t)dir * n + src];
+          gi[i] = g[(int64_t)dir * n + src];
+        }
       }
       o39_moments(fi, gi, c->shifted, &rho, u, &T, P, c->cv);
       for (int j = 0; j < NM; j++) l[j] = lm[(int64_t)j * n + k];
@@ -527,10 +633,19 @@ void o39_step_periodic(int nx, int ny, int nz, double *f, double *g, double *lm,
     st->n_alpha_hi += ahi;
     st->n_nan += nan;
     st->n_kappa_err += kerr;
-    st->n_repair += o39_damp_periodic(nx, ny, nz, fo, c->shifted, jacobi);
+    st->n_repair += o39_damp_faces(nx, ny, nz, fc, fo, c->shifted, jacobi);
     memcpy(f, fo, sizeof(double) * (size_t)(n * NQ));
     memcpy(g, go, sizeof(double) * (size_t)(n * NQ));
   }
   free(fo);
   free(go);
+}
+
+void o39_step_periodic(int nx, int ny, int nz, double *f, double *g, double *lm, double *alpha_out,
+                       const o39_consts *c, int64_t step0, int nsteps, int ref_lu, int jacobi,
+                       o39_stats *st) {
+  const int32_t fc[6] = {O39_FACE_PERIODIC, O39_FACE_PERIODIC, O39_FACE_PERIODIC,
+                         O39_FACE_PERIODIC, O39_FACE_PERIODIC, O39_FACE_PERIODIC};
+  o39_step_faces(nx, ny, nz, fc, NULL, NULL, f, g, lm, alpha_out, c, step0, nsteps, ref_lu, jacobi,
+                 st);
 }

@@ -130,6 +130,14 @@ def lib() -> ctypes.CDLL:
                                       ctypes.POINTER(ctypes.c_int64)]
         L.o39_step_periodic.argtypes = [I, I, I, dp, dp, dp, dp, c39, ctypes.c_int64, I, I, I,
                                         ctypes.POINTER(Stats39)]
+        ip = ctypes.POINTER(ctypes.c_int32)
+        L.o39_damp_faces.argtypes = [I, I, I, ip, dp, I, I]
+        L.o39_damp_faces.restype = ctypes.c_int64
+        L.o39_source.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, I, I, I, I, ip, I,
+                                 ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(I)]
+        L.o39_source.restype = I
+        L.o39_step_faces.argtypes = [I, I, I, ip, dp, dp, dp, dp, dp, dp, c39, ctypes.c_int64, I,
+                                     I, I, ctypes.POINTER(Stats39)]
         for nm in ("o_tri_volume", "o_box_test"):
             getattr(L, nm).argtypes = [dp, dp, dp]
         L.o_link_code.argtypes = [dp, dp, D]
@@ -447,6 +455,77 @@ def step_periodic39(shape, f, g, lm, c, step0=0, nsteps=1, ref_lu=False, jacobi=
                             _dp(alpha) if alpha is not None else None, ctypes.byref(c), step0,
                             nsteps, int(ref_lu), int(jacobi), ctypes.byref(st))
     return f, g, lm, alpha, st.as_dict()
+
+
+FACE_PERIODIC = 2147483647
+
+
+def _faces39(faces, shifted):
+    fc = np.ascontiguousarray(faces, np.int32)
+    assert fc.shape == (6,)
+    for k, v in enumerate(fc):
+        assert v in (FACE_PERIODIC, -1000, -6, -10, -11), v
+        assert (v == FACE_PERIODIC) == (fc[(k + 3) % 6] == FACE_PERIODIC)
+        assert not (shifted and v == -11 and k % 3 == 0), "no inviscid x face on the shifted lattice"
+    return fc
+
+
+def freestream39(c):
+    """fs_fieq / fs_gieq: the equilibrium of rho = 1, u = (Mach, 0, 0), T = 1/gamma
+    (freestream_basis_calcs.f90:35-71, basic_fluids_calcs.f95:31-48)."""
+    T = 1.0 / c.gama
+    u = np.array([c.mach, 0.0, 0.0])
+    P = np.array([T + u[0] ** 2, T, T, 0.0, 0.0, 0.0])
+    lm0 = np.zeros(13)
+    lm0[0] = -1.5 * np.log(2 * np.pi * T) - u[0] ** 2 / (2 * T)
+    lm0[1] = u[0] / T
+    lm0[4:7] = -1.0 / (2 * T)
+    lm, _, it = lm_solve39(1.0, u, T, P, lm0, bool(c.shifted), maxit=200)
+    assert it >= 0
+    fe = feq39(lm, 1.0, bool(c.shifted))
+    return fe, c.lead_g * T * fe
+
+
+def step_faces39(shape, faces, f, g, lm, c, step0=0, nsteps=1, ref_lu=False, jacobi=True,
+                 keep_alpha=False):
+    """Steps of a box whose faces (x-, y-, z-, x+, y+, z+) carry FACE_PERIODIC,
+    -1000 (freestream), -6 (pure outflow), -10 (bounce-back) or -11 (inviscid)."""
+    nx, ny, nz = shape
+    fc = _faces39(faces, bool(c.shifted))
+    fsf, fsg = freestream39(c)
+    f = np.array(f, np.float64)
+    g = np.array(g, np.float64)
+    lm = np.array(lm, np.float64)
+    alpha = np.zeros(nx * ny * nz) if keep_alpha else None
+    st = Stats39()
+    lib().o39_step_faces(nx, ny, nz, fc.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), _dp(fsf),
+                         _dp(fsg), _dp(f), _dp(g), _dp(lm),
+                         _dp(alpha) if alpha is not None else None, ctypes.byref(c), step0,
+                         nsteps, int(ref_lu), int(jacobi), ctypes.byref(st))
+    return f, g, lm, alpha, st.as_dict()
+
+
+def source39(node, k, shape, faces, shifted=False):
+    """(None, None) for the freestream value, else (source node (x, y, z), direction)."""
+    nx, ny, nz = shape
+    fc = _faces39(faces, shifted)
+    src, d = ctypes.c_int64(), ctypes.c_int()
+    fs = lib().o39_source(node[0], node[1], node[2], k, nx, ny, nz,
+                          fc.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), int(shifted),
+                          ctypes.byref(src), ctypes.byref(d))
+    if fs:
+        return None, None
+    v = src.value
+    return (v % nx, (v // nx) % ny, v // (nx * ny)), d.value
+
+
+def damp_faces39(shape, faces, fo, shifted=False, jacobi=True):
+    nx, ny, nz = shape
+    fc = _faces39(faces, shifted)
+    fo = np.array(fo, np.float64)
+    nrep = lib().o39_damp_faces(nx, ny, nz, fc.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                _dp(fo), int(shifted), int(jacobi))
+    return fo, nrep
 
 
 def macro39(f, g, c):

@@ -153,12 +153,20 @@ int o39_collide(const double f[39], const double g[39], const double lm[13], dou
 void o39_moments(const double f[39], const double g[39], int shifted, double *rho, double u[3],
                  double *T, double P[6], double cv);
 int64_t o39_damp_periodic(int nx, int ny, int nz, double *fo, int shifted, int jacobi);
+int64_t o39_damp_faces(int nx, int ny, int nz, const int32_t fc[6], double *fo, int shifted,
+                       int jacobi);
 void o39_init_fields(int nx, int ny, int nz, const double *rho, const double *ux, const double *uy,
                      const double *uz, const double *T, const o39_consts *c, double *f, double *g,
                      double *lm, int64_t *bad);
 void o39_step_periodic(int nx, int ny, int nz, double *f, double *g, double *lm, double *alpha_out,
                        const o39_consts *c, int64_t step0, int nsteps, int ref_lu, int jacobi,
                        o39_stats *st);
+int o39_source(int64_t a, int64_t b, int64_t cc, int k, int nx, int ny, int nz, const int32_t fc[6],
+               int shifted, int64_t *src, int *dir);
+void o39_step_faces(int nx, int ny, int nz, const int32_t fc[6], const double *fs_f,
+                    const double *fs_g, double *f, double *g, double *lm, double *alpha_out,
+                    const o39_consts *c, int64_t step0, int nsteps, int ref_lu, int jacobi,
+                    o39_stats *st);
 
 #ifdef __cplusplus
 }

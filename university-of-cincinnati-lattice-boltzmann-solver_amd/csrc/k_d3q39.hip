@@ -57,6 +57,75 @@ __device__ __forceinline__ void wave_stats39(unsigned long long* __restrict__ st
   }
 }
 
+// Pull of population i into node (x, y, z) whose upstream node x - c_i may lie
+// across a non-periodic face: streaming.f95:365-474 in pull form.  The first
+// face crossed in the chain order -x, -y, -z, +x, +y, +z of
+// shifted_state_directions_lvl.f95:194-686 decides: -1000 the freestream
+// equilibrium (freestream_edge_bc.f95:18-19), -6 the node's own population
+// (pure_outflow_bc.f95:17-20), -10 its opposite (bounceback.f95:13-19; opp is
+// the identity on the shifted lattice), -11 the specular image about the
+// half-way plane of every crossed inviscid face (inviscid_edge.f95:28-1243),
+// after which a face still crossed decides as above.
+template <bool kShifted, typename S>
+__device__ __forceinline__ void pull_face39(int x, int y, int z, int i, const Box& b,
+                                                      const Faces39& fc, const Ref39& rf,
+                                                      const S* __restrict__ fA,
+                                                      const S* __restrict__ gA, S* fv, S* gv) {
+  const int nb[3] = {b.nx, b.ny, b.nz};
+  int s[3] = {x - cx39<kShifted>(i), y - CY39[i], z - CZ39[i]};
+  int first = -1;
+#pragma unroll
+  for (int f = 5; f >= 0; --f) {
+    const int d = f < 3 ? f : f - 3;
+    const bool out = f < 3 ? s[d] < 0 : s[d] >= nb[d];
+    if (out && fc.code[f] != kFacePeriodic) first = f;
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (s[d] < 0 && fc.code[d] == kFacePeriodic) s[d] += nb[d];
+    if (s[d] >= nb[d] && fc.code[d + 3] == kFacePeriodic) s[d] -= nb[d];
+  }
+  int dir = i;
+  if (first >= 0 && fc.code[first] == -11) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (s[d] < 0 && fc.code[d] == -11) {
+        s[d] = -1 - s[d];
+        dir = kMirror39.m[d][dir];
+      } else if (s[d] >= nb[d] && fc.code[d + 3] == -11) {
+        s[d] = 2 * nb[d] - 1 - s[d];
+        dir = kMirror39.m[d][dir];
+      }
+    }
+    first = -1;
+#pragma unroll
+    for (int f = 5; f >= 0; --f) {
+      const int d = f < 3 ? f : f - 3;
+      if (f < 3 ? s[d] < 0 : s[d] >= nb[d]) first = f;
+    }
+  }
+  const unsigned n = (unsigned)b.n;
+  unsigned src;
+  if (first < 0) {
+    src = (unsigned)s[0] + (unsigned)nb[0] * ((unsigned)s[1] + (unsigned)nb[1] * (unsigned)s[2]);
+  } else if (fc.code[first] == -1000) {
+    *fv = (S)(fc.f[i] - rf.f[i]);
+    *gv = (S)(fc.g[i] - rf.g[i]);
+    return;
+  } else {
+    src = (unsigned)x + (unsigned)nb[0] * ((unsigned)y + (unsigned)nb[1] * (unsigned)z);
+    dir = (fc.code[first] == -10 && !kShifted) ? OPP39[i] : i;
+  }
+  const S fr = fA[(size_t)dir * n + src], gr = gA[(size_t)dir * n + src];
+  if (dir == i) {
+    *fv = fr;
+    *gv = gr;
+  } else {  // FP32 storage holds deviations: re-base onto direction i's reference
+    *fv = (S)(((double)fr + rf.f[dir]) - rf.f[i]);
+    *gv = (S)(((double)gr + rf.g[dir]) - rf.g[i]);
+  }
+}
+
 // calculate_alpha.f95:118-289 over the populations f (LDS, full value f_i =
 // sf + fref) and the last Newton iteration's equilibrium (LDS, sd).  Returns
 // alpha; *st = 0 (root), 1 (alpha = 2), 2 / 3 (clamped low / high), -1 (no
@@ -135,12 +204,12 @@ __device__ __forceinline__ double alpha_estimate39(const S (*sf)[64], const doub
   return na;
 }
 
-template <bool kFp32, bool kShifted, bool kTurb, bool kKeepAlpha>
+template <bool kFp32, bool kShifted, bool kTurb, bool kKeepAlpha, bool kFaces>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void k39_step(
     const void* __restrict__ fA_, const void* __restrict__ gA_, void* __restrict__ fB_,
     void* __restrict__ gB_, double* __restrict__ lmv, double* __restrict__ alpha, Box b,
     Gas39 gs, Ref39 rf, long long step, unsigned long long* __restrict__ stats,
-    unsigned* __restrict__ neg_count, unsigned* __restrict__ neg_list) {
+    unsigned* __restrict__ neg_count, unsigned* __restrict__ neg_list, Faces39 fc) {
   using S = typename std::conditional<kFp32, float, double>::type;
   const S* __restrict__ fA = static_cast<const S*>(fA_);
   const S* __restrict__ gA = static_cast<const S*>(gA_);
@@ -160,18 +229,39 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
     const int x = (int)(node % (unsigned)nx);
     const int y = (int)((node / (unsigned)nx) % (unsigned)ny);
     const int z = (int)(node / ((unsigned)nx * (unsigned)ny));
+    // a node within 4 of a non-periodic face pulls through the face rules
+    bool near = false;
+    if (kFaces) {
+      const int c3[3] = {x, y, z}, n3[3] = {nx, ny, nz};
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        near |= (fc.code[d] != kFacePeriodic && c3[d] < 4) ||
+                (fc.code[d + 3] != kFacePeriodic && c3[d] >= n3[d] - 4);
+    }
+    if (near) {
+#pragma unroll 1
+      for (int i = 0; i < Q39; ++i) {
+        S fs, gsv;
+        pull_face39<kShifted>(x, y, z, i, b, fc, rf, fA, gA, &fs, &gsv);
+        sf[i][t] = fs;
+        sg[i][t] = gsv;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < Q39; ++i) {
+        const int cx = cx39<kShifted>(i), cy = CY39[i], cz = CZ39[i];
+        const unsigned src = (unsigned)wrapm(x - cx, nx) +
+                             (unsigned)nx * ((unsigned)wrapm(y - cy, ny) +
+                                             (unsigned)ny * (unsigned)wrapm(z - cz, nz));
+        sf[i][t] = fA[(size_t)i * n + src];
+        sg[i][t] = gA[(size_t)i * n + src];
+      }
+    }
     double rho = 0.0, mx = 0.0, my = 0.0, mz = 0.0, en = 0.0;
 #pragma unroll
     for (int i = 0; i < Q39; ++i) {
       const int cx = cx39<kShifted>(i), cy = CY39[i], cz = CZ39[i];
-      const unsigned src = (unsigned)wrapm(x - cx, nx) +
-                           (unsigned)nx * ((unsigned)wrapm(y - cy, ny) +
-                                           (unsigned)ny * (unsigned)wrapm(z - cz, nz));
-      const S fs = fA[(size_t)i * n + src];
-      const S gsv = gA[(size_t)i * n + src];
-      sf[i][t] = fs;
-      sg[i][t] = gsv;
-      const double f = (double)fs + rf.f[i], g = (double)gsv + rf.g[i];
+      const double f = (double)sf[i][t] + rf.f[i], g = (double)sg[i][t] + rf.g[i];
       const int cm = cx * cx + cy * cy + cz * cz;
       rho += f;
       if (cx) mx = fma((double)cx, f, mx);
@@ -285,6 +375,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
   wave_stats39(stats, active, iters, noconv, ast, kerr);
 }
 
+// node (x, y, z) wrapped over the periodic axes of b; -1 across another face
+__device__ __forceinline__ long long damp_nb(int x, int y, int z, const Box& b) {
+  int s[3] = {x, y, z};
+  const int nb[3] = {b.nx, b.ny, b.nz};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (s[d] < 0 || s[d] >= nb[d]) {
+      if (!b.per[d]) return -1;
+      s[d] = wrapm(s[d], nb[d]);
+    }
+  }
+  return (long long)s[0] + (long long)nb[0] * ((long long)s[1] + (long long)nb[1] * s[2]);
+}
+
 // damp_it.f95:64-92 for the listed nodes: each negative post-collision
 // population takes the mean of its positive neighbours along +-c_i (or the
 // mean of one with |f|, or |f|), all read before any repair is written;
@@ -310,14 +414,12 @@ __global__ __launch_bounds__(256) void k39_damp_compute(const void* __restrict__
       double r = __builtin_nan("");
       if (v < 0.0) {
         const int cx = cx39<kShifted>(i), cy = CY39[i], cz = CZ39[i];
-        const unsigned kp = (unsigned)wrapm(x + cx, b.nx) +
-                            (unsigned)b.nx * ((unsigned)wrapm(y + cy, b.ny) +
-                                              (unsigned)b.ny * (unsigned)wrapm(z + cz, b.nz));
-        const unsigned km = (unsigned)wrapm(x - cx, b.nx) +
-                            (unsigned)b.nx * ((unsigned)wrapm(y - cy, b.ny) +
-                                              (unsigned)b.ny * (unsigned)wrapm(z - cz, b.nz));
-        const double fp = (double)f[(size_t)i * n + kp] + rf.f[i];
-        const double fm = (double)f[(size_t)i * n + km] + rf.f[i];
+        // a neighbour across a non-periodic face counts as not positive: the
+        // one-sided form of damp_it.f95:109-123
+        const long long kp = damp_nb(x + cx, y + cy, z + cz, b);
+        const long long km = damp_nb(x - cx, y - cy, z - cz, b);
+        const double fp = kp >= 0 ? (double)f[(size_t)i * n + (size_t)kp] + rf.f[i] : -1.0;
+        const double fm = km >= 0 ? (double)f[(size_t)i * n + (size_t)km] + rf.f[i] : -1.0;
         if (fp > 0.0 && fm > 0.0) r = (fp + fm) / 2.0;
         else if (fp > 0.0 && fm < 0.0) r = (fp + fabs(v)) / 2;
         else if (fp < 0.0 && fm > 0.0) r = (fm + fabs(v)) / 2;
@@ -410,13 +512,17 @@ __global__ void k39_to_fp64(const void* __restrict__ src_, double* __restrict__ 
 hipError_t launch_d3q39_step(const Box& b, const Step39& a, hipStream_t s) {
   const unsigned blocks = (unsigned)((b.n + 63) / 64);
   hipLaunchKernelGGL(k39_clear, dim3(1), dim3(1), 0, s, a.neg_count);
-#define UCLBS_K39(F, SH, TU, KA)                                                                  \
-  hipLaunchKernelGGL((k39_step<F, SH, TU, KA>), dim3(blocks), dim3(64), 0, s, a.fin, a.gin,     \
+#define UCLBS_K39(F, SH, TU, KA, FC)                                                              \
+  hipLaunchKernelGGL((k39_step<F, SH, TU, KA, FC>), dim3(blocks), dim3(64), 0, s, a.fin, a.gin, \
                      a.fout, a.gout, a.lm, a.alpha, b, a.gas, a.ref, (long long)a.step, a.stats, \
-                     a.neg_count, a.neg_list)
+                     a.neg_count, a.neg_list, a.faces)
+#define UCLBS_K39_FC(F, SH, TU, KA) \
+  do {                              \
+    if (a.has_faces) UCLBS_K39(F, SH, TU, KA, true); else UCLBS_K39(F, SH, TU, KA, false); \
+  } while (0)
 #define UCLBS_K39_KA(F, SH, TU) \
   do {                          \
-    if (a.alpha) UCLBS_K39(F, SH, TU, true); else UCLBS_K39(F, SH, TU, false); \
+    if (a.alpha) UCLBS_K39_FC(F, SH, TU, true); else UCLBS_K39_FC(F, SH, TU, false); \
   } while (0)
 #define UCLBS_K39_TU(F, SH) \
   do {                      \
@@ -429,6 +535,7 @@ hipError_t launch_d3q39_step(const Box& b, const Step39& a, hipStream_t s) {
   }
 #undef UCLBS_K39_TU
 #undef UCLBS_K39_KA
+#undef UCLBS_K39_FC
 #undef UCLBS_K39
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;

@@ -210,6 +210,14 @@ struct Ref39 {
   double f[39];
   double g[39];
 };
+// Faces of a D3Q39 box (x-, y-, z-, x+, y+, z+): kFacePeriodic, -1000
+// freestream, -6 pure outflow, -10 bounce-back, -11 inviscid; f / g the
+// freestream equilibria (full values).
+struct Faces39 {
+  int code[6];
+  double f[39];
+  double g[39];
+};
 struct Step39 {
   const void* fin;
   const void* gin;
@@ -225,6 +233,8 @@ struct Step39 {
   unsigned* neg_count;  // nodes with a negative post-collision population
   unsigned* neg_list;
   double* rep;          // [n][39] repair values
+  int has_faces;        // any non-periodic face
+  Faces39 faces;
 };
 hipError_t launch_d3q39_step(const Box& b, const Step39& a, hipStream_t s);
 hipError_t launch_d3q39_init(const Box& b, const double* fields, void* f, void* g, double* lm,

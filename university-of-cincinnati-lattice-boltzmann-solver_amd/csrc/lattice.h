@@ -37,4 +37,27 @@ constexpr int OPP39[39] = {0, 4, 5, 6, 1, 2, 3, 13, 14, 11, 12, 9, 10, 7, 8, 18,
                            17, 25, 28, 31, 26, 21, 24, 29, 22, 27, 32, 23, 30, 36, 37, 38, 33,
                            34, 35};
 
+// kMirror39.m[d][k]: the direction whose velocity is c_k with component d negated
+// (the specular image at a wall normal to axis d).  Built from the un-shifted
+// velocities; for d = y, z it holds on the shifted lattice as well, whose
+// shift is along x.
+struct Mirror39 {
+  int m[3][39];
+};
+constexpr Mirror39 make_mirror39() {
+  Mirror39 r{};
+  for (int d = 0; d < 3; ++d)
+    for (int k = 0; k < 39; ++k) {
+      const int c[3] = {CX39[k], CY39[k], CZ39[k]};
+      for (int j = 0; j < 39; ++j) {
+        const int e[3] = {CX39[j], CY39[j], CZ39[j]};
+        bool same = true;
+        for (int a = 0; a < 3; ++a) same = same && (e[a] == (a == d ? -c[a] : c[a]));
+        if (same) r.m[d][k] = j;
+      }
+    }
+  return r;
+}
+constexpr Mirror39 kMirror39 = make_mirror39();
+
 }  // namespace uclbs

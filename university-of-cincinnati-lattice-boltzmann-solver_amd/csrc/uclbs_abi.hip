@@ -112,6 +112,7 @@ struct Ctx {
   int shifted39 = 0;
   uclbs::Gas39 gas39{};
   uclbs::Ref39 ref39{};
+  uclbs::Ref39 fs39{};        // freestream equilibria fs_fieq / fs_gieq (full values)
   double* d_ref39 = nullptr;  // f_ref[39] then g_ref[39]
   int64_t err_seen = 0;       // D3Q39 error-path counts already reported
   // learned alpha: FP32 master parameters, Adam moments, bf16 inference copy
@@ -326,6 +327,15 @@ int launch_steps39(Ctx* c, Level* L, int nsteps, hipEvent_t* ev) {
   a.rep = L->rep;
   a.lm = L->lm;
   a.alpha = c->keep_alpha ? L->alpha : nullptr;
+  a.has_faces = 0;
+  for (int f = 0; f < 6; ++f) {
+    a.faces.code[f] = L->face_code[f];
+    a.has_faces |= L->face_code[f] != uclbs::kFacePeriodic;
+  }
+  for (int i = 0; i < uclbs::Q39; ++i) {
+    a.faces.f[i] = c->fs39.f[i];
+    a.faces.g[i] = c->fs39.g[i];
+  }
   for (int s = 0; s < nsteps; ++s) {
     a.fin = L->f[L->cur];
     a.gin = L->g[L->cur];
@@ -559,8 +569,6 @@ int uclbs_define_level(void* ctx, int lvl, const int32_t lo[3], const int32_t hi
   L.box.n = (int64_t)n[0] * n[1] * n[2];
   if (c->cfg.lattice == UCLBS_LATTICE_D3Q39) {
     if (!c->gas_set) return set_err(UCLBS_E_STATE, "D3Q39: call uclbs_set_gas before defining levels");
-    if (!(L.box.per[0] && L.box.per[1] && L.box.per[2]))
-      return set_err(UCLBS_E_UNSUPPORTED, "D3Q39 levels are periodic boxes in this build");
     for (int d = 0; d < 3; ++d)
       if (n[d] < 5) return set_err(UCLBS_E_ARG, "D3Q39 box extent %d on axis %d is below 5", n[d], d);
   }
@@ -1103,12 +1111,18 @@ int uclbs_set_faces(void* ctx, int lvl, const int32_t face_code[6]) {
   if (L->multibox) return set_err(UCLBS_E_UNSUPPORTED, "no link tables on levels of many boxes");
   if (L->refined) return set_err(UCLBS_E_UNSUPPORTED, "level %d is a refined level (no link tables)", lvl);
   if (!face_code) return set_err(UCLBS_E_ARG, "null face_code");
+  const bool q39 = c->cfg.lattice == UCLBS_LATTICE_D3Q39;
   for (int f = 0; f < 6; ++f) {
     const int32_t v = face_code[f];
     const bool per = v == uclbs::kFacePeriodic;
-    const bool ok = per || v == -10 || v == -1000 || (v <= -101 && v >= -199) ||
-                    (v <= -201 && v >= -299);
-    if (!ok) return set_err(UCLBS_E_ARG, "face %d: code %d is not a boundary code", f, v);
+    const bool ok = q39 ? (per || v == -10 || v == -1000 || v == -6 || v == -11)
+                        : (per || v == -10 || v == -1000 || (v <= -101 && v >= -199) ||
+                           (v <= -201 && v >= -299));
+    if (!ok)
+      return set_err(UCLBS_E_ARG, "face %d: code %d is not a %s boundary code", f, v,
+                     q39 ? "D3Q39" : "D3Q27");
+    if (q39 && v == -11 && f % 3 == 0 && c->shifted39)
+      return set_err(UCLBS_E_ARG, "face %d: no inviscid face normal to the shift axis", f);
     if (per != (bool)L->box.per[f % 3])
       return set_err(UCLBS_E_ARG, "face %d: periodic code must match the level's periodic axes", f);
   }
@@ -1430,9 +1444,11 @@ int uclbs_set_gas(void* ctx, const uclbs_gas* gas) {
   g.turbulent = gas->turbulent != 0;
   g.maxit = gas->lm_maxit > 0 ? gas->lm_maxit : 50;
   c->shifted39 = gas->shifted < 0 ? (gas->mach >= 1.2 ? 1 : 0) : (gas->shifted != 0);
-  // FP32 storage keeps deviations from the freestream equilibrium
+  // freestream equilibria (freestream_basis_calcs.f90:35-71: the multiplier
+  // solve of rho = 1, u = Mach along x, T = 1/gamma; fs_gieq = (2 cv - 3) T
+  // fs_fieq); FP32 storage keeps deviations from them
   c->ref39 = uclbs::Ref39{};
-  if (c->cfg.precision == UCLBS_PREC_FP32_SHIFTED) {
+  {
     const double u0 = gas->mach, T = temperature, rho = 1.0;
     const double qf = rho * u0 * u0 + 5 * rho * T;
     const double tg[uclbs::kM39] = {rho, rho * u0, 0, 0, rho * (T + u0 * u0), rho * T, rho * T,
@@ -1445,10 +1461,11 @@ int uclbs_set_gas(void* ctx, const uclbs_gas* gas) {
     if (it < 0) return set_err(UCLBS_E_NUMERIC, "freestream multipliers did not converge");
     for (int i = 0; i < uclbs::Q39; ++i) {
       const double e = c->shifted39 ? uclbs::expo39<true>(lm, rho, i) : uclbs::expo39<false>(lm, rho, i);
-      c->ref39.f[i] = e;
-      c->ref39.g[i] = g.lead_g * T * e;
+      c->fs39.f[i] = e;
+      c->fs39.g[i] = g.lead_g * T * e;
     }
   }
+  if (c->cfg.precision == UCLBS_PREC_FP32_SHIFTED) c->ref39 = c->fs39;
   int rc = set_device(c);
   if (rc) return rc;
   if (!c->d_ref39) HIP_TRY(hipMalloc(&c->d_ref39, 2 * uclbs::Q39 * sizeof(double)));
