@@ -57,6 +57,14 @@ __device__ __forceinline__ void wave_stats39(unsigned long long* __restrict__ st
   }
 }
 
+// code of face f (lane-varying f: selects, not an indexed kernel-argument read)
+__device__ __forceinline__ int face_code39(const Faces39& fc, int f) {
+  int v = fc.code[0];
+#pragma unroll
+  for (int k = 1; k < 6; ++k) v = (f == k) ? fc.code[k] : v;
+  return v;
+}
+
 // Pull of population i into node (x, y, z) whose upstream node x - c_i may lie
 // across a non-periodic face: streaming.f95:365-474 in pull form.  The first
 // face crossed in the chain order -x, -y, -z, +x, +y, +z of
@@ -68,9 +76,9 @@ __device__ __forceinline__ void wave_stats39(unsigned long long* __restrict__ st
 // after which a face still crossed decides as above.
 template <bool kShifted, typename S>
 __device__ __forceinline__ void pull_face39(int x, int y, int z, int i, const Box& b,
-                                                      const Faces39& fc, const Ref39& rf,
-                                                      const S* __restrict__ fA,
-                                                      const S* __restrict__ gA, S* fv, S* gv) {
+                                            const Faces39& fc, const Ref39& rf,
+                                            const int (*smir)[Q39], const S* __restrict__ fA,
+                                            const S* __restrict__ gA, S* fv, S* gv) {
   const int nb[3] = {b.nx, b.ny, b.nz};
   int s[3] = {x - cx39<kShifted>(i), y - CY39[i], z - CZ39[i]};
   int first = -1;
@@ -86,15 +94,16 @@ __device__ __forceinline__ void pull_face39(int x, int y, int z, int i, const Bo
     if (s[d] >= nb[d] && fc.code[d + 3] == kFacePeriodic) s[d] -= nb[d];
   }
   int dir = i;
-  if (first >= 0 && fc.code[first] == -11) {
+  int code = first >= 0 ? face_code39(fc, first) : 0;
+  if (code == -11) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       if (s[d] < 0 && fc.code[d] == -11) {
         s[d] = -1 - s[d];
-        dir = kMirror39.m[d][dir];
+        dir = smir[d][dir];
       } else if (s[d] >= nb[d] && fc.code[d + 3] == -11) {
         s[d] = 2 * nb[d] - 1 - s[d];
-        dir = kMirror39.m[d][dir];
+        dir = smir[d][dir];
       }
     }
     first = -1;
@@ -103,18 +112,19 @@ __device__ __forceinline__ void pull_face39(int x, int y, int z, int i, const Bo
       const int d = f < 3 ? f : f - 3;
       if (f < 3 ? s[d] < 0 : s[d] >= nb[d]) first = f;
     }
+    code = first >= 0 ? face_code39(fc, first) : 0;
   }
   const unsigned n = (unsigned)b.n;
   unsigned src;
   if (first < 0) {
     src = (unsigned)s[0] + (unsigned)nb[0] * ((unsigned)s[1] + (unsigned)nb[1] * (unsigned)s[2]);
-  } else if (fc.code[first] == -1000) {
+  } else if (code == -1000) {
     *fv = (S)(fc.f[i] - rf.f[i]);
     *gv = (S)(fc.g[i] - rf.g[i]);
     return;
   } else {
     src = (unsigned)x + (unsigned)nb[0] * ((unsigned)y + (unsigned)nb[1] * (unsigned)z);
-    dir = (fc.code[first] == -10 && !kShifted) ? OPP39[i] : i;
+    dir = (code == -10 && !kShifted) ? OPP39[i] : i;
   }
   const S fr = fA[(size_t)dir * n + src], gr = gA[(size_t)dir * n + src];
   if (dir == i) {
@@ -204,12 +214,16 @@ __device__ __forceinline__ double alpha_estimate39(const S (*sf)[64], const doub
   return na;
 }
 
-template <bool kFp32, bool kShifted, bool kTurb, bool kKeepAlpha, bool kFaces>
+// kNear = false: every node of the box (near-face nodes skipped when the box
+// has faces); kNear = true: the listed near-face nodes, pulled through the
+// face rules.
+template <bool kFp32, bool kShifted, bool kTurb, bool kKeepAlpha, bool kNear>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void k39_step(
     const void* __restrict__ fA_, const void* __restrict__ gA_, void* __restrict__ fB_,
     void* __restrict__ gB_, double* __restrict__ lmv, double* __restrict__ alpha, Box b,
     Gas39 gs, Ref39 rf, long long step, unsigned long long* __restrict__ stats,
-    unsigned* __restrict__ neg_count, unsigned* __restrict__ neg_list, Faces39 fc) {
+    unsigned* __restrict__ neg_count, unsigned* __restrict__ neg_list, Faces39 fc, int has_faces,
+    const unsigned* __restrict__ near_list, unsigned n_near) {
   using S = typename std::conditional<kFp32, float, double>::type;
   const S* __restrict__ fA = static_cast<const S*>(fA_);
   const S* __restrict__ gA = static_cast<const S*>(gA_);
@@ -218,10 +232,32 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
   __shared__ S sf[Q39][64];
   __shared__ S sg[Q39][64];
   __shared__ double sd[Q39][64];
+  __shared__ int smir[kNear ? 3 : 1][Q39];  // mirror directions for the inviscid faces
   const int t = threadIdx.x;
+  if (kNear) {
+    if (t < Q39) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) smir[d][t] = kMirror39.m[d][t];
+    }
+    __syncthreads();
+  }
   const unsigned n = (unsigned)b.n;
-  const unsigned node = blockIdx.x * 64u + (unsigned)t;
-  const bool active = node < n;
+  const unsigned idx = blockIdx.x * 64u + (unsigned)t;
+  unsigned node;
+  bool active;
+  if (kNear) {
+    active = idx < n_near;
+    node = active ? near_list[idx] : 0u;
+  } else {
+    node = idx;
+    active = node < n;
+    if (active && has_faces) {
+      const int x = (int)(node % (unsigned)b.nx);
+      const int y = (int)((node / (unsigned)b.nx) % (unsigned)b.ny);
+      const int z = (int)(node / ((unsigned)b.nx * (unsigned)b.ny));
+      active = !near_face39(x, y, z, fc.code, b.nx, b.ny, b.nz, kShifted);
+    }
+  }
   int iters = 0, ast = 0;
   bool noconv = false, kerr = false;
   if (active) {
@@ -229,20 +265,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
     const int x = (int)(node % (unsigned)nx);
     const int y = (int)((node / (unsigned)nx) % (unsigned)ny);
     const int z = (int)(node / ((unsigned)nx * (unsigned)ny));
-    // a node within 4 of a non-periodic face pulls through the face rules
-    bool near = false;
-    if (kFaces) {
-      const int c3[3] = {x, y, z}, n3[3] = {nx, ny, nz};
-#pragma unroll
-      for (int d = 0; d < 3; ++d)
-        near |= (fc.code[d] != kFacePeriodic && c3[d] < 4) ||
-                (fc.code[d + 3] != kFacePeriodic && c3[d] >= n3[d] - 4);
-    }
-    if (near) {
-#pragma unroll 1
+    if (kNear) {
+#pragma unroll 3
       for (int i = 0; i < Q39; ++i) {
         S fs, gsv;
-        pull_face39<kShifted>(x, y, z, i, b, fc, rf, fA, gA, &fs, &gsv);
+        pull_face39<kShifted>(x, y, z, i, b, fc, rf, smir, fA, gA, &fs, &gsv);
         sf[i][t] = fs;
         sg[i][t] = gsv;
       }
@@ -511,14 +538,17 @@ __global__ void k39_to_fp64(const void* __restrict__ src_, double* __restrict__ 
 
 hipError_t launch_d3q39_step(const Box& b, const Step39& a, hipStream_t s) {
   const unsigned blocks = (unsigned)((b.n + 63) / 64);
+  const unsigned near_blocks = (a.n_near + 63u) / 64u;
+  if (a.has_faces && a.n_near && !a.near_list) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k39_clear, dim3(1), dim3(1), 0, s, a.neg_count);
-#define UCLBS_K39(F, SH, TU, KA, FC)                                                              \
-  hipLaunchKernelGGL((k39_step<F, SH, TU, KA, FC>), dim3(blocks), dim3(64), 0, s, a.fin, a.gin, \
+#define UCLBS_K39(F, SH, TU, KA, NR, NB)                                                         \
+  hipLaunchKernelGGL((k39_step<F, SH, TU, KA, NR>), dim3(NB), dim3(64), 0, s, a.fin, a.gin,     \
                      a.fout, a.gout, a.lm, a.alpha, b, a.gas, a.ref, (long long)a.step, a.stats, \
-                     a.neg_count, a.neg_list, a.faces)
-#define UCLBS_K39_FC(F, SH, TU, KA) \
-  do {                              \
-    if (a.has_faces) UCLBS_K39(F, SH, TU, KA, true); else UCLBS_K39(F, SH, TU, KA, false); \
+                     a.neg_count, a.neg_list, a.faces, a.has_faces, a.near_list, a.n_near)
+#define UCLBS_K39_FC(F, SH, TU, KA)                                              \
+  do {                                                                           \
+    UCLBS_K39(F, SH, TU, KA, false, blocks);                                     \
+    if (a.has_faces && a.n_near) UCLBS_K39(F, SH, TU, KA, true, near_blocks);    \
   } while (0)
 #define UCLBS_K39_KA(F, SH, TU) \
   do {                          \

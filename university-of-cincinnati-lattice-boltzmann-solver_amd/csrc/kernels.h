@@ -218,6 +218,21 @@ struct Faces39 {
   double f[39];
   double g[39];
 };
+// A node of a D3Q39 box whose upstream nodes x - c_i may lie across a
+// non-periodic face: x - c_d < 0 needs x < max c_d, x - c_d >= n needs
+// x >= n + min c_d (c_x in -3..3, on the shifted lattice -2..4; c_y, c_z in -3..3).
+__host__ __device__ inline bool near_face39(int x, int y, int z, const int code[6], int nx, int ny,
+                                            int nz, bool shifted) {
+  const int c3[3] = {x, y, z}, n3[3] = {nx, ny, nz};
+  bool near = false;
+  for (int d = 0; d < 3; ++d) {
+    const int lo_reach = (d == 0 && shifted) ? 4 : 3;
+    const int hi_reach = (d == 0 && shifted) ? 2 : 3;
+    near = near || (code[d] != kFacePeriodic && c3[d] < lo_reach) ||
+           (code[d + 3] != kFacePeriodic && c3[d] >= n3[d] - hi_reach);
+  }
+  return near;
+}
 struct Step39 {
   const void* fin;
   const void* gin;
@@ -233,8 +248,10 @@ struct Step39 {
   unsigned* neg_count;  // nodes with a negative post-collision population
   unsigned* neg_list;
   double* rep;          // [n][39] repair values
-  int has_faces;        // any non-periodic face
+  int has_faces;        // any non-periodic face: the main launch skips near-face nodes
   Faces39 faces;
+  const unsigned* near_list;  // near-face nodes, stepped by a second launch
+  unsigned n_near;
 };
 hipError_t launch_d3q39_step(const Box& b, const Step39& a, hipStream_t s);
 hipError_t launch_d3q39_init(const Box& b, const double* fields, void* f, void* g, double* lm,

@@ -62,6 +62,9 @@ struct Level {
   double* lm = nullptr;
   unsigned* neg_count = nullptr;
   unsigned* neg_list = nullptr;
+  unsigned* near_list = nullptr;  // D3Q39 near-face nodes (rebuilt when the faces change)
+  unsigned n_near = 0;
+  bool near_dirty = true;
   double* rep = nullptr;
   int64_t step = 0;
   // placement in the level's global index space; levels >= 1 carry a
@@ -176,6 +179,9 @@ void free_level(Level& L) {
   if (L.lm) { (void)hipFree(L.lm); L.lm = nullptr; }
   if (L.neg_count) { (void)hipFree(L.neg_count); L.neg_count = nullptr; }
   if (L.neg_list) { (void)hipFree(L.neg_list); L.neg_list = nullptr; }
+  if (L.near_list) { (void)hipFree(L.near_list); L.near_list = nullptr; }
+  L.n_near = 0;
+  L.near_dirty = true;
   if (L.rep) { (void)hipFree(L.rep); L.rep = nullptr; }
   if (L.ghosts) { (void)hipFree(L.ghosts); L.ghosts = nullptr; }
   L.nghost = 0;
@@ -336,6 +342,26 @@ int launch_steps39(Ctx* c, Level* L, int nsteps, hipEvent_t* ev) {
     a.faces.f[i] = c->fs39.f[i];
     a.faces.g[i] = c->fs39.g[i];
   }
+  if (a.has_faces && L->near_dirty) {
+    // nodes whose pulls may cross a face: stepped by their own launch
+    std::vector<unsigned> near;
+    const uclbs::Box& bx = L->box;
+    for (int z = 0; z < bx.nz; ++z)
+      for (int y = 0; y < bx.ny; ++y)
+        for (int x = 0; x < bx.nx; ++x)
+          if (uclbs::near_face39(x, y, z, L->face_code, bx.nx, bx.ny, bx.nz, c->shifted39 != 0))
+            near.push_back((unsigned)(x + (int64_t)bx.nx * (y + (int64_t)bx.ny * z)));
+    if (L->near_list) { HIP_TRY(hipFree(L->near_list)); L->near_list = nullptr; }
+    L->n_near = (unsigned)near.size();
+    if (!near.empty()) {
+      HIP_TRY(hipMalloc(&L->near_list, near.size() * sizeof(unsigned)));
+      HIP_TRY(hipMemcpy(L->near_list, near.data(), near.size() * sizeof(unsigned),
+                        hipMemcpyHostToDevice));
+    }
+    L->near_dirty = false;
+  }
+  a.near_list = L->near_list;
+  a.n_near = L->n_near;
   for (int s = 0; s < nsteps; ++s) {
     a.fin = L->f[L->cur];
     a.gin = L->g[L->cur];
@@ -1127,6 +1153,7 @@ int uclbs_set_faces(void* ctx, int lvl, const int32_t face_code[6]) {
       return set_err(UCLBS_E_ARG, "face %d: periodic code must match the level's periodic axes", f);
   }
   for (int f = 0; f < 6; ++f) L->face_code[f] = face_code[f];
+  L->near_dirty = true;
   return UCLBS_OK;
 }
 
