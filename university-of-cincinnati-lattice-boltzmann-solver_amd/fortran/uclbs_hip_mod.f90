@@ -17,6 +17,7 @@ module uclbs_hip
   integer(c_int32_t), parameter, public :: UCLBS_ALPHA_NEWTON = 0, UCLBS_ALPHA_FIXED2 = 1, &
                                            UCLBS_ALPHA_LEARNED = 2
   integer(c_int32_t), parameter, public :: UCLBS_FACE_PERIODIC = 2147483647
+  integer(c_int32_t), parameter, public :: UCLBS_FACE_OUTFLOW = -6, UCLBS_FACE_INVISCID = -11
 
   type, bind(c), public :: uclbs_config
     integer(c_int32_t) :: lattice = UCLBS_LATTICE_D3Q27
