@@ -145,7 +145,13 @@ int uclbs_set_bc_states(void *ctx, int n_in, const double *in_states, int n_out,
  * q codes as lbgrid/inter_bb_setup.f95:44-246 and linkwise.f95:45-57, face and
  * interior codes as lbgrid/state_directions.f95.  Node (a,b,c) of the box sits
  * at origin + (a,b,c)*dx (amr_processes.f95:1155-1168).  The table stays
- * resident; *n_tie counts rays re-cast because of a tie. */
+ * resident; *n_tie counts rays re-cast because of a tie.
+ * On a D3Q39 context of the plain lattice the table has 39 components, with
+ * the long links derived from the shorter ones as state_directions.f95:95-185
+ * does (integer divisions and its slot slip for the (0,3,0) link included);
+ * an outflow face gives its links the outside node's code 0.  Such a level is
+ * not stepped (no D3Q39 wall-link rule in this build); the shifted lattice's
+ * table (shifted_state_directions.f95) is refused. */
 int uclbs_build_links(void *ctx, int lvl, const double origin[3], double dx, int64_t ntri,
                       const double *tri, const double *nrm, int64_t *n_solid, int64_t *n_tie);
 /* Takes a reference-form table state[q*n + node] (int32, the `state` imultifab

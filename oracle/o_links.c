@@ -280,3 +280,115 @@ int64_t o_build_links27(const o_box *bx, int ntri, const double *tri, const doub
   if (n_tie) *n_tie = ties;
   return nsolid;
 }
+
+/* D3Q39 table on the plain lattice: lbgrid/state_directions.f95 with the
+ * velocities of directions_and_weights.f95:436-444.  Links are set direction
+ * by direction (the reference's outer loop), so a long link can read the
+ * codes of its shorter links, which are already final:
+ *   (+-2,0,0)-type, Fortran 16-21 (:95-125): when the speed-1 link i-14 is a
+ *     wall link s, the code is (s + 1000) / 2 (integer division toward zero;
+ *     s = -1001 gives 0, a fluid code);
+ *   (+-3,0,0)-type, Fortran 34-39 (:127-185): (s1 + 1000) / 3 from the
+ *     speed-1 link i-32, else ((s2 + 1000) * 2) / 3 from the speed-2 link
+ *     i-18 -- except Fortran 35, which reads its own slot state(...,35)
+ *     (:141-146), a value the pass has not written yet (AMReX leaves the
+ *     array uninitialised; the table starts at 0 here, as on the device).
+ * Every other link: face code (an outflow face, -6, gives 0), neighbour's
+ * node code, or the wall code of the segment node -> node + c dx
+ * (inter_bb_setup.f95:44-246), as on D3Q27.
+ * state must hold 39 x n int32 and is cleared here. */
+int64_t o_build_links39(const o_box *bx, int ntri, const double *tri, const double *nrm,
+                        int32_t *state, int64_t *n_tie) {
+  const int64_t n = (int64_t)bx->nx * bx->ny * bx->nz;
+  memset(state, 0, sizeof(int32_t) * (size_t)(39 * n));
+  double *tbb = (double *)malloc(sizeof(double) * 6 * (size_t)(ntri > 0 ? ntri : 1));
+  double gbb[6] = {-1e300, 1e300, -1e300, 1e300, -1e300, 1e300};
+  for (int i = 0; i < ntri; i++) {
+    tri_bbox(tri + 9 * (size_t)i, tbb + 6 * (size_t)i);
+    for (int d = 0; d < 3; d++) {
+      if (tbb[6 * i + 2 * d] > gbb[2 * d]) gbb[2 * d] = tbb[6 * i + 2 * d];
+      if (tbb[6 * i + 2 * d + 1] < gbb[2 * d + 1]) gbb[2 * d + 1] = tbb[6 * i + 2 * d + 1];
+    }
+  }
+  int64_t ties = 0, nsolid = 0;
+#ifdef _OPENMP
+#pragma omp parallel for collapse(2) schedule(dynamic, 4) reduction(+ : ties, nsolid)
+#endif
+  for (int c = 0; c < bx->nz; c++)
+    for (int b = 0; b < bx->ny; b++)
+      for (int a = 0; a < bx->nx; a++) {
+        int64_t k = a + (int64_t)bx->nx * (b + (int64_t)bx->ny * c);
+        int tie = 0;
+        int solid = ntri > 0 ? o_classify_node(bx, ntri, tri, nrm, tbb, gbb, a, b, c, &tie) : 0;
+        state[k] = solid ? -1001 : 0;
+        ties += tie;
+        nsolid += solid;
+      }
+#ifdef _OPENMP
+#pragma omp parallel for collapse(2) schedule(dynamic, 4) reduction(+ : ties)
+#endif
+  for (int c = 0; c < bx->nz; c++)
+    for (int b = 0; b < bx->ny; b++)
+      for (int a = 0; a < bx->nx; a++) {
+        int64_t k = a + (int64_t)bx->nx * (b + (int64_t)bx->ny * c);
+        int32_t own = state[k];
+        for (int i = 1; i < 39; i++) {      /* Fortran direction i + 1 */
+          int32_t code;
+          if (own == -1001 || own == -666) {
+            state[(int64_t)i * n + k] = -666;
+            continue;
+          }
+          if (i >= 15 && i <= 20) {          /* Fortran 16-21 */
+            int32_t s1 = state[(int64_t)(i - 14) * n + k];
+            if (s1 < -1000) {
+              state[(int64_t)i * n + k] = (s1 + 1000) / 2;
+              continue;
+            }
+          } else if (i >= 33) {              /* Fortran 34-39 */
+            int32_t s1 = state[(int64_t)(i - 32) * n + k];
+            int32_t s2 = state[(int64_t)(i - 18) * n + k];
+            int32_t s2r = (i == 34) ? state[(int64_t)i * n + k] : s2;
+            if (s1 < -1000) {
+              state[(int64_t)i * n + k] = (s1 + 1000) / 3;
+              continue;
+            } else if (s2 < -1000) {
+              state[(int64_t)i * n + k] = ((s2r + 1000) * 2) / 3;
+              continue;
+            }
+          }
+          int cx = O39_CX[i], cy = O39_CY[i], cz = O39_CZ[i];
+          int f = face_of(bx, a, b, c, cx, cy, cz);
+          int na = a + cx, nb = b + cy, nc = c + cz;
+          int wrap = 1;
+          if (f >= 0) {
+            for (int g = 0; g < 6 && wrap; g++) {
+              int out = (g == 0 && na < 0) || (g == 1 && nb < 0) || (g == 2 && nc < 0) ||
+                        (g == 3 && na > bx->nx - 1) || (g == 4 && nb > bx->ny - 1) ||
+                        (g == 5 && nc > bx->nz - 1);
+              if (out && bx->face_code[g] != O_FACE_PERIODIC) { wrap = 0; f = g; }
+            }
+          }
+          if (f >= 0 && !wrap) {
+            /* an outflow face (outlet type 6) gives the code of the node
+             * outside, a fluid node of level 0 (state_directions_lvl.f95:131) */
+            code = bx->face_code[f] == -6 ? 0 : bx->face_code[f];
+          } else {
+            na = ((na % bx->nx) + bx->nx) % bx->nx;
+            nb = ((nb % bx->ny) + bx->ny) % bx->ny;
+            nc = ((nc % bx->nz) + bx->nz) % bx->nz;
+            int32_t nbr = state[na + (int64_t)bx->nx * (nb + (int64_t)bx->ny * nc)];
+            if (nbr == -1001) {
+              int tie = 0;
+              code = o_wall_code(bx, ntri, tri, nrm, tbb, a, b, c, cx, cy, cz, &tie);
+              ties += tie;
+            } else {
+              code = nbr;
+            }
+          }
+          state[(int64_t)i * n + k] = code;
+        }
+      }
+  free(tbb);
+  if (n_tie) *n_tie = ties;
+  return nsolid;
+}

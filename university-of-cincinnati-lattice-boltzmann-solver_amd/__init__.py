@@ -445,12 +445,12 @@ class Solver:
 
     def set_links(self, lvl: int, state: np.ndarray):
         n = self.n_nodes(lvl)
-        st = np.ascontiguousarray(state, dtype=np.int32).reshape(27, n)
+        st = np.ascontiguousarray(state, dtype=np.int32).reshape(self.q, n)
         self._check(self.lib.uclbs_set_links(self.ctx, lvl, _iptr(st), n))
 
     def download_links(self, lvl: int) -> np.ndarray:
         n = self.n_nodes(lvl)
-        out = np.empty((27, n), dtype=np.int32)
+        out = np.empty((self.q, n), dtype=np.int32)
         self._check(self.lib.uclbs_download_links(self.ctx, lvl, _iptr(out), n))
         return out
 

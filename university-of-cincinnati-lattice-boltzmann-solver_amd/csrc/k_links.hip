@@ -227,6 +227,75 @@ __global__ __launch_bounds__(256) void k_links(LinkGeom g, int32_t* __restrict__
   if (ties) atomicAdd(counts + 1, (unsigned long long)ties);
 }
 
+// D3Q39 table on the plain lattice (lbgrid/state_directions.f95): one thread
+// per node sets its links in the reference's direction order, so a long link
+// reads the final codes of its shorter links (:95-185): (s + 1000) / 2 for the
+// (+-2,0,0)-type from the speed-1 wall link, (s1 + 1000) / 3 or
+// ((s2 + 1000) * 2) / 3 for the (+-3,0,0)-type -- the (0,3,0) link reading its
+// own slot, as the reference reads state(...,35); the table is cleared first.
+__global__ __launch_bounds__(256) void k_links39(LinkGeom g, int32_t* __restrict__ state,
+                                                 unsigned long long* __restrict__ counts) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= g.box.n) return;
+  const int a = (int)(k % g.box.nx);
+  const int64_t yz = k / g.box.nx;
+  const int b = (int)(yz % g.box.ny), c = (int)(yz / g.box.ny);
+  const int64_t n = g.box.n;
+  const int32_t own = state[k];
+  int ties = 0;
+  for (int i = 1; i < 39; ++i) {
+    int32_t* slot = state + (int64_t)i * n + k;
+    if (own == -1001 || own == -666) {
+      *slot = -666;
+      continue;
+    }
+    if (i >= 15 && i <= 20) {
+      const int32_t s1 = state[(int64_t)(i - 14) * n + k];
+      if (s1 < -1000) {
+        *slot = (s1 + 1000) / 2;
+        continue;
+      }
+    } else if (i >= 33) {
+      const int32_t s1 = state[(int64_t)(i - 32) * n + k];
+      const int32_t s2 = state[(int64_t)(i - 18) * n + k];
+      if (s1 < -1000) {
+        *slot = (s1 + 1000) / 3;
+        continue;
+      }
+      if (s2 < -1000) {
+        const int32_t v = (i == 34) ? *slot : s2;
+        *slot = ((v + 1000) * 2) / 3;
+        continue;
+      }
+    }
+    const int cx = CX39[i], cy = CY39[i], cz = CZ39[i];
+    int na = a + cx, nb = b + cy, nc = c + cz;
+    const bool out[6] = {na < 0, nb < 0, nc < 0, na > g.box.nx - 1, nb > g.box.ny - 1,
+                         nc > g.box.nz - 1};
+    int face = -1;
+    for (int f = 0; f < 6; ++f)
+      if (out[f] && g.face_code[f] != kFacePeriodic) { face = f; break; }
+    int32_t code;
+    if (face >= 0) {
+      code = g.face_code[face];
+    } else {
+      na = (na + g.box.nx) % g.box.nx;
+      nb = (nb + g.box.ny) % g.box.ny;
+      nc = (nc + g.box.nz) % g.box.nz;
+      const int32_t nbr = state[na + (int64_t)g.box.nx * (nb + (int64_t)g.box.ny * nc)];
+      if (nbr == -1001) {
+        int tie = 0;
+        code = wall_code(g, a, b, c, cx, cy, cz, &tie);
+        ties += tie;
+      } else {
+        code = nbr;
+      }
+    }
+    *slot = code;
+  }
+  if (ties) atomicAdd(counts + 1, (unsigned long long)ties);
+}
+
 // node flags: interior fluid (every link a fluid code), face (fluid whose
 // negative links all leave the box with exactly the face's code, so the
 // streaming kernel can rebuild them from coordinates), boundary (any other
@@ -273,6 +342,17 @@ hipError_t launch_build_links27(const LinkGeom& g, int32_t* state, unsigned long
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   k_links<<<blocks_for(g.box.n, 256), 256, 0, s>>>(g, state, counts);
+  return hipGetLastError();
+}
+
+hipError_t launch_build_links39(const LinkGeom& g, int32_t* state, unsigned long long* counts,
+                                hipStream_t s) {
+  hipError_t e = hipMemsetAsync(state, 0, (size_t)g.box.n * 39 * sizeof(int32_t), s);
+  if (e != hipSuccess) return e;
+  k_classify<<<blocks_for(g.box.n, 256), 256, 0, s>>>(g, state, counts);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  k_links39<<<blocks_for(g.box.n, 256), 256, 0, s>>>(g, state, counts);
   return hipGetLastError();
 }
 

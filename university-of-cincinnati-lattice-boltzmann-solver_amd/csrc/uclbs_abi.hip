@@ -63,6 +63,7 @@ struct Level {
   unsigned* neg_count = nullptr;
   unsigned* neg_list = nullptr;
   unsigned* near_list = nullptr;  // D3Q39 near-face nodes (rebuilt when the faces change)
+  bool table39 = false;           // a D3Q39 link table is resident (not stepped)
   unsigned n_near = 0;
   bool near_dirty = true;
   double* rep = nullptr;
@@ -182,6 +183,7 @@ void free_level(Level& L) {
   if (L.near_list) { (void)hipFree(L.near_list); L.near_list = nullptr; }
   L.n_near = 0;
   L.near_dirty = true;
+  L.table39 = false;
   if (L.rep) { (void)hipFree(L.rep); L.rep = nullptr; }
   if (L.ghosts) { (void)hipFree(L.ghosts); L.ghosts = nullptr; }
   L.nghost = 0;
@@ -322,6 +324,9 @@ int check_face_tables(const Ctx* c, const Level* L) {
 }
 
 int launch_steps39(Ctx* c, Level* L, int nsteps, hipEvent_t* ev) {
+  if (L->table39)
+    return set_err(UCLBS_E_UNSUPPORTED, "D3Q39 levels with a link table are not stepped: the "
+                                        "D3Q39 wall-link rule is not part of this build");
   uclbs::Step39 a{};
   a.gas = c->gas39;
   a.ref = c->ref39;
@@ -1196,8 +1201,10 @@ int uclbs_build_links(void* ctx, int lvl, const double origin[3], double dx, int
   if (rc) return rc;
   if (L->multibox) return set_err(UCLBS_E_UNSUPPORTED, "no link tables on levels of many boxes");
   if (L->refined) return set_err(UCLBS_E_UNSUPPORTED, "level %d is a refined level (no link tables)", lvl);
-  if (c->cfg.lattice != UCLBS_LATTICE_D3Q27)
-    return set_err(UCLBS_E_UNSUPPORTED, "build_links: D3Q27 tables only");
+  const bool q39 = c->cfg.lattice == UCLBS_LATTICE_D3Q39;
+  if (q39 && c->shifted39)
+    return set_err(UCLBS_E_UNSUPPORTED, "build_links: D3Q39 tables of the plain lattice only "
+                                        "(shifted_state_directions is not part of this build)");
   if (!origin || !(dx > 0.0) || ntri < 0 || ntri > (1 << 26) || (ntri && (!tri || !nrm)))
     return set_err(UCLBS_E_ARG, "build_links: bad geometry arguments");
   if ((rc = set_device(c))) return rc;
@@ -1206,7 +1213,11 @@ int uclbs_build_links(void* ctx, int lvl, const double origin[3], double dx, int
   g.box = L->box;
   for (int d = 0; d < 3; ++d) g.origin[d] = origin[d];
   g.dx = dx;
-  for (int f = 0; f < 6; ++f) g.face_code[f] = L->face_code[f];
+  for (int f = 0; f < 6; ++f) {
+    // an outflow face (outlet type 6) gives its links the code of the node
+    // outside, a fluid node of level 0 (state_directions_lvl.f95:131-132)
+    g.face_code[f] = (q39 && L->face_code[f] == -6) ? 0 : L->face_code[f];
+  }
   g.ntri = (int)ntri;
   std::vector<double> tbb((size_t)(ntri > 0 ? ntri : 1) * 6);
   double gbb[6] = {-1e300, 1e300, -1e300, 1e300, -1e300, 1e300};
@@ -1232,13 +1243,14 @@ int uclbs_build_links(void* ctx, int lvl, const double origin[3], double dx, int
   if (e == hipSuccess && ntri) e = hipMemcpy(dnrm, nrm, (size_t)ntri * 3 * sizeof(double), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(dtbb, tbb.data(), nt * 6 * sizeof(double), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemset(dcnt, 0, 2 * sizeof(unsigned long long));
-  if (e == hipSuccess && !L->state) e = hipMalloc(&L->state, (size_t)n * 27 * sizeof(int32_t));
+  if (e == hipSuccess && !L->state) e = hipMalloc(&L->state, (size_t)n * q_of(c) * sizeof(int32_t));
   unsigned long long hc[2] = {0, 0};
   if (e == hipSuccess) {
     g.tri = dtri;
     g.nrm = dnrm;
     g.tbb = dtbb;
-    e = uclbs::launch_build_links27(g, L->state, dcnt, c->stream);
+    e = q39 ? uclbs::launch_build_links39(g, L->state, dcnt, c->stream)
+            : uclbs::launch_build_links27(g, L->state, dcnt, c->stream);
   }
   if (e == hipSuccess) e = hipMemcpyAsync(hc, dcnt, sizeof hc, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -1247,7 +1259,12 @@ int uclbs_build_links(void* ctx, int lvl, const double origin[3], double dx, int
   (void)hipFree(dtbb);
   (void)hipFree(dcnt);
   if (e != hipSuccess) return set_err(UCLBS_E_HIP, "build_links failed: %s", hipGetErrorString(e));
-  if ((rc = finalize_links(c, L))) return rc;
+  if (q39) {
+    L->table39 = true;
+    L->nsolid = (int64_t)hc[0];
+  } else if ((rc = finalize_links(c, L))) {
+    return rc;
+  }
   if (n_solid) *n_solid = (int64_t)hc[0];
   if (n_tie) *n_tie = (int64_t)hc[1];
   return UCLBS_OK;
@@ -1262,8 +1279,13 @@ int uclbs_set_links(void* ctx, int lvl, const int32_t* state, int64_t n_nodes) {
   if (L->refined) return set_err(UCLBS_E_UNSUPPORTED, "level %d is a refined level (no link tables)", lvl);
   if (!state || n_nodes != L->box.n) return set_err(UCLBS_E_ARG, "set_links: wrong node count");
   if ((rc = set_device(c))) return rc;
-  if (!L->state) HIP_TRY(hipMalloc(&L->state, (size_t)n_nodes * 27 * sizeof(int32_t)));
-  HIP_TRY(hipMemcpy(L->state, state, (size_t)n_nodes * 27 * sizeof(int32_t), hipMemcpyHostToDevice));
+  const size_t bytes = (size_t)n_nodes * q_of(c) * sizeof(int32_t);
+  if (!L->state) HIP_TRY(hipMalloc(&L->state, bytes));
+  HIP_TRY(hipMemcpy(L->state, state, bytes, hipMemcpyHostToDevice));
+  if (c->cfg.lattice == UCLBS_LATTICE_D3Q39) {
+    L->table39 = true;
+    return UCLBS_OK;
+  }
   return finalize_links(c, L);
 }
 
@@ -1275,7 +1297,8 @@ int uclbs_download_links(void* ctx, int lvl, int32_t* state, int64_t n_nodes) {
   if (!L->state) return set_err(UCLBS_E_STATE, "level %d has no link table", lvl);
   if (!state || n_nodes != L->box.n) return set_err(UCLBS_E_ARG, "download_links: wrong node count");
   if ((rc = set_device(c))) return rc;
-  HIP_TRY(hipMemcpy(state, L->state, (size_t)n_nodes * 27 * sizeof(int32_t), hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(state, L->state, (size_t)n_nodes * q_of(c) * sizeof(int32_t),
+                    hipMemcpyDeviceToHost));
   return UCLBS_OK;
 }
 
@@ -1284,6 +1307,8 @@ int uclbs_link_counts(void* ctx, int lvl, int64_t* n_boundary, int64_t* n_solid)
   Level* L;
   int rc = check_level(c, lvl, &L);
   if (rc) return rc;
+  if (L->table39)
+    return set_err(UCLBS_E_UNSUPPORTED, "D3Q39 tables: n_solid comes from uclbs_build_links");
   if (!L->has_links) return set_err(UCLBS_E_STATE, "level %d has no link table", lvl);
   if (n_boundary) *n_boundary = L->nb;
   if (n_solid) *n_solid = L->nsolid;
