@@ -52,7 +52,7 @@ struct SeriesThr {
   float m_mid;              // order kMidOrder, the second in-register tier
 };
 constexpr int kFastOrder = 6;
-constexpr int kMidOrder = 10;
+constexpr int kMidOrder = 12;
 inline double series_limit(int N, double eps) {
   return __builtin_pow(eps * N * (N - 1.0), 1.0 / (N - 1.0)) / 2.2;
 }
