@@ -210,7 +210,7 @@ template <bool kShifted, bool kKeepAlpha>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k27_step_boxes(
     const void* __restrict__ fin_, void* __restrict__ fout_, double* __restrict__ alpha_out,
     unsigned long long* __restrict__ stats, unsigned nn, StepArgs a,
-    const uint8_t* __restrict__ flags, const uint2* __restrict__ alias,
+    const uint8_t* __restrict__ flags, const uint4* __restrict__ alias,
     const BoxTile* __restrict__ tiles, const BoxDesc* __restrict__ boxes, int tile_nodes) {
   using T = typename std::conditional<kShifted, float, double>::type;
   const T* __restrict__ fin = static_cast<const T*>(fin_);
@@ -220,18 +220,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   const unsigned local = t.start + threadIdx.x;
   const bool in_box = (int)threadIdx.x < tile_nodes && local < bx.n;
   const unsigned node = bx.off + (in_box ? local : 0u);
-  const uint8_t fl = in_box ? flags[node] : kFlagGhost;
-  const bool active = fl == kFlagInterior;  // counted node updates
-  const bool mirror = fl == kFlagMirror;
-  // the node whose update this lane computes, and that node's box strides
-  unsigned src = node, sy = bx.nx, sz = bx.nx * bx.ny;
-  if (mirror) {
-    const uint2 al = alias[node];
-    const BoxDesc sb = boxes[al.y];
-    src = al.x;
-    sy = sb.nx;
-    sz = sb.nx * sb.ny;
+  // the valid nodes are the inside of the stored box, the ghost shell its
+  // outer layer: known from the coordinates, so only shell lanes read their
+  // flag and mirror entry (together), and the pulls of a wavefront wait for
+  // one load instead of a chain of three
+  const unsigned sxy = bx.nx * bx.ny;
+  const unsigned lx = local % bx.nx, lyz = local / bx.nx, ly = lyz % bx.ny, lz = lyz / bx.ny;
+  const bool shell = lx == 0u || ly == 0u || lz == 0u || lx + 1u == bx.nx || ly + 1u == bx.ny ||
+                     lz + 1u == bx.n / sxy;
+  const bool active = in_box && !shell;  // counted node updates
+  uint8_t fl = kFlagInterior;
+  uint4 al = make_uint4(0u, 0u, 0u, 0u);
+  if (in_box && shell) {
+    fl = flags[node];
+    al = alias[node];
   }
+  const bool mirror = in_box && shell && fl == kFlagMirror;
+  // the node whose update this lane computes, and that node's box strides
+  const unsigned src = mirror ? al.x : node;
+  const unsigned sy = mirror ? al.y : bx.nx;
+  const unsigned sz = mirror ? al.z : sxy;
   const bool pull = active || mirror;
   T f[27];
 #pragma unroll
@@ -720,7 +728,7 @@ hipError_t launch_d3q27_step_links(const Box& b, const void* fin, void* fout, do
 
 hipError_t launch_d3q27_step_boxes(unsigned nn, const void* fin, void* fout, double* alpha,
                                    int keep_alpha, unsigned long long* stats, const StepArgs& a,
-                                   const uint8_t* flags, const uint2* alias, const BoxTile* tiles,
+                                   const uint8_t* flags, const uint4* alias, const BoxTile* tiles,
                                    unsigned ntiles, const BoxDesc* boxes, int tile_nodes,
                                    hipStream_t s) {
   if (!ntiles) return hipSuccess;

@@ -153,10 +153,11 @@ struct C2FStencil { // a ghost node interpolated from the coarser level
   unsigned src[8];  // coarse stored nodes, tensor order (z outer, x inner)
   float w[8];       // their weights (products of 1 and 1/2; 0 = unused)
 };
-// alias[k] = {valid node mirrored by node k, its box} for kFlagMirror nodes
+// alias[k] = {valid node mirrored by node k, its box's row and plane strides,
+// 0} for kFlagMirror nodes
 hipError_t launch_d3q27_step_boxes(unsigned nn, const void* fin, void* fout, double* alpha,
                                    int keep_alpha, unsigned long long* stats, const StepArgs& a,
-                                   const uint8_t* flags, const uint2* alias, const BoxTile* tiles,
+                                   const uint8_t* flags, const uint4* alias, const BoxTile* tiles,
                                    unsigned ntiles, const BoxDesc* boxes, int tile_nodes,
                                    hipStream_t s);
 // dst[i][p.x] = src[i][p.y] for each pair p and each population i in p.z

@@ -83,7 +83,7 @@ struct Level {
   unsigned ntiles = 0;
   int tile_nodes = 256;
   uint3* d_copies = nullptr;  // same-level ghost copies {dst, src, population mask}
-  uint2* d_alias = nullptr;   // {mirrored valid node, its box} of every kFlagMirror node
+  uint4* d_alias = nullptr;   // {mirrored valid node, its box strides, 0} of every kFlagMirror node
   unsigned ncopies = 0;
   uclbs::C2FStencil* d_c2f = nullptr;  // ghosts interpolated from the coarser level
   unsigned nc2f = 0;
@@ -870,7 +870,7 @@ int uclbs_define_level_boxes(void* ctx, int lvl, int nbox, const int32_t* lo, co
   build_buckets(L);
   // ghost sources
   std::vector<uint8_t> hf((size_t)off, uclbs::kFlagInterior);
-  std::vector<uint2> alias((size_t)off, make_uint2(0u, 0u));
+  std::vector<uint4> alias((size_t)off, make_uint4(0u, 0u, 0u, 0u));
   std::vector<uint3> copies;  // none: mirrors are updated by the streaming kernel
   std::vector<uclbs::C2FStencil> c2f;
   const Level* C = lvl > 0 ? &c->levels[lvl - 1] : nullptr;
@@ -901,7 +901,8 @@ int uclbs_define_level_boxes(void* ctx, int lvl, int nbox, const int32_t* lo, co
           const int64_t src = valid_index(L, G, &sbox);
           if (src >= 0) {
             hf[k] = uclbs::kFlagMirror;
-            alias[k] = make_uint2((unsigned)src, (unsigned)sbox);
+            const uclbs::BoxDesc& sb = L.hbox[sbox];
+            alias[k] = make_uint4((unsigned)src, sb.nx, sb.nx * sb.ny, 0u);
             continue;
           }
           uclbs::C2FStencil st{};
