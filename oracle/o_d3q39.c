@@ -418,7 +418,7 @@ static int axis_out(int64_t s, int64_t n, int f) { return f < 3 ? s < 0 : s >= n
  * lies across a non-periodic face (a link whose code is negative, which the
  * reference repairs from the other side only), else its flat index. */
 static int64_t damp_neighbour(int64_t a, int64_t b, int64_t cc, int dx, int dy, int dz, int nx,
-                              int ny, int nz, const int32_t *fc) {
+                              int ny, int nz, const int32_t *fc, int ghost) {
   int64_t s[3] = {a + dx, b + dy, cc + dz};
   const int64_t n[3] = {nx, ny, nz};
   for (int d = 0; d < 3; d++) {
@@ -429,6 +429,8 @@ static int64_t damp_neighbour(int64_t a, int64_t b, int64_t cc, int dx, int dy, 
       if (fc[d + 3] != O39_FACE_PERIODIC) return -1;
       s[d] -= n[d];
     }
+    /* a refined level's ghost layer holds no post-collision values */
+    if (ghost > 0 && (s[d] < ghost || s[d] >= n[d] - ghost)) return -1;
   }
   return s[0] + n[0] * (s[1] + n[1] * s[2]);
 }
@@ -439,8 +441,23 @@ static int64_t damp_neighbour(int64_t a, int64_t b, int64_t cc, int dx, int dy, 
  * is the one-sided form of :109-123.  jacobi = 1 reads the neighbours' values
  * from before any repair (the order-free form the GPU uses); jacobi = 0 sweeps
  * in the reference order (direction, then z, y, x) and sees earlier repairs. */
+static int in_ghost(int64_t a, int64_t b, int64_t cc, int nx, int ny, int nz, int ghost) {
+  return ghost > 0 && (a < ghost || b < ghost || cc < ghost || a >= nx - ghost || b >= ny - ghost ||
+                       cc >= nz - ghost);
+}
+
+static int64_t damp_core(int nx, int ny, int nz, const int32_t fc[6], int ghost, double *fo,
+                         int shifted, int jacobi);
+
 int64_t o39_damp_faces(int nx, int ny, int nz, const int32_t fc[6], double *fo, int shifted,
                        int jacobi) {
+  return damp_core(nx, ny, nz, fc, 0, fo, shifted, jacobi);
+}
+
+/* a refined level's ghost layer (depth ghost) is not repaired: its nodes are
+ * refilled from the coarser level */
+static int64_t damp_core(int nx, int ny, int nz, const int32_t fc[6], int ghost, double *fo,
+                         int shifted, int jacobi) {
   int cx[NQ], cy[NQ], cz[NQ];
   o39_velocities(shifted, cx, cy, cz);
   const int64_t n = (int64_t)nx * ny * nz;
@@ -455,8 +472,9 @@ int64_t o39_damp_faces(int nx, int ny, int nz, const int32_t fc[6], double *fo, 
       double v = src[(int64_t)i * n + k];
       if (!(v < 0.0)) continue;
       int64_t a = k % nx, b = (k / nx) % ny, cc = k / ((int64_t)nx * ny);
-      int64_t kp = damp_neighbour(a, b, cc, cx[i], cy[i], cz[i], nx, ny, nz, fc);
-      int64_t km = damp_neighbour(a, b, cc, -cx[i], -cy[i], -cz[i], nx, ny, nz, fc);
+      if (in_ghost(a, b, cc, nx, ny, nz, ghost)) continue;
+      int64_t kp = damp_neighbour(a, b, cc, cx[i], cy[i], cz[i], nx, ny, nz, fc, ghost);
+      int64_t km = damp_neighbour(a, b, cc, -cx[i], -cy[i], -cz[i], nx, ny, nz, fc, ghost);
       double fp = kp >= 0 ? src[(int64_t)i * n + kp] : -1.0;
       double fm = km >= 0 ? src[(int64_t)i * n + km] : -1.0;
       double r;
@@ -571,10 +589,35 @@ int o39_source(int64_t a, int64_t b, int64_t cc, int k, int nx, int ny, int nz, 
 /* One or more steps of a box whose faces carry the codes fc (see source39);
  * fs_f / fs_g are the freestream equilibria (freestream_basis_calcs.f90:66-71)
  * and may be NULL when no face is -1000. */
+static void step_core(int nx, int ny, int nz, const int32_t fc[6], int ghost, const double *fs_f,
+                      const double *fs_g, double *f, double *g, double *lm, double *alpha_out,
+                      const o39_consts *c, int64_t step0, int nsteps, int ref_lu, int jacobi,
+                      o39_stats *st);
+
 void o39_step_faces(int nx, int ny, int nz, const int32_t fc[6], const double *fs_f,
                     const double *fs_g, double *f, double *g, double *lm, double *alpha_out,
                     const o39_consts *c, int64_t step0, int nsteps, int ref_lu, int jacobi,
                     o39_stats *st) {
+  step_core(nx, ny, nz, fc, 0, fs_f, fs_g, f, g, lm, alpha_out, c, step0, nsteps, ref_lu, jacobi,
+            st);
+}
+
+/* A refined level stored with a ghost layer of depth ghost: only the valid
+ * nodes inside it are streamed and collided (their links stay inside the
+ * stored box), the layer keeps its values (it is refilled from the coarser
+ * level before the next step). */
+void o39_step_ghost(int nx, int ny, int nz, int ghost, double *f, double *g, double *lm,
+                    double *alpha_out, const o39_consts *c, int64_t step0, int nsteps, int ref_lu,
+                    int jacobi, o39_stats *st) {
+  const int32_t fc[6] = {-10, -10, -10, -10, -10, -10};
+  step_core(nx, ny, nz, fc, ghost, NULL, NULL, f, g, lm, alpha_out, c, step0, nsteps, ref_lu,
+            jacobi, st);
+}
+
+static void step_core(int nx, int ny, int nz, const int32_t fc[6], int ghost, const double *fs_f,
+                      const double *fs_g, double *f, double *g, double *lm, double *alpha_out,
+                      const o39_consts *c, int64_t step0, int nsteps, int ref_lu, int jacobi,
+                      o39_stats *st) {
   int cx[NQ], cy[NQ], cz[NQ];
   o39_velocities(c->shifted, cx, cy, cz);
   const int64_t n = (int64_t)nx * ny * nz;
@@ -589,6 +632,13 @@ void o39_step_faces(int nx, int ny, int nz, const int32_t fc[6], const double *f
     for (int64_t k = 0; k < n; k++) {
       int64_t a = k % nx, b = (k / nx) % ny, cc = k / ((int64_t)nx * ny);
       double fi[NQ], gi[NQ], l[NM], e[NQ], u[3], P[6], rho, T;
+      if (in_ghost(a, b, cc, nx, ny, nz, ghost)) {
+        for (int i = 0; i < NQ; i++) {
+          fo[(int64_t)i * n + k] = f[(int64_t)i * n + k];
+          go[(int64_t)i * n + k] = g[(int64_t)i * n + k];
+        }
+        continue;
+      }
       for (int i = 0; i < NQ; i++) {
         int64_t src;
         int dir;
@@ -625,7 +675,12 @@ void o39_step_faces(int nx, int ny, int nz, const int32_t fc[6], const double *f
         go[(int64_t)i * n + k] = gq[i];
       }
     }
-    st->nodes += n;
+    {
+      int64_t valid = n;
+      if (ghost > 0)
+        valid = (int64_t)(nx - 2 * ghost) * (ny - 2 * ghost) * (nz - 2 * ghost);
+      st->nodes += valid;
+    }
     st->lm_iters += it_sum;
     st->lm_noconv += noconv;
     st->n_alpha2 += a2;
@@ -633,7 +688,7 @@ void o39_step_faces(int nx, int ny, int nz, const int32_t fc[6], const double *f
     st->n_alpha_hi += ahi;
     st->n_nan += nan;
     st->n_kappa_err += kerr;
-    st->n_repair += o39_damp_faces(nx, ny, nz, fc, fo, c->shifted, jacobi);
+    st->n_repair += damp_core(nx, ny, nz, fc, ghost, fo, c->shifted, jacobi);
     memcpy(f, fo, sizeof(double) * (size_t)(n * NQ));
     memcpy(g, go, sizeof(double) * (size_t)(n * NQ));
   }

@@ -135,6 +135,8 @@ def lib() -> ctypes.CDLL:
         ip = ctypes.POINTER(ctypes.c_int32)
         L.o39_damp_faces.argtypes = [I, I, I, ip, dp, I, I]
         L.o39_damp_faces.restype = ctypes.c_int64
+        L.o39_step_ghost.argtypes = [I, I, I, I, dp, dp, dp, dp, c39, ctypes.c_int64, I, I, I,
+                                     ctypes.POINTER(Stats39)]
         L.o39_source.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, I, I, I, I, ip, I,
                                  ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(I)]
         L.o39_source.restype = I
@@ -535,6 +537,58 @@ def source39(node, k, shape, faces, shifted=False):
         return None, None
     v = src.value
     return (v % nx, (v // nx) % ny, v // (nx * ny)), d.value
+
+
+def step_ghost39(shape, ghost, f, g, lm, c, step0=0, nsteps=1, ref_lu=False, jacobi=True,
+                 keep_alpha=False):
+    """Steps of a refined level stored with a ghost layer of depth `ghost`:
+    only the valid nodes inside it are updated."""
+    nx, ny, nz = shape
+    f = np.array(f, np.float64)
+    g = np.array(g, np.float64)
+    lm = np.array(lm, np.float64)
+    alpha = np.zeros(nx * ny * nz) if keep_alpha else None
+    st = Stats39()
+    lib().o39_step_ghost(nx, ny, nz, int(ghost), _dp(f), _dp(g), _dp(lm),
+                         _dp(alpha) if alpha is not None else None, ctypes.byref(c), step0, nsteps,
+                         int(ref_lu), int(jacobi), ctypes.byref(st))
+    return f, g, lm, alpha, st.as_dict()
+
+
+def coarse_to_fine39(cprev_f, cnew_f, cprev_g, cnew_g, clm, cshape, corigin, fshape, forigin,
+                     theta, c, nodes=None):
+    """Fine f, g, multipliers at the stored fine nodes (all if None) from the
+    coarse level (interlevel_interpolation.f95:114-493): f, g, rho, T
+    interpolated in space and time, the multipliers in space; a negative or NaN
+    f or g becomes the equilibrium of the interpolated coarse state
+    (:417-431).  Returns (f, g, lm, n_repaired)."""
+    import o_amr
+    args = (cshape, corigin, fshape, forigin, theta, nodes)
+    fi = o_amr.coarse_to_fine(cprev_f, cnew_f, *args)
+    gi = o_amr.coarse_to_fine(cprev_g, cnew_g, *args)
+    lmi = o_amr.coarse_to_fine(clm, clm, *args)
+
+    def rho_t(f, g):
+        rho, ux, uy, uz, T = macro39(f, g, c)
+        return np.stack([rho, T])
+    rt = o_amr.coarse_to_fine(rho_t(cprev_f, cprev_g), rho_t(cnew_f, cnew_g), *args)
+    bad = ~((fi >= 0) & (gi >= 0))
+    nrep = int(bad.sum())
+    for k in np.nonzero(bad.any(0))[0]:
+        e = feq39(lmi[:, k], rt[0, k], bool(c.shifted))
+        m = bad[:, k]
+        fi[m, k] = e[m]
+        gi[m, k] = c.lead_g * rt[1, k] * e[m]
+    return fi, gi, lmi, nrep
+
+
+def ghost_nodes39(fshape, ghost=4):
+    nx, ny, nz = fshape
+    k = np.arange(nx * ny * nz)
+    x, y, z = k % nx, (k // nx) % ny, k // (nx * ny)
+    g = (x < ghost) | (y < ghost) | (z < ghost) | (x >= nx - ghost) | (y >= ny - ghost) | \
+        (z >= nz - ghost)
+    return np.nonzero(g)[0]
 
 
 def damp_faces39(shape, faces, fo, shifted=False, jacobi=True):

@@ -163,6 +163,9 @@ void o39_init_fields(int nx, int ny, int nz, const double *rho, const double *ux
 void o39_step_periodic(int nx, int ny, int nz, double *f, double *g, double *lm, double *alpha_out,
                        const o39_consts *c, int64_t step0, int nsteps, int ref_lu, int jacobi,
                        o39_stats *st);
+void o39_step_ghost(int nx, int ny, int nz, int ghost, double *f, double *g, double *lm,
+                    double *alpha_out, const o39_consts *c, int64_t step0, int nsteps, int ref_lu,
+                    int jacobi, o39_stats *st);
 int o39_source(int64_t a, int64_t b, int64_t cc, int k, int nx, int ny, int nz, const int32_t fc[6],
                int shifted, int64_t *src, int *dir);
 void o39_step_faces(int nx, int ny, int nz, const int32_t fc[6], const double *fs_f,

@@ -14,7 +14,9 @@ Pinned by:
     outflow out, inviscid sides) is a fixed point;
   * a box closed by bounce-back or inviscid faces conserves mass and energy
     (every lost population is replaced by exactly one missing one);
-  * the repair's one-sided rule at a face (damp_it.f95:109-123)."""
+  * the repair's one-sided rule at a face (damp_it.f95:109-123);
+  * a refined level's interior step (ghost layer of depth 4, not updated)
+    equals the periodic step when the layer holds the periodic continuation."""
 import numpy as np
 import pytest
 
@@ -221,3 +223,30 @@ def test_damp_one_sided_at_faces():
     assert out[4, 5] == (0.5 + 0.1) / 2           # one side: x + c = 4
     per, _ = o.damp_faces39(shape, [P] * 6, fo)
     assert per[1, 0] == (0.3 + 0.5) / 2            # periodic: both sides positive
+
+
+@pytest.mark.parametrize("shifted,mach", [(False, 0.3), (True, 1.5)])
+def test_ghost_layer_step_equals_periodic_step(shifted, mach):
+    """A refined level's interior step (ghost layer of depth 4, not updated):
+    with the layer holding the periodic continuation of the box, the valid
+    nodes step exactly as the periodic box does."""
+    shape = (7, 6, 5)
+    G = 4
+    c = o.consts39(mach=mach, shifted=shifted)
+    f0, g0, lm0 = o.init39(shape, *_fields(shape, mach), c)
+    nx, ny, nz = shape
+    ext = (nx + 2 * G, ny + 2 * G, nz + 2 * G)
+
+    def pad(a, q):
+        a = a.reshape(q, nz, ny, nx)
+        return np.pad(a, ((0, 0), (G, G), (G, G), (G, G)), mode="wrap").reshape(q, -1)
+
+    fp, gp, _, _, stp = o.step_periodic39(shape, f0, g0, lm0, c, 40, 1)
+    fg, gg, _, _, stg = o.step_ghost39(ext, G, pad(f0, 39), pad(g0, 39), pad(lm0, 13), c, 40, 1)
+    inner = (slice(None), slice(G, G + nz), slice(G, G + ny), slice(G, G + nx))
+    assert np.array_equal(fg.reshape(39, *ext[::-1])[inner].reshape(39, -1), fp)
+    assert np.array_equal(gg.reshape(39, *ext[::-1])[inner].reshape(39, -1), gp)
+    assert stg["nodes"] == stp["nodes"] == nx * ny * nz
+    # the layer is left as it was
+    gh = o.ghost_nodes39(ext, G)
+    assert np.array_equal(fg[:, gh], pad(f0, 39)[:, gh])

@@ -228,12 +228,13 @@ class Solver:
 
     def define_level(self, lvl: int, shape: Sequence[int], periodic=(1, 1, 1), lo=(0, 0, 0)):
         """shape: valid nodes; a refined level (lvl >= 1, periodic (0,0,0)) is
-        stored with a one-node ghost shell, so its arrays have shape + 2."""
+        stored with a ghost layer (one node on D3Q27, four on D3Q39), so its
+        arrays have shape + 2 (D3Q27) or shape + 8 (D3Q39)."""
         lo3 = (ctypes.c_int32 * 3)(*lo)
         hi3 = (ctypes.c_int32 * 3)(*[lo[d] + shape[d] - 1 for d in range(3)])
         per = (ctypes.c_int32 * 3)(*[int(p) for p in periodic])
         self._check(self.lib.uclbs_define_level(self.ctx, lvl, lo3, hi3, per))
-        g = 2 if lvl > 0 else 0
+        g = (8 if self.q == 39 else 2) if lvl > 0 else 0
         self.shapes[lvl] = tuple(int(s) + g for s in shape)
 
     def define_level_boxes(self, lvl: int, boxes):

@@ -223,7 +223,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
     void* __restrict__ gB_, double* __restrict__ lmv, double* __restrict__ alpha, Box b,
     Gas39 gs, Ref39 rf, long long step, unsigned long long* __restrict__ stats,
     unsigned* __restrict__ neg_count, unsigned* __restrict__ neg_list, Faces39 fc, int has_faces,
-    const unsigned* __restrict__ near_list, unsigned n_near) {
+    const unsigned* __restrict__ near_list, unsigned n_near, int ghost) {
   using S = typename std::conditional<kFp32, float, double>::type;
   const S* __restrict__ fA = static_cast<const S*>(fA_);
   const S* __restrict__ gA = static_cast<const S*>(gA_);
@@ -251,11 +251,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
   } else {
     node = idx;
     active = node < n;
-    if (active && has_faces) {
+    if (active && (has_faces || ghost > 0)) {
       const int x = (int)(node % (unsigned)b.nx);
       const int y = (int)((node / (unsigned)b.nx) % (unsigned)b.ny);
       const int z = (int)(node / ((unsigned)b.nx * (unsigned)b.ny));
-      active = !near_face39(x, y, z, fc.code, b.nx, b.ny, b.nz, kShifted);
+      if (has_faces) active = !near_face39(x, y, z, fc.code, b.nx, b.ny, b.nz, kShifted);
+      // a refined level steps its valid nodes; its ghost layer (deeper than
+      // the longest link) is filled from the coarser level
+      if (ghost > 0)
+        active = active && x >= ghost && y >= ghost && z >= ghost && x < b.nx - ghost &&
+                 y < b.ny - ghost && z < b.nz - ghost;
     }
   }
   int iters = 0, ast = 0;
@@ -403,7 +408,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
 }
 
 // node (x, y, z) wrapped over the periodic axes of b; -1 across another face
-__device__ __forceinline__ long long damp_nb(int x, int y, int z, const Box& b) {
+// or inside a refined level's ghost layer (which holds no post-collision values)
+__device__ __forceinline__ long long damp_nb(int x, int y, int z, const Box& b, int ghost) {
   int s[3] = {x, y, z};
   const int nb[3] = {b.nx, b.ny, b.nz};
 #pragma unroll
@@ -412,6 +418,7 @@ __device__ __forceinline__ long long damp_nb(int x, int y, int z, const Box& b) 
       if (!b.per[d]) return -1;
       s[d] = wrapm(s[d], nb[d]);
     }
+    if (ghost > 0 && (s[d] < ghost || s[d] >= nb[d] - ghost)) return -1;
   }
   return (long long)s[0] + (long long)nb[0] * ((long long)s[1] + (long long)nb[1] * s[2]);
 }
@@ -425,7 +432,8 @@ __global__ __launch_bounds__(256) void k39_damp_compute(const void* __restrict__
                                                         Ref39 rf, const unsigned* __restrict__ count,
                                                         const unsigned* __restrict__ list,
                                                         double* __restrict__ rep,
-                                                        unsigned long long* __restrict__ stats) {
+                                                        unsigned long long* __restrict__ stats,
+                                                        int ghost) {
   using S = typename std::conditional<kFp32, float, double>::type;
   const S* __restrict__ f = static_cast<const S*>(f_);
   const unsigned cnt = *count;
@@ -443,8 +451,8 @@ __global__ __launch_bounds__(256) void k39_damp_compute(const void* __restrict__
         const int cx = cx39<kShifted>(i), cy = CY39[i], cz = CZ39[i];
         // a neighbour across a non-periodic face counts as not positive: the
         // one-sided form of damp_it.f95:109-123
-        const long long kp = damp_nb(x + cx, y + cy, z + cz, b);
-        const long long km = damp_nb(x - cx, y - cy, z - cz, b);
+        const long long kp = damp_nb(x + cx, y + cy, z + cz, b, ghost);
+        const long long km = damp_nb(x - cx, y - cy, z - cz, b, ghost);
         const double fp = kp >= 0 ? (double)f[(size_t)i * n + (size_t)kp] + rf.f[i] : -1.0;
         const double fm = km >= 0 ? (double)f[(size_t)i * n + (size_t)km] + rf.f[i] : -1.0;
         if (fp > 0.0 && fm > 0.0) r = (fp + fm) / 2.0;
@@ -513,6 +521,145 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
   }
 }
 
+// Ghost nodes of a refined D3Q39 level from the coarser level
+// (interlevel_interpolation.f95:114-493 through amrex_fillcoarsepatch with
+// amrex_interp_node_bilinear): per axis the coincident coarse node or the mean
+// of the two around (ratio 2, nodal), in time (1 - theta) * previous + theta *
+// current, for f and g in their storage form (the reference state is the same
+// on every level) and for rho and T of the coarse nodes.  A population with
+// f or g negative or NaN becomes the equilibrium of the interpolated coarse
+// state, f = rho exp(lambda . m_i), g = (2 cv - 3) T f (:417-431); the coarse
+// level keeps no previous multipliers, so lambda is interpolated in space from
+// its current ones.  The multipliers are stored too (warm start of a node that
+// later joins the valid region).
+template <bool kFp32, bool kShifted>
+__global__ __launch_bounds__(64) void k39_coarse_to_fine(
+    const void* __restrict__ cpf_, const void* __restrict__ cnf_, const void* __restrict__ cpg_,
+    const void* __restrict__ cng_, const double* __restrict__ clm, LevelMap cm,
+    void* __restrict__ ff_, void* __restrict__ fg_, double* __restrict__ flm, LevelMap fm,
+    const unsigned* __restrict__ ghosts, unsigned nghost, double theta, Ref39 rf, Gas39 gs,
+    unsigned long long* __restrict__ stats) {
+  using S = typename std::conditional<kFp32, float, double>::type;
+  const S* __restrict__ cpf = static_cast<const S*>(cpf_);
+  const S* __restrict__ cnf = static_cast<const S*>(cnf_);
+  const S* __restrict__ cpg = static_cast<const S*>(cpg_);
+  const S* __restrict__ cng = static_cast<const S*>(cng_);
+  S* __restrict__ ff = static_cast<S*>(ff_);
+  S* __restrict__ fg = static_cast<S*>(fg_);
+  const unsigned j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nghost) return;
+  const unsigned node = ghosts ? ghosts[j] : j;
+  const unsigned fnx = (unsigned)fm.box.nx, fny = (unsigned)fm.box.ny;
+  const int loc[3] = {(int)(node % fnx), (int)((node / fnx) % fny), (int)(node / (fnx * fny))};
+  const int cn[3] = {cm.box.nx, cm.box.ny, cm.box.nz};
+  int ci[3][2];
+  double wt[3][2];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int G = fm.origin[d] + loc[d];
+    const int odd = G & 1;
+    int c0 = (G >> 1) - cm.origin[d];
+    int c1 = c0 + odd;
+    if (cm.per[d]) {
+      c0 = (c0 % cn[d] + cn[d]) % cn[d];
+      c1 = (c1 % cn[d] + cn[d]) % cn[d];
+    }
+    ci[d][0] = c0;
+    ci[d][1] = c1;
+    wt[d][0] = odd ? 0.5 : 1.0;
+    wt[d][1] = odd ? 0.5 : 0.0;
+  }
+  const unsigned cnn = (unsigned)cm.box.n, fnn = (unsigned)fm.box.n;
+  double f[Q39], g[Q39], lm[kM39];
+#pragma unroll
+  for (int i = 0; i < Q39; ++i) f[i] = g[i] = 0.0;
+#pragma unroll
+  for (int k = 0; k < kM39; ++k) lm[k] = 0.0;
+  double rho_i = 0.0, T_i = 0.0;
+  for (int cz = 0; cz < 2; ++cz)
+    for (int cy = 0; cy < 2; ++cy)
+      for (int cx = 0; cx < 2; ++cx) {
+        const double w = wt[0][cx] * wt[1][cy] * wt[2][cz];
+        if (w == 0.0) continue;
+        const unsigned k = (unsigned)ci[0][cx] +
+                           (unsigned)cm.box.nx * ((unsigned)ci[1][cy] + (unsigned)cm.box.ny * (unsigned)ci[2][cz]);
+        // rho and T of the coarse node at both times (calculate_flow_variables.f95:201-228)
+        double r[2] = {0.0, 0.0}, mx[2] = {0.0, 0.0}, my[2] = {0.0, 0.0}, mz[2] = {0.0, 0.0},
+               en[2] = {0.0, 0.0};
+#pragma unroll
+        for (int i = 0; i < Q39; ++i) {
+          const double ap = (double)cpf[(size_t)i * cnn + k] + rf.f[i];
+          const double an = (double)cnf[(size_t)i * cnn + k] + rf.f[i];
+          const double bp = (double)cpg[(size_t)i * cnn + k] + rf.g[i];
+          const double bn = (double)cng[(size_t)i * cnn + k] + rf.g[i];
+          f[i] += w * ((1.0 - theta) * (double)cpf[(size_t)i * cnn + k] + theta * (double)cnf[(size_t)i * cnn + k]);
+          g[i] += w * ((1.0 - theta) * (double)cpg[(size_t)i * cnn + k] + theta * (double)cng[(size_t)i * cnn + k]);
+          const int ccx = cx39<kShifted>(i), ccy = CY39[i], ccz = CZ39[i];
+          const double cm2 = (double)(ccx * ccx + ccy * ccy + ccz * ccz);
+          r[0] += ap; r[1] += an;
+          mx[0] += ccx * ap; mx[1] += ccx * an;
+          my[0] += ccy * ap; my[1] += ccy * an;
+          mz[0] += ccz * ap; mz[1] += ccz * an;
+          en[0] += cm2 * ap + bp; en[1] += cm2 * an + bn;
+        }
+        double Tn[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          const double ux = mx[t] / r[t], uy = my[t] / r[t], uz = mz[t] / r[t];
+          Tn[t] = (en[t] - r[t] * (ux * ux + uy * uy + uz * uz)) / (2 * gs.cv * r[t]);
+        }
+        rho_i += w * ((1.0 - theta) * r[0] + theta * r[1]);
+        T_i += w * ((1.0 - theta) * Tn[0] + theta * Tn[1]);
+#pragma unroll
+        for (int q = 0; q < kM39; ++q) lm[q] += w * clm[(size_t)q * cnn + k];
+      }
+  unsigned long long nrep = 0;
+#pragma unroll
+  for (int i = 0; i < Q39; ++i) {
+    const double fv = f[i] + rf.f[i], gv = g[i] + rf.g[i];
+    if (!(fv >= 0.0) || !(gv >= 0.0)) {
+      const double e = expo39<kShifted>(lm, rho_i, i);
+      f[i] = e - rf.f[i];
+      g[i] = gs.lead_g * T_i * e - rf.g[i];
+      ++nrep;
+    }
+    ff[(size_t)i * fnn + node] = (S)f[i];
+    fg[(size_t)i * fnn + node] = (S)g[i];
+  }
+#pragma unroll
+  for (int q = 0; q < kM39; ++q) flm[(size_t)q * fnn + node] = lm[q];
+  if (nrep) atomicAdd(stats + (size_t)(blockIdx.x & (kStatSlots - 1)) * kStatFields + ST_REPAIR, nrep);
+}
+
+// fine_to_coarse.f95:25-65 (amrex_average_down_nodes at ratio 2: the
+// coincident fine node) for f, g and the multipliers of the covered coarse
+// nodes; both levels hold the same storage form
+template <typename S>
+__global__ __launch_bounds__(256) void k39_fine_to_coarse(
+    const S* __restrict__ ff, const S* __restrict__ fg, const double* __restrict__ flm, LevelMap fm,
+    S* __restrict__ cf, S* __restrict__ cg, double* __restrict__ clm, LevelMap cm, Range3 cover) {
+  const int ex = cover.hi[0] - cover.lo[0] + 1, ey = cover.hi[1] - cover.lo[1] + 1;
+  const int ez = cover.hi[2] - cover.lo[2] + 1;
+  const unsigned j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ex <= 0 || ey <= 0 || ez <= 0 || j >= (unsigned)(ex * ey * ez)) return;
+  const int C[3] = {cover.lo[0] + (int)(j % (unsigned)ex), cover.lo[1] + (int)((j / (unsigned)ex) % (unsigned)ey),
+                    cover.lo[2] + (int)(j / ((unsigned)ex * (unsigned)ey))};
+  int cl[3], fl[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    cl[d] = C[d] - cm.origin[d];
+    fl[d] = 2 * C[d] - fm.origin[d];
+  }
+  const unsigned ck = (unsigned)cl[0] + (unsigned)cm.box.nx * ((unsigned)cl[1] + (unsigned)cm.box.ny * (unsigned)cl[2]);
+  const unsigned fk = (unsigned)fl[0] + (unsigned)fm.box.nx * ((unsigned)fl[1] + (unsigned)fm.box.ny * (unsigned)fl[2]);
+  const size_t cnn = (size_t)cm.box.n, fnn = (size_t)fm.box.n;
+  for (int i = 0; i < Q39; ++i) {
+    cf[i * cnn + ck] = ff[i * fnn + fk];
+    cg[i * cnn + ck] = fg[i * fnn + fk];
+  }
+  for (int q = 0; q < kM39; ++q) clm[q * cnn + ck] = flm[q * fnn + fk];
+}
+
 // host layout (FP64 full values) <-> device storage
 template <bool kFp32>
 __global__ void k39_from_fp64(const double* __restrict__ src, void* __restrict__ dst_,
@@ -544,7 +691,8 @@ hipError_t launch_d3q39_step(const Box& b, const Step39& a, hipStream_t s) {
 #define UCLBS_K39(F, SH, TU, KA, NR, NB)                                                         \
   hipLaunchKernelGGL((k39_step<F, SH, TU, KA, NR>), dim3(NB), dim3(64), 0, s, a.fin, a.gin,     \
                      a.fout, a.gout, a.lm, a.alpha, b, a.gas, a.ref, (long long)a.step, a.stats, \
-                     a.neg_count, a.neg_list, a.faces, a.has_faces, a.near_list, a.n_near)
+                     a.neg_count, a.neg_list, a.faces, a.has_faces, a.near_list, a.n_near,   \
+                     a.ghost)
 #define UCLBS_K39_FC(F, SH, TU, KA)                                              \
   do {                                                                           \
     UCLBS_K39(F, SH, TU, KA, false, blocks);                                     \
@@ -575,19 +723,19 @@ hipError_t launch_d3q39_step(const Box& b, const Step39& a, hipStream_t s) {
   if (a.fp32) {
     if (a.shifted)
       hipLaunchKernelGGL((k39_damp_compute<true, true>), rg, rb, 0, s, a.fout, b, a.ref,
-                         a.neg_count, a.neg_list, a.rep, a.stats);
+                         a.neg_count, a.neg_list, a.rep, a.stats, a.ghost);
     else
       hipLaunchKernelGGL((k39_damp_compute<true, false>), rg, rb, 0, s, a.fout, b, a.ref,
-                         a.neg_count, a.neg_list, a.rep, a.stats);
+                         a.neg_count, a.neg_list, a.rep, a.stats, a.ghost);
     hipLaunchKernelGGL((k39_damp_apply<true>), rg, rb, 0, s, a.fout, b, a.ref, a.neg_count,
                        a.neg_list, a.rep);
   } else {
     if (a.shifted)
       hipLaunchKernelGGL((k39_damp_compute<false, true>), rg, rb, 0, s, a.fout, b, a.ref,
-                         a.neg_count, a.neg_list, a.rep, a.stats);
+                         a.neg_count, a.neg_list, a.rep, a.stats, a.ghost);
     else
       hipLaunchKernelGGL((k39_damp_compute<false, false>), rg, rb, 0, s, a.fout, b, a.ref,
-                         a.neg_count, a.neg_list, a.rep, a.stats);
+                         a.neg_count, a.neg_list, a.rep, a.stats, a.ghost);
     hipLaunchKernelGGL((k39_damp_apply<false>), rg, rb, 0, s, a.fout, b, a.ref, a.neg_count,
                        a.neg_list, a.rep);
   }
@@ -631,6 +779,44 @@ hipError_t launch_d3q39_to_fp64(const Box& b, const void* src, double* dst, int 
   const unsigned blocks = (unsigned)std::min<size_t>((tot + 255) / 256, 65536);
   if (fp32) hipLaunchKernelGGL((k39_to_fp64<true>), dim3(blocks), dim3(256), 0, s, src, dst, ref, (size_t)b.n);
   else hipLaunchKernelGGL((k39_to_fp64<false>), dim3(blocks), dim3(256), 0, s, src, dst, ref, (size_t)b.n);
+  return hipGetLastError();
+}
+
+hipError_t launch_coarse_to_fine39(const void* cpf, const void* cnf, const void* cpg,
+                                   const void* cng, const double* clm, const LevelMap& cm, void* ff,
+                                   void* fg, double* flm, const LevelMap& fm,
+                                   const unsigned* ghosts, unsigned nghost, double theta, int fp32,
+                                   int shifted, const Ref39& rf, const Gas39& gs,
+                                   unsigned long long* stats, hipStream_t s) {
+  if (nghost == 0) return hipSuccess;
+  const dim3 g((nghost + 63) / 64), b(64);
+#define UCLBS_C2F39(F, SH)                                                                      \
+  hipLaunchKernelGGL((k39_coarse_to_fine<F, SH>), g, b, 0, s, cpf, cnf, cpg, cng, clm, cm, ff, fg, \
+                     flm, fm, ghosts, nghost, theta, rf, gs, stats)
+  if (fp32) {
+    if (shifted) UCLBS_C2F39(true, true); else UCLBS_C2F39(true, false);
+  } else {
+    if (shifted) UCLBS_C2F39(false, true); else UCLBS_C2F39(false, false);
+  }
+#undef UCLBS_C2F39
+  return hipGetLastError();
+}
+
+hipError_t launch_fine_to_coarse39(const void* ff, const void* fg, const double* flm,
+                                   const LevelMap& fm, void* cf, void* cg, double* clm,
+                                   const LevelMap& cm, const Range3& cover, int fp32, hipStream_t s) {
+  const long long cnt = (long long)(cover.hi[0] - cover.lo[0] + 1) * (cover.hi[1] - cover.lo[1] + 1) *
+                        (cover.hi[2] - cover.lo[2] + 1);
+  if (cnt <= 0) return hipSuccess;
+  const dim3 g((unsigned)((cnt + 255) / 256)), b(256);
+  if (fp32)
+    hipLaunchKernelGGL((k39_fine_to_coarse<float>), g, b, 0, s, static_cast<const float*>(ff),
+                       static_cast<const float*>(fg), flm, fm, static_cast<float*>(cf),
+                       static_cast<float*>(cg), clm, cm, cover);
+  else
+    hipLaunchKernelGGL((k39_fine_to_coarse<double>), g, b, 0, s, static_cast<const double*>(ff),
+                       static_cast<const double*>(fg), flm, fm, static_cast<double*>(cf),
+                       static_cast<double*>(cg), clm, cm, cover);
   return hipGetLastError();
 }
 

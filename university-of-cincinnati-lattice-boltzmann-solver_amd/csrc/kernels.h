@@ -255,8 +255,18 @@ struct Step39 {
   Faces39 faces;
   const unsigned* near_list;  // near-face nodes, stepped by a second launch
   unsigned n_near;
+  int ghost;            // depth of a refined level's ghost layer (not stepped), else 0
 };
 hipError_t launch_d3q39_step(const Box& b, const Step39& a, hipStream_t s);
+hipError_t launch_coarse_to_fine39(const void* cpf, const void* cnf, const void* cpg,
+                                   const void* cng, const double* clm, const LevelMap& cm, void* ff,
+                                   void* fg, double* flm, const LevelMap& fm,
+                                   const unsigned* ghosts, unsigned nghost, double theta, int fp32,
+                                   int shifted, const Ref39& rf, const Gas39& gs,
+                                   unsigned long long* stats, hipStream_t s);
+hipError_t launch_fine_to_coarse39(const void* ff, const void* fg, const double* flm,
+                                   const LevelMap& fm, void* cf, void* cg, double* clm,
+                                   const LevelMap& cm, const Range3& cover, int fp32, hipStream_t s);
 hipError_t launch_d3q39_init(const Box& b, const double* fields, void* f, void* g, double* lm,
                              int fp32, int shifted, const Gas39& gs, const Ref39& rf,
                              unsigned long long* bad, hipStream_t s);

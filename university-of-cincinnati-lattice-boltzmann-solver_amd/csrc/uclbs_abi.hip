@@ -64,6 +64,8 @@ struct Level {
   unsigned* neg_list = nullptr;
   unsigned* near_list = nullptr;  // D3Q39 near-face nodes (rebuilt when the faces change)
   bool table39 = false;           // a D3Q39 link table is resident (not stepped)
+  int ghost_depth = 0;            // ghost layer of a refined level (1 on D3Q27, 4 on D3Q39)
+  uclbs::Gas39 gas39{};           // D3Q39 constants of this level (dt, epsilon, beta scaling)
   unsigned n_near = 0;
   bool near_dirty = true;
   double* rep = nullptr;
@@ -328,7 +330,8 @@ int launch_steps39(Ctx* c, Level* L, int nsteps, hipEvent_t* ev) {
     return set_err(UCLBS_E_UNSUPPORTED, "D3Q39 levels with a link table are not stepped: the "
                                         "D3Q39 wall-link rule is not part of this build");
   uclbs::Step39 a{};
-  a.gas = c->gas39;
+  a.gas = L->gas39;
+  a.ghost = L->refined ? L->ghost_depth : 0;
   a.ref = c->ref39;
   a.fp32 = c->cfg.precision == UCLBS_PREC_FP32_SHIFTED;
   a.shifted = c->shifted39;
@@ -341,7 +344,7 @@ int launch_steps39(Ctx* c, Level* L, int nsteps, hipEvent_t* ev) {
   a.has_faces = 0;
   for (int f = 0; f < 6; ++f) {
     a.faces.code[f] = L->face_code[f];
-    a.has_faces |= L->face_code[f] != uclbs::kFacePeriodic;
+    a.has_faces |= !L->refined && L->face_code[f] != uclbs::kFacePeriodic;
   }
   for (int i = 0; i < uclbs::Q39; ++i) {
     a.faces.f[i] = c->fs39.f[i];
@@ -559,12 +562,13 @@ int uclbs_define_level(void* ctx, int lvl, const int32_t lo[3], const int32_t hi
   Level& L = c->levels[lvl];
   free_level(L);
   c->structure++;
+  // ghost layer of a refined level: one node on D3Q27, four on D3Q39 (links
+  // reach three nodes, four along x on the shifted lattice)
+  const int G = lvl > 0 ? (c->cfg.lattice == UCLBS_LATTICE_D3Q39 ? 4 : 1) : 0;
   if (lvl > 0) {
     // a refined level: nested in level lvl-1 with ratio 2, not periodic, and
     // the coarse nodes its ghost shell interpolates from inside the coarser
     // level's valid nodes (no wrap across a periodic coarse face)
-    if (c->cfg.lattice != UCLBS_LATTICE_D3Q27)
-      return set_err(UCLBS_E_UNSUPPORTED, "refined levels are D3Q27 in this build");
     if (lvl - 1 >= (int)c->levels.size() || !c->levels[lvl - 1].defined)
       return set_err(UCLBS_E_STATE, "define level %d before level %d", lvl - 1, lvl);
     const Level& P = c->levels[lvl - 1];
@@ -576,7 +580,7 @@ int uclbs_define_level(void* ctx, int lvl, const int32_t lo[3], const int32_t hi
     for (int d = 0; d < 3; ++d) {
       if (periodic[d]) return set_err(UCLBS_E_ARG, "a refined level is not periodic");
       const int cv_lo = P.lo[d], cv_hi = P.hi[d];
-      const int g_lo = (lo[d] - 1) >> 1, g_hi = (hi[d] + 2) >> 1;
+      const int g_lo = (lo[d] - G) >> 1, g_hi = (hi[d] + G + 1) >> 1;
       if (g_lo < cv_lo || g_hi > cv_hi)
         return set_err(UCLBS_E_ARG, "level %d box [%d,%d] on axis %d needs coarse nodes "
                                     "[%d,%d] outside level %d's [%d,%d]",
@@ -585,9 +589,9 @@ int uclbs_define_level(void* ctx, int lvl, const int32_t lo[3], const int32_t hi
   }
   int n[3];
   for (int d = 0; d < 3; ++d) {
-    n[d] = hi[d] - lo[d] + 1 + (lvl > 0 ? 2 : 0);
+    n[d] = hi[d] - lo[d] + 1 + 2 * G;
     if (n[d] < 3) return set_err(UCLBS_E_ARG, "box extent %d on axis %d is below 3", n[d], d);
-    L.origin[d] = lo[d] - (lvl > 0 ? 1 : 0);
+    L.origin[d] = lo[d] - G;
     L.lo[d] = lo[d];
     L.hi[d] = hi[d];
     L.box.per[d] = periodic[d] != 0;
@@ -628,15 +632,23 @@ int uclbs_define_level(void* ctx, int lvl, const int32_t lo[3], const int32_t hi
   L.step = 0;
   L.cur = 0;
   L.defined = true;
+  L.ghost_depth = G;
+  if (c->cfg.lattice == UCLBS_LATTICE_D3Q39) {
+    // timing_calcs.f95:24-26 on level lvl: dt and epsilon halve per level
+    L.gas39 = c->gas39;
+    L.gas39.lvl = lvl;
+    L.gas39.dt = c->gas39.dt / std::ldexp(1.0, lvl);
+    L.gas39.eps = L.gas39.dt / 1.0;
+  }
   if (lvl > 0) {
-    // flags (ghost shell / interior) and the ghost list
+    // flags (ghost layer / interior) and the ghost list
     L.refined = true;
     std::vector<uint8_t> hf((size_t)L.box.n, uclbs::kFlagInterior);
     std::vector<uint32_t> gl;
     for (int cz = 0; cz < n[2]; ++cz)
       for (int cy = 0; cy < n[1]; ++cy)
         for (int cx = 0; cx < n[0]; ++cx)
-          if (cx == 0 || cy == 0 || cz == 0 || cx == n[0] - 1 || cy == n[1] - 1 || cz == n[2] - 1) {
+          if (cx < G || cy < G || cz < G || cx >= n[0] - G || cy >= n[1] - G || cz >= n[2] - G) {
             const size_t k = (size_t)cx + (size_t)n[0] * ((size_t)cy + (size_t)n[1] * cz);
             hf[k] = uclbs::kFlagGhost;
             gl.push_back((uint32_t)k);
@@ -1654,6 +1666,21 @@ static int interlevel_fill(Ctx* c, int lvl, double theta, bool all_nodes) {
   if ((rc = set_device(c))) return rc;
   const Level& C = c->levels[lvl - 1];
   const int sh = c->cfg.precision == UCLBS_PREC_FP32_SHIFTED;
+  if (c->cfg.lattice == UCLBS_LATTICE_D3Q39) {
+    const uclbs::LevelMap cm = level_map(C), fm = level_map(*F);
+    HIP_TRY(uclbs::launch_coarse_to_fine39(
+        C.f[1 - C.cur], C.f[C.cur], C.g[1 - C.cur], C.g[C.cur], C.lm, cm, F->f[F->cur],
+        F->g[F->cur], F->lm, fm, all_nodes ? nullptr : F->ghosts,
+        all_nodes ? (unsigned)F->box.n : F->nghost, theta, sh, c->shifted39, c->ref39, C.gas39,
+        c->d_stats, c->stream));
+    if (all_nodes) {
+      HIP_TRY(hipMemcpyAsync(F->f[1 - F->cur], F->f[F->cur], level_bytes(c, F),
+                             hipMemcpyDeviceToDevice, c->stream));
+      HIP_TRY(hipMemcpyAsync(F->g[1 - F->cur], F->g[F->cur], level_bytes(c, F),
+                             hipMemcpyDeviceToDevice, c->stream));
+    }
+    return UCLBS_OK;
+  }
   if (F->multibox) {
     if (!all_nodes)
       return hip_rc(uclbs::launch_c2f_stencils27(C.f[1 - C.cur], C.f[C.cur], (unsigned)C.box.n,
@@ -1734,6 +1761,11 @@ int uclbs_fine_to_coarse(void* ctx, int lvl) {
     cover.lo[d] = (F->lo[d] + 1) >> 1;  // coarse nodes whose fine image 2C is valid
     cover.hi[d] = F->hi[d] >> 1;
   }
+  if (c->cfg.lattice == UCLBS_LATTICE_D3Q39)
+    return hip_rc(uclbs::launch_fine_to_coarse39(F->f[F->cur], F->g[F->cur], F->lm, level_map(*F),
+                                                 C.f[C.cur], C.g[C.cur], C.lm, level_map(C), cover,
+                                                 c->cfg.precision == UCLBS_PREC_FP32_SHIFTED,
+                                                 c->stream));
   HIP_TRY(uclbs::launch_fine_to_coarse27(F->f[F->cur], level_map(*F), C.f[C.cur], level_map(C), cover,
                                          c->cfg.precision == UCLBS_PREC_FP32_SHIFTED, c->stream));
   return UCLBS_OK;
