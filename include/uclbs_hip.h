@@ -208,13 +208,20 @@ int uclbs_init_fields(void *ctx, int lvl, const double *rho, const double *u, co
 int uclbs_upload_state(void *ctx, int lvl, const double *f, const double *g, const double *lm,
                        int64_t n_nodes);
 int uclbs_download_state(void *ctx, int lvl, double *f, double *g, double *lm, int64_t n_nodes);
-/* ---- refined levels (D3Q27) ------------------------------------------------
+/* ---- refined levels ---------------------------------------------------------
  * uclbs_define_level with lvl >= 1 defines a box nested in level lvl-1 at
  * refinement ratio 2 (lo/hi in level lvl's own node indices; node G of level
  * lvl sits at coordinate G/2 of level lvl-1), stored with a one-node ghost
  * shell: its arrays (upload/download) hold (nx+2)(ny+2)(nz+2) nodes.  Lattice
  * viscosity doubles per level (acoustic scaling).  The ghost shell is filled
- * from the coarser level before each of the level's steps: */
+ * from the coarser level before each of the level's steps.
+ * On a D3Q39 context the ghost layer is four nodes deep (links reach three
+ * nodes, four along x on the shifted lattice; arrays (nx+8)(ny+8)(nz+8)): the
+ * fill interpolates f, g and the coarse nodes' rho and T in space and time and
+ * the multipliers in space, and replaces a negative f or g by the equilibrium
+ * of the interpolated coarse state (interlevel_interpolation.f95:417-431);
+ * uclbs_fine_to_coarse injects f, g and the multipliers; each level runs with
+ * dt and epsilon halved per level (timing_calcs.f95:24-26). */
 /* theta in [0, 1]: time between level lvl-1's previous and current states
  * (interlevel_interpolation.f95:114-493: amrex_fillcoarsepatch with
  * amrex_interp_node_bilinear; negative populations -> coarse equilibrium) */
