@@ -253,7 +253,9 @@ int uclbs_define_level_boxes(void *ctx, int lvl, int nbox, const int32_t *lo, co
  * level at theta 0, its substep, ghosts at theta 1/2, its substep, injection.
  * use_graph != 0: the launches of two coarse steps are captured once as a
  * hipGraph and replayed (re-captured when the levels, kernel variants or
- * buffer parities change; an odd remainder runs eagerly). */
+ * buffer parities change; an odd remainder runs eagerly).  On a D3Q39
+ * context the start-up ramps read the step counter, so steps run eagerly
+ * until every level is past its start-up. */
 int uclbs_advance(void *ctx, int ncoarse, int use_graph);
 
 /* Picks the tile (block) size of the batched streaming kernel of a level of

@@ -136,3 +136,40 @@ def test_two_level_d3q39_freestream_fixed_point(pkg):
     f1f, _, _ = s.download_state(1)
     assert np.abs(f1 - f0).max() < 1e-6 and np.abs(g1 - g0).max() < 1e-6
     assert np.abs(_valid(f1f, 39) - _valid(f1i, 39)).max() < 1e-6
+
+
+def test_two_level_d3q39_native_advance_and_graph(pkg, orc):
+    """uclbs_advance runs the schedule of Solver.advance in the library: the
+    same fields bit for bit, eager and as hipGraph replays (which start once
+    both levels are past the start-up ramps), replay after replay with host
+    reads in between."""
+    mach, shifted = 1.5, True
+    c0 = orc.consts39(mach=mach, shifted=shifted, lvl=0)
+    f0, g0, lm0 = orc.init39(CSHAPE, *fields(CSHAPE, mach), c0)
+
+    def make():
+        s = pkg.Solver(lattice="d3q39", precision="fp32s")
+        s.set_gas(mach=mach, shifted=int(shifted))
+        s.define_level(0, CSHAPE)
+        s.upload_state(0, f0, g0, lm0)
+        s.define_level(1, FVALID, periodic=(0, 0, 0), lo=FLO)
+        s.init_from_coarse(1)
+        s.set_timestep(0, 10)
+        s.set_timestep(1, 20)
+        return s
+    py, eager, graph = make(), make(), make()
+    py.advance(16)             # crosses the 25-step start-up of level 0
+    eager.advance_native(16, graph=False)
+    graph.advance_native(16, graph=True)
+    for lvl in (0, 1):
+        a = py.download_state(lvl)
+        for other in (eager, graph):
+            b = other.download_state(lvl)
+            assert all(np.array_equal(x, y) for x, y in zip(a, b)), lvl
+    for rep in range(3):
+        eager.advance_native(2, graph=False)
+        graph.advance_native(2, graph=True)   # past the start-up: one replay
+        for lvl in (0, 1):
+            a, b = eager.download_state(lvl), graph.download_state(lvl)
+            assert all(np.array_equal(x, y) for x, y in zip(a, b)), (rep, lvl)
+    assert eager.stats()["node_updates"] == graph.stats()["node_updates"]

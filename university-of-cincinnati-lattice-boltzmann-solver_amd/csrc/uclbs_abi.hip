@@ -325,6 +325,28 @@ int check_face_tables(const Ctx* c, const Level* L) {
   return UCLBS_OK;
 }
 
+// nodes of a D3Q39 box with faces whose pulls may cross a face: stepped by
+// their own launch; the list is rebuilt when the faces change
+static int build_near39(Ctx* c, Level* L) {
+  if (!L->near_dirty) return UCLBS_OK;
+  std::vector<unsigned> near;
+  const uclbs::Box& bx = L->box;
+  for (int z = 0; z < bx.nz; ++z)
+    for (int y = 0; y < bx.ny; ++y)
+      for (int x = 0; x < bx.nx; ++x)
+        if (uclbs::near_face39(x, y, z, L->face_code, bx.nx, bx.ny, bx.nz, c->shifted39 != 0))
+          near.push_back((unsigned)(x + (int64_t)bx.nx * (y + (int64_t)bx.ny * z)));
+  if (L->near_list) { HIP_TRY(hipFree(L->near_list)); L->near_list = nullptr; }
+  L->n_near = (unsigned)near.size();
+  if (!near.empty()) {
+    HIP_TRY(hipMalloc(&L->near_list, near.size() * sizeof(unsigned)));
+    HIP_TRY(hipMemcpy(L->near_list, near.data(), near.size() * sizeof(unsigned),
+                      hipMemcpyHostToDevice));
+  }
+  L->near_dirty = false;
+  return UCLBS_OK;
+}
+
 int launch_steps39(Ctx* c, Level* L, int nsteps, hipEvent_t* ev) {
   if (L->table39)
     return set_err(UCLBS_E_UNSUPPORTED, "D3Q39 levels with a link table are not stepped: the "
@@ -350,23 +372,9 @@ int launch_steps39(Ctx* c, Level* L, int nsteps, hipEvent_t* ev) {
     a.faces.f[i] = c->fs39.f[i];
     a.faces.g[i] = c->fs39.g[i];
   }
-  if (a.has_faces && L->near_dirty) {
-    // nodes whose pulls may cross a face: stepped by their own launch
-    std::vector<unsigned> near;
-    const uclbs::Box& bx = L->box;
-    for (int z = 0; z < bx.nz; ++z)
-      for (int y = 0; y < bx.ny; ++y)
-        for (int x = 0; x < bx.nx; ++x)
-          if (uclbs::near_face39(x, y, z, L->face_code, bx.nx, bx.ny, bx.nz, c->shifted39 != 0))
-            near.push_back((unsigned)(x + (int64_t)bx.nx * (y + (int64_t)bx.ny * z)));
-    if (L->near_list) { HIP_TRY(hipFree(L->near_list)); L->near_list = nullptr; }
-    L->n_near = (unsigned)near.size();
-    if (!near.empty()) {
-      HIP_TRY(hipMalloc(&L->near_list, near.size() * sizeof(unsigned)));
-      HIP_TRY(hipMemcpy(L->near_list, near.data(), near.size() * sizeof(unsigned),
-                        hipMemcpyHostToDevice));
-    }
-    L->near_dirty = false;
+  if (a.has_faces) {
+    int rc = build_near39(c, L);
+    if (rc) return rc;
   }
   a.near_list = L->near_list;
   a.n_near = L->n_near;
@@ -1818,13 +1826,27 @@ int uclbs_advance(void* ctx, int ncoarse, int use_graph) {
   if (!c) return set_err(UCLBS_E_ARG, "null context");
   if (ncoarse < 0) return set_err(UCLBS_E_ARG, "ncoarse < 0");
   if (c->levels.empty() || !c->levels[0].defined) return set_err(UCLBS_E_STATE, "level 0 undefined");
-  if (c->cfg.lattice != UCLBS_LATTICE_D3Q27)
-    return set_err(UCLBS_E_UNSUPPORTED, "uclbs_advance drives D3Q27 levels");
   int rc = set_device(c);
   if (rc) return rc;
   const int nl = levels_in_use(c);
-  const int pairs = use_graph ? ncoarse / 2 : 0;
+  const bool q39 = c->cfg.lattice == UCLBS_LATTICE_D3Q39;
+  // the D3Q39 start-up ramps read the step counter, which a replayed graph
+  // would repeat: graphs only once every level is past the start-up
+  bool ramp = false;
+  if (q39)
+    for (int l = 0; l < nl; ++l) ramp = ramp || c->levels[l].step < c->levels[l].gas39.startup;
+  const int pairs = (use_graph && !ramp) ? ncoarse / 2 : 0;
   if (pairs > 0) {
+    // host-side tables are built before the capture
+    if (q39)
+      for (int l = 0; l < nl; ++l) {
+        Level& L = c->levels[l];
+        bool faces = false;
+        for (int f = 0; f < 6; ++f) faces = faces || (!L.refined && L.face_code[f] != uclbs::kFacePeriodic);
+        if (faces && (rc = build_near39(c, &L))) return rc;
+        if (L.table39)
+          return set_err(UCLBS_E_UNSUPPORTED, "D3Q39 levels with a link table are not stepped");
+      }
     const std::vector<int> sig = graph_signature(c);
     if (!c->graph || c->graph_sig != sig) {
       if (c->graph) (void)hipGraphExecDestroy(c->graph);
@@ -1832,12 +1854,15 @@ int uclbs_advance(void* ctx, int ncoarse, int use_graph) {
       // capture two coarse steps: every level's buffer and queue parities
       // return to where they were, so the graph replays as it stands
       const int64_t done0 = c->steps_done;
+      std::vector<int64_t> steps0;
+      for (const Level& L : c->levels) steps0.push_back(L.step);
       HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       rc = advance_rec(c, 0);
       if (!rc) rc = advance_rec(c, 0);
       hipGraph_t g = nullptr;
       const hipError_t e = hipStreamEndCapture(c->stream, &g);
-      c->steps_done = done0;
+      c->steps_done = done0;  // the captured work has not run yet
+      for (size_t l = 0; l < steps0.size(); ++l) c->levels[l].step = steps0[l];
       if (rc) {
         if (g) (void)hipGraphDestroy(g);
         return rc;
@@ -1856,8 +1881,10 @@ int uclbs_advance(void* ctx, int ncoarse, int use_graph) {
       c->graph_sig = sig;
     }
     for (int k = 0; k < pairs; ++k) HIP_TRY(hipGraphLaunch(c->graph, c->stream));
-    for (int l = 0; l < nl; ++l)
+    for (int l = 0; l < nl; ++l) {
       c->steps_done += (int64_t)pairs * (2ll << l);  // level l steps 2^l times per coarse step
+      c->levels[l].step += (int64_t)pairs * (2ll << l);
+    }
   }
   for (int k = 2 * pairs; k < ncoarse; ++k)
     if ((rc = advance_rec(c, 0))) return rc;
