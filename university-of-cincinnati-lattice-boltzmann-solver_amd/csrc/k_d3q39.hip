@@ -152,7 +152,7 @@ __device__ __forceinline__ double alpha_estimate39(const S (*sf)[64], const doub
   const double mft = mfp / sqrt(3 * kb * dim_temp / gs.molec_weight);
   const double beta_mfp = gs.dt / (2 * mft + gs.dt);
   double cs1 = 0.0, cs2 = 0.0, cs3 = 0.0;
-#pragma unroll 1
+#pragma unroll 3
   for (int i = 0; i < Q39; ++i) {
     const double f = (double)sf[i][t] + rf.f[i];
     const double d = sd[i][t] - f;
@@ -171,7 +171,7 @@ __device__ __forceinline__ double alpha_estimate39(const S (*sf)[64], const doub
   if (asm_ > alpha_hi) asm_ = alpha_hi - 1.0e-9;
   if (step < gs.startup) asm_ = (asm_ - alpha_start) / (double)gs.startup * (double)step;
   double ft1 = 0, ft2 = 0, ft3 = 0, ft4 = 0, st1 = 0, st2 = 0, third = 0;
-#pragma unroll 1
+#pragma unroll 3
   for (int i = 0; i < Q39; ++i) {
     const double f = (double)sf[i][t] + rf.f[i];
     const double df = f < 0.0 ? fabs(f) : f;
