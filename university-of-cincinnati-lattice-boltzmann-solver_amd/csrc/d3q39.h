@@ -121,7 +121,9 @@ __host__ __device__ inline int lm_newton39(double rho, const double tg[kM39], do
     double r[kM39];
 #pragma unroll
     for (int k = 0; k < kM39; ++k) r[k] = J[jidx(k, 0)] - tg[k];
-    // L D L^T in place: J[jidx(k,k)] <- D_k, J[jidx(k,l)] <- L_kl (l < k)
+    // L D L^T in place: J[jidx(k,k)] <- D_k, J[jidx(k,l)] <- L_kl (l < k);
+    // one division per pivot, the rest multiply by its reciprocal
+    double rD[kM39];
 #pragma unroll
     for (int k = 0; k < kM39; ++k) {
 #pragma unroll
@@ -129,12 +131,13 @@ __host__ __device__ inline int lm_newton39(double rho, const double tg[kM39], do
         double s = J[jidx(k, l)];
 #pragma unroll
         for (int m = 0; m < l; ++m) s -= J[jidx(k, m)] * J[jidx(l, m)] * J[jidx(m, m)];
-        J[jidx(k, l)] = s / J[jidx(l, l)];
+        J[jidx(k, l)] = s * rD[l];
       }
       double d = J[jidx(k, k)];
 #pragma unroll
       for (int m = 0; m < k; ++m) d -= J[jidx(k, m)] * J[jidx(k, m)] * J[jidx(m, m)];
       J[jidx(k, k)] = d;
+      rD[k] = 1.0 / d;
     }
     // solve L y = r, D z = y, L^T delta = z
 #pragma unroll
@@ -143,7 +146,7 @@ __host__ __device__ inline int lm_newton39(double rho, const double tg[kM39], do
       for (int m = 0; m < k; ++m) r[k] -= J[jidx(k, m)] * r[m];
     }
 #pragma unroll
-    for (int k = 0; k < kM39; ++k) r[k] /= J[jidx(k, k)];
+    for (int k = 0; k < kM39; ++k) r[k] *= rD[k];
 #pragma unroll
     for (int k = kM39 - 1; k >= 0; --k) {
 #pragma unroll
