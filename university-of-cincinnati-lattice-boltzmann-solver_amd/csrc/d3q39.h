@@ -83,6 +83,43 @@ struct Mono39Table {
 __constant__ constexpr Mono39Table kMonoT{};
 #define kMono39 kMonoT.v
 
+// Jacobian entries that hold the same moment: J_kl = sum_i e_i m_k(c_i) m_l(c_i)
+// depends only on the weights m_k m_l over the 39 directions, and only 60 of
+// the 91 (k, l) pairs have distinct weight vectors (x*x = 1*x^2, x^2*y^2 =
+// xy*xy, ...).  rep[p] is the first pair of class p, cls[jidx(k, l)] its class.
+struct JDup39 {
+  int n[2];
+  signed char rk[2][91], rl[2][91], cls[2][91];
+  constexpr JDup39() : n(), rk(), rl(), cls() {
+    for (int sh = 0; sh < 2; ++sh) {
+      int cnt = 0;
+      for (int k = 0; k < 13; ++k)
+        for (int l = 0; l <= k; ++l) {
+          int found = -1;
+          for (int p = 0; p < cnt && found < 0; ++p) {
+            bool same = true;
+            for (int i = 0; i < 39 && same; ++i) {
+              const int a = sh ? mono39<true>(i, k) * mono39<true>(i, l)
+                               : mono39<false>(i, k) * mono39<false>(i, l);
+              const int b = sh ? mono39<true>(i, rk[sh][p]) * mono39<true>(i, rl[sh][p])
+                               : mono39<false>(i, rk[sh][p]) * mono39<false>(i, rl[sh][p]);
+              same = a == b;
+            }
+            if (same) found = p;
+          }
+          if (found < 0) {
+            rk[sh][cnt] = (signed char)k;
+            rl[sh][cnt] = (signed char)l;
+            found = cnt++;
+          }
+          cls[sh][k * (k + 1) / 2 + l] = (signed char)found;
+        }
+      n[sh] = cnt;
+    }
+  }
+};
+constexpr JDup39 kJDup39{};
+
 struct NoSink39 {
   __host__ __device__ void operator()(int, double) const {}
 };
@@ -99,9 +136,10 @@ __host__ __device__ inline int lm_newton39(double rho, const double tg[kM39], do
   while (mx > 1.0e-7) {
     if (it >= maxit) return -1;
     ++it;
-    double J[kJ39];
+    constexpr int nd = kJDup39.n[kShifted ? 1 : 0];
+    double Jd[nd];
 #pragma unroll
-    for (int p = 0; p < kJ39; ++p) J[p] = 0.0;
+    for (int p = 0; p < nd; ++p) Jd[p] = 0.0;
 #pragma unroll 3
     for (int i = 0; i < Q39; ++i) {
       double ex = lm[0];
@@ -113,10 +151,14 @@ __host__ __device__ inline int lm_newton39(double rho, const double tg[kM39], do
 #pragma unroll
       for (int k = 0; k < kM39; ++k) me[k] = (double)kMono39[kShifted][i][k] * e;
 #pragma unroll
-      for (int k = 0; k < kM39; ++k)
-#pragma unroll
-        for (int l = 0; l <= k; ++l) J[jidx(k, l)] = fma((double)kMono39[kShifted][i][l], me[k], J[jidx(k, l)]);
+      for (int p = 0; p < nd; ++p) {
+        constexpr int sh = kShifted ? 1 : 0;
+        Jd[p] = fma((double)kMono39[kShifted][i][kJDup39.rl[sh][p]], me[kJDup39.rk[sh][p]], Jd[p]);
+      }
     }
+    double J[kJ39];
+#pragma unroll
+    for (int q = 0; q < kJ39; ++q) J[q] = Jd[kJDup39.cls[kShifted ? 1 : 0][q]];
     // residual: current moments (row 0 of J) minus targets
     double r[kM39];
 #pragma unroll
