@@ -12,10 +12,10 @@ the boundary-node kernel, populations FP32 stored as deviations from the
 lattice weights, already resident in HBM.  `--workload tgv` runs the periodic
 Taylor-Green box instead, `--workload c3` the D3Q39 two-population
 compressible step on a 384 x 192 x 96 box at Mach 1.5 (shifted lattice,
-freestream inflow, pure outflow, inviscid sidewalls; the wedge is not part of
-this build), reported against its algorithmic 728 B per node update.
+freestream inflow, pure outflow, inviscid sidewalls; the wedge geometry is not
+included), reported against its algorithmic 728 B per node update.
 
-Prints ONE JSON line (the driver's contract).  `value` is MLUPS over the timed
+Prints ONE JSON line.  `value` is MLUPS over the timed
 region (wall clock, synchronised on both sides); `roofline` uses the
 algorithmic 216 B per node update (27 populations x 4 B, read + write) and the
 kernel's average duration from HIP events recorded on the library's stream.
@@ -330,7 +330,7 @@ def main():
                           "copy_GBps": round(pkg.copy_bandwidth(nb, reps=5), 1)}))
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+        print(json.dumps({"error": "this benchmark runs on one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding is a design note only)"}))
         sys.exit(2)
 

@@ -150,7 +150,7 @@ int uclbs_set_bc_states(void *ctx, int n_in, const double *in_states, int n_out,
  * the long links derived from the shorter ones as state_directions.f95:95-185
  * does (integer divisions and its slot slip for the (0,3,0) link included);
  * an outflow face gives its links the outside node's code 0.  Such a level is
- * not stepped (no D3Q39 wall-link rule in this build); the shifted lattice's
+ * not stepped (no D3Q39 wall-link rule in this library); the shifted lattice's
  * table (shifted_state_directions.f95) is refused. */
 int uclbs_build_links(void *ctx, int lvl, const double origin[3], double dx, int64_t ntri,
                       const double *tri, const double *nrm, int64_t *n_solid, int64_t *n_tie);

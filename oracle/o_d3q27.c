@@ -3,7 +3,7 @@
  * The reference has no D3Q27 collision (general/background_calcs/
  * directions_and_weights.f95:226-237 selects dir=27 only for mach <= 1e-4 and
  * looping/timestepping/collision.f95:211-222 handles only 19 and 39), so this
- * file states the published method the build uses in its place, reusing the
+ * file states the published method the GPU library uses in its place, reusing the
  * reference's building blocks:
  *   - equilibrium: product-form entropic equilibrium (Ansumali, Karlin,
  *     Ottinger, Europhys. Lett. 63 (2003) 798), exact mass and momentum;

@@ -8,7 +8,7 @@
  *
  * Every routine restates an algorithm of the UCLBS reference (Fortran,
  * v0.08) or, where the reference has no implementation (D3Q27), the
- * published method the build uses in its place.  Each definition below
+ * published method the GPU library uses in its place.  Each definition below
  * names the reference file:line it follows.  All arithmetic is FP64, as in
  * the reference (general/modules/precise.f95:12).
  *

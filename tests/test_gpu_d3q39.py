@@ -3,7 +3,7 @@ oracle on the same inputs.
   FP64 storage: f, g relative L2 <= 1e-11, multipliers max abs <= 1e-8,
                 alpha relative L2 <= 1e-11 and max abs <= 1e-9 (the
                 closed-form estimate amplifies last-bit differences of the
-                multipliers at nodes near equilibrium; the build factorises the Jacobian as
+                multipliers at nodes near equilibrium; the kernel factorises the Jacobian as
                 L D L^T, the oracle as LU: same Newton steps up to round-off)
   FP32 storage: rho, u, T relative L2 <= 1e-6 against the FP64 oracle
 The startup ramps (25 steps) are crossed in every parity run."""

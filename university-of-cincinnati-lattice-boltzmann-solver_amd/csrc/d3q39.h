@@ -16,7 +16,7 @@
 // matrix J = sum_i f_eq,i m_i m_i^T; it is symmetric positive definite and is
 // factorised here as L D L^T in packed form, in place (the reference's
 // lu_decomp.f95 performs the same elimination unsymmetrically, with two index
-// slips at :105 and :441 that this build does not reproduce).
+// slips at :105 and :441 that this library does not reproduce).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>

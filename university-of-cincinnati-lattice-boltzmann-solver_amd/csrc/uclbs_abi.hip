@@ -350,7 +350,7 @@ static int build_near39(Ctx* c, Level* L) {
 int launch_steps39(Ctx* c, Level* L, int nsteps, hipEvent_t* ev) {
   if (L->table39)
     return set_err(UCLBS_E_UNSUPPORTED, "D3Q39 levels with a link table are not stepped: the "
-                                        "D3Q39 wall-link rule is not part of this build");
+                                        "D3Q39 wall-link rule is not part of this library");
   uclbs::Step39 a{};
   a.gas = L->gas39;
   a.ghost = L->refined ? L->ghost_depth : 0;
@@ -811,7 +811,7 @@ int uclbs_define_level_boxes(void* ctx, int lvl, int nbox, const int32_t* lo, co
   if (lvl < 0 || lvl > 31) return set_err(UCLBS_E_ARG, "level %d out of range", lvl);
   if (nbox < 1) return set_err(UCLBS_E_ARG, "no boxes");
   if (c->cfg.lattice != UCLBS_LATTICE_D3Q27)
-    return set_err(UCLBS_E_UNSUPPORTED, "levels of many boxes are D3Q27 in this build");
+    return set_err(UCLBS_E_UNSUPPORTED, "levels of many boxes are D3Q27 in this library");
   int rc = set_device(c);
   if (rc) return rc;
   if (lvl > 0 && (lvl - 1 >= (int)c->levels.size() || !c->levels[lvl - 1].defined))
@@ -1225,7 +1225,7 @@ int uclbs_build_links(void* ctx, int lvl, const double origin[3], double dx, int
   const bool q39 = c->cfg.lattice == UCLBS_LATTICE_D3Q39;
   if (q39 && c->shifted39)
     return set_err(UCLBS_E_UNSUPPORTED, "build_links: D3Q39 tables of the plain lattice only "
-                                        "(shifted_state_directions is not part of this build)");
+                                        "(shifted_state_directions is not part of this library)");
   if (!origin || !(dx > 0.0) || ntri < 0 || ntri > (1 << 26) || (ntri && (!tri || !nrm)))
     return set_err(UCLBS_E_ARG, "build_links: bad geometry arguments");
   if ((rc = set_device(c))) return rc;
