@@ -185,8 +185,11 @@ __device__ __forceinline__ double alpha_estimate39(const S (*sf)[64], const doub
       ft4 += f6;
     }
     st1 += f2;
-    if (x1 >= 0.0)
-      st2 += f3 * (2.0 / (4 + asm_ * x1) + 1.0 / (4 + 2 * asm_ * x1) + 2.0 / (4 + 3 * asm_ * x1));
+    if (x1 >= 0.0) {
+      // 2/D1 + 1/D2 + 2/D3 over one division
+      const double D1 = 4 + asm_ * x1, D2 = 4 + 2 * asm_ * x1, D3 = 4 + 3 * asm_ * x1;
+      st2 += f3 * ((2.0 * D2 * D3 + D1 * D3 + 2.0 * D1 * D2) / (D1 * D2 * D3));
+    }
     third = third - (60 * f2 + 60 * f3 + 11 * f4) / (60 + 90 * x1 + 36 * x2 + 3 * x3);
   }
   const double bm2 = beta_mfp * beta_mfp, b3 = bm2 * beta_mfp;
