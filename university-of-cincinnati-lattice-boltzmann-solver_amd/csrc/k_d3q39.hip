@@ -157,10 +157,13 @@ __device__ __forceinline__ double alpha_estimate39(const S (*sf)[64], const doub
     const double f = (double)sf[i][t] + rf.f[i];
     const double d = sd[i][t] - f;
     const double df = f < 0.0 ? fabs(f) : f;
-    const double x = d / df;
-    cs1 = cs1 + df * (2.0 * (x * x) / (2 + x));
-    cs2 = cs2 + df * (x * x);
-    if (x <= 0.0 && x >= -1.0) cs3 = cs3 + df * (x * x * x);
+    // x = d / df, df 2x^2/(2+x) and df x^2 over one division: e = df (2 + x)
+    // is positive (the equilibrium is), r = 1 / (df e)
+    const double e = 2.0 * df + d, r = 1.0 / (df * e), dd = d * d;
+    const double x = d * e * r, q2 = dd * e * r;
+    cs1 = cs1 + 2.0 * dd * df * r;
+    cs2 = cs2 + q2;
+    if (x <= 0.0 && x >= -1.0) cs3 = cs3 + q2 * x;
   }
   *st = 0;
   if (fabs(cs3) < 1.24e-16) {
@@ -185,12 +188,14 @@ __device__ __forceinline__ double alpha_estimate39(const S (*sf)[64], const doub
       ft4 += f6;
     }
     st1 += f2;
-    if (x1 >= 0.0) {
-      // 2/D1 + 1/D2 + 2/D3 over one division
-      const double D1 = 4 + asm_ * x1, D2 = 4 + 2 * asm_ * x1, D3 = 4 + 3 * asm_ * x1;
-      st2 += f3 * ((2.0 * D2 * D3 + D1 * D3 + 2.0 * D1 * D2) / (D1 * D2 * D3));
-    }
-    third = third - (60 * f2 + 60 * f3 + 11 * f4) / (60 + 90 * x1 + 36 * x2 + 3 * x3);
+    // 2/D1 + 1/D2 + 2/D3 (x1 >= 0 only) and the third-order quotient share
+    // one division: r = 1 / (B E), B = D1 D2 D3 or 1
+    const double D1 = 4 + asm_ * x1, D2 = 4 + 2 * asm_ * x1, D3 = 4 + 3 * asm_ * x1;
+    const bool pos = x1 >= 0.0;
+    const double B = pos ? D1 * D2 * D3 : 1.0, E = 60 + 90 * x1 + 36 * x2 + 3 * x3;
+    const double r = 1.0 / (B * E);
+    if (pos) st2 += f3 * ((2.0 * D2 * D3 + D1 * D3 + 2.0 * D1 * D2) * E * r);
+    third = third - (60 * f2 + 60 * f3 + 11 * f4) * B * r;
   }
   const double bm2 = beta_mfp * beta_mfp, b3 = bm2 * beta_mfp;
   const double a2h = alpha_hi * alpha_hi, a3h = a2h * alpha_hi;
