@@ -84,6 +84,48 @@ def test_d3q39_fp64_parity(pkg, orc, shifted, mach):
 
 
 @pytest.mark.parametrize("shifted,mach", CASES)
+def test_d3q39_fp64_parity_far_from_equilibrium(pkg, orc, shifted, mach):
+    """A state 50% off equilibrium with 1% of the f populations negative,
+    stepped past the start-up ramps: the alpha estimate's low clamp
+    (calculate_alpha.f95:118-289) and its |f| form for negative populations,
+    and the damp_it repair of hundreds of nodes, against the oracle."""
+    shape = (12, 10, 8)
+    steps, step0 = 3, 30
+    c = orc.consts39(mach=mach, shifted=shifted)
+    f0, g0, lm0 = orc.init39(shape, *fields(shape, mach), c)
+    rng = np.random.default_rng(20261202)
+    f0 = f0 * (1 + 0.5 * rng.uniform(-1, 1, f0.shape))
+    g0 = g0 * (1 + 0.5 * rng.uniform(-1, 1, g0.shape))
+    neg = rng.random(f0.shape) < 0.01
+    f0[neg] *= -0.1
+    f_o, g_o, lm_o, a_o, st_o = orc.step_periodic39(shape, f0, g0, lm0, c, step0, steps,
+                                                    keep_alpha=True)
+    assert st_o["n_alpha_lo"] > 0 and st_o["n_repair"] > 100  # the branches are reached
+    s = solver(pkg, shape, "fp64", mach, shifted)
+    s.upload_state(0, f0, g0, lm0)
+    s.set_timestep(0, step0)
+    s.step(0, steps)
+    f_g, g_g, lm_g = s.download_state(0)
+    st = s.stats()
+    a_g = s.alpha(0)
+    e_f, e_g = rel_l2(f_g, f_o), rel_l2(g_g, g_o)
+    e_lm = float(np.abs(lm_g - lm_o).max())
+    e_a = float(np.abs(a_g - a_o).max())
+    e_al2 = float(np.linalg.norm(a_g - a_o) / np.linalg.norm(a_o))
+    print(f"D3Q39 fp64 far shifted={shifted}: relL2 f {e_f:.2e} g {e_g:.2e} max|lm| {e_lm:.2e} "
+          f"max|alpha| {e_a:.2e} relL2 alpha {e_al2:.2e} bound {st['n_bound']} vs "
+          f"{st_o['n_alpha_lo'] + st_o['n_alpha_hi']} repairs {st['n_repair']} vs {st_o['n_repair']}")
+    assert e_f < 1e-11 and e_g < 1e-11 and e_lm < 1e-8 and e_al2 < 1e-11 and e_a < 1e-9
+    assert st["node_updates"] == st_o["nodes"] and st["n_noconv"] == 0
+    assert st["n_repair"] == st_o["n_repair"]
+    assert st["n_trivial"] == st_o["n_alpha2"]
+    assert st["n_bound"] == st_o["n_alpha_lo"] + st_o["n_alpha_hi"]
+    # the D3Q39 step counts an estimate with no real root as n_nonpos and a
+    # conductivity outside its fit as n_fallback
+    assert st["n_nonpos"] == st_o["n_nan"] and st["n_fallback"] == st_o["n_kappa_err"]
+
+
+@pytest.mark.parametrize("shifted,mach", CASES)
 def test_d3q39_fp32_against_fp64_oracle(pkg, orc, shifted, mach):
     shape = (12, 10, 8)
     steps = 30
