@@ -166,19 +166,22 @@ __host__ __device__ inline int lm_newton39(double rho, const double tg[kM39], do
     for (int k = 0; k < kM39; ++k) r[k] = J[jidx(k, 0)] - tg[k];
     // L D L^T in place: J[jidx(k,k)] <- D_k, J[jidx(k,l)] <- L_kl (l < k);
     // one division per pivot, the rest multiply by its reciprocal
+    // v[m] = L_km D_m of the current row: one FMA per update term
     double rD[kM39];
 #pragma unroll
     for (int k = 0; k < kM39; ++k) {
+      double v[kM39];
 #pragma unroll
       for (int l = 0; l < k; ++l) {
         double s = J[jidx(k, l)];
 #pragma unroll
-        for (int m = 0; m < l; ++m) s -= J[jidx(k, m)] * J[jidx(l, m)] * J[jidx(m, m)];
+        for (int m = 0; m < l; ++m) s = fma(-v[m], J[jidx(l, m)], s);
+        v[l] = s;
         J[jidx(k, l)] = s * rD[l];
       }
       double d = J[jidx(k, k)];
 #pragma unroll
-      for (int m = 0; m < k; ++m) d -= J[jidx(k, m)] * J[jidx(k, m)] * J[jidx(m, m)];
+      for (int m = 0; m < k; ++m) d = fma(-v[m], J[jidx(k, m)], d);
       J[jidx(k, k)] = d;
       rD[k] = 1.0 / d;
     }
