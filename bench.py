@@ -152,7 +152,9 @@ def learned_alpha(pkg, s, L, nu, collect_steps=3, train_steps=4000, batch=8192, 
     up, ul = np.stack(mp[1:]), np.stack(ml[1:])
     rel = float(np.linalg.norm(ul - up) / np.linalg.norm(up))
     ip, il = out["plain"]["newton_iters_per_node"], out["learned"]["newton_iters_per_node"]
-    return {"plain_iters_per_node": round(ip, 4), "learned_iters_per_node": round(il, 4),
+    finite = all(math.isfinite(v) for v in (l0, l1, ip, il, rel))
+    return {"status": "ok" if finite else "failed",
+            "plain_iters_per_node": round(ip, 4), "learned_iters_per_node": round(il, 4),
             "iters_ratio": round(ip / il, 3), "learned_fallback_rate": out["learned"]["fallback_rate"],
             "u_rel_l2_learned_vs_plain": rel, "steps": steps, "train_pairs": npairs,
             "train_steps": train_steps, "batch": batch, "loss_first": l0, "loss_last": l1,
@@ -406,7 +408,10 @@ def main():
                         "n_ibb_links": st["n_ibb_links"], "n_repair": st["n_repair"]},
     }
     if args.workload == "c2" and not args.no_learned:
-        out["learned_alpha"] = learned_alpha(pkg, s, L, nu)
+        try:
+            out["learned_alpha"] = learned_alpha(pkg, s, L, nu)
+        except pkg.UclbsError as exc:  # e.g. a non-finite training loss
+            out["learned_alpha"] = {"status": "failed", "error": str(exc)}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_budget)
     print(json.dumps(out), flush=True)

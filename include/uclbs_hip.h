@@ -254,7 +254,7 @@ int uclbs_define_level_boxes(void *ctx, int lvl, int nbox, const int32_t *lo, co
  * use_graph != 0: the launches of two coarse steps are captured once as a
  * hipGraph and replayed (re-captured when the levels, kernel variants or
  * buffer parities change; an odd remainder runs eagerly).  On a D3Q39
- * context the start-up ramps read the step counter, so steps run eagerly
+ * context the start-up ramps read the finest level's step counter, so steps run eagerly
  * until every level is past its start-up. */
 int uclbs_advance(void *ctx, int ncoarse, int use_graph);
 
@@ -264,8 +264,11 @@ int uclbs_advance(void *ctx, int ncoarse, int use_graph);
  * reference. */
 int uclbs_autotune(void *ctx, int lvl, int *tile_nodes);
 
-/* The level's step counter (reference timestep(lvl)), which drives the
- * start-up ramps of beta, epsilon and alpha; advanced by every step. */
+/* The level's step counter (reference timestep(lvl)), advanced by every step
+ * of the level.  On a D3Q39 context the start-up ramps of beta, epsilon and
+ * alpha of every level read the finest level's counter, the reference's
+ * timestep(amr_max_lvl) (calculate_alpha.f95:185-187, collision.f95:277-278,
+ * find_q_bar.f95:36, find_little_q_bar.f95:26). */
 int uclbs_set_timestep(void *ctx, int lvl, int64_t step);
 
 /* ---- learned alpha (D3Q27, FP32-shifted storage) --------------------------

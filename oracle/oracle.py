@@ -154,6 +154,12 @@ REF_PATH = os.path.join(_DIR, "_ref", "libuclbs_ref.so")
 _ref = None
 
 
+def ref_available():
+    """Whether the compiled reference routines exist, without loading them
+    (test collection, also under `-m gpu`, must not map the library)."""
+    return os.path.exists(REF_PATH)
+
+
 def ref_lib():
     """The reference's own FP64 routines compiled from its sources (built by
     oracle/ref/Makefile where the reference tree is present), or None."""

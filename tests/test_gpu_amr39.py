@@ -43,6 +43,10 @@ FORIGIN = tuple(FLO[d] - G for d in range(3))
 
 
 def oracle_run(orc, c0, c1, f0, g0, lm0, ncoarse, step0=100):
+    """The library's two-level schedule on the oracle.  Every step's start-up
+    ramps read the finest level's counter at that moment (the reference's
+    timestep(amr_max_lvl)): the coarse step sees the fine counter before the
+    fine level's two substeps."""
     sys.path.insert(0, os.path.dirname(orc.__file__))
     import o_amr
     corg = (0, 0, 0)
@@ -52,7 +56,7 @@ def oracle_run(orc, c0, c1, f0, g0, lm0, ncoarse, step0=100):
     cs, fs = step0, step0
     for _ in range(ncoarse):
         fp, gp = fc, gc
-        fc, gc, lmc, _, _ = orc.step_periodic39(CSHAPE, fc, gc, lmc, c0, cs, 1)
+        fc, gc, lmc, _, _ = orc.step_periodic39(CSHAPE, fc, gc, lmc, c0, fs, 1)
         cs += 1
         for theta in (0.0, 0.5):
             fi, gi, li, _ = orc.coarse_to_fine39(fp, fc, gp, gc, lmc, CSHAPE, corg, FSTORED, FORIGIN,
@@ -102,6 +106,27 @@ def test_two_level_d3q39_fp64_parity(pkg, orc, shifted, mach):
     assert max(e) < 1e-11 and elm < 1e-8
     st = s.stats()
     assert st["n_noconv"] == 0 and st["n_nonpos"] == 0
+
+
+@pytest.mark.parametrize("shifted,mach", [(False, 0.3), (True, 1.5)])
+def test_two_level_d3q39_fp64_parity_in_startup(pkg, orc, shifted, mach):
+    """Both levels inside the 25-step start-up window (fine counter 2 .. 7):
+    the coarse level's beta, epsilon and alpha ramps follow the fine counter."""
+    c0 = orc.consts39(mach=mach, shifted=shifted, lvl=0)
+    c1 = orc.consts39(mach=mach, shifted=shifted, lvl=1)
+    f0, g0, lm0 = orc.init39(CSHAPE, *fields(CSHAPE, mach), c0)
+    fc, gc, lmc, ff, gf, lmf = oracle_run(orc, c0, c1, f0, g0, lm0, 3, step0=2)
+    s = gpu_run(pkg, "fp64", mach, shifted, f0, g0, lm0, 3, step0=2)
+    fcg, gcg, lmcg = s.download_state(0)
+    ffg, gfg, lmfg = s.download_state(1)
+    e = [rel_l2(fcg, fc), rel_l2(gcg, gc), rel_l2(_valid(ffg, 39), _valid(ff, 39)),
+         rel_l2(_valid(gfg, 39), _valid(gf, 39))]
+    print("start-up window: relL2 coarse f %.2e g %.2e fine f %.2e g %.2e" % tuple(e))
+    assert max(e) < 1e-11
+    # the coarse level's own counter would give a different coarse state
+    c_own, *_ = orc.step_periodic39(CSHAPE, f0, g0, lm0, c0, 2, 1)
+    c_fine, *_ = orc.step_periodic39(CSHAPE, f0, g0, lm0, c0, 6, 1)
+    assert rel_l2(c_own, c_fine) > 1e-9
 
 
 def test_two_level_d3q39_fp32(pkg, orc):

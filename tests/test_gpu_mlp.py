@@ -135,3 +135,47 @@ def test_logged_features_match_oracle(pkg):
     err = np.abs(X.astype(np.float64) - phi).max(0)
     print("logged features vs oracle, max abs per feature:", " ".join(f"{e:.1e}" for e in err))
     assert np.all(err <= 1e-4)
+
+
+def test_learned_alpha_at_bench_schedule_duct(pkg):
+    """Regression: pairs logged in the impulsive start of the walled duct (the
+    benchmark's 5 + 20 step schedule, scaled to 96^3) are finite, include
+    near-equilibrium nodes (m down to 1e-8, where the ratio features once
+    overflowed FP32), train to a finite, falling loss, and the trained start
+    needs fewer Newton updates than the plain start."""
+    import importlib.util
+    import os
+    spec = importlib.util.spec_from_file_location(
+        "bench", os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    L = 96
+    s, _, _, _ = bench.make_c2(pkg, L, 0.002, "newton")
+    s.step(0, 25)
+    s.mlp_reserve(3 * L ** 3)
+    for _ in range(3):
+        n = s.mlp_collect(0, 1)
+    X, y = s.mlp_samples(n)
+    print(f"duct {L}^3 after 25 steps: {n} pairs, |phi| max {np.abs(X).max():.3f}, "
+          f"min m 10^{4 * (X[:, 0].min() - 1):.2f}")
+    assert n > L ** 3
+    assert np.isfinite(X).all() and np.isfinite(y).all()
+    assert X[:, 0].min() < 1 - 1.5  # some m below 10^-6 (log10(m) / 4 + 1 < -0.5)
+    assert np.abs(X[:, 1:5]).max() <= 1.0 + 1e-5  # |S_n / S_2| / m^(n-2) <= 1
+    s.mlp_init(20260105)
+    l0, l1 = s.mlp_train(800, 4096, 2e-3, 20260105)
+    print(f"training loss {l0:.3e} -> {l1:.3e}")
+    assert math.isfinite(l0) and math.isfinite(l1) and l1 < 0.5 * l0
+    params = s.mlp_get_params()
+    f_state = s.download_f(0)
+    sl, _, _, _ = bench.make_c2(pkg, L, 0.002, "learned")
+    sl.mlp_set_params(*params)
+    sl.upload_f(0, f_state)
+    its = {}
+    for name, solver in (("plain", s), ("learned", sl)):
+        solver.reset_stats()
+        solver.step(0, 10)
+        st = solver.stats()
+        its[name] = st["newton_iters"] / st["node_updates"]
+    print(f"Newton updates/node: plain {its['plain']:.3f}, learned {its['learned']:.3f}")
+    assert its["learned"] < its["plain"]

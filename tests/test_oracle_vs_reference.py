@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 pytestmark = pytest.mark.skipif(
-    __import__("oracle").ref_lib() is None, reason="reference routines not built (no reference tree)")
+    not __import__("oracle").ref_available(), reason="reference routines not built (no reference tree)")
 
 
 def _p(a):

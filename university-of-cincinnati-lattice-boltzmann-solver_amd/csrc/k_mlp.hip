@@ -44,49 +44,58 @@ __device__ __forceinline__ float gelu_grad(float x) {
 // FP32 (they feed a BF16 network; the moments and equilibrium behind D are the
 // streaming kernel's FP64-summed ones).  Returns m = max|x| (0 for a node at
 // equilibrium or with f <= 0).
+//
+// The ratio features are formed over the scaled deviations xh = x / m, which
+// lie in [-1, 1]: with R_k = sum_i D_i xh_i^k,
+//   (S_n / S_2) / m^(n-2) = R_(n-1) / R_1,   S_2 = m R_1,
+// so no intermediate leaves the range of its result.  (Forming
+// S_n * (1 / S_2) * (1 / m)^(n-2) instead overflows FP32 near equilibrium:
+// at m = 1e-7, S_2 = 1e-14 the factor (1 / S_2) m^-4 is 1e42.)
 __device__ __forceinline__ float features27(const float (&f)[27], float (&phi)[kIn]) {
-  float D[27];
+  float D[27], x[27];
   double rho, u[3];
   prep27_shifted(f, D, rho, u);
   float m = 0.0f, xmin = 0.0f;
   bool nonpos = false;
-  float S[7] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int i = 0; i < 27; ++i) {
     const float fi = (float)W27[i] + f[i];
     nonpos |= !(fi > 0.0f);
-    const float x = D[i] * __builtin_amdgcn_rcpf(fi);
-    m = fmaxf(m, fabsf(x));
-    xmin = fminf(xmin, x);
-    float t = D[i] * x;
-    S[2] += t;
+    x[i] = D[i] * __builtin_amdgcn_rcpf(fi);
+    m = fmaxf(m, fabsf(x[i]));
+    xmin = fminf(xmin, x[i]);
+  }
+  const bool ok = !nonpos && m > 1e-30f && m < 1e30f;
+  const float mi = ok ? __builtin_amdgcn_rcpf(m) : 0.0f;
+  float R[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int n = 3; n <= 6; ++n) {
-      t *= x;
-      S[n] += t;
+  for (int i = 0; i < 27; ++i) {
+    const float xh = x[i] * mi;
+    float t = D[i];
+#pragma unroll
+    for (int k = 1; k <= 5; ++k) {
+      t *= xh;
+      R[k] += t;
     }
   }
-  if (nonpos || !(m > 1e-30f) || !(S[2] > 0.0f)) {
+  if (!ok || !(R[1] > 0.0f)) {
 #pragma unroll
     for (int j = 0; j < kIn; ++j) phi[j] = 0.0f;
     return 0.0f;
   }
   constexpr float kLog10_2 = 0.30102999566398120f;
-  const float mi = __builtin_amdgcn_rcpf(m), r2 = __builtin_amdgcn_rcpf(S[2]);
+  const float r1 = __builtin_amdgcn_rcpf(R[1]);
   phi[0] = __builtin_amdgcn_logf(m) * (kLog10_2 * 0.25f) + 1.0f;  // log10(m) / 4 + 1
-  float q = r2;
 #pragma unroll
-  for (int n = 3; n <= 6; ++n) {
-    q *= mi;
-    phi[n - 2] = S[n] * q;  // (S_n / S_2) / m^(n-2)
-  }
+  for (int n = 3; n <= 6; ++n) phi[n - 2] = R[n - 1] * r1;  // (S_n / S_2) / m^(n-2)
   phi[5] = xmin * mi;
-  phi[6] = S[2] * mi * mi * __builtin_amdgcn_rcpf((float)rho);
+  phi[6] = R[1] * mi * __builtin_amdgcn_rcpf((float)rho);  // S_2 / (rho m^2)
+  const float S2 = m * R[1];
   phi[7] = (float)(5.0 * u[0]);
   phi[8] = (float)(5.0 * u[1]);
   phi[9] = (float)(5.0 * u[2]);
   phi[10] = (float)(25.0 * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
-  phi[11] = __builtin_amdgcn_logf(S[2]) * (kLog10_2 * 0.125f) + 1.0f;  // log10(S2) / 8 + 1
+  phi[11] = __builtin_amdgcn_logf(S2) * (kLog10_2 * 0.125f) + 1.0f;  // log10(S2) / 8 + 1
   return m;
 }
 
@@ -111,8 +120,8 @@ __device__ __forceinline__ void pull27(const float* __restrict__ fin, unsigned n
     f[i] = (fin + (size_t)i * nn)[ox[CX27[i] + 1] + oy[CY27[i] + 1] + oz[CZ27[i] + 1]];
 }
 
-// Sample logging: every stride-th interior node with m above 1e-8 appends
-// (features, node, m) to the sample buffer; the target is filled after the
+// Sample logging: every stride-th interior node with m above 1e-8 and finite
+// features appends (features, node, m) to the sample buffer; the target is filled after the
 // plain step has produced alpha (k_collect_targets).
 __global__ __launch_bounds__(256) void k27_collect(const float* __restrict__ fin, Box b,
                                                    const uint8_t* __restrict__ flags, int stride,
@@ -131,6 +140,9 @@ __global__ __launch_bounds__(256) void k27_collect(const float* __restrict__ fin
     pull27(fin, gid, b, f);
     m = features27(f, phi);
     take = m > 1e-8f;
+    // a pair with a non-finite feature never enters the training set
+#pragma unroll
+    for (int j = 0; j < kIn; ++j) take = take && __builtin_isfinite(phi[j]);
   }
   const unsigned long long bm = __ballot(take);
   if (!bm) return;

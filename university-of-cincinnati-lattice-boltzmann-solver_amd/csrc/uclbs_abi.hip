@@ -148,6 +148,18 @@ int hip_rc(hipError_t e) {
   return e == hipSuccess ? UCLBS_OK : set_err(UCLBS_E_HIP, "HIP error: %s", hipGetErrorString(e));
 }
 
+// A change of what the captured kernels read (levels, link tables, faces,
+// boundary tables) drops the captured advance; the next uclbs_advance with
+// use_graph captures it again.
+void invalidate_graph(Ctx* c) {
+  c->structure++;
+  if (c->graph) {
+    (void)hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+  }
+  c->graph_sig.clear();
+}
+
 size_t elem_size(const Ctx* c) {
   return c->cfg.precision == UCLBS_PREC_FP64 ? sizeof(double) : sizeof(float);
 }
@@ -280,7 +292,7 @@ hipError_t alloc_far_queue(Level& L, int64_t nb) {
 
 // flags + ascending boundary list from the link table
 int finalize_links(Ctx* c, Level* L) {
-  c->structure++;
+  invalidate_graph(c);
   const int64_t n = L->box.n;
   if (!L->flags) HIP_TRY(hipMalloc(&L->flags, (size_t)n));
   HIP_TRY(uclbs::launch_flags27(L->state, L->box, L->face_code, L->flags, c->stream));
@@ -347,6 +359,13 @@ static int build_near39(Ctx* c, Level* L) {
   return UCLBS_OK;
 }
 
+// the finest level of a hierarchy: level 0 and the refined levels nested under it
+int finest_level(const Ctx* c) {
+  int n = 1;
+  while (n < (int)c->levels.size() && c->levels[n].defined && c->levels[n].refined) ++n;
+  return n - 1;
+}
+
 int launch_steps39(Ctx* c, Level* L, int nsteps, hipEvent_t* ev) {
   if (L->table39)
     return set_err(UCLBS_E_UNSUPPORTED, "D3Q39 levels with a link table are not stepped: the "
@@ -383,7 +402,10 @@ int launch_steps39(Ctx* c, Level* L, int nsteps, hipEvent_t* ev) {
     a.gin = L->g[L->cur];
     a.fout = L->f[1 - L->cur];
     a.gout = L->g[1 - L->cur];
-    a.step = (long long)L->step;
+    // every level's start-up ramps read the finest level's counter, the
+    // reference's timestep(amr_max_lvl) (calculate_alpha.f95:185-187,
+    // collision.f95:277-278, find_q_bar.f95:36, find_little_q_bar.f95:26)
+    a.step = (long long)c->levels[finest_level(c)].step;
     if (ev) HIP_TRY(hipEventRecord(ev[2 * s], c->stream));
     HIP_TRY(uclbs::launch_d3q39_step(L->box, a, c->stream));
     if (ev) HIP_TRY(hipEventRecord(ev[2 * s + 1], c->stream));
@@ -569,7 +591,7 @@ int uclbs_define_level(void* ctx, int lvl, const int32_t lo[3], const int32_t hi
   if ((int)c->levels.size() <= lvl) c->levels.resize(lvl + 1);
   Level& L = c->levels[lvl];
   free_level(L);
-  c->structure++;
+  invalidate_graph(c);
   // ghost layer of a refined level: one node on D3Q27, four on D3Q39 (links
   // reach three nodes, four along x on the shifted lattice)
   const int G = lvl > 0 ? (c->cfg.lattice == UCLBS_LATTICE_D3Q39 ? 4 : 1) : 0;
@@ -843,7 +865,7 @@ int uclbs_define_level_boxes(void* ctx, int lvl, int nbox, const int32_t* lo, co
   if ((int)c->levels.size() <= lvl) c->levels.resize(lvl + 1);
   Level& L = c->levels[lvl];
   free_level(L);
-  c->structure++;
+  invalidate_graph(c);
   L.multibox = true;
   L.blo.assign(lo, lo + 3 * nbox);
   L.bhi.assign(hi, hi + 3 * nbox);
@@ -1180,6 +1202,7 @@ int uclbs_set_faces(void* ctx, int lvl, const int32_t face_code[6]) {
   }
   for (int f = 0; f < 6; ++f) L->face_code[f] = face_code[f];
   L->near_dirty = true;
+  invalidate_graph(c);
   return UCLBS_OK;
 }
 
@@ -1192,6 +1215,8 @@ int uclbs_set_bc_states(void* ctx, int n_in, const double* in_states, int n_out,
     return set_err(UCLBS_E_ARG, "bad inlet/outlet counts");
   int rc = set_device(c);
   if (rc) return rc;
+  // a captured advance holds the old tables' addresses
+  invalidate_graph(c);
   auto table = [&](int m, const double* st, double** dptr) -> int {
     if (*dptr) { (void)hipFree(*dptr); *dptr = nullptr; }
     if (m == 0) return UCLBS_OK;
@@ -1283,6 +1308,7 @@ int uclbs_build_links(void* ctx, int lvl, const double origin[3], double dx, int
   if (q39) {
     L->table39 = true;
     L->nsolid = (int64_t)hc[0];
+    invalidate_graph(c);
   } else if ((rc = finalize_links(c, L))) {
     return rc;
   }
@@ -1305,6 +1331,7 @@ int uclbs_set_links(void* ctx, int lvl, const int32_t* state, int64_t n_nodes) {
   HIP_TRY(hipMemcpy(L->state, state, bytes, hipMemcpyHostToDevice));
   if (c->cfg.lattice == UCLBS_LATTICE_D3Q39) {
     L->table39 = true;
+    invalidate_graph(c);
     return UCLBS_OK;
   }
   return finalize_links(c, L);
@@ -1804,20 +1831,25 @@ static int advance_rec(Ctx* c, int lvl) {
 static std::vector<int> graph_signature(const Ctx* c) {
   std::vector<int> sig = {(int)c->structure, (int)c->keep_alpha, c->cfg.alpha_mode,
                           (int)c->levels.size()};
+  // addresses of the boundary tables the captured kernels read
+  for (const void* p : {(const void*)c->d_feq_in, (const void*)c->d_feq_out, (const void*)c->d_feq_fs}) {
+    sig.push_back((int)(uintptr_t)p);
+    sig.push_back((int)((uintptr_t)p >> 32));
+  }
   for (const Level& L : c->levels) {
     sig.push_back(L.defined);
     sig.push_back(L.cur);
     sig.push_back(L.far_parity);
     sig.push_back(L.tile_nodes);
+    sig.push_back((int)(uintptr_t)L.near_list);
+    sig.push_back((int)((uintptr_t)L.near_list >> 32));
+    sig.push_back((int)L.n_near);
+    for (int f = 0; f < 6; ++f) sig.push_back(L.face_code[f]);
   }
   return sig;
 }
 
-static int levels_in_use(const Ctx* c) {
-  int n = 1;
-  while (n < (int)c->levels.size() && c->levels[n].defined && c->levels[n].refined) ++n;
-  return n;
-}
+static int levels_in_use(const Ctx* c) { return finest_level(c) + 1; }
 
 extern "C" {
 
@@ -1830,11 +1862,9 @@ int uclbs_advance(void* ctx, int ncoarse, int use_graph) {
   if (rc) return rc;
   const int nl = levels_in_use(c);
   const bool q39 = c->cfg.lattice == UCLBS_LATTICE_D3Q39;
-  // the D3Q39 start-up ramps read the step counter, which a replayed graph
-  // would repeat: graphs only once every level is past the start-up
-  bool ramp = false;
-  if (q39)
-    for (int l = 0; l < nl; ++l) ramp = ramp || c->levels[l].step < c->levels[l].gas39.startup;
+  // the D3Q39 start-up ramps read the finest level's step counter, which a
+  // replayed graph would repeat: graphs only once it is past the start-up
+  const bool ramp = q39 && c->levels[nl - 1].step < c->levels[nl - 1].gas39.startup;
   const int pairs = (use_graph && !ramp) ? ncoarse / 2 : 0;
   if (pairs > 0) {
     // host-side tables are built before the capture
@@ -1855,14 +1885,30 @@ int uclbs_advance(void* ctx, int ncoarse, int use_graph) {
       // return to where they were, so the graph replays as it stands
       const int64_t done0 = c->steps_done;
       std::vector<int64_t> steps0;
-      for (const Level& L : c->levels) steps0.push_back(L.step);
+      std::vector<int> cur0, par0;
+      for (const Level& L : c->levels) {
+        steps0.push_back(L.step);
+        cur0.push_back(L.cur);
+        par0.push_back(L.far_parity);
+      }
       HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       rc = advance_rec(c, 0);
       if (!rc) rc = advance_rec(c, 0);
       hipGraph_t g = nullptr;
       const hipError_t e = hipStreamEndCapture(c->stream, &g);
-      c->steps_done = done0;  // the captured work has not run yet
-      for (size_t l = 0; l < steps0.size(); ++l) c->levels[l].step = steps0[l];
+      bool parities_return = true;
+      for (size_t l = 0; l < cur0.size(); ++l)
+        parities_return = parities_return && c->levels[l].cur == cur0[l] &&
+                          c->levels[l].far_parity == par0[l];
+      // the captured work has not run yet: counters and buffer parities are
+      // those of the start (a capture that failed part-way may have flipped
+      // the parities of launches that were never recorded)
+      c->steps_done = done0;
+      for (size_t l = 0; l < steps0.size(); ++l) {
+        c->levels[l].step = steps0[l];
+        c->levels[l].cur = cur0[l];
+        c->levels[l].far_parity = par0[l];
+      }
       if (rc) {
         if (g) (void)hipGraphDestroy(g);
         return rc;
@@ -1873,7 +1919,7 @@ int uclbs_advance(void* ctx, int ncoarse, int use_graph) {
       (void)hipGraphDestroy(g);
       if (ei != hipSuccess)
         return set_err(UCLBS_E_HIP, "graph instantiation failed: %s", hipGetErrorString(ei));
-      if (graph_signature(c) != sig) {
+      if (!parities_return) {
         (void)hipGraphExecDestroy(x);
         return set_err(UCLBS_E_STATE, "captured advance does not return to its start state");
       }
@@ -2200,6 +2246,9 @@ int uclbs_mlp_train(void* ctx, int64_t steps, int batch, double lr, uint64_t see
     double ms[2];
     HIP_TRY(hipMemcpyAsync(ms, c->d_scalar + 2, sizeof(ms), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
+    if (!std::isfinite(ms[0]) || !std::isfinite(ms[1]))
+      return set_err(UCLBS_E_NUMERIC, "mlp_train: non-finite training targets (mean %g, std %g)",
+                     ms[0], ms[1]);
     c->mlp_ymean = (float)ms[0];
     c->mlp_ystd = (float)ms[1];
     c->samples_normalised = true;
@@ -2227,6 +2276,8 @@ int uclbs_mlp_train(void* ctx, int64_t steps, int batch, double lr, uint64_t see
   }
   if (loss_first) *loss_first = l0;
   if (loss_last) *loss_last = steps == 1 ? l0 : l1;
+  if (!std::isfinite(l0) || !std::isfinite(steps == 1 ? l0 : l1))
+    return set_err(UCLBS_E_NUMERIC, "mlp_train: non-finite loss (first %g, last %g)", l0, l1);
   return mlp_publish(c);
 }
 
