@@ -292,6 +292,14 @@ int uclbs_mlp_loss_grad(void *ctx, const float *X, const float *y, int B, float 
                         double *loss);
 /* network output for a host batch; use_mfma 1: BF16 MFMA path, 0: FP32 path */
 int uclbs_mlp_forward(void *ctx, const float *X, int B, int use_mfma, float *yhat);
+/* uclbs_mlp_forward with use_mfma = 1 for callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  It calls no hipMalloc / hipFree: the batch
+ * is staged through a fixed area of the library (65,536 rows, 3.4 MB per
+ * device, held until the process ends); B above 65,536 is refused with
+ * UCLBS_E_ARG.  Every call on one device shares that area, so calls that
+ * overlap on one device (two host threads, two contexts) race. */
+int uclbs_mlp_forward_noalloc(void *ctx, const float *X, int B, float *yhat);
 /* device sample buffer for (features, y) training pairs */
 int uclbs_mlp_reserve(void *ctx, int64_t capacity);
 /* logs every stride-th interior node of the current state, advances the level

@@ -32,7 +32,7 @@ EXPORTS = (
     "uclbs_mlp_collect", "uclbs_mlp_download_samples", "uclbs_mlp_download_sample_nodes",
     "uclbs_mlp_train", "uclbs_interlevel",
     "uclbs_init_from_coarse", "uclbs_fine_to_coarse", "uclbs_define_level_boxes",
-    "uclbs_advance", "uclbs_autotune",
+    "uclbs_advance", "uclbs_autotune", "uclbs_mlp_forward_noalloc",
 )
 MLP_PARAMS = 9217
 FACE_PERIODIC = 2147483647
@@ -130,6 +130,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "uclbs_mlp_get_params": (I, [P, fp, fp, fp]),
         "uclbs_mlp_loss_grad": (I, [P, fp, fp, I, fp, dp]),
         "uclbs_mlp_forward": (I, [P, fp, I, I, fp]),
+        "uclbs_mlp_forward_noalloc": (I, [P, fp, I, fp]),
         "uclbs_mlp_reserve": (I, [P, I64]),
         "uclbs_mlp_collect": (I, [P, I, I, ctypes.POINTER(I64)]),
         "uclbs_mlp_download_samples": (I, [P, fp, fp, I64, ctypes.POINTER(I64)]),
@@ -384,6 +385,14 @@ class Solver:
         X = np.ascontiguousarray(X, np.float32)
         out = np.empty(X.shape[0], np.float32)
         self._check(self.lib.uclbs_mlp_forward(self.ctx, _fptr(X), X.shape[0], int(mfma), _fptr(out)))
+        return out
+
+    def mlp_forward_noalloc(self, X):
+        """BF16-MFMA forward through the library's fixed staging area (no
+        device allocation; at most 65,536 rows)."""
+        X = np.ascontiguousarray(X, np.float32)
+        out = np.empty(X.shape[0], np.float32)
+        self._check(self.lib.uclbs_mlp_forward_noalloc(self.ctx, _fptr(X), X.shape[0], _fptr(out)))
         return out
 
     def mlp_reserve(self, capacity: int):

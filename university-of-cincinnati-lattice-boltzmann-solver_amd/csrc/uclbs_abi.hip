@@ -9,6 +9,7 @@
 #include <cstring>
 #include <string>
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "../../include/uclbs_hip.h"
@@ -16,9 +17,39 @@
 #include "lattice.h"
 #include "philox.h"
 
+// Staging area of uclbs_mlp_forward_noalloc: the features (12 floats per
+// row) and the network outputs (1 float per row) of at most
+// kNoAllocRows rows.  One copy per device, held until the process ends; every
+// call on a device shares it.  Host code never uses this name as an address
+// (it is a host-side stand-in): the device address comes from
+// hipGetSymbolAddress, once per device, in uclbs_create.
+constexpr int kNoAllocRows = 65536;
+constexpr size_t kScratchFloats = (size_t)kNoAllocRows * 13;
+static __device__ __align__(256) float scratch[kScratchFloats];
+
 namespace {
 
 thread_local std::string g_err;
+
+// device address of `scratch` per device, resolved once per device
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_dev[kMaxDevices] = {};
+
+// with `device` current: the device's copy of `scratch`
+hipError_t scratch_address(int device, float** out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_dev[device] = static_cast<float*>(p);
+  }
+  *out = g_scratch_dev[device];
+  return hipSuccess;
+}
 
 int set_err(int code, const char* fmt, ...) {
   char buf[512];
@@ -138,6 +169,8 @@ struct Ctx {
   float* batch_x = nullptr;
   unsigned* batch_idx = nullptr;
   double* d_scalar = nullptr;  // loss, mean, std
+  // this device's staging area of uclbs_mlp_forward_noalloc
+  float* scratch = nullptr;
   // captured multi-level advance (uclbs_advance with use_graph)
   hipGraphExec_t graph = nullptr;
   std::vector<int> graph_sig;
@@ -545,6 +578,7 @@ int uclbs_create(const uclbs_config* cfg, void** out) {
   if (e == hipSuccess)
     e = hipMemset(c->d_stats, 0,
                   sizeof(unsigned long long) * uclbs::kStatSlots * uclbs::kStatFields);
+  if (e == hipSuccess) e = scratch_address(c->device, &c->scratch);
   if (e != hipSuccess) {
     int rc = set_err(UCLBS_E_HIP, "context setup failed: %s", hipGetErrorString(e));
     uclbs_destroy(c);
@@ -2140,6 +2174,27 @@ int uclbs_mlp_forward(void* ctx, const float* X, int B, int use_mfma, float* yha
   (void)hipFree(dX);
   if (dy) (void)hipFree(dy);
   if (e != hipSuccess) return set_err(UCLBS_E_HIP, "mlp_forward: %s", hipGetErrorString(e));
+  return UCLBS_OK;
+}
+
+int uclbs_mlp_forward_noalloc(void* ctx, const float* X, int B, float* yhat) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || !X || !yhat || B <= 0) return set_err(UCLBS_E_ARG, "bad argument");
+  if (B > kNoAllocRows)
+    return set_err(UCLBS_E_ARG, "mlp_forward_noalloc: %d rows exceed the %d-row staging area", B,
+                   kNoAllocRows);
+  if (!c->mlp_ready) return set_err(UCLBS_E_STATE, "no network");
+  if (!c->scratch) return set_err(UCLBS_E_STATE, "no staging area on this device");
+  int rc = set_device(c);
+  if (rc) return rc;
+  // features in rows [0, B) of the first part, outputs in the second; rows past
+  // B (left by an earlier, larger call) are never read
+  float* dX = c->scratch;
+  float* dy = c->scratch + (size_t)kNoAllocRows * 12;
+  HIP_TRY(hipMemcpyAsync(dX, X, (size_t)B * 12 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(uclbs::launch_mlp_infer(dX, B, c->mlp_wbf, c->mlp_p, dy, c->stream));
+  HIP_TRY(hipMemcpyAsync(yhat, dy, (size_t)B * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
   return UCLBS_OK;
 }
 
