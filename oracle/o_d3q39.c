@@ -302,23 +302,14 @@ double o39_alpha_estimate(const double f[39], const double diff[39], double T, i
   return a;
 }
 
-/* collision.f95:206-293 for one node; feq from the stored multipliers */
-int o39_collide(const double f[39], const double g[39], const double lm[13], double rho,
-                const double u[3], double T, double alpha, int64_t step, const o39_consts *c,
-                double fout[39], double gout[39]) {
+/* find_q_bar.f95:52-220: the quasi-equilibrium correction of the
+ * deviation df, from its central third moments Q_abc about u:
+ * qbar_i = eps (sum_abc Q_abc c_a c_b c_c - 3 T sum_abc Q_abc delta_ab c_c) / (6 T^3)
+ * (yi and kroen_val of directions_and_weights.f95:629-796) */
+void o39_qbar(const double df[39], const double u[3], double T, double eps, int shifted,
+              double qbar[39]) {
   int cx[NQ], cy[NQ], cz[NQ];
-  o39_velocities(c->shifted, cx, cy, cz);
-  double feq[NQ], df[NQ], dg[NQ];
-  o39_feq(lm, rho, c->shifted, feq);
-  for (int i = 0; i < NQ; i++) {
-    df[i] = feq[i] - f[i];
-    dg[i] = c->lead_g * T * feq[i] - g[i];
-  }
-  /* epsilon_calculator (amr_processes.f95:1789-1814) */
-  double eps = step < c->startup ? (((double)step + 1.0) / (double)c->startup) * c->eps : c->eps;
-  /* find_q_bar.f95:52-220: central third moments Q_abc of the deviation */
-  double Q[3][3][3];
-  memset(Q, 0, sizeof(Q));
+  o39_velocities(shifted, cx, cy, cz);
   double xxx = 0, yyy = 0, zzz = 0, xxy = 0, xxz = 0, xyy = 0, xzz = 0, yyz = 0, yzz = 0, xyz = 0;
   for (int i = 0; i < NQ; i++) {
     double a = cx[i] - u[0], b = cy[i] - u[1], e = cz[i] - u[2];
@@ -337,7 +328,6 @@ int o39_collide(const double f[39], const double g[39], const double lm[13], dou
                      xxy, xyy, xyz, xyy, yyy, yyz, xyz, yyz, yzz,
                      xxz, xyz, xzz, xyz, yyz, yzz, xzz, yzz, zzz};
   double tc = 6.0 * T * T * T;
-  double qbar[NQ];
   for (int i = 0; i < NQ; i++) {
     int cv[3] = {cx[i], cy[i], cz[i]};
     double sy = 0.0, sk = 0.0;
@@ -350,7 +340,13 @@ int o39_collide(const double f[39], const double g[39], const double lm[13], dou
         }
     qbar[i] = eps * (sy - 3 * T * sk) / tc;
   }
-  /* find_little_q_bar.f95:23-52 */
+}
+
+/* find_little_q_bar.f95:23-52: hq_i = (eps / T) c_i . sum_j (c_j - u) dg_j */
+void o39_little_qbar(const double dg[39], const double u[3], double T, double eps, int shifted,
+                     double hq[39]) {
+  int cx[NQ], cy[NQ], cz[NQ];
+  o39_velocities(shifted, cx, cy, cz);
   double lq[3] = {0, 0, 0};
   for (int i = 0; i < NQ; i++) {
     lq[0] += (cx[i] - u[0]) * dg[i];
@@ -358,6 +354,26 @@ int o39_collide(const double f[39], const double g[39], const double lm[13], dou
     lq[2] += (cz[i] - u[2]) * dg[i];
   }
   double ql = eps / T;
+  for (int i = 0; i < NQ; i++) hq[i] = ql * (cx[i] * lq[0] + cy[i] * lq[1] + cz[i] * lq[2]);
+}
+
+/* collision.f95:206-293 for one node; feq from the stored multipliers */
+int o39_collide(const double f[39], const double g[39], const double lm[13], double rho,
+                const double u[3], double T, double alpha, int64_t step, const o39_consts *c,
+                double fout[39], double gout[39]) {
+  int cx[NQ], cy[NQ], cz[NQ];
+  o39_velocities(c->shifted, cx, cy, cz);
+  double feq[NQ], df[NQ], dg[NQ];
+  o39_feq(lm, rho, c->shifted, feq);
+  for (int i = 0; i < NQ; i++) {
+    df[i] = feq[i] - f[i];
+    dg[i] = c->lead_g * T * feq[i] - g[i];
+  }
+  /* epsilon_calculator (amr_processes.f95:1789-1814) */
+  double eps = step < c->startup ? (((double)step + 1.0) / (double)c->startup) * c->eps : c->eps;
+  double qbar[NQ], hq[NQ];
+  o39_qbar(df, u, T, eps, c->shifted, qbar);
+  o39_little_qbar(dg, u, T, eps, c->shifted, hq);
   /* quick_sutherlands / quick_therm_cond, beta_1/2_calculator */
   double mu = o39_sutherland(T, c, NULL, NULL), kappa = 0.0;
   int rc = o39_therm_cond(rho, T, c, &kappa);
@@ -370,9 +386,8 @@ int o39_collide(const double f[39], const double g[39], const double lm[13], dou
     b2 = 1.0 / ((2.0 * kappa) / (rho * c->cp * T) + 1.0);
   }
   for (int i = 0; i < NQ; i++) {
-    double hq = ql * (cx[i] * lq[0] + cy[i] * lq[1] + cz[i] * lq[2]);
     fout[i] = f[i] + alpha * b1 * (df[i] - qbar[i]) + b2 * qbar[i];
-    gout[i] = g[i] + alpha * b1 * (dg[i] - hq) + b2 * hq;
+    gout[i] = g[i] + alpha * b1 * (dg[i] - hq[i]) + b2 * hq[i];
   }
   return rc;
 }

@@ -125,6 +125,8 @@ def lib() -> ctypes.CDLL:
         L.o39_alpha_estimate.argtypes = [dp, dp, D, ctypes.c_int64, c39, ctypes.POINTER(I)]
         L.o39_alpha_estimate.restype = D
         L.o39_collide.argtypes = [dp, dp, dp, D, dp, D, D, ctypes.c_int64, c39, dp, dp]
+        L.o39_qbar.argtypes = [dp, dp, D, D, I, dp]
+        L.o39_little_qbar.argtypes = [dp, dp, D, D, I, dp]
         L.o39_moments.argtypes = [dp, dp, I, dp, dp, dp, dp, D]
         L.o39_damp_periodic.argtypes = [I, I, I, dp, I, I]
         L.o39_damp_periodic.restype = ctypes.c_int64
@@ -454,6 +456,23 @@ def collide39(f, g, lm, rho, u, T, alpha, step, c):
                       _dp(np.ascontiguousarray(u, np.float64)), float(T), float(alpha), int(step),
                       ctypes.byref(c), _dp(fo), _dp(go))
     return fo, go
+
+
+def qbar39(df, u, T, eps, shifted=False):
+    """find_q_bar: the quasi-equilibrium correction of the deviation df."""
+    out = np.zeros(39)
+    lib().o39_qbar(_dp(np.ascontiguousarray(df, np.float64)), _dp(np.ascontiguousarray(u, np.float64)),
+                   float(T), float(eps), int(shifted), _dp(out))
+    return out
+
+
+def little_qbar39(dg, u, T, eps, shifted=False):
+    """find_little_q_bar: the energy-flux correction of the g deviation dg."""
+    out = np.zeros(39)
+    lib().o39_little_qbar(_dp(np.ascontiguousarray(dg, np.float64)),
+                          _dp(np.ascontiguousarray(u, np.float64)), float(T), float(eps),
+                          int(shifted), _dp(out))
+    return out
 
 
 def init39(shape, rho, ux, uy, uz, T, c):

@@ -149,6 +149,10 @@ int o39_lm_solve(double rho, const double u[3], double T, const double P[6], dou
                  double expo_last[39], int shifted, int ref_mode, int maxit);
 double o39_alpha_estimate(const double f[39], const double diff[39], double T, int64_t step,
                           const o39_consts *c, int *status);
+void o39_qbar(const double df[39], const double u[3], double T, double eps, int shifted,
+              double qbar[39]);
+void o39_little_qbar(const double dg[39], const double u[3], double T, double eps, int shifted,
+                     double hq[39]);
 int o39_collide(const double f[39], const double g[39], const double lm[13], double rho,
                 const double u[3], double T, double alpha, int64_t step, const o39_consts *c,
                 double fout[39], double gout[39]);
