@@ -15,29 +15,20 @@
  *                     rho_v_bc.f95:23-29: fi(x,opp(k)) = inl_fieq(opp(k)))
  *   -299..-200        f_i = outlet equilibrium j (outlet_bc.f95:23-58)
  *   -1000             f_i = freestream equilibrium (freestream_edge_bc.f95:18-19)
- *   <= -1001          interpolated bounce-back, below
+ *   <= -1001          linear interpolated bounce-back, below
  *   other (-666 ...)  f_i = fout_opp(i)(x)
  * then the node collides as in o_d3q27.c.  Solid nodes keep their values.
  *
- * Interpolated bounce-back (the reference's structure, interpolative_bc/,
- * restated for the isothermal D3Q27 lattice, which the reference lacks):
- *   q_k = |s_k + 1001| / 1e9 for every wall link k of x (target_finder.f95:40);
- *   target velocity u_t = mean over wall links of q_k u(x - c_k) / (q_k + 1),
- *     u(x) when x - c_k is not a fluid link (target_finder.f95:36-83);
- *   target density rho_t = sum over k of f_k(x), with f_k replaced by
- *     f_opp(k)(x) when the link towards x - c_k is a wall (target_finder.f95
- *     :56-60); fout_opp(k)(x) if both sides are walls;
- *   f_i(x) = feq_opp(i)(rho_t, u_t) + f_neq_i, the equilibrium of the
- *     outgoing direction as in interpolative_bounceback_bc.f95:2917 with
- *     streaming.f95:1132, and the Grad/Chapman-Enskog non-equilibrium
- *     f_neq_i = -3 tau rho_t w_i (c_ia c_ib - delta_ab/3) du_b/dx_a,
- *     tau = 1/(2 beta);
- *   du/dx_a from u(x + e_a) and u(x - e_a), a wall side replaced by u = 0
- *     at distance q, a non-fluid, non-wall side by u_t at distance 1
- *     (derivatives.f95:753, first_order_vel_cond_unknown);
- *   a negative result is replaced by its magnitude and counted
- *     (interpolative_bounceback_bc.f95:2969-2973).
- * u(y) are the moments of fout at y (the collision conserves them).
+ * Wall links (the reference's D3Q27 lattice has no wall rule; its D3Q39
+ * interpolated bounce-back lives in interpolative_bc/).  The missing
+ * population f_i(x) of a link whose wall lies at q = |s + 1001| / 1e9
+ * (target_finder.f95:40, linkwise.f95:52-56) of the way from x towards
+ * x - c_i is the linear interpolated bounce-back of Bouzidi, Firdaouss &
+ * Lallemand (2001) on the post-collision populations, k = opp(i):
+ *   q <  1/2:  f_i(x) = 2q fout_k(x) + (1 - 2q) fout_k(x + c_i)
+ *   q >= 1/2:  f_i(x) = fout_k(x) / (2q) + (1 - 1/(2q)) fout_i(x)
+ * (halfway bounce-back fout_k(x) where x + c_i is not a fluid node).  Both
+ * forms are convex combinations, so no population turns negative.
  */
 #include "uclbs_oracle.h"
 #include <math.h>
@@ -63,7 +54,6 @@ void o27_step_box(const o_box *bx, const int32_t *state, const double *fin, doub
                   const o27_box_params *p, double *alpha_out, o_stats *st, o_bc_stats *bst) {
   const int64_t n = (int64_t)bx->nx * bx->ny * bx->nz;
   int64_t it = 0, nb = 0, nn = 0, nt = 0, nc = 0, nibb = 0, nrep = 0, nodes = 0;
-  const double tau = 1.0 / (2.0 * p->beta);
 #ifdef _OPENMP
   if (p->nthreads > 0) omp_set_num_threads(p->nthreads);
 #pragma omp parallel for collapse(2) schedule(static) \
@@ -105,75 +95,26 @@ void o27_step_box(const o_box *bx, const int32_t *state, const double *fin, doub
           }
         }
         if (any_wall) {
-          /* target state */
-          double ut[3] = {0, 0, 0}, ux[3];
-          int cnt = 0;
-          vel_at(bx, fin, k, ux);
-          for (int kk = 1; kk < 27; kk++) {
-            const int32_t s = state[(int64_t)kk * n + k];
-            if (!is_wall(s)) continue;
-            double q = fabs((double)s + 1001.0) / 1.0e9;
-            double uu[3];
-            if (state[(int64_t)O27_OPP[kk] * n + k] >= 0) {
-              int xs = wrapi(x - O27_CX[kk], bx->nx), ys = wrapi(y - O27_CY[kk], bx->ny),
-                  zs = wrapi(z - O27_CZ[kk], bx->nz);
-              vel_at(bx, fin, idx(bx, xs, ys, zs), uu);
-            } else {
-              uu[0] = ux[0]; uu[1] = ux[1]; uu[2] = ux[2];
-            }
-            for (int d = 0; d < 3; d++) ut[d] += q * uu[d] / (q + 1.0);
-            cnt++;
-          }
-          for (int d = 0; d < 3; d++) ut[d] /= cnt;
-          double rt = 0.0;
-          for (int kk = 0; kk < 27; kk++) {
-            if (!unknown[kk]) rt += f[kk];
-            else if (!unknown[O27_OPP[kk]]) rt += f[O27_OPP[kk]];
-            else rt += fin[(int64_t)O27_OPP[kk] * n + k];
-          }
-          /* velocity gradient G[a][b] = du_b/dx_a */
-          double G[3][3];
-          for (int a = 0; a < 3; a++) {
-            const int dp = 1 + a, dm = 4 + a; /* +e_a, -e_a in the D3Q27 order */
-            double hp, hm, up[3], um[3];
-            const int32_t sp = state[(int64_t)dp * n + k], sm = state[(int64_t)dm * n + k];
-            int ex[3] = {0, 0, 0};
-            ex[a] = 1;
-            if (sp >= 0) {
-              hp = 1.0;
-              vel_at(bx, fin, idx(bx, wrapi(x + ex[0], bx->nx), wrapi(y + ex[1], bx->ny),
-                                  wrapi(z + ex[2], bx->nz)), up);
-            } else if (is_wall(sp)) {
-              hp = fabs((double)sp + 1001.0) / 1.0e9;
-              up[0] = up[1] = up[2] = 0.0;
-            } else {
-              hp = 1.0;
-              up[0] = ut[0]; up[1] = ut[1]; up[2] = ut[2];
-            }
-            if (sm >= 0) {
-              hm = 1.0;
-              vel_at(bx, fin, idx(bx, wrapi(x - ex[0], bx->nx), wrapi(y - ex[1], bx->ny),
-                                  wrapi(z - ex[2], bx->nz)), um);
-            } else if (is_wall(sm)) {
-              hm = fabs((double)sm + 1001.0) / 1.0e9;
-              um[0] = um[1] = um[2] = 0.0;
-            } else {
-              hm = 1.0;
-              um[0] = ut[0]; um[1] = ut[1]; um[2] = ut[2];
-            }
-            for (int b2 = 0; b2 < 3; b2++) G[a][b2] = (up[b2] - um[b2]) / (hp + hm);
-          }
-          double feq_t[27];
-          o27_feq(rt, ut[0], ut[1], ut[2], feq_t);
+          /* linear interpolated bounce-back (Bouzidi, Firdaouss & Lallemand,
+           * Phys. Fluids 13, 3452 (2001)) from the post-collision values fin:
+           * k = opp(i) points into the wall, which lies at fraction q of the
+           * link from x towards x - c_i */
           for (int i = 1; i < 27; i++) {
             if (!unknown[i]) continue;
-            const int c[3] = {O27_CX[i], O27_CY[i], O27_CZ[i]};
-            double qs = 0.0;
-            for (int a = 0; a < 3; a++)
-              for (int b2 = 0; b2 < 3; b2++)
-                qs += ((double)(c[a] * c[b2]) - (a == b2 ? 1.0 / 3.0 : 0.0)) * G[a][b2];
-            double v = feq_t[O27_OPP[i]] - 3.0 * tau * rt * O27_W[i] * qs;
-            if (v < 0.0) { v = -v; nrep++; }
+            const int kk = O27_OPP[i];
+            const double q = fabs((double)state[(int64_t)kk * n + k] + 1001.0) / 1.0e9;
+            const double fk = fin[(int64_t)kk * n + k];
+            double v;
+            if (q >= 0.5) {
+              v = fk / (2.0 * q) + (1.0 - 1.0 / (2.0 * q)) * fin[(int64_t)i * n + k];
+            } else if (state[(int64_t)i * n + k] >= 0) {
+              /* the fluid node behind x, x + c_i */
+              const int xs = wrapi(x + O27_CX[i], bx->nx), ys = wrapi(y + O27_CY[i], bx->ny),
+                        zs = wrapi(z + O27_CZ[i], bx->nz);
+              v = 2.0 * q * fk + (1.0 - 2.0 * q) * fin[(int64_t)kk * n + idx(bx, xs, ys, zs)];
+            } else {
+              v = fk; /* no fluid node behind x: halfway bounce-back */
+            }
             f[i] = v;
             nibb++;
           }

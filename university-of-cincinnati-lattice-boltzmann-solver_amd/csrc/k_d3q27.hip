@@ -444,39 +444,22 @@ __global__ void k27_init_tgv(void* __restrict__ dst_, Box dom, int64_t nn, int64
 //   -1xx / -2xx    inlet / outlet equilibrium table (inlet_bc.f95, outlet_bc.f95,
 //                  rho_v_bc.f95: fi(x,opp(k)) = inl_fieq(opp(k)))
 //   -1000          freestream equilibrium (freestream_edge_bc.f95:18-19)
-//   <= -1001       interpolated bounce-back:
-//                  q_k = |s_k + 1001|/1e9 (target_finder.f95:40); target
-//                  u_t = mean_k q_k u(x - c_k)/(q_k + 1) (target_finder.f95:36-83);
-//                  rho_t from the known populations, an unknown f_k replaced by
-//                  f_opp(k) (target_finder.f95:56-60); f_i = feq_opp(i)(rho_t,u_t)
-//                  (interpolative_bounceback_bc.f95:2917 + streaming.f95:1132)
-//                  + f_neq_i, f_neq_i = -3 tau rho_t w_i (c_ia c_ib - d_ab/3) du_b/dx_a
-//                  with du/dx_a from u(x +- e_a), a wall side at distance q with
-//                  u = 0; a negative value is replaced by its magnitude
-//                  (interpolative_bounceback_bc.f95:2969-2973) and counted
+//   <= -1001       linear interpolated bounce-back (Bouzidi, Firdaouss &
+//                  Lallemand 2001) on the post-collision populations: the wall
+//                  lies at q = |s + 1001|/1e9 (target_finder.f95:40) of the link
+//                  from x towards x - c_i, k = opp(i) points into it;
+//                  q < 1/2: f_i = 2q fout_k(x) + (1 - 2q) fout_k(x + c_i),
+//                  q >= 1/2: f_i = fout_k(x)/(2q) + (1 - 1/(2q)) fout_i(x),
+//                  halfway bounce-back where x + c_i is not a fluid node
+//                  (the reference's D3Q27 lattice has no wall rule; both forms
+//                  are convex combinations, so no population turns negative)
 //   other          fout_opp(i)(x)
-// u(y) are the moments of fout at y (conserved by the collision).  The node
-// then collides with the general-path collide.  All BC arithmetic is FP64.
+// The node then collides with the streaming kernel's collide.  All BC
+// arithmetic is FP64.
 template <bool kShifted>
 __device__ __forceinline__ double ldf(const void* f, int i, int64_t n, int64_t k) {
   if constexpr (kShifted) return W27[i] + (double)static_cast<const float*>(f)[(int64_t)i * n + k];
   else return static_cast<const double*>(f)[(int64_t)i * n + k];
-}
-
-template <bool kShifted>
-__device__ void vel_at(const void* f, int64_t n, int64_t k, double u[3]) {
-  double r = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
-#pragma unroll
-  for (int i = 0; i < 27; ++i) {
-    const double v = ldf<kShifted>(f, i, n, k);
-    r += v;
-    jx += CX27[i] * v;
-    jy += CY27[i] * v;
-    jz += CZ27[i] * v;
-  }
-  u[0] = jx / r;
-  u[1] = jy / r;
-  u[2] = jz / r;
 }
 
 __device__ __forceinline__ int wrapc(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
@@ -493,29 +476,6 @@ __device__ void feq27_fp64(double rho, const double u[3], double* feq) {
 #pragma unroll
   for (int i = 0; i < 27; ++i)
     feq[i] = rho * W27[i] * p[0][CX27[i] + 1] * p[1][CY27[i] + 1] * p[2][CZ27[i] + 1];
-}
-
-// u(x - c_q) for every boundary-list node x and direction q (q = 0: x itself),
-// one thread per pair, so the boundary collide reads its interpolation and
-// gradient velocities instead of forming them one neighbour after another.
-// Sources beyond a non-periodic face wrap; their codes never ask for them.
-template <bool kShifted>
-__global__ __launch_bounds__(256) void k27_bnd_vel(const void* __restrict__ fin, Box b, BcArgs bc) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t np = bc.nlist * 27;
-  if (t >= np) return;
-  const int q = (int)(t % 27);
-  const int64_t k = bc.list[t / 27];
-  const int x = (int)(k % b.nx);
-  const int64_t yz = k / b.nx;
-  const int y = (int)(yz % b.ny), z = (int)(yz / b.ny);
-  const int64_t src = wrapc(x - CX27[q], b.nx) +
-                      (int64_t)b.nx * (wrapc(y - CY27[q], b.ny) + (int64_t)b.ny * wrapc(z - CZ27[q], b.nz));
-  double u[3];
-  vel_at<kShifted>(fin, b.n, src, u);
-  bc.vel[t] = u[0];
-  bc.vel[np + t] = u[1];
-  bc.vel[2 * np + t] = u[2];
 }
 
 template <bool kShifted>
@@ -559,84 +519,25 @@ __global__ __launch_bounds__(64) void k27_boundary(const void* __restrict__ fin,
       f[i] = ldf<kShifted>(fin, OPP27[i], n, k);
     }
   }
-  int nibb = 0, nrep = 0;
+  int nibb = 0;
   if (any_wall && active) {
-    // u(x - c_q) of this node, formed by k27_bnd_vel
-    const int64_t np = bc.nlist * 27;
-    auto vel = [&](int q, double u[3]) __attribute__((always_inline)) {
-      const int64_t t = j * 27 + q;
-      u[0] = bc.vel[t];
-      u[1] = bc.vel[np + t];
-      u[2] = bc.vel[2 * np + t];
-    };
-    double ut[3] = {0.0, 0.0, 0.0};
-    int cnt = 0;
 #pragma unroll 1
-    for (int kk = 1; kk < 27; ++kk) {
-      const int32_t s = bc.state[(int64_t)kk * n + k];
-      if (s > -1001) continue;
-      const double q = fabs((double)s + 1001.0) / 1.0e9;
-      double uu[3];
-      vel(bc.state[(int64_t)OPP27[kk] * n + k] >= 0 ? kk : 0, uu);
-      for (int d = 0; d < 3; ++d) ut[d] += q * uu[d] / (q + 1.0);
-      cnt++;
-    }
-    for (int d = 0; d < 3; ++d) ut[d] /= cnt;
-    double rt = 0.0;
-    for (int kk = 0; kk < 27; ++kk) {
-      if (!unknown[kk]) rt += f[kk];
-      else if (!unknown[OPP27[kk]]) rt += f[OPP27[kk]];
-      else rt += ldf<kShifted>(fin, OPP27[kk], n, k);
-    }
-    double G[3][3];
-    for (int ax = 0; ax < 3; ++ax) {
-      const int dp = 1 + ax, dm = 4 + ax;
-      const int32_t sp = bc.state[(int64_t)dp * n + k], sm = bc.state[(int64_t)dm * n + k];
-      double hp, hm, up[3], um[3];
-      // x + e_a = x - c_dm, x - e_a = x - c_dp
-      if (sp >= 0) {
-        hp = 1.0;
-        vel(dm, up);
-      } else if (sp <= -1001) {
-        hp = fabs((double)sp + 1001.0) / 1.0e9;
-        up[0] = up[1] = up[2] = 0.0;
-      } else {
-        hp = 1.0;
-        up[0] = ut[0]; up[1] = ut[1]; up[2] = ut[2];
-      }
-      if (sm >= 0) {
-        hm = 1.0;
-        vel(dp, um);
-      } else if (sm <= -1001) {
-        hm = fabs((double)sm + 1001.0) / 1.0e9;
-        um[0] = um[1] = um[2] = 0.0;
-      } else {
-        hm = 1.0;
-        um[0] = ut[0]; um[1] = ut[1]; um[2] = ut[2];
-      }
-      for (int c2 = 0; c2 < 3; ++c2) G[ax][c2] = (up[c2] - um[c2]) / (hp + hm);
-    }
-    double pt[3][3];  // 1-D factors of feq(rt, ut)
-    for (int d = 0; d < 3; ++d) {
-      const double sq = sqrt(1.0 + 3.0 * ut[d] * ut[d]);
-      const double bb = 2.0 - sq;
-      pt[d][1] = bb;
-      pt[d][2] = bb * (2.0 * ut[d] + sq) / (1.0 - ut[d]);
-      pt[d][0] = bb * (1.0 - ut[d]) / (2.0 * ut[d] + sq);
-    }
-    const double tau = 1.0 / (2.0 * a.beta);
-#pragma unroll
     for (int i = 1; i < 27; ++i) {
       if (!unknown[i]) continue;
-      const int c[3] = {CX27[i], CY27[i], CZ27[i]};
-      double qs = 0.0;
-      for (int p = 0; p < 3; ++p)
-        for (int r = 0; r < 3; ++r)
-          qs += ((double)(c[p] * c[r]) - (p == r ? 1.0 / 3.0 : 0.0)) * G[p][r];
-      const int o = OPP27[i];
-      const double feq_o = rt * W27[o] * pt[0][CX27[o] + 1] * pt[1][CY27[o] + 1] * pt[2][CZ27[o] + 1];
-      double v = feq_o - 3.0 * tau * rt * W27[i] * qs;
-      if (v < 0.0) { v = -v; nrep++; }
+      const int kk = OPP27[i];
+      const double q = fabs((double)bc.state[(int64_t)kk * n + k] + 1001.0) / 1.0e9;
+      const double fk = ldf<kShifted>(fin, kk, n, k);
+      double v;
+      if (q >= 0.5) {
+        v = fk / (2.0 * q) + (1.0 - 1.0 / (2.0 * q)) * ldf<kShifted>(fin, i, n, k);
+      } else if (bc.state[(int64_t)i * n + k] >= 0) {
+        // the fluid node behind x, x + c_i
+        const int64_t src = wrapc(x + CX27[i], b.nx) +
+                            (int64_t)b.nx * (wrapc(y + CY27[i], b.ny) + (int64_t)b.ny * wrapc(z + CZ27[i], b.nz));
+        v = 2.0 * q * fk + (1.0 - 2.0 * q) * ldf<kShifted>(fin, kk, n, src);
+      } else {
+        v = fk;
+      }
       f[i] = v;
       nibb++;
     }
@@ -667,11 +568,10 @@ __global__ __launch_bounds__(64) void k27_boundary(const void* __restrict__ fin,
   }
   wave_stats(stats, active, ao.iters, ao.status == ALPHA_DEFER ? -1 : ao.status, false,
              ao.logpath);
-  if (active && (nibb || nrep)) {
+  if (active && nibb) {
     unsigned long long* sl = stats + (size_t)((blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64) &
                                               (kStatSlots - 1)) * kStatFields;
-    if (nibb) atomicAdd(sl + ST_IBB, (unsigned long long)nibb);
-    if (nrep) atomicAdd(sl + ST_REPAIR, (unsigned long long)nrep);
+    atomicAdd(sl + ST_IBB, (unsigned long long)nibb);
   }
 }
 
@@ -748,10 +648,6 @@ hipError_t launch_d3q27_boundary(const Box& b, const void* fin, void* fout, doub
                                  int keep_alpha, unsigned long long* stats, const StepArgs& a,
                                  const BcArgs& bc, hipStream_t s) {
   if (bc.nlist == 0) return hipSuccess;
-  if (!bc.vel) return hipErrorInvalidValue;
-  const unsigned gv = blocks_for(bc.nlist * 27, 256);
-  if (a.shifted) k27_bnd_vel<true><<<gv, 256, 0, s>>>(fin, b, bc);
-  else k27_bnd_vel<false><<<gv, 256, 0, s>>>(fin, b, bc);
   const unsigned gb = blocks_for(bc.nlist, 64);
   if (a.shifted) k27_boundary<true><<<gb, 64, 0, s>>>(fin, fout, alpha, keep_alpha, stats, b, a, bc);
   else k27_boundary<false><<<gb, 64, 0, s>>>(fin, fout, alpha, keep_alpha, stats, b, a, bc);

@@ -72,7 +72,6 @@ struct BcArgs {
   const double* feq_in;   // [n_in][27]
   const double* feq_out;  // [n_out][27]
   const double* feq_fs;   // [27] or null
-  double* vel;            // [3][nlist * 27]: u(x - c_q) of list node j at j * 27 + q
 };
 
 struct LinkGeom {

@@ -85,7 +85,6 @@ struct Level {
   int32_t* state = nullptr;   // [27][n]
   uint8_t* flags = nullptr;   // [n]
   uint32_t* blist = nullptr;  // boundary fluid nodes, ascending
-  double* bvel = nullptr;     // neighbour velocities of the boundary nodes, [3][nb * 27]
   int64_t nb = 0, nsolid = 0;
   bool has_links = false;
   // D3Q39: second population, multipliers, repair list, step counter
@@ -221,7 +220,6 @@ void free_level(Level& L) {
   if (L.state) { (void)hipFree(L.state); L.state = nullptr; }
   if (L.flags) { (void)hipFree(L.flags); L.flags = nullptr; }
   if (L.blist) { (void)hipFree(L.blist); L.blist = nullptr; }
-  if (L.bvel) { (void)hipFree(L.bvel); L.bvel = nullptr; }
   for (auto& p : L.g)
     if (p) { (void)hipFree(p); p = nullptr; }
   if (L.lm) { (void)hipFree(L.lm); L.lm = nullptr; }
@@ -281,7 +279,6 @@ uclbs::BcArgs bc_args(const Ctx* c, const Level* L) {
   bc.feq_in = c->d_feq_in;
   bc.feq_out = c->d_feq_out;
   bc.feq_fs = c->d_feq_fs;
-  bc.vel = L->bvel;
   return bc;
 }
 
@@ -339,12 +336,10 @@ int finalize_links(Ctx* c, Level* L) {
     nsolid += hf[k] == uclbs::kFlagSolid;
   }
   if (L->blist) { (void)hipFree(L->blist); L->blist = nullptr; }
-  if (L->bvel) { (void)hipFree(L->bvel); L->bvel = nullptr; }
   L->nb = (int64_t)list.size();
   L->nsolid = nsolid;
   HIP_TRY(alloc_far_queue(*L, L->nb));
   if (L->nb) {
-    HIP_TRY(hipMalloc(&L->bvel, (size_t)L->nb * 27 * 3 * sizeof(double)));
     HIP_TRY(hipMalloc(&L->blist, list.size() * sizeof(uint32_t)));
     HIP_TRY(hipMemcpy(L->blist, list.data(), list.size() * sizeof(uint32_t),
                       hipMemcpyHostToDevice));
