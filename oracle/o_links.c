@@ -392,3 +392,112 @@ int64_t o_build_links39(const o_box *bx, int ntri, const double *tri, const doub
   if (n_tie) *n_tie = ties;
   return nsolid;
 }
+
+/* D3Q39 table on the shifted lattice (c_x + 1): lbgrid/
+ * shifted_state_directions.f95:47-737.  Every link i of a fluid node x is
+ * keyed on its upstream node u = x - c_i (the node the pull reads):
+ *   u beyond a non-periodic face (first in the order x-, y-, z-, x+, y+, z+,
+ *     :60-329):  the wall code of the segment x -> u when u is a solid node
+ *     (inter_bb_setup.f95 with the ray reversed for the shifted lattice,
+ *     :37-47), else the face code; an outflow face (outlet type 6) gives the
+ *     code of the downstream node x + c_i (:88-89);
+ *   u inside (or wrapped across a periodic face):  the wall code when u is
+ *     solid, else the downstream node's code (:331-339);
+ *   a code of -1001 (a solid downstream node, or a wall segment that crosses
+ *     no triangle) becomes the level, 0 (:345).
+ * No long link is derived from a shorter one (each is ray-tested), and the
+ * shifted rest direction (Fortran 1) keeps the node's own code.  A node
+ * outside the box (an upstream beyond a face, a downstream beyond the box)
+ * is classified by the same ray cast as the box's nodes; an out-of-box
+ * downstream node's code is 0 (fluid) or, solid, -1001 -> 0 as above.
+ * state must hold 39 x n int32. */
+int64_t o_build_links39s(const o_box *bx, int ntri, const double *tri, const double *nrm,
+                         int32_t *state, int64_t *n_tie) {
+  const int64_t n = (int64_t)bx->nx * bx->ny * bx->nz;
+  memset(state, 0, sizeof(int32_t) * (size_t)(39 * n));
+  double *tbb = (double *)malloc(sizeof(double) * 6 * (size_t)(ntri > 0 ? ntri : 1));
+  double gbb[6] = {-1e300, 1e300, -1e300, 1e300, -1e300, 1e300};
+  for (int i = 0; i < ntri; i++) {
+    tri_bbox(tri + 9 * (size_t)i, tbb + 6 * (size_t)i);
+    for (int d = 0; d < 3; d++) {
+      if (tbb[6 * i + 2 * d] > gbb[2 * d]) gbb[2 * d] = tbb[6 * i + 2 * d];
+      if (tbb[6 * i + 2 * d + 1] < gbb[2 * d + 1]) gbb[2 * d + 1] = tbb[6 * i + 2 * d + 1];
+    }
+  }
+  int64_t ties = 0, nsolid = 0;
+#ifdef _OPENMP
+#pragma omp parallel for collapse(2) schedule(dynamic, 4) reduction(+ : ties, nsolid)
+#endif
+  for (int c = 0; c < bx->nz; c++)
+    for (int b = 0; b < bx->ny; b++)
+      for (int a = 0; a < bx->nx; a++) {
+        int64_t k = a + (int64_t)bx->nx * (b + (int64_t)bx->ny * c);
+        int tie = 0;
+        int solid = ntri > 0 ? o_classify_node(bx, ntri, tri, nrm, tbb, gbb, a, b, c, &tie) : 0;
+        state[k] = solid ? -1001 : 0;
+        ties += tie;
+        nsolid += solid;
+      }
+  const int nd[3] = {bx->nx, bx->ny, bx->nz};
+#ifdef _OPENMP
+#pragma omp parallel for collapse(2) schedule(dynamic, 4) reduction(+ : ties)
+#endif
+  for (int c = 0; c < bx->nz; c++)
+    for (int b = 0; b < bx->ny; b++)
+      for (int a = 0; a < bx->nx; a++) {
+        int64_t k = a + (int64_t)bx->nx * (b + (int64_t)bx->ny * c);
+        int32_t own = state[k];
+        for (int i = 1; i < 39; i++) {
+          int32_t code;
+          if (own == -1001 || own == -666) {
+            state[(int64_t)i * n + k] = -666;
+            continue;
+          }
+          const int cv[3] = {O39_CX[i] + 1, O39_CY[i], O39_CZ[i]};
+          const int x[3] = {a, b, c};
+          int up[3], dn[3];
+          for (int d = 0; d < 3; d++) { up[d] = x[d] - cv[d]; dn[d] = x[d] + cv[d]; }
+          int face = -1;
+          for (int f = 0; f < 6 && face < 0; f++) {
+            int d = f % 3;
+            int out = f < 3 ? up[d] < 0 : up[d] > nd[d] - 1;
+            if (out && bx->face_code[f] != O_FACE_PERIODIC) face = f;
+          }
+          for (int d = 0; d < 3; d++) {
+            if (up[d] < 0 && bx->face_code[d] == O_FACE_PERIODIC) up[d] += nd[d];
+            if (up[d] > nd[d] - 1 && bx->face_code[d + 3] == O_FACE_PERIODIC) up[d] -= nd[d];
+            if (dn[d] < 0 && bx->face_code[d] == O_FACE_PERIODIC) dn[d] += nd[d];
+            if (dn[d] > nd[d] - 1 && bx->face_code[d + 3] == O_FACE_PERIODIC) dn[d] -= nd[d];
+          }
+          int up_in = 1, dn_in = 1;
+          for (int d = 0; d < 3; d++) {
+            up_in = up_in && up[d] >= 0 && up[d] < nd[d];
+            dn_in = dn_in && dn[d] >= 0 && dn[d] < nd[d];
+          }
+          int up_solid;
+          if (up_in) {
+            up_solid = state[up[0] + (int64_t)bx->nx * (up[1] + (int64_t)bx->ny * up[2])] < -1000;
+          } else {
+            int tie = 0;
+            up_solid = ntri > 0 &&
+                       o_classify_node(bx, ntri, tri, nrm, tbb, gbb, up[0], up[1], up[2], &tie);
+            ties += tie;
+          }
+          int32_t down = dn_in ? state[dn[0] + (int64_t)bx->nx * (dn[1] + (int64_t)bx->ny * dn[2])] : 0;
+          if (up_solid) {
+            int tie = 0;
+            code = o_wall_code(bx, ntri, tri, nrm, tbb, a, b, c, -cv[0], -cv[1], -cv[2], &tie);
+            ties += tie;
+          } else if (face >= 0) {
+            code = bx->face_code[face] == -6 ? down : bx->face_code[face];
+          } else {
+            code = down;
+          }
+          if (code == -1001) code = 0;
+          state[(int64_t)i * n + k] = code;
+        }
+      }
+  free(tbb);
+  if (n_tie) *n_tie = ties;
+  return nsolid;
+}

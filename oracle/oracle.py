@@ -110,6 +110,8 @@ def lib() -> ctypes.CDLL:
         L.o_build_links27.restype = ctypes.c_int64
         L.o_build_links39.argtypes = L.o_build_links27.argtypes
         L.o_build_links39.restype = ctypes.c_int64
+        L.o_build_links39s.argtypes = L.o_build_links27.argtypes
+        L.o_build_links39s.restype = ctypes.c_int64
         L.o27_step_box.argtypes = [ctypes.POINTER(OBox), ctypes.POINTER(ctypes.c_int32), dp, dp,
                                    ctypes.POINTER(BoxParams27), dp, ctypes.POINTER(OStats),
                                    ctypes.POINTER(BcStats)]
@@ -326,6 +328,22 @@ def build_links39(shape, faces, tri, nrm, dx=1.0, origin=(0.0, 0.0, 0.0)):
                                _dp(nrm) if len(nrm) else None,
                                st.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                                ctypes.byref(ties))
+    return st, int(ns), int(ties.value)
+
+
+def build_links39s(shape, faces, tri, nrm, dx=1.0, origin=(0.0, 0.0, 0.0)):
+    """Reference-form D3Q39 link table (39, n) int32 on the shifted lattice
+    (every link keyed on its upstream node); returns (state, n_solid, n_tie)."""
+    b = make_box(shape, faces, dx, origin)
+    n = shape[0] * shape[1] * shape[2]
+    st = np.empty((39, n), dtype=np.int32)
+    tri = np.ascontiguousarray(tri, dtype=np.float64)
+    nrm = np.ascontiguousarray(nrm, dtype=np.float64)
+    ties = ctypes.c_int64()
+    ns = lib().o_build_links39s(ctypes.byref(b), len(tri), _dp(tri) if len(tri) else None,
+                                _dp(nrm) if len(nrm) else None,
+                                st.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                ctypes.byref(ties))
     return st, int(ns), int(ties.value)
 
 

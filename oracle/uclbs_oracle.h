@@ -103,6 +103,8 @@ int64_t o_build_links27(const o_box *bx, int ntri, const double *tri, const doub
                         int32_t *state, int64_t *n_tie);
 int64_t o_build_links39(const o_box *bx, int ntri, const double *tri, const double *nrm,
                         int32_t *state, int64_t *n_tie);
+int64_t o_build_links39s(const o_box *bx, int ntri, const double *tri, const double *nrm,
+                         int32_t *state, int64_t *n_tie);
 /* geometry helpers of the table builder, exported for checks against the
  * reference's own routines: tri_volume.f95, box_test.f95 (bb = xmax, xmin,
  * ymax, ymin, zmax, zmin), linkwise.f95:45-57 */

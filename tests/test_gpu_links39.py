@@ -1,8 +1,11 @@
-"""D3Q39 link table on the device (plain lattice, state_directions.f95 with the
-long-link derivation) against the oracle: bit-exact int32 equality of the
-whole reference-form table, on a walled box with a sphere and on a box with a
-periodic axis; the shifted lattice and stepping a level that holds a D3Q39
-table fail loudly."""
+"""D3Q39 link tables on the device against the oracle: bit-exact int32
+equality of the whole reference-form table -- the plain lattice
+(state_directions.f95 with the long-link derivation) on a walled box with a
+sphere and on a box with a periodic axis, and the shifted lattice
+(shifted_state_directions.f95, links keyed on the upstream node) with a
+sphere, with a periodic axis, and with a body cut by a face (upstream nodes
+beyond the face classified by the ray cast); stepping a plain-lattice level
+that holds a table fails loudly (no plain-lattice D3Q39 wall rule)."""
 import numpy as np
 import pytest
 
@@ -35,12 +38,27 @@ def test_d3q39_links_bit_exact(pkg, orc, faces):
     assert np.count_nonzero(st_o[15:21] < -1000) > 50 and np.count_nonzero(st_o[34] == 666) > 0
 
 
+@pytest.mark.parametrize("faces,center", [
+    ([-1000, -11, -11, -6, -11, -11], (17.5, 15.25, 13.75)),
+    ([-1000, -11, P, -6, -11, P], (17.5, 15.25, 13.75)),
+    ([-1000, -11, -11, -6, -11, -11], (17.5, 2.25, 13.75))])   # sphere cut by the y- face
+def test_d3q39_shifted_links_bit_exact(pkg, orc, faces, center):
+    shape = (40, 30, 28)
+    tri, nrm = pkg.icosphere(5, 8.3, center, 20260102)
+    st_o, ns_o, nt_o = orc.build_links39s(shape, faces, tri, nrm)
+    s = _solver(pkg, shape, faces, mach=1.5, shifted=1)
+    ns_g, nt_g = s.build_links(0, tri, nrm)
+    st_g = s.download_links(0)
+    print(f"shifted faces {faces} centre {center}: solid {ns_g}, wall links "
+          f"{int(np.count_nonzero(st_o[1:] <= -1001))}, ties {nt_g}", flush=True)
+    assert ns_g == ns_o and nt_g == nt_o
+    assert np.array_equal(st_g, st_o)
+    assert np.count_nonzero(st_o[1:] <= -1001) > 1000
+
+
 def test_d3q39_links_refusals(pkg):
     shape = (20, 16, 16)
     tri, nrm = pkg.icosphere(3, 4.0, (9.5, 8.25, 7.75), 1)
-    s = _solver(pkg, shape, [-1000, -11, -11, -6, -11, -11], mach=1.5, shifted=1)
-    with pytest.raises(pkg.UclbsError):
-        s.build_links(0, tri, nrm)             # shifted_state_directions: not built
     s = _solver(pkg, shape, [-1000, -10, -11, -6, -10, -11])
     s.build_links(0, tri, nrm)
     s.init_fields(0, 1.0, 0.3, 0.0, 0.0, 1 / 1.4)

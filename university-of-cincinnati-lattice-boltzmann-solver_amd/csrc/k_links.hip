@@ -107,13 +107,9 @@ __constant__ double kRecast[3][3] = {{0.5773502691896258, 0.6311775151936213, 0.
                                      {-0.7071067811865476, 0.3826834323650898, 0.5946035575013605},
                                      {0.2672612419124244, -0.5345224838248488, 0.8017837257372732}};
 
-__global__ __launch_bounds__(256) void k_classify(LinkGeom g, int32_t* __restrict__ state,
-                                                  unsigned long long* __restrict__ counts) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= g.box.n) return;
-  const int a = (int)(k % g.box.nx);
-  const int64_t yz = k / g.box.nx;
-  const int b = (int)(yz % g.box.ny), c = (int)(yz / g.box.ny);
+// fluid (0) or solid (1) of the node at index position (a, b, c) -- inside the
+// box or not -- by the ray cast of fluid_or_solid.f95; *tie set on a re-cast
+__device__ int classify_pos(const LinkGeom& g, int a, int b, int c, int* tie) {
   const double e1[3] = {g.origin[0] + (double)a * g.dx, g.origin[1] + (double)b * g.dx,
                         g.origin[2] + (double)c * g.dx};
   double cen[3], ext = 0.0;
@@ -126,7 +122,8 @@ __global__ __launch_bounds__(256) void k_classify(LinkGeom g, int32_t* __restric
   const double L = fabs(100.0 * ext);
   const double dl[3] = {e1[0] - cen[0], e1[1] - cen[1], e1[2] - cen[2]};
   const double dist = sqrt(dl[0] * dl[0] + dl[1] * dl[1] + dl[2] * dl[2]);
-  int solid = 0, tie = 0;
+  int solid = 0;
+  *tie = 0;
   if (g.ntri > 0) {
     for (int attempt = 0; attempt < 4; ++attempt) {
       double e2[3];
@@ -135,14 +132,26 @@ __global__ __launch_bounds__(256) void k_classify(LinkGeom g, int32_t* __restric
       } else {
         const int r = attempt == 0 ? 0 : attempt - 1;
         for (int d = 0; d < 3; ++d) e2[d] = e1[d] + L * kRecast[r][d];
-        if (attempt == 0) tie = 1;
+        if (attempt == 0) *tie = 1;
       }
       if (box_out(e1, e2, g.gbb)) { solid = 0; break; }
       const int p = cast_parity(e1, e2, g.ntri, g.tri, g.nrm, g.tbb);
       if (p >= 0) { solid = p; break; }
-      tie = 1;
+      *tie = 1;
     }
   }
+  return solid;
+}
+
+__global__ __launch_bounds__(256) void k_classify(LinkGeom g, int32_t* __restrict__ state,
+                                                  unsigned long long* __restrict__ counts) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= g.box.n) return;
+  const int a = (int)(k % g.box.nx);
+  const int64_t yz = k / g.box.nx;
+  const int b = (int)(yz % g.box.ny), c = (int)(yz / g.box.ny);
+  int tie = 0;
+  const int solid = classify_pos(g, a, b, c, &tie);
   state[k] = solid ? -1001 : 0;
   if (solid) atomicAdd(counts + 0, 1ull);
   if (tie) atomicAdd(counts + 1, 1ull);
@@ -296,6 +305,74 @@ __global__ __launch_bounds__(256) void k_links39(LinkGeom g, int32_t* __restrict
   if (ties) atomicAdd(counts + 1, (unsigned long long)ties);
 }
 
+// D3Q39 table on the shifted lattice (lbgrid/shifted_state_directions.f95):
+// every link i of a fluid node x is keyed on its upstream node u = x - c_i
+// (c_x shifted by +1): the wall code of the segment x -> u when u is solid
+// (inside the box, or outside it by the same ray cast), else the code of the
+// first non-periodic face u lies beyond (x-, y-, z-, x+, y+, z+; an outflow
+// face, -6, gives the downstream node's code), else the downstream node
+// x + c_i's code; -1001 becomes 0 (a solid downstream node, or a segment that
+// crosses no triangle).  No link is derived from a shorter one.
+__global__ __launch_bounds__(256) void k_links39s(LinkGeom g, int32_t* __restrict__ state,
+                                                  unsigned long long* __restrict__ counts) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= g.box.n) return;
+  const int nd[3] = {g.box.nx, g.box.ny, g.box.nz};
+  const int x[3] = {(int)(k % nd[0]), (int)((k / nd[0]) % nd[1]), (int)(k / ((int64_t)nd[0] * nd[1]))};
+  const int64_t n = g.box.n;
+  const int32_t own = state[k];
+  int ties = 0;
+  for (int i = 1; i < 39; ++i) {
+    int32_t* slot = state + (int64_t)i * n + k;
+    if (own == -1001 || own == -666) {
+      *slot = -666;
+      continue;
+    }
+    const int cv[3] = {CX39[i] + 1, CY39[i], CZ39[i]};
+    int up[3], dn[3];
+    for (int d = 0; d < 3; ++d) {
+      up[d] = x[d] - cv[d];
+      dn[d] = x[d] + cv[d];
+    }
+    int face = -1;
+    for (int f = 0; f < 6 && face < 0; ++f) {
+      const int d = f % 3;
+      const bool out = f < 3 ? up[d] < 0 : up[d] > nd[d] - 1;
+      if (out && g.face_code[f] != kFacePeriodic) face = f;
+    }
+    bool up_in = true, dn_in = true;
+    for (int d = 0; d < 3; ++d) {
+      if (up[d] < 0 && g.face_code[d] == kFacePeriodic) up[d] += nd[d];
+      if (up[d] > nd[d] - 1 && g.face_code[d + 3] == kFacePeriodic) up[d] -= nd[d];
+      if (dn[d] < 0 && g.face_code[d] == kFacePeriodic) dn[d] += nd[d];
+      if (dn[d] > nd[d] - 1 && g.face_code[d + 3] == kFacePeriodic) dn[d] -= nd[d];
+      up_in = up_in && up[d] >= 0 && up[d] < nd[d];
+      dn_in = dn_in && dn[d] >= 0 && dn[d] < nd[d];
+    }
+    int up_solid;
+    if (up_in) {
+      up_solid = state[up[0] + (int64_t)nd[0] * (up[1] + (int64_t)nd[1] * up[2])] < -1000;
+    } else {
+      int tie = 0;
+      up_solid = classify_pos(g, up[0], up[1], up[2], &tie);
+      ties += tie;
+    }
+    const int32_t down = dn_in ? state[dn[0] + (int64_t)nd[0] * (dn[1] + (int64_t)nd[1] * dn[2])] : 0;
+    int32_t code;
+    if (up_solid) {
+      int tie = 0;
+      code = wall_code(g, x[0], x[1], x[2], -cv[0], -cv[1], -cv[2], &tie);
+      ties += tie;
+    } else if (face >= 0) {
+      code = g.face_code[face] == -6 ? down : g.face_code[face];
+    } else {
+      code = down;
+    }
+    *slot = code == -1001 ? 0 : code;
+  }
+  if (ties) atomicAdd(counts + 1, (unsigned long long)ties);
+}
+
 // node flags: interior fluid (every link a fluid code), face (fluid whose
 // negative links all leave the box with exactly the face's code, so the
 // streaming kernel can rebuild them from coordinates), boundary (any other
@@ -353,6 +430,17 @@ hipError_t launch_build_links39(const LinkGeom& g, int32_t* state, unsigned long
   e = hipGetLastError();
   if (e != hipSuccess) return e;
   k_links39<<<blocks_for(g.box.n, 256), 256, 0, s>>>(g, state, counts);
+  return hipGetLastError();
+}
+
+hipError_t launch_build_links39s(const LinkGeom& g, int32_t* state, unsigned long long* counts,
+                                 hipStream_t s) {
+  hipError_t e = hipMemsetAsync(state, 0, (size_t)g.box.n * 39 * sizeof(int32_t), s);
+  if (e != hipSuccess) return e;
+  k_classify<<<blocks_for(g.box.n, 256), 256, 0, s>>>(g, state, counts);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  k_links39s<<<blocks_for(g.box.n, 256), 256, 0, s>>>(g, state, counts);
   return hipGetLastError();
 }
 

@@ -90,6 +90,9 @@ hipError_t launch_build_links27(const LinkGeom& g, int32_t* state, unsigned long
                                 hipStream_t s);
 hipError_t launch_build_links39(const LinkGeom& g, int32_t* state, unsigned long long* counts,
                                 hipStream_t s);
+// shifted lattice (c_x + 1): links keyed on their upstream node
+hipError_t launch_build_links39s(const LinkGeom& g, int32_t* state, unsigned long long* counts,
+                                 hipStream_t s);
 hipError_t launch_flags27(const int32_t* state, const Box& b, const int32_t face_code[6],
                           uint8_t* flags, hipStream_t s);
 hipError_t launch_d3q27_step_links(const Box& b, const void* fin, void* fout, double* alpha,

@@ -1277,9 +1277,6 @@ int uclbs_build_links(void* ctx, int lvl, const double origin[3], double dx, int
   if (L->multibox) return set_err(UCLBS_E_UNSUPPORTED, "no link tables on levels of many boxes");
   if (L->refined) return set_err(UCLBS_E_UNSUPPORTED, "level %d is a refined level (no link tables)", lvl);
   const bool q39 = c->cfg.lattice == UCLBS_LATTICE_D3Q39;
-  if (q39 && c->shifted39)
-    return set_err(UCLBS_E_UNSUPPORTED, "build_links: D3Q39 tables of the plain lattice only "
-                                        "(shifted_state_directions is not part of this library)");
   if (!origin || !(dx > 0.0) || ntri < 0 || ntri > (1 << 26) || (ntri && (!tri || !nrm)))
     return set_err(UCLBS_E_ARG, "build_links: bad geometry arguments");
   if ((rc = set_device(c))) return rc;
@@ -1291,7 +1288,8 @@ int uclbs_build_links(void* ctx, int lvl, const double origin[3], double dx, int
   for (int f = 0; f < 6; ++f) {
     // an outflow face (outlet type 6) gives its links the code of the node
     // outside, a fluid node of level 0 (state_directions_lvl.f95:131-132)
-    g.face_code[f] = (q39 && L->face_code[f] == -6) ? 0 : L->face_code[f];
+    // (the shifted lattice's outflow links take the downstream node's code)
+    g.face_code[f] = (q39 && !c->shifted39 && L->face_code[f] == -6) ? 0 : L->face_code[f];
   }
   g.ntri = (int)ntri;
   std::vector<double> tbb((size_t)(ntri > 0 ? ntri : 1) * 6);
@@ -1324,8 +1322,9 @@ int uclbs_build_links(void* ctx, int lvl, const double origin[3], double dx, int
     g.tri = dtri;
     g.nrm = dnrm;
     g.tbb = dtbb;
-    e = q39 ? uclbs::launch_build_links39(g, L->state, dcnt, c->stream)
-            : uclbs::launch_build_links27(g, L->state, dcnt, c->stream);
+    e = !q39 ? uclbs::launch_build_links27(g, L->state, dcnt, c->stream)
+        : c->shifted39 ? uclbs::launch_build_links39s(g, L->state, dcnt, c->stream)
+                       : uclbs::launch_build_links39(g, L->state, dcnt, c->stream);
   }
   if (e == hipSuccess) e = hipMemcpyAsync(hc, dcnt, sizeof hc, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
