@@ -11,9 +11,10 @@ entropic alpha (FP64 Newton) + relaxation kernel over the interior nodes plus
 the boundary-node kernel, populations FP32 stored as deviations from the
 lattice weights, already resident in HBM.  `--workload tgv` runs the periodic
 Taylor-Green box instead, `--workload c3` the D3Q39 two-population
-compressible step on a 384 x 192 x 96 box at Mach 1.5 (shifted lattice,
-freestream inflow, pure outflow, inviscid sidewalls; the wedge geometry is not
-included), reported against its algorithmic 728 B per node update.
+compressible step of the supersonic wedge case (384 x 192 x 96, Mach 1.5,
+shifted lattice, 10 degree wedge with the shifted-lattice wall rule, 2000
+steps with the shock angle measured at the end), reported against its
+algorithmic 728 B per node update.
 
 Prints ONE JSON line.  `value` is MLUPS over the timed
 region (wall clock, synchronised on both sides); `roofline` uses the
@@ -163,26 +164,36 @@ def learned_alpha(pkg, s, L, nu, collect_steps=3, train_steps=4000, batch=8192, 
             "learned_mlups": round(out["learned"]["mlups"], 1)}
 
 
+C3_SHAPE = (384, 192, 96)
+C3_FACES = [-1000, -1000, -11, -6, -1000, -11]   # freestream x-/y-/y+, outflow x+, inviscid z
+C3_APEX, C3_LENGTH, C3_HALF_ANGLE, C3_SEED = (96.3, 96.27), 64.0, 10.0, 20260103
+C3_STEPS = 2000
+
+
 def bench_c3(pkg, args):
-    """D3Q39 f + g step, FP32 deviations from the freestream equilibrium, FP64
-    multipliers; one JSON line in the same form (not the headline metric)."""
-    import numpy as np
-    shape = (384, 192, 96)
+    """D3Q39 supersonic wedge: 384 x 192 x 96 shifted lattice at Mach 1.5, a
+    10 degree half-angle wedge (seeded triangulation, 37,248 triangles, apex
+    (96.3, 96.27), length 64, spanning z) with its wall links rebuilt by the
+    shifted-lattice wall rule every step, freestream inflow, outflow, inviscid
+    sidewalls; FP32 deviations from the freestream equilibrium, FP64
+    multipliers.  From the uniform start: `warmup` untimed steps, `steps`
+    timed, the rest of the 2000-step run untimed; then the shock angle and
+    shock-layer density on the z mid-plane against theta-beta-Mach."""
+    import importlib
+    sh = importlib.import_module("uclbs_amd.shock")
+    shape = C3_SHAPE
     n = shape[0] * shape[1] * shape[2]
-    nx, ny, nz = shape
-    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
-    x, y, z = x.ravel(), y.ravel(), z.ravel()
     T0 = 1 / 1.4
     t_setup = time.perf_counter()
     s = pkg.Solver(lattice="d3q39", precision="fp32s")
     s.set_gas(mach=1.5, shifted=1, turbulent=True)
     s.define_level(0, shape, periodic=(0, 0, 0))
-    s.set_faces(0, [-1000, -11, -11, -6, -11, -11])  # freestream in, outflow, inviscid sides
-    s.init_fields(0, 1.0 + 0.01 * np.sin(2 * np.pi * (x / nx + y / ny)),
-                  1.5 + 0.02 * np.sin(2 * np.pi * x / nx), 0.01 * np.sin(2 * np.pi * y / ny),
-                  0.01 * np.cos(2 * np.pi * (x / nx + z / nz)),
-                  T0 * (1 + 0.01 * np.cos(2 * np.pi * z / nz)))
-    s.set_timestep(0, 100)
+    s.set_faces(0, C3_FACES)
+    tri, nrm = pkg.wedge(C3_APEX, C3_LENGTH, C3_HALF_ANGLE, (-5.0, shape[2] + 5.0), 64, 96,
+                         C3_SEED)
+    n_solid, _ = s.build_links(0, tri, nrm)
+    s.init_fields(0, 1.0, 1.5, 0.0, 0.0, T0)
+    s.set_timestep(0, 0)
     setup_s = time.perf_counter() - t_setup
     s.step(0, args.warmup)
     s.reset_stats()
@@ -192,26 +203,39 @@ def bench_c3(pkg, args):
     s.sync()
     wall = time.perf_counter() - t0
     st = s.stats()
+    rest = max(0, C3_STEPS - args.warmup - args.steps)
+    s.step(0, rest)
+    rho = s.macro39(0)[0].reshape(shape[2], shape[1], shape[0])[shape[2] // 2]
+    solid = (s.download_links(0)[0] < 0).reshape(shape[2], shape[1], shape[0])[shape[2] // 2]
+    beta, rho_rh, _ = sh.oblique_shock(1.5, C3_HALF_ANGLE)
+    ang, post, ncol = sh.shock_fit(rho, solid, C3_APEX, C3_HALF_ANGLE, (8, 56))
     bpn = 2 * 39 * 4 * 2 + 13 * 4 * 2
     avg = main_ms / 1e3 / args.steps
     achieved = bpn * n / avg / 1e9
-    out = {"metric": "MLUPS, D3Q39 two-population step (secondary workload)",
+    out = {"metric": "MLUPS, D3Q39 supersonic wedge step (secondary workload)",
            "value": round(n * args.steps / wall / 1e6, 1), "unit": "MLUPS", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup,
            "ms_per_step": round(wall * 1e3 / args.steps, 4), "higher_is_better": True,
            "scaling": "weak", "vs_baseline": None,
            "dtype": "f32 populations (shifted), f64 multipliers/moments/alpha",
            "data": "synthetic",
-           "config": {"workload": "D3Q39 384x192x96, Mach 1.5 shifted lattice, freestream "
-                                  "inflow x-, pure outflow x+, inviscid y/z sides, perturbed "
-                                  "freestream, alpha estimate on (no wedge)",
-                      "nodes": n, "lattice": "D3Q39", "setup_s": round(setup_s, 2)},
+           "config": {"workload": "D3Q39 384x192x96, Mach 1.5 shifted lattice, 10 degree wedge "
+                                  "(seed 20260103), freestream inflow x- and far field y, "
+                                  "outflow x+, inviscid z sidewalls, uniform start",
+                      "nodes": n, "solid_nodes": n_solid, "lattice": "D3Q39",
+                      "setup_s": round(setup_s, 2), "total_steps": args.warmup + args.steps + rest},
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                        "traffic": None, "kernel": "k39_step",
+                        "traffic": None,
+                        "kernel": "one D3Q39 step: wall rebuild, k39_step (interior, near-face "
+                                  "and wall nodes), repair pass",
                         "avg_launch_ms": round(avg * 1e3, 4)},
+           "shock": {"angle_deg": round(ang, 3), "theory_deg": round(beta, 3),
+                     "layer_rho": round(post, 4), "rankine_hugoniot_rho": round(rho_rh, 4),
+                     "columns": ncol, "at_step": args.warmup + args.steps + rest},
            "lm_stats": {"newton_iters_per_node": round(st["newton_iters"] / max(1, st["node_updates"]), 4),
                         "n_noconv": st["n_noconv"], "n_repair": st["n_repair"],
+                        "n_wall_links": st["n_ibb_links"],
                         "n_alpha_trivial": st["n_trivial"], "n_alpha_bound": st["n_bound"]}}
     print(json.dumps(out), flush=True)
 

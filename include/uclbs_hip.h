@@ -169,6 +169,18 @@ int uclbs_init_uniform(void *ctx, int lvl, double rho, double u, double v, doubl
  * generated stand-in for the STL import of geometry/geom_import.f95:48-133. */
 int uclbs_geom_icosphere(int level, double radius, const double center[3], uint64_t seed,
                          double *tri, double *nrm, int64_t capacity, int64_t *ntri);
+/* Symmetric wedge extruded along z: apex line at (apex_x, apex_y), faces at
+ * +-half_angle_deg to the x axis, base at apex_x + length, spanning z0..z1.
+ * Each side face is an n_len x n_span grid of triangle pairs whose interior
+ * vertices are moved inside the face plane by Philox uniforms from the seed
+ * (the surface is exact, the triangulation seeded); the end caps are fans over
+ * the side faces' edge points, so the soup is watertight.  6*n_len*n_span +
+ * 6*n_len triangles, outward normals; with tri or nrm NULL only *ntri is set.
+ * A generated stand-in for the STL import of geometry/geom_import.f95:48-133
+ * (the reference's supersonic wedge case). */
+int uclbs_geom_wedge(double apex_x, double apex_y, double length, double half_angle_deg,
+                     double z0, double z1, int n_len, int n_span, uint64_t seed, double *tri,
+                     double *nrm, int64_t capacity, int64_t *ntri);
 
 /* ---- D3Q39 two-population compressible model ----------------------------
  * A context created with lattice = UCLBS_LATTICE_D3Q39 advances f and g
@@ -208,6 +220,12 @@ int uclbs_init_fields(void *ctx, int lvl, const double *rho, const double *u, co
 int uclbs_upload_state(void *ctx, int lvl, const double *f, const double *g, const double *lm,
                        int64_t n_nodes);
 int uclbs_download_state(void *ctx, int lvl, double *f, double *g, double *lm, int64_t n_nodes);
+/* Flow variables of the current f, g per node, computed on the device as
+ * calculate_flow_variables.f95:201-221 does: rho, u, v, w, temp and press =
+ * rho (temp + |u|^2 / 2) (the reference's press, :221); any output may be
+ * NULL.  n_nodes FP64 each, layout as f. */
+int uclbs_download_macro39(void *ctx, int lvl, double *rho, double *u, double *v, double *w,
+                           double *temp, double *press, int64_t n_nodes);
 /* ---- refined levels ---------------------------------------------------------
  * uclbs_define_level with lvl >= 1 defines a box nested in level lvl-1 at
  * refinement ratio 2 (lo/hi in level lvl's own node indices; node G of level

@@ -146,6 +146,9 @@ def lib() -> ctypes.CDLL:
         L.o39_source.restype = I
         L.o39_step_faces.argtypes = [I, I, I, ip, dp, dp, dp, dp, dp, dp, c39, ctypes.c_int64, I,
                                      I, I, ctypes.POINTER(Stats39)]
+        L.o39_step_walls.argtypes = [I, I, I, ip, ip, dp, dp, dp, dp, dp, dp, c39, ctypes.c_int64,
+                                     ctypes.c_int64, I, I, ctypes.POINTER(Stats39),
+                                     ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
         for nm in ("o_tri_volume", "o_box_test"):
             getattr(L, nm).argtypes = [dp, dp, dp]
         L.o_link_code.argtypes = [dp, dp, D]
@@ -566,6 +569,30 @@ def step_faces39(shape, faces, f, g, lm, c, step0=0, nsteps=1, ref_lu=False, jac
                          _dp(alpha) if alpha is not None else None, ctypes.byref(c), step0,
                          nsteps, int(ref_lu), int(jacobi), ctypes.byref(st))
     return f, g, lm, alpha, st.as_dict()
+
+
+def step_walls39(shape, faces, state, f, g, lm, c, step0=0, nsteps=1, lvl_step0=None,
+                 jacobi=True, keep_alpha=False):
+    """Steps of a shifted-lattice box with a link table (build_links39s):
+    wall links rebuilt by the reference's shifted-lattice wall rule.  Returns
+    (f, g, lm, alpha, stats, n_wall_links, n_wall_repairs)."""
+    nx, ny, nz = shape
+    fc = _faces39(faces, bool(c.shifted))
+    fsf, fsg = freestream39(c)
+    f = np.array(f, np.float64)
+    g = np.array(g, np.float64)
+    lm = np.array(lm, np.float64)
+    st_ = np.ascontiguousarray(state, np.int32)
+    alpha = np.zeros(nx * ny * nz) if keep_alpha else None
+    st = Stats39()
+    nwl, nwr = ctypes.c_int64(), ctypes.c_int64()
+    ip = ctypes.POINTER(ctypes.c_int32)
+    lib().o39_step_walls(nx, ny, nz, fc.ctypes.data_as(ip), st_.ctypes.data_as(ip), _dp(fsf),
+                         _dp(fsg), _dp(f), _dp(g), _dp(lm),
+                         _dp(alpha) if alpha is not None else None, ctypes.byref(c), step0,
+                         step0 if lvl_step0 is None else lvl_step0, nsteps, int(jacobi),
+                         ctypes.byref(st), ctypes.byref(nwl), ctypes.byref(nwr))
+    return f, g, lm, alpha, st.as_dict(), nwl.value, nwr.value
 
 
 def source39(node, k, shape, faces, shifted=False):

@@ -25,7 +25,7 @@ EXPORTS = (
     "uclbs_step_level", "uclbs_step_level_async", "uclbs_time_steps", "uclbs_sync",
     "uclbs_get_stats", "uclbs_reset_stats", "uclbs_set_faces", "uclbs_set_bc_states",
     "uclbs_build_links", "uclbs_set_links", "uclbs_download_links", "uclbs_link_counts",
-    "uclbs_init_uniform", "uclbs_geom_icosphere", "uclbs_time_kernels",
+    "uclbs_init_uniform", "uclbs_geom_icosphere", "uclbs_geom_wedge", "uclbs_download_macro39", "uclbs_time_kernels",
     "uclbs_copy_bandwidth", "uclbs_set_gas", "uclbs_init_fields", "uclbs_upload_state",
     "uclbs_download_state", "uclbs_set_timestep", "uclbs_mlp_init", "uclbs_mlp_set_params",
     "uclbs_mlp_get_params", "uclbs_mlp_loss_grad", "uclbs_mlp_forward", "uclbs_mlp_reserve",
@@ -118,6 +118,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "uclbs_link_counts": (I, [P, I, ctypes.POINTER(I64), ctypes.POINTER(I64)]),
         "uclbs_init_uniform": (I, [P, I, D, D, D, D]),
         "uclbs_geom_icosphere": (I, [I, D, dp, U64, dp, dp, I64, ctypes.POINTER(I64)]),
+        "uclbs_geom_wedge": (I, [D, D, D, D, D, D, I, I, U64, dp, dp, I64, ctypes.POINTER(I64)]),
+        "uclbs_download_macro39": (I, [P, I, dp, dp, dp, dp, dp, dp, I64]),
         "uclbs_time_kernels": (I, [P, I, I, dp, dp]),
         "uclbs_copy_bandwidth": (I, [I64, I, dp]),
         "uclbs_set_gas": (I, [P, ctypes.POINTER(Gas)]),
@@ -179,6 +181,25 @@ def icosphere(level: int, radius: float, center, seed: int):
                                   nt.value, ctypes.byref(nt))
     if rc:
         raise UclbsError(f"icosphere failed ({rc})")
+    return tri, nrm
+
+
+def wedge(apex, length: float, half_angle_deg: float, z_span, n_len: int, n_span: int,
+          seed: int):
+    """Triangle soup (tri (T,9), nrm (T,3)) of a symmetric wedge extruded along
+    z with a seeded triangulation of its faces (uclbs_geom_wedge)."""
+    lib = load_library()
+    nt = ctypes.c_int64()
+    args = (float(apex[0]), float(apex[1]), float(length), float(half_angle_deg),
+            float(z_span[0]), float(z_span[1]), int(n_len), int(n_span), int(seed))
+    rc = lib.uclbs_geom_wedge(*args, None, None, 0, ctypes.byref(nt))
+    if rc:
+        raise UclbsError(f"wedge size query failed ({rc})")
+    tri = np.empty((nt.value, 9))
+    nrm = np.empty((nt.value, 3))
+    rc = lib.uclbs_geom_wedge(*args, _dptr(tri), _dptr(nrm), nt.value, ctypes.byref(nt))
+    if rc:
+        raise UclbsError(f"wedge failed ({rc})")
     return tri, nrm
 
 
@@ -359,6 +380,14 @@ class Solver:
         return f, g, lm
 
     # ---- learned alpha ---------------------------------------------------------
+    def macro39(self, lvl: int):
+        """(rho, u, v, w, T, press) of a D3Q39 level, computed on the device
+        (press = rho (T + |u|^2 / 2) as calculate_flow_variables.f95:221)."""
+        n = self.n_nodes(lvl)
+        arrs = [np.empty(n, dtype=np.float64) for _ in range(6)]
+        self._check(self.lib.uclbs_download_macro39(self.ctx, lvl, *[_dptr(a) for a in arrs], n))
+        return arrs
+
     def mlp_init(self, seed: int):
         self._check(self.lib.uclbs_mlp_init(self.ctx, seed))
 

@@ -101,3 +101,93 @@ extern "C" int uclbs_geom_icosphere(int level, double radius, const double cente
   }
   return UCLBS_OK;
 }
+
+namespace {
+
+// point of a wedge face from its parameters: corners p0 -> p1 along the face,
+// z0 -> z1 across the span; s, t exactly 0 or 1 give the corner lines exactly,
+// so faces that share an edge share its vertices bit for bit
+V3 face_point(V3 p0, V3 p1, double s, double z0, double z1, double t) {
+  V3 p = add(scl(p0, 1.0 - s), scl(p1, s));
+  p.z = z0 * (1.0 - t) + z1 * t;
+  return p;
+}
+
+}  // namespace
+
+extern "C" int uclbs_geom_wedge(double apex_x, double apex_y, double length,
+                                double half_angle_deg, double z0, double z1, int n_len,
+                                int n_span, uint64_t seed, double* tri, double* nrm,
+                                int64_t capacity, int64_t* ntri) {
+  if (!(length > 0.0) || !(half_angle_deg > 0.0 && half_angle_deg < 60.0) || !(z1 > z0) ||
+      n_len < 1 || n_span < 1 || n_len > 4096 || n_span > 4096 || !ntri)
+    return UCLBS_E_ARG;
+  const int64_t count = 6 * (int64_t)n_len * n_span + 6 * (int64_t)n_len;
+  *ntri = count;
+  if (!tri || !nrm) return UCLBS_OK;  // size query
+  if (capacity < count) return UCLBS_E_ARG;
+  const double h = length * std::tan(half_angle_deg * 3.14159265358979323846 / 180.0);
+  const V3 A{apex_x, apex_y, 0.0}, B{apex_x + length, apex_y + h, 0.0},
+      C{apex_x + length, apex_y - h, 0.0};
+  const V3 cen{(A.x + B.x + C.x) / 3.0, (A.y + B.y + C.y) / 3.0, 0.5 * (z0 + z1)};
+  int64_t w = 0;
+  auto emit = [&](V3 a, V3 b, V3 d) {
+    V3 n = cross(sub(b, a), sub(d, a));
+    if (dot(n, sub(a, cen)) < 0.0) {  // outward orientation (the wedge is convex)
+      std::swap(b, d);
+      n = scl(n, -1.0);
+    }
+    n = unit(n);
+    const V3 vs[3] = {a, b, d};
+    for (int k = 0; k < 3; ++k) {
+      tri[w * 9 + k * 3 + 0] = vs[k].x;
+      tri[w * 9 + k * 3 + 1] = vs[k].y;
+      tri[w * 9 + k * 3 + 2] = vs[k].z;
+    }
+    nrm[w * 3 + 0] = n.x;
+    nrm[w * 3 + 1] = n.y;
+    nrm[w * 3 + 2] = n.z;
+    ++w;
+  };
+  // the three side faces (upper A->B, base B->C, lower C->A), each an
+  // n_len x n_span grid whose interior vertices are moved inside the face
+  // plane by up to a quarter cell (Philox uniforms from the seed): the surface
+  // is the exact prism whatever the seed, the triangle edges are not
+  const V3 corner[4] = {A, B, C, A};
+  std::vector<V3> grid((size_t)(n_len + 1) * (n_span + 1));
+  for (int f = 0; f < 3; ++f) {
+    for (int j = 0; j <= n_span; ++j)
+      for (int i = 0; i <= n_len; ++i) {
+        double s = (double)i / n_len, t = (double)j / n_span;
+        if (i > 0 && i < n_len && j > 0 && j < n_span) {
+          const uint64_t id = ((uint64_t)f * 4097 + (uint64_t)j) * 4097 + (uint64_t)i;
+          s += (uclbs::philox_uniform(id, 0, seed) - 0.5) * 0.5 / n_len;
+          t += (uclbs::philox_uniform(id, 1, seed) - 0.5) * 0.5 / n_span;
+        }
+        if (i == n_len) s = 1.0;
+        if (j == n_span) t = 1.0;
+        grid[(size_t)j * (n_len + 1) + i] = face_point(corner[f], corner[f + 1], s, z0, z1, t);
+      }
+    for (int j = 0; j < n_span; ++j)
+      for (int i = 0; i < n_len; ++i) {
+        const V3 v00 = grid[(size_t)j * (n_len + 1) + i], v10 = grid[(size_t)j * (n_len + 1) + i + 1];
+        const V3 v01 = grid[(size_t)(j + 1) * (n_len + 1) + i],
+                 v11 = grid[(size_t)(j + 1) * (n_len + 1) + i + 1];
+        emit(v00, v10, v11);
+        emit(v00, v11, v01);
+      }
+  }
+  // end caps: fans from the cap centroid over the boundary points the side
+  // faces put on the edges at z0 and z1 (no T-junctions)
+  for (int e = 0; e < 2; ++e) {
+    const double t = (double)e, ze = e ? z1 : z0;
+    const V3 cc{cen.x, cen.y, ze};
+    for (int f = 0; f < 3; ++f)
+      for (int i = 0; i < n_len; ++i) {
+        const double s0 = (double)i / n_len, s1 = i + 1 == n_len ? 1.0 : (double)(i + 1) / n_len;
+        emit(cc, face_point(corner[f], corner[f + 1], s0, z0, z1, t),
+             face_point(corner[f], corner[f + 1], s1, z0, z1, t));
+      }
+  }
+  return UCLBS_OK;
+}

@@ -222,23 +222,31 @@ __device__ __forceinline__ double alpha_estimate39(const S (*sf)[64], const doub
   return na;
 }
 
-// kNear = false: every node of the box (near-face nodes skipped when the box
-// has faces); kNear = true: the listed near-face nodes, pulled through the
-// face rules.
-template <bool kFp32, bool kShifted, bool kTurb, bool kKeepAlpha, bool kNear>
+// kMode 0: every node of the box (near-face nodes skipped when the box has
+// faces; with a link table, every node whose class is not 0); kMode 1: the
+// listed near-face nodes, pulled through the face rules; kMode 2: the listed
+// wall nodes, their wall links taken from the rebuilt values of
+// k39_wall_rebuild and every other link pulled through the face rules.
+template <bool kFp32, bool kShifted, bool kTurb, bool kKeepAlpha, int kMode>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void k39_step(
     const void* __restrict__ fA_, const void* __restrict__ gA_, void* __restrict__ fB_,
     void* __restrict__ gB_, double* __restrict__ lmv, double* __restrict__ alpha, Box b,
     Gas39 gs, Ref39 rf, long long step, unsigned long long* __restrict__ stats,
     unsigned* __restrict__ neg_count, unsigned* __restrict__ neg_list, Faces39 fc, int has_faces,
-    const unsigned* __restrict__ near_list, unsigned n_near, int ghost) {
+    const unsigned* __restrict__ near_list, unsigned n_near, int ghost, Walls39 wl) {
+  constexpr bool kNear = kMode != 0;
   using S = typename std::conditional<kFp32, float, double>::type;
   const S* __restrict__ fA = static_cast<const S*>(fA_);
   const S* __restrict__ gA = static_cast<const S*>(gA_);
   S* __restrict__ fB = static_cast<S*>(fB_);
   S* __restrict__ gB = static_cast<S*>(gB_);
-  __shared__ S sf[Q39][64];
-  __shared__ S sg[Q39][64];
+  // staged populations (deviations from the reference equilibrium) in the
+  // storage type; the wall nodes stage in FP64, since their rebuilt wall
+  // populations sit far from the freestream and one FP32 rounding of those
+  // would move the node's moments ~100x more than storage rounding does
+  using SL = typename std::conditional<kMode == 2, double, S>::type;
+  __shared__ SL sf[Q39][64];
+  __shared__ SL sg[Q39][64];
   __shared__ double sd[Q39][64];
   __shared__ int smir[kNear ? 3 : 1][Q39];  // mirror directions for the inviscid faces
   const int t = threadIdx.x;
@@ -253,13 +261,18 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
   const unsigned idx = blockIdx.x * 64u + (unsigned)t;
   unsigned node;
   bool active;
-  if (kNear) {
+  if (kMode == 1) {
     active = idx < n_near;
     node = active ? near_list[idx] : 0u;
+  } else if (kMode == 2) {
+    active = idx < wl.n;
+    node = active ? wl.list[idx] : 0u;
   } else {
     node = idx;
     active = node < n;
-    if (active && (has_faces || ghost > 0)) {
+    if (active && wl.cls) {
+      active = wl.cls[node] == 0;
+    } else if (active && (has_faces || ghost > 0)) {
       const int x = (int)(node % (unsigned)b.nx);
       const int y = (int)((node / (unsigned)b.nx) % (unsigned)b.ny);
       const int z = (int)(node / ((unsigned)b.nx * (unsigned)b.ny));
@@ -278,7 +291,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
     const int x = (int)(node % (unsigned)nx);
     const int y = (int)((node / (unsigned)nx) % (unsigned)ny);
     const int z = (int)(node / ((unsigned)nx * (unsigned)ny));
-    if (kNear) {
+    if (kMode == 2) {
+#pragma unroll 3
+      for (int i = 0; i < Q39; ++i) {
+        if (i > 0 && wl.state[(size_t)i * n + node] <= -1001) {
+          sf[i][t] = wl.wf[(size_t)idx * Q39 + i] - rf.f[i];
+          sg[i][t] = wl.wg[(size_t)idx * Q39 + i] - rf.g[i];
+        } else {
+          S fs, gsv;
+          pull_face39<kShifted>(x, y, z, i, b, fc, rf, smir, fA, gA, &fs, &gsv);
+          sf[i][t] = fs;
+          sg[i][t] = gsv;
+        }
+      }
+    } else if (kNear) {
 #pragma unroll 3
       for (int i = 0; i < Q39; ++i) {
         S fs, gsv;
@@ -415,6 +441,333 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
   wave_stats39(stats, active, iters, noconv, ast, kerr);
 }
 
+// ---- shifted-lattice walls ---------------------------------------------------
+// rho, u, T of node k from the input buffers (the previous step's
+// post-collision populations)
+template <typename S>
+__device__ __forceinline__ void macro_at39(const S* __restrict__ fA, const S* __restrict__ gA,
+                                           unsigned n, unsigned k, const Ref39& rf,
+                                           const Gas39& gs, double* rho, double u[3], double* T) {
+  double r = 0.0, mx = 0.0, my = 0.0, mz = 0.0, en = 0.0;
+#pragma unroll 3
+  for (int i = 0; i < Q39; ++i) {
+    const int cx = cx39<true>(i), cy = CY39[i], cz = CZ39[i];
+    const double f = (double)fA[(size_t)i * n + k] + rf.f[i],
+                 g = (double)gA[(size_t)i * n + k] + rf.g[i];
+    r += f;
+    mx += (double)cx * f;
+    my += (double)cy * f;
+    mz += (double)cz * f;
+    en += (double)(cx * cx + cy * cy + cz * cz) * f + g;
+  }
+  u[0] = mx / r;
+  u[1] = my / r;
+  u[2] = mz / r;
+  const double vm = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  *rho = r;
+  *T = (en - r * vm) / (2 * gs.cv * r);
+}
+
+// node index of (x, y, z) wrapped over the periodic faces; -1 beyond another
+__device__ __forceinline__ long long pos39(int x, int y, int z, const Box& b, const Faces39& fc) {
+  int s[3] = {x, y, z};
+  const int nb[3] = {b.nx, b.ny, b.nz};
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (s[d] < 0) {
+      if (fc.code[d] != kFacePeriodic) return -1;
+      s[d] += nb[d];
+    } else if (s[d] >= nb[d]) {
+      if (fc.code[d + 3] != kFacePeriodic) return -1;
+      s[d] -= nb[d];
+    }
+  }
+  return (long long)s[0] + (long long)nb[0] * ((long long)s[1] + (long long)nb[1] * s[2]);
+}
+
+// The shifted lattice's wall rule for the listed wall nodes (streaming.f95:
+// 478-1133, target_finder.f95:87-127, derivatives.f95:816-1236,
+// shifted_ibb.f95:48-269), one thread per node: the target state (when a link
+// of speed <= sqrt(3) is a wall link) -- u_t, T_t the q-weighted means over
+// the wall links of the node's own u, T (or the downstream node's where the
+// real opposite link is null), rho_t from this step's pulled populations
+// with a wall link's population replaced by its real opposite's, the target
+// multipliers by Newton from the node's (or, while the level's counter is
+// <= 4, from fixed start-up guesses) -- else the node's own state; the
+// velocity and temperature gradients from the 7-point stencil (wall-kind
+// stencils by the sign of the link's unshifted component); then for every
+// wall link f_i = feq_i(target) + eps k_f (P_comp + Q_comp) and
+// g_i = (2 cv - 3) T_t feq_i + eps k_g (3 k1 + g_a + R_comp) with the
+// reference's 27-entry tensors and term slips, a negative value replaced by
+// its magnitude.  Fields of a node are the moments of its input-buffer
+// populations; a stencil node beyond a non-periodic face counts as not fluid.
+template <bool kFp32>
+__global__ __launch_bounds__(64) void k39_wall_rebuild(const void* __restrict__ fA_,
+                                                       const void* __restrict__ gA_,
+                                                       const double* __restrict__ lmv, Box b,
+                                                       Gas39 gs, Ref39 rf, Faces39 fc,
+                                                       long long step, Walls39 wl,
+                                                       unsigned long long* __restrict__ stats) {
+  using S = typename std::conditional<kFp32, float, double>::type;
+  const S* __restrict__ fA = static_cast<const S*>(fA_);
+  const S* __restrict__ gA = static_cast<const S*>(gA_);
+  __shared__ int smir[3][Q39];
+  __shared__ double sfi[Q39][64];
+  const int t = threadIdx.x;
+  if (t < Q39) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) smir[d][t] = kMirror39.m[d][t];
+  }
+  __syncthreads();
+  const unsigned j = blockIdx.x * 64u + (unsigned)t;
+  if (j >= wl.n) return;
+  const unsigned n = (unsigned)b.n;
+  const unsigned node = wl.list[j];
+  const int x = (int)(node % (unsigned)b.nx);
+  const int y = (int)((node / (unsigned)b.nx) % (unsigned)b.ny);
+  const int z = (int)(node / ((unsigned)b.nx * (unsigned)b.ny));
+  const int32_t* __restrict__ st = wl.state;
+  // this step's pulled populations (full values)
+  for (int i = 0; i < Q39; ++i) {
+    S fs, gsv;
+    pull_face39<true>(x, y, z, i, b, fc, rf, smir, fA, gA, &fs, &gsv);
+    sfi[i][t] = (double)fs + rf.f[i];
+  }
+  double rho0, u0[3], T0;
+  macro_at39(fA, gA, n, node, rf, gs, &rho0, u0, &T0);
+  bool adjacent = false;
+  for (int i = 1; i <= 14; ++i) adjacent = adjacent || st[(size_t)i * n + node] < -1000;
+  double rho_t, ut[3], Tt, lmt[kM39];
+  int noconv = 0;
+  if (adjacent) {
+    int count = 0;
+    ut[0] = ut[1] = ut[2] = 0.0;
+    Tt = 0.0;
+    for (int i = 1; i < Q39; ++i) {
+      const int32_t s = st[(size_t)i * n + node];
+      if (s > -1001) continue;
+      const double q = fabs((double)s + 1001.0) / 1.0e9;
+      ++count;
+      double rn = rho0, un[3] = {u0[0], u0[1], u0[2]}, Tn = T0;
+      if (st[(size_t)OPP39[i] * n + node] == -666) {
+        const long long d = pos39(x + cx39<true>(i), y + CY39[i], z + CZ39[i], b, fc);
+        if (d >= 0) macro_at39(fA, gA, n, (unsigned)d, rf, gs, &rn, un, &Tn);
+      }
+      for (int e = 0; e < 3; ++e) ut[e] += (q * un[e]) / (q + 1.0);
+      Tt += (q * Tn + Tn) / (q + 1.0);
+    }
+    for (int e = 0; e < 3; ++e) ut[e] /= count;
+    Tt /= count;
+    rho_t = 0.0;
+    for (int i = 0; i < Q39; ++i)
+      rho_t += st[(size_t)i * n + node] < -1000 ? sfi[OPP39[i]][t] : sfi[i][t];
+    if (wl.lvl_step <= 4) {
+      // freestream_basis_calcs.f90:154-189
+      const bool hi = wl.mach > 1.0;
+      const double g0[kM39] = {hi ? -2.2 : -2.0, hi ? 0.4 : 0.0, 0.0, 0.0, hi ? -0.45 : -0.4,
+                               hi ? -0.45 : -0.4, hi ? -0.45 : -0.4, 0.0, 0.0, 0.0,
+                               hi ? 0.015 : 0.002, 0.0, 0.0};
+      for (int k = 0; k < kM39; ++k) lmt[k] = g0[k];
+    } else {
+      for (int k = 0; k < kM39; ++k) lmt[k] = lmv[(size_t)k * n + node];
+    }
+    const double vm = ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2];
+    const double qf = rho_t * vm + 5 * rho_t * Tt;
+    const double tg[kM39] = {rho_t, rho_t * ut[0], rho_t * ut[1], rho_t * ut[2],
+                             rho_t * (Tt + ut[0] * ut[0]), rho_t * (Tt + ut[1] * ut[1]),
+                             rho_t * (Tt + ut[2] * ut[2]), rho_t * ut[0] * ut[1],
+                             rho_t * ut[0] * ut[2], rho_t * ut[1] * ut[2], qf * ut[0],
+                             qf * ut[1], qf * ut[2]};
+    double lmp[kM39];
+    noconv = lm_newton39<true>(rho_t, tg, lmt, lmp, gs.maxit) < 0;
+  } else {
+    rho_t = rho0;
+    for (int e = 0; e < 3; ++e) ut[e] = u0[e];
+    Tt = T0;
+    for (int k = 0; k < kM39; ++k) lmt[k] = lmv[(size_t)k * n + node];
+  }
+  // 7-point stencil: x, x + e_a, x - e_a
+  double su[7][3], sT[7];
+  int scode[7];
+  for (int m = 0; m < 7; ++m) {
+    const int a = m == 0 ? -1 : (m - 1) % 3, sg = m == 0 ? 0 : (m <= 3 ? 1 : -1);
+    const long long d = m == 0 ? (long long)node
+                               : pos39(x + (a == 0 ? sg : 0), y + (a == 1 ? sg : 0),
+                                       z + (a == 2 ? sg : 0), b, fc);
+    scode[m] = d >= 0 ? st[(size_t)d] : -1;
+    if (scode[m] >= 0) {
+      double rr, uu[3], TT;
+      if (m == 0) {
+        uu[0] = u0[0]; uu[1] = u0[1]; uu[2] = u0[2]; TT = T0;
+      } else {
+        macro_at39(fA, gA, n, (unsigned)d, rf, gs, &rr, uu, &TT);
+      }
+      for (int e = 0; e < 3; ++e) su[m][e] = uu[e];
+      sT[m] = TT;
+    } else {
+      su[m][0] = su[m][1] = su[m][2] = 0.0;
+      sT[m] = 1.0 / gs.gama;
+    }
+  }
+  const double eps =
+      step < gs.startup ? (((double)step + 1.0) / (double)gs.startup) * gs.eps : gs.eps;
+  double dmu, dte;
+  const double mu = sutherland39(Tt, gs, &dmu, &dte);
+  double kappa = 0.0;
+  const bool kerr = !therm_cond39(rho_t, Tt, gs, &kappa);
+  double b1, b2;
+  if (step < gs.startup) {
+    b1 = gs.init_beta1 + gs.beta1_scale * (double)step;
+    b2 = gs.init_beta2 + gs.beta2_scale * (double)step;
+  } else {
+    b1 = 1.0 / ((ldexp(1.0, gs.lvl + 1) * mu) / (rho_t * Tt) + 1.0);
+    b2 = 1.0 / ((2.0 * kappa) / (rho_t * gs.cp * Tt) + 1.0);
+  }
+  const double cv = gs.cv, T = Tt, u = ut[0], v = ut[1], w = ut[2];
+  const double k_f = -(rho_t * T) / 2.0, k_g = -(rho_t * T * (2.0 * cv - 3.0)) / 2.0;
+  unsigned long long nrep = 0, nlinks = 0;
+  for (int i = 1; i < Q39; ++i) {
+    if (st[(size_t)i * n + node] > -1001) continue;
+    ++nlinks;
+    const int X = cx39<true>(i), Y = CY39[i], Z = CZ39[i];
+    const double link = sqrt((double)(X * X + Y * Y + Z * Z));
+    const int ud[3] = {CX39[i], CY39[i], CZ39[i]};  // unshifted components: stencil kinds
+    double du[3][3], dT[3];
+    for (int ax = 0; ax < 3; ++ax) {
+      const int kind = ud[ax] > 0 ? 1 : (ud[ax] < 0 ? -1 : 0);
+      const double wd = fabs((double)scode[1 + ax] + 1001.0) / 1.0e9 * link;
+      const bool up_ok = scode[4 + ax] >= 0, down_ok = scode[1 + ax] >= 0;
+      for (int bb = 0; bb < 3; ++bb) {
+        if (kind > 0) {
+          du[ax][bb] = (0.0 - ut[bb]) / (1.0 + wd);
+        } else if (kind < 0) {
+          du[ax][bb] = (ut[bb] - 0.0) / (1.0 + wd);
+        } else {
+          const double rest = su[0][bb], up = su[4 + ax][bb], down = su[1 + ax][bb];
+          du[ax][bb] = (up_ok && down_ok) ? (down - up) / 2.0
+                       : up_ok            ? rest - up
+                       : down_ok          ? down - rest
+                                          : 0.0;
+        }
+      }
+      if (kind != 0) {
+        dT[ax] = 0.0;
+      } else {
+        const double rest = sT[0], up = sT[4 + ax], down = sT[1 + ax];
+        dT[ax] = (up_ok && down_ok) ? (down - up) / 2.0
+                 : up_ok            ? rest - up
+                 : down_ok          ? down - rest
+                                    : 0.0;
+      }
+    }
+    const double dudx = du[0][0], dudy = du[1][0], dudz = du[2][0];
+    const double dvdx = du[0][1], dvdy = du[1][1], dvdz = du[2][1];
+    const double dwdx = du[0][2], dwdy = du[1][2], dwdz = du[2][2];
+    const double dTdx = dT[0], dTdy = dT[1], dTdz = dT[2];
+    double p[28];  // 1-based as in shifted_ibb.f95
+    p[1] = 2 * dudx - dudx / cv; p[2] = 2 * dudx - dvdy / cv; p[3] = 2 * dudx - dwdz / cv;
+    p[4] = p[5] = p[6] = dudy + dvdx;
+    p[7] = p[8] = p[9] = dudz + dwdx;
+    p[10] = p[11] = p[12] = dudy + dvdx;
+    p[13] = 2 * dvdy - dudx / cv; p[14] = 2 * dvdy - dvdy / cv; p[15] = 2 * dvdy - dwdz / cv;
+    p[16] = p[17] = p[18] = dvdz + dwdy;
+    p[19] = p[20] = p[21] = dudz + dwdx;
+    p[22] = p[23] = p[24] = dvdz + dwdy;
+    p[25] = 2 * dwdz - dudx / cv; p[26] = 2 * dwdz - dvdy / cv; p[27] = 2 * dwdz - dwdz / cv;
+    for (int m = 1; m <= 27; ++m) p[m] = p[m] / b1;
+    const double tx = dTdx / b2, ty = dTdy / b2, tz = dTdz / b2;
+    double qq[28];
+    qq[1] = 3 * (tx + u * p[1]);
+    qq[2] = ty + v * p[2] + 2 * u * p[4];
+    qq[3] = tz + w * p[3] + 2 * u * p[7];
+    qq[4] = ty + 2 * u * p[4] + v * p[1];
+    qq[5] = tx + u * p[13] + 2 * v * p[4];
+    qq[6] = u * p[22] + v * p[7] + w * p[4];
+    qq[7] = tz + w * p[3] + 2 * u * p[7];
+    qq[8] = u * p[22] + v * p[7] + w * p[4];
+    qq[9] = tx + u * p[25] + 2 * w * p[7];
+    qq[10] = ty + v * p[2] + 2 * u * p[4];
+    qq[11] = tx + 2 * v * p[4] + u * p[13];
+    qq[12] = u * p[22] + v * p[7] + w * p[4];
+    qq[13] = tx + u * p[13] + 2 * v * p[4];
+    qq[14] = 3 * (ty + v * p[14]);
+    qq[15] = tz + w * p[15] + 2 * v * p[22];
+    qq[16] = u * p[22] + v * p[7] + w * p[4];
+    qq[17] = tz + w * p[15] + 2 * v * p[22];
+    qq[18] = ty + v * p[26] + 2 * w * p[22];
+    qq[19] = tx + w * p[3] + 2 * u * p[7];
+    qq[20] = u * p[22] + v * p[7] + w * p[4];
+    qq[21] = tx + u * p[25] + 2 * w * p[7];
+    qq[22] = u * p[22] + v * p[7] + w * p[4];
+    qq[23] = tz + w * p[15] + 2 * v * p[22];
+    qq[24] = ty + v * p[26] + 2 * w * p[22];
+    qq[25] = tx + u * p[25] + 2 * w * p[7];
+    qq[26] = ty + v * p[26] + 2 * w * p[22];
+    qq[27] = 3 * (tz + w * p[27]);
+    const double Xd = X, Yd = Y, Zd = Z;
+    const double sxx = Xd * Xd - T, syy = Yd * Yd - T, szz = Zd * Zd - T;
+    const double pc[28] = {0, sxx, sxx, sxx, Xd * Yd, Xd * Yd, Xd * Yd, Xd * Zd, Xd * Zd, Xd * Zd,
+                           Yd * Xd, Yd * Xd, Yd * Xd, syy, syy, syy, Yd * Zd, Yd * Zd, Yd * Zd,
+                           Zd * Xd, Zd * Xd, Zd * Xd, Zd * Yd, Zd * Yd, Zd * Yd, szz, szz, szz};
+    double psum = 0.0;
+    for (int m = 1; m <= 27; ++m) psum += p[m] * pc[m];
+    psum += p[25] * szz;  // P_zzx counted twice (shifted_ibb.f95:170)
+    const double p_comp = psum / (2 * T * T);
+    const double qc[28] = {0,
+                           Xd * Xd * Xd - 3 * Xd * T, Xd * Xd * Yd - 3 * Yd * T,
+                           Xd * Xd * Zd - 3 * Zd * T, Xd * Xd * Yd, Xd * Yd * Yd, Xd * Yd * Zd,
+                           Xd * Xd * Zd, Xd * Yd * Zd, Xd * Zd * Zd, Xd * Xd * Yd, Yd * Yd * Xd,
+                           Xd * Yd * Zd, Yd * Yd * Xd - 3 * Xd * T, Yd * Yd * Yd - 3 * Yd * T,
+                           Yd * Yd * Zd - 3 * Zd * T, Xd * Yd * Zd, Yd * Yd * Zd, Zd * Zd * Yd,
+                           Xd * Xd * Zd, Xd * Yd * Zd, Zd * Zd * Xd, Xd * Yd * Zd, Zd * Yd * Yd,
+                           Zd * Zd * Yd, Zd * Zd * Xd - 3 * Xd * T, Zd * Zd * Yd - 3 * Yd * T,
+                           Zd * Zd * Zd - 3 * Zd * T};
+    double qsum = 0.0;
+    for (int m = 1; m <= 27; ++m) qsum += qq[m] * qc[m];
+    const double q_comp = qsum / (6 * T * T * T);
+    const double feq = expo39<true>(lmt, rho_t, i);
+    double fo = feq + eps * k_f * (p_comp + q_comp);
+    const double k_one = 3 * (dTdx + dTdy + dTdz) / (b1 * cv);
+    const double g_alpha = Xd / T * (9 * dTdx / b2 + u * k_one) + Yd / T * (9 * dTdy / b2 + v * k_one) +
+                           Zd / T * (9 * dTdz / b2 + w * k_one);
+    double r[28];
+    r[1] = 2 * T * dudx + 2 * u * dTdx - dudx / cv * (2 * T + u * u);
+    r[2] = 2 * T * dudx + 2 * u * dTdx - dvdy / cv * (2 * T + u * u);
+    r[3] = 2 * T * dudx + 2 * u * dTdx - dwdz / cv * (2 * T + u * u);
+    r[4] = T * (dudy + dvdx) + u * dTdy + v * dTdx - dudx / cv * u * v;
+    r[5] = T * (dudy + dvdx) + u * dTdy + v * dTdx - dvdy / cv * u * v;
+    r[6] = T * (dudy + dvdx) + u * dTdy + v * dTdx - dwdz / cv * u * v;
+    r[7] = T * (dudz + dwdx) + u * dTdz + w * dTdx - dudx / cv * u * w;
+    r[8] = T * (dudz + dwdx) + u * dTdz + w * dTdx - dvdy / cv * u * w;
+    r[9] = T * (dudz + dwdx) + u * dTdz + w * dTdx - dwdz / cv * u * w;
+    r[10] = r[4]; r[11] = r[5]; r[12] = r[6];
+    r[13] = 2 * T * dvdy + 2 * v * dTdy - dudx / cv * (2 * T + v * v);
+    r[14] = 2 * T * dvdy + 2 * v * dTdy - dvdy / cv * (2 * T + v * v);
+    r[15] = 2 * T * dvdy + 2 * v * dTdy - dwdz / cv * (2 * T + v * v);
+    r[16] = T * (dvdz + dwdy) + w * dTdy + v * dTdz - dudx / cv * v * w;
+    r[17] = T * (dvdz + dwdy) + w * dTdy + v * dTdz - dvdy / cv * v * w;
+    r[18] = T * (dvdz + dwdy) + w * dTdy + v * dTdz - dwdz / cv * v * w;
+    r[19] = r[7]; r[20] = r[8]; r[21] = r[9];
+    r[22] = r[13]; r[23] = r[14]; r[24] = r[15];  // the zy entries repeat yy (shifted_ibb.f95:231-233)
+    r[25] = 2 * T * dwdz + 2 * w * dTdz - dudx / cv * (2 * T + w * w);
+    r[26] = 2 * T * dwdz + 2 * w * dTdz - dvdy / cv * (2 * T + w * w);
+    r[27] = 2 * T * dwdz + 2 * w * dTdz - dwdz / cv * (2 * T + w * w);
+    double rsum = 0.0;
+    for (int m = 1; m <= 27; ++m) rsum += r[m] * pc[m];
+    const double gi_neq = k_g * (3 * k_one + g_alpha + rsum / (2 * b1 * T));
+    double go = gs.lead_g * T * feq + eps * gi_neq;
+    if (fo < 0.0) { fo = fabs(fo); ++nrep; }
+    if (go < 0.0) { go = fabs(go); ++nrep; }
+    wl.wf[(size_t)j * Q39 + i] = fo;
+    wl.wg[(size_t)j * Q39 + i] = go;
+  }
+  unsigned long long* sl = stats + (size_t)(blockIdx.x & (kStatSlots - 1)) * kStatFields;
+  atomicAdd(sl + ST_IBB, nlinks);
+  if (nrep) atomicAdd(sl + ST_REPAIR, nrep);
+  if (noconv) atomicAdd(sl + ST_NOCONV, 1ull);
+  if (kerr) atomicAdd(sl + ST_FALLBACK, 1ull);
+}
+
 // node (x, y, z) wrapped over the periodic axes of b; -1 across another face
 // or inside a refined level's ghost layer (which holds no post-collision values)
 __device__ __forceinline__ long long damp_nb(int x, int y, int z, const Box& b, int ghost) {
@@ -441,7 +794,7 @@ __global__ __launch_bounds__(256) void k39_damp_compute(const void* __restrict__
                                                         const unsigned* __restrict__ list,
                                                         double* __restrict__ rep,
                                                         unsigned long long* __restrict__ stats,
-                                                        int ghost) {
+                                                        int ghost, const int32_t* __restrict__ state) {
   using S = typename std::conditional<kFp32, float, double>::type;
   const S* __restrict__ f = static_cast<const S*>(f_);
   const unsigned cnt = *count;
@@ -461,13 +814,27 @@ __global__ __launch_bounds__(256) void k39_damp_compute(const void* __restrict__
         // one-sided form of damp_it.f95:109-123
         const long long kp = damp_nb(x + cx, y + cy, z + cz, b, ghost);
         const long long km = damp_nb(x - cx, y - cy, z - cz, b, ghost);
-        const double fp = kp >= 0 ? (double)f[(size_t)i * n + (size_t)kp] + rf.f[i] : -1.0;
-        const double fm = km >= 0 ? (double)f[(size_t)i * n + (size_t)km] + rf.f[i] : -1.0;
-        if (fp > 0.0 && fm > 0.0) r = (fp + fm) / 2.0;
-        else if (fp > 0.0 && fm < 0.0) r = (fp + fabs(v)) / 2;
-        else if (fp < 0.0 && fm > 0.0) r = (fm + fabs(v)) / 2;
-        else r = fabs(v);
-        ++nrep;
+        // with a link table (damp_it.f95:125-177): the + neighbour counts when
+        // the link code is >= -500 (the rest direction: the node's code), the
+        // - neighbour when its node code is >= -500; with neither, no repair
+        bool pok = kp >= 0, mok = km >= 0;
+        if (state) {
+          pok = pok && state[(size_t)(i == 0 ? 0 : i) * n + node] >= -500;
+          mok = mok && state[(size_t)km] >= -500;
+        }
+        const double fp = pok ? (double)f[(size_t)i * n + (size_t)kp] + rf.f[i] : -1.0;
+        const double fm = mok ? (double)f[(size_t)i * n + (size_t)km] + rf.f[i] : -1.0;
+        if (!state || (pok && mok)) {
+          if (fp > 0.0 && fm > 0.0) r = (fp + fm) / 2.0;
+          else if (fp > 0.0 && fm < 0.0) r = (fp + fabs(v)) / 2;
+          else if (fp < 0.0 && fm > 0.0) r = (fm + fabs(v)) / 2;
+          else r = fabs(v);
+        } else if (pok) {
+          r = fp > 0.0 ? (fp + fabs(v)) / 2 : fabs(v);
+        } else if (mok) {
+          r = fm > 0.0 ? (fm + fabs(v)) / 2 : fabs(v);
+        }
+        if (r == r) ++nrep;
       }
       rep[(size_t)j * Q39 + i] = r;
     }
@@ -679,6 +1046,42 @@ __global__ void k39_from_fp64(const double* __restrict__ src, void* __restrict__
     dst[k] = (S)(src[k] - ref[k / n]);
 }
 
+// flow variables of calculate_flow_variables.f95:201-221 per node from the
+// current populations: rho, u, v, w, T and the reference's press =
+// rho (T + |u|^2 / 2); out is [6][n]
+template <bool kFp32, bool kShifted>
+__global__ __launch_bounds__(256) void k39_macro(const void* __restrict__ f_,
+                                                 const void* __restrict__ g_, Box b, Ref39 rf,
+                                                 double cv, double* __restrict__ out) {
+  using S = typename std::conditional<kFp32, float, double>::type;
+  const S* __restrict__ fA = static_cast<const S*>(f_);
+  const S* __restrict__ gA = static_cast<const S*>(g_);
+  const size_t n = (size_t)b.n;
+  for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
+       k += (size_t)gridDim.x * blockDim.x) {
+    double r = 0.0, mx = 0.0, my = 0.0, mz = 0.0, en = 0.0;
+#pragma unroll
+    for (int i = 0; i < Q39; ++i) {
+      const int cx = cx39<kShifted>(i), cy = CY39[i], cz = CZ39[i];
+      const double f = (double)fA[(size_t)i * n + k] + rf.f[i],
+                   g = (double)gA[(size_t)i * n + k] + rf.g[i];
+      r += f;
+      mx += (double)cx * f;
+      my += (double)cy * f;
+      mz += (double)cz * f;
+      en += (double)(cx * cx + cy * cy + cz * cz) * f + g;
+    }
+    const double u = mx / r, v = my / r, w = mz / r, vm = u * u + v * v + w * w;
+    const double T = (en - r * vm) / (2 * cv * r);
+    out[k] = r;
+    out[n + k] = u;
+    out[2 * n + k] = v;
+    out[3 * n + k] = w;
+    out[4 * n + k] = T;
+    out[5 * n + k] = r * (T + vm / 2.0);
+  }
+}
+
 template <bool kFp32>
 __global__ void k39_to_fp64(const void* __restrict__ src_, double* __restrict__ dst,
                             const double* __restrict__ ref, size_t n) {
@@ -695,16 +1098,28 @@ hipError_t launch_d3q39_step(const Box& b, const Step39& a, hipStream_t s) {
   const unsigned blocks = (unsigned)((b.n + 63) / 64);
   const unsigned near_blocks = (a.n_near + 63u) / 64u;
   if (a.has_faces && a.n_near && !a.near_list) return hipErrorInvalidValue;
+  const unsigned wall_blocks = (a.walls.n + 63u) / 64u;
+  if (a.walls.cls && !a.shifted) return hipErrorInvalidValue;  // shifted-lattice wall rule only
   hipLaunchKernelGGL(k39_clear, dim3(1), dim3(1), 0, s, a.neg_count);
+  // the wall nodes' rebuilt populations, from the input buffers
+  if (a.walls.cls && a.walls.n) {
+    if (a.fp32)
+      hipLaunchKernelGGL((k39_wall_rebuild<true>), dim3(wall_blocks), dim3(64), 0, s, a.fin, a.gin,
+                         a.lm, b, a.gas, a.ref, a.faces, (long long)a.step, a.walls, a.stats);
+    else
+      hipLaunchKernelGGL((k39_wall_rebuild<false>), dim3(wall_blocks), dim3(64), 0, s, a.fin, a.gin,
+                         a.lm, b, a.gas, a.ref, a.faces, (long long)a.step, a.walls, a.stats);
+  }
 #define UCLBS_K39(F, SH, TU, KA, NR, NB)                                                         \
   hipLaunchKernelGGL((k39_step<F, SH, TU, KA, NR>), dim3(NB), dim3(64), 0, s, a.fin, a.gin,     \
                      a.fout, a.gout, a.lm, a.alpha, b, a.gas, a.ref, (long long)a.step, a.stats, \
                      a.neg_count, a.neg_list, a.faces, a.has_faces, a.near_list, a.n_near,   \
-                     a.ghost)
+                     a.ghost, a.walls)
 #define UCLBS_K39_FC(F, SH, TU, KA)                                              \
   do {                                                                           \
-    UCLBS_K39(F, SH, TU, KA, false, blocks);                                     \
-    if (a.has_faces && a.n_near) UCLBS_K39(F, SH, TU, KA, true, near_blocks);    \
+    UCLBS_K39(F, SH, TU, KA, 0, blocks);                                         \
+    if (a.has_faces && a.n_near) UCLBS_K39(F, SH, TU, KA, 1, near_blocks);       \
+    if (SH && a.walls.cls && a.walls.n) UCLBS_K39(F, SH, TU, KA, 2, wall_blocks); \
   } while (0)
 #define UCLBS_K39_KA(F, SH, TU) \
   do {                          \
@@ -731,19 +1146,19 @@ hipError_t launch_d3q39_step(const Box& b, const Step39& a, hipStream_t s) {
   if (a.fp32) {
     if (a.shifted)
       hipLaunchKernelGGL((k39_damp_compute<true, true>), rg, rb, 0, s, a.fout, b, a.ref,
-                         a.neg_count, a.neg_list, a.rep, a.stats, a.ghost);
+                         a.neg_count, a.neg_list, a.rep, a.stats, a.ghost, a.walls.state);
     else
       hipLaunchKernelGGL((k39_damp_compute<true, false>), rg, rb, 0, s, a.fout, b, a.ref,
-                         a.neg_count, a.neg_list, a.rep, a.stats, a.ghost);
+                         a.neg_count, a.neg_list, a.rep, a.stats, a.ghost, a.walls.state);
     hipLaunchKernelGGL((k39_damp_apply<true>), rg, rb, 0, s, a.fout, b, a.ref, a.neg_count,
                        a.neg_list, a.rep);
   } else {
     if (a.shifted)
       hipLaunchKernelGGL((k39_damp_compute<false, true>), rg, rb, 0, s, a.fout, b, a.ref,
-                         a.neg_count, a.neg_list, a.rep, a.stats, a.ghost);
+                         a.neg_count, a.neg_list, a.rep, a.stats, a.ghost, a.walls.state);
     else
       hipLaunchKernelGGL((k39_damp_compute<false, false>), rg, rb, 0, s, a.fout, b, a.ref,
-                         a.neg_count, a.neg_list, a.rep, a.stats, a.ghost);
+                         a.neg_count, a.neg_list, a.rep, a.stats, a.ghost, a.walls.state);
     hipLaunchKernelGGL((k39_damp_apply<false>), rg, rb, 0, s, a.fout, b, a.ref, a.neg_count,
                        a.neg_list, a.rep);
   }
@@ -787,6 +1202,19 @@ hipError_t launch_d3q39_to_fp64(const Box& b, const void* src, double* dst, int 
   const unsigned blocks = (unsigned)std::min<size_t>((tot + 255) / 256, 65536);
   if (fp32) hipLaunchKernelGGL((k39_to_fp64<true>), dim3(blocks), dim3(256), 0, s, src, dst, ref, (size_t)b.n);
   else hipLaunchKernelGGL((k39_to_fp64<false>), dim3(blocks), dim3(256), 0, s, src, dst, ref, (size_t)b.n);
+  return hipGetLastError();
+}
+
+hipError_t launch_d3q39_macro(const Box& b, const void* f, const void* g, int fp32, int shifted,
+                              const Ref39& rf, double cv, double* out, hipStream_t s) {
+  const unsigned blocks = (unsigned)std::min<size_t>(((size_t)b.n + 255) / 256, 65536);
+  if (fp32) {
+    if (shifted) hipLaunchKernelGGL((k39_macro<true, true>), dim3(blocks), dim3(256), 0, s, f, g, b, rf, cv, out);
+    else hipLaunchKernelGGL((k39_macro<true, false>), dim3(blocks), dim3(256), 0, s, f, g, b, rf, cv, out);
+  } else {
+    if (shifted) hipLaunchKernelGGL((k39_macro<false, true>), dim3(blocks), dim3(256), 0, s, f, g, b, rf, cv, out);
+    else hipLaunchKernelGGL((k39_macro<false, false>), dim3(blocks), dim3(256), 0, s, f, g, b, rf, cv, out);
+  }
   return hipGetLastError();
 }
 

@@ -238,6 +238,19 @@ __host__ __device__ inline bool near_face39(int x, int y, int z, const int code[
   }
   return near;
 }
+// A shifted-lattice D3Q39 level with a link table: node classes (0 plain
+// pull, 1 near a face, 2 wall links, 3 solid), the table, the wall nodes and
+// the rebuilt f and g of their wall links ([n][39] FP64, full values).
+struct Walls39 {
+  const uint8_t* cls;     // [n] or null (no table)
+  const int32_t* state;   // [39][n]
+  const unsigned* list;   // wall nodes
+  unsigned n;
+  double* wf;
+  double* wg;
+  long long lvl_step;     // the level's own step counter (start-up multiplier guesses)
+  double mach;
+};
 struct Step39 {
   const void* fin;
   const void* gin;
@@ -258,6 +271,7 @@ struct Step39 {
   const unsigned* near_list;  // near-face nodes, stepped by a second launch
   unsigned n_near;
   int ghost;            // depth of a refined level's ghost layer (not stepped), else 0
+  Walls39 walls;        // cls == null: no link table
 };
 hipError_t launch_d3q39_step(const Box& b, const Step39& a, hipStream_t s);
 hipError_t launch_coarse_to_fine39(const void* cpf, const void* cnf, const void* cpg,
@@ -274,6 +288,8 @@ hipError_t launch_d3q39_init(const Box& b, const double* fields, void* f, void* 
                              unsigned long long* bad, hipStream_t s);
 hipError_t launch_d3q39_from_fp64(const Box& b, const double* src, void* dst, int fp32,
                                   const double* ref, hipStream_t s);
+hipError_t launch_d3q39_macro(const Box& b, const void* f, const void* g, int fp32, int shifted,
+                              const Ref39& rf, double cv, double* out, hipStream_t s);
 hipError_t launch_d3q39_to_fp64(const Box& b, const void* src, double* dst, int fp32,
                                 const double* ref, hipStream_t s);
 

@@ -93,7 +93,13 @@ struct Level {
   unsigned* neg_count = nullptr;
   unsigned* neg_list = nullptr;
   unsigned* near_list = nullptr;  // D3Q39 near-face nodes (rebuilt when the faces change)
-  bool table39 = false;           // a D3Q39 link table is resident (not stepped)
+  bool table39 = false;           // a D3Q39 link table is resident
+  uint8_t* cls39 = nullptr;       // shifted D3Q39 table: node classes (0 plain, 1 near a face,
+                                  // 2 wall links, 3 solid)
+  unsigned* wall_list = nullptr;  // nodes of class 2
+  unsigned n_wall = 0;
+  double* wall_f = nullptr;       // rebuilt f, g of the wall links, [n_wall][39] each
+  double* wall_g = nullptr;
   int ghost_depth = 0;            // ghost layer of a refined level (1 on D3Q27, 4 on D3Q39)
   uclbs::Gas39 gas39{};           // D3Q39 constants of this level (dt, epsilon, beta scaling)
   unsigned n_near = 0;
@@ -146,6 +152,7 @@ struct Ctx {
   // D3Q39 gas constants and the reference-state populations of the FP32 storage
   bool gas_set = false;
   int shifted39 = 0;
+  double gas39_mach = 0.0;
   uclbs::Gas39 gas39{};
   uclbs::Ref39 ref39{};
   uclbs::Ref39 fs39{};        // freestream equilibria fs_fieq / fs_gieq (full values)
@@ -229,6 +236,12 @@ void free_level(Level& L) {
   L.n_near = 0;
   L.near_dirty = true;
   L.table39 = false;
+  for (void* p : {(void*)L.cls39, (void*)L.wall_list, (void*)L.wall_f, (void*)L.wall_g})
+    if (p) (void)hipFree(p);
+  L.cls39 = nullptr;
+  L.wall_list = nullptr;
+  L.wall_f = L.wall_g = nullptr;
+  L.n_wall = 0;
   if (L.rep) { (void)hipFree(L.rep); L.rep = nullptr; }
   if (L.ghosts) { (void)hipFree(L.ghosts); L.ghosts = nullptr; }
   L.nghost = 0;
@@ -369,6 +382,54 @@ int check_face_tables(const Ctx* c, const Level* L) {
 // their own launch; the list is rebuilt when the faces change
 static int build_near39(Ctx* c, Level* L) {
   if (!L->near_dirty) return UCLBS_OK;
+  if (L->table39 && c->shifted39) {
+    // node classes from the table: solid, wall links (i >= 1 coded <= -1001),
+    // near a face, plain; the near-face and wall launches take their lists
+    const int64_t n = L->box.n;
+    std::vector<int32_t> st((size_t)n * uclbs::Q39);
+    HIP_TRY(hipMemcpy(st.data(), L->state, st.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    std::vector<uint8_t> cls((size_t)n);
+    std::vector<unsigned> near, wall;
+    const uclbs::Box& bx = L->box;
+    for (int64_t k = 0; k < n; ++k) {
+      uint8_t cl = 0;
+      if (st[(size_t)k] < 0) {
+        cl = 3;
+      } else {
+        bool w = false;
+        for (int i = 1; i < uclbs::Q39 && !w; ++i) w = st[(size_t)i * n + k] <= -1001;
+        const int x = (int)(k % bx.nx), y = (int)((k / bx.nx) % bx.ny), z = (int)(k / ((int64_t)bx.nx * bx.ny));
+        if (w) cl = 2;
+        else if (uclbs::near_face39(x, y, z, L->face_code, bx.nx, bx.ny, bx.nz, true)) cl = 1;
+      }
+      cls[(size_t)k] = cl;
+      if (cl == 1) near.push_back((unsigned)k);
+      if (cl == 2) wall.push_back((unsigned)k);
+    }
+    for (void* p : {(void*)L->cls39, (void*)L->wall_list, (void*)L->wall_f, (void*)L->wall_g,
+                    (void*)L->near_list})
+      if (p) HIP_TRY(hipFree(p));
+    L->cls39 = nullptr;
+    L->wall_list = L->near_list = nullptr;
+    L->wall_f = L->wall_g = nullptr;
+    HIP_TRY(hipMalloc(&L->cls39, (size_t)n));
+    HIP_TRY(hipMemcpy(L->cls39, cls.data(), (size_t)n, hipMemcpyHostToDevice));
+    L->n_near = (unsigned)near.size();
+    L->n_wall = (unsigned)wall.size();
+    if (!near.empty()) {
+      HIP_TRY(hipMalloc(&L->near_list, near.size() * sizeof(unsigned)));
+      HIP_TRY(hipMemcpy(L->near_list, near.data(), near.size() * sizeof(unsigned), hipMemcpyHostToDevice));
+    }
+    if (!wall.empty()) {
+      HIP_TRY(hipMalloc(&L->wall_list, wall.size() * sizeof(unsigned)));
+      HIP_TRY(hipMemcpy(L->wall_list, wall.data(), wall.size() * sizeof(unsigned), hipMemcpyHostToDevice));
+      HIP_TRY(hipMalloc(&L->wall_f, wall.size() * uclbs::Q39 * sizeof(double)));
+      HIP_TRY(hipMalloc(&L->wall_g, wall.size() * uclbs::Q39 * sizeof(double)));
+    }
+    L->near_dirty = false;
+    invalidate_graph(c);
+    return UCLBS_OK;
+  }
   std::vector<unsigned> near;
   const uclbs::Box& bx = L->box;
   for (int z = 0; z < bx.nz; ++z)
@@ -395,9 +456,10 @@ int finest_level(const Ctx* c) {
 }
 
 int launch_steps39(Ctx* c, Level* L, int nsteps, hipEvent_t* ev) {
-  if (L->table39)
-    return set_err(UCLBS_E_UNSUPPORTED, "D3Q39 levels with a link table are not stepped: the "
-                                        "D3Q39 wall-link rule is not part of this library");
+  if (L->table39 && !c->shifted39)
+    return set_err(UCLBS_E_UNSUPPORTED, "plain-lattice D3Q39 levels with a link table are not "
+                                        "stepped: only the shifted lattice's wall rule "
+                                        "(shifted_ibb) is part of this library");
   uclbs::Step39 a{};
   a.gas = L->gas39;
   a.ghost = L->refined ? L->ghost_depth : 0;
@@ -419,12 +481,22 @@ int launch_steps39(Ctx* c, Level* L, int nsteps, hipEvent_t* ev) {
     a.faces.f[i] = c->fs39.f[i];
     a.faces.g[i] = c->fs39.g[i];
   }
-  if (a.has_faces) {
+  if (a.has_faces || L->table39) {
     int rc = build_near39(c, L);
     if (rc) return rc;
   }
   a.near_list = L->near_list;
   a.n_near = L->n_near;
+  if (L->table39) {
+    a.has_faces = 1;
+    a.walls.cls = L->cls39;
+    a.walls.state = L->state;
+    a.walls.list = L->wall_list;
+    a.walls.n = L->n_wall;
+    a.walls.wf = L->wall_f;
+    a.walls.wg = L->wall_g;
+    a.walls.mach = c->gas39_mach;
+  }
   for (int s = 0; s < nsteps; ++s) {
     a.fin = L->f[L->cur];
     a.gin = L->g[L->cur];
@@ -434,6 +506,7 @@ int launch_steps39(Ctx* c, Level* L, int nsteps, hipEvent_t* ev) {
     // reference's timestep(amr_max_lvl) (calculate_alpha.f95:185-187,
     // collision.f95:277-278, find_q_bar.f95:36, find_little_q_bar.f95:26)
     a.step = (long long)c->levels[finest_level(c)].step;
+    a.walls.lvl_step = (long long)L->step;
     if (ev) HIP_TRY(hipEventRecord(ev[2 * s], c->stream));
     HIP_TRY(uclbs::launch_d3q39_step(L->box, a, c->stream));
     if (ev) HIP_TRY(hipEventRecord(ev[2 * s + 1], c->stream));
@@ -1335,6 +1408,7 @@ int uclbs_build_links(void* ctx, int lvl, const double origin[3], double dx, int
   if (e != hipSuccess) return set_err(UCLBS_E_HIP, "build_links failed: %s", hipGetErrorString(e));
   if (q39) {
     L->table39 = true;
+    L->near_dirty = true;
     L->nsolid = (int64_t)hc[0];
     invalidate_graph(c);
   } else if ((rc = finalize_links(c, L))) {
@@ -1359,6 +1433,7 @@ int uclbs_set_links(void* ctx, int lvl, const int32_t* state, int64_t n_nodes) {
   HIP_TRY(hipMemcpy(L->state, state, bytes, hipMemcpyHostToDevice));
   if (c->cfg.lattice == UCLBS_LATTICE_D3Q39) {
     L->table39 = true;
+    L->near_dirty = true;
     invalidate_graph(c);
     return UCLBS_OK;
   }
@@ -1572,6 +1647,7 @@ int uclbs_set_gas(void* ctx, const uclbs_gas* gas) {
   g.turbulent = gas->turbulent != 0;
   g.maxit = gas->lm_maxit > 0 ? gas->lm_maxit : 50;
   c->shifted39 = gas->shifted < 0 ? (gas->mach >= 1.2 ? 1 : 0) : (gas->shifted != 0);
+  c->gas39_mach = gas->mach;
   // freestream equilibria (freestream_basis_calcs.f90:35-71: the multiplier
   // solve of rho = 1, u = Mach along x, T = 1/gamma; fs_gieq = (2 cv - 3) T
   // fs_fieq); FP32 storage keeps deviations from them
@@ -1707,6 +1783,30 @@ int uclbs_download_state(void* ctx, int lvl, double* f, double* g, double* lm, i
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   (void)hipFree(tmp);
   if (e != hipSuccess) return set_err(UCLBS_E_HIP, "download_state failed: %s", hipGetErrorString(e));
+  return UCLBS_OK;
+}
+
+int uclbs_download_macro39(void* ctx, int lvl, double* rho, double* u, double* v, double* w,
+                           double* temp, double* press, int64_t n_nodes) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  Level* L;
+  int rc = check39(c, lvl, &L);
+  if (rc) return rc;
+  if (n_nodes != L->box.n)
+    return set_err(UCLBS_E_ARG, "download_macro39: %lld nodes (expected %lld)",
+                   (long long)n_nodes, (long long)L->box.n);
+  const size_t n = (size_t)L->box.n, bytes = n * sizeof(double);
+  double* tmp = nullptr;
+  HIP_TRY(hipMalloc(&tmp, 6 * bytes));
+  hipError_t e = uclbs::launch_d3q39_macro(L->box, L->f[L->cur], L->g[L->cur],
+                                           c->cfg.precision == UCLBS_PREC_FP32_SHIFTED,
+                                           c->shifted39, c->ref39, L->gas39.cv, tmp, c->stream);
+  double* outs[6] = {rho, u, v, w, temp, press};
+  for (int k = 0; k < 6 && e == hipSuccess; ++k)
+    if (outs[k]) e = hipMemcpyAsync(outs[k], tmp + k * n, bytes, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(tmp);
+  if (e != hipSuccess) return set_err(UCLBS_E_HIP, "download_macro39 failed: %s", hipGetErrorString(e));
   return UCLBS_OK;
 }
 
@@ -1901,9 +2001,9 @@ int uclbs_advance(void* ctx, int ncoarse, int use_graph) {
         Level& L = c->levels[l];
         bool faces = false;
         for (int f = 0; f < 6; ++f) faces = faces || (!L.refined && L.face_code[f] != uclbs::kFacePeriodic);
-        if (faces && (rc = build_near39(c, &L))) return rc;
-        if (L.table39)
-          return set_err(UCLBS_E_UNSUPPORTED, "D3Q39 levels with a link table are not stepped");
+        if ((faces || L.table39) && (rc = build_near39(c, &L))) return rc;
+        if (L.table39 && !c->shifted39)
+          return set_err(UCLBS_E_UNSUPPORTED, "plain-lattice D3Q39 levels with a link table are not stepped");
       }
     const std::vector<int> sig = graph_signature(c);
     if (!c->graph || c->graph_sig != sig) {
