@@ -33,10 +33,12 @@
  *     reference's shifted branch reads a target temperature never set for
  *     such a node).
  *   derivatives (derivatives.f95:816-1236, 1236-...): along axis a of wall
- *     link i, the stencil kind is the sign of the link's unshifted component
- *     (+: (0 - u_t) / (1 + w), -: (u_t - 0) / (1 + w), 0: the local
- *     difference of u(x - e_a), u(x), u(x + e_a) over the fluid ones), except
- *     the shifted rest direction, whose x stencil is the + kind;
+ *     link i, the stencil kind of the reference's case table (+: (0 - u_t) /
+ *     (1 + w), -: (u_t - 0) / (1 + w), 0: the local difference of
+ *     u(x - e_a), u(x), u(x + e_a) over the fluid ones): the sign of the
+ *     link's unshifted component, the shifted rest direction taking the +
+ *     kind along x, and the reversed kinds for the four links (2,0,-2),
+ *     (-2,0,2), (0,2,-2), (0,-2,2);
  *     w = |s_a + 1001| 1e-9 |c_i| with s_a the node code of x + e_a (the
  *     reference's down_state); temperature stencils of the wall kinds are 0.
  *   rebuilt populations (shifted_ibb.f95:48-269):
@@ -95,13 +97,22 @@ static int64_t pos_index(int64_t a, int64_t b, int64_t cc, int nx, int ny, int n
   return s[0] + nd[0] * (s[1] + nd[1] * s[2]);
 }
 
-/* stencil kind of direction i (shifted numbering = unshifted) along axis a:
- * 1 the + kind, -1 the - kind, 0 local */
-static int dkind(int i, int a) {
-  if (i == 0) return a == 0 ? 1 : 0;
-  const int c = a == 0 ? O39_CX[i] : (a == 1 ? O39_CY[i] : O39_CZ[i]);
-  return c > 0 ? 1 : (c < 0 ? -1 : 0);
-}
+/* stencil kind of direction i along axis a (derivatives.f95:816-1235, the
+ * case table of first_order_shifted_vel_cond_unknown; the temperature
+ * function at :1236 has the same kinds): 1 where the case calls dudx_ibb with
+ * winds = .false., -1 with winds = .true., 0 where it calls dudx_local.  It
+ * is the sign of the link's unshifted component, the rest direction taking
+ * the + kind along x, except for (2,0,-2), (-2,0,2), (0,2,-2) and (0,-2,2)
+ * (Fortran directions 28, 30, 31, 33), whose cases carry the reversed kinds. */
+static const signed char WALL_KIND[NQ][3] = {
+    {1, 0, 0},   {1, 0, 0},  {0, 1, 0},  {0, 0, 1},   {-1, 0, 0},  {0, -1, 0}, {0, 0, -1},
+    {1, 1, 1},   {-1, 1, 1}, {-1, -1, 1}, {1, -1, 1}, {1, 1, -1},  {-1, 1, -1}, {-1, -1, -1},
+    {1, -1, -1}, {1, 0, 0},  {0, 1, 0},  {0, 0, 1},   {-1, 0, 0},  {0, -1, 0}, {0, 0, -1},
+    {1, 1, 0},   {1, 0, 1},  {0, 1, 1},  {-1, 1, 0},  {-1, -1, 0}, {1, -1, 0}, {-1, 0, 1},
+    {-1, 0, -1}, {1, 0, -1}, {0, -1, 1}, {0, -1, -1}, {0, 1, -1},  {1, 0, 0},  {0, 1, 0},
+    {0, 0, 1},   {-1, 0, 0}, {0, -1, 0}, {0, 0, -1}};
+
+static int dkind(int i, int a) { return WALL_KIND[i][a]; }
 
 /* dudx_local (derivatives.f95:405-418) */
 static double d_local(double rest, double up, double down, int up_ok, int down_ok) {
@@ -130,6 +141,140 @@ static void startup_lm(double mach, double lm[NM]) {
   static const double lo[NM] = {-2.0, 0.0, 0.0, 0.0, -0.4, -0.4, -0.4, 0.0, 0.0, 0.0,
                                 0.002, 0.0, 0.0};
   memcpy(lm, mach > 1.0 ? hi : lo, sizeof(double) * NM);
+}
+
+/* Velocity and temperature gradients at a wall node for wall link i
+ * (derivatives.f95:816-1236): su/sT/scode hold u, T and the node code of the
+ * 7-point stencil x, x + e_x, x + e_y, x + e_z, x - e_x, x - e_y, x - e_z (a
+ * code < 0 marks a stencil node that is not fluid).  Along axis a the kind is
+ * WALL_KIND[i][a]; + kind: (0 - u_t) / (1 + w), - kind: (u_t - 0) / (1 + w),
+ * with w = |s_a + 1001| 1e-9 |c_i| from the code s_a of x + e_a; kind 0: the
+ * central difference over fluid neighbours, else the one-sided one, else 0.
+ * Temperature gradients of the wall kinds are 0.  du[a][b] = d u_b / d x_a. */
+void o39_wall_derivs(int i, const double ut[3], double su[7][3], const double sT[7],
+                     const int scode[7], int shifted, double du[3][3], double dT[3]) {
+  int cx[NQ], cy[NQ], cz[NQ];
+  o39_velocities(shifted, cx, cy, cz);
+  const double link = sqrt((double)(cx[i] * cx[i] + cy[i] * cy[i] + cz[i] * cz[i]));
+  for (int ax = 0; ax < 3; ax++) {
+    const int kind = dkind(i, ax);
+    const double wd = fabs((double)scode[1 + ax] + 1001.0) / 1.0e9 * link;
+    const int up_ok = scode[4 + ax] >= 0, down_ok = scode[1 + ax] >= 0;
+    for (int bb = 0; bb < 3; bb++) {
+      if (kind > 0) du[ax][bb] = (0.0 - ut[bb]) / (1.0 + wd);
+      else if (kind < 0) du[ax][bb] = (ut[bb] - 0.0) / (1.0 + wd);
+      else du[ax][bb] = d_local(su[0][bb], su[4 + ax][bb], su[1 + ax][bb], up_ok, down_ok);
+    }
+    dT[ax] = kind != 0 ? 0.0 : d_local(sT[0], sT[4 + ax], sT[1 + ax], up_ok, down_ok);
+  }
+}
+
+/* The rebuilt f and g of wall link i (shifted_ibb.f95:48-259) before the
+ * magnitude repair: target rho_t, u_t, T_t with feq_i the target equilibrium
+ * of direction i, gradients du[a][b] = d u_b / d x_a and dT, ramped eps and
+ * beta_1, beta_2. */
+void o39_wall_grad(int i, double rho_t, const double ut[3], double Tt, double feq_i,
+                   double du[3][3], const double dT[3], double eps, double b1, double b2,
+                   const o39_consts *c, double *f_out, double *g_out) {
+  int cx[NQ], cy[NQ], cz[NQ];
+  o39_velocities(c->shifted, cx, cy, cz);
+  const double cv = c->cv, T = Tt, u = ut[0], v = ut[1], w = ut[2];
+  const double k_f = -(rho_t * T) / 2.0, k_g = -(rho_t * T * (2.0 * cv - 3.0)) / 2.0;
+  const double dudx = du[0][0], dudy = du[1][0], dudz = du[2][0];
+  const double dvdx = du[0][1], dvdy = du[1][1], dvdz = du[2][1];
+  const double dwdx = du[0][2], dwdy = du[1][2], dwdz = du[2][2];
+  const double dTdx = dT[0], dTdy = dT[1], dTdz = dT[2];
+  double p[28], qq[28], r[28]; /* 1-based as in shifted_ibb.f95 */
+  p[1] = 2 * dudx - dudx / cv; p[2] = 2 * dudx - dvdy / cv; p[3] = 2 * dudx - dwdz / cv;
+  p[4] = p[5] = p[6] = dudy + dvdx;
+  p[7] = p[8] = p[9] = dudz + dwdx;
+  p[10] = p[11] = p[12] = dudy + dvdx;
+  p[13] = 2 * dvdy - dudx / cv; p[14] = 2 * dvdy - dvdy / cv; p[15] = 2 * dvdy - dwdz / cv;
+  p[16] = p[17] = p[18] = dvdz + dwdy;
+  p[19] = p[20] = p[21] = dudz + dwdx;
+  p[22] = p[23] = p[24] = dvdz + dwdy;
+  p[25] = 2 * dwdz - dudx / cv; p[26] = 2 * dwdz - dvdy / cv; p[27] = 2 * dwdz - dwdz / cv;
+  for (int j = 1; j <= 27; j++) p[j] = p[j] / b1;
+  const double tx = dTdx / b2, ty = dTdy / b2, tz = dTdz / b2;
+  qq[1] = 3 * (tx + u * p[1]);
+  qq[2] = ty + v * p[2] + 2 * u * p[4];
+  qq[3] = tz + w * p[3] + 2 * u * p[7];
+  qq[4] = ty + 2 * u * p[4] + v * p[1];
+  qq[5] = tx + u * p[13] + 2 * v * p[4];
+  qq[6] = u * p[22] + v * p[7] + w * p[4];
+  qq[7] = tz + w * p[3] + 2 * u * p[7];
+  qq[8] = u * p[22] + v * p[7] + w * p[4];
+  qq[9] = tx + u * p[25] + 2 * w * p[7];
+  qq[10] = ty + v * p[2] + 2 * u * p[4];
+  qq[11] = tx + 2 * v * p[4] + u * p[13];
+  qq[12] = u * p[22] + v * p[7] + w * p[4];
+  qq[13] = tx + u * p[13] + 2 * v * p[4];
+  qq[14] = 3 * (ty + v * p[14]);
+  qq[15] = tz + w * p[15] + 2 * v * p[22];
+  qq[16] = u * p[22] + v * p[7] + w * p[4];
+  qq[17] = tz + w * p[15] + 2 * v * p[22];
+  qq[18] = ty + v * p[26] + 2 * w * p[22];
+  qq[19] = tx + w * p[3] + 2 * u * p[7];
+  qq[20] = u * p[22] + v * p[7] + w * p[4];
+  qq[21] = tx + u * p[25] + 2 * w * p[7];
+  qq[22] = u * p[22] + v * p[7] + w * p[4];
+  qq[23] = tz + w * p[15] + 2 * v * p[22];
+  qq[24] = ty + v * p[26] + 2 * w * p[22];
+  qq[25] = tx + u * p[25] + 2 * w * p[7];
+  qq[26] = ty + v * p[26] + 2 * w * p[22];
+  qq[27] = 3 * (tz + w * p[27]);
+  const double X = cx[i], Y = cy[i], Z = cz[i];
+  /* second-order basis of the 27 (a, b, *) entries */
+  const double sxx = X * X - T, syy = Y * Y - T, szz = Z * Z - T;
+  const double pc[28] = {0,   sxx, sxx, sxx, X * Y, X * Y, X * Y, X * Z, X * Z, X * Z,
+                         Y * X, Y * X, Y * X, syy, syy, syy, Y * Z, Y * Z, Y * Z,
+                         Z * X, Z * X, Z * X, Z * Y, Z * Y, Z * Y, szz, szz, szz};
+  double psum = 0.0;
+  for (int j = 1; j <= 27; j++) psum += p[j] * pc[j];
+  psum += p[25] * szz; /* P_zzx counted twice (shifted_ibb.f95:170) */
+  const double p_comp = psum / (2 * T * T);
+  const double qc[28] = {0,
+                         X * X * X - 3 * X * T, X * X * Y - 3 * Y * T, X * X * Z - 3 * Z * T,
+                         X * X * Y, X * Y * Y, X * Y * Z, X * X * Z, X * Y * Z, X * Z * Z,
+                         X * X * Y, Y * Y * X, X * Y * Z,
+                         Y * Y * X - 3 * X * T, Y * Y * Y - 3 * Y * T, Y * Y * Z - 3 * Z * T,
+                         X * Y * Z, Y * Y * Z, Z * Z * Y, X * X * Z, X * Y * Z, Z * Z * X,
+                         X * Y * Z, Z * Y * Y, Z * Z * Y,
+                         Z * Z * X - 3 * X * T, Z * Z * Y - 3 * Y * T, Z * Z * Z - 3 * Z * T};
+  double qsum = 0.0;
+  for (int j = 1; j <= 27; j++) qsum += qq[j] * qc[j];
+  const double q_comp = qsum / (6 * T * T * T);
+  const double fo = feq_i + eps * k_f * (p_comp + q_comp);
+  const double k_one = 3 * (dTdx + dTdy + dTdz) / (b1 * cv);
+  const double g_alpha = X / T * (9 * dTdx / b2 + u * k_one) + Y / T * (9 * dTdy / b2 + v * k_one) +
+                         Z / T * (9 * dTdz / b2 + w * k_one);
+  r[1] = 2 * T * dudx + 2 * u * dTdx - dudx / cv * (2 * T + u * u);
+  r[2] = 2 * T * dudx + 2 * u * dTdx - dvdy / cv * (2 * T + u * u);
+  r[3] = 2 * T * dudx + 2 * u * dTdx - dwdz / cv * (2 * T + u * u);
+  r[4] = T * (dudy + dvdx) + u * dTdy + v * dTdx - dudx / cv * u * v;
+  r[5] = T * (dudy + dvdx) + u * dTdy + v * dTdx - dvdy / cv * u * v;
+  r[6] = T * (dudy + dvdx) + u * dTdy + v * dTdx - dwdz / cv * u * v;
+  r[7] = T * (dudz + dwdx) + u * dTdz + w * dTdx - dudx / cv * u * w;
+  r[8] = T * (dudz + dwdx) + u * dTdz + w * dTdx - dvdy / cv * u * w;
+  r[9] = T * (dudz + dwdx) + u * dTdz + w * dTdx - dwdz / cv * u * w;
+  r[10] = r[4]; r[11] = r[5]; r[12] = r[6];
+  r[13] = 2 * T * dvdy + 2 * v * dTdy - dudx / cv * (2 * T + v * v);
+  r[14] = 2 * T * dvdy + 2 * v * dTdy - dvdy / cv * (2 * T + v * v);
+  r[15] = 2 * T * dvdy + 2 * v * dTdy - dwdz / cv * (2 * T + v * v);
+  r[16] = T * (dvdz + dwdy) + w * dTdy + v * dTdz - dudx / cv * v * w;
+  r[17] = T * (dvdz + dwdy) + w * dTdy + v * dTdz - dvdy / cv * v * w;
+  r[18] = T * (dvdz + dwdy) + w * dTdy + v * dTdz - dwdz / cv * v * w;
+  r[19] = r[7]; r[20] = r[8]; r[21] = r[9];
+  r[22] = r[13]; r[23] = r[14]; r[24] = r[15]; /* the zy entries repeat yy (:231-233) */
+  r[25] = 2 * T * dwdz + 2 * w * dTdz - dudx / cv * (2 * T + w * w);
+  r[26] = 2 * T * dwdz + 2 * w * dTdz - dvdy / cv * (2 * T + w * w);
+  r[27] = 2 * T * dwdz + 2 * w * dTdz - dwdz / cv * (2 * T + w * w);
+  double rsum = 0.0;
+  for (int j = 1; j <= 27; j++) rsum += r[j] * pc[j];
+  const double gi_neq = k_g * (3 * k_one + g_alpha + rsum / (2 * b1 * T));
+  const double go = c->lead_g * T * feq_i + eps * gi_neq;
+  *f_out = fo;
+  *g_out = go;
 }
 
 /* The rebuilt f and g of the wall links of fluid node k (positions a, b, cc).
@@ -212,117 +357,13 @@ int o39_wall_rebuild(int nx, int ny, int nz, const int32_t fc[6], const int32_t 
   if (o39_therm_cond(rho_t, Tt, c, &kappa)) *kerr = 1;
   double b1, b2;
   betas(mu, kappa, rho_t, Tt, step, c, &b1, &b2);
-  const double cv = c->cv, T = Tt, u = ut[0], v = ut[1], w = ut[2];
-  const double k_f = -(rho_t * T) / 2.0, k_g = -(rho_t * T * (2.0 * cv - 3.0)) / 2.0;
   int nrep = 0;
   for (int i = 1; i < NQ; i++) {
     if (state[(int64_t)i * n + k] > -1001) continue;
-    const double link = sqrt((double)(cx[i] * cx[i] + cy[i] * cy[i] + cz[i] * cz[i]));
-    double du[3][3], dT[3]; /* du[a][b] = d u_b / d x_a */
-    for (int ax = 0; ax < 3; ax++) {
-      const int kind = dkind(i, ax);
-      const double wd = fabs((double)scode[1 + ax] + 1001.0) / 1.0e9 * link;
-      const int up_ok = scode[4 + ax] >= 0, down_ok = scode[1 + ax] >= 0;
-      for (int bb = 0; bb < 3; bb++) {
-        if (kind > 0) du[ax][bb] = (0.0 - ut[bb]) / (1.0 + wd);
-        else if (kind < 0) du[ax][bb] = (ut[bb] - 0.0) / (1.0 + wd);
-        else du[ax][bb] = d_local(su[0][bb], su[4 + ax][bb], su[1 + ax][bb], up_ok, down_ok);
-      }
-      dT[ax] = kind != 0 ? 0.0 : d_local(sT[0], sT[4 + ax], sT[1 + ax], up_ok, down_ok);
-    }
-    const double dudx = du[0][0], dudy = du[1][0], dudz = du[2][0];
-    const double dvdx = du[0][1], dvdy = du[1][1], dvdz = du[2][1];
-    const double dwdx = du[0][2], dwdy = du[1][2], dwdz = du[2][2];
-    const double dTdx = dT[0], dTdy = dT[1], dTdz = dT[2];
-    double p[28], qq[28], r[28]; /* 1-based as in shifted_ibb.f95 */
-    p[1] = 2 * dudx - dudx / cv; p[2] = 2 * dudx - dvdy / cv; p[3] = 2 * dudx - dwdz / cv;
-    p[4] = p[5] = p[6] = dudy + dvdx;
-    p[7] = p[8] = p[9] = dudz + dwdx;
-    p[10] = p[11] = p[12] = dudy + dvdx;
-    p[13] = 2 * dvdy - dudx / cv; p[14] = 2 * dvdy - dvdy / cv; p[15] = 2 * dvdy - dwdz / cv;
-    p[16] = p[17] = p[18] = dvdz + dwdy;
-    p[19] = p[20] = p[21] = dudz + dwdx;
-    p[22] = p[23] = p[24] = dvdz + dwdy;
-    p[25] = 2 * dwdz - dudx / cv; p[26] = 2 * dwdz - dvdy / cv; p[27] = 2 * dwdz - dwdz / cv;
-    for (int j = 1; j <= 27; j++) p[j] = p[j] / b1;
-    const double tx = dTdx / b2, ty = dTdy / b2, tz = dTdz / b2;
-    qq[1] = 3 * (tx + u * p[1]);
-    qq[2] = ty + v * p[2] + 2 * u * p[4];
-    qq[3] = tz + w * p[3] + 2 * u * p[7];
-    qq[4] = ty + 2 * u * p[4] + v * p[1];
-    qq[5] = tx + u * p[13] + 2 * v * p[4];
-    qq[6] = u * p[22] + v * p[7] + w * p[4];
-    qq[7] = tz + w * p[3] + 2 * u * p[7];
-    qq[8] = u * p[22] + v * p[7] + w * p[4];
-    qq[9] = tx + u * p[25] + 2 * w * p[7];
-    qq[10] = ty + v * p[2] + 2 * u * p[4];
-    qq[11] = tx + 2 * v * p[4] + u * p[13];
-    qq[12] = u * p[22] + v * p[7] + w * p[4];
-    qq[13] = tx + u * p[13] + 2 * v * p[4];
-    qq[14] = 3 * (ty + v * p[14]);
-    qq[15] = tz + w * p[15] + 2 * v * p[22];
-    qq[16] = u * p[22] + v * p[7] + w * p[4];
-    qq[17] = tz + w * p[15] + 2 * v * p[22];
-    qq[18] = ty + v * p[26] + 2 * w * p[22];
-    qq[19] = tx + w * p[3] + 2 * u * p[7];
-    qq[20] = u * p[22] + v * p[7] + w * p[4];
-    qq[21] = tx + u * p[25] + 2 * w * p[7];
-    qq[22] = u * p[22] + v * p[7] + w * p[4];
-    qq[23] = tz + w * p[15] + 2 * v * p[22];
-    qq[24] = ty + v * p[26] + 2 * w * p[22];
-    qq[25] = tx + u * p[25] + 2 * w * p[7];
-    qq[26] = ty + v * p[26] + 2 * w * p[22];
-    qq[27] = 3 * (tz + w * p[27]);
-    const double X = cx[i], Y = cy[i], Z = cz[i];
-    /* second-order basis of the 27 (a, b, *) entries */
-    const double sxx = X * X - T, syy = Y * Y - T, szz = Z * Z - T;
-    const double pc[28] = {0,   sxx, sxx, sxx, X * Y, X * Y, X * Y, X * Z, X * Z, X * Z,
-                           Y * X, Y * X, Y * X, syy, syy, syy, Y * Z, Y * Z, Y * Z,
-                           Z * X, Z * X, Z * X, Z * Y, Z * Y, Z * Y, szz, szz, szz};
-    double psum = 0.0;
-    for (int j = 1; j <= 27; j++) psum += p[j] * pc[j];
-    psum += p[25] * szz; /* P_zzx counted twice (shifted_ibb.f95:170) */
-    const double p_comp = psum / (2 * T * T);
-    const double qc[28] = {0,
-                           X * X * X - 3 * X * T, X * X * Y - 3 * Y * T, X * X * Z - 3 * Z * T,
-                           X * X * Y, X * Y * Y, X * Y * Z, X * X * Z, X * Y * Z, X * Z * Z,
-                           X * X * Y, Y * Y * X, X * Y * Z,
-                           Y * Y * X - 3 * X * T, Y * Y * Y - 3 * Y * T, Y * Y * Z - 3 * Z * T,
-                           X * Y * Z, Y * Y * Z, Z * Z * Y, X * X * Z, X * Y * Z, Z * Z * X,
-                           X * Y * Z, Z * Y * Y, Z * Z * Y,
-                           Z * Z * X - 3 * X * T, Z * Z * Y - 3 * Y * T, Z * Z * Z - 3 * Z * T};
-    double qsum = 0.0;
-    for (int j = 1; j <= 27; j++) qsum += qq[j] * qc[j];
-    const double q_comp = qsum / (6 * T * T * T);
-    double fo = feq[i] + eps * k_f * (p_comp + q_comp);
-    const double k_one = 3 * (dTdx + dTdy + dTdz) / (b1 * cv);
-    const double g_alpha = X / T * (9 * dTdx / b2 + u * k_one) + Y / T * (9 * dTdy / b2 + v * k_one) +
-                           Z / T * (9 * dTdz / b2 + w * k_one);
-    r[1] = 2 * T * dudx + 2 * u * dTdx - dudx / cv * (2 * T + u * u);
-    r[2] = 2 * T * dudx + 2 * u * dTdx - dvdy / cv * (2 * T + u * u);
-    r[3] = 2 * T * dudx + 2 * u * dTdx - dwdz / cv * (2 * T + u * u);
-    r[4] = T * (dudy + dvdx) + u * dTdy + v * dTdx - dudx / cv * u * v;
-    r[5] = T * (dudy + dvdx) + u * dTdy + v * dTdx - dvdy / cv * u * v;
-    r[6] = T * (dudy + dvdx) + u * dTdy + v * dTdx - dwdz / cv * u * v;
-    r[7] = T * (dudz + dwdx) + u * dTdz + w * dTdx - dudx / cv * u * w;
-    r[8] = T * (dudz + dwdx) + u * dTdz + w * dTdx - dvdy / cv * u * w;
-    r[9] = T * (dudz + dwdx) + u * dTdz + w * dTdx - dwdz / cv * u * w;
-    r[10] = r[4]; r[11] = r[5]; r[12] = r[6];
-    r[13] = 2 * T * dvdy + 2 * v * dTdy - dudx / cv * (2 * T + v * v);
-    r[14] = 2 * T * dvdy + 2 * v * dTdy - dvdy / cv * (2 * T + v * v);
-    r[15] = 2 * T * dvdy + 2 * v * dTdy - dwdz / cv * (2 * T + v * v);
-    r[16] = T * (dvdz + dwdy) + w * dTdy + v * dTdz - dudx / cv * v * w;
-    r[17] = T * (dvdz + dwdy) + w * dTdy + v * dTdz - dvdy / cv * v * w;
-    r[18] = T * (dvdz + dwdy) + w * dTdy + v * dTdz - dwdz / cv * v * w;
-    r[19] = r[7]; r[20] = r[8]; r[21] = r[9];
-    r[22] = r[13]; r[23] = r[14]; r[24] = r[15]; /* the zy entries repeat yy (:231-233) */
-    r[25] = 2 * T * dwdz + 2 * w * dTdz - dudx / cv * (2 * T + w * w);
-    r[26] = 2 * T * dwdz + 2 * w * dTdz - dvdy / cv * (2 * T + w * w);
-    r[27] = 2 * T * dwdz + 2 * w * dTdz - dwdz / cv * (2 * T + w * w);
-    double rsum = 0.0;
-    for (int j = 1; j <= 27; j++) rsum += r[j] * pc[j];
-    const double gi_neq = k_g * (3 * k_one + g_alpha + rsum / (2 * b1 * T));
-    double go = c->lead_g * T * feq[i] + eps * gi_neq;
+    double du[3][3], dT[3];
+    o39_wall_derivs(i, ut, su, sT, scode, c->shifted, du, dT);
+    double fo, go;
+    o39_wall_grad(i, rho_t, ut, Tt, feq[i], du, dT, eps, b1, b2, c, &fo, &go);
     if (fo < 0.0) { fo = fabs(fo); nrep++; }
     if (go < 0.0) { go = fabs(go); nrep++; }
     fw[i] = fo;

@@ -149,6 +149,12 @@ def lib() -> ctypes.CDLL:
         L.o39_step_walls.argtypes = [I, I, I, ip, ip, dp, dp, dp, dp, dp, dp, c39, ctypes.c_int64,
                                      ctypes.c_int64, I, I, ctypes.POINTER(Stats39),
                                      ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
+        L.o39_wall_derivs.argtypes = [I, dp, dp, dp, ip, I, dp, dp]
+        L.o39_wall_grad.argtypes = [I, D, dp, D, D, dp, dp, D, D, D, c39, dp, dp]
+        L.o39_wall_rebuild.argtypes = [I, I, I, ip, ip, dp, dp, dp, ctypes.c_int64, dp, c39,
+                                       ctypes.c_int64, ctypes.c_int64, dp, dp,
+                                       ctypes.POINTER(I), ctypes.POINTER(I)]
+        L.o39_wall_rebuild.restype = I
         for nm in ("o_tri_volume", "o_box_test"):
             getattr(L, nm).argtypes = [dp, dp, dp]
         L.o_link_code.argtypes = [dp, dp, D]
@@ -593,6 +599,51 @@ def step_walls39(shape, faces, state, f, g, lm, c, step0=0, nsteps=1, lvl_step0=
                          step0 if lvl_step0 is None else lvl_step0, nsteps, int(jacobi),
                          ctypes.byref(st), ctypes.byref(nwl), ctypes.byref(nwr))
     return f, g, lm, alpha, st.as_dict(), nwl.value, nwr.value
+
+
+def wall_derivs39(i, ut, su, sT, scode, shifted=True):
+    """Gradients of wall link i from the 7-point stencil (su (7, 3), sT (7,),
+    node codes scode (7,)): returns (du (3, 3) with du[a, b] = d u_b / d x_a,
+    dT (3,))."""
+    du = np.zeros((3, 3))
+    dT = np.zeros(3)
+    lib().o39_wall_derivs(int(i), _dp(np.ascontiguousarray(ut, np.float64)),
+                          _dp(np.ascontiguousarray(su, np.float64)),
+                          _dp(np.ascontiguousarray(sT, np.float64)),
+                          np.ascontiguousarray(scode, np.int32).ctypes.data_as(
+                              ctypes.POINTER(ctypes.c_int32)), int(shifted), _dp(du), _dp(dT))
+    return du, dT
+
+
+def wall_grad39(i, rho_t, ut, Tt, feq_i, du, dT, eps, b1, b2, c):
+    """Rebuilt (f, g) of wall link i before the magnitude repair."""
+    fo, go = np.zeros(1), np.zeros(1)
+    lib().o39_wall_grad(int(i), float(rho_t), _dp(np.ascontiguousarray(ut, np.float64)),
+                        float(Tt), float(feq_i), _dp(np.ascontiguousarray(du, np.float64)),
+                        _dp(np.ascontiguousarray(dT, np.float64)), float(eps), float(b1),
+                        float(b2), ctypes.byref(c), _dp(fo), _dp(go))
+    return float(fo[0]), float(go[0])
+
+
+def wall_rebuild39(shape, faces, state, f, g, lm, k, fi, c, step, lvl_step):
+    """Wall-link populations of fluid node k rebuilt from the input buffers
+    f, g, lm and its pulled populations fi: (fw, gw, n_repair, noconv, kerr);
+    entries of non-wall links are left at NaN."""
+    nx, ny, nz = shape
+    fc = _faces39(faces, True)
+    fw = np.full(39, np.nan)
+    gw = np.full(39, np.nan)
+    nc, ke = ctypes.c_int(), ctypes.c_int()
+    ip = ctypes.POINTER(ctypes.c_int32)
+    st = np.ascontiguousarray(state, np.int32)
+    nrep = lib().o39_wall_rebuild(nx, ny, nz, fc.ctypes.data_as(ip), st.ctypes.data_as(ip),
+                                  _dp(np.ascontiguousarray(f, np.float64)),
+                                  _dp(np.ascontiguousarray(g, np.float64)),
+                                  _dp(np.ascontiguousarray(lm, np.float64)), int(k),
+                                  _dp(np.ascontiguousarray(fi, np.float64)), ctypes.byref(c),
+                                  int(step), int(lvl_step), _dp(fw), _dp(gw), ctypes.byref(nc),
+                                  ctypes.byref(ke))
+    return fw, gw, nrep, nc.value, ke.value
 
 
 def source39(node, k, shape, faces, shifted=False):

@@ -175,6 +175,13 @@ void o39_step_ghost(int nx, int ny, int nz, int ghost, double *f, double *g, dou
 int o39_source(int64_t a, int64_t b, int64_t cc, int k, int nx, int ny, int nz, const int32_t fc[6],
                int shifted, int64_t *src, int *dir);
 /* o_walls39.c: the shifted lattice's wall rule and a step with a link table */
+/* wall-link gradients and rebuilt populations of the shifted-lattice wall
+ * rule (o_walls39.c) */
+void o39_wall_derivs(int i, const double ut[3], double su[7][3], const double sT[7],
+                     const int scode[7], int shifted, double du[3][3], double dT[3]);
+void o39_wall_grad(int i, double rho_t, const double ut[3], double Tt, double feq_i,
+                   double du[3][3], const double dT[3], double eps, double b1, double b2,
+                   const o39_consts *c, double *f_out, double *g_out);
 int o39_wall_rebuild(int nx, int ny, int nz, const int32_t fc[6], const int32_t *state,
                      const double *f, const double *g, const double *lm, int64_t k,
                      const double fi[39], const o39_consts *c, int64_t step, int64_t lvl_step,

@@ -495,7 +495,7 @@ __device__ __forceinline__ long long pos39(int x, int y, int z, const Box& b, co
 // multipliers by Newton from the node's (or, while the level's counter is
 // <= 4, from fixed start-up guesses) -- else the node's own state; the
 // velocity and temperature gradients from the 7-point stencil (wall-kind
-// stencils by the sign of the link's unshifted component); then for every
+// stencils from the reference's case table, WALLKIND39); then for every
 // wall link f_i = feq_i(target) + eps k_f (P_comp + Q_comp) and
 // g_i = (2 cv - 3) T_t feq_i + eps k_g (3 k1 + g_a + R_comp) with the
 // reference's 27-entry tensors and term slips, a negative value replaced by
@@ -631,10 +631,9 @@ __global__ __launch_bounds__(64) void k39_wall_rebuild(const void* __restrict__ 
     ++nlinks;
     const int X = cx39<true>(i), Y = CY39[i], Z = CZ39[i];
     const double link = sqrt((double)(X * X + Y * Y + Z * Z));
-    const int ud[3] = {CX39[i], CY39[i], CZ39[i]};  // unshifted components: stencil kinds
     double du[3][3], dT[3];
     for (int ax = 0; ax < 3; ++ax) {
-      const int kind = ud[ax] > 0 ? 1 : (ud[ax] < 0 ? -1 : 0);
+      const int kind = WALLKIND39[i][ax];
       const double wd = fabs((double)scode[1 + ax] + 1001.0) / 1.0e9 * link;
       const bool up_ok = scode[4 + ax] >= 0, down_ok = scode[1 + ax] >= 0;
       for (int bb = 0; bb < 3; ++bb) {

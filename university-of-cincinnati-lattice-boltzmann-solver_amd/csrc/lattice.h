@@ -37,6 +37,19 @@ constexpr int OPP39[39] = {0, 4, 5, 6, 1, 2, 3, 13, 14, 11, 12, 9, 10, 7, 8, 18,
                            17, 25, 28, 31, 26, 21, 24, 29, 22, 27, 32, 23, 30, 36, 37, 38, 33,
                            34, 35};
 
+// Stencil kind of the shifted-lattice wall derivatives of link i along axis a
+// (derivatives.f95:816-1235): 1 for the case's dudx_ibb with winds = .false.,
+// -1 with winds = .true., 0 for the local difference.  The sign of the
+// unshifted component (the rest direction: + along x), except for (2,0,-2),
+// (-2,0,2), (0,2,-2), (0,-2,2), whose cases carry the reversed kinds.
+constexpr signed char WALLKIND39[39][3] = {
+    {1, 0, 0},   {1, 0, 0},  {0, 1, 0},   {0, 0, 1},   {-1, 0, 0},  {0, -1, 0}, {0, 0, -1},
+    {1, 1, 1},   {-1, 1, 1}, {-1, -1, 1}, {1, -1, 1},  {1, 1, -1},  {-1, 1, -1}, {-1, -1, -1},
+    {1, -1, -1}, {1, 0, 0},  {0, 1, 0},   {0, 0, 1},   {-1, 0, 0},  {0, -1, 0}, {0, 0, -1},
+    {1, 1, 0},   {1, 0, 1},  {0, 1, 1},   {-1, 1, 0},  {-1, -1, 0}, {1, -1, 0}, {-1, 0, 1},
+    {-1, 0, -1}, {1, 0, -1}, {0, -1, 1},  {0, -1, -1}, {0, 1, -1},  {1, 0, 0},  {0, 1, 0},
+    {0, 0, 1},   {-1, 0, 0}, {0, -1, 0},  {0, 0, -1}};
+
 // kMirror39.m[d][k]: the direction whose velocity is c_k with component d negated
 // (the specular image at a wall normal to axis d).  Built from the un-shifted
 // velocities; for d = y, z it holds on the shifted lattice as well, whose
