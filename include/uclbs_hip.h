@@ -150,8 +150,13 @@ int uclbs_set_bc_states(void *ctx, int n_in, const double *in_states, int n_out,
  * the long links derived from the shorter ones as state_directions.f95:95-185
  * does (integer divisions and its slot slip for the (0,3,0) link included);
  * an outflow face gives its links the outside node's code 0.  Such a level is
- * not stepped (no D3Q39 wall-link rule in this library); the shifted lattice's
- * table (shifted_state_directions.f95) is refused. */
+ * not stepped (the plain-lattice D3Q39 wall rule is not in this library).
+ * On a D3Q39 context of the shifted lattice the table is that of
+ * lbgrid/shifted_state_directions.f95: link i of node x is keyed on its
+ * upstream node x - c_i (a wall code where that node is solid, the face code
+ * where it lies across a face, the downstream node's code otherwise), and
+ * stepping rebuilds the wall links by the shifted-lattice wall rule
+ * (shifted_ibb.f95, target_finder.f95:87-127, derivatives.f95:816-1236). */
 int uclbs_build_links(void *ctx, int lvl, const double origin[3], double dx, int64_t ntri,
                       const double *tri, const double *nrm, int64_t *n_solid, int64_t *n_tie);
 /* Takes a reference-form table state[q*n + node] (int32, the `state` imultifab
