@@ -203,7 +203,7 @@ def bench_c3(pkg, args):
     s.sync()
     wall = time.perf_counter() - t0
     st = s.stats()
-    rest = max(0, C3_STEPS - args.warmup - args.steps)
+    rest = max(0, args.c3_total - args.warmup - args.steps)
     s.step(0, rest)
     rho = s.macro39(0)[0].reshape(shape[2], shape[1], shape[0])[shape[2] // 2]
     solid = (s.download_links(0)[0] < 0).reshape(shape[2], shape[1], shape[0])[shape[2] // 2]
@@ -339,6 +339,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--workload", default="c2", choices=["c2", "tgv", "c3", "c4"])
+    ap.add_argument("--c3-total", type=int, default=C3_STEPS,
+                    help="total steps of the c3 run from the uniform start (the shock is "
+                         "measured at the end; shorter runs serve profiling)")
     ap.add_argument("--nu", type=float, default=0.002, help="C2 viscosity (lattice units)")
     ap.add_argument("--alpha", default="newton", choices=["newton", "fixed2", "learned"],
                     help="C2: the timed form; the learned-vs-plain comparison runs after "

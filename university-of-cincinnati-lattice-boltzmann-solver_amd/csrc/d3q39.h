@@ -65,29 +65,10 @@ __host__ __device__ inline double expo39(const double* __restrict__ lm, double r
   return rho * exp(e);
 }
 
-// Monomials of every direction for both lattices as a constant table: the
-// Newton loop walks the 39 directions three at a time (the fully unrolled
-// Jacobian, with its weights folded into ~2k specialised FMAs, made the kernel
-// six times slower, measured: its code no longer fits the instruction cache;
-// three directions per pass overlap their exponential chains, 8% faster than
-// one), reading each direction's 13 weights as FP64 scalars (stored as
-// doubles: no conversion per use, 8% faster than a byte table).
-struct Mono39Table {
-  double v[2][39][13];
-  constexpr Mono39Table() : v() {
-    for (int sh = 0; sh < 2; ++sh)
-      for (int i = 0; i < 39; ++i)
-        for (int k = 0; k < 13; ++k)
-          v[sh][i][k] = (double)(sh ? mono39<true>(i, k) : mono39<false>(i, k));
-  }
-};
-__constant__ constexpr Mono39Table kMonoT{};
-#define kMono39 kMonoT.v
-
 // Jacobian entries that hold the same moment: J_kl = sum_i e_i m_k(c_i) m_l(c_i)
-// depends only on the weights m_k m_l over the 39 directions, and only 60 of
-// the 91 (k, l) pairs have distinct weight vectors (x*x = 1*x^2, x^2*y^2 =
-// xy*xy, ...).  rep[p] is the first pair of class p, cls[jidx(k, l)] its class.
+// depends only on the weights m_k m_l over the 39 directions, and only 59 of
+// the 91 (k, l) pairs have distinct weight vectors on the shifted lattice (57
+// on the plain one: x*x = 1*x^2, x^2*y^2 = xy*xy, ...).  rep[p] is the first pair of class p, cls[jidx(k, l)] its class.
 struct JDup39 {
   int n[2];
   signed char rk[2][91], rl[2][91], cls[2][91];
@@ -125,6 +106,71 @@ struct NoSink39 {
   __host__ __device__ void operator()(int, double) const {}
 };
 
+// Directions grouped by which velocity components are zero (bit 0: c_x, bit 1:
+// c_y, bit 2: c_z).  Half of the 39 x 59 Jacobian weights m_k m_l vanish
+// because a component does (1,171 of 2,301 on the shifted lattice, 931 of 2,223
+// on the plain one); within a group the vanishing ones are known at compile
+// time, so the accumulation of a group skips them.  Each group is unrolled
+// with its weights as literals: with the zeros gone the unrolled Newton body
+// fits the instruction cache, and the weights no longer wait on scalar loads
+// from a constant table, which is what bounded the kernel at one wave per
+// SIMD (C3 step 8.9 -> 7.2 ms; skipping the zeros alone, with the weights
+// still loaded, gave nothing).
+template <bool kShifted>
+__host__ __device__ constexpr int zpat39(int i) {
+  return (cx39<kShifted>(i) == 0 ? 1 : 0) | (CY39[i] == 0 ? 2 : 0) | (CZ39[i] == 0 ? 4 : 0);
+}
+
+struct Pat39Table {
+  int count[2][8];
+  int dir[2][8][39];
+  constexpr Pat39Table() : count(), dir() {
+    for (int sh = 0; sh < 2; ++sh)
+      for (int i = 0; i < 39; ++i) {
+        const int p = sh ? zpat39<true>(i) : zpat39<false>(i);
+        dir[sh][p][count[sh][p]++] = i;
+      }
+  }
+};
+constexpr Pat39Table kPatC{};
+
+// monomial k vanishes on every direction of zero pattern pat
+__host__ __device__ constexpr bool mono_zero39(int pat, int k) {
+  const bool zx = pat & 1, zy = pat & 2, zz = pat & 4;
+  return k == 0 ? false : (k == 1 || k == 4 || k == 10) ? zx : (k == 2 || k == 5 || k == 11) ? zy
+       : (k == 3 || k == 6 || k == 12) ? zz : k == 7 ? (zx || zy) : k == 8 ? (zx || zz)
+       : (zy || zz);
+}
+
+// equilibrium of the directions of zero pattern kPat and their Jacobian terms
+template <bool kShifted, int kPat, class Sink>
+__host__ __device__ __forceinline__ void jac_group39(const double* __restrict__ lm, double rho,
+                                                     double* __restrict__ Jd, Sink& sink) {
+  constexpr int sh = kShifted ? 1 : 0;
+  constexpr int cnt = kPatC.count[sh][kPat];
+  constexpr int nd = kJDup39.n[sh];
+#pragma unroll
+  for (int j = 0; j < cnt; ++j) {
+    const int i = kPatC.dir[sh][kPat][j];
+    double ex = lm[0];
+#pragma unroll
+    for (int k = 1; k < kM39; ++k)
+      if (!mono_zero39(kPat, k)) ex = fma((double)mono39<kShifted>(i, k), lm[k], ex);
+    const double e = rho * exp(ex);
+    sink(i, e);
+    double me[kM39];
+    me[0] = e;
+#pragma unroll
+    for (int k = 1; k < kM39; ++k) me[k] = mono_zero39(kPat, k) ? 0.0 : (double)mono39<kShifted>(i, k) * e;
+#pragma unroll
+    for (int p = 0; p < nd; ++p) {
+      const int a = kJDup39.rl[sh][p], b = kJDup39.rk[sh][p];
+      if (!mono_zero39(kPat, a) && !mono_zero39(kPat, b))
+        Jd[p] = fma((double)mono39<kShifted>(i, a), me[b], Jd[p]);
+    }
+  }
+}
+
 // Newton on the moment constraints; lm in/out, lm_prev = multipliers of the
 // last iteration before its update; sink(i, e) receives each iteration's
 // equilibrium (so the caller keeps the last one).  Returns iterations, or -1
@@ -141,22 +187,14 @@ __host__ __device__ inline int lm_newton39(double rho, const double tg[kM39], do
     double Jd[nd];
 #pragma unroll
     for (int p = 0; p < nd; ++p) Jd[p] = 0.0;
-#pragma unroll 3
-    for (int i = 0; i < Q39; ++i) {
-      double ex = lm[0];
-#pragma unroll
-      for (int k = 1; k < kM39; ++k) ex = fma((double)kMono39[kShifted][i][k], lm[k], ex);
-      const double e = rho * exp(ex);
-      sink(i, e);
-      double me[kM39];
-#pragma unroll
-      for (int k = 0; k < kM39; ++k) me[k] = (double)kMono39[kShifted][i][k] * e;
-#pragma unroll
-      for (int p = 0; p < nd; ++p) {
-        constexpr int sh = kShifted ? 1 : 0;
-        Jd[p] = fma((double)kMono39[kShifted][i][kJDup39.rl[sh][p]], me[kJDup39.rk[sh][p]], Jd[p]);
-      }
-    }
+    jac_group39<kShifted, 0>(lm, rho, Jd, sink);
+    jac_group39<kShifted, 1>(lm, rho, Jd, sink);
+    jac_group39<kShifted, 2>(lm, rho, Jd, sink);
+    jac_group39<kShifted, 3>(lm, rho, Jd, sink);
+    jac_group39<kShifted, 4>(lm, rho, Jd, sink);
+    jac_group39<kShifted, 5>(lm, rho, Jd, sink);
+    jac_group39<kShifted, 6>(lm, rho, Jd, sink);
+    jac_group39<kShifted, 7>(lm, rho, Jd, sink);
     double J[kJ39];
 #pragma unroll
     for (int q = 0; q < kJ39; ++q) J[q] = Jd[kJDup39.cls[kShifted ? 1 : 0][q]];
