@@ -25,7 +25,9 @@
 #pragma clang fp contract(off)
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
+#include <vector>
 
 #include "kernels.h"
 #include "lattice.h"
@@ -70,33 +72,65 @@ __device__ __forceinline__ int tri_volume(const double* xc, const double* e, con
   return 0;
 }
 
-// parity of one cast: 0 even, 1 odd, -1 tie
-__device__ int cast_parity(const double* e1, const double* e2, int ntri,
-                           const double* __restrict__ tri, const double* __restrict__ nrm,
-                           const double* __restrict__ tbb) {
+// Candidate triangles of a segment: the triangles whose box box_out does not
+// cull, found through the hierarchy.  A node whose box (the exact union of its
+// triangles' boxes) box_out culls holds only triangles it culls one by one
+// (the comparisons are monotone in the box bounds), so the candidates are the
+// same set the reference's scan over every triangle keeps; visit(i) is called
+// for each, in no particular order, and stops the walk by returning false.
+template <class Visit>
+__device__ __forceinline__ void for_candidates(const LinkGeom& g, const double* e1,
+                                               const double* e2, Visit visit) {
+  if (g.ntri <= 0) return;
+  int stack[40];
+  int sp = 0;
+  stack[sp++] = 0;
+  while (sp > 0) {
+    const BvhNode& nd = g.bvh[stack[--sp]];
+    if (box_out(e1, e2, nd.bb)) continue;
+    if (nd.b < 0) {
+      for (int j = 0; j < -nd.b; ++j) {
+        const int i = g.bvh_idx[nd.a + j];
+        if (box_out(e1, e2, g.tbb + 6 * (size_t)i)) continue;
+        if (!visit(i)) return;
+      }
+    } else {
+      stack[sp++] = nd.b;
+      stack[sp++] = nd.a;
+    }
+  }
+}
+
+// parity of one cast: 0 even, 1 odd, -1 tie.  The parity counts do not depend
+// on the order of the triangles and any tie gives -1, so the walk order of the
+// hierarchy gives the reference's result.
+__device__ int cast_parity(const LinkGeom& g, const double* e1, const double* e2) {
   int cross = 0, edge = 0;
-  for (int i = 0; i < ntri; ++i) {
-    if (box_out(e1, e2, tbb + 6 * (size_t)i)) continue;
-    const double* t = tri + 9 * (size_t)i;
-    const double* n = nrm + 3 * (size_t)i;
+  bool tie = false;
+  for_candidates(g, e1, e2, [&](int i) {
+    const double* t = g.tri + 9 * (size_t)i;
+    const double* n = g.nrm + 3 * (size_t)i;
     const double dot = n[0] * t[0] + n[1] * t[1] + n[2] * t[2];
     const double upper = dot - (e1[0] * n[0] + e1[1] * n[1] + e1[2] * n[2]);
     const double lower =
         (e2[0] - e1[0]) * n[0] + (e2[1] - e1[1]) * n[1] + (e2[2] - e1[2]) * n[2];
-    if (lower >= kMinusEps && lower <= kPlusEps) return -1;
+    if (lower >= kMinusEps && lower <= kPlusEps) { tie = true; return false; }
     const double pc = upper / lower;
     if (pc > 0.0 && pc < 1.0) {
       const double xc[3] = {e1[0] + pc * (e2[0] - e1[0]), e1[1] + pc * (e2[1] - e1[1]),
                             e1[2] + pc * (e2[2] - e1[2])};
-      if (upper >= lower + kMinusEps && upper <= lower + kPlusEps) return -1;
+      if (upper >= lower + kMinusEps && upper <= lower + kPlusEps) { tie = true; return false; }
       const int v = tri_volume(xc, e1, t);
       if (v == 1) cross++;
       else if (v == -1) edge++;
-      else if (v <= -2) return -1;
+      else if (v <= -2) { tie = true; return false; }
     } else if (upper >= -kVSmall && upper <= kVSmall) {
-      return -1;
+      tie = true;
+      return false;
     }
-  }
+    return true;
+  });
+  if (tie) return -1;
   if (edge >= 2) return -1;
   if ((edge % 2) == 1 && (cross % 2) == 0) { edge = 0; cross++; }
   edge /= 2;
@@ -135,7 +169,7 @@ __device__ int classify_pos(const LinkGeom& g, int a, int b, int c, int* tie) {
         if (attempt == 0) *tie = 1;
       }
       if (box_out(e1, e2, g.gbb)) { solid = 0; break; }
-      const int p = cast_parity(e1, e2, g.ntri, g.tri, g.nrm, g.tbb);
+      const int p = cast_parity(g, e1, e2);
       if (p >= 0) { solid = p; break; }
       *tie = 1;
     }
@@ -157,6 +191,12 @@ __global__ __launch_bounds__(256) void k_classify(LinkGeom g, int32_t* __restric
   if (tie) atomicAdd(counts + 1, 1ull);
 }
 
+// Wall code of the link from node (a, b, c) along (cx, cy, cz): the reference
+// takes the first triangle in index order that the segment crosses
+// (inter_bb_setup.f95:44-246), flagging a tie on any earlier candidate whose
+// plane holds the node.  Over the hierarchy's candidates: the crossing
+// triangle of least index, and a tie when a tie candidate has a lower index
+// or the crossing itself is one.
 __device__ int32_t wall_code(const LinkGeom& g, int a, int b, int c, int cx, int cy, int cz,
                              int* tie) {
   const double e1[3] = {g.origin[0] + (double)a * g.dx, g.origin[1] + (double)b * g.dx,
@@ -164,33 +204,40 @@ __device__ int32_t wall_code(const LinkGeom& g, int a, int b, int c, int cx, int
   const double e2[3] = {e1[0] + (double)cx * g.dx, e1[1] + (double)cy * g.dx,
                         e1[2] + (double)cz * g.dx};
   const double link = sqrt((double)(cx * cx + cy * cy + cz * cz)) * g.dx;
-  for (int i = 0; i < g.ntri; ++i) {
-    if (box_out(e1, e2, g.tbb + 6 * (size_t)i)) continue;
+  int best = 0x7fffffff, tie_min = 0x7fffffff;
+  bool best_tie = false;
+  int32_t best_code = 0;
+  for_candidates(g, e1, e2, [&](int i) {
+    if (i > best) return true;  // a later triangle changes nothing
     const double* t = g.tri + 9 * (size_t)i;
     const double* n = g.nrm + 3 * (size_t)i;
     const double dot = n[0] * t[0] + n[1] * t[1] + n[2] * t[2];
     const double upper = dot - (e1[0] * n[0] + e1[1] * n[1] + e1[2] * n[2]);
     const double lower =
         (e2[0] - e1[0]) * n[0] + (e2[1] - e1[1]) * n[1] + (e2[2] - e1[2]) * n[2];
-    if (lower >= kMinusEps && lower <= kPlusEps) continue;
+    if (lower >= kMinusEps && lower <= kPlusEps) return true;
     const double pc = upper / lower;
     if (pc > 0.0 && pc < 1.0 + kPlusEps) {
       const double xc[3] = {e1[0] + pc * (e2[0] - e1[0]), e1[1] + pc * (e2[1] - e1[1]),
                             e1[2] + pc * (e2[2] - e1[2])};
-      bool hit;
-      if (upper >= lower - kVSmall && upper <= lower + kVSmall) { hit = true; *tie = 1; }
+      bool hit, htie = false;
+      if (upper >= lower - kVSmall && upper <= lower + kVSmall) { hit = true; htie = true; }
       else hit = tri_volume(xc, e1, t) != 0;
       if (hit) {
         const double d = sqrt((e1[0] - xc[0]) * (e1[0] - xc[0]) +
                               (e1[1] - xc[1]) * (e1[1] - xc[1]) +
                               (e1[2] - xc[2]) * (e1[2] - xc[2]));
-        return (int32_t)(-1001 - (int64_t)floor(d / link * 1e9));
+        best = i;
+        best_tie = htie;
+        best_code = (int32_t)(-1001 - (int64_t)floor(d / link * 1e9));
       }
     } else if (upper >= -kVSmall && upper <= kVSmall) {
-      *tie = 1;
+      if (i < tie_min) tie_min = i;
     }
-  }
-  return 0;
+    return true;
+  });
+  if (tie_min < best || (best != 0x7fffffff && best_tie)) *tie = 1;
+  return best == 0x7fffffff ? 0 : best_code;
 }
 
 __global__ __launch_bounds__(256) void k_links(LinkGeom g, int32_t* __restrict__ state,
@@ -450,6 +497,76 @@ hipError_t launch_flags27(const int32_t* state, const Box& b, const int32_t face
   for (int f = 0; f < 6; ++f) fc.code[f] = face_code[f];
   k_flags<<<blocks_for(b.n, 256), 256, 0, s>>>(state, b, fc, flags);
   return hipGetLastError();
+}
+
+namespace {
+
+struct BvhBuild {
+  const double* tbb;
+  std::vector<BvhNode>* nodes;
+  std::vector<int32_t>* idx;
+  std::vector<double> cen;  // [ntri][3] box centres
+  int depth = 0;
+
+  int build(int lo, int hi, int level) {
+    if (level > depth) depth = level;
+    const int me = (int)nodes->size();
+    nodes->push_back(BvhNode{});
+    BvhNode nd{};
+    for (int d = 0; d < 3; ++d) {
+      nd.bb[2 * d] = -1e300;
+      nd.bb[2 * d + 1] = 1e300;
+    }
+    double cmin[3] = {1e300, 1e300, 1e300}, cmax[3] = {-1e300, -1e300, -1e300};
+    for (int k = lo; k < hi; ++k) {
+      const int t = (*idx)[k];
+      for (int d = 0; d < 3; ++d) {
+        if (tbb[t * 6 + 2 * d] > nd.bb[2 * d]) nd.bb[2 * d] = tbb[t * 6 + 2 * d];
+        if (tbb[t * 6 + 2 * d + 1] < nd.bb[2 * d + 1]) nd.bb[2 * d + 1] = tbb[t * 6 + 2 * d + 1];
+        cmin[d] = std::min(cmin[d], cen[t * 3 + d]);
+        cmax[d] = std::max(cmax[d], cen[t * 3 + d]);
+      }
+    }
+    if (hi - lo <= 4) {
+      nd.a = lo;
+      nd.b = -(hi - lo);
+      (*nodes)[me] = nd;
+      return me;
+    }
+    int ax = 0;
+    for (int d = 1; d < 3; ++d)
+      if (cmax[d] - cmin[d] > cmax[ax] - cmin[ax]) ax = d;
+    const int mid = (lo + hi) / 2;
+    std::nth_element(idx->begin() + lo, idx->begin() + mid, idx->begin() + hi,
+                     [&](int32_t p, int32_t q) {
+                       const double cp = cen[p * 3 + ax], cq = cen[q * 3 + ax];
+                       return cp < cq || (cp == cq && p < q);
+                     });
+    const int l = build(lo, mid, level + 1);
+    const int r = build(mid, hi, level + 1);
+    nd.a = l;
+    nd.b = r;
+    (*nodes)[me] = nd;
+    return me;
+  }
+};
+
+}  // namespace
+
+int build_bvh(const double* tbb, int ntri, std::vector<BvhNode>* nodes,
+              std::vector<int32_t>* idx) {
+  nodes->clear();
+  idx->resize((size_t)ntri);
+  if (ntri <= 0) return 0;
+  BvhBuild b{tbb, nodes, idx, std::vector<double>((size_t)ntri * 3)};
+  for (int t = 0; t < ntri; ++t) {
+    (*idx)[t] = t;
+    for (int d = 0; d < 3; ++d)
+      b.cen[t * 3 + d] = 0.5 * (tbb[t * 6 + 2 * d] + tbb[t * 6 + 2 * d + 1]);
+  }
+  nodes->reserve((size_t)ntri);
+  b.build(0, ntri, 0);
+  return b.depth;
 }
 
 }  // namespace uclbs

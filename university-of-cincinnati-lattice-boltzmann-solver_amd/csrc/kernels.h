@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <vector>
 
 #include "alpha_solve.h"
 #include "d3q39.h"
@@ -74,6 +75,14 @@ struct BcArgs {
   const double* feq_fs;   // [27] or null
 };
 
+// Node of the bounding-volume hierarchy over the triangles: bb is the exact
+// union (max / min) of the boxes of the triangles below it; an inner node has
+// children a, b; a leaf (b < 0) lists -b triangles from idx[a].
+struct BvhNode {
+  double bb[6];  // xmax, xmin, ymax, ymin, zmax, zmin
+  int32_t a, b;
+};
+
 struct LinkGeom {
   Box box;
   double origin[3];
@@ -84,7 +93,17 @@ struct LinkGeom {
   const double* nrm;  // [ntri][3]
   const double* tbb;  // [ntri][6]: xmax, xmin, ymax, ymin, zmax, zmin
   double gbb[6];      // geometry bounding box, same order
+  const BvhNode* bvh;     // hierarchy (node 0 the root), null for no triangles
+  const int32_t* bvh_idx; // triangle indices of the leaves
 };
+
+// Builds the hierarchy on the host (median split of the longest centroid axis,
+// leaves of at most 4 triangles) from the triangle boxes tbb; returns its
+// depth (the device walk keeps a stack of 40 nodes, enough for depth 38: a
+// median split of 2^26 triangles has depth 25).
+int build_bvh(const double* tbb, int ntri, std::vector<BvhNode>* nodes,
+              std::vector<int32_t>* idx);
+constexpr int kBvhMaxDepth = 38;
 
 hipError_t launch_build_links27(const LinkGeom& g, int32_t* state, unsigned long long* counts,
                                 hipStream_t s);

@@ -1378,10 +1378,24 @@ int uclbs_build_links(void* ctx, int lvl, const double origin[3], double dx, int
       if (mn < gbb[2 * d + 1]) gbb[2 * d + 1] = mn;
     }
   for (int i = 0; i < 6; ++i) g.gbb[i] = gbb[i];
+  std::vector<uclbs::BvhNode> bvh;
+  std::vector<int32_t> bvh_idx;
+  if (uclbs::build_bvh(tbb.data(), (int)ntri, &bvh, &bvh_idx) > uclbs::kBvhMaxDepth)
+    return set_err(UCLBS_E_ARG, "build_links: triangle hierarchy deeper than %d",
+                   uclbs::kBvhMaxDepth);
   double *dtri = nullptr, *dnrm = nullptr, *dtbb = nullptr;
+  uclbs::BvhNode* dbvh = nullptr;
+  int32_t* dbidx = nullptr;
   unsigned long long* dcnt = nullptr;
   const size_t nt = (size_t)(ntri > 0 ? ntri : 1);
+  const size_t nb = bvh.empty() ? 1 : bvh.size();
   hipError_t e = hipMalloc(&dtri, nt * 9 * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&dbvh, nb * sizeof(uclbs::BvhNode));
+  if (e == hipSuccess) e = hipMalloc(&dbidx, nt * sizeof(int32_t));
+  if (e == hipSuccess && !bvh.empty())
+    e = hipMemcpy(dbvh, bvh.data(), bvh.size() * sizeof(uclbs::BvhNode), hipMemcpyHostToDevice);
+  if (e == hipSuccess && ntri)
+    e = hipMemcpy(dbidx, bvh_idx.data(), (size_t)ntri * sizeof(int32_t), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMalloc(&dnrm, nt * 3 * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&dtbb, nt * 6 * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&dcnt, 2 * sizeof(unsigned long long));
@@ -1395,6 +1409,8 @@ int uclbs_build_links(void* ctx, int lvl, const double origin[3], double dx, int
     g.tri = dtri;
     g.nrm = dnrm;
     g.tbb = dtbb;
+    g.bvh = dbvh;
+    g.bvh_idx = dbidx;
     e = !q39 ? uclbs::launch_build_links27(g, L->state, dcnt, c->stream)
         : c->shifted39 ? uclbs::launch_build_links39s(g, L->state, dcnt, c->stream)
                        : uclbs::launch_build_links39(g, L->state, dcnt, c->stream);
@@ -1404,6 +1420,8 @@ int uclbs_build_links(void* ctx, int lvl, const double origin[3], double dx, int
   (void)hipFree(dtri);
   (void)hipFree(dnrm);
   (void)hipFree(dtbb);
+  (void)hipFree(dbvh);
+  (void)hipFree(dbidx);
   (void)hipFree(dcnt);
   if (e != hipSuccess) return set_err(UCLBS_E_HIP, "build_links failed: %s", hipGetErrorString(e));
   if (q39) {
