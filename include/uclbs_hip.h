@@ -225,6 +225,20 @@ int uclbs_init_fields(void *ctx, int lvl, const double *rho, const double *u, co
 int uclbs_upload_state(void *ctx, int lvl, const double *f, const double *g, const double *lm,
                        int64_t n_nodes);
 int uclbs_download_state(void *ctx, int lvl, double *f, double *g, double *lm, int64_t n_nodes);
+/* ---- checkpoint / restart --------------------------------------------------
+ * The reference's restart writer is commented out (restart_write.f95); these
+ * give a versioned dump of the device state.  uclbs_checkpoint_write saves,
+ * for every defined level, the current f (and g) buffer in storage form, the
+ * D3Q39 multipliers and the step counter, with the level's set-up
+ * fingerprint (extent, refined / many-box flags, face codes, D3Q39 gas
+ * constants, a hash of the link table) and a checksum of the payload.
+ * uclbs_checkpoint_read restores them into a context set up the same way
+ * (same lattice and precision, same levels, faces, gas and links): it returns
+ * UCLBS_E_STATE when the set-up differs and UCLBS_E_ARG for a file that is not
+ * a checkpoint or is damaged; after an error the context's state is
+ * unspecified.  A restored context continues bit for bit as the saved one. */
+int uclbs_checkpoint_write(void *ctx, const char *path);
+int uclbs_checkpoint_read(void *ctx, const char *path);
 /* Flow variables of the current f, g per node, computed on the device as
  * calculate_flow_variables.f95:201-221 does: rho, u, v, w, temp and press =
  * rho (temp + |u|^2 / 2) (the reference's press, :221); any output may be

@@ -25,7 +25,8 @@ EXPORTS = (
     "uclbs_step_level", "uclbs_step_level_async", "uclbs_time_steps", "uclbs_sync",
     "uclbs_get_stats", "uclbs_reset_stats", "uclbs_set_faces", "uclbs_set_bc_states",
     "uclbs_build_links", "uclbs_set_links", "uclbs_download_links", "uclbs_link_counts",
-    "uclbs_init_uniform", "uclbs_geom_icosphere", "uclbs_geom_wedge", "uclbs_download_macro39", "uclbs_time_kernels",
+    "uclbs_init_uniform", "uclbs_geom_icosphere", "uclbs_geom_wedge", "uclbs_download_macro39", "uclbs_checkpoint_write",
+    "uclbs_checkpoint_read", "uclbs_time_kernels",
     "uclbs_copy_bandwidth", "uclbs_set_gas", "uclbs_init_fields", "uclbs_upload_state",
     "uclbs_download_state", "uclbs_set_timestep", "uclbs_mlp_init", "uclbs_mlp_set_params",
     "uclbs_mlp_get_params", "uclbs_mlp_loss_grad", "uclbs_mlp_forward", "uclbs_mlp_reserve",
@@ -120,6 +121,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "uclbs_geom_icosphere": (I, [I, D, dp, U64, dp, dp, I64, ctypes.POINTER(I64)]),
         "uclbs_geom_wedge": (I, [D, D, D, D, D, D, I, I, U64, dp, dp, I64, ctypes.POINTER(I64)]),
         "uclbs_download_macro39": (I, [P, I, dp, dp, dp, dp, dp, dp, I64]),
+        "uclbs_checkpoint_write": (I, [P, ctypes.c_char_p]),
+        "uclbs_checkpoint_read": (I, [P, ctypes.c_char_p]),
         "uclbs_time_kernels": (I, [P, I, I, dp, dp]),
         "uclbs_copy_bandwidth": (I, [I64, I, dp]),
         "uclbs_set_gas": (I, [P, ctypes.POINTER(Gas)]),
@@ -380,6 +383,14 @@ class Solver:
         return f, g, lm
 
     # ---- learned alpha ---------------------------------------------------------
+    def checkpoint_write(self, path: str):
+        """Versioned dump of every level's device state (uclbs_checkpoint_write)."""
+        self._check(self.lib.uclbs_checkpoint_write(self.ctx, os.fsencode(path)))
+
+    def checkpoint_read(self, path: str):
+        """Restore a dump into this context, set up as the saved one was."""
+        self._check(self.lib.uclbs_checkpoint_read(self.ctx, os.fsencode(path)))
+
     def macro39(self, lvl: int):
         """(rho, u, v, w, T, press) of a D3Q39 level, computed on the device
         (press = rho (T + |u|^2 / 2) as calculate_flow_variables.f95:221)."""

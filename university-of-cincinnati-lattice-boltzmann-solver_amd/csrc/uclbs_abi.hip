@@ -2448,4 +2448,189 @@ int uclbs_mlp_train(void* ctx, int64_t steps, int batch, double lr, uint64_t see
   return mlp_publish(c);
 }
 
+
+// ---- checkpoint / restart ------------------------------------------------------
+// File: "UCLBSCKP", version, lattice, precision, level count; per defined
+// level a header (level index, node count and box extents, refined / many-box
+// flags, step counter, face codes, the D3Q39 gas constants of the level, a
+// hash of the link table), then the current f (and g) buffer in storage form
+// and the D3Q39 multipliers, then a checksum of that payload.  Restoring
+// needs a context set up the same way (levels, faces, gas, links): every
+// header field is checked against it before the level is overwritten.
+
+namespace {
+
+constexpr char kCkMagic[8] = {'U', 'C', 'L', 'B', 'S', 'C', 'K', 'P'};
+constexpr uint32_t kCkVersion = 1;
+
+struct CkFile {
+  char magic[8];
+  uint32_t version, lattice, precision, nlev;
+};
+
+struct CkLevel {
+  int32_t lvl, flags;  // flags: 1 g and multipliers present, 2 link table present
+  int64_t n;
+  int32_t nx, ny, nz, refined, multibox, shifted;
+  int64_t step;
+  int32_t face_code[6];
+  uint64_t table_hash;
+  uint64_t bytes_f, bytes_g, bytes_lm;
+  uclbs::Gas39 gas;
+};
+
+// FNV-1a over 64-bit words (a tail shorter than a word byte by byte)
+uint64_t ck_hash(const void* data, size_t bytes, uint64_t h = 1469598103934665603ull) {
+  const unsigned char* p = static_cast<const unsigned char*>(data);
+  size_t k = 0;
+  for (; k + 8 <= bytes; k += 8) {
+    uint64_t w;
+    std::memcpy(&w, p + k, 8);
+    h = (h ^ w) * 1099511628211ull;
+  }
+  for (; k < bytes; ++k) h = (h ^ p[k]) * 1099511628211ull;
+  return h;
+}
+
+int ck_table_hash(Ctx* c, Level* L, uint64_t* h) {
+  *h = 0;
+  if (!L->state) return UCLBS_OK;
+  std::vector<int32_t> t((size_t)L->box.n * q_of(c));
+  HIP_TRY(hipMemcpyAsync(t.data(), L->state, t.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
+                         c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  *h = ck_hash(t.data(), t.size() * sizeof(int32_t));
+  return UCLBS_OK;
+}
+
+int ck_header(Ctx* c, Level* L, int lvl, CkLevel* h) {
+  std::memset(h, 0, sizeof *h);
+  const bool q39 = c->cfg.lattice == UCLBS_LATTICE_D3Q39;
+  h->lvl = lvl;
+  h->flags = (q39 ? 1 : 0) | (L->state ? 2 : 0);
+  h->n = L->box.n;
+  h->nx = L->box.nx;
+  h->ny = L->box.ny;
+  h->nz = L->box.nz;
+  h->refined = L->refined;
+  h->multibox = L->multibox;
+  h->shifted = q39 ? c->shifted39 : 0;
+  h->step = L->step;
+  for (int f = 0; f < 6; ++f) h->face_code[f] = L->face_code[f];
+  h->bytes_f = level_bytes(c, L);
+  h->bytes_g = q39 ? level_bytes(c, L) : 0;
+  h->bytes_lm = q39 ? (uint64_t)L->box.n * uclbs::kM39 * sizeof(double) : 0;
+  if (q39) h->gas = L->gas39;
+  return ck_table_hash(c, L, &h->table_hash);
+}
+
+}  // namespace
+
+int uclbs_checkpoint_write(void* ctx, const char* path) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || !path) return set_err(UCLBS_E_ARG, "checkpoint_write: null argument");
+  int rc = set_device(c);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  FILE* fp = std::fopen(path, "wb");
+  if (!fp) return set_err(UCLBS_E_ARG, "checkpoint_write: cannot open %s", path);
+  CkFile fh{};
+  std::memcpy(fh.magic, kCkMagic, 8);
+  fh.version = kCkVersion;
+  fh.lattice = (uint32_t)c->cfg.lattice;
+  fh.precision = (uint32_t)c->cfg.precision;
+  for (const Level& L : c->levels) fh.nlev += L.defined ? 1 : 0;
+  bool ok = std::fwrite(&fh, sizeof fh, 1, fp) == 1;
+  std::vector<char> buf;
+  for (int lvl = 0; ok && lvl < (int)c->levels.size(); ++lvl) {
+    Level* L = &c->levels[lvl];
+    if (!L->defined) continue;
+    CkLevel h;
+    if ((rc = ck_header(c, L, lvl, &h))) { std::fclose(fp); return rc; }
+    ok = std::fwrite(&h, sizeof h, 1, fp) == 1;
+    uint64_t sum = 1469598103934665603ull;
+    const void* src[3] = {L->f[L->cur], h.bytes_g ? L->g[L->cur] : nullptr,
+                          h.bytes_lm ? L->lm : nullptr};
+    const uint64_t len[3] = {h.bytes_f, h.bytes_g, h.bytes_lm};
+    for (int k = 0; ok && k < 3; ++k) {
+      if (!len[k]) continue;
+      buf.resize(len[k]);
+      hipError_t e = hipMemcpyAsync(buf.data(), src[k], len[k], hipMemcpyDeviceToHost, c->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+      if (e != hipSuccess) {
+        std::fclose(fp);
+        return set_err(UCLBS_E_HIP, "checkpoint_write: %s", hipGetErrorString(e));
+      }
+      sum = ck_hash(buf.data(), len[k], sum);
+      ok = std::fwrite(buf.data(), 1, len[k], fp) == len[k];
+    }
+    ok = ok && std::fwrite(&sum, sizeof sum, 1, fp) == 1;
+  }
+  ok = (std::fclose(fp) == 0) && ok;
+  if (!ok) return set_err(UCLBS_E_ARG, "checkpoint_write: write to %s failed", path);
+  return UCLBS_OK;
+}
+
+int uclbs_checkpoint_read(void* ctx, const char* path) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || !path) return set_err(UCLBS_E_ARG, "checkpoint_read: null argument");
+  int rc = set_device(c);
+  if (rc) return rc;
+  FILE* fp = std::fopen(path, "rb");
+  if (!fp) return set_err(UCLBS_E_ARG, "checkpoint_read: cannot open %s", path);
+  auto fail = [&](int code, const char* why) {
+    std::fclose(fp);
+    return set_err(code, "checkpoint_read %s: %s", path, why);
+  };
+  CkFile fh{};
+  if (std::fread(&fh, sizeof fh, 1, fp) != 1 || std::memcmp(fh.magic, kCkMagic, 8) != 0)
+    return fail(UCLBS_E_ARG, "not a checkpoint file");
+  if (fh.version != kCkVersion) return fail(UCLBS_E_ARG, "unknown checkpoint version");
+  if (fh.lattice != (uint32_t)c->cfg.lattice || fh.precision != (uint32_t)c->cfg.precision)
+    return fail(UCLBS_E_STATE, "lattice or storage precision differs from the context");
+  uint32_t ndef = 0;
+  for (const Level& L : c->levels) ndef += L.defined ? 1 : 0;
+  if (fh.nlev != ndef) return fail(UCLBS_E_STATE, "level count differs from the context");
+  std::vector<char> buf[3];
+  for (uint32_t r = 0; r < fh.nlev; ++r) {
+    CkLevel h;
+    if (std::fread(&h, sizeof h, 1, fp) != 1) return fail(UCLBS_E_ARG, "truncated file");
+    Level* L;
+    if (h.lvl < 0 || check_level(c, h.lvl, &L) != UCLBS_OK)
+      return fail(UCLBS_E_STATE, "a saved level is not defined in the context");
+    CkLevel mine;
+    if ((rc = ck_header(c, L, h.lvl, &mine))) { std::fclose(fp); return rc; }
+    mine.step = h.step;  // the one field restored, not compared
+    if (std::memcmp(&h, &mine, sizeof h) != 0)
+      return fail(UCLBS_E_STATE, "level set-up (extent, faces, gas, link table) differs from the context");
+    uint64_t sum = 1469598103934665603ull, saved;
+    const uint64_t len[3] = {h.bytes_f, h.bytes_g, h.bytes_lm};
+    for (int k = 0; k < 3; ++k) {
+      buf[k].resize(len[k]);
+      if (len[k] && std::fread(buf[k].data(), 1, len[k], fp) != len[k])
+        return fail(UCLBS_E_ARG, "truncated file");
+      sum = ck_hash(buf[k].data(), len[k], sum);
+    }
+    if (std::fread(&saved, sizeof saved, 1, fp) != 1) return fail(UCLBS_E_ARG, "truncated file");
+    if (saved != sum) return fail(UCLBS_E_ARG, "checksum mismatch (damaged file)");
+    void* dst[3] = {L->f[L->cur], len[1] ? L->g[L->cur] : nullptr, len[2] ? L->lm : nullptr};
+    void* dst2[2] = {L->f[1 - L->cur], len[1] ? L->g[1 - L->cur] : nullptr};
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < 3 && e == hipSuccess; ++k)
+      if (len[k]) e = hipMemcpyAsync(dst[k], buf[k].data(), len[k], hipMemcpyHostToDevice, c->stream);
+    // the other buffer mirrors the restored one (nodes no kernel writes)
+    for (int k = 0; k < 2 && e == hipSuccess; ++k)
+      if (len[k]) e = hipMemcpyAsync(dst2[k], dst[k], len[k], hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+      std::fclose(fp);
+      return set_err(UCLBS_E_HIP, "checkpoint_read: %s", hipGetErrorString(e));
+    }
+    L->step = h.step;
+  }
+  std::fclose(fp);
+  invalidate_graph(c);
+  return UCLBS_OK;
+}
+
 }  // extern "C"
