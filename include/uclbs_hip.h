@@ -239,6 +239,22 @@ int uclbs_download_state(void *ctx, int lvl, double *f, double *g, double *lm, i
  * unspecified.  A restored context continues bit for bit as the saved one. */
 int uclbs_checkpoint_write(void *ctx, const char *path);
 int uclbs_checkpoint_read(void *ctx, const char *path);
+/* D3Q39 inlets and outlets with stored states (nml_inlet_outlet; face codes
+ * -100-j for inlet j and -200-j for outlet j, j from 1).  in_states /
+ * out_states hold (rho, u, v, w, T) per inlet / outlet, non-dimensional as the
+ * reference derives them (inlet_outlet_base_values); types as the reference's
+ * inlet_type / outlet_type: 1 rho-v, 2 pressure-temperature, 3 freestream,
+ * 4 v-temperature, 7 (all four: the missing population is the equilibrium
+ * of the stored state in its own direction, inlet_bc.f95:21-53 /
+ * outlet_bc.f95:23-58; type 3 the freestream equilibrium, freestream_bc.f95),
+ * and for outlets 6, pure outflow (pure_outflow_bc.f95).  The equilibria
+ * come from the multiplier solve of each state from the reference's starting
+ * guesses (inlet_outlet_basis_calcs.f90), with its two slips: g of every row
+ * of a family uses the temperature of the last state solved after it, and an
+ * outlet of type 2 reads the inlet table of its number (press_temp_bc.f95).
+ * Needs uclbs_set_gas first. */
+int uclbs_set_bc_states39(void *ctx, int n_in, const double *in_states, const int32_t *in_types,
+                          int n_out, const double *out_states, const int32_t *out_types);
 /* Flow variables of the current f, g per node, computed on the device as
  * calculate_flow_variables.f95:201-221 does: rho, u, v, w, temp and press =
  * rho (temp + |u|^2 / 2) (the reference's press, :221); any output may be

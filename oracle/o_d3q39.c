@@ -525,10 +525,15 @@ int64_t o39_damp_periodic(int nx, int ny, int nz, double *fo, int shifted, int j
  *          velocity (inviscid_edge.f95:28-1243 tabulates the same remap as
  *          direction inviscid_dir written at x + loc; a face crossed after the
  *          reflection that is not inviscid then decides as above).
- * Returns 1 for the freestream value, else 0 with (*src, *dir) to read. */
+ *   -100-j / -200-j  inlet j / outlet j (inlet_bc.f95:21-53, outlet_bc.f95:
+ *          23-58): face_row[f] >= 0 the row of the inlet / outlet table (the
+ *          equilibrium of the stored state, in direction k), -2 the freestream
+ *          pair (type 3, freestream_bc.f95), -1 pure outflow (type 6, as -6).
+ * Returns 1 for the freestream value, 2 + row for a table row, else 0 with
+ * (*src, *dir) to read. */
 static int source39(int64_t a, int64_t b, int64_t cc, int k, int nx, int ny, int nz,
-                    const int32_t *fc, const int *cx, const int *cy, const int *cz, int shifted,
-                    int64_t *src, int *dir) {
+                    const int32_t *fc, const int32_t *face_row, const int *cx, const int *cy,
+                    const int *cz, int shifted, int64_t *src, int *dir) {
   const int64_t n[3] = {nx, ny, nz};
   int64_t s[3] = {a - cx[k], b - cy[k], cc - cz[k]};
   int first = -1;
@@ -557,6 +562,11 @@ static int source39(int64_t a, int64_t b, int64_t cc, int k, int nx, int ny, int
     return 0;
   }
   if (fc[first] == -1000) return 1;
+  if (fc[first] <= -101 && fc[first] >= -299) {
+    const int r = face_row ? face_row[first] : -1;
+    if (r == -2) return 1;
+    if (r >= 0) return 2 + r;
+  }
   *src = own;
   *dir = (fc[first] == -10 && !shifted) ? O39_OPP[k] : k;
   return 0;
@@ -598,23 +608,31 @@ int o39_source(int64_t a, int64_t b, int64_t cc, int k, int nx, int ny, int nz, 
                int shifted, int64_t *src, int *dir) {
   int cx[NQ], cy[NQ], cz[NQ];
   o39_velocities(shifted, cx, cy, cz);
-  return source39(a, b, cc, k, nx, ny, nz, fc, cx, cy, cz, shifted, src, dir);
+  return source39(a, b, cc, k, nx, ny, nz, fc, NULL, cx, cy, cz, shifted, src, dir);
+}
+
+int o39_source_bc(int64_t a, int64_t b, int64_t cc, int k, int nx, int ny, int nz,
+                  const int32_t fc[6], const int32_t *face_row, int shifted, int64_t *src,
+                  int *dir) {
+  int cx[NQ], cy[NQ], cz[NQ];
+  o39_velocities(shifted, cx, cy, cz);
+  return source39(a, b, cc, k, nx, ny, nz, fc, face_row, cx, cy, cz, shifted, src, dir);
 }
 
 /* One or more steps of a box whose faces carry the codes fc (see source39);
  * fs_f / fs_g are the freestream equilibria (freestream_basis_calcs.f90:66-71)
  * and may be NULL when no face is -1000. */
 static void step_core(int nx, int ny, int nz, const int32_t fc[6], int ghost, const double *fs_f,
-                      const double *fs_g, double *f, double *g, double *lm, double *alpha_out,
-                      const o39_consts *c, int64_t step0, int nsteps, int ref_lu, int jacobi,
-                      o39_stats *st);
+                      const double *fs_g, const int32_t *face_row, const double *bc_tab,
+                      double *f, double *g, double *lm, double *alpha_out, const o39_consts *c,
+                      int64_t step0, int nsteps, int ref_lu, int jacobi, o39_stats *st);
 
 void o39_step_faces(int nx, int ny, int nz, const int32_t fc[6], const double *fs_f,
-                    const double *fs_g, double *f, double *g, double *lm, double *alpha_out,
-                    const o39_consts *c, int64_t step0, int nsteps, int ref_lu, int jacobi,
-                    o39_stats *st) {
-  step_core(nx, ny, nz, fc, 0, fs_f, fs_g, f, g, lm, alpha_out, c, step0, nsteps, ref_lu, jacobi,
-            st);
+                    const double *fs_g, const int32_t *face_row, const double *bc_tab, double *f,
+                    double *g, double *lm, double *alpha_out, const o39_consts *c, int64_t step0,
+                    int nsteps, int ref_lu, int jacobi, o39_stats *st) {
+  step_core(nx, ny, nz, fc, 0, fs_f, fs_g, face_row, bc_tab, f, g, lm, alpha_out, c, step0, nsteps,
+            ref_lu, jacobi, st);
 }
 
 /* A refined level stored with a ghost layer of depth ghost: only the valid
@@ -625,14 +643,14 @@ void o39_step_ghost(int nx, int ny, int nz, int ghost, double *f, double *g, dou
                     double *alpha_out, const o39_consts *c, int64_t step0, int nsteps, int ref_lu,
                     int jacobi, o39_stats *st) {
   const int32_t fc[6] = {-10, -10, -10, -10, -10, -10};
-  step_core(nx, ny, nz, fc, ghost, NULL, NULL, f, g, lm, alpha_out, c, step0, nsteps, ref_lu,
-            jacobi, st);
+  step_core(nx, ny, nz, fc, ghost, NULL, NULL, NULL, NULL, f, g, lm, alpha_out, c, step0, nsteps,
+            ref_lu, jacobi, st);
 }
 
 static void step_core(int nx, int ny, int nz, const int32_t fc[6], int ghost, const double *fs_f,
-                      const double *fs_g, double *f, double *g, double *lm, double *alpha_out,
-                      const o39_consts *c, int64_t step0, int nsteps, int ref_lu, int jacobi,
-                      o39_stats *st) {
+                      const double *fs_g, const int32_t *face_row, const double *bc_tab,
+                      double *f, double *g, double *lm, double *alpha_out, const o39_consts *c,
+                      int64_t step0, int nsteps, int ref_lu, int jacobi, o39_stats *st) {
   int cx[NQ], cy[NQ], cz[NQ];
   o39_velocities(c->shifted, cx, cy, cz);
   const int64_t n = (int64_t)nx * ny * nz;
@@ -657,9 +675,14 @@ static void step_core(int nx, int ny, int nz, const int32_t fc[6], int ghost, co
       for (int i = 0; i < NQ; i++) {
         int64_t src;
         int dir;
-        if (source39(a, b, cc, i, nx, ny, nz, fc, cx, cy, cz, c->shifted, &src, &dir)) {
+        const int r = source39(a, b, cc, i, nx, ny, nz, fc, face_row, cx, cy, cz, c->shifted,
+                               &src, &dir);
+        if (r == 1) {
           fi[i] = fs_f[i];
           gi[i] = fs_g[i];
+        } else if (r >= 2) {
+          fi[i] = bc_tab[(int64_t)(r - 2) * 2 * NQ + i];
+          gi[i] = bc_tab[(int64_t)(r - 2) * 2 * NQ + NQ + i];
         } else {
           fi[i] = f[(int64_t)dir * n + src];
           gi[i] = g[(int64_t)dir * n + src];
@@ -716,6 +739,6 @@ void o39_step_periodic(int nx, int ny, int nz, double *f, double *g, double *lm,
                        o39_stats *st) {
   const int32_t fc[6] = {O39_FACE_PERIODIC, O39_FACE_PERIODIC, O39_FACE_PERIODIC,
                          O39_FACE_PERIODIC, O39_FACE_PERIODIC, O39_FACE_PERIODIC};
-  o39_step_faces(nx, ny, nz, fc, NULL, NULL, f, g, lm, alpha_out, c, step0, nsteps, ref_lu, jacobi,
-                 st);
+  o39_step_faces(nx, ny, nz, fc, NULL, NULL, NULL, NULL, f, g, lm, alpha_out, c, step0, nsteps,
+                 ref_lu, jacobi, st);
 }

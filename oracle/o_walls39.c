@@ -419,7 +419,8 @@ static int64_t damp_walls(int nx, int ny, int nz, const int32_t fc[6], const int
 }
 
 void o39_step_walls(int nx, int ny, int nz, const int32_t fc[6], const int32_t *state,
-                    const double *fs_f, const double *fs_g, double *f, double *g, double *lm,
+                    const double *fs_f, const double *fs_g, const int32_t *face_row,
+                    const double *bc_tab, double *f, double *g, double *lm,
                     double *alpha_out, const o39_consts *c, int64_t step0, int64_t lvl_step0,
                     int nsteps, int jacobi, o39_stats *st, int64_t *n_wall_links,
                     int64_t *n_wall_repair) {
@@ -450,9 +451,13 @@ void o39_step_walls(int nx, int ny, int nz, const int32_t fc[6], const int32_t *
       for (int i = 0; i < NQ; i++) {
         int64_t src;
         int dir;
-        if (o39_source(a, b, cc, i, nx, ny, nz, fc, c->shifted, &src, &dir)) {
+        const int r = o39_source_bc(a, b, cc, i, nx, ny, nz, fc, face_row, c->shifted, &src, &dir);
+        if (r == 1) {
           fi[i] = fs_f[i];
           gi[i] = fs_g[i];
+        } else if (r >= 2) {
+          fi[i] = bc_tab[(int64_t)(r - 2) * 2 * NQ + i];
+          gi[i] = bc_tab[(int64_t)(r - 2) * 2 * NQ + NQ + i];
         } else {
           fi[i] = f[(int64_t)dir * n + src];
           gi[i] = g[(int64_t)dir * n + src];

@@ -144,11 +144,12 @@ def lib() -> ctypes.CDLL:
         L.o39_source.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, I, I, I, I, ip, I,
                                  ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(I)]
         L.o39_source.restype = I
-        L.o39_step_faces.argtypes = [I, I, I, ip, dp, dp, dp, dp, dp, dp, c39, ctypes.c_int64, I,
-                                     I, I, ctypes.POINTER(Stats39)]
-        L.o39_step_walls.argtypes = [I, I, I, ip, ip, dp, dp, dp, dp, dp, dp, c39, ctypes.c_int64,
-                                     ctypes.c_int64, I, I, ctypes.POINTER(Stats39),
-                                     ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
+        L.o39_step_faces.argtypes = [I, I, I, ip, dp, dp, ip, dp, dp, dp, dp, dp, c39,
+                                     ctypes.c_int64, I, I, I, ctypes.POINTER(Stats39)]
+        L.o39_step_walls.argtypes = [I, I, I, ip, ip, dp, dp, ip, dp, dp, dp, dp, dp, c39,
+                                     ctypes.c_int64, ctypes.c_int64, I, I,
+                                     ctypes.POINTER(Stats39), ctypes.POINTER(ctypes.c_int64),
+                                     ctypes.POINTER(ctypes.c_int64)]
         L.o39_wall_derivs.argtypes = [I, dp, dp, dp, ip, I, dp, dp]
         L.o39_wall_grad.argtypes = [I, D, dp, D, D, dp, dp, D, D, D, c39, dp, dp]
         L.o39_wall_rebuild.argtypes = [I, I, I, ip, ip, dp, dp, dp, ctypes.c_int64, dp, c39,
@@ -536,7 +537,7 @@ def _faces39(faces, shifted):
     fc = np.ascontiguousarray(faces, np.int32)
     assert fc.shape == (6,)
     for k, v in enumerate(fc):
-        assert v in (FACE_PERIODIC, -1000, -6, -10, -11), v
+        assert v in (FACE_PERIODIC, -1000, -6, -10, -11) or -299 <= v <= -101, v
         assert (v == FACE_PERIODIC) == (fc[(k + 3) % 6] == FACE_PERIODIC)
         assert not (shifted and v == -11 and k % 3 == 0), "no inviscid x face on the shifted lattice"
     return fc
@@ -558,10 +559,62 @@ def freestream39(c):
     return fe, c.lead_g * T * fe
 
 
+def bc_tables39(c, inlets=(), outlets=()):
+    """Inlet / outlet equilibrium table of inlet_outlet_basis_calcs.f90, one row
+    (39 f, then 39 g) per inlet, then per outlet; each item (rho, u, v, w, T,
+    type).  Types 1, 2, 4, 7: the multiplier solve of the state from the
+    reference's starting guesses (chi -2.5, pi_aa -0.3; shifted-lattice
+    outlets zeta = u, lambda = u / 25); g of every row of the family is then
+    (2 c_v - 3) T f with this state's T (the reference assigns the whole
+    array).  Type 3: the freestream pair.  Type 6 (outlets): no row values."""
+    fsf, fsg = freestream39(c)
+    rows = np.zeros((max(1, len(inlets) + len(outlets)), 78))
+
+    def family(items, base, outlet):
+        for j, it in enumerate(items):
+            rho, u, v, w, T, t = it
+            r = rows[base + j]
+            if t in (1, 2, 4, 7):
+                lm0 = np.array([-2.5, 0, 0, 0, -0.3, -0.3, -0.3, 0, 0, 0, 0, 0, 0], float)
+                if outlet and c.shifted:
+                    lm0[1:4] = (u, v, w)
+                    lm0[10:13] = (u / 25, v / 25, w / 25)
+                P = rho * np.array([T + u * u, T + v * v, T + w * w, u * v, u * w, v * w])
+                lm, _, itn = lm_solve39(rho, [u, v, w], T, P, lm0, bool(c.shifted), maxit=200)
+                assert itn >= 0
+                r[:39] = feq39(lm, rho, bool(c.shifted))
+                for k in range(len(items)):
+                    rows[base + k, 39:] = rows[base + k, :39] * c.lead_g * T
+            elif t == 3:
+                r[:39], r[39:] = fsf, fsg
+    family(list(inlets), 0, False)
+    family(list(outlets), len(inlets), True)
+    return rows
+
+
+def face_rows39(faces, inlets=(), outlets=()):
+    """Table row each face reads (inlet_bc.f95:31-53, outlet_bc.f95:33-58):
+    -2 freestream (type 3), -1 pure outflow (type 6) or no inlet / outlet,
+    an outlet of type 2 the row of the inlet of its number."""
+    out = np.full(6, -1, np.int32)
+    nin = len(inlets)
+    for f, v in enumerate(faces):
+        if -199 <= v <= -101:
+            j = -v - 100
+            out[f] = -2 if inlets[j - 1][5] == 3 else j - 1
+        elif -299 <= v <= -201:
+            j = -v - 200
+            t = outlets[j - 1][5]
+            out[f] = -1 if t == 6 else -2 if t == 3 else (j - 1) if t == 2 else nin + j - 1
+    return out
+
+
 def step_faces39(shape, faces, f, g, lm, c, step0=0, nsteps=1, ref_lu=False, jacobi=True,
-                 keep_alpha=False):
+                 keep_alpha=False, inlets=(), outlets=()):
     """Steps of a box whose faces (x-, y-, z-, x+, y+, z+) carry FACE_PERIODIC,
-    -1000 (freestream), -6 (pure outflow), -10 (bounce-back) or -11 (inviscid)."""
+    -1000 (freestream), -6 (pure outflow), -10 (bounce-back), -11 (inviscid),
+    -100-j (inlet j) or -200-j (outlet j) with inlets / outlets given as
+    (rho, u, v, w, T, type)."""
     nx, ny, nz = shape
     fc = _faces39(faces, bool(c.shifted))
     fsf, fsg = freestream39(c)
@@ -570,15 +623,18 @@ def step_faces39(shape, faces, f, g, lm, c, step0=0, nsteps=1, ref_lu=False, jac
     lm = np.array(lm, np.float64)
     alpha = np.zeros(nx * ny * nz) if keep_alpha else None
     st = Stats39()
-    lib().o39_step_faces(nx, ny, nz, fc.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), _dp(fsf),
-                         _dp(fsg), _dp(f), _dp(g), _dp(lm),
+    rows = np.ascontiguousarray(face_rows39(faces, inlets, outlets))
+    tab = np.ascontiguousarray(bc_tables39(c, inlets, outlets))
+    ip = ctypes.POINTER(ctypes.c_int32)
+    lib().o39_step_faces(nx, ny, nz, fc.ctypes.data_as(ip), _dp(fsf), _dp(fsg),
+                         rows.ctypes.data_as(ip), _dp(tab), _dp(f), _dp(g), _dp(lm),
                          _dp(alpha) if alpha is not None else None, ctypes.byref(c), step0,
                          nsteps, int(ref_lu), int(jacobi), ctypes.byref(st))
     return f, g, lm, alpha, st.as_dict()
 
 
 def step_walls39(shape, faces, state, f, g, lm, c, step0=0, nsteps=1, lvl_step0=None,
-                 jacobi=True, keep_alpha=False):
+                 jacobi=True, keep_alpha=False, inlets=(), outlets=()):
     """Steps of a shifted-lattice box with a link table (build_links39s):
     wall links rebuilt by the reference's shifted-lattice wall rule.  Returns
     (f, g, lm, alpha, stats, n_wall_links, n_wall_repairs)."""
@@ -593,8 +649,10 @@ def step_walls39(shape, faces, state, f, g, lm, c, step0=0, nsteps=1, lvl_step0=
     st = Stats39()
     nwl, nwr = ctypes.c_int64(), ctypes.c_int64()
     ip = ctypes.POINTER(ctypes.c_int32)
+    rows = np.ascontiguousarray(face_rows39(faces, inlets, outlets))
+    tab = np.ascontiguousarray(bc_tables39(c, inlets, outlets))
     lib().o39_step_walls(nx, ny, nz, fc.ctypes.data_as(ip), st_.ctypes.data_as(ip), _dp(fsf),
-                         _dp(fsg), _dp(f), _dp(g), _dp(lm),
+                         _dp(fsg), rows.ctypes.data_as(ip), _dp(tab), _dp(f), _dp(g), _dp(lm),
                          _dp(alpha) if alpha is not None else None, ctypes.byref(c), step0,
                          step0 if lvl_step0 is None else lvl_step0, nsteps, int(jacobi),
                          ctypes.byref(st), ctypes.byref(nwl), ctypes.byref(nwr))

@@ -174,6 +174,12 @@ void o39_step_ghost(int nx, int ny, int nz, int ghost, double *f, double *g, dou
                     int jacobi, o39_stats *st);
 int o39_source(int64_t a, int64_t b, int64_t cc, int k, int nx, int ny, int nz, const int32_t fc[6],
                int shifted, int64_t *src, int *dir);
+/* as o39_source with inlet / outlet faces: face_row[f] is the table row of
+ * face f (-2 the freestream pair, -1 pure outflow); returns 2 + row for a
+ * table row */
+int o39_source_bc(int64_t a, int64_t b, int64_t cc, int k, int nx, int ny, int nz,
+                  const int32_t fc[6], const int32_t *face_row, int shifted, int64_t *src,
+                  int *dir);
 /* o_walls39.c: the shifted lattice's wall rule and a step with a link table */
 /* wall-link gradients and rebuilt populations of the shifted-lattice wall
  * rule (o_walls39.c) */
@@ -187,14 +193,17 @@ int o39_wall_rebuild(int nx, int ny, int nz, const int32_t fc[6], const int32_t 
                      const double fi[39], const o39_consts *c, int64_t step, int64_t lvl_step,
                      double fw[39], double gw[39], int *noconv, int *kerr);
 void o39_step_walls(int nx, int ny, int nz, const int32_t fc[6], const int32_t *state,
-                    const double *fs_f, const double *fs_g, double *f, double *g, double *lm,
+                    const double *fs_f, const double *fs_g, const int32_t *face_row,
+                    const double *bc_tab, double *f, double *g, double *lm,
                     double *alpha_out, const o39_consts *c, int64_t step0, int64_t lvl_step0,
                     int nsteps, int jacobi, o39_stats *st, int64_t *n_wall_links,
                     int64_t *n_wall_repair);
+/* face_row / bc_tab: inlet and outlet faces (NULL when there are none): the
+ * table holds rows of 39 f then 39 g values */
 void o39_step_faces(int nx, int ny, int nz, const int32_t fc[6], const double *fs_f,
-                    const double *fs_g, double *f, double *g, double *lm, double *alpha_out,
-                    const o39_consts *c, int64_t step0, int nsteps, int ref_lu, int jacobi,
-                    o39_stats *st);
+                    const double *fs_g, const int32_t *face_row, const double *bc_tab, double *f,
+                    double *g, double *lm, double *alpha_out, const o39_consts *c, int64_t step0,
+                    int nsteps, int ref_lu, int jacobi, o39_stats *st);
 
 #ifdef __cplusplus
 }

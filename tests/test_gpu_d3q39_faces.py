@@ -146,7 +146,7 @@ def test_face_code_errors(pkg):
     with pytest.raises(pkg.UclbsError):
         s.set_faces(0, [-11, -11, -11, -6, -11, -11])   # inviscid x face on the shifted lattice
     with pytest.raises(pkg.UclbsError):
-        s.set_faces(0, [-101, -11, -11, -6, -11, -11])  # no D3Q39 inlet tables
+        s.set_faces(0, [-300, -11, -11, -6, -11, -11])  # not a face code
     with pytest.raises(pkg.UclbsError):
         s.set_faces(0, [P, -11, -11, -6, -11, -11])     # periodic code on a walled axis
     s.set_faces(0, C3_FACES)

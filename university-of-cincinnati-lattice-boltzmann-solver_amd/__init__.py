@@ -26,7 +26,7 @@ EXPORTS = (
     "uclbs_get_stats", "uclbs_reset_stats", "uclbs_set_faces", "uclbs_set_bc_states",
     "uclbs_build_links", "uclbs_set_links", "uclbs_download_links", "uclbs_link_counts",
     "uclbs_init_uniform", "uclbs_geom_icosphere", "uclbs_geom_wedge", "uclbs_download_macro39", "uclbs_checkpoint_write",
-    "uclbs_checkpoint_read", "uclbs_time_kernels",
+    "uclbs_checkpoint_read", "uclbs_set_bc_states39", "uclbs_time_kernels",
     "uclbs_copy_bandwidth", "uclbs_set_gas", "uclbs_init_fields", "uclbs_upload_state",
     "uclbs_download_state", "uclbs_set_timestep", "uclbs_mlp_init", "uclbs_mlp_set_params",
     "uclbs_mlp_get_params", "uclbs_mlp_loss_grad", "uclbs_mlp_forward", "uclbs_mlp_reserve",
@@ -122,6 +122,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "uclbs_geom_wedge": (I, [D, D, D, D, D, D, I, I, U64, dp, dp, I64, ctypes.POINTER(I64)]),
         "uclbs_download_macro39": (I, [P, I, dp, dp, dp, dp, dp, dp, I64]),
         "uclbs_checkpoint_write": (I, [P, ctypes.c_char_p]),
+        "uclbs_set_bc_states39": (I, [P, I, dp, i3, I, dp, i3]),
         "uclbs_checkpoint_read": (I, [P, ctypes.c_char_p]),
         "uclbs_time_kernels": (I, [P, I, I, dp, dp]),
         "uclbs_copy_bandwidth": (I, [I64, I, dp]),
@@ -383,6 +384,17 @@ class Solver:
         return f, g, lm
 
     # ---- learned alpha ---------------------------------------------------------
+    def set_bc_states39(self, inlets=(), outlets=()):
+        """D3Q39 inlets / outlets: sequences of (rho, u, v, w, T, type)."""
+        def pack(items):
+            st = np.ascontiguousarray([list(t[:5]) for t in items] or [[0.0] * 5], dtype=np.float64)
+            ty = np.ascontiguousarray([int(t[5]) for t in items] or [0], dtype=np.int32)
+            return st, ty
+        si, ti = pack(inlets)
+        so, to = pack(outlets)
+        self._check(self.lib.uclbs_set_bc_states39(self.ctx, len(inlets), _dptr(si), _iptr(ti),
+                                                   len(outlets), _dptr(so), _iptr(to)))
+
     def checkpoint_write(self, path: str):
         """Versioned dump of every level's device state (uclbs_checkpoint_write)."""
         self._check(self.lib.uclbs_checkpoint_write(self.ctx, os.fsencode(path)))

@@ -65,6 +65,13 @@ __device__ __forceinline__ int face_code39(const Faces39& fc, int f) {
   return v;
 }
 
+__device__ __forceinline__ int face_tab39(const Faces39& fc, int f) {
+  int v = fc.tab[0];
+#pragma unroll
+  for (int k = 1; k < 6; ++k) v = (f == k) ? fc.tab[k] : v;
+  return v;
+}
+
 // Pull of population i into node (x, y, z) whose upstream node x - c_i may lie
 // across a non-periodic face: streaming.f95:365-474 in pull form.  The first
 // face crossed in the chain order -x, -y, -z, +x, +y, +z of
@@ -73,7 +80,9 @@ __device__ __forceinline__ int face_code39(const Faces39& fc, int f) {
 // (pure_outflow_bc.f95:17-20), -10 its opposite (bounceback.f95:13-19; opp is
 // the identity on the shifted lattice), -11 the specular image about the
 // half-way plane of every crossed inviscid face (inviscid_edge.f95:28-1243),
-// after which a face still crossed decides as above.
+// after which a face still crossed decides as above; an inlet (-100-j) or an
+// outlet (-200-j) the equilibrium of its stored state in direction i, or, for
+// a pure-outflow outlet, the node's own population as -6.
 template <bool kShifted, typename S>
 __device__ __forceinline__ void pull_face39(int x, int y, int z, int i, const Box& b,
                                             const Faces39& fc, const Ref39& rf,
@@ -121,6 +130,20 @@ __device__ __forceinline__ void pull_face39(int x, int y, int z, int i, const Bo
   } else if (code == -1000) {
     *fv = (S)(fc.f[i] - rf.f[i]);
     *gv = (S)(fc.g[i] - rf.g[i]);
+    return;
+  } else if (code <= -101 && code >= -299 && face_tab39(fc, first) != -1) {
+    // inlet / outlet with a stored state: its equilibrium in direction i
+    // (inlet_bc.f95:21-53, outlet_bc.f95:23-58 via rho_v_bc / press_temp_bc /
+    // v_temp_bc), or the freestream pair for type 3 (freestream_bc.f95)
+    const int r = face_tab39(fc, first);
+    if (r >= 0) {
+      const double* row = fc.bc + (size_t)r * (2 * Q39);
+      *fv = (S)(row[i] - rf.f[i]);
+      *gv = (S)(row[Q39 + i] - rf.g[i]);
+    } else {
+      *fv = (S)(fc.f[i] - rf.f[i]);
+      *gv = (S)(fc.g[i] - rf.g[i]);
+    }
     return;
   } else {
     src = (unsigned)x + (unsigned)nb[0] * ((unsigned)y + (unsigned)nb[1] * (unsigned)z);

@@ -241,6 +241,12 @@ struct Faces39 {
   int code[6];
   double f[39];
   double g[39];
+  // inlet / outlet faces (codes -101..-199, -201..-299): the row of bc holding
+  // the equilibrium the face supplies (f then g, 78 doubles a row), -2 for the
+  // freestream pair f, g above (type 3), or -1 for a pure-outflow outlet (the
+  // node's own population, as -6)
+  int tab[6];
+  const double* bc;
 };
 // A node of a D3Q39 box whose upstream nodes x - c_i may lie across a
 // non-periodic face: x - c_d < 0 needs x < max c_d, x - c_d >= n needs

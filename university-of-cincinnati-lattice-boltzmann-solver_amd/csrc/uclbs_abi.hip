@@ -157,6 +157,10 @@ struct Ctx {
   uclbs::Ref39 ref39{};
   uclbs::Ref39 fs39{};        // freestream equilibria fs_fieq / fs_gieq (full values)
   double* d_ref39 = nullptr;  // f_ref[39] then g_ref[39]
+  // D3Q39 inlets / outlets with stored states: types and equilibrium rows
+  // (inlets, then outlets; f then g, 78 doubles a row)
+  std::vector<int> in_types39, out_types39;
+  double* d_bc39 = nullptr;
   int64_t err_seen = 0;       // D3Q39 error-path counts already reported
   // learned alpha: FP32 master parameters, Adam moments, bf16 inference copy
   float* mlp_p = nullptr;
@@ -455,6 +459,41 @@ int finest_level(const Ctx* c) {
   return n - 1;
 }
 
+// rows of the inlet / outlet table the faces of L read (inlet_bc.f95:31-53,
+// outlet_bc.f95:33-58): type 3 the freestream pair itself (freestream_bc.f95,
+// -2); an inlet its own row; an outlet of type 6 none (pure outflow, -1); an
+// outlet of type 2 the row of the inlet of the same number (press_temp_bc.f95
+// reads inl_fieq for outlets too); other outlets their own row
+int resolve_bc39(Ctx* c, Level* L, uclbs::Faces39* fc) {
+  const int n_in = (int)c->in_types39.size(), n_out = (int)c->out_types39.size();
+  fc->bc = c->d_bc39;
+  for (int f = 0; f < 6; ++f) {
+    const int v = L->face_code[f];
+    fc->tab[f] = -1;
+    if (v <= -101 && v >= -199) {
+      const int j = -v - 100;
+      if (j > n_in) return set_err(UCLBS_E_STATE, "face %d: inlet %d has no state (uclbs_set_bc_states39)", f, j);
+      fc->tab[f] = c->in_types39[j - 1] == 3 ? -2 : j - 1;
+    } else if (v <= -201 && v >= -299) {
+      const int j = -v - 200;
+      if (j > n_out) return set_err(UCLBS_E_STATE, "face %d: outlet %d has no state (uclbs_set_bc_states39)", f, j);
+      const int t = c->out_types39[j - 1];
+      if (t == 6) {
+        fc->tab[f] = -1;
+      } else if (t == 3) {
+        fc->tab[f] = -2;
+      } else if (t == 2) {
+        if (j > n_in)
+          return set_err(UCLBS_E_STATE, "face %d: outlet %d of type 2 reads inlet %d's state, which is not set", f, j, j);
+        fc->tab[f] = j - 1;
+      } else {
+        fc->tab[f] = n_in + j - 1;
+      }
+    }
+  }
+  return UCLBS_OK;
+}
+
 int launch_steps39(Ctx* c, Level* L, int nsteps, hipEvent_t* ev) {
   if (L->table39 && !c->shifted39)
     return set_err(UCLBS_E_UNSUPPORTED, "plain-lattice D3Q39 levels with a link table are not "
@@ -480,6 +519,10 @@ int launch_steps39(Ctx* c, Level* L, int nsteps, hipEvent_t* ev) {
   for (int i = 0; i < uclbs::Q39; ++i) {
     a.faces.f[i] = c->fs39.f[i];
     a.faces.g[i] = c->fs39.g[i];
+  }
+  {
+    int rc = resolve_bc39(c, L, &a.faces);
+    if (rc) return rc;
   }
   if (a.has_faces || L->table39) {
     int rc = build_near39(c, L);
@@ -668,6 +711,7 @@ int uclbs_destroy(void* ctx) {
   if (c->d_feq_out) (void)hipFree(c->d_feq_out);
   if (c->d_feq_fs) (void)hipFree(c->d_feq_fs);
   if (c->d_ref39) (void)hipFree(c->d_ref39);
+  if (c->d_bc39) (void)hipFree(c->d_bc39);
   for (void* p : {(void*)c->mlp_p, (void*)c->mlp_g, (void*)c->mlp_m, (void*)c->mlp_v,
                   (void*)c->mlp_wbf, (void*)c->samples.X, (void*)c->samples.y,
                   (void*)c->samples.node_of, (void*)c->samples.m_of, (void*)c->samples.count,
@@ -1291,7 +1335,8 @@ int uclbs_set_faces(void* ctx, int lvl, const int32_t face_code[6]) {
   for (int f = 0; f < 6; ++f) {
     const int32_t v = face_code[f];
     const bool per = v == uclbs::kFacePeriodic;
-    const bool ok = q39 ? (per || v == -10 || v == -1000 || v == -6 || v == -11)
+    const bool ok = q39 ? (per || v == -10 || v == -1000 || v == -6 || v == -11 ||
+                           (v <= -101 && v >= -199) || (v <= -201 && v >= -299))
                         : (per || v == -10 || v == -1000 || (v <= -101 && v >= -199) ||
                            (v <= -201 && v >= -299));
     if (!ok)
@@ -1696,6 +1741,91 @@ int uclbs_set_gas(void* ctx, const uclbs_gas* gas) {
   return UCLBS_OK;
 }
 
+// One inlet or outlet table row: the multiplier solve of the stored state from
+// the reference's starting guesses (inlet_outlet_basis_calcs.f90: chi -2.5,
+// pi_aa -0.3, the rest 0; shifted-lattice outlets zeta = u, lambda = u / 25).
+static int bc39_row(Ctx* c, const double* st, bool outlet, double* row) {
+  const double rho = st[0], u = st[1], v = st[2], w = st[3], T = st[4];
+  if (!(rho > 0.0) || !(T > 0.0) || !std::isfinite(u + v + w))
+    return set_err(UCLBS_E_ARG, "inlet/outlet state: rho and T must be positive");
+  const double qf = rho * (u * u + v * v + w * w) + 5 * rho * T;
+  const double tg[uclbs::kM39] = {rho, rho * u, rho * v, rho * w, rho * (T + u * u),
+                                  rho * (T + v * v), rho * (T + w * w), rho * u * v, rho * u * w,
+                                  rho * v * w, qf * u, qf * v, qf * w};
+  double lm[uclbs::kM39] = {-2.5, 0, 0, 0, -0.3, -0.3, -0.3, 0, 0, 0, 0, 0, 0};
+  if (outlet && c->shifted39) {
+    lm[1] = u; lm[2] = v; lm[3] = w;
+    lm[10] = u / 25; lm[11] = v / 25; lm[12] = w / 25;
+  }
+  double lmp[uclbs::kM39];
+  const int it = c->shifted39 ? uclbs::lm_newton39<true>(rho, tg, lm, lmp, 200)
+                              : uclbs::lm_newton39<false>(rho, tg, lm, lmp, 200);
+  if (it < 0) return set_err(UCLBS_E_NUMERIC, "inlet/outlet multipliers did not converge");
+  for (int i = 0; i < uclbs::Q39; ++i)
+    row[i] = c->shifted39 ? uclbs::expo39<true>(lm, rho, i) : uclbs::expo39<false>(lm, rho, i);
+  return UCLBS_OK;
+}
+
+int uclbs_set_bc_states39(void* ctx, int n_in, const double* in_states, const int32_t* in_types,
+                          int n_out, const double* out_states, const int32_t* out_types) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c) return set_err(UCLBS_E_ARG, "null context");
+  if (c->cfg.lattice != UCLBS_LATTICE_D3Q39)
+    return set_err(UCLBS_E_UNSUPPORTED, "set_bc_states39 is for D3Q39 contexts (uclbs_set_bc_states for D3Q27)");
+  if (!c->gas_set) return set_err(UCLBS_E_STATE, "set_bc_states39 needs uclbs_set_gas first");
+  if (n_in < 0 || n_in > 99 || n_out < 0 || n_out > 99 || (n_in && (!in_states || !in_types)) ||
+      (n_out && (!out_states || !out_types)))
+    return set_err(UCLBS_E_ARG, "bad inlet/outlet counts");
+  for (int j = 0; j < n_in; ++j) {
+    const int t = in_types[j];
+    if (t != 1 && t != 2 && t != 3 && t != 4 && t != 7)
+      return set_err(UCLBS_E_ARG, "inlet %d: type %d is not 1, 2, 3, 4 or 7", j + 1, t);
+  }
+  for (int j = 0; j < n_out; ++j) {
+    const int t = out_types[j];
+    if (t != 1 && t != 2 && t != 3 && t != 4 && t != 6 && t != 7)
+      return set_err(UCLBS_E_ARG, "outlet %d: type %d is not 1, 2, 3, 4, 6 or 7", j + 1, t);
+  }
+  const double lead = c->gas39.lead_g;
+  std::vector<double> tab((size_t)std::max(1, n_in + n_out) * 2 * uclbs::Q39, 0.0);
+  // inlet_outlet_basis_calcs.f90: each stored state's row, then g = (2 cv - 3)
+  // T f over every row of its family with that state's T (the reference
+  // assigns the whole g array in the loop, so the last such state's T holds
+  // for all rows solved before it); a type-3 row is the freestream pair
+  auto family = [&](int n, const double* st, const int32_t* ty, int base, bool outlet) -> int {
+    for (int j = 0; j < n; ++j) {
+      double* row = &tab[(size_t)(base + j) * 2 * uclbs::Q39];
+      const int t = ty[j];
+      if (t == 1 || t == 2 || t == 4 || t == 7) {
+        int rc = bc39_row(c, st + 5 * j, outlet, row);
+        if (rc) return rc;
+        const double T = st[5 * j + 4];
+        for (int k = 0; k < n; ++k) {
+          double* r = &tab[(size_t)(base + k) * 2 * uclbs::Q39];
+          for (int i = 0; i < uclbs::Q39; ++i) r[uclbs::Q39 + i] = r[i] * lead * T;
+        }
+      } else if (t == 3) {
+        for (int i = 0; i < uclbs::Q39; ++i) {
+          row[i] = c->fs39.f[i];
+          row[uclbs::Q39 + i] = c->fs39.g[i];
+        }
+      }
+    }
+    return UCLBS_OK;
+  };
+  int rc = family(n_in, in_states, in_types, 0, false);
+  if (!rc) rc = family(n_out, out_states, out_types, n_in, true);
+  if (rc) return rc;
+  if ((rc = set_device(c))) return rc;
+  invalidate_graph(c);
+  if (c->d_bc39) { (void)hipFree(c->d_bc39); c->d_bc39 = nullptr; }
+  HIP_TRY(hipMalloc(&c->d_bc39, tab.size() * sizeof(double)));
+  HIP_TRY(hipMemcpy(c->d_bc39, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+  c->in_types39.assign(in_types, in_types + n_in);
+  c->out_types39.assign(out_types, out_types + n_out);
+  return UCLBS_OK;
+}
+
 static int check39(Ctx* c, int lvl, Level** L) {
   int rc = check_level(c, lvl, L);
   if (rc) return rc;
@@ -1978,7 +2108,8 @@ static std::vector<int> graph_signature(const Ctx* c) {
   std::vector<int> sig = {(int)c->structure, (int)c->keep_alpha, c->cfg.alpha_mode,
                           (int)c->levels.size()};
   // addresses of the boundary tables the captured kernels read
-  for (const void* p : {(const void*)c->d_feq_in, (const void*)c->d_feq_out, (const void*)c->d_feq_fs}) {
+  for (const void* p : {(const void*)c->d_feq_in, (const void*)c->d_feq_out, (const void*)c->d_feq_fs,
+                        (const void*)c->d_bc39}) {
     sig.push_back((int)(uintptr_t)p);
     sig.push_back((int)((uintptr_t)p >> 32));
   }
