@@ -225,6 +225,25 @@ int uclbs_init_fields(void *ctx, int lvl, const double *rho, const double *u, co
 int uclbs_upload_state(void *ctx, int lvl, const double *f, const double *g, const double *lm,
                        int64_t n_nodes);
 int uclbs_download_state(void *ctx, int lvl, double *f, double *g, double *lm, int64_t n_nodes);
+/* ---- AMReX patch data on the device ------------------------------------------
+ * The reference's level state lives in MultiFabs on the device
+ * (amr_info_holder.f95:50-76: mffi, mfgi and the 13 multiplier fabs, nodal,
+ * ghost width nghosts_mid).  uclbs_import_fab copies the valid region of such
+ * a FAB -- FP64, component-major, every component the level's box grown by
+ * `ghost` nodes on each side, x fastest, components comp_stride elements
+ * apart -- into the level's state (populations converted to the storage form
+ * on the device), with no host copy; uclbs_export_fab writes the state back
+ * into the FAB's valid region (ghost cells untouched).  field: UCLBS_FIELD_F
+ * (D3Q27 f or D3Q39 f), UCLBS_FIELD_G (D3Q39 g), UCLBS_FIELD_LM (D3Q39
+ * multipliers, 13 components in the order of uclbs_upload_state).  The pointer
+ * must be device memory of the context's device (checked); writes to it must
+ * be complete before the call, and both calls return when the copy is done.
+ * One-box levels that are not refined. */
+enum { UCLBS_FIELD_F = 0, UCLBS_FIELD_G = 1, UCLBS_FIELD_LM = 2 };
+int uclbs_import_fab(void *ctx, int lvl, int field, const double *fab, int ghost,
+                     int64_t comp_stride);
+int uclbs_export_fab(void *ctx, int lvl, int field, double *fab, int ghost, int64_t comp_stride);
+
 /* ---- checkpoint / restart --------------------------------------------------
  * The reference's restart writer is commented out (restart_write.f95); these
  * give a versioned dump of the device state.  uclbs_checkpoint_write saves,

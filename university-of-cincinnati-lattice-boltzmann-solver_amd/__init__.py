@@ -26,7 +26,7 @@ EXPORTS = (
     "uclbs_get_stats", "uclbs_reset_stats", "uclbs_set_faces", "uclbs_set_bc_states",
     "uclbs_build_links", "uclbs_set_links", "uclbs_download_links", "uclbs_link_counts",
     "uclbs_init_uniform", "uclbs_geom_icosphere", "uclbs_geom_wedge", "uclbs_download_macro39", "uclbs_checkpoint_write",
-    "uclbs_checkpoint_read", "uclbs_set_bc_states39", "uclbs_time_kernels",
+    "uclbs_checkpoint_read", "uclbs_set_bc_states39", "uclbs_import_fab", "uclbs_export_fab", "uclbs_time_kernels",
     "uclbs_copy_bandwidth", "uclbs_set_gas", "uclbs_init_fields", "uclbs_upload_state",
     "uclbs_download_state", "uclbs_set_timestep", "uclbs_mlp_init", "uclbs_mlp_set_params",
     "uclbs_mlp_get_params", "uclbs_mlp_loss_grad", "uclbs_mlp_forward", "uclbs_mlp_reserve",
@@ -123,6 +123,8 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "uclbs_download_macro39": (I, [P, I, dp, dp, dp, dp, dp, dp, I64]),
         "uclbs_checkpoint_write": (I, [P, ctypes.c_char_p]),
         "uclbs_set_bc_states39": (I, [P, I, dp, i3, I, dp, i3]),
+        "uclbs_import_fab": (I, [P, I, I, ctypes.c_void_p, I, I64]),
+        "uclbs_export_fab": (I, [P, I, I, ctypes.c_void_p, I, I64]),
         "uclbs_checkpoint_read": (I, [P, ctypes.c_char_p]),
         "uclbs_time_kernels": (I, [P, I, I, dp, dp]),
         "uclbs_copy_bandwidth": (I, [I64, I, dp]),
@@ -394,6 +396,19 @@ class Solver:
         so, to = pack(outlets)
         self._check(self.lib.uclbs_set_bc_states39(self.ctx, len(inlets), _dptr(si), _iptr(ti),
                                                    len(outlets), _dptr(so), _iptr(to)))
+
+    FIELD = {"f": 0, "g": 1, "lm": 2}
+
+    def import_fab(self, lvl: int, field: str, device_ptr: int, ghost: int, comp_stride: int):
+        """Level state from an AMReX FArrayBox in device memory (address device_ptr,
+        FP64, component-major, ghost width, component stride in elements)."""
+        self._check(self.lib.uclbs_import_fab(self.ctx, lvl, self.FIELD[field],
+                                              ctypes.c_void_p(device_ptr), ghost, comp_stride))
+
+    def export_fab(self, lvl: int, field: str, device_ptr: int, ghost: int, comp_stride: int):
+        """Level state into the valid region of a device FArrayBox."""
+        self._check(self.lib.uclbs_export_fab(self.ctx, lvl, self.FIELD[field],
+                                              ctypes.c_void_p(device_ptr), ghost, comp_stride))
 
     def checkpoint_write(self, path: str):
         """Versioned dump of every level's device state (uclbs_checkpoint_write)."""

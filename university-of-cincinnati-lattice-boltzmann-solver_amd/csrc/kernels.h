@@ -313,6 +313,12 @@ hipError_t launch_d3q39_init(const Box& b, const double* fields, void* f, void* 
                              unsigned long long* bad, hipStream_t s);
 hipError_t launch_d3q39_from_fp64(const Box& b, const double* src, void* dst, int fp32,
                                   const double* ref, hipStream_t s);
+// AMReX FArrayBox (device, FP64, component-major, ghost width, component
+// stride in elements) <-> [ncomp][n] in the box's node order
+hipError_t launch_fab_gather(const double* fab, int ghost, int64_t cstride, const Box& b,
+                             int ncomp, double* dense, hipStream_t s);
+hipError_t launch_fab_scatter(const double* dense, const Box& b, int ncomp, int ghost,
+                              int64_t cstride, double* fab, hipStream_t s);
 hipError_t launch_d3q39_macro(const Box& b, const void* f, const void* g, int fp32, int shifted,
                               const Ref39& rf, double cv, double* out, hipStream_t s);
 hipError_t launch_d3q39_to_fp64(const Box& b, const void* src, double* dst, int fp32,

@@ -2580,6 +2580,109 @@ int uclbs_mlp_train(void* ctx, int64_t steps, int batch, double lr, uint64_t see
 }
 
 
+// ---- AMReX patch data on the device -------------------------------------------
+
+namespace {
+
+int fab_args(Ctx* c, int lvl, int field, const void* fab, int ghost, int64_t cstride, Level** L,
+             int* ncomp) {
+  int rc = check_level(c, lvl, L);
+  if (rc) return rc;
+  if ((*L)->refined || (*L)->multibox)
+    return set_err(UCLBS_E_UNSUPPORTED, "FAB exchange is for one-box levels that are not refined");
+  const bool q39 = c->cfg.lattice == UCLBS_LATTICE_D3Q39;
+  if (field == UCLBS_FIELD_F) *ncomp = q_of(c);
+  else if (q39 && field == UCLBS_FIELD_G) *ncomp = uclbs::Q39;
+  else if (q39 && field == UCLBS_FIELD_LM) *ncomp = uclbs::kM39;
+  else return set_err(UCLBS_E_ARG, "FAB field %d does not exist on this lattice", field);
+  const uclbs::Box& b = (*L)->box;
+  const int64_t cells = (int64_t)(b.nx + 2 * ghost) * (b.ny + 2 * ghost) * (b.nz + 2 * ghost);
+  if (!fab || ghost < 0 || ghost > 8 || cstride < cells)
+    return set_err(UCLBS_E_ARG, "FAB: ghost width %d / component stride %lld do not hold a %dx%dx%d box",
+                   ghost, (long long)cstride, b.nx, b.ny, b.nz);
+  if ((rc = set_device(c))) return rc;
+  hipPointerAttribute_t attr{};
+  if (hipPointerGetAttributes(&attr, fab) != hipSuccess ||
+      (attr.type != hipMemoryTypeDevice && attr.type != hipMemoryTypeManaged)) {
+    (void)hipGetLastError();
+    return set_err(UCLBS_E_ARG, "FAB pointer is not device memory");
+  }
+  return UCLBS_OK;
+}
+
+}  // namespace
+
+int uclbs_import_fab(void* ctx, int lvl, int field, const double* fab, int ghost,
+                     int64_t comp_stride) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  Level* L;
+  int ncomp = 0;
+  int rc = fab_args(c, lvl, field, fab, ghost, comp_stride, &L, &ncomp);
+  if (rc) return rc;
+  const bool fp32 = c->cfg.precision == UCLBS_PREC_FP32_SHIFTED;
+  hipError_t e = hipSuccess;
+  if (field == UCLBS_FIELD_LM) {
+    e = uclbs::launch_fab_gather(fab, ghost, comp_stride, L->box, ncomp, L->lm, c->stream);
+  } else {
+    double* tmp = nullptr;
+    e = hipMalloc(&tmp, (size_t)L->box.n * ncomp * sizeof(double));
+    if (e == hipSuccess)
+      e = uclbs::launch_fab_gather(fab, ghost, comp_stride, L->box, ncomp, tmp, c->stream);
+    void* cur = field == UCLBS_FIELD_F ? L->f[L->cur] : L->g[L->cur];
+    void* other = field == UCLBS_FIELD_F ? L->f[1 - L->cur] : L->g[1 - L->cur];
+    if (e == hipSuccess) {
+      if (c->cfg.lattice == UCLBS_LATTICE_D3Q27)
+        e = uclbs::launch_d3q27_from_fp64(L->box, tmp, cur, fp32, c->stream);
+      else
+        e = uclbs::launch_d3q39_from_fp64(L->box, tmp, cur, fp32,
+                                          c->d_ref39 + (field == UCLBS_FIELD_G ? uclbs::Q39 : 0),
+                                          c->stream);
+    }
+    // the other buffer mirrors the imported one (nodes no kernel writes)
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(other, cur, level_bytes(c, L) / q_of(c) * ncomp,
+                         hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (tmp) (void)hipFree(tmp);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return set_err(UCLBS_E_HIP, "import_fab: %s", hipGetErrorString(e));
+  invalidate_graph(c);
+  return UCLBS_OK;
+}
+
+int uclbs_export_fab(void* ctx, int lvl, int field, double* fab, int ghost, int64_t comp_stride) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  Level* L;
+  int ncomp = 0;
+  int rc = fab_args(c, lvl, field, fab, ghost, comp_stride, &L, &ncomp);
+  if (rc) return rc;
+  const bool fp32 = c->cfg.precision == UCLBS_PREC_FP32_SHIFTED;
+  hipError_t e = hipSuccess;
+  if (field == UCLBS_FIELD_LM) {
+    e = uclbs::launch_fab_scatter(L->lm, L->box, ncomp, ghost, comp_stride, fab, c->stream);
+  } else {
+    double* tmp = nullptr;
+    e = hipMalloc(&tmp, (size_t)L->box.n * ncomp * sizeof(double));
+    const void* cur = field == UCLBS_FIELD_F ? L->f[L->cur] : L->g[L->cur];
+    if (e == hipSuccess) {
+      if (c->cfg.lattice == UCLBS_LATTICE_D3Q27)
+        e = uclbs::launch_d3q27_to_fp64(L->box, cur, tmp, fp32, c->stream);
+      else
+        e = uclbs::launch_d3q39_to_fp64(L->box, cur, tmp, fp32,
+                                        c->d_ref39 + (field == UCLBS_FIELD_G ? uclbs::Q39 : 0),
+                                        c->stream);
+    }
+    if (e == hipSuccess)
+      e = uclbs::launch_fab_scatter(tmp, L->box, ncomp, ghost, comp_stride, fab, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (tmp) (void)hipFree(tmp);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return set_err(UCLBS_E_HIP, "export_fab: %s", hipGetErrorString(e));
+  return UCLBS_OK;
+}
+
 // ---- checkpoint / restart ------------------------------------------------------
 // File: "UCLBSCKP", version, lattice, precision, level count; per defined
 // level a header (level index, node count and box extents, refined / many-box
