@@ -409,6 +409,63 @@ int uclbs_copy_bandwidth(int64_t nbytes, int reps, double *gbps);
 int uclbs_get_stats(void *ctx, uclbs_stats *out);
 int uclbs_reset_stats(void *ctx);
 
+/* ---- output: Plot3D files and microphones (output/plot3d_output.f95,
+ * output/mic_setup_stuff/mic_setup.f95, mic_write.f95,
+ * output/utilities/mic_initialize.f95) ---------------------------------- */
+
+/* Physical placement of level 0: amrex_problo / amrex_probhi of the domain
+ * (amr_processes.f95:1155-1168 get_real_coords).  Level l has spacing
+ * (probhi - problo) / (cells of level 0 x 2^l) and its first domain node at
+ * problo.  Without this call problo is 0 and level 0's spacing is 1. */
+int uclbs_set_placement(void *ctx, const double problo[3], const double probhi[3]);
+
+/* Header values of every solution block (plot3d_output.f95:256-258) */
+typedef struct uclbs_plot3d_header {
+  double mach, reynolds, time, gamma, t_ref, stag_temperature, cv, cp, gas_r;
+  int32_t timestep;      /* names the files and ends each block header */
+  int32_t max_level;     /* levels 0..max_level are written (clipped to the finest) */
+  int64_t max_subrecord; /* bytes per Fortran subrecord; 0: gfortran's 2147483639 */
+} uclbs_plot3d_header;
+
+/* Replaces plot3d_output (plot3d_output.f95:76-305): writes
+ * dir/uclbs_grid_<timestep, 9 digits>.x (Plot3D multi-block grid: block
+ * count, extents, then per block x, y, z as REAL and IBLANK = 1 on fluid
+ * nodes, -1 on code -666, 0 otherwise) and dir/uclbs_data_<timestep>.q
+ * (extents with 6 variables, then per block the header record and rho, u,
+ * v, w, T, alpha as FP64), Fortran unformatted sequential records.  One
+ * block per box of each level, its valid region.  T is 1/3 on D3Q27;
+ * alpha is the kept entropic alpha (uclbs_set_keep_alpha), else 2.
+ * Asynchronous: the flow variables are computed and copied to pinned host
+ * memory on the library's stream, a host thread writes the files; the
+ * stream goes on stepping meanwhile.  uclbs_output_wait (or the next
+ * uclbs_plot3d_write, or uclbs_destroy) waits for the files and reports a
+ * failed write. */
+int uclbs_plot3d_write(void *ctx, const char *dir, const uclbs_plot3d_header *h);
+int uclbs_output_wait(void *ctx);
+
+/* Replaces mic_setup + mic_initialize (mic_setup.f95:45-225,
+ * mic_initialize.f95:17-70): n microphones at physical positions loc[n][3]
+ * recording the fields of type (bits 32 pressure, 16 temperature,
+ * 8 density, 4 u, 2 v, 1 w) on level min(level, finest).  Each gets the
+ * eight corners of its cell and the reference's weights (distances with
+ * + rdx on the far corners, divided by their sum over fluid corners; corners
+ * at x index <= 0 weigh 0); files dir/microphoneNNNNN.txt get the header
+ * line.  valid[i] (may be NULL): 0 invalid (outside [problo, probhi] or
+ * no fluid corner), 1 sampled, 2 valid but outside the level's boxes (file
+ * with the header only).  Set up after the levels and link tables. */
+int uclbs_mic_setup(void *ctx, const char *dir, int n, const double *loc, const int32_t *type,
+                    const int32_t *level, int32_t *valid);
+/* Replaces mic_write (mic_write.f95:45-190) in two halves: _sample
+ * enqueues the flow variables at every microphone's corners into a device
+ * ring (no host synchronisation); _flush copies the held samples back and
+ * appends one line per sample to each file (F12.5 values: p, T, rho, u, v, w
+ * as selected).  A full ring flushes by itself. */
+int uclbs_mic_sample(void *ctx);
+int uclbs_mic_flush(void *ctx);
+/* corners (level positions pos[8][3]), weights and stored nodes (-1: not a
+ * fluid node) of microphone i (0-based) */
+int uclbs_mic_weights(void *ctx, int i, int32_t pos[24], double w[8], int64_t node[8]);
+
 #ifdef __cplusplus
 }
 #endif

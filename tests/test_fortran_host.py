@@ -107,3 +107,41 @@ def test_fortran_two_level_loop_matches_python_schedule(pkg, tmp_path):
     ref = np.stack(s.macro(0))
     assert np.abs(data - ref).max() < 1e-12
     assert np.all(np.isfinite(data))
+
+
+@pytest.mark.gpu
+def test_fortran_driver_writes_plot3d_and_microphone(orc, tmp_path):
+    """The Fortran host writes the final D3Q39 state through
+    uclbs_plot3d_write / uclbs_mic_*: the grid file equals the oracle's byte
+    for byte (domain placed at [0, n dx], dx = 1e-3, no link table: IBLANK 1),
+    the solution block's rho, u, v, w, T are within 1e-12 relative of the
+    oracle's flow variables of the state the driver wrote, and the
+    microphone file has its header and one sample line."""
+    import o_output as oo
+    n3, steps = (24, 16, 12), 6
+    out = tmp_path / "d3q39.bin"
+    r = subprocess.run([DRIVER39, str(out), *map(str, n3), str(steps), "1", str(tmp_path)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    n = n3[0] * n3[1] * n3[2]
+    data = np.fromfile(out, dtype=np.float64)
+    per = (39 + 39 + 13) * n
+    f1, g1 = data[per:per + 39 * n].reshape(39, n), data[per + 39 * n:per + 78 * n].reshape(39, n)
+    c = orc.consts39(mach=1.5, shifted=True)
+    m = np.stack(list(orc.macro39(f1, g1, c)) + [np.full(n, 2.0)])
+    plo, dx, dlo = oo.level_geom((0.0, 0.0, 0.0), [v * 1e-3 for v in n3], n3, (0, 0, 0), 0)
+    blocks = [dict(dims=n3, start=oo.block_start(plo, dx, dlo, (0, 0, 0)), dx=dx,
+                   blank=np.ones(n, np.int32), fields=m)]
+    ts = f"{steps:09d}"
+    assert (tmp_path / f"uclbs_grid_{ts}.x").read_bytes() == oo.grid_file(blocks)
+    recs = oo.read_records((tmp_path / f"uclbs_data_{ts}.q").read_bytes())
+    h = dict(mach=1.5, reynolds=0.0, time=float(steps), gamma=1.4, t_ref=0.0, stag_temperature=0.0,
+             cv=0.0, cp=0.0, gas_r=0.0, timestep=steps)
+    assert len(recs) == 4 and recs[2] == oo.header_record(h)
+    got = np.frombuffer(recs[3], np.float64).reshape(6, n)
+    for k in range(5):
+        assert np.abs(got[k] - m[k]).max() <= 1e-12 * np.abs(m[k]).max(), k
+    assert np.all(got[5] == 2.0)
+    lines = (tmp_path / "microphone00001.txt").read_text().splitlines()
+    assert lines[0] == oo.mic_header(63).rstrip("\n") and len(lines) == 2
+    assert len(lines[1]) == 6 * 13 - 1

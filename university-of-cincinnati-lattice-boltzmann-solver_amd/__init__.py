@@ -33,7 +33,9 @@ EXPORTS = (
     "uclbs_mlp_collect", "uclbs_mlp_download_samples", "uclbs_mlp_download_sample_nodes",
     "uclbs_mlp_train", "uclbs_interlevel",
     "uclbs_init_from_coarse", "uclbs_fine_to_coarse", "uclbs_define_level_boxes",
-    "uclbs_advance", "uclbs_autotune", "uclbs_mlp_forward_noalloc",
+    "uclbs_advance", "uclbs_autotune", "uclbs_mlp_forward_noalloc", "uclbs_set_placement",
+    "uclbs_plot3d_write", "uclbs_output_wait", "uclbs_mic_setup", "uclbs_mic_sample",
+    "uclbs_mic_flush", "uclbs_mic_weights",
 )
 MLP_PARAMS = 9217
 FACE_PERIODIC = 2147483647
@@ -55,6 +57,16 @@ class Gas(ctypes.Structure):
                 ("startup_timesteps", ctypes.c_int32), ("turbulent", ctypes.c_int32),
                 ("shifted", ctypes.c_int32), ("lm_maxit", ctypes.c_int32),
                 ("reserved", ctypes.c_int32 * 4)]
+
+
+class Plot3DHeader(ctypes.Structure):
+    """uclbs_plot3d_header: the values of each solution block's header record
+    (plot3d_output.f95:256-258), the timestep naming the files, the last level
+    written and the Fortran subrecord length (0: gfortran's default)."""
+    _fields_ = [(k, ctypes.c_double) for k in ("mach", "reynolds", "time", "gamma", "t_ref",
+                                               "stag_temperature", "cv", "cp", "gas_r")] + [
+        ("timestep", ctypes.c_int32), ("max_level", ctypes.c_int32),
+        ("max_subrecord", ctypes.c_int64)]
 
 
 class Config(ctypes.Structure):
@@ -152,6 +164,13 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "uclbs_interlevel": (I, [P, I, D]),
         "uclbs_init_from_coarse": (I, [P, I]),
         "uclbs_fine_to_coarse": (I, [P, I]),
+        "uclbs_set_placement": (I, [P, dp, dp]),
+        "uclbs_plot3d_write": (I, [P, ctypes.c_char_p, ctypes.POINTER(Plot3DHeader)]),
+        "uclbs_output_wait": (I, [P]),
+        "uclbs_mic_setup": (I, [P, ctypes.c_char_p, I, dp, i3, i3, i3]),
+        "uclbs_mic_sample": (I, [P]),
+        "uclbs_mic_flush": (I, [P]),
+        "uclbs_mic_weights": (I, [P, I, i3, dp, ctypes.POINTER(I64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -359,6 +378,7 @@ class Solver:
     # ---- D3Q39 ---------------------------------------------------------------
     def set_gas(self, gama=1.4, temperature=300.0, pressure=101325.0, mach=0.3, dx=1e-3,
                 startup=25, turbulent=True, shifted=-1, lm_maxit=50):
+        self._gas = dict(gama=gama, temperature=temperature, pressure=pressure, mach=mach)
         g = Gas(gama=gama, temperature=temperature, pressure=pressure, mach=mach, dx=dx,
                 startup_timesteps=startup, turbulent=int(turbulent), shifted=shifted,
                 lm_maxit=lm_maxit)
@@ -417,6 +437,65 @@ class Solver:
     def checkpoint_read(self, path: str):
         """Restore a dump into this context, set up as the saved one was."""
         self._check(self.lib.uclbs_checkpoint_read(self.ctx, os.fsencode(path)))
+
+    # ---- output: Plot3D and microphones -------------------------------------------
+    def set_placement(self, problo, probhi):
+        """amrex_problo / amrex_probhi of the level-0 domain."""
+        lo = np.ascontiguousarray(problo, np.float64)
+        hi = np.ascontiguousarray(probhi, np.float64)
+        self._check(self.lib.uclbs_set_placement(self.ctx, _dptr(lo), _dptr(hi)))
+
+    def plot3d(self, directory: str, timestep: int, time: float = 0.0, max_level: int = 31,
+               wait: bool = True, max_subrecord: int = 0, **values):
+        """Write uclbs_grid_<timestep>.x / uclbs_data_<timestep>.q into directory
+        (plot3d_output.f95:76-305).  values: the header's mach, reynolds,
+        gamma, t_ref, stag_temperature, cv, cp, gas_r (mach and gamma default
+        to the gas of set_gas, the rest to 0).  wait=False returns once the
+        copy is enqueued; output_wait() then waits for the files."""
+        gas = getattr(self, "_gas", {})
+        h = Plot3DHeader(timestep=int(timestep), max_level=int(max_level),
+                         max_subrecord=int(max_subrecord), time=float(time),
+                         mach=float(values.pop("mach", gas.get("mach", 0.0))),
+                         gamma=float(values.pop("gamma", gas.get("gama", 0.0))))
+        for k, v in values.items():
+            if k not in ("reynolds", "t_ref", "stag_temperature", "cv", "cp", "gas_r"):
+                raise TypeError(f"unknown header value {k}")
+            setattr(h, k, float(v))
+        self._check(self.lib.uclbs_plot3d_write(self.ctx, os.fsencode(directory), ctypes.byref(h)))
+        if wait:
+            self.output_wait()
+
+    def output_wait(self):
+        self._check(self.lib.uclbs_output_wait(self.ctx))
+
+    def mic_setup(self, directory: str, locations, types, levels) -> np.ndarray:
+        """Microphones at physical locations (n x 3) recording the fields of
+        types (bits 32 p, 16 T, 8 rho, 4 u, 2 v, 1 w) on levels; returns the
+        validity per microphone (0 invalid, 1 sampled, 2 outside the level's
+        boxes)."""
+        loc = np.ascontiguousarray(locations, np.float64).reshape(-1, 3)
+        n = loc.shape[0]
+        ty = np.ascontiguousarray(np.broadcast_to(np.asarray(types, np.int32), (n,)))
+        lv = np.ascontiguousarray(np.broadcast_to(np.asarray(levels, np.int32), (n,)))
+        valid = np.zeros(max(n, 1), np.int32)
+        self._check(self.lib.uclbs_mic_setup(self.ctx, os.fsencode(directory), n, _dptr(loc),
+                                             _iptr(ty), _iptr(lv), _iptr(valid)))
+        return valid[:n]
+
+    def mic_sample(self):
+        self._check(self.lib.uclbs_mic_sample(self.ctx))
+
+    def mic_flush(self):
+        self._check(self.lib.uclbs_mic_flush(self.ctx))
+
+    def mic_weights(self, i: int):
+        """(corner positions 8 x 3, weights 8, stored nodes 8) of microphone i."""
+        pos = np.zeros(24, np.int32)
+        w = np.zeros(8)
+        node = np.zeros(8, np.int64)
+        self._check(self.lib.uclbs_mic_weights(self.ctx, i, _iptr(pos), _dptr(w),
+                                               node.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))))
+        return pos.reshape(8, 3), w, node
 
     def macro39(self, lvl: int):
         """(rho, u, v, w, T, press) of a D3Q39 level, computed on the device

@@ -363,28 +363,56 @@ __global__ void k27_to_fp64(const void* __restrict__ src_, double* __restrict__ 
   }
 }
 
+// density and momentum of stored node k of a level of n nodes
 template <bool kShifted>
-__global__ void k27_macro(const void* __restrict__ src_, Box b, double* __restrict__ rho,
-                          double* __restrict__ ux, double* __restrict__ uy,
-                          double* __restrict__ uz) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= b.n) return;
-  double r = 0.0, jx = 0.0, jy = 0.0, jz = 0.0;
+__device__ __forceinline__ void moments27(const void* __restrict__ src_, int64_t n, int64_t k,
+                                          double& r, double& jx, double& jy, double& jz) {
+  r = jx = jy = jz = 0.0;
 #pragma unroll
   for (int i = 0; i < 27; ++i) {
     double v;
-    if constexpr (kShifted) v = (double)static_cast<const float*>(src_)[(int64_t)i * b.n + k];
-    else v = static_cast<const double*>(src_)[(int64_t)i * b.n + k];
+    if constexpr (kShifted) v = (double)static_cast<const float*>(src_)[(int64_t)i * n + k];
+    else v = static_cast<const double*>(src_)[(int64_t)i * n + k];
     r += v;
     jx += CX27[i] * v;
     jy += CY27[i] * v;
     jz += CZ27[i] * v;
   }
   if constexpr (kShifted) r += 1.0;
+}
+
+template <bool kShifted>
+__global__ void k27_macro(const void* __restrict__ src_, Box b, double* __restrict__ rho,
+                          double* __restrict__ ux, double* __restrict__ uy,
+                          double* __restrict__ uz) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= b.n) return;
+  double r, jx, jy, jz;
+  moments27<kShifted>(src_, b.n, k, r, jx, jy, jz);
   rho[k] = r;
   ux[k] = jx / r;
   uy[k] = jy / r;
   uz[k] = jz / r;
+}
+
+// flow variables at listed stored nodes: out[6][m] = rho, u, v, w, T, p with
+// the isothermal lattice temperature T = 1/3 and p = rho (T + |u|^2 / 2), the
+// pressure column of calculate_flow_variables.f95:220-221
+template <bool kShifted>
+__global__ void k27_probe(const void* __restrict__ src_, int64_t n,
+                          const uint32_t* __restrict__ nodes, unsigned m,
+                          double* __restrict__ out) {
+  const unsigned j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  double r, jx, jy, jz;
+  moments27<kShifted>(src_, n, (int64_t)nodes[j], r, jx, jy, jz);
+  const double u = jx / r, v = jy / r, w = jz / r, T = 1.0 / 3.0;
+  out[j] = r;
+  out[m + j] = u;
+  out[2 * m + j] = v;
+  out[3 * m + j] = w;
+  out[4 * m + j] = T;
+  out[5 * m + j] = r * (T + (u * u + v * v + w * w) / 2.0);
 }
 
 // Taylor-Green vortex: u = U0 (sin X cos Y cos Z, -cos X sin Y cos Z, 0) plus
@@ -687,6 +715,14 @@ hipError_t launch_d3q27_macro(const Box& b, const void* f, int shifted, double* 
                               double* ux, double* uy, double* uz, hipStream_t s) {
   if (shifted) k27_macro<true><<<blocks_for(b.n, 256), 256, 0, s>>>(f, b, rho, ux, uy, uz);
   else k27_macro<false><<<blocks_for(b.n, 256), 256, 0, s>>>(f, b, rho, ux, uy, uz);
+  return hipGetLastError();
+}
+
+hipError_t launch_d3q27_probe(const void* f, int64_t n, int shifted, const uint32_t* nodes,
+                              unsigned m, double* out, hipStream_t s) {
+  if (m == 0) return hipSuccess;
+  if (shifted) k27_probe<true><<<blocks_for(m, 64), 64, 0, s>>>(f, n, nodes, m, out);
+  else k27_probe<false><<<blocks_for(m, 64), 64, 0, s>>>(f, n, nodes, m, out);
   return hipGetLastError();
 }
 

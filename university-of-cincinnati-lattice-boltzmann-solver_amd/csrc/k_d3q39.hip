@@ -1068,40 +1068,64 @@ __global__ void k39_from_fp64(const double* __restrict__ src, void* __restrict__
     dst[k] = (S)(src[k] - ref[k / n]);
 }
 
-// flow variables of calculate_flow_variables.f95:201-221 per node from the
-// current populations: rho, u, v, w, T and the reference's press =
-// rho (T + |u|^2 / 2); out is [6][n]
+// flow variables of calculate_flow_variables.f95:201-221 at stored node k of a
+// level of n nodes: rho, u, v, w, T and the reference's press = rho (T + |u|^2 / 2)
+template <bool kFp32, bool kShifted>
+__device__ __forceinline__ void flow39(const void* __restrict__ f_, const void* __restrict__ g_,
+                                       size_t n, size_t k, const Ref39& rf, double cv,
+                                       double o[6]) {
+  using S = typename std::conditional<kFp32, float, double>::type;
+  const S* __restrict__ fA = static_cast<const S*>(f_);
+  const S* __restrict__ gA = static_cast<const S*>(g_);
+  double r = 0.0, mx = 0.0, my = 0.0, mz = 0.0, en = 0.0;
+#pragma unroll
+  for (int i = 0; i < Q39; ++i) {
+    const int cx = cx39<kShifted>(i), cy = CY39[i], cz = CZ39[i];
+    const double f = (double)fA[(size_t)i * n + k] + rf.f[i],
+                 g = (double)gA[(size_t)i * n + k] + rf.g[i];
+    r += f;
+    mx += (double)cx * f;
+    my += (double)cy * f;
+    mz += (double)cz * f;
+    en += (double)(cx * cx + cy * cy + cz * cz) * f + g;
+  }
+  const double u = mx / r, v = my / r, w = mz / r, vm = u * u + v * v + w * w;
+  const double T = (en - r * vm) / (2 * cv * r);
+  o[0] = r;
+  o[1] = u;
+  o[2] = v;
+  o[3] = w;
+  o[4] = T;
+  o[5] = r * (T + vm / 2.0);
+}
+
+// every node: out is [6][n]
 template <bool kFp32, bool kShifted>
 __global__ __launch_bounds__(256) void k39_macro(const void* __restrict__ f_,
                                                  const void* __restrict__ g_, Box b, Ref39 rf,
                                                  double cv, double* __restrict__ out) {
-  using S = typename std::conditional<kFp32, float, double>::type;
-  const S* __restrict__ fA = static_cast<const S*>(f_);
-  const S* __restrict__ gA = static_cast<const S*>(g_);
   const size_t n = (size_t)b.n;
   for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n;
        k += (size_t)gridDim.x * blockDim.x) {
-    double r = 0.0, mx = 0.0, my = 0.0, mz = 0.0, en = 0.0;
+    double o[6];
+    flow39<kFp32, kShifted>(f_, g_, n, k, rf, cv, o);
 #pragma unroll
-    for (int i = 0; i < Q39; ++i) {
-      const int cx = cx39<kShifted>(i), cy = CY39[i], cz = CZ39[i];
-      const double f = (double)fA[(size_t)i * n + k] + rf.f[i],
-                   g = (double)gA[(size_t)i * n + k] + rf.g[i];
-      r += f;
-      mx += (double)cx * f;
-      my += (double)cy * f;
-      mz += (double)cz * f;
-      en += (double)(cx * cx + cy * cy + cz * cz) * f + g;
-    }
-    const double u = mx / r, v = my / r, w = mz / r, vm = u * u + v * v + w * w;
-    const double T = (en - r * vm) / (2 * cv * r);
-    out[k] = r;
-    out[n + k] = u;
-    out[2 * n + k] = v;
-    out[3 * n + k] = w;
-    out[4 * n + k] = T;
-    out[5 * n + k] = r * (T + vm / 2.0);
+    for (int j = 0; j < 6; ++j) out[j * n + k] = o[j];
   }
+}
+
+// listed stored nodes: out is [6][m]
+template <bool kFp32, bool kShifted>
+__global__ __launch_bounds__(64) void k39_probe(const void* __restrict__ f_,
+                                                const void* __restrict__ g_, size_t n,
+                                                const uint32_t* __restrict__ nodes, unsigned m,
+                                                Ref39 rf, double cv, double* __restrict__ out) {
+  const unsigned j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= m) return;
+  double o[6];
+  flow39<kFp32, kShifted>(f_, g_, n, nodes[j], rf, cv, o);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) out[(size_t)c * m + j] = o[c];
 }
 
 template <bool kFp32>
@@ -1236,6 +1260,21 @@ hipError_t launch_d3q39_macro(const Box& b, const void* f, const void* g, int fp
   } else {
     if (shifted) hipLaunchKernelGGL((k39_macro<false, true>), dim3(blocks), dim3(256), 0, s, f, g, b, rf, cv, out);
     else hipLaunchKernelGGL((k39_macro<false, false>), dim3(blocks), dim3(256), 0, s, f, g, b, rf, cv, out);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_d3q39_probe(const void* f, const void* g, int64_t n, int fp32, int shifted,
+                              const Ref39& rf, double cv, const uint32_t* nodes, unsigned m,
+                              double* out, hipStream_t s) {
+  if (m == 0) return hipSuccess;
+  const dim3 grid((m + 63) / 64), blk(64);
+  if (fp32) {
+    if (shifted) hipLaunchKernelGGL((k39_probe<true, true>), grid, blk, 0, s, f, g, (size_t)n, nodes, m, rf, cv, out);
+    else hipLaunchKernelGGL((k39_probe<true, false>), grid, blk, 0, s, f, g, (size_t)n, nodes, m, rf, cv, out);
+  } else {
+    if (shifted) hipLaunchKernelGGL((k39_probe<false, true>), grid, blk, 0, s, f, g, (size_t)n, nodes, m, rf, cv, out);
+    else hipLaunchKernelGGL((k39_probe<false, false>), grid, blk, 0, s, f, g, (size_t)n, nodes, m, rf, cv, out);
   }
   return hipGetLastError();
 }

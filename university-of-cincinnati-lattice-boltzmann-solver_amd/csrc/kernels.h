@@ -319,6 +319,14 @@ hipError_t launch_fab_gather(const double* fab, int ghost, int64_t cstride, cons
                              int ncomp, double* dense, hipStream_t s);
 hipError_t launch_fab_scatter(const double* dense, const Box& b, int ncomp, int ghost,
                               int64_t cstride, double* fab, hipStream_t s);
+hipError_t launch_blank_gather(const int32_t* code, int ghost, const Box& b, int32_t* blank,
+                               hipStream_t s);
+hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s);
+hipError_t launch_d3q27_probe(const void* f, int64_t n, int shifted, const uint32_t* nodes,
+                              unsigned m, double* out, hipStream_t s);
+hipError_t launch_d3q39_probe(const void* f, const void* g, int64_t n, int fp32, int shifted,
+                              const Ref39& rf, double cv, const uint32_t* nodes, unsigned m,
+                              double* out, hipStream_t s);
 hipError_t launch_d3q39_macro(const Box& b, const void* f, const void* g, int fp32, int shifted,
                               const Ref39& rf, double cv, double* out, hipStream_t s);
 hipError_t launch_d3q39_to_fp64(const Box& b, const void* src, double* dst, int fp32,

@@ -10,6 +10,7 @@
 #include <string>
 #include <algorithm>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "../../include/uclbs_hip.h"
@@ -134,6 +135,27 @@ struct Level {
   std::vector<std::vector<int>> buckets;
 };
 
+// a microphone (mic_setup.f95:45-225): its level, the eight corners of the
+// cell it falls in and their weights
+struct Mic {
+  int number = 0;     // 1-based, names the file microphoneNNNNN.txt
+  int level = 0;
+  int valid = 0;      // 0 invalid, 1 sampled, 2 valid but outside the level's boxes
+  int mask = 0;       // recorded fields: 32 p, 16 T, 8 rho, 4 u, 2 v, 1 w
+  int32_t pos[8][3] = {};   // corner positions in the level's index space
+  int64_t node[8] = {};     // stored node of each corner (-1: not a fluid node)
+  double w[8] = {};
+  unsigned slot = 0;        // first of its eight entries in the level's probe list
+  std::string path;
+};
+
+struct MicLevel {
+  int lvl = 0;
+  unsigned m = 0;              // probed nodes (eight per sampled microphone)
+  uint32_t* d_nodes = nullptr;
+  size_t off = 0;              // doubles from the start of a sample
+};
+
 struct Ctx {
   uclbs_config cfg{};
   int device = 0;
@@ -185,7 +207,31 @@ struct Ctx {
   hipGraphExec_t graph = nullptr;
   std::vector<int> graph_sig;
   int64_t structure = 0;  // bumped by every change of levels or links
+  // physical placement of the level-0 domain (amrex_problo, amrex_probhi);
+  // unset: problo 0 and unit spacing at level 0
+  bool placed = false;
+  double problo[3] = {0.0, 0.0, 0.0}, probhi[3] = {0.0, 0.0, 0.0};
+  // Plot3D writer: flow variables of one level, the packed blocks on the
+  // device and in pinned host memory, and the thread that writes the files
+  double* out_fields = nullptr;
+  size_t out_fields_cap = 0;  // doubles
+  char* out_dev = nullptr;
+  char* out_host = nullptr;
+  size_t out_cap = 0;  // bytes of out_dev and of out_host
+  hipEvent_t out_ev = nullptr;
+  std::thread out_thread;
+  int out_rc = 0;
+  std::string out_msg;
+  // microphones: set-up, probe lists per level, the device ring of samples
+  std::vector<Mic> mics;
+  std::vector<MicLevel> mic_levels;
+  double* mic_ring = nullptr;
+  size_t mic_stride = 0;  // doubles per sample
+  int mic_cap = 0, mic_count = 0;
+  int64_t mic_structure = -1;
 };
+
+void release_output(Ctx* c);  // joins the writer thread, frees the output buffers
 
 int hip_rc(hipError_t e) {
   return e == hipSuccess ? UCLBS_OK : set_err(UCLBS_E_HIP, "HIP error: %s", hipGetErrorString(e));
@@ -704,6 +750,7 @@ int uclbs_destroy(void* ctx) {
   if (!c) return UCLBS_OK;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  release_output(c);
   if (c->graph) (void)hipGraphExecDestroy(c->graph);
   for (auto& L : c->levels) free_level(L);
   if (c->d_stats) (void)hipFree(c->d_stats);
@@ -2864,6 +2911,605 @@ int uclbs_checkpoint_read(void* ctx, const char* path) {
   }
   std::fclose(fp);
   invalidate_graph(c);
+  return UCLBS_OK;
+}
+
+}  // extern "C"
+
+// ---- output: Plot3D files and microphones -------------------------------------------
+
+namespace {
+
+// spacing and domain corner of level lvl: AMReX geometry with refinement ratio
+// 2 over the level-0 domain (dx = (probhi - problo) / cells, the domain's
+// first node at problo); unplaced contexts use problo 0 and spacing 1 at level 0
+void level_geom(const Ctx* c, int lvl, double lo[3], double dx[3], int dlo[3]) {
+  const Level& L0 = c->levels[0];
+  for (int d = 0; d < 3; ++d) {
+    const int n0 = L0.hi[d] - L0.lo[d] + 1;
+    lo[d] = c->placed ? c->problo[d] : 0.0;
+    const double hi = c->placed ? c->probhi[d] : (double)n0;
+    dx[d] = (hi - lo[d]) / ((double)n0 * (double)(1 << lvl));
+    dlo[d] = L0.lo[d] * (1 << lvl);
+  }
+}
+
+// one Plot3D block: the valid region of one box of one level
+struct P3Block {
+  int lvl = 0;
+  int dims[3] = {0, 0, 0};
+  double start[3] = {0, 0, 0}, dx[3] = {0, 0, 0};
+  size_t nv = 0;
+  size_t dbl_off = 0;  // byte offsets of its fields and its IBLANK in the packed buffer
+  size_t int_off = 0;
+};
+
+// Fortran unformatted sequential records (gfortran layout): 4-byte length
+// markers before and after each subrecord; a record longer than maxsub bytes
+// is split into subrecords whose leading marker is negative when more follow
+// and whose trailing marker is negative when one precedes
+struct RecWriter {
+  std::FILE* fp = nullptr;
+  int64_t maxsub = 2147483639;
+  int64_t rem = 0, sub = 0, sublen = 0;
+  bool first = true, ok = true;
+  void marker(int64_t v) {
+    const int32_t m = (int32_t)v;
+    ok = ok && std::fwrite(&m, 4, 1, fp) == 1;
+  }
+  void open_sub() {
+    sublen = std::min(rem, maxsub);
+    sub = sublen;
+    marker(rem > sublen ? -sublen : sublen);
+  }
+  void close_sub() {
+    marker(first ? sublen : -sublen);
+    first = false;
+    rem -= sublen;
+  }
+  void begin(int64_t total) {
+    rem = total;
+    first = true;
+    open_sub();
+  }
+  void put(const void* p, size_t n) {
+    const char* q = static_cast<const char*>(p);
+    while (n && ok) {
+      if (sub == 0) {
+        close_sub();
+        open_sub();
+      }
+      const size_t k = (size_t)std::min<int64_t>((int64_t)n, sub);
+      ok = ok && std::fwrite(q, 1, k, fp) == k;
+      q += k;
+      n -= k;
+      sub -= (int64_t)k;
+    }
+  }
+  void end() { close_sub(); }
+};
+
+struct P3Job {
+  Ctx* c = nullptr;
+  std::string grid, data;
+  uclbs_plot3d_header h{};
+  std::vector<P3Block> blocks;
+};
+
+// the files of plot3d_output.f95:76-305 from the packed host copy
+void write_plot3d(P3Job job) {
+  Ctx* c = job.c;
+  auto fail = [&](const char* what, const std::string& path) {
+    c->out_rc = UCLBS_E_ARG;
+    c->out_msg = std::string("plot3d: ") + what + " " + path;
+  };
+  const hipError_t e = hipEventSynchronize(c->out_ev);
+  if (e != hipSuccess) {
+    c->out_rc = UCLBS_E_HIP;
+    c->out_msg = std::string("plot3d: copy to the host failed: ") + hipGetErrorString(e);
+    return;
+  }
+  const int32_t nb = (int32_t)job.blocks.size();
+  std::vector<int32_t> dims;
+  for (const P3Block& b : job.blocks) dims.insert(dims.end(), b.dims, b.dims + 3);
+  const int64_t maxsub = job.h.max_subrecord > 0 ? job.h.max_subrecord : 2147483639;
+  std::vector<char> iobuf(1 << 23);
+  // grid: block count, block extents, then per block x, y, z (REAL) and IBLANK
+  std::FILE* fp = std::fopen(job.grid.c_str(), "wb");
+  if (!fp) return fail("cannot create", job.grid);
+  std::setvbuf(fp, iobuf.data(), _IOFBF, iobuf.size());
+  RecWriter r{fp, maxsub};
+  r.begin(4);
+  r.put(&nb, 4);
+  r.end();
+  r.begin((int64_t)dims.size() * 4);
+  r.put(dims.data(), dims.size() * 4);
+  r.end();
+  std::vector<float> tmp;
+  for (const P3Block& b : job.blocks) {
+    r.begin((int64_t)b.nv * 16);
+    for (int ax = 0; ax < 3; ++ax) {
+      // start + dx (i - lo) in double, stored as REAL (plot3d_output.f95:270-272)
+      tmp.resize(b.nv);
+      size_t k = 0;
+      for (int cz = 0; cz < b.dims[2]; ++cz)
+        for (int cy = 0; cy < b.dims[1]; ++cy)
+          for (int cx = 0; cx < b.dims[0]; ++cx) {
+            const int i = ax == 0 ? cx : (ax == 1 ? cy : cz);
+            tmp[k++] = (float)(b.start[ax] + b.dx[ax] * (double)i);
+          }
+      r.put(tmp.data(), b.nv * 4);
+    }
+    r.put(c->out_host + b.int_off, b.nv * 4);
+    r.end();
+  }
+  const bool gok = r.ok && std::fclose(fp) == 0;
+  if (!gok) return fail("write failed:", job.grid);
+  // solution: block count, extents with the variable count 6 and a 0, then
+  // per block the header record and rho, u, v, w, T, alpha (FP64)
+  fp = std::fopen(job.data.c_str(), "wb");
+  if (!fp) return fail("cannot create", job.data);
+  std::setvbuf(fp, iobuf.data(), _IOFBF, iobuf.size());
+  r = RecWriter{fp, maxsub};
+  r.begin(4);
+  r.put(&nb, 4);
+  r.end();
+  dims.push_back(6);
+  dims.push_back(0);
+  r.begin((int64_t)dims.size() * 4);
+  r.put(dims.data(), dims.size() * 4);
+  r.end();
+  const uclbs_plot3d_header& h = job.h;
+  const double hd1[7] = {h.mach, 0.0, h.reynolds, h.time, h.gamma, 0.0, h.t_ref};
+  const double hd2[7] = {h.stag_temperature, h.cv, h.cp, h.gas_r, h.gas_r, h.mach, h.time};
+  const int32_t zero = 0, ts = h.timestep;
+  for (const P3Block& b : job.blocks) {
+    r.begin(120);
+    r.put(hd1, 56);
+    r.put(&zero, 4);
+    r.put(hd2, 56);
+    r.put(&ts, 4);
+    r.end();
+    r.begin((int64_t)b.nv * 48);
+    r.put(c->out_host + b.dbl_off, b.nv * 48);
+    r.end();
+  }
+  const bool dok = r.ok && std::fclose(fp) == 0;
+  if (!dok) return fail("write failed:", job.data);
+}
+
+int join_output(Ctx* c) {
+  if (c->out_thread.joinable()) c->out_thread.join();
+  const int rc = c->out_rc;
+  c->out_rc = 0;
+  if (rc) return set_err(rc, "%s", c->out_msg.c_str());
+  return UCLBS_OK;
+}
+
+void free_mics(Ctx* c) {
+  for (MicLevel& ml : c->mic_levels)
+    if (ml.d_nodes) (void)hipFree(ml.d_nodes);
+  c->mic_levels.clear();
+  c->mics.clear();
+  if (c->mic_ring) (void)hipFree(c->mic_ring);
+  c->mic_ring = nullptr;
+  c->mic_stride = 0;
+  c->mic_cap = c->mic_count = 0;
+  c->mic_structure = -1;
+}
+
+// Fortran F12.5 editing: 12 asterisks when the value does not fit
+void f12_5(double v, std::string& out) {
+  char buf[48];
+  if (std::isnan(v)) std::snprintf(buf, sizeof buf, "%12s", "NaN");
+  else if (std::isinf(v)) std::snprintf(buf, sizeof buf, "%12s", v > 0 ? "Infinity" : "-Infinity");
+  else if (std::snprintf(buf, sizeof buf, "%12.5f", v) > 12)
+    std::snprintf(buf, sizeof buf, "************");
+  out += buf;
+}
+
+void release_output(Ctx* c) {
+  if (c->out_thread.joinable()) c->out_thread.join();
+  if (c->out_fields) (void)hipFree(c->out_fields);
+  if (c->out_dev) (void)hipFree(c->out_dev);
+  if (c->out_host) (void)hipHostFree(c->out_host);
+  if (c->out_ev) (void)hipEventDestroy(c->out_ev);
+  c->out_fields = nullptr;
+  c->out_dev = c->out_host = nullptr;
+  c->out_ev = nullptr;
+  c->out_fields_cap = c->out_cap = 0;
+  free_mics(c);
+}
+
+}  // namespace
+
+extern "C" {
+
+int uclbs_set_placement(void* ctx, const double problo[3], const double probhi[3]) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || !problo || !probhi) return set_err(UCLBS_E_ARG, "null argument");
+  for (int d = 0; d < 3; ++d)
+    if (!(probhi[d] > problo[d]) || !std::isfinite(problo[d]) || !std::isfinite(probhi[d]))
+      return set_err(UCLBS_E_ARG, "placement: probhi must exceed problo on axis %d", d);
+  for (int d = 0; d < 3; ++d) {
+    c->problo[d] = problo[d];
+    c->probhi[d] = probhi[d];
+  }
+  c->placed = true;
+  return UCLBS_OK;
+}
+
+int uclbs_output_wait(void* ctx) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c) return set_err(UCLBS_E_ARG, "null context");
+  return join_output(c);
+}
+
+int uclbs_plot3d_write(void* ctx, const char* dir, const uclbs_plot3d_header* h) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || !h) return set_err(UCLBS_E_ARG, "null argument");
+  if (h->timestep < 0 || h->timestep > 999999999)
+    return set_err(UCLBS_E_ARG, "plot3d: timestep %d does not fit the file names' nine digits",
+                   h->timestep);
+  if (h->max_level < 0) return set_err(UCLBS_E_ARG, "plot3d: max_level below 0");
+  if (h->max_subrecord < 0 || h->max_subrecord > 2147483639 ||
+      (h->max_subrecord > 0 && h->max_subrecord < 4))
+    return set_err(UCLBS_E_ARG, "plot3d: max_subrecord outside [4, 2147483639] (0: default)");
+  Level* L0;
+  int rc = check_level(c, 0, &L0);
+  if (rc) return rc;
+  if ((rc = join_output(c))) return rc;  // the previous files are complete
+  if ((rc = set_device(c))) return rc;
+  const bool d39 = c->cfg.lattice == UCLBS_LATTICE_D3Q39;
+  const int fp32 = c->cfg.precision == UCLBS_PREC_FP32_SHIFTED;
+  int top = 0;
+  while (top < h->max_level && top + 1 < (int)c->levels.size() && c->levels[top + 1].defined) ++top;
+  // blocks in level order, a level's boxes in their order (plot3d_output.f95:101-131)
+  P3Job job;
+  job.c = c;
+  job.h = *h;
+  size_t dbl = 0, ints = 0, max_n = 0;
+  for (int l = 0; l <= top; ++l) {
+    const Level& L = c->levels[l];
+    max_n = std::max(max_n, (size_t)L.box.n);
+    double plo[3], dx[3];
+    int dlo[3];
+    level_geom(c, l, plo, dx, dlo);
+    const int nbox = L.multibox ? (int)L.hbox.size() : 1;
+    for (int b = 0; b < nbox; ++b) {
+      P3Block k;
+      k.lvl = l;
+      k.nv = 1;
+      for (int d = 0; d < 3; ++d) {
+        const int lo = L.multibox ? L.blo[3 * b + d] : L.lo[d];
+        const int hi = L.multibox ? L.bhi[3 * b + d] : L.hi[d];
+        k.dims[d] = hi - lo + 1;
+        k.nv *= (size_t)k.dims[d];
+        k.dx[d] = dx[d];
+        k.start[d] = plo[d] + ((double)lo - (double)dlo[d]) * dx[d];  // get_real_coords
+      }
+      k.dbl_off = dbl;
+      dbl += k.nv * 48;
+      job.blocks.push_back(k);
+      ints += k.nv * 4;
+    }
+  }
+  {
+    size_t io = dbl;  // the IBLANK arrays follow every block's fields
+    for (P3Block& k : job.blocks) {
+      k.int_off = io;
+      io += k.nv * 4;
+    }
+  }
+  const size_t bytes = dbl + ints;
+  if (max_n * 6 > c->out_fields_cap) {
+    if (c->out_fields) (void)hipFree(c->out_fields);
+    c->out_fields = nullptr;
+    c->out_fields_cap = 0;
+    HIP_TRY(hipMalloc(&c->out_fields, max_n * 6 * sizeof(double)));
+    c->out_fields_cap = max_n * 6;
+  }
+  if (bytes > c->out_cap) {
+    if (c->out_dev) (void)hipFree(c->out_dev);
+    if (c->out_host) (void)hipHostFree(c->out_host);
+    c->out_dev = c->out_host = nullptr;
+    c->out_cap = 0;
+    HIP_TRY(hipMalloc(&c->out_dev, bytes));
+    HIP_TRY(hipHostMalloc((void**)&c->out_host, bytes, hipHostMallocDefault));
+    c->out_cap = bytes;
+  }
+  if (!c->out_ev) HIP_TRY(hipEventCreateWithFlags(&c->out_ev, hipEventDisableTiming));
+  size_t bi = 0;
+  for (int l = 0; l <= top; ++l) {
+    Level& L = c->levels[l];
+    const int64_t n = L.box.n;
+    double* of = c->out_fields;
+    // rho, u, v, w, T, alpha of every stored node (calculate_flow_variables.f95:
+    // 201-221; alpha 2 where the entropic estimate is not kept, its value where
+    // the reference does not compute it)
+    if (d39) {
+      HIP_TRY(uclbs::launch_d3q39_macro(L.box, L.f[L.cur], L.g[L.cur], fp32, c->shifted39,
+                                        c->ref39, L.gas39.cv, of, c->stream));
+    } else {
+      HIP_TRY(uclbs::launch_d3q27_macro(L.box, L.f[L.cur], fp32, of, of + n, of + 2 * n,
+                                        of + 3 * n, c->stream));
+      HIP_TRY(uclbs::launch_fill(of + 4 * n, n, 1.0 / 3.0, c->stream));
+    }
+    if (c->keep_alpha)
+      HIP_TRY(hipMemcpyAsync(of + 5 * n, L.alpha, (size_t)n * sizeof(double),
+                             hipMemcpyDeviceToDevice, c->stream));
+    else
+      HIP_TRY(uclbs::launch_fill(of + 5 * n, n, 2.0, c->stream));
+    const int nbox = L.multibox ? (int)L.hbox.size() : 1;
+    for (int b = 0; b < nbox; ++b, ++bi) {
+      const P3Block& k = job.blocks[bi];
+      uclbs::Box vb{};
+      vb.nx = k.dims[0];
+      vb.ny = k.dims[1];
+      vb.nz = k.dims[2];
+      vb.n = (int64_t)k.nv;
+      const int64_t base = L.multibox ? (int64_t)L.hbox[b].off : 0;
+      const int ghost = L.multibox ? 1 : L.lo[0] - L.origin[0];
+      HIP_TRY(uclbs::launch_fab_gather(of + base, ghost, n, vb, 6,
+                                       reinterpret_cast<double*>(c->out_dev + k.dbl_off), c->stream));
+      HIP_TRY(uclbs::launch_blank_gather(L.state ? L.state + base : nullptr, ghost, vb,
+                                         reinterpret_cast<int32_t*>(c->out_dev + k.int_off),
+                                         c->stream));
+    }
+  }
+  HIP_TRY(hipMemcpyAsync(c->out_host, c->out_dev, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipEventRecord(c->out_ev, c->stream));
+  char num[16];
+  std::snprintf(num, sizeof num, "%09d", h->timestep);
+  const std::string pre = dir && *dir ? std::string(dir) + "/" : std::string();
+  job.grid = pre + "uclbs_grid_" + num + ".x";
+  job.data = pre + "uclbs_data_" + num + ".q";
+  c->out_rc = 0;
+  c->out_msg.clear();
+  try {
+    c->out_thread = std::thread(write_plot3d, std::move(job));
+  } catch (...) {
+    (void)hipEventSynchronize(c->out_ev);
+    return set_err(UCLBS_E_STATE, "plot3d: cannot start the writer thread");
+  }
+  return UCLBS_OK;
+}
+
+int uclbs_mic_setup(void* ctx, const char* dir, int n, const double* loc, const int32_t* type,
+                    const int32_t* level, int32_t* valid) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c) return set_err(UCLBS_E_ARG, "null context");
+  if (n < 0 || n > 99999) return set_err(UCLBS_E_ARG, "mic_setup: %d microphones (0..99999)", n);
+  if (n && (!loc || !type || !level)) return set_err(UCLBS_E_ARG, "null argument");
+  Level* L0;
+  int rc = check_level(c, 0, &L0);
+  if (rc) return rc;
+  if ((rc = set_device(c))) return rc;
+  for (int i = 0; i < n; ++i) {
+    if (type[i] < 0 || type[i] > 63) return set_err(UCLBS_E_ARG, "mic %d: type %d outside 0..63", i + 1, type[i]);
+    if (level[i] < 0) return set_err(UCLBS_E_ARG, "mic %d: level below 0", i + 1);
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  free_mics(c);
+  int fine = 0;
+  while (fine + 1 < (int)c->levels.size() && c->levels[fine + 1].defined) ++fine;
+  double glo[3], ghi[3];
+  {
+    double dx0[3];
+    int dlo0[3];
+    level_geom(c, 0, glo, dx0, dlo0);
+    for (int d = 0; d < 3; ++d)
+      ghi[d] = c->placed ? c->probhi[d] : (double)(L0->hi[d] - L0->lo[d] + 1);
+  }
+  static const int kCorner[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1},
+                                    {1, 1, 0}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+  const std::string pre = dir && *dir ? std::string(dir) + "/" : std::string();
+  std::vector<std::vector<uint32_t>> lists(fine + 1);
+  for (int i = 0; i < n; ++i) {
+    Mic m;
+    m.number = i + 1;
+    m.mask = type[i];
+    const double* p = loc + 3 * i;
+    bool in = true;
+    for (int d = 0; d < 3; ++d) in = in && !(p[d] > ghi[d] || p[d] < glo[d]);
+    if (!in) {  // mic_setup.f95:45-53
+      c->mics.push_back(m);
+      continue;
+    }
+    const int l = std::min(level[i], fine);
+    m.level = l;
+    const Level& L = c->levels[l];
+    double plo[3], dx[3];
+    int dlo[3], st[3];
+    level_geom(c, l, plo, dx, dlo);
+    const double rdx = dx[0];  // the level's rdx (x spacing) on every axis
+    double lo[3];
+    for (int d = 0; d < 3; ++d) {
+      st[d] = dlo[d] + (int)std::floor((p[d] - plo[d]) / rdx);
+      lo[d] = plo[d] + ((double)st[d] - (double)dlo[d]) * dx[d];
+    }
+    bool owned = true;
+    for (int d = 0; d < 3; ++d) owned = owned && st[d] >= L.lo[d] && st[d] <= L.hi[d];
+    owned = owned && valid_index(L, st) >= 0;
+    if (!owned) {
+      // outside every box of the level: a valid microphone that records nothing
+      m.valid = 2;
+      c->mics.push_back(m);
+      continue;
+    }
+    double dist[8] = {}, total = 0.0;
+    int bad = 0;
+    for (int j = 0; j < 8; ++j) {
+      int G[3];
+      for (int d = 0; d < 3; ++d) G[d] = m.pos[j][d] = st[d] + kCorner[j][d];
+      int64_t node = valid_index(L, G);
+      if (node >= 0 && L.state) {  // a link table holds the node codes
+        int32_t code = 0;
+        HIP_TRY(hipMemcpy(&code, L.state + node, sizeof code, hipMemcpyDeviceToHost));
+        if (code < 0) node = -1;
+      }
+      m.node[j] = node;
+      if (node < 0) {
+        ++bad;
+        continue;
+      }
+      // distance with the reference's + rdx for the far corners
+      // (mic_setup.f95:86-198): the mirror image of the corner about the cell's
+      // first node
+      double s = 0.0;
+      for (int d = 0; d < 3; ++d) {
+        const double t = kCorner[j][d] ? p[d] - lo[d] + rdx : p[d] - lo[d];
+        s += t * t;
+      }
+      dist[j] = std::sqrt(s);
+      total += dist[j];
+    }
+    if (bad == 8) {  // mic_setup.f95:200-205
+      c->mics.push_back(m);
+      continue;
+    }
+    m.valid = 1;
+    for (int j = 0; j < 8; ++j)  // mic_setup.f95:207-213: corners at x index <= 0 weigh 0
+      m.w[j] = (m.node[j] >= 0 && m.pos[j][0] - dlo[0] > 0) ? dist[j] / total : 0.0;
+    int64_t first = -1;
+    for (int j = 0; j < 8 && first < 0; ++j) first = m.node[j];
+    m.slot = (unsigned)lists[l].size();
+    for (int j = 0; j < 8; ++j) lists[l].push_back((uint32_t)(m.node[j] >= 0 ? m.node[j] : first));
+    c->mics.push_back(m);
+  }
+  // files with their header lines (mic_initialize.f95:17-70)
+  static const char* kHead[6] = {"P,          ;", "Temp,       ;", "Density,    ;",
+                                 "U-vel,      ;", "V-vel,      ;", "W-vel,      ;"};
+  for (Mic& m : c->mics) {
+    if (!m.valid) continue;
+    char name[32];
+    std::snprintf(name, sizeof name, "microphone%05d.txt", m.number);
+    m.path = pre + name;
+    std::string line = " ";
+    for (int k = 0; k < 6; ++k)
+      if (m.mask & (32 >> k)) line += kHead[k];
+    line += "\n";
+    std::FILE* fp = std::fopen(m.path.c_str(), "w");
+    if (!fp || std::fputs(line.c_str(), fp) < 0 || std::fclose(fp) != 0) {
+      std::string path = m.path;
+      free_mics(c);
+      return set_err(UCLBS_E_ARG, "mic_setup: cannot write %s", path.c_str());
+    }
+  }
+  size_t off = 0;
+  for (int l = 0; l <= fine; ++l) {
+    if (lists[l].empty()) continue;
+    MicLevel ml;
+    ml.lvl = l;
+    ml.m = (unsigned)lists[l].size();
+    ml.off = off;
+    off += 6 * (size_t)ml.m;
+    hipError_t e = hipMalloc(&ml.d_nodes, lists[l].size() * sizeof(uint32_t));
+    if (e == hipSuccess)
+      e = hipMemcpy(ml.d_nodes, lists[l].data(), lists[l].size() * sizeof(uint32_t),
+                    hipMemcpyHostToDevice);
+    c->mic_levels.push_back(ml);
+    if (e != hipSuccess) {
+      free_mics(c);
+      return set_err(UCLBS_E_HIP, "mic_setup: %s", hipGetErrorString(e));
+    }
+  }
+  c->mic_stride = off;
+  if (off) {
+    // a ring of samples of at most 64 MiB (4096 samples at most)
+    c->mic_cap = (int)std::max<size_t>(1, std::min<size_t>(4096, (64u << 20) / (off * sizeof(double))));
+    hipError_t e = hipMalloc(&c->mic_ring, (size_t)c->mic_cap * off * sizeof(double));
+    if (e != hipSuccess) {
+      free_mics(c);
+      return set_err(UCLBS_E_NOMEM, "mic_setup: %s", hipGetErrorString(e));
+    }
+  }
+  c->mic_structure = c->structure;
+  if (valid)
+    for (int i = 0; i < n; ++i) valid[i] = c->mics[i].valid;
+  return UCLBS_OK;
+}
+
+int uclbs_mic_weights(void* ctx, int i, int32_t pos[24], double w[8], int64_t node[8]) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c) return set_err(UCLBS_E_ARG, "null context");
+  if (i < 0 || i >= (int)c->mics.size()) return set_err(UCLBS_E_ARG, "mic %d was not set up", i + 1);
+  const Mic& m = c->mics[i];
+  for (int j = 0; j < 8; ++j) {
+    if (pos)
+      for (int d = 0; d < 3; ++d) pos[3 * j + d] = m.pos[j][d];
+    if (w) w[j] = m.w[j];
+    if (node) node[j] = m.node[j];
+  }
+  return UCLBS_OK;
+}
+
+int uclbs_mic_flush(void* ctx) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c) return set_err(UCLBS_E_ARG, "null context");
+  if (c->mic_count == 0) return UCLBS_OK;
+  int rc = set_device(c);
+  if (rc) return rc;
+  const size_t cnt = (size_t)c->mic_count;
+  std::vector<double> h(cnt * c->mic_stride);
+  HIP_TRY(hipMemcpyAsync(h.data(), c->mic_ring, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                         c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  c->mic_count = 0;
+  // probe columns rho, u, v, w, T, p; the file's order is p, T, rho, u, v, w
+  static const int kCol[6] = {5, 4, 0, 1, 2, 3};
+  std::string text;
+  for (const Mic& m : c->mics) {
+    if (m.valid != 1 || !m.mask) continue;
+    const MicLevel* ml = nullptr;
+    for (const MicLevel& x : c->mic_levels)
+      if (x.lvl == m.level) ml = &x;
+    text.clear();
+    for (size_t s = 0; s < cnt; ++s) {
+      const double* smp = h.data() + s * c->mic_stride + ml->off;
+      bool firstcol = true;
+      for (int k = 0; k < 6; ++k) {
+        if (!(m.mask & (32 >> k))) continue;
+        // the weighted sum over the eight corners in order (mic_write.f95:45-105)
+        double v = 0.0;
+        for (int j = 0; j < 8; ++j)
+          if (m.node[j] >= 0) v = v + m.w[j] * smp[(size_t)kCol[k] * ml->m + m.slot + j];
+        if (!firstcol) text += ' ';
+        f12_5(v, text);
+        firstcol = false;
+      }
+      text += '\n';
+    }
+    std::FILE* fp = std::fopen(m.path.c_str(), "a");
+    if (!fp || std::fputs(text.c_str(), fp) < 0 || std::fclose(fp) != 0)
+      return set_err(UCLBS_E_ARG, "mic_flush: cannot append to %s", m.path.c_str());
+  }
+  return UCLBS_OK;
+}
+
+int uclbs_mic_sample(void* ctx) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c) return set_err(UCLBS_E_ARG, "null context");
+  if (c->mic_stride == 0) return UCLBS_OK;
+  if (c->mic_structure != c->structure)
+    return set_err(UCLBS_E_STATE, "mic_sample: the levels or link tables changed after "
+                                  "uclbs_mic_setup (set the microphones up again)");
+  int rc;
+  if (c->mic_count == c->mic_cap && (rc = uclbs_mic_flush(ctx))) return rc;
+  if ((rc = set_device(c))) return rc;
+  const int fp32 = c->cfg.precision == UCLBS_PREC_FP32_SHIFTED;
+  double* base = c->mic_ring + (size_t)c->mic_count * c->mic_stride;
+  for (const MicLevel& ml : c->mic_levels) {
+    const Level& L = c->levels[ml.lvl];
+    if (c->cfg.lattice == UCLBS_LATTICE_D3Q39)
+      HIP_TRY(uclbs::launch_d3q39_probe(L.f[L.cur], L.g[L.cur], L.box.n, fp32, c->shifted39,
+                                        c->ref39, L.gas39.cv, ml.d_nodes, ml.m, base + ml.off,
+                                        c->stream));
+    else
+      HIP_TRY(uclbs::launch_d3q27_probe(L.f[L.cur], L.box.n, fp32, ml.d_nodes, ml.m,
+                                        base + ml.off, c->stream));
+  }
+  c->mic_count++;
   return UCLBS_OK;
 }
 

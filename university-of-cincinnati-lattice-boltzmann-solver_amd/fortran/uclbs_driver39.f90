@@ -4,8 +4,10 @@
 ! inviscid y and z sides) around a perturbed freestream.  Writes the state
 ! after initialisation and after the steps (f, g, multipliers, FP64, the
 ! library's host layout) to the file named on the command line, then the step
-! statistics on stdout.
-!   uclbs_driver39 <out.bin> [nx ny nz] [steps] [precision 0|1]
+! statistics on stdout.  With an output directory it also writes the final
+! state as Plot3D files (domain placed at [0, n dx], dx = 1e-3) and one
+! microphone's sample, as the reference's lb_output would (lb_output.f95:44,71).
+!   uclbs_driver39 <out.bin> [nx ny nz] [steps] [precision 0|1] [output directory]
 program uclbs_driver39
   use iso_c_binding
   use uclbs_hip
@@ -14,6 +16,10 @@ program uclbs_driver39
   type(uclbs_config) :: cfg
   type(uclbs_gas) :: gas
   type(uclbs_stats) :: st
+  type(uclbs_plot3d_header) :: hdr
+  real(c_double) :: mloc(3, 1)
+  integer(c_int32_t) :: mtype(1), mlvl(1), mvalid(1)
+  character(len=256) :: outdir
   type(c_ptr) :: ctx
   integer(c_int32_t) :: lo(3), hi(3), per(3), faces(6)
   integer(c_int64_t) :: timestep(0:0), nn
@@ -85,6 +91,24 @@ program uclbs_driver39
   call uclbs_check(uclbs_download_state(ctx, 0_c_int, f, g, lm, nn), 'uclbs_download_state')
   write(unit_out) f, g, lm
   close(unit_out)
+  if (command_argument_count() >= 7) then
+    call get_command_argument(7, outdir)
+    call uclbs_check(uclbs_set_placement(ctx, (/0.0_c_double, 0.0_c_double, 0.0_c_double/), &
+                     real(n, c_double) * 1.0e-3_c_double), 'uclbs_set_placement')
+    hdr%mach = 1.5_c_double
+    hdr%gamma = 1.4_c_double
+    hdr%time = time(0)
+    hdr%timestep = int(timestep(0), c_int32_t)
+    call uclbs_check(uclbs_plot3d_write(ctx, trim(outdir)//c_null_char, hdr), 'uclbs_plot3d_write')
+    mloc(:, 1) = (/5.5e-3_c_double, 4.25e-3_c_double, 3.1e-3_c_double/)
+    mtype = 63
+    mlvl = 0
+    call uclbs_check(uclbs_mic_setup(ctx, trim(outdir)//c_null_char, 1_c_int, mloc, mtype, mlvl, &
+                     mvalid), 'uclbs_mic_setup')
+    call uclbs_check(uclbs_mic_sample(ctx), 'uclbs_mic_sample')
+    call uclbs_check(uclbs_mic_flush(ctx), 'uclbs_mic_flush')
+    call uclbs_check(uclbs_output_wait(ctx), 'uclbs_output_wait')
+  end if
   call uclbs_check(uclbs_get_stats(ctx, st), 'uclbs_get_stats')
   write(*, '(a,i0,a,i0,a,i0,a,i0)') 'steps ', timestep(0), ' node_updates ', st%node_updates, &
       ' newton_iters ', st%newton_iters, ' noconv ', st%n_noconv

@@ -42,6 +42,16 @@ module uclbs_hip
                           n_noconv, n_fallback, n_logpath, n_ibb_links, n_repair
   end type
 
+  ! Plot3D header values (include/uclbs_hip.h uclbs_plot3d_header)
+  type, bind(c), public :: uclbs_plot3d_header
+    real(c_double) :: mach = 0, reynolds = 0, time = 0, gamma = 0, t_ref = 0, &
+                      stag_temperature = 0, cv = 0, cp = 0, gas_r = 0
+    integer(c_int32_t) :: timestep = 0, max_level = 31
+    integer(c_int64_t) :: max_subrecord = 0
+  end type
+
+  public :: uclbs_set_placement, uclbs_plot3d_write, uclbs_output_wait, uclbs_mic_setup, &
+            uclbs_mic_sample, uclbs_mic_flush
   public :: uclbs_create, uclbs_destroy, uclbs_define_level, uclbs_upload_f, &
             uclbs_download_f, uclbs_download_macro, uclbs_set_keep_alpha, &
             uclbs_download_alpha, uclbs_init_tgv, uclbs_init_uniform, uclbs_set_faces, &
@@ -304,6 +314,46 @@ module uclbs_hip
       import :: c_ptr, c_size_t
       type(c_ptr), value :: p
       integer(c_size_t) :: n
+    end function
+    ! output: the plot3d_output / mic_setup / mic_write calls of lb_output.f95:44,71
+    function uclbs_set_placement(ctx, problo, probhi) bind(c, name='uclbs_set_placement') result(rc)
+      import :: c_ptr, c_int, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: problo(3), probhi(3)
+      integer(c_int) :: rc
+    end function
+    function uclbs_plot3d_write(ctx, dir, h) bind(c, name='uclbs_plot3d_write') result(rc)
+      import :: c_ptr, c_int, c_char, uclbs_plot3d_header
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: dir(*)
+      type(uclbs_plot3d_header), intent(in) :: h
+      integer(c_int) :: rc
+    end function
+    function uclbs_output_wait(ctx) bind(c, name='uclbs_output_wait') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+    function uclbs_mic_setup(ctx, dir, n, loc, mtype, lvl, valid) &
+        bind(c, name='uclbs_mic_setup') result(rc)
+      import :: c_ptr, c_int, c_char, c_double, c_int32_t
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: dir(*)
+      integer(c_int), value :: n
+      real(c_double), intent(in) :: loc(3, *)
+      integer(c_int32_t), intent(in) :: mtype(*), lvl(*)
+      integer(c_int32_t), intent(out) :: valid(*)
+      integer(c_int) :: rc
+    end function
+    function uclbs_mic_sample(ctx) bind(c, name='uclbs_mic_sample') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+    function uclbs_mic_flush(ctx) bind(c, name='uclbs_mic_flush') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int) :: rc
     end function
   end interface
 
