@@ -149,8 +149,11 @@ int uclbs_set_bc_states(void *ctx, int n_in, const double *in_states, int n_out,
  * On a D3Q39 context of the plain lattice the table has 39 components, with
  * the long links derived from the shorter ones as state_directions.f95:95-185
  * does (integer divisions and its slot slip for the (0,3,0) link included);
- * an outflow face gives its links the outside node's code 0.  Such a level is
- * not stepped (the plain-lattice D3Q39 wall rule is not in this library).
+ * an outflow face gives its links the outside node's code 0.  Stepping
+ * rebuilds the population opposite each wall link by the plain-lattice wall
+ * rule (target_finder.f95:36-88, interpolative_bounceback_bc.f95:834-2838
+ * with derivatives.f95:237-583: the Hermite projection its 38 cases expand
+ * plus each case's departures from it).
  * On a D3Q39 context of the shifted lattice the table is that of
  * lbgrid/shifted_state_directions.f95: link i of node x is keyed on its
  * upstream node x - c_i (a wall code where that node is solid, the face code

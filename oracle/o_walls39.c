@@ -372,6 +372,264 @@ int o39_wall_rebuild(int nx, int ny, int nz, const int32_t fc[6], const int32_t 
   return nrep;
 }
 
+/* ---- plain lattice (Mach < 1.2) ----------------------------------------------
+ *
+ * A fluid node x whose link j (x -> x + c_j) carries a wall code s <= -1001
+ * has the population of the opposite direction i = opp(j), which the pull
+ * would read from the solid node x + c_j, rebuilt
+ * (streaming.f95:478-1133 with the plain branch, which calls
+ * interpolative_bounceback_bc with direction i and stores the result in
+ * fi(x, i)):
+ *   target (target_finder.f95:36-88), when a link of speed <= sqrt(3) is a
+ *     wall link: over the wall links j with q_j = |s_j + 1001| 1e-9,
+ *       u_t = mean of q_j u_n / (q_j + 1), T_t = mean of (q_j T_n + T_n) /
+ *       (q_j + 1), n the node x + c_opp(j) when the opposite link's code is
+ *       >= 0 (the node's own values where that node lies beyond a face);
+ *       otherwise u_t += q_j u(x) / (q_j + 1), T_t += (q_j T(x) + T_inf) /
+ *       (q_j + 1), T_inf = 1 / gamma (the freestream temperature);
+ *       rho_t = sum over directions of f_opp(i)(x) where link opp(i) is a
+ *       wall link, else f_i(x) (this step's pulled values);
+ *     target multipliers by Newton from the node's multipliers (the plain
+ *     branch has no start-up guesses);
+ *   otherwise the node's own rho, u, T and multipliers;
+ *   derivatives (interpolative_bounceback_bc.f95:834-2838 with
+ *     derivatives.f95:237-583): along axis a with c_i,a > 0 the wall form
+ *     (u_t - 0) / (1 + w), c_i,a < 0: (0 - u_t) / (1 + w), with
+ *     w = |s_i + 1001| 1e-9 |c_i| from the node's own code of link i (the
+ *     direction of the rebuilt population, as the reference reads it);
+ *     temperature derivatives of the wall form are 0; c_i,a = 0: the local
+ *     difference of u(x - e_a), u(x), u(x + e_a) over those whose link code
+ *     from x is >= 0 (the stencil values: a fluid node's own, 0 and T_inf
+ *     for a node that is not fluid or lies beyond a face);
+ *   rebuilt populations: f_i = feq_i(target) + eps k_f F_i,
+ *     g_i = (2 c_v - 3) T_t feq_i + eps k_g G_i, k_f = -rho_t T_t / 2,
+ *     k_g = -rho_t T_t (2 c_v - 3) / 2, where F_i, G_i are the Hermite
+ *     projection the reference's 38 cases expand plus each case's slip
+ *     terms (a table: o39p_set_slips); no magnitude repair (the plain branch
+ *     has none; damp_it repairs negative populations after the collision).
+ */
+static const double *g_slip = NULL;
+static const int *g_slip_start = NULL;
+
+void o39p_set_slips(const double *slip, const int *start) {
+  g_slip = slip;
+  g_slip_start = start;
+}
+
+static double ipow(double x, int e) {
+  double r = 1.0;
+  for (int k = 0; k < (e < 0 ? -e : e); k++) r *= x;
+  return e < 0 ? 1.0 / r : r;
+}
+
+/* F_i and G_i: the Hermite projection of direction i (1..38) plus its slips */
+void o39p_wall_neq(int i, double T, const double ut[3], double du[3][3], const double dT[3],
+                   double b1, double b2, double cv, double *F, double *G) {
+  const int c[3] = {O39_CX[i], O39_CY[i], O39_CZ[i]};
+  const double div = du[0][0] + du[1][1] + du[2][2];
+  double h2p = 0.0, h2r = 0.0, h3q = 0.0;
+  for (int a = 0; a < 3; a++)
+    for (int b = 0; b < 3; b++) {
+      const double h2 = (double)(c[a] * c[b]) - (a == b ? T : 0.0);
+      double P, R;
+      if (a == b) {
+        P = 6.0 * du[a][a] - div / cv;
+        R = 6.0 * T * du[a][a] + 6.0 * ut[a] * dT[a] - (2.0 * T + ut[a] * ut[a]) * div / cv;
+      } else {
+        P = 3.0 * (du[a][b] + du[b][a]);
+        R = 3.0 * (T * (du[a][b] + du[b][a]) + ut[a] * dT[b] + ut[b] * dT[a]) -
+            div / cv * ut[a] * ut[b];
+      }
+      h2p += h2 * P;
+      h2r += h2 * R;
+      for (int e = 0; e < 3; e++) {
+        const double h3 = (double)(c[a] * c[b] * c[e]) -
+                          T * (double)(c[a] * (b == e) + c[b] * (a == e) + c[e] * (a == b));
+        if (h3 == 0.0) continue;
+        /* Q_abe: the three ways to split {a, b, e} into a pair and a single */
+        const int sp[3][3] = {{a, b, e}, {a, e, b}, {b, e, a}};
+        double q = 0.0;
+        for (int m = 0; m < 3; m++) {
+          const int p = sp[m][0], r = sp[m][1], s = sp[m][2];
+          const double pi = p == r ? 2.0 * du[p][p] - du[s][s] / cv : du[p][r] + du[r][p];
+          q += (p == r ? dT[s] / b2 : 0.0) + ut[s] * pi / b1;
+        }
+        h3q += h3 * q;
+      }
+    }
+  double f = h2p / (2.0 * b1 * T * T) + h3q / (6.0 * T * T * T);
+  double gg = 9.0 * div / (b1 * cv) + h2r / (2.0 * b1 * T * T);
+  for (int a = 0; a < 3; a++)
+    gg += (double)c[a] * (9.0 * dT[a] / b2 + 3.0 * ut[a] * div / (b1 * cv)) / T;
+  if (g_slip) {
+    const double D[12] = {du[0][0], du[1][0], du[2][0], du[0][1], du[1][1], du[2][1],
+                          du[0][2], du[1][2], du[2][2], dT[0], dT[1], dT[2]};
+    for (int part = 0; part < 2; part++) {
+      const int lo = g_slip_start[38 * part + i - 1], hi = g_slip_start[38 * part + i];
+      double add = 0.0;
+      for (int m = lo; m < hi; m++) {
+        const double *t = g_slip + 8 * (size_t)m;
+        add += t[0] * D[(int)t[1]] / ((int)t[2] == 1 ? b1 : b2) * ipow(ut[0], (int)t[3]) *
+               ipow(ut[1], (int)t[4]) * ipow(ut[2], (int)t[5]) * ipow(T, (int)t[6]) *
+               ipow(cv, (int)t[7]);
+      }
+      if (part == 0) f += add;
+      else gg += add;
+    }
+  }
+  *F = f;
+  *G = gg;
+}
+
+/* The rebuilt f and g of the plain-lattice wall links of fluid node k:
+ * fw[i], gw[i] for every direction i whose opposite link is a wall link.
+ * fi: this step's pulled populations of the node. */
+int o39p_wall_rebuild(int nx, int ny, int nz, const int32_t fc[6], const int32_t *state,
+                      const double *f, const double *g, const double *lm, int64_t k,
+                      const double fi[NQ], const o39_consts *c, int64_t step, double fw[NQ],
+                      double gw[NQ], int *noconv, int *kerr) {
+  const int64_t n = (int64_t)nx * ny * nz;
+  const int64_t a = k % nx, b = (k / nx) % ny, cc = k / ((int64_t)nx * ny);
+  const macro39 own = node_macro(f, g, n, k, c);
+  const double t_inf = 1.0 / c->gama;
+  int adjacent = 0;
+  for (int j = 1; j <= 14; j++) adjacent |= state[(int64_t)j * n + k] < -1000;
+  double rho_t, ut[3], Tt, lmt[NM], feq[NQ];
+  *noconv = 0;
+  *kerr = 0;
+  if (adjacent) {
+    int count = 0;
+    ut[0] = ut[1] = ut[2] = 0.0;
+    Tt = 0.0;
+    for (int j = 1; j < NQ; j++) {
+      const int32_t s = state[(int64_t)j * n + k];
+      if (s > -1001) continue;
+      const double q = fabs((double)s + 1001.0) / 1.0e9;
+      count++;
+      const int o = O39_OPP[j];
+      if (state[(int64_t)o * n + k] >= 0) {
+        macro39 nb = own;
+        const int64_t d = pos_index(a + O39_CX[o], b + O39_CY[o], cc + O39_CZ[o], nx, ny, nz, fc);
+        if (d >= 0) nb = node_macro(f, g, n, d, c);
+        for (int e = 0; e < 3; e++) ut[e] += (q * nb.u[e]) / (q + 1.0);
+        Tt += (q * nb.T + nb.T) / (q + 1.0);
+      } else {
+        for (int e = 0; e < 3; e++) ut[e] += (q * own.u[e]) / (q + 1.0);
+        Tt += (q * own.T + t_inf) / (q + 1.0);
+      }
+    }
+    for (int e = 0; e < 3; e++) ut[e] /= count;
+    Tt /= count;
+    rho_t = 0.0;
+    for (int i = 0; i < NQ; i++)
+      rho_t += state[(int64_t)O39_OPP[i] * n + k] < -1000 ? fi[O39_OPP[i]] : fi[i];
+    for (int m = 0; m < NM; m++) lmt[m] = lm[(int64_t)m * n + k];
+    double P[6] = {rho_t * Tt + rho_t * ut[0] * ut[0], rho_t * Tt + rho_t * ut[1] * ut[1],
+                   rho_t * Tt + rho_t * ut[2] * ut[2], rho_t * ut[0] * ut[1],
+                   rho_t * ut[0] * ut[2], rho_t * ut[1] * ut[2]};
+    double e[NQ];
+    if (o39_lm_solve(rho_t, ut, Tt, P, lmt, e, 0, 0, c->maxit) < 0) *noconv = 1;
+  } else {
+    rho_t = own.rho;
+    for (int e = 0; e < 3; e++) ut[e] = own.u[e];
+    Tt = own.T;
+    for (int m = 0; m < NM; m++) lmt[m] = lm[(int64_t)m * n + k];
+  }
+  o39_feq(lmt, rho_t, 0, feq);
+  /* 7-point stencil values x, x + e_a, x - e_a */
+  static const int ex[7][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1},
+                               {-1, 0, 0}, {0, -1, 0}, {0, 0, -1}};
+  double su[7][3], sT[7];
+  for (int m = 0; m < 7; m++) {
+    const int64_t d = pos_index(a + ex[m][0], b + ex[m][1], cc + ex[m][2], nx, ny, nz, fc);
+    if (d >= 0 && state[d] >= 0) {
+      const macro39 mm = m == 0 ? own : node_macro(f, g, n, d, c);
+      for (int e = 0; e < 3; e++) su[m][e] = mm.u[e];
+      sT[m] = mm.T;
+    } else {
+      su[m][0] = su[m][1] = su[m][2] = 0.0;
+      sT[m] = t_inf;
+    }
+  }
+  const double eps =
+      step < c->startup ? (((double)step + 1.0) / (double)c->startup) * c->eps : c->eps;
+  const double mu = o39_sutherland(Tt, c, NULL, NULL);
+  double kappa = 0.0;
+  if (o39_therm_cond(rho_t, Tt, c, &kappa)) *kerr = 1;
+  double b1, b2;
+  betas(mu, kappa, rho_t, Tt, step, c, &b1, &b2);
+  const double k_f = -(rho_t * Tt) / 2.0, k_g = -(rho_t * Tt * (2.0 * c->cv - 3.0)) / 2.0;
+  int nlinks = 0;
+  for (int i = 1; i < NQ; i++) {
+    if (state[(int64_t)O39_OPP[i] * n + k] > -1001) continue;
+    const int ci[3] = {O39_CX[i], O39_CY[i], O39_CZ[i]};
+    const double link = sqrt((double)(ci[0] * ci[0] + ci[1] * ci[1] + ci[2] * ci[2]));
+    const double wd = fabs((double)state[(int64_t)i * n + k] + 1001.0) / 1.0e9 * link;
+    double du[3][3], dT[3];
+    for (int ax = 0; ax < 3; ax++) {
+      const int up_ok = state[(int64_t)(4 + ax) * n + k] >= 0;
+      const int down_ok = state[(int64_t)(1 + ax) * n + k] >= 0;
+      for (int e = 0; e < 3; e++) {
+        if (ci[ax] > 0) du[ax][e] = (ut[e] - 0.0) / (1.0 + wd);
+        else if (ci[ax] < 0) du[ax][e] = (0.0 - ut[e]) / (1.0 + wd);
+        else du[ax][e] = d_local(su[0][e], su[4 + ax][e], su[1 + ax][e], up_ok, down_ok);
+      }
+      dT[ax] = ci[ax] != 0 ? 0.0 : d_local(sT[0], sT[4 + ax], sT[1 + ax], up_ok, down_ok);
+    }
+    double F, G;
+    o39p_wall_neq(i, Tt, ut, du, dT, b1, b2, c->cv, &F, &G);
+    fw[i] = feq[i] + eps * k_f * F;
+    gw[i] = c->lead_g * Tt * feq[i] + eps * k_g * G;
+    nlinks++;
+  }
+  return nlinks;
+}
+
+/* damp_it.f95:64-123 with a plain-lattice link table: a negative
+ * post-collision population takes the mean of its positive neighbours along
+ * +-c_i where the node's link codes of i / opp(i) are >= 0 (the rest
+ * direction: the node's code), the mean of one with |f|, or |f| */
+static int64_t damp_walls_plain(int nx, int ny, int nz, const int32_t fc[6], const int32_t *state,
+                                double *fo, int jacobi) {
+  const int64_t n = (int64_t)nx * ny * nz;
+  double *src = fo;
+  if (jacobi) {
+    src = (double *)malloc(sizeof(double) * (size_t)(n * NQ));
+    memcpy(src, fo, sizeof(double) * (size_t)(n * NQ));
+  }
+  int64_t nrep = 0;
+  for (int i = 0; i < NQ; i++) {
+    for (int64_t k = 0; k < n; k++) {
+      const double v = src[(int64_t)i * n + k];
+      if (!(v < 0.0) || state[k] < 0) continue;
+      const int64_t a = k % nx, b = (k / nx) % ny, cc = k / ((int64_t)nx * ny);
+      const int64_t kp = pos_index(a + O39_CX[i], b + O39_CY[i], cc + O39_CZ[i], nx, ny, nz, fc);
+      const int64_t km = pos_index(a - O39_CX[i], b - O39_CY[i], cc - O39_CZ[i], nx, ny, nz, fc);
+      const int pok = kp >= 0 && state[(int64_t)i * n + k] >= 0;
+      const int mok = km >= 0 && state[(int64_t)O39_OPP[i] * n + k] >= 0;
+      const double fp = pok ? src[(int64_t)i * n + kp] : -1.0;
+      const double fm = mok ? src[(int64_t)i * n + km] : -1.0;
+      double r;
+      if (pok && mok) {
+        if (fp > 0.0 && fm > 0.0) r = (fp + fm) / 2.0;
+        else if (fp > 0.0 && fm < 0.0) r = (fp + fabs(v)) / 2;
+        else if (fp < 0.0 && fm > 0.0) r = (fm + fabs(v)) / 2;
+        else r = fabs(v);
+      } else if (pok) {
+        r = fp > 0.0 ? (fp + fabs(v)) / 2 : fabs(v);
+      } else if (mok) {
+        r = fm > 0.0 ? (fm + fabs(v)) / 2 : fabs(v);
+      } else {
+        r = fabs(v);
+      }
+      fo[(int64_t)i * n + k] = r;
+      nrep++;
+    }
+  }
+  if (jacobi) free(src);
+  return nrep;
+}
+
 /* damp_it.f95:125-177 with the link table (shifted lattice) */
 static int64_t damp_walls(int nx, int ny, int nz, const int32_t fc[6], const int32_t *state,
                           double *fo, int jacobi) {
@@ -464,7 +722,7 @@ void o39_step_walls(int nx, int ny, int nz, const int32_t fc[6], const int32_t *
         }
         if (i > 0 && state[(int64_t)i * n + k] <= -1001) any_wall = 1;
       }
-      if (any_wall) {
+      if (any_wall && c->shifted) {
         double fw[NQ], gw[NQ];
         int nc = 0, ke = 0;
         nwr += o39_wall_rebuild(nx, ny, nz, fc, state, f, g, lm, k, fi, c, step, lvl_step, fw, gw,
@@ -476,6 +734,19 @@ void o39_step_walls(int nx, int ny, int nz, const int32_t fc[6], const int32_t *
             fi[i] = fw[i];
             gi[i] = gw[i];
             nwl++;
+          }
+      } else if (any_wall) {
+        /* plain lattice: direction i is rebuilt where link opp(i) is a wall link */
+        double fw[NQ], gw[NQ];
+        int nc = 0, ke = 0;
+        nwl += o39p_wall_rebuild(nx, ny, nz, fc, state, f, g, lm, k, fi, c, step, fw, gw, &nc,
+                                 &ke);
+        noconv += nc;
+        kerr += ke;
+        for (int i = 1; i < NQ; i++)
+          if (state[(int64_t)O39_OPP[i] * n + k] <= -1001) {
+            fi[i] = fw[i];
+            gi[i] = gw[i];
           }
       }
       o39_moments(fi, gi, c->shifted, &rho, u, &T, P, c->cv);
@@ -512,7 +783,8 @@ void o39_step_walls(int nx, int ny, int nz, const int32_t fc[6], const int32_t *
     st->n_alpha_hi += ahi;
     st->n_nan += nan;
     st->n_kappa_err += kerr;
-    st->n_repair += damp_walls(nx, ny, nz, fc, state, fo, jacobi);
+    st->n_repair += c->shifted ? damp_walls(nx, ny, nz, fc, state, fo, jacobi)
+                               : damp_walls_plain(nx, ny, nz, fc, state, fo, jacobi);
     memcpy(f, fo, sizeof(double) * (size_t)(n * NQ));
     memcpy(g, go, sizeof(double) * (size_t)(n * NQ));
   }

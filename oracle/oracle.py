@@ -156,6 +156,8 @@ def lib() -> ctypes.CDLL:
                                        ctypes.c_int64, ctypes.c_int64, dp, dp,
                                        ctypes.POINTER(I), ctypes.POINTER(I)]
         L.o39_wall_rebuild.restype = I
+        L.o39p_set_slips.argtypes = [dp, ctypes.POINTER(I)]
+        L.o39p_wall_neq.argtypes = [I, D, dp, dp, dp, D, D, D, dp, dp]
         for nm in ("o_tri_volume", "o_box_test"):
             getattr(L, nm).argtypes = [dp, dp, dp]
         L.o_link_code.argtypes = [dp, dp, D]
@@ -633,11 +635,57 @@ def step_faces39(shape, faces, f, g, lm, c, step0=0, nsteps=1, ref_lu=False, jac
     return f, g, lm, alpha, st.as_dict()
 
 
+_IBB_SLIPS = None
+
+
+def ibb_slips39():
+    """The plain-lattice wall rule's slip terms (tests/golden/ibb39_plain.json,
+    written from the reference's cases by tests/golden/make_ibb39_plain.py),
+    handed to the C oracle: (flat (n, 8) table, start indices)."""
+    global _IBB_SLIPS
+    if _IBB_SLIPS is None:
+        import json
+        path = os.path.join(os.path.dirname(_DIR), "tests", "golden", "ibb39_plain.json")
+        d = json.load(open(path))
+        rows, start = [], [0]
+        for part in ("slips_f", "slips_g"):
+            for t in d[part]:
+                rows += t
+                start.append(len(rows))
+        tab = np.ascontiguousarray(np.array(rows, np.float64).reshape(-1, 8))
+        st = np.ascontiguousarray(start, np.int32)
+        lib().o39p_set_slips(_dp(tab), st.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        _IBB_SLIPS = (tab, st)
+    return _IBB_SLIPS
+
+
+def wall_neq39p(i, T, ut, du, dT, b1, b2, cv, slips=True):
+    """F_i, G_i of the plain-lattice wall rule (direction i, 1..38): the
+    Hermite projection plus the case's slips (slips=False: the projection
+    alone).  du[a, b] = d u_b / d x_a."""
+    if slips:
+        ibb_slips39()
+    else:
+        lib().o39p_set_slips(None, None)
+    F, G = ctypes.c_double(), ctypes.c_double()
+    lib().o39p_wall_neq(int(i), float(T), _dp(np.ascontiguousarray(ut, np.float64)),
+                        _dp(np.ascontiguousarray(du, np.float64)),
+                        _dp(np.ascontiguousarray(dT, np.float64)), float(b1), float(b2),
+                        float(cv), ctypes.byref(F), ctypes.byref(G))
+    if not slips:
+        global _IBB_SLIPS
+        _IBB_SLIPS = None
+    return F.value, G.value
+
+
 def step_walls39(shape, faces, state, f, g, lm, c, step0=0, nsteps=1, lvl_step0=None,
                  jacobi=True, keep_alpha=False, inlets=(), outlets=()):
-    """Steps of a shifted-lattice box with a link table (build_links39s):
-    wall links rebuilt by the reference's shifted-lattice wall rule.  Returns
-    (f, g, lm, alpha, stats, n_wall_links, n_wall_repairs)."""
+    """Steps of a box with a link table: on the shifted lattice (table of
+    build_links39s) wall links rebuilt by the reference's shifted-lattice wall
+    rule, on the plain lattice (table of build_links39) by its plain-lattice
+    rule.  Returns (f, g, lm, alpha, stats, n_wall_links, n_wall_repairs)."""
+    if not c.shifted:
+        ibb_slips39()
     nx, ny, nz = shape
     fc = _faces39(faces, bool(c.shifted))
     fsf, fsg = freestream39(c)

@@ -192,6 +192,15 @@ int o39_wall_rebuild(int nx, int ny, int nz, const int32_t fc[6], const int32_t 
                      const double *f, const double *g, const double *lm, int64_t k,
                      const double fi[39], const o39_consts *c, int64_t step, int64_t lvl_step,
                      double fw[39], double gw[39], int *noconv, int *kerr);
+/* plain-lattice wall rule (o_walls39.c): slip table, F/G of a direction,
+ * the rebuilt populations of a node */
+void o39p_set_slips(const double *slip, const int *start);
+void o39p_wall_neq(int i, double T, const double ut[3], double du[3][3], const double dT[3],
+                   double b1, double b2, double cv, double *F, double *G);
+int o39p_wall_rebuild(int nx, int ny, int nz, const int32_t fc[6], const int32_t *state,
+                      const double *f, const double *g, const double *lm, int64_t k,
+                      const double fi[39], const o39_consts *c, int64_t step, double fw[39],
+                      double gw[39], int *noconv, int *kerr);
 void o39_step_walls(int nx, int ny, int nz, const int32_t fc[6], const int32_t *state,
                     const double *fs_f, const double *fs_g, const int32_t *face_row,
                     const double *bc_tab, double *f, double *g, double *lm,

@@ -4,8 +4,9 @@ equality of the whole reference-form table -- the plain lattice
 sphere and on a box with a periodic axis, and the shifted lattice
 (shifted_state_directions.f95, links keyed on the upstream node) with a
 sphere, with a periodic axis, and with a body cut by a face (upstream nodes
-beyond the face classified by the ray cast); stepping a plain-lattice level
-that holds a table fails loudly (no plain-lattice D3Q39 wall rule)."""
+beyond the face classified by the ray cast); a plain-lattice level that
+holds a table with wall links steps (its wall rule is checked against the
+oracle in test_gpu_walls39p.py)."""
 import numpy as np
 import pytest
 
@@ -56,11 +57,13 @@ def test_d3q39_shifted_links_bit_exact(pkg, orc, faces, center):
     assert np.count_nonzero(st_o[1:] <= -1001) > 1000
 
 
-def test_d3q39_links_refusals(pkg):
+def test_d3q39_plain_table_steps(pkg):
     shape = (20, 16, 16)
     tri, nrm = pkg.icosphere(3, 4.0, (9.5, 8.25, 7.75), 1)
     s = _solver(pkg, shape, [-1000, -10, -11, -6, -10, -11])
     s.build_links(0, tri, nrm)
     s.init_fields(0, 1.0, 0.3, 0.0, 0.0, 1 / 1.4)
-    with pytest.raises(pkg.UclbsError):
-        s.step(0, 1)                           # no D3Q39 wall-link rule
+    s.step(0, 5)
+    st = s.stats()
+    assert st["n_ibb_links"] > 0 and st["n_nonpos"] == 0
+    assert np.isfinite(s.macro39(0)[0]).all()

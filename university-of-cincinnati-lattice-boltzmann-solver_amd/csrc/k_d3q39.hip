@@ -19,6 +19,7 @@
 // collision.f95:206-293 with find_q_bar.f95 / find_little_q_bar.f95 /
 // quick_calcs.f95 / amr_processes.f95:1751-1814, and damp_it.f95:64-92.
 #include "d3q39.h"
+#include "ibb39_slips.h"
 #include "kernels.h"
 
 #include <type_traits>
@@ -317,7 +318,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
     if (kMode == 2) {
 #pragma unroll 3
       for (int i = 0; i < Q39; ++i) {
-        if (i > 0 && wl.state[(size_t)i * n + node] <= -1001) {
+        if (i > 0 && wl.state[(size_t)(kShifted ? i : OPP39[i]) * n + node] <= -1001) {
           sf[i][t] = wl.wf[(size_t)idx * Q39 + i] - rf.f[i];
           sg[i][t] = wl.wg[(size_t)idx * Q39 + i] - rf.g[i];
         } else {
@@ -464,17 +465,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 2))) void
   wave_stats39(stats, active, iters, noconv, ast, kerr);
 }
 
-// ---- shifted-lattice walls ---------------------------------------------------
+// ---- walls ---------------------------------------------------------------------
 // rho, u, T of node k from the input buffers (the previous step's
 // post-collision populations)
-template <typename S>
+template <bool kShifted, typename S>
 __device__ __forceinline__ void macro_at39(const S* __restrict__ fA, const S* __restrict__ gA,
                                            unsigned n, unsigned k, const Ref39& rf,
                                            const Gas39& gs, double* rho, double u[3], double* T) {
   double r = 0.0, mx = 0.0, my = 0.0, mz = 0.0, en = 0.0;
 #pragma unroll 3
   for (int i = 0; i < Q39; ++i) {
-    const int cx = cx39<true>(i), cy = CY39[i], cz = CZ39[i];
+    const int cx = cx39<kShifted>(i), cy = CY39[i], cz = CZ39[i];
     const double f = (double)fA[(size_t)i * n + k] + rf.f[i],
                  g = (double)gA[(size_t)i * n + k] + rf.g[i];
     r += f;
@@ -557,7 +558,7 @@ __global__ __launch_bounds__(64) void k39_wall_rebuild(const void* __restrict__ 
     sfi[i][t] = (double)fs + rf.f[i];
   }
   double rho0, u0[3], T0;
-  macro_at39(fA, gA, n, node, rf, gs, &rho0, u0, &T0);
+  macro_at39<true>(fA, gA, n, node, rf, gs, &rho0, u0, &T0);
   bool adjacent = false;
   for (int i = 1; i <= 14; ++i) adjacent = adjacent || st[(size_t)i * n + node] < -1000;
   double rho_t, ut[3], Tt, lmt[kM39];
@@ -574,7 +575,7 @@ __global__ __launch_bounds__(64) void k39_wall_rebuild(const void* __restrict__ 
       double rn = rho0, un[3] = {u0[0], u0[1], u0[2]}, Tn = T0;
       if (st[(size_t)OPP39[i] * n + node] == -666) {
         const long long d = pos39(x + cx39<true>(i), y + CY39[i], z + CZ39[i], b, fc);
-        if (d >= 0) macro_at39(fA, gA, n, (unsigned)d, rf, gs, &rn, un, &Tn);
+        if (d >= 0) macro_at39<true>(fA, gA, n, (unsigned)d, rf, gs, &rn, un, &Tn);
       }
       for (int e = 0; e < 3; ++e) ut[e] += (q * un[e]) / (q + 1.0);
       Tt += (q * Tn + Tn) / (q + 1.0);
@@ -623,7 +624,7 @@ __global__ __launch_bounds__(64) void k39_wall_rebuild(const void* __restrict__ 
       if (m == 0) {
         uu[0] = u0[0]; uu[1] = u0[1]; uu[2] = u0[2]; TT = T0;
       } else {
-        macro_at39(fA, gA, n, (unsigned)d, rf, gs, &rr, uu, &TT);
+        macro_at39<true>(fA, gA, n, (unsigned)d, rf, gs, &rr, uu, &TT);
       }
       for (int e = 0; e < 3; ++e) su[m][e] = uu[e];
       sT[m] = TT;
@@ -790,6 +791,258 @@ __global__ __launch_bounds__(64) void k39_wall_rebuild(const void* __restrict__ 
   if (kerr) atomicAdd(sl + ST_FALLBACK, 1ull);
 }
 
+// x^e for a small integer e
+__device__ __forceinline__ double ipow39(double x, int e) {
+  double r = 1.0;
+  for (int k = 0; k < (e < 0 ? -e : e); ++k) r *= x;
+  return e < 0 ? 1.0 / r : r;
+}
+
+// Non-equilibrium parts F_i, G_i of the plain lattice's rebuilt population i
+// (1..38): the Hermite projection the reference's hand-expanded cases of
+// interpolative_bounceback_bc.f95:834-2838 carry out,
+//   F = H2(c):P / (2 b1 T^2) + H3(c):Q / (6 T^3),
+//   G = 9 div / (b1 cv) + c.(9 grad T / b2 + 3 u div / (b1 cv)) / T
+//       + H2(c):R / (2 b1 T^2),
+// P_ab = 3 (d_a u_b + d_b u_a) - delta_ab div / cv, Q_abe the sum over the
+// three pair/single splits (p q | s) of delta_pq d_s T / b2 + u_s Pi_pq^s / b1
+// with Pi_pp^s = 2 d_p u_p - d_s u_s / cv and Pi_pq = d_p u_q + d_q u_p,
+// R_aa = 6 T d_a u_a + 6 u_a d_a T - (2 T + u_a^2) div / cv and
+// R_ab = 3 (T (d_a u_b + d_b u_a) + u_a d_b T + u_b d_a T) - u_a u_b div / cv;
+// plus the terms by which the cases depart from it (kIbbSlip).
+// du[a][b] = d u_b / d x_a.
+__device__ void ibb_neq_plain39(int i, double T, const double ut[3], const double du[3][3],
+                                const double dT[3], double b1, double b2, double cv, double* F,
+                                double* G) {
+  const int c[3] = {cx39<false>(i), CY39[i], CZ39[i]};
+  const double div = du[0][0] + du[1][1] + du[2][2];
+  double h2p = 0.0, h2r = 0.0, h3q = 0.0;
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) {
+      const double h2 = (double)(c[a] * c[b]) - (a == b ? T : 0.0);
+      double P, R;
+      if (a == b) {
+        P = 6.0 * du[a][a] - div / cv;
+        R = 6.0 * T * du[a][a] + 6.0 * ut[a] * dT[a] - (2.0 * T + ut[a] * ut[a]) * div / cv;
+      } else {
+        P = 3.0 * (du[a][b] + du[b][a]);
+        R = 3.0 * (T * (du[a][b] + du[b][a]) + ut[a] * dT[b] + ut[b] * dT[a]) -
+            div / cv * ut[a] * ut[b];
+      }
+      h2p += h2 * P;
+      h2r += h2 * R;
+      for (int e = 0; e < 3; ++e) {
+        const double h3 = (double)(c[a] * c[b] * c[e]) -
+                          T * (double)(c[a] * (b == e) + c[b] * (a == e) + c[e] * (a == b));
+        if (h3 == 0.0) continue;
+        const int sp[3][3] = {{a, b, e}, {a, e, b}, {b, e, a}};
+        double q = 0.0;
+        for (int m = 0; m < 3; ++m) {
+          const int p = sp[m][0], r = sp[m][1], sg = sp[m][2];
+          const double pi = p == r ? 2.0 * du[p][p] - du[sg][sg] / cv : du[p][r] + du[r][p];
+          q += (p == r ? dT[sg] / b2 : 0.0) + ut[sg] * pi / b1;
+        }
+        h3q += h3 * q;
+      }
+    }
+  double f = h2p / (2.0 * b1 * T * T) + h3q / (6.0 * T * T * T);
+  double g = 9.0 * div / (b1 * cv) + h2r / (2.0 * b1 * T * T);
+  for (int a = 0; a < 3; ++a) g += (double)c[a] * (9.0 * dT[a] / b2 + 3.0 * ut[a] * div / (b1 * cv)) / T;
+  const double D[12] = {du[0][0], du[1][0], du[2][0], du[0][1], du[1][1], du[2][1],
+                        du[0][2], du[1][2], du[2][2], dT[0], dT[1], dT[2]};
+  for (int part = 0; part < 2; ++part) {
+    double add = 0.0;
+    for (int m = kIbbSlipStart[38 * part + i - 1]; m < kIbbSlipStart[38 * part + i]; ++m) {
+      const IbbSlip& t = kIbbSlip[m];
+      add += t.coef * D[t.d] / (t.beta == 1 ? b1 : b2) * ipow39(ut[0], t.eu) * ipow39(ut[1], t.ev) *
+             ipow39(ut[2], t.ew) * ipow39(T, t.eT) * ipow39(cv, t.ecv);
+    }
+    if (part == 0) f += add;
+    else g += add;
+  }
+  *F = f;
+  *G = g;
+}
+
+// The plain lattice's wall rule for the listed wall nodes (streaming.f95:
+// 478-1133 with the plain branch, target_finder.f95:36-88,
+// interpolative_bounceback_bc.f95:834-2838, derivatives.f95:237-583), one
+// thread per node.  Link j (x -> x + c_j) coded <= -1001 makes the population
+// i = opp(j), which would be pulled from the solid node x + c_j, a rebuilt
+// one.  Target (when a link of speed <= sqrt(3) is a wall link): u_t, T_t
+// the means over the wall links of q u_n / (q + 1) and (q T_n + T_n) /
+// (q + 1), n = x + c_opp(j) when that link's code is >= 0 (the node itself
+// beyond a face), else q u(x) / (q + 1) and (q T(x) + T_inf) / (q + 1);
+// rho_t from this step's pulled populations with a missing one replaced by
+// its opposite; multipliers by Newton from the node's; otherwise the node's
+// own state.  Gradients: along an axis where c_i is positive
+// (u_t - 0) / (1 + w), negative (0 - u_t) / (1 + w), w = |s_i + 1001| 1e-9
+// |c_i| from the node's own code of link i, temperature 0; else the local
+// difference over the neighbours whose link code is >= 0.  Then
+// f_i = feq_i(target) + eps k_f F_i, g_i = (2 cv - 3) T_t feq_i + eps k_g G_i
+// (ibb_neq_plain39); no magnitude repair (the repair pass handles negatives).
+template <bool kFp32>
+__global__ __launch_bounds__(64) void k39_wall_rebuild_plain(const void* __restrict__ fA_,
+                                                             const void* __restrict__ gA_,
+                                                             const double* __restrict__ lmv,
+                                                             Box b, Gas39 gs, Ref39 rf,
+                                                             Faces39 fc, long long step,
+                                                             Walls39 wl,
+                                                             unsigned long long* __restrict__ stats) {
+  using S = typename std::conditional<kFp32, float, double>::type;
+  const S* __restrict__ fA = static_cast<const S*>(fA_);
+  const S* __restrict__ gA = static_cast<const S*>(gA_);
+  __shared__ int smir[3][Q39];
+  __shared__ double sfi[Q39][64];
+  const int t = threadIdx.x;
+  if (t < Q39) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) smir[d][t] = kMirror39.m[d][t];
+  }
+  __syncthreads();
+  const unsigned j = blockIdx.x * 64u + (unsigned)t;
+  if (j >= wl.n) return;
+  const unsigned n = (unsigned)b.n;
+  const unsigned node = wl.list[j];
+  const int x = (int)(node % (unsigned)b.nx);
+  const int y = (int)((node / (unsigned)b.nx) % (unsigned)b.ny);
+  const int z = (int)(node / ((unsigned)b.nx * (unsigned)b.ny));
+  const int32_t* __restrict__ st = wl.state;
+  for (int i = 0; i < Q39; ++i) {
+    S fs, gsv;
+    pull_face39<false>(x, y, z, i, b, fc, rf, smir, fA, gA, &fs, &gsv);
+    sfi[i][t] = (double)fs + rf.f[i];
+  }
+  double rho0, u0[3], T0;
+  macro_at39<false>(fA, gA, n, node, rf, gs, &rho0, u0, &T0);
+  const double t_inf = 1.0 / gs.gama;
+  bool adjacent = false;
+  for (int i = 1; i <= 14; ++i) adjacent = adjacent || st[(size_t)i * n + node] < -1000;
+  double rho_t, ut[3], Tt, lmt[kM39];
+  int noconv = 0;
+  for (int k = 0; k < kM39; ++k) lmt[k] = lmv[(size_t)k * n + node];
+  if (adjacent) {
+    int count = 0;
+    ut[0] = ut[1] = ut[2] = 0.0;
+    Tt = 0.0;
+    for (int i = 1; i < Q39; ++i) {
+      const int32_t s = st[(size_t)i * n + node];
+      if (s > -1001) continue;
+      const double q = fabs((double)s + 1001.0) / 1.0e9;
+      ++count;
+      const int o = OPP39[i];
+      if (st[(size_t)o * n + node] >= 0) {
+        double rn = rho0, un[3] = {u0[0], u0[1], u0[2]}, Tn = T0;
+        const long long d = pos39(x + cx39<false>(o), y + CY39[o], z + CZ39[o], b, fc);
+        if (d >= 0) macro_at39<false>(fA, gA, n, (unsigned)d, rf, gs, &rn, un, &Tn);
+        for (int e = 0; e < 3; ++e) ut[e] += (q * un[e]) / (q + 1.0);
+        Tt += (q * Tn + Tn) / (q + 1.0);
+      } else {
+        for (int e = 0; e < 3; ++e) ut[e] += (q * u0[e]) / (q + 1.0);
+        Tt += (q * T0 + t_inf) / (q + 1.0);
+      }
+    }
+    for (int e = 0; e < 3; ++e) ut[e] /= count;
+    Tt /= count;
+    rho_t = 0.0;
+    for (int i = 0; i < Q39; ++i)
+      rho_t += st[(size_t)OPP39[i] * n + node] < -1000 ? sfi[OPP39[i]][t] : sfi[i][t];
+    const double vm = ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2];
+    const double qf = rho_t * vm + 5 * rho_t * Tt;
+    const double tg[kM39] = {rho_t, rho_t * ut[0], rho_t * ut[1], rho_t * ut[2],
+                             rho_t * (Tt + ut[0] * ut[0]), rho_t * (Tt + ut[1] * ut[1]),
+                             rho_t * (Tt + ut[2] * ut[2]), rho_t * ut[0] * ut[1],
+                             rho_t * ut[0] * ut[2], rho_t * ut[1] * ut[2], qf * ut[0],
+                             qf * ut[1], qf * ut[2]};
+    double lmp[kM39];
+    noconv = lm_newton39<false>(rho_t, tg, lmt, lmp, gs.maxit) < 0;
+  } else {
+    rho_t = rho0;
+    for (int e = 0; e < 3; ++e) ut[e] = u0[e];
+    Tt = T0;
+  }
+  // stencil values: x, x + e_a, x - e_a (a fluid node's own; 0 and T_inf else)
+  double su[7][3], sT[7];
+  for (int m = 0; m < 7; ++m) {
+    const int a = m == 0 ? -1 : (m - 1) % 3, sg = m == 0 ? 0 : (m <= 3 ? 1 : -1);
+    const long long d = m == 0 ? (long long)node
+                               : pos39(x + (a == 0 ? sg : 0), y + (a == 1 ? sg : 0),
+                                       z + (a == 2 ? sg : 0), b, fc);
+    if (d >= 0 && st[(size_t)d] >= 0) {
+      double rr, uu[3], TT;
+      if (m == 0) {
+        uu[0] = u0[0]; uu[1] = u0[1]; uu[2] = u0[2]; TT = T0;
+      } else {
+        macro_at39<false>(fA, gA, n, (unsigned)d, rf, gs, &rr, uu, &TT);
+      }
+      for (int e = 0; e < 3; ++e) su[m][e] = uu[e];
+      sT[m] = TT;
+    } else {
+      su[m][0] = su[m][1] = su[m][2] = 0.0;
+      sT[m] = t_inf;
+    }
+  }
+  const double eps =
+      step < gs.startup ? (((double)step + 1.0) / (double)gs.startup) * gs.eps : gs.eps;
+  double dmu, dte;
+  const double mu = sutherland39(Tt, gs, &dmu, &dte);
+  double kappa = 0.0;
+  const bool kerr = !therm_cond39(rho_t, Tt, gs, &kappa);
+  double b1, b2;
+  if (step < gs.startup) {
+    b1 = gs.init_beta1 + gs.beta1_scale * (double)step;
+    b2 = gs.init_beta2 + gs.beta2_scale * (double)step;
+  } else {
+    b1 = 1.0 / ((ldexp(1.0, gs.lvl + 1) * mu) / (rho_t * Tt) + 1.0);
+    b2 = 1.0 / ((2.0 * kappa) / (rho_t * gs.cp * Tt) + 1.0);
+  }
+  const double k_f = -(rho_t * Tt) / 2.0, k_g = -(rho_t * Tt * (2.0 * gs.cv - 3.0)) / 2.0;
+  unsigned long long nlinks = 0;
+  for (int i = 1; i < Q39; ++i) {
+    if (st[(size_t)OPP39[i] * n + node] > -1001) continue;
+    ++nlinks;
+    const int ci[3] = {cx39<false>(i), CY39[i], CZ39[i]};
+    const double link = sqrt((double)(ci[0] * ci[0] + ci[1] * ci[1] + ci[2] * ci[2]));
+    const double wd = fabs((double)st[(size_t)i * n + node] + 1001.0) / 1.0e9 * link;
+    double du[3][3], dT[3];
+    for (int ax = 0; ax < 3; ++ax) {
+      const bool up_ok = st[(size_t)(4 + ax) * n + node] >= 0;
+      const bool down_ok = st[(size_t)(1 + ax) * n + node] >= 0;
+      for (int e = 0; e < 3; ++e) {
+        if (ci[ax] > 0) {
+          du[ax][e] = (ut[e] - 0.0) / (1.0 + wd);
+        } else if (ci[ax] < 0) {
+          du[ax][e] = (0.0 - ut[e]) / (1.0 + wd);
+        } else {
+          const double rest = su[0][e], up = su[4 + ax][e], down = su[1 + ax][e];
+          du[ax][e] = (up_ok && down_ok) ? (down - up) / 2.0
+                      : up_ok            ? rest - up
+                      : down_ok          ? down - rest
+                                         : 0.0;
+        }
+      }
+      if (ci[ax] != 0) {
+        dT[ax] = 0.0;
+      } else {
+        const double rest = sT[0], up = sT[4 + ax], down = sT[1 + ax];
+        dT[ax] = (up_ok && down_ok) ? (down - up) / 2.0
+                 : up_ok            ? rest - up
+                 : down_ok          ? down - rest
+                                    : 0.0;
+      }
+    }
+    double F, G;
+    ibb_neq_plain39(i, Tt, ut, du, dT, b1, b2, gs.cv, &F, &G);
+    const double feq = expo39<false>(lmt, rho_t, i);
+    wl.wf[(size_t)j * Q39 + i] = feq + eps * k_f * F;
+    wl.wg[(size_t)j * Q39 + i] = gs.lead_g * Tt * feq + eps * k_g * G;
+  }
+  unsigned long long* sl = stats + (size_t)(blockIdx.x & (kStatSlots - 1)) * kStatFields;
+  atomicAdd(sl + ST_IBB, nlinks);
+  if (noconv) atomicAdd(sl + ST_NOCONV, 1ull);
+  if (kerr) atomicAdd(sl + ST_FALLBACK, 1ull);
+}
+
 // node (x, y, z) wrapped over the periodic axes of b; -1 across another face
 // or inside a refined level's ghost layer (which holds no post-collision values)
 __device__ __forceinline__ long long damp_nb(int x, int y, int z, const Box& b, int ghost) {
@@ -839,14 +1092,20 @@ __global__ __launch_bounds__(256) void k39_damp_compute(const void* __restrict__
         // with a link table (damp_it.f95:125-177): the + neighbour counts when
         // the link code is >= -500 (the rest direction: the node's code), the
         // - neighbour when its node code is >= -500; with neither, no repair
+        // a plain-lattice table (damp_it.f95:64-123): the + neighbour counts
+        // when the node's link code of i is >= 0, the - neighbour when that
+        // of opp(i) is (the rest direction: the node's code); with neither, |f|
         bool pok = kp >= 0, mok = km >= 0;
-        if (state) {
+        if (state && kShifted) {
           pok = pok && state[(size_t)(i == 0 ? 0 : i) * n + node] >= -500;
           mok = mok && state[(size_t)km] >= -500;
+        } else if (state) {
+          pok = pok && state[(size_t)i * n + node] >= 0;
+          mok = mok && state[(size_t)OPP39[i] * n + node] >= 0;
         }
         const double fp = pok ? (double)f[(size_t)i * n + (size_t)kp] + rf.f[i] : -1.0;
         const double fm = mok ? (double)f[(size_t)i * n + (size_t)km] + rf.f[i] : -1.0;
-        if (!state || (pok && mok)) {
+        if (pok && mok) {
           if (fp > 0.0 && fm > 0.0) r = (fp + fm) / 2.0;
           else if (fp > 0.0 && fm < 0.0) r = (fp + fabs(v)) / 2;
           else if (fp < 0.0 && fm > 0.0) r = (fm + fabs(v)) / 2;
@@ -855,6 +1114,8 @@ __global__ __launch_bounds__(256) void k39_damp_compute(const void* __restrict__
           r = fp > 0.0 ? (fp + fabs(v)) / 2 : fabs(v);
         } else if (mok) {
           r = fm > 0.0 ? (fm + fabs(v)) / 2 : fabs(v);
+        } else if (!(state && kShifted)) {
+          r = fabs(v);
         }
         if (r == r) ++nrep;
       }
@@ -1145,16 +1406,19 @@ hipError_t launch_d3q39_step(const Box& b, const Step39& a, hipStream_t s) {
   const unsigned near_blocks = (a.n_near + 63u) / 64u;
   if (a.has_faces && a.n_near && !a.near_list) return hipErrorInvalidValue;
   const unsigned wall_blocks = (a.walls.n + 63u) / 64u;
-  if (a.walls.cls && !a.shifted) return hipErrorInvalidValue;  // shifted-lattice wall rule only
   hipLaunchKernelGGL(k39_clear, dim3(1), dim3(1), 0, s, a.neg_count);
   // the wall nodes' rebuilt populations, from the input buffers
   if (a.walls.cls && a.walls.n) {
-    if (a.fp32)
-      hipLaunchKernelGGL((k39_wall_rebuild<true>), dim3(wall_blocks), dim3(64), 0, s, a.fin, a.gin,
-                         a.lm, b, a.gas, a.ref, a.faces, (long long)a.step, a.walls, a.stats);
-    else
-      hipLaunchKernelGGL((k39_wall_rebuild<false>), dim3(wall_blocks), dim3(64), 0, s, a.fin, a.gin,
-                         a.lm, b, a.gas, a.ref, a.faces, (long long)a.step, a.walls, a.stats);
+#define UCLBS_WALLS39(K)                                                                         \
+  hipLaunchKernelGGL(K, dim3(wall_blocks), dim3(64), 0, s, a.fin, a.gin, a.lm, b, a.gas, a.ref, \
+                     a.faces, (long long)a.step, a.walls, a.stats)
+    if (a.shifted) {
+      if (a.fp32) UCLBS_WALLS39(k39_wall_rebuild<true>); else UCLBS_WALLS39(k39_wall_rebuild<false>);
+    } else {
+      if (a.fp32) UCLBS_WALLS39(k39_wall_rebuild_plain<true>);
+      else UCLBS_WALLS39(k39_wall_rebuild_plain<false>);
+    }
+#undef UCLBS_WALLS39
   }
 #define UCLBS_K39(F, SH, TU, KA, NR, NB)                                                         \
   hipLaunchKernelGGL((k39_step<F, SH, TU, KA, NR>), dim3(NB), dim3(64), 0, s, a.fin, a.gin,     \
@@ -1165,7 +1429,7 @@ hipError_t launch_d3q39_step(const Box& b, const Step39& a, hipStream_t s) {
   do {                                                                           \
     UCLBS_K39(F, SH, TU, KA, 0, blocks);                                         \
     if (a.has_faces && a.n_near) UCLBS_K39(F, SH, TU, KA, 1, near_blocks);       \
-    if (SH && a.walls.cls && a.walls.n) UCLBS_K39(F, SH, TU, KA, 2, wall_blocks); \
+    if (a.walls.cls && a.walls.n) UCLBS_K39(F, SH, TU, KA, 2, wall_blocks);      \
   } while (0)
 #define UCLBS_K39_KA(F, SH, TU) \
   do {                          \

@@ -432,7 +432,7 @@ int check_face_tables(const Ctx* c, const Level* L) {
 // their own launch; the list is rebuilt when the faces change
 static int build_near39(Ctx* c, Level* L) {
   if (!L->near_dirty) return UCLBS_OK;
-  if (L->table39 && c->shifted39) {
+  if (L->table39) {
     // node classes from the table: solid, wall links (i >= 1 coded <= -1001),
     // near a face, plain; the near-face and wall launches take their lists
     const int64_t n = L->box.n;
@@ -450,7 +450,8 @@ static int build_near39(Ctx* c, Level* L) {
         for (int i = 1; i < uclbs::Q39 && !w; ++i) w = st[(size_t)i * n + k] <= -1001;
         const int x = (int)(k % bx.nx), y = (int)((k / bx.nx) % bx.ny), z = (int)(k / ((int64_t)bx.nx * bx.ny));
         if (w) cl = 2;
-        else if (uclbs::near_face39(x, y, z, L->face_code, bx.nx, bx.ny, bx.nz, true)) cl = 1;
+        else if (uclbs::near_face39(x, y, z, L->face_code, bx.nx, bx.ny, bx.nz, c->shifted39 != 0))
+          cl = 1;
       }
       cls[(size_t)k] = cl;
       if (cl == 1) near.push_back((unsigned)k);
@@ -541,10 +542,6 @@ int resolve_bc39(Ctx* c, Level* L, uclbs::Faces39* fc) {
 }
 
 int launch_steps39(Ctx* c, Level* L, int nsteps, hipEvent_t* ev) {
-  if (L->table39 && !c->shifted39)
-    return set_err(UCLBS_E_UNSUPPORTED, "plain-lattice D3Q39 levels with a link table are not "
-                                        "stepped: only the shifted lattice's wall rule "
-                                        "(shifted_ibb) is part of this library");
   uclbs::Step39 a{};
   a.gas = L->gas39;
   a.ghost = L->refined ? L->ghost_depth : 0;
@@ -2198,8 +2195,6 @@ int uclbs_advance(void* ctx, int ncoarse, int use_graph) {
         bool faces = false;
         for (int f = 0; f < 6; ++f) faces = faces || (!L.refined && L.face_code[f] != uclbs::kFacePeriodic);
         if ((faces || L.table39) && (rc = build_near39(c, &L))) return rc;
-        if (L.table39 && !c->shifted39)
-          return set_err(UCLBS_E_UNSUPPORTED, "plain-lattice D3Q39 levels with a link table are not stepped");
       }
     const std::vector<int> sig = graph_signature(c);
     if (!c->graph || c->graph_sig != sig) {
