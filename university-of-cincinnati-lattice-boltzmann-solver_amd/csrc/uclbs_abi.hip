@@ -392,6 +392,17 @@ int finalize_links(Ctx* c, Level* L) {
   std::vector<uint8_t> hf((size_t)n);
   HIP_TRY(hipMemcpyAsync(hf.data(), L->flags, (size_t)n, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
+  if (L->refined) {
+    // a refined level's ghost shell is filled from the coarser level, never stepped
+    const int G = L->ghost_depth;
+    const int nx = L->box.nx, ny = L->box.ny, nz = L->box.nz;
+    for (int64_t k = 0; k < n; ++k) {
+      const int x = (int)(k % nx), y = (int)((k / nx) % ny), z = (int)(k / ((int64_t)nx * ny));
+      if (x < G || y < G || z < G || x >= nx - G || y >= ny - G || z >= nz - G)
+        hf[(size_t)k] = uclbs::kFlagGhost;
+    }
+    HIP_TRY(hipMemcpy(L->flags, hf.data(), (size_t)n, hipMemcpyHostToDevice));
+  }
   std::vector<uint32_t> list;
   int64_t nsolid = 0;
   for (int64_t k = 0; k < n; ++k) {
@@ -792,8 +803,6 @@ int uclbs_define_level(void* ctx, int lvl, const int32_t lo[3], const int32_t hi
     if (lvl - 1 >= (int)c->levels.size() || !c->levels[lvl - 1].defined)
       return set_err(UCLBS_E_STATE, "define level %d before level %d", lvl - 1, lvl);
     const Level& P = c->levels[lvl - 1];
-    if (P.has_links && !P.refined)
-      return set_err(UCLBS_E_UNSUPPORTED, "refined levels nest in a level without link tables");
     if (P.multibox)
       return set_err(UCLBS_E_UNSUPPORTED, "a single-box level does not nest in a level of many "
                                           "boxes (use uclbs_define_level_boxes)");
@@ -1437,8 +1446,9 @@ int uclbs_build_links(void* ctx, int lvl, const double origin[3], double dx, int
   int rc = check_level(c, lvl, &L);
   if (rc) return rc;
   if (L->multibox) return set_err(UCLBS_E_UNSUPPORTED, "no link tables on levels of many boxes");
-  if (L->refined) return set_err(UCLBS_E_UNSUPPORTED, "level %d is a refined level (no link tables)", lvl);
   const bool q39 = c->cfg.lattice == UCLBS_LATTICE_D3Q39;
+  if (L->refined && q39)
+    return set_err(UCLBS_E_UNSUPPORTED, "level %d is a refined D3Q39 level (no link tables)", lvl);
   if (!origin || !(dx > 0.0) || ntri < 0 || ntri > (1 << 26) || (ntri && (!tri || !nrm)))
     return set_err(UCLBS_E_ARG, "build_links: bad geometry arguments");
   if ((rc = set_device(c))) return rc;
@@ -1532,7 +1542,8 @@ int uclbs_set_links(void* ctx, int lvl, const int32_t* state, int64_t n_nodes) {
   int rc = check_level(c, lvl, &L);
   if (rc) return rc;
   if (L->multibox) return set_err(UCLBS_E_UNSUPPORTED, "no link tables on levels of many boxes");
-  if (L->refined) return set_err(UCLBS_E_UNSUPPORTED, "level %d is a refined level (no link tables)", lvl);
+  if (L->refined && c->cfg.lattice == UCLBS_LATTICE_D3Q39)
+    return set_err(UCLBS_E_UNSUPPORTED, "level %d is a refined D3Q39 level (no link tables)", lvl);
   if (!state || n_nodes != L->box.n) return set_err(UCLBS_E_ARG, "set_links: wrong node count");
   if ((rc = set_device(c))) return rc;
   const size_t bytes = (size_t)n_nodes * q_of(c) * sizeof(int32_t);
