@@ -144,7 +144,12 @@ int uclbs_set_bc_states(void *ctx, int n_in, const double *in_states, int n_out,
  * nrm[t*3 + axis]): node codes as lbgrid/fluid_or_solid.f95:86-442, wall-link
  * q codes as lbgrid/inter_bb_setup.f95:44-246 and linkwise.f95:45-57, face and
  * interior codes as lbgrid/state_directions.f95.  Node (a,b,c) of the box sits
- * at origin + (a,b,c)*dx (amr_processes.f95:1155-1168).  The table stays
+ * at origin + (a,b,c)*dx (amr_processes.f95:1155-1168); on a refined D3Q27
+ * level the box is the stored one (its ghost shell included) and dx the
+ * level's spacing.  On a D3Q27 level of many boxes (refined levels) origin
+ * is the position of level index (0,0,0): each box's stored nodes are built
+ * at origin + (index)*dx, and a ghost node that shows a boundary or solid
+ * node of another box is copied from it before each step.  The table stays
  * resident; *n_tie counts rays re-cast because of a tie.
  * On a D3Q39 context of the plain lattice the table has 39 components, with
  * the long links derived from the shorter ones as state_directions.f95:95-185

@@ -76,3 +76,48 @@ def test_two_level_walls_match_oracle(pkg, orc):
           f"{rel_l2(fc[:, flc], u0[:, flc]):.2e}; wall links {s.stats()['n_ibb_links']}")
     assert rel_l2(fc[:, flc], u0[:, flc]) > 1e-4
     assert e_c < 1e-12 and e_f < 1e-12
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32s"])
+def test_refined_boxes_with_walls_match_one_box(pkg, orc, prec):
+    """The refined region split into uneven boxes (the sphere crossing box
+    faces), its link table built box by box, equals the same region as one
+    box bit for bit after 6 coarse steps: coarse level and refined valid
+    nodes, wall-link counts.  The alpha tolerance is 1e-12 in both
+    precisions: the one-box and many-box kernels run the same arithmetic,
+    but the compiler may round its intermediate products differently in
+    each, which the early exits of a looser tolerance (1e-6 in FP32) pass
+    on at the 1e-8 level of the FP32 storage."""
+    from test_gpu_boxes import _tiling
+    fvalid = tuple(FHI[d] - FLO[d] + 1 for d in range(3))
+    fphys = tuple(0.5 * (FLO[d] - 1) for d in range(3))
+    tri, nrm = pkg.icosphere(3, 3.1, (12.25, 7.75, 8.1), 20260102)
+    boxes = _tiling(FLO, FHI, (5, 7, 9), 11)
+    runs = []
+    for many in (False, True):
+        s = pkg.Solver(precision=prec, nu=NU, alpha_tol=1e-12)
+        s.define_level(0, COARSE, periodic=(0, 0, 0))
+        s.set_faces(0, FACES)
+        s.set_bc_states(inlets=[(1.0, U, 0.0, 0.0)], outlets=[(1.0, U, 0.0, 0.0)])
+        s.build_links(0, tri, nrm)
+        if many:
+            s.define_level_boxes(1, boxes)
+            s.build_links(1, tri, nrm, origin=(0.0, 0.0, 0.0), dx=0.5)
+        else:
+            s.define_level(1, fvalid, periodic=(0, 0, 0), lo=FLO)
+            s.build_links(1, tri, nrm, origin=fphys, dx=0.5)
+        s.init_uniform(0, 1.0, (U, 0.0, 0.0))
+        s.init_from_coarse(1)
+        s.advance(6)
+        fc, ff = s.download_f(0), s.download_f(1)
+        if many:
+            fine = s.stored_to_dense(1, ff, fvalid[::-1], origin=FLO)
+        else:
+            fine = ff.reshape(27, fvalid[2] + 2, fvalid[1] + 2, fvalid[0] + 2)[:, 1:-1, 1:-1, 1:-1]
+        runs.append((fc, fine, s.stats()["n_ibb_links"]))
+        s.close()
+    print(f"{prec}: {len(boxes)} boxes; wall links {runs[0][2]} vs {runs[1][2]}; max|d| coarse "
+          f"{np.abs(runs[0][0] - runs[1][0]).max():.1e} fine {np.abs(runs[0][1] - runs[1][1]).max():.1e}")
+    assert len(boxes) > 8 and runs[0][2] == runs[1][2] > 0
+    assert np.array_equal(runs[0][0], runs[1][0])
+    assert np.array_equal(runs[0][1], runs[1][1])

@@ -206,7 +206,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 // copy pass (a deferred mirror is queued too, marked, and solved by the far
 // kernel like s).  Ghosts at a coarse-fine interface are interpolated before
 // the step.
-template <bool kShifted, bool kKeepAlpha>
+template <bool kShifted, bool kKeepAlpha, bool kLinks>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k27_step_boxes(
     const void* __restrict__ fin_, void* __restrict__ fout_, double* __restrict__ alpha_out,
     unsigned long long* __restrict__ stats, unsigned nn, StepArgs a,
@@ -228,13 +228,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   const unsigned lx = local % bx.nx, lyz = local / bx.nx, ly = lyz % bx.ny, lz = lyz / bx.ny;
   const bool shell = lx == 0u || ly == 0u || lz == 0u || lx + 1u == bx.nx || ly + 1u == bx.ny ||
                      lz + 1u == bx.n / sxy;
-  const bool active = in_box && !shell;  // counted node updates
   uint8_t fl = kFlagInterior;
   uint4 al = make_uint4(0u, 0u, 0u, 0u);
   if (in_box && shell) {
     fl = flags[node];
     al = alias[node];
+  } else if (kLinks && in_box) {
+    // with a link table a valid node may be a boundary node (stepped by the
+    // boundary kernel) or solid (left as it is)
+    fl = flags[node];
   }
+  const bool active = in_box && !shell && fl == kFlagInterior;  // counted node updates
   const bool mirror = in_box && shell && fl == kFlagMirror;
   // the node whose update this lane computes, and that node's box strides
   const unsigned src = mirror ? al.x : node;
@@ -252,8 +256,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   else ao = collide27_fp64(f, a.beta, ac, 2.0, rho, u);
   // ghost-shell lanes store too (values nobody reads: the shell is refilled
   // before the next step), so every line of the box is written whole and the
-  // L2 does not fetch partly written lines
-  if (in_box) {
+  // L2 does not fetch partly written lines; boundary and solid valid nodes
+  // are written by the boundary kernel or not at all
+  if (in_box && (!kLinks || shell || active)) {
 #pragma unroll
     for (int i = 0; i < 27; ++i) pop(fout, i, nn, node) = f[i];
   }
@@ -516,9 +521,21 @@ __global__ __launch_bounds__(64) void k27_boundary(const void* __restrict__ fin,
   const bool active = j < bc.nlist;
   const int64_t k = bc.list[active ? j : 0];
   const int64_t n = b.n;
-  const int x = (int)(k % b.nx);
-  const int64_t yz = k / b.nx;
-  const int y = (int)(yz % b.ny), z = (int)(yz / b.ny);
+  // the node's box: the level's one box (periodic axes wrap) or, on a level
+  // of many boxes, the box whose ghost shell holds the node's neighbours
+  int64_t base = 0;
+  int bnx = b.nx, bny = b.ny, bnz = b.nz;
+  if (bc.boxes) {
+    const BoxDesc bx = bc.boxes[bc.box_of[active ? j : 0]];
+    base = bx.off;
+    bnx = (int)bx.nx;
+    bny = (int)bx.ny;
+    bnz = (int)(bx.n / (bx.nx * bx.ny));
+  }
+  const int64_t lk = k - base;
+  const int x = (int)(lk % bnx);
+  const int64_t yz = lk / bnx;
+  const int y = (int)(yz % bny), z = (int)(yz / bny);
   double f[27];
   bool unknown[27];
   bool any_wall = false;
@@ -528,8 +545,8 @@ __global__ __launch_bounds__(64) void k27_boundary(const void* __restrict__ fin,
     if (i == 0) { f[0] = ldf<kShifted>(fin, 0, n, k); continue; }
     const int32_t s = bc.state[(int64_t)OPP27[i] * n + k];
     if (s >= 0) {
-      const int64_t src = wrapc(x - CX27[i], b.nx) +
-                          (int64_t)b.nx * (wrapc(y - CY27[i], b.ny) + (int64_t)b.ny * wrapc(z - CZ27[i], b.nz));
+      const int64_t src = base + wrapc(x - CX27[i], bnx) +
+                          (int64_t)bnx * (wrapc(y - CY27[i], bny) + (int64_t)bny * wrapc(z - CZ27[i], bnz));
       f[i] = ldf<kShifted>(fin, i, n, src);
     } else if (s == -10) {
       f[i] = ldf<kShifted>(fin, OPP27[i], n, k);
@@ -560,8 +577,8 @@ __global__ __launch_bounds__(64) void k27_boundary(const void* __restrict__ fin,
         v = fk / (2.0 * q) + (1.0 - 1.0 / (2.0 * q)) * ldf<kShifted>(fin, i, n, k);
       } else if (bc.state[(int64_t)i * n + k] >= 0) {
         // the fluid node behind x, x + c_i
-        const int64_t src = wrapc(x + CX27[i], b.nx) +
-                            (int64_t)b.nx * (wrapc(y + CY27[i], b.ny) + (int64_t)b.ny * wrapc(z + CZ27[i], b.nz));
+        const int64_t src = base + wrapc(x + CX27[i], bnx) +
+                            (int64_t)bnx * (wrapc(y + CY27[i], bny) + (int64_t)bny * wrapc(z + CZ27[i], bnz));
         v = 2.0 * q * fk + (1.0 - 2.0 * q) * ldf<kShifted>(fin, kk, n, src);
       } else {
         v = fk;
@@ -658,17 +675,21 @@ hipError_t launch_d3q27_step_boxes(unsigned nn, const void* fin, void* fout, dou
                                    int keep_alpha, unsigned long long* stats, const StepArgs& a,
                                    const uint8_t* flags, const uint4* alias, const BoxTile* tiles,
                                    unsigned ntiles, const BoxDesc* boxes, int tile_nodes,
-                                   hipStream_t s) {
+                                   int links, hipStream_t s) {
   if (!ntiles) return hipSuccess;
   if (tile_nodes < 64 || tile_nodes > 256 || tile_nodes % 64) return hipErrorInvalidValue;
   const dim3 g(ntiles), blk((unsigned)tile_nodes);
+#define UCLBS_K27B(SH, KA, LK) \
+  k27_step_boxes<SH, KA, LK><<<g, blk, 0, s>>>(fin, fout, alpha, stats, nn, a, flags, alias, tiles, boxes, tile_nodes)
+#define UCLBS_K27B_LK(SH, KA) \
+  do { if (links) UCLBS_K27B(SH, KA, true); else UCLBS_K27B(SH, KA, false); } while (0)
   if (a.shifted) {
-    if (keep_alpha) k27_step_boxes<true, true><<<g, blk, 0, s>>>(fin, fout, alpha, stats, nn, a, flags, alias, tiles, boxes, tile_nodes);
-    else k27_step_boxes<true, false><<<g, blk, 0, s>>>(fin, fout, alpha, stats, nn, a, flags, alias, tiles, boxes, tile_nodes);
+    if (keep_alpha) UCLBS_K27B_LK(true, true); else UCLBS_K27B_LK(true, false);
   } else {
-    if (keep_alpha) k27_step_boxes<false, true><<<g, blk, 0, s>>>(fin, fout, alpha, stats, nn, a, flags, alias, tiles, boxes, tile_nodes);
-    else k27_step_boxes<false, false><<<g, blk, 0, s>>>(fin, fout, alpha, stats, nn, a, flags, alias, tiles, boxes, tile_nodes);
+    if (keep_alpha) UCLBS_K27B_LK(false, true); else UCLBS_K27B_LK(false, false);
   }
+#undef UCLBS_K27B_LK
+#undef UCLBS_K27B
   return hipGetLastError();
 }
 

@@ -491,6 +491,27 @@ hipError_t launch_build_links39s(const LinkGeom& g, int32_t* state, unsigned lon
   return hipGetLastError();
 }
 
+// valid nodes of a level of many boxes (flag kFlagInterior on entry); the
+// shell keeps its ghost / mirror flags
+__global__ void k_flags_boxes(const int32_t* __restrict__ state, int64_t n,
+                              uint8_t* __restrict__ flags) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n || flags[k] != kFlagInterior) return;
+  uint8_t fl = kFlagInterior;
+  if (state[k] < 0) {
+    fl = kFlagSolid;
+  } else {
+    for (int i = 1; i < 27; ++i)
+      if (state[(int64_t)i * n + k] < 0) fl = kFlagBoundary;
+  }
+  flags[k] = fl;
+}
+
+hipError_t launch_flags_boxes27(const int32_t* state, int64_t n, uint8_t* flags, hipStream_t s) {
+  k_flags_boxes<<<blocks_for(n, 256), 256, 0, s>>>(state, n, flags);
+  return hipGetLastError();
+}
+
 hipError_t launch_flags27(const int32_t* state, const Box& b, const int32_t face_code[6],
                           uint8_t* flags, hipStream_t s) {
   FaceCodes fc;

@@ -64,6 +64,7 @@ hipError_t launch_d3q27_step_periodic(const Box& b, const void* fin, void* fout,
 // Boundary data of a level whose links are defined: the reference-form link
 // table, the list of fluid nodes with a boundary link, and the equilibrium
 // populations of the inlets / outlets / freestream (full FP64 values).
+struct BoxDesc;
 struct BcArgs {
   const int32_t* state;   // [27][n]
   const uint8_t* flags;   // [n]: kFlag* values
@@ -73,6 +74,10 @@ struct BcArgs {
   const double* feq_in;   // [n_in][27]
   const double* feq_out;  // [n_out][27]
   const double* feq_fs;   // [27] or null
+  // a level of many boxes: the boxes and the box of each listed node (its
+  // neighbours lie in that box or its ghost shell); null for a single box
+  const BoxDesc* boxes;
+  const uint32_t* box_of;
 };
 
 // Node of the bounding-volume hierarchy over the triangles: bb is the exact
@@ -180,7 +185,11 @@ hipError_t launch_d3q27_step_boxes(unsigned nn, const void* fin, void* fout, dou
                                    int keep_alpha, unsigned long long* stats, const StepArgs& a,
                                    const uint8_t* flags, const uint4* alias, const BoxTile* tiles,
                                    unsigned ntiles, const BoxDesc* boxes, int tile_nodes,
-                                   hipStream_t s);
+                                   int links, hipStream_t s);
+// flags of the valid nodes of a level of many boxes from its link table:
+// solid (node code < 0), boundary (a link code < 0), interior; ghost-shell
+// flags are kept
+hipError_t launch_flags_boxes27(const int32_t* state, int64_t n, uint8_t* flags, hipStream_t s);
 // dst[i][p.x] = src[i][p.y] for each pair p and each population i in p.z
 // (bit i; ghost copies carry only the populations the box pulls from a ghost
 // node, injections all 27)

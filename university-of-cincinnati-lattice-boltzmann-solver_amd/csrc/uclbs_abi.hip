@@ -86,6 +86,7 @@ struct Level {
   int32_t* state = nullptr;   // [27][n]
   uint8_t* flags = nullptr;   // [n]
   uint32_t* blist = nullptr;  // boundary fluid nodes, ascending
+  uint32_t* bbox_of = nullptr;  // a level of many boxes: the box of each boundary node
   int64_t nb = 0, nsolid = 0;
   bool has_links = false;
   // D3Q39: second population, multipliers, repair list, step counter
@@ -277,6 +278,7 @@ void free_level(Level& L) {
   if (L.state) { (void)hipFree(L.state); L.state = nullptr; }
   if (L.flags) { (void)hipFree(L.flags); L.flags = nullptr; }
   if (L.blist) { (void)hipFree(L.blist); L.blist = nullptr; }
+  if (L.bbox_of) { (void)hipFree(L.bbox_of); L.bbox_of = nullptr; }
   for (auto& p : L.g)
     if (p) { (void)hipFree(p); p = nullptr; }
   if (L.lm) { (void)hipFree(L.lm); L.lm = nullptr; }
@@ -342,6 +344,8 @@ uclbs::BcArgs bc_args(const Ctx* c, const Level* L) {
   bc.feq_in = c->d_feq_in;
   bc.feq_out = c->d_feq_out;
   bc.feq_fs = c->d_feq_fs;
+  bc.boxes = L->multibox ? L->d_boxes : nullptr;
+  bc.box_of = L->bbox_of;
   return bc;
 }
 
@@ -629,12 +633,21 @@ int launch_steps_boxes(Ctx* c, Level* L, int nsteps, hipEvent_t* ev) {
     a.defer_next = L->defer_count + (1 - L->far_parity) * (uclbs::kFarShards + 1);
     HIP_TRY(uclbs::launch_copy_pairs27(L->f[L->cur], nn, L->f[L->cur], nn, L->d_copies, L->ncopies,
                                        a.shifted, c->stream));
+    if (L->state && L->nb) {
+      // boundary-list nodes on the side stream, concurrent with the box kernel
+      HIP_TRY(hipEventRecord(c->ev_fork, c->stream));
+      HIP_TRY(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+      HIP_TRY(uclbs::launch_d3q27_boundary(L->box, L->f[L->cur], L->f[1 - L->cur], L->alpha,
+                                           c->keep_alpha, c->d_stats, a, bc_args(c, L), c->side));
+      HIP_TRY(hipEventRecord(c->ev_join, c->side));
+    }
     if (ev) HIP_TRY(hipEventRecord(ev[2 * s], c->stream));
     HIP_TRY(uclbs::launch_d3q27_step_boxes(nn, L->f[L->cur], L->f[1 - L->cur], L->alpha,
                                            c->keep_alpha, c->d_stats, a, L->flags, L->d_alias,
                                            L->d_tiles, L->ntiles, L->d_boxes, L->tile_nodes,
-                                           c->stream));
+                                           L->state != nullptr, c->stream));
     if (ev) HIP_TRY(hipEventRecord(ev[2 * s + 1], c->stream));
+    if (L->state && L->nb) HIP_TRY(hipStreamWaitEvent(c->stream, c->ev_join, 0));
     if (a.shifted) {
       HIP_TRY(uclbs::launch_d3q27_far(L->box, L->f[1 - L->cur], L->alpha, c->keep_alpha,
                                       c->d_stats, a, c->stream));
@@ -975,6 +988,71 @@ static int64_t valid_index(const Level& L, const int G[3], int* box_out = nullpt
   return -1;
 }
 
+// A level of many boxes with a link table: flags of the valid nodes, the
+// boundary list with each node's box, and the ghost nodes that show a
+// boundary or solid node turned from mirrors (which repeat a plain update)
+// into copies of that node, made before each step.
+static int finalize_links_boxes(Ctx* c, Level* L) {
+  invalidate_graph(c);
+  const int64_t n = L->box.n;
+  HIP_TRY(uclbs::launch_flags_boxes27(L->state, n, L->flags, c->stream));
+  std::vector<uint8_t> hf((size_t)n);
+  HIP_TRY(hipMemcpyAsync(hf.data(), L->flags, (size_t)n, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  std::vector<uint3> copies;
+  std::vector<uint32_t> list, box_of;
+  int64_t nsolid = 0;
+  for (size_t b = 0; b < L->hbox.size(); ++b) {
+    const uclbs::BoxDesc& bx = L->hbox[b];
+    const unsigned nz = bx.n / (bx.nx * bx.ny);
+    for (unsigned z = 0; z < nz; ++z)
+      for (unsigned y = 0; y < bx.ny; ++y)
+        for (unsigned x = 0; x < bx.nx; ++x) {
+          const unsigned k = bx.off + x + bx.nx * (y + bx.ny * z);
+          const bool shell = x == 0 || y == 0 || z == 0 || x + 1 == bx.nx || y + 1 == bx.ny || z + 1 == nz;
+          if (!shell) {
+            if (hf[k] == uclbs::kFlagBoundary) {
+              list.push_back(k);
+              box_of.push_back((uint32_t)b);
+            }
+            nsolid += hf[k] == uclbs::kFlagSolid;
+            continue;
+          }
+          if (hf[k] != uclbs::kFlagMirror) continue;
+          const int G[3] = {L->blo[3 * b] - 1 + (int)x, L->blo[3 * b + 1] - 1 + (int)y,
+                            L->blo[3 * b + 2] - 1 + (int)z};
+          const int64_t src = valid_index(*L, G, nullptr);
+          if (src >= 0 && hf[(size_t)src] != uclbs::kFlagInterior) {
+            hf[k] = uclbs::kFlagGhost;
+            copies.push_back(make_uint3(k, (unsigned)src, uclbs::kAllPops27));
+          }
+        }
+  }
+  HIP_TRY(hipMemcpy(L->flags, hf.data(), (size_t)n, hipMemcpyHostToDevice));
+  if (L->d_copies) { HIP_TRY(hipFree(L->d_copies)); L->d_copies = nullptr; }
+  L->ncopies = (unsigned)copies.size();
+  if (!copies.empty()) {
+    HIP_TRY(hipMalloc(&L->d_copies, copies.size() * sizeof(uint3)));
+    HIP_TRY(hipMemcpy(L->d_copies, copies.data(), copies.size() * sizeof(uint3), hipMemcpyHostToDevice));
+  }
+  if (L->blist) { HIP_TRY(hipFree(L->blist)); L->blist = nullptr; }
+  if (L->bbox_of) { HIP_TRY(hipFree(L->bbox_of)); L->bbox_of = nullptr; }
+  L->nb = (int64_t)list.size();
+  L->nsolid = nsolid;
+  if (L->nb) {
+    HIP_TRY(hipMalloc(&L->blist, list.size() * sizeof(uint32_t)));
+    HIP_TRY(hipMemcpy(L->blist, list.data(), list.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&L->bbox_of, box_of.size() * sizeof(uint32_t)));
+    HIP_TRY(hipMemcpy(L->bbox_of, box_of.data(), box_of.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+  }
+  // the far queue's last shard takes the boundary nodes
+  if (L->defer_list) { HIP_TRY(hipFree(L->defer_list)); L->defer_list = nullptr; }
+  HIP_TRY(hipMalloc(&L->defer_list,
+                    ((size_t)L->defer_cap * uclbs::kFarShards + (size_t)L->nb + 1) * sizeof(uint2)));
+  L->has_links = true;
+  return UCLBS_OK;
+}
+
 // coarse stencil of fine position G: per axis the coincident coarse node
 // (G even) or the two neighbours with weight 1/2 (G odd), tensor order z, y, x
 // -- the order k27_coarse_to_fine sums in
@@ -1037,8 +1115,6 @@ int uclbs_define_level_boxes(void* ctx, int lvl, int nbox, const int32_t* lo, co
   if (rc) return rc;
   if (lvl > 0 && (lvl - 1 >= (int)c->levels.size() || !c->levels[lvl - 1].defined))
     return set_err(UCLBS_E_STATE, "define level %d before level %d", lvl - 1, lvl);
-  if (lvl > 0 && c->levels[lvl - 1].has_links && !c->levels[lvl - 1].refined)
-    return set_err(UCLBS_E_UNSUPPORTED, "refined levels nest in a level without link tables");
   // validity and disjointness of the valid regions
   int64_t vol = 0, stored = 0;
   for (int b = 0; b < nbox; ++b)
@@ -1445,8 +1521,9 @@ int uclbs_build_links(void* ctx, int lvl, const double origin[3], double dx, int
   Level* L;
   int rc = check_level(c, lvl, &L);
   if (rc) return rc;
-  if (L->multibox) return set_err(UCLBS_E_UNSUPPORTED, "no link tables on levels of many boxes");
   const bool q39 = c->cfg.lattice == UCLBS_LATTICE_D3Q39;
+  if (L->multibox && L->box.per[0])
+    return set_err(UCLBS_E_UNSUPPORTED, "no link tables on a periodic level of many boxes");
   if (L->refined && q39)
     return set_err(UCLBS_E_UNSUPPORTED, "level %d is a refined D3Q39 level (no link tables)", lvl);
   if (!origin || !(dx > 0.0) || ntri < 0 || ntri > (1 << 26) || (ntri && (!tri || !nrm)))
@@ -1510,9 +1587,39 @@ int uclbs_build_links(void* ctx, int lvl, const double origin[3], double dx, int
     g.tbb = dtbb;
     g.bvh = dbvh;
     g.bvh_idx = dbidx;
-    e = !q39 ? uclbs::launch_build_links27(g, L->state, dcnt, c->stream)
-        : c->shifted39 ? uclbs::launch_build_links39s(g, L->state, dcnt, c->stream)
-                       : uclbs::launch_build_links39(g, L->state, dcnt, c->stream);
+    if (L->multibox) {
+      // box by box: the box's stored nodes (ghost shell included) at level
+      // position blo - 1 + (a, b, c), i.e. origin + (blo - 1 + (a, b, c)) dx;
+      // built into a scratch table and placed at the box's offset
+      size_t maxn = 0;
+      for (const uclbs::BoxDesc& bx : L->hbox) maxn = std::max<size_t>(maxn, bx.n);
+      int32_t* tmp = nullptr;
+      e = hipMalloc(&tmp, maxn * 27 * sizeof(int32_t));
+      for (size_t b = 0; b < L->hbox.size() && e == hipSuccess; ++b) {
+        const uclbs::BoxDesc& bx = L->hbox[b];
+        uclbs::LinkGeom gb = g;
+        gb.box.nx = (int)bx.nx;
+        gb.box.ny = (int)bx.ny;
+        gb.box.nz = (int)(bx.n / (bx.nx * bx.ny));
+        gb.box.n = bx.n;
+        for (int d = 0; d < 3; ++d) {
+          gb.box.per[d] = 1;  // a valid node's links stay in its stored box
+          gb.origin[d] = origin[d] + (double)(L->blo[3 * b + d] - 1) * dx;
+        }
+        for (int f = 0; f < 6; ++f) gb.face_code[f] = uclbs::kFacePeriodic;
+        e = uclbs::launch_build_links27(gb, tmp, dcnt, c->stream);
+        if (e == hipSuccess)
+          e = hipMemcpy2DAsync(L->state + bx.off, (size_t)n * sizeof(int32_t), tmp,
+                               (size_t)bx.n * sizeof(int32_t), (size_t)bx.n * sizeof(int32_t), 27,
+                               hipMemcpyDeviceToDevice, c->stream);
+      }
+      if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+      if (tmp) (void)hipFree(tmp);
+    } else {
+      e = !q39 ? uclbs::launch_build_links27(g, L->state, dcnt, c->stream)
+          : c->shifted39 ? uclbs::launch_build_links39s(g, L->state, dcnt, c->stream)
+                         : uclbs::launch_build_links39(g, L->state, dcnt, c->stream);
+    }
   }
   if (e == hipSuccess) e = hipMemcpyAsync(hc, dcnt, sizeof hc, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -1528,7 +1635,7 @@ int uclbs_build_links(void* ctx, int lvl, const double origin[3], double dx, int
     L->near_dirty = true;
     L->nsolid = (int64_t)hc[0];
     invalidate_graph(c);
-  } else if ((rc = finalize_links(c, L))) {
+  } else if ((rc = L->multibox ? finalize_links_boxes(c, L) : finalize_links(c, L))) {
     return rc;
   }
   if (n_solid) *n_solid = (int64_t)hc[0];
