@@ -324,7 +324,8 @@ int uclbs_fine_to_coarse(void *ctx, int lvl);
  * needs its coarse stencil inside level l-1's valid nodes), and
  * uclbs_fine_to_coarse injects the level's even nodes.  Host arrays of the
  * level (uclbs_upload_f, uclbs_download_f, uclbs_download_macro) are in this
- * stored order.  D3Q27, plain or fixed alpha; no link tables on such levels. */
+ * stored order.  D3Q27.  Link tables: uclbs_build_links builds them box by
+ * box (the level's origin is the position of index 0). */
 int uclbs_define_level_boxes(void *ctx, int lvl, int nbox, const int32_t *lo, const int32_t *hi);
 
 /* Advances level 0 by ncoarse steps and every finer level by two substeps per
@@ -473,6 +474,56 @@ int uclbs_mic_flush(void *ctx);
 /* corners (level positions pos[8][3]), weights and stored nodes (-1: not a
  * fluid node) of microphone i (0-based) */
 int uclbs_mic_weights(void *ctx, int i, int32_t pos[24], double w[8], int64_t node[8]);
+
+
+/* ---- regrid -----------------------------------------------------------------
+ * The dynamic half of the reference's AMR (regrid_question_mark.f95:38-184,
+ * calling AMReX's regrid with the callbacks tag_nodes
+ * (amr_processes.f95:549-664) and remake (amr_processes.f95:670-1080)):
+ * tag a level by its GGM indicator, cluster the tags into boxes, remake the
+ * next finer level on those boxes, then re-identify its nodes by building
+ * its link table again (uclbs_build_links; the reference's
+ * amr_code/new_node_values). */
+
+/* Replaces remake (amr_processes.f95:670-1080): level lvl (>= 1, D3Q27) is
+ * defined again as the nbox boxes lo/hi (as uclbs_define_level_boxes), every
+ * node interpolated from level lvl-1 as uclbs_init_from_coarse does, then
+ * every node at a position where the old level had a valid node takes the
+ * old node's populations (both buffers).  The step counter carries over.
+ * Finer levels become stale (stepping, filling or injecting them fails) until
+ * each is regridded in turn, coarse to fine, or cleared.  When the old level
+ * had a link table the new one must get its own (uclbs_build_links or
+ * uclbs_set_links) before it steps.  On an error the old level stays. */
+int uclbs_regrid_level(void *ctx, int lvl, int nbox, const int32_t *lo, const int32_t *hi);
+/* nodes the last uclbs_regrid_level copied from the old level */
+int uclbs_regrid_copied(void *ctx, int64_t *n);
+/* Replaces clear_lvl (amr_processes.f95:478-545): frees level lvl (>= 1) and
+ * every finer level. */
+int uclbs_clear_level(void *ctx, int lvl);
+/* Replaces ggm_calculator (GGM/ggm_calculator.f95:47-68: populate_q,
+ * calculate_grad_mag, calculate_gradgrad_mag) on a one-box D3Q27 level: the
+ * magnitude of the gradient of the gradient magnitude of a quantity
+ * (1 density, 2 u, 3 v, 4 w, 7 velocity magnitude) from the current
+ * populations, each derivative by derivative_windage_secondord
+ * (derivatives.f95:644-700; scheme 0 central, 1 upwind, 2 downwind by the
+ * node's velocity), stencil nodes usable where the node's link codes 1+a,
+ * 4+a, 15+a, 18+a are >= 0 (every in-box node without a link table).  ggm
+ * (may be NULL): [n] stored nodes; tags (may be NULL): 1 on the level's valid
+ * nodes with ggm >= limit (calculate_gradgrad_mag.f95:67-71), 0 elsewhere;
+ * ntagged (may be NULL): their count. */
+int uclbs_tag_ggm(void *ctx, int lvl, int quantity, int scheme, double limit, double *ggm,
+                  uint8_t *tags, int64_t *ntagged);
+/* Boxes over tagged nodes (AMReX's cluster step between tag_nodes and
+ * remake; host code, no device): tags [z][y][x] over lo..hi inclusive,
+ * nonzero = tagged.  Tags grow by buffer nodes, coarsen by blocking, cluster
+ * by Berger-Rigoutsos signatures until each box is at least efficiency
+ * tagged, refine back, clip to lo..hi and split to at most max_size nodes per
+ * axis (0: no limit; else a multiple of blocking).  Writes *nbox boxes
+ * (disjoint, covering every grown tag) to box_lo/box_hi [nbox][3];
+ * UCLBS_E_ARG with *nbox set when more than max_boxes are needed. */
+int uclbs_cluster_tags(const uint8_t *tags, const int32_t lo[3], const int32_t hi[3],
+                       double efficiency, int blocking, int buffer, int max_size, int max_boxes,
+                       int32_t *box_lo, int32_t *box_hi, int *nbox);
 
 #ifdef __cplusplus
 }

@@ -35,7 +35,8 @@ EXPORTS = (
     "uclbs_init_from_coarse", "uclbs_fine_to_coarse", "uclbs_define_level_boxes",
     "uclbs_advance", "uclbs_autotune", "uclbs_mlp_forward_noalloc", "uclbs_set_placement",
     "uclbs_plot3d_write", "uclbs_output_wait", "uclbs_mic_setup", "uclbs_mic_sample",
-    "uclbs_mic_flush", "uclbs_mic_weights",
+    "uclbs_mic_flush", "uclbs_mic_weights", "uclbs_regrid_level", "uclbs_regrid_copied",
+    "uclbs_clear_level", "uclbs_tag_ggm", "uclbs_cluster_tags",
 )
 MLP_PARAMS = 9217
 FACE_PERIODIC = 2147483647
@@ -163,6 +164,16 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         "uclbs_mlp_train": (I, [P, I64, I, D, U64, dp, dp]),
         "uclbs_interlevel": (I, [P, I, D]),
         "uclbs_init_from_coarse": (I, [P, I]),
+        "uclbs_regrid_level": (I, [P, I, I, ctypes.POINTER(ctypes.c_int32),
+                                   ctypes.POINTER(ctypes.c_int32)]),
+        "uclbs_regrid_copied": (I, [P, ctypes.POINTER(I64)]),
+        "uclbs_clear_level": (I, [P, I]),
+        "uclbs_tag_ggm": (I, [P, I, I, I, D, dp, ctypes.POINTER(ctypes.c_uint8),
+                              ctypes.POINTER(I64)]),
+        "uclbs_cluster_tags": (I, [ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(ctypes.c_int32),
+                                   ctypes.POINTER(ctypes.c_int32), D, I, I, I, I,
+                                   ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                   ctypes.POINTER(I)]),
         "uclbs_fine_to_coarse": (I, [P, I]),
         "uclbs_set_placement": (I, [P, dp, dp]),
         "uclbs_plot3d_write": (I, [P, ctypes.c_char_p, ctypes.POINTER(Plot3DHeader)]),
@@ -238,6 +249,32 @@ def copy_bandwidth(nbytes: int, reps: int = 10) -> float:
     return g.value
 
 
+def cluster_tags(tags: np.ndarray, lo=(0, 0, 0), efficiency: float = 0.7, blocking: int = 1,
+                 buffer: int = 0, max_size: int = 0):
+    """uclbs_cluster_tags: boxes [(lo, hi), ...] (inclusive) over the nonzero
+    entries of tags [z, y, x], whose node (0, 0, 0) sits at position lo."""
+    lib = load_library()
+    t = np.ascontiguousarray(tags, dtype=np.uint8)
+    nz, ny, nx = t.shape
+    lo3 = np.ascontiguousarray(lo, np.int32)
+    hi3 = np.ascontiguousarray([lo[0] + nx - 1, lo[1] + ny - 1, lo[2] + nz - 1], np.int32)
+    cap = 64
+    while True:
+        blo = np.zeros(3 * cap, np.int32)
+        bhi = np.zeros(3 * cap, np.int32)
+        nb = ctypes.c_int()
+        rc = lib.uclbs_cluster_tags(t.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), _iptr(lo3),
+                                    _iptr(hi3), float(efficiency), int(blocking), int(buffer),
+                                    int(max_size), cap, _iptr(blo), _iptr(bhi), ctypes.byref(nb))
+        if rc == 0:
+            break
+        if nb.value <= cap:
+            raise UclbsError(f"cluster_tags failed ({rc}): {lib.uclbs_last_error().decode()}")
+        cap = nb.value
+    return [(tuple(int(v) for v in blo[3 * b:3 * b + 3]), tuple(int(v) for v in bhi[3 * b:3 * b + 3]))
+            for b in range(nb.value)]
+
+
 class Solver:
     """One library context (one device stream, resident level state)."""
 
@@ -283,6 +320,9 @@ class Solver:
         self._check(self.lib.uclbs_define_level(self.ctx, lvl, lo3, hi3, per))
         g = (8 if self.q == 39 else 2) if lvl > 0 else 0
         self.shapes[lvl] = tuple(int(s) + g for s in shape)
+        self.places = getattr(self, "places", {})
+        self.places[lvl] = (tuple(int(v) for v in lo), tuple(int(v) for v in shape),
+                            tuple(int(p) for p in periodic))
 
     def define_level_boxes(self, lvl: int, boxes):
         """boxes: [(lo, hi), ...] valid nodes inclusive in the level's index
@@ -293,10 +333,87 @@ class Solver:
         pi = ctypes.POINTER(ctypes.c_int32)
         self._check(self.lib.uclbs_define_level_boxes(self.ctx, lvl, len(boxes),
                                                       lo.ctypes.data_as(pi), hi.ctypes.data_as(pi)))
+        self._set_boxes(lvl, boxes)
+
+    def _set_boxes(self, lvl: int, boxes):
         self.boxes = getattr(self, "boxes", {})
         self.boxes[lvl] = [(tuple(int(v) for v in b[0]), tuple(int(v) for v in b[1])) for b in boxes]
         n = sum(int(np.prod([h - l + 3 for l, h in zip(*b)])) for b in self.boxes[lvl])
         self.shapes[lvl] = (n, 1, 1)
+        getattr(self, "places", {}).pop(lvl, None)
+
+    # ---- regrid ----------------------------------------------------------------
+    def regrid_level(self, lvl: int, boxes) -> int:
+        """uclbs_regrid_level: level lvl remade on boxes [(lo, hi), ...] (AMReX
+        remake); returns the number of nodes taken from the old level.  Finer
+        levels become stale until regridded or cleared; a level that had a link
+        table needs build_links again before it steps."""
+        lo = np.ascontiguousarray([b[0] for b in boxes], np.int32).reshape(-1)
+        hi = np.ascontiguousarray([b[1] for b in boxes], np.int32).reshape(-1)
+        pi = ctypes.POINTER(ctypes.c_int32)
+        self._check(self.lib.uclbs_regrid_level(self.ctx, lvl, len(boxes), lo.ctypes.data_as(pi),
+                                                hi.ctypes.data_as(pi)))
+        self._set_boxes(lvl, boxes)
+        n = ctypes.c_int64()
+        self._check(self.lib.uclbs_regrid_copied(self.ctx, ctypes.byref(n)))
+        return n.value
+
+    def clear_level(self, lvl: int):
+        """uclbs_clear_level: drop level lvl and every finer level."""
+        self._check(self.lib.uclbs_clear_level(self.ctx, lvl))
+        for d in (self.shapes, getattr(self, "boxes", {}), getattr(self, "places", {})):
+            for k in [k for k in d if k >= lvl]:
+                del d[k]
+
+    def tag_ggm(self, lvl: int, quantity: int = 1, scheme: int = 0, limit: float = 0.04):
+        """uclbs_tag_ggm on a one-box level: (ggm [n], tags [n] uint8, count),
+        both in the level's stored order."""
+        n = self.n_nodes(lvl)
+        ggm = np.empty(n)
+        tags = np.empty(n, np.uint8)
+        cnt = ctypes.c_int64()
+        self._check(self.lib.uclbs_tag_ggm(self.ctx, lvl, quantity, scheme, float(limit), _dptr(ggm),
+                                           tags.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                           ctypes.byref(cnt)))
+        return ggm, tags, cnt.value
+
+    def refine_by_ggm(self, lvl: int, quantity: int = 1, scheme: int = 0, limit: float = 0.04,
+                      efficiency: float = 0.7, blocking: int = 2, buffer: int = 2,
+                      max_size: int = 32):
+        """One regrid of level lvl (>= 1) from the GGM tags of the one-box
+        level lvl-1 (regrid_question_mark.f95:38-184 through AMReX's regrid):
+        tag, cluster within level lvl-1's valid nodes one node in from its
+        non-periodic faces (the new level's ghost shell must nest), refine the
+        boxes by 2, then regrid level lvl (define and fill it from level lvl-1
+        when it does not exist yet).  No tags: level lvl and finer are
+        cleared.  Returns the new boxes (level lvl's index space)."""
+        if lvl - 1 not in getattr(self, "places", {}):
+            raise UclbsError(f"refine_by_ggm: level {lvl - 1} must be a one-box level")
+        clo, cshape, cper = self.places[lvl - 1]
+        g = 2 if lvl - 1 > 0 else 0
+        _, tags, cnt = self.tag_ggm(lvl - 1, quantity, scheme, limit)
+        st = tags.reshape(tuple(v + g for v in cshape)[::-1])
+        if g:
+            st = st[1:-1, 1:-1, 1:-1]
+        inset = [0 if cper[d] else 1 for d in range(3)]
+        rlo = [clo[d] + inset[d] for d in range(3)]
+        rhi = [clo[d] + cshape[d] - 1 - inset[d] for d in range(3)]
+        region = np.ascontiguousarray(st[inset[2]:cshape[2] - inset[2], inset[1]:cshape[1] - inset[1],
+                                         inset[0]:cshape[0] - inset[0]])
+        boxes = cluster_tags(region, rlo, efficiency=efficiency, blocking=blocking, buffer=buffer,
+                             max_size=max_size) if region.any() else []
+        assert rhi == [rlo[d] + region.shape[2 - d] - 1 for d in range(3)]
+        if not boxes:
+            if lvl in self.shapes:
+                self.clear_level(lvl)
+            return []
+        fine = [(tuple(2 * v for v in lo), tuple(2 * v for v in hi)) for lo, hi in boxes]
+        if lvl in self.shapes:
+            self.regrid_level(lvl, fine)
+        else:
+            self.define_level_boxes(lvl, fine)
+            self.init_from_coarse(lvl)
+        return fine
 
     def box_layout(self, lvl: int):
         """(offset, stored extents (x, y, z), lo of the valid region) per box."""

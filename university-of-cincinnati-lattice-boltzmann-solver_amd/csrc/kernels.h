@@ -333,6 +333,12 @@ hipError_t launch_blank_gather(const int32_t* code, int ghost, const Box& b, int
 hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t s);
 hipError_t launch_d3q27_probe(const void* f, int64_t n, int shifted, const uint32_t* nodes,
                               unsigned m, double* out, hipStream_t s);
+// gradient-of-gradient-magnitude indicator of a one-box D3Q27 level (scheme 0
+// central, 1 upwind, 2 downwind); work 5 n doubles, ggm n doubles, tags n bytes
+hipError_t launch_d3q27_ggm(const Box& b, const void* f, int shifted, const int32_t* state,
+                            int quantity, int scheme, double limit, const int vlo[3],
+                            const int vhi[3], double* work, double* ggm, uint8_t* tags,
+                            hipStream_t s);
 hipError_t launch_d3q39_probe(const void* f, const void* g, int64_t n, int fp32, int shifted,
                               const Ref39& rf, double cv, const uint32_t* nodes, unsigned m,
                               double* out, hipStream_t s);

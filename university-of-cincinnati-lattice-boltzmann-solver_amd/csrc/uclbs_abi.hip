@@ -17,6 +17,7 @@
 #include "kernels.h"
 #include "lattice.h"
 #include "philox.h"
+#include "cluster.h"
 
 // Staging area of uclbs_mlp_forward_noalloc: the features (12 floats per
 // row) and the network outputs (1 float per row) of at most
@@ -114,6 +115,8 @@ struct Level {
   uint32_t* ghosts = nullptr;
   unsigned nghost = 0;
   bool refined = false;
+  bool stale = false;   // its coarser level was regridded: regrid or define it again
+  bool relink = false;  // regridded from a level with a link table: build or set its links
   // a level of many boxes (uclbs_define_level_boxes)
   bool multibox = false;
   std::vector<int32_t> blo, bhi;  // valid region of each box, [nbox][3], level index space
@@ -230,6 +233,7 @@ struct Ctx {
   size_t mic_stride = 0;  // doubles per sample
   int mic_cap = 0, mic_count = 0;
   int64_t mic_structure = -1;
+  int64_t regrid_copied = 0;  // nodes the last uclbs_regrid_level took from the old level
 };
 
 void release_output(Ctx* c);  // joins the writer thread, frees the output buffers
@@ -423,6 +427,7 @@ int finalize_links(Ctx* c, Level* L) {
                       hipMemcpyHostToDevice));
   }
   L->has_links = true;
+  L->relink = false;
   return UCLBS_OK;
 }
 
@@ -660,6 +665,12 @@ int launch_steps_boxes(Ctx* c, Level* L, int nsteps, hipEvent_t* ev) {
 }
 
 int launch_steps(Ctx* c, Level* L, int nsteps, hipEvent_t* ev = nullptr) {
+  if (L->stale)
+    return set_err(UCLBS_E_STATE, "level %d: its coarser level was regridded (regrid or define it "
+                                  "again)", (int)(L - c->levels.data()));
+  if (L->relink)
+    return set_err(UCLBS_E_STATE, "level %d was regridded from a level with a link table: build "
+                                  "or set its links", (int)(L - c->levels.data()));
   if (c->cfg.lattice == UCLBS_LATTICE_D3Q39) return launch_steps39(c, L, nsteps, ev);
   if (L->multibox) return launch_steps_boxes(c, L, nsteps, ev);
   uclbs::StepArgs a = step_args(c, (int)(L - c->levels.data()));
@@ -1050,6 +1061,7 @@ static int finalize_links_boxes(Ctx* c, Level* L) {
   HIP_TRY(hipMalloc(&L->defer_list,
                     ((size_t)L->defer_cap * uclbs::kFarShards + (size_t)L->nb + 1) * sizeof(uint2)));
   L->has_links = true;
+  L->relink = false;
   return UCLBS_OK;
 }
 
@@ -2134,6 +2146,9 @@ static int interlevel_fill(Ctx* c, int lvl, double theta, bool all_nodes) {
   Level* F;
   int rc = check_level(c, lvl, &F);
   if (rc) return rc;
+  if (F->stale)
+    return set_err(UCLBS_E_STATE, "level %d: its coarser level was regridded (regrid or define it "
+                                  "again)", lvl);
   if (!F->refined) return set_err(UCLBS_E_ARG, "level %d is not a refined level", lvl);
   if (!(theta >= 0.0 && theta <= 1.0)) return set_err(UCLBS_E_ARG, "theta outside [0, 1]");
   if ((rc = set_device(c))) return rc;
@@ -2223,6 +2238,9 @@ int uclbs_fine_to_coarse(void* ctx, int lvl) {
   int rc = check_level(c, lvl, &F);
   if (rc) return rc;
   if (!F->refined) return set_err(UCLBS_E_ARG, "level %d is not a refined level", lvl);
+  if (F->stale)
+    return set_err(UCLBS_E_STATE, "level %d: its coarser level was regridded (regrid or define it "
+                                  "again)", lvl);
   if ((rc = set_device(c))) return rc;
   Level& C = c->levels[lvl - 1];
   if (F->multibox)
@@ -2241,6 +2259,164 @@ int uclbs_fine_to_coarse(void* ctx, int lvl) {
                                                  c->stream));
   HIP_TRY(uclbs::launch_fine_to_coarse27(F->f[F->cur], level_map(*F), C.f[C.cur], level_map(C), cover,
                                          c->cfg.precision == UCLBS_PREC_FP32_SHIFTED, c->stream));
+  return UCLBS_OK;
+}
+
+int uclbs_regrid_level(void* ctx, int lvl, int nbox, const int32_t* lo, const int32_t* hi) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  Level* F;
+  int rc = check_level(c, lvl, &F);
+  if (rc) return rc;
+  if (c->cfg.lattice != UCLBS_LATTICE_D3Q27)
+    return set_err(UCLBS_E_UNSUPPORTED, "regrid: D3Q27 levels (levels of many boxes are D3Q27)");
+  if (!F->refined) return set_err(UCLBS_E_ARG, "regrid: level %d is not a refined level", lvl);
+  if ((rc = set_device(c))) return rc;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  // The old level steps aside; the new one is defined in its place (its
+  // interpolation and injection tables against the coarser level as it is
+  // now) and filled as AMReX's remake fills a regridded level
+  // (amr_processes.f95:670-1080): interpolated from the coarser level, then
+  // the old level's valid nodes copied wherever the new level has a node at
+  // the same position.  A stale level (its coarser level regridded first)
+  // regrids the same way: its own data is still good.
+  Level old = c->levels[lvl];
+  c->levels[lvl] = Level();
+  rc = uclbs_define_level_boxes(ctx, lvl, nbox, lo, hi);
+  if (!rc) rc = uclbs_init_from_coarse(ctx, lvl);
+  if (rc) {
+    free_level(c->levels[lvl]);
+    c->levels[lvl] = old;
+    return rc;
+  }
+  Level& N = c->levels[lvl];
+  std::vector<uint3> pairs;
+  for (size_t b = 0; b < N.hbox.size(); ++b) {
+    const uclbs::BoxDesc& bx = N.hbox[b];
+    const unsigned nz = bx.n / (bx.nx * bx.ny);
+    for (unsigned z = 0; z < nz; ++z)
+      for (unsigned y = 0; y < bx.ny; ++y)
+        for (unsigned x = 0; x < bx.nx; ++x) {
+          const int G[3] = {N.blo[3 * b] - 1 + (int)x, N.blo[3 * b + 1] - 1 + (int)y,
+                            N.blo[3 * b + 2] - 1 + (int)z};
+          const int64_t src = valid_index(old, G);
+          if (src >= 0)
+            pairs.push_back(make_uint3(bx.off + x + bx.nx * (y + bx.ny * z), (unsigned)src,
+                                       uclbs::kAllPops27));
+        }
+  }
+  // {dst (new level), src (old level), populations}, into both buffers
+  uint3* dp = nullptr;
+  hipError_t e = hipSuccess;
+  if (!pairs.empty()) {
+    e = hipMalloc(&dp, pairs.size() * sizeof(uint3));
+    if (e == hipSuccess)
+      e = hipMemcpy(dp, pairs.data(), pairs.size() * sizeof(uint3), hipMemcpyHostToDevice);
+    const int sh = c->cfg.precision == UCLBS_PREC_FP32_SHIFTED;
+    for (int k = 0; k < 2 && e == hipSuccess; ++k)
+      e = uclbs::launch_copy_pairs27(old.f[old.cur], (unsigned)old.box.n,
+                                     N.f[k == 0 ? N.cur : 1 - N.cur], (unsigned)N.box.n, dp,
+                                     (unsigned)pairs.size(), sh, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  }
+  if (dp) (void)hipFree(dp);
+  N.step = old.step;
+  N.relink = old.state != nullptr;
+  c->regrid_copied = (int64_t)pairs.size();
+  free_level(old);
+  if (e != hipSuccess) return set_err(UCLBS_E_HIP, "regrid: %s", hipGetErrorString(e));
+  // a finer level's interpolation and injection tables point into the old level
+  for (int l = lvl + 1; l < (int)c->levels.size() && c->levels[l].defined; ++l) c->levels[l].stale = true;
+  invalidate_graph(c);
+  return UCLBS_OK;
+}
+
+int uclbs_regrid_copied(void* ctx, int64_t* n) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || !n) return set_err(UCLBS_E_ARG, "null argument");
+  *n = c->regrid_copied;
+  return UCLBS_OK;
+}
+
+int uclbs_clear_level(void* ctx, int lvl) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  Level* L;
+  int rc = check_level(c, lvl, &L);
+  if (rc) return rc;
+  if (lvl == 0) return set_err(UCLBS_E_ARG, "clear_level: level 0 stays (destroy the context)");
+  if ((rc = set_device(c))) return rc;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (int l = (int)c->levels.size() - 1; l >= lvl; --l) free_level(c->levels[l]);
+  c->levels.resize(lvl);
+  invalidate_graph(c);
+  return UCLBS_OK;
+}
+
+int uclbs_tag_ggm(void* ctx, int lvl, int quantity, int scheme, double limit, double* ggm,
+                  uint8_t* tags, int64_t* ntagged) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  Level* L;
+  int rc = check_level(c, lvl, &L);
+  if (rc) return rc;
+  if (c->cfg.lattice != UCLBS_LATTICE_D3Q27)
+    return set_err(UCLBS_E_UNSUPPORTED, "tag_ggm: D3Q27 levels");
+  if (L->multibox)
+    return set_err(UCLBS_E_UNSUPPORTED, "tag_ggm: one-box levels (level %d has many boxes)", lvl);
+  if (quantity != 1 && quantity != 2 && quantity != 3 && quantity != 4 && quantity != 7)
+    return set_err(UCLBS_E_ARG, "tag_ggm: quantity %d (1 density, 2 u, 3 v, 4 w, 7 |u|)", quantity);
+  if (scheme < 0 || scheme > 2)
+    return set_err(UCLBS_E_ARG, "tag_ggm: scheme %d (0 central, 1 upwind, 2 downwind)", scheme);
+  if ((rc = set_device(c))) return rc;
+  const int64_t n = L->box.n;
+  int vlo[3], vhi[3];
+  for (int d = 0; d < 3; ++d) {
+    vlo[d] = L->lo[d] - L->origin[d];
+    vhi[d] = L->hi[d] - L->origin[d];
+  }
+  double* work = nullptr;
+  uint8_t* dtag = nullptr;
+  hipError_t e = hipMalloc(&work, (size_t)n * 6 * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&dtag, (size_t)n);
+  if (e == hipSuccess)
+    e = uclbs::launch_d3q27_ggm(L->box, L->f[L->cur], c->cfg.precision == UCLBS_PREC_FP32_SHIFTED,
+                                L->state, quantity, scheme, limit, vlo, vhi, work, work + 5 * n,
+                                dtag, c->stream);
+  if (e == hipSuccess && ggm)
+    e = hipMemcpyAsync(ggm, work + 5 * n, (size_t)n * sizeof(double), hipMemcpyDeviceToHost,
+                       c->stream);
+  std::vector<uint8_t> ht;
+  if (e == hipSuccess && (tags || ntagged)) {
+    ht.resize((size_t)n);
+    e = hipMemcpyAsync(ht.data(), dtag, (size_t)n, hipMemcpyDeviceToHost, c->stream);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (work) (void)hipFree(work);
+  if (dtag) (void)hipFree(dtag);
+  if (e != hipSuccess) return set_err(UCLBS_E_HIP, "tag_ggm: %s", hipGetErrorString(e));
+  if (tags) std::memcpy(tags, ht.data(), (size_t)n);
+  if (ntagged) {
+    int64_t t = 0;
+    for (uint8_t v : ht) t += v;
+    *ntagged = t;
+  }
+  return UCLBS_OK;
+}
+
+int uclbs_cluster_tags(const uint8_t* tags, const int32_t lo[3], const int32_t hi[3],
+                       double efficiency, int blocking, int buffer, int max_size, int max_boxes,
+                       int32_t* box_lo, int32_t* box_hi, int* nbox) {
+  if (!tags || !lo || !hi || !nbox) return set_err(UCLBS_E_ARG, "null argument");
+  std::vector<uclbs::ClusterBox> out;
+  const char* msg = uclbs::cluster_tags(tags, lo, hi, efficiency, blocking, buffer, max_size, out);
+  if (msg) return set_err(UCLBS_E_ARG, "cluster_tags: %s", msg);
+  *nbox = (int)out.size();
+  if ((int)out.size() > max_boxes)
+    return set_err(UCLBS_E_ARG, "cluster_tags: %d boxes, room for %d", (int)out.size(), max_boxes);
+  if (!out.empty() && (!box_lo || !box_hi)) return set_err(UCLBS_E_ARG, "null box arrays");
+  for (size_t b = 0; b < out.size(); ++b)
+    for (int d = 0; d < 3; ++d) {
+      box_lo[3 * b + d] = out[b].lo[d];
+      box_hi[3 * b + d] = out[b].hi[d];
+    }
   return UCLBS_OK;
 }
 

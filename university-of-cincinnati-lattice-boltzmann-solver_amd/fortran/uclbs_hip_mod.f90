@@ -51,7 +51,8 @@ module uclbs_hip
   end type
 
   public :: uclbs_set_placement, uclbs_plot3d_write, uclbs_output_wait, uclbs_mic_setup, &
-            uclbs_mic_sample, uclbs_mic_flush
+            uclbs_mic_sample, uclbs_mic_flush, uclbs_regrid_level, uclbs_clear_level, &
+            uclbs_tag_ggm, uclbs_cluster_tags
   public :: uclbs_create, uclbs_destroy, uclbs_define_level, uclbs_upload_f, &
             uclbs_download_f, uclbs_download_macro, uclbs_set_keep_alpha, &
             uclbs_download_alpha, uclbs_init_tgv, uclbs_init_uniform, uclbs_set_faces, &
@@ -353,6 +354,45 @@ module uclbs_hip
     function uclbs_mic_flush(ctx) bind(c, name='uclbs_mic_flush') result(rc)
       import :: c_ptr, c_int
       type(c_ptr), value :: ctx
+      integer(c_int) :: rc
+    end function
+    ! remake (amr_processes.f95:670-1080): lo, hi (3, nbox) level indices
+    function uclbs_regrid_level(ctx, lvl, nbox, lo, hi) bind(c, name='uclbs_regrid_level') result(rc)
+      import :: c_ptr, c_int, c_int32_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: lvl, nbox
+      integer(c_int32_t), intent(in) :: lo(*), hi(*)
+      integer(c_int) :: rc
+    end function
+    ! clear_lvl (amr_processes.f95:478-545)
+    function uclbs_clear_level(ctx, lvl) bind(c, name='uclbs_clear_level') result(rc)
+      import :: c_ptr, c_int
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: lvl
+      integer(c_int) :: rc
+    end function
+    ! ggm_calculator (ggm_calculator.f95:47-68); ggm, tags over the stored nodes
+    function uclbs_tag_ggm(ctx, lvl, quantity, scheme, limit, ggm, tags, ntagged) &
+        bind(c, name='uclbs_tag_ggm') result(rc)
+      import :: c_ptr, c_int, c_double, c_int8_t, c_int64_t
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: lvl, quantity, scheme
+      real(c_double), value :: limit
+      real(c_double), intent(out) :: ggm(*)
+      integer(c_int8_t), intent(out) :: tags(*)
+      integer(c_int64_t), intent(out) :: ntagged
+      integer(c_int) :: rc
+    end function
+    ! boxes over tags(lo(1):hi(1), lo(2):hi(2), lo(3):hi(3)) (host only)
+    function uclbs_cluster_tags(tags, lo, hi, efficiency, blocking, buffer, max_size, max_boxes, &
+        box_lo, box_hi, nbox) bind(c, name='uclbs_cluster_tags') result(rc)
+      import :: c_int, c_double, c_int8_t, c_int32_t
+      integer(c_int8_t), intent(in) :: tags(*)
+      integer(c_int32_t), intent(in) :: lo(3), hi(3)
+      real(c_double), value :: efficiency
+      integer(c_int), value :: blocking, buffer, max_size, max_boxes
+      integer(c_int32_t), intent(out) :: box_lo(*), box_hi(*)
+      integer(c_int), intent(out) :: nbox
       integer(c_int) :: rc
     end function
   end interface
