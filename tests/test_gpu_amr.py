@@ -96,3 +96,65 @@ def test_two_level_tgv_tracks_single_level(pkg):
     print(f"two-level TGV vs single level after {steps} coarse steps: u relL2 {e:.3e}, "
           f"mass drift {dm:.2e}, node updates {st['node_updates']}")
     assert e < 5e-2 and dm < 1e-4
+
+
+@pytest.mark.parametrize("many", [False, True])
+def test_uniform_flow_is_a_fixed_point_of_the_advance(pkg, many):
+    """A uniform stream over a periodic level 0 and a refined level (one box,
+    or uneven boxes) stays uniform through the full two-level advance: ghost
+    interpolation in space and time, two substeps, injection (FP64, 8 coarse
+    steps, every stored node of both levels within 1e-13)."""
+    from test_gpu_boxes import _tiling
+    u = (0.05, 0.02, -0.03)
+    s = pkg.Solver(precision="fp64", nu=0.01, alpha_tol=1e-12)
+    s.define_level(0, (24, 20, 16))
+    if many:
+        s.define_level_boxes(1, _tiling((8, 6, 6), (31, 29, 23), (5, 7, 9), 3))
+    else:
+        s.define_level(1, (24, 24, 18), periodic=(0, 0, 0), lo=(8, 6, 6))
+    s.init_uniform(0, 1.0, u)
+    s.init_from_coarse(1)
+    f0, f1 = s.download_f(0), s.download_f(1)
+    assert np.abs(f1 - f1[:, :1]).max() < 1e-15
+    s.advance(8)
+    d0 = np.abs(s.download_f(0) - f0).max()
+    d1 = np.abs(s.download_f(1) - f1).max()
+    print(f"uniform stream, {'many boxes' if many else 'one box'}: max|df| coarse {d0:.1e} fine {d1:.1e}")
+    assert d0 < 1e-13 and d1 < 1e-13
+
+
+def test_two_level_interface_error_converges(pkg):
+    """Two-level TGV (FP64; a refined box over the middle of the periodic
+    level 0) against the single level at the same coarse resolution, at equal
+    Reynolds and Mach number and equal time (acoustic scaling: nu ~ L, steps
+    ~ L).  Both converge to the same flow, so their difference must fall with
+    the resolution.  The reference fills the fine ghosts with the coarse
+    populations themselves (amrex_fillcoarsepatch of fi,
+    interlevel_interpolation.f95:250-252), without rescaling their
+    non-equilibrium part to the fine level's relaxation time, so the interface
+    is first order: the difference must fall at least 1.6x from L = 16 to
+    L = 32 (measured 1.95x: 10.7% -> 5.5%)."""
+    u0, re = 0.05, 40.0
+    errs = []
+    for L in (16, 32):
+        nu = u0 * L / re
+        steps = int(round(0.75 * L / u0 / 8))
+        runs = []
+        for two_level in (False, True):
+            s = pkg.Solver(precision="fp64", nu=nu, alpha_tol=1e-12)
+            s.define_level(0, (L, L, L))
+            s.init_tgv(0, u0, 0.0, 1)
+            if two_level:
+                s.define_level(1, (L + 1, L + 1, L + 1), periodic=(0, 0, 0), lo=(L // 2, L // 2, L // 2))
+                s.init_from_coarse(1)
+                s.advance(steps)
+            else:
+                s.step(0, steps)
+            _, ux, uy, uz = s.macro(0)
+            runs.append(np.stack([ux, uy, uz]))
+            s.close()
+        e = float(np.linalg.norm(runs[1] - runs[0]) / np.linalg.norm(runs[0]))
+        errs.append(e)
+        print(f"L = {L}: {steps} coarse steps, two-level vs single level u relL2 {e:.3e}")
+    print(f"interface error ratio L16 / L32: {errs[0] / errs[1]:.2f}")
+    assert errs[1] < errs[0] / 1.6 and errs[0] < 0.15

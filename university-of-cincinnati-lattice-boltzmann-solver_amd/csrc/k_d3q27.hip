@@ -73,6 +73,13 @@ template <typename T>
 __device__ __forceinline__ T& pop(T* __restrict__ f, unsigned i, unsigned nn, unsigned k) {
   return const_cast<T&>(pop(const_cast<const T*>(f), i, nn, k));
 }
+// the step's output populations: a streaming store (nontemporal), since no
+// kernel of this step reads them back and each line is written once
+template <typename T>
+__device__ __forceinline__ void pop_stream(T* __restrict__ f, unsigned i, unsigned nn, unsigned k,
+                                           T v) {
+  __builtin_nontemporal_store(v, &pop(f, i, nn, k));
+}
 
 // appends the wavefront's deferred nodes, with their alpha seeds, to shard
 // `shard` of the far-from-equilibrium queue (one atomic per wavefront; the
@@ -187,7 +194,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     // a deferred node stores its pre-collision state; the far-from-equilibrium
     // kernel collides it in place
 #pragma unroll
-    for (int i = 0; i < 27; ++i) pop(fout, i, nn, node) = f[i];
+    for (int i = 0; i < 27; ++i) pop_stream(fout, i, nn, node, f[i]);
     if (kKeepAlpha) alpha_out[node] = ao.alpha;
   }
   if constexpr (kShifted)
@@ -260,7 +267,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
   // are written by the boundary kernel or not at all
   if (in_box && (!kLinks || shell || active)) {
 #pragma unroll
-    for (int i = 0; i < 27; ++i) pop(fout, i, nn, node) = f[i];
+    for (int i = 0; i < 27; ++i) pop_stream(fout, i, nn, node, f[i]);
   }
   if (kKeepAlpha && active) alpha_out[node] = ao.alpha;
   if constexpr (kShifted)
