@@ -161,11 +161,14 @@ __device__ int classify_pos(const LinkGeom& g, int a, int b, int c, int* tie) {
   if (g.ntri > 0) {
     for (int attempt = 0; attempt < 4; ++attempt) {
       double e2[3];
-      if (attempt == 0 && dist > 0.0) {
+      if (attempt == 0 && g.ray_len > 0.0) {
+        for (int d = 0; d < 3; ++d) e2[d] = e1[d] + g.ray_len * g.ray[d];
+      } else if (attempt == 0 && dist > 0.0) {
         for (int d = 0; d < 3; ++d) e2[d] = e1[d] + L * dl[d] / dist;
       } else {
         const int r = attempt == 0 ? 0 : attempt - 1;
-        for (int d = 0; d < 3; ++d) e2[d] = e1[d] + L * kRecast[r][d];
+        const double Lr = g.ray_len > 0.0 ? g.ray_len : L;
+        for (int d = 0; d < 3; ++d) e2[d] = e1[d] + Lr * kRecast[r][d];
         if (attempt == 0) *tie = 1;
       }
       if (box_out(e1, e2, g.gbb)) { solid = 0; break; }

@@ -100,6 +100,11 @@ struct LinkGeom {
   double gbb[6];      // geometry bounding box, same order
   const BvhNode* bvh;     // hierarchy (node 0 the root), null for no triangles
   const int32_t* bvh_idx; // triangle indices of the leaves
+  // ray_len > 0: the solid test casts every node's first ray along the unit
+  // vector ray, ray_len long (re-casts ray_len long too), instead of away from
+  // the box centre
+  double ray[3];
+  double ray_len;
 };
 
 // Builds the hierarchy on the host (median split of the longest centroid axis,
