@@ -742,3 +742,25 @@ void o39_step_periodic(int nx, int ny, int nz, double *f, double *g, double *lm,
   o39_step_faces(nx, ny, nz, fc, NULL, NULL, NULL, NULL, f, g, lm, alpha_out, c, step0, nsteps,
                  ref_lu, jacobi, st);
 }
+
+/* nrep LU updates on the same inputs (timing against the reference's
+   lu_decomp): the sum of each call's max|delta| and first updated multiplier */
+double o39_lu_batch(int nrep, const double *J, const double *mom, const double *lm0, int ref_mode) {
+  double s = 0.0, lm[13];
+  for (int k = 0; k < nrep; ++k) {
+    memcpy(lm, lm0, sizeof lm);
+    const double mx = o39_lu_step(J, mom, lm, ref_mode);
+    s = (s + mx) + lm[0];
+  }
+  return s;
+}
+
+/* o39_sutherland + o39_therm_cond at n nodes (timing): the sum of mu + kappa */
+double o39_quick_batch(int n, const double *rho, const double *T, const o39_consts *c) {
+  double s = 0.0, dm, dt, kappa;
+  for (int k = 0; k < n; ++k) {
+    s += o39_sutherland(T[k], c, &dm, &dt);
+    if (o39_therm_cond(rho[k], T[k], c, &kappa) == 0) s += kappa;
+  }
+  return s;
+}

@@ -96,4 +96,39 @@ contains
                    lm(5), lm(6), lm(7), lm(8), lm(9), lm(10), lm(11), lm(12), lm(13), max_diff)
   end function
 
+  ! nrep calls of lu_decomp on the same inputs (timing): returns the sum of
+  ! each call's max_diff and first updated multiplier
+  function ref_lu_decomp_batch(nrep, jac, tgt, lm) bind(c, name='ref_lu_decomp_batch') result(s)
+    integer(c_int), value :: nrep
+    real(c_double), intent(in) :: jac(13, 13), tgt(13), lm(13)
+    real(c_double) :: s
+    real(dp) :: j(13, 13), l(13), max_diff
+    integer :: k
+    s = 0.0D0
+    do k = 1, nrep
+      j = jac
+      l = lm
+      max_diff = 0.0D0
+      call lu_decomp(j, 13, tgt(1), tgt(2), tgt(3), tgt(4), tgt(5), tgt(6), tgt(7), tgt(8), &
+                     tgt(9), tgt(10), tgt(11), tgt(12), tgt(13), l(1), l(2), l(3), l(4), &
+                     l(5), l(6), l(7), l(8), l(9), l(10), l(11), l(12), l(13), max_diff)
+      s = s + max_diff + l(1)
+    end do
+  end function
+
+  ! quick_sutherlands and quick_therm_cond at n nodes (timing): returns the
+  ! sum of mu + kappa
+  function ref_quick_batch(n, rho, temp) bind(c, name='ref_quick_batch') result(s)
+    integer(c_int), value :: n
+    real(c_double), intent(in) :: rho(n), temp(n)
+    real(c_double) :: s, mu, kappa
+    integer :: k
+    s = 0.0D0
+    do k = 1, n
+      call quick_sutherlands(mu, temp(k))
+      call quick_therm_cond(kappa, rho(k), temp(k))
+      s = s + mu + kappa
+    end do
+  end function
+
 end module

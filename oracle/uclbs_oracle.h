@@ -147,6 +147,8 @@ double o39_sutherland(double T, const o39_consts *c, double *dim_mu, double *dim
 int o39_therm_cond(double rho, double T, const o39_consts *c, double *kappa);
 void o39_feq(const double lm[13], double rho, int shifted, double feq[39]);
 double o39_lu_step(const double *J, const double *mom, double *lm, int ref_mode);
+double o39_lu_batch(int nrep, const double *J, const double *mom, const double *lm0, int ref_mode);
+double o39_quick_batch(int n, const double *rho, const double *T, const o39_consts *c);
 int o39_lm_solve(double rho, const double u[3], double T, const double P[6], double lm[13],
                  double expo_last[39], int shifted, int ref_mode, int maxit);
 double o39_alpha_estimate(const double f[39], const double diff[39], double T, int64_t step,
