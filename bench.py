@@ -434,8 +434,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic.get("bytes_per_launch") if traffic else None,
-                     "kernel": "k27_step_periodic (streaming kernel, FP32 shifted)",
+                     "kernel": (traffic or {}).get("kernel", "k27_step_periodic") +
+                               " (streaming kernel, FP32 shifted)",
                      "achieved_whole_step": round(achieved_step, 1),
+                     "frac_whole_step": round(achieved_step / HBM_PEAK_GBS, 4),
                      "measured_copy_GBps": round(ceiling, 1),
                      "frac_of_copy": round(achieved / ceiling, 4),
                      "avg_launch_ms": round(avg_launch_s * 1e3, 4)},
