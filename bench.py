@@ -91,6 +91,10 @@ def cpu_baseline(budget_s: float):
                       f"{threads} OpenMP threads on {cpu_model()}"}
 
 
+FP64_PEAK_TFS = 78.6  # MI355X FP64 vector / matrix peak
+C3_FLOPS_PER_NODE = 24.0e3  # counted FP64 flops per D3Q39 node update (C3, developed flow)
+
+
 def read_traffic(name):
     p = os.path.join(ROOT, "profiles", name)
     try:
@@ -229,7 +233,14 @@ def bench_c3(pkg, args):
                         "traffic": None,
                         "kernel": "one D3Q39 step: wall rebuild, k39_step (interior, near-face "
                                   "and wall nodes), repair pass",
-                        "avg_launch_ms": round(avg * 1e3, 4)},
+                        "avg_launch_ms": round(avg * 1e3, 4),
+                        # the step is bound by FP64 VALU work: 24.0k FP64 flops per node
+                        # update counted by SQ_INSTS_VALU_FLOPS_FP64 on the developed flow
+                        # (profiles/d3q39_c3_counters.md)
+                        "fp64": {"flops_per_node": C3_FLOPS_PER_NODE,
+                                 "achieved": round(C3_FLOPS_PER_NODE * n / avg / 1e12, 2),
+                                 "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
+                                 "frac": round(C3_FLOPS_PER_NODE * n / avg / 1e12 / FP64_PEAK_TFS, 4)}},
            "shock": {"angle_deg": round(ang, 3), "theory_deg": round(beta, 3),
                      "layer_rho": round(post, 4), "rankine_hugoniot_rho": round(rho_rh, 4),
                      "columns": ncol, "at_step": args.warmup + args.steps + rest},
