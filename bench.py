@@ -91,6 +91,39 @@ def cpu_baseline(budget_s: float):
                       f"{threads} OpenMP threads on {cpu_model()}"}
 
 
+def cpu_baseline_c2(s, L, nu, budget_s: float):
+    """FP64 oracle (C/OpenMP restatement of the path) on the same C2 duct: the
+    device's link table and its current populations, stepped by the oracle's
+    box step (faces, inlet/outlet equilibria, wall links, entropic Newton) for
+    as many steps as fit the budget (at least 2)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle  # test infrastructure: the CPU baseline leg only
+    try:
+        threads = len(os.sched_getaffinity(0))
+    except AttributeError:
+        threads = os.cpu_count() or 1
+    threads = max(1, min(threads, 16))
+    u_in = 0.2 / math.sqrt(3.0)
+    state = s.download_links(0)
+    f = s.download_f(0)
+    kw = dict(inlets=[(1.0, u_in, 0.0, 0.0)], outlets=[(1.0, u_in, 0.0, 0.0)], tol=1e-10,
+              nthreads=threads, want_alpha=False)
+    beta = 1 / (6 * nu + 1)
+    shape = (L, L, L)
+    t0 = time.perf_counter()
+    f, _, _, _ = oracle.step_box27(shape, C2_FACES, state, f, beta, nsteps=1, **kw)
+    dt1 = time.perf_counter() - t0
+    steps = max(2, min(50, int(budget_s / max(dt1, 1e-6))))
+    t0 = time.perf_counter()
+    oracle.step_box27(shape, C2_FACES, state, f, beta, nsteps=steps, **kw)
+    dt = time.perf_counter() - t0
+    return {"value": round(L ** 3 * steps / dt / 1e6, 3), "unit": "MLUPS", "cores": threads,
+            "kind": "port",
+            "sample": f"C2 {L}^3 duct with the sphere (the device's link table and state after "
+                      f"the timed steps), D3Q27 FP64 entropic Newton, {steps} steps, "
+                      f"{threads} OpenMP threads on {cpu_model()}"}
+
+
 FP64_PEAK_TFS = 78.6  # MI355X FP64 vector / matrix peak
 # FP64 flops per D3Q39 node update = base + per Newton iteration, from
 # SQ_INSTS_VALU_FLOPS_FP64 over the three k39_step launches of a C3 step at two
@@ -107,12 +140,15 @@ def read_traffic(name):
         return None
 
 
+C2_FACES = [-101, -10, -10, -201, -10, -10]  # inlet x-, walls y/z, outlet x+
+
+
 def make_c2(pkg, L, nu, alpha_mode):
     """The C2 duct: faces, inlet/outlet states, icosphere link tables, uniform start."""
     u_in = 0.2 / math.sqrt(3.0)
     s = pkg.Solver(precision="fp32s", nu=nu, alpha_mode=alpha_mode, alpha_tol=1e-10)
     s.define_level(0, (L, L, L), periodic=(0, 0, 0))
-    s.set_faces(0, [-101, -10, -10, -201, -10, -10])
+    s.set_faces(0, C2_FACES)
     s.set_bc_states(inlets=[(1.0, u_in, 0.0, 0.0)], outlets=[(1.0, u_in, 0.0, 0.0)])
     sc = L / 256.0
     tri, nrm = pkg.icosphere(5, 24.0 * sc, (80.5 * sc, 128.5 * sc, 128.5 * sc), 20260102)
@@ -467,7 +503,8 @@ def main():
         except pkg.UclbsError as exc:  # e.g. a non-finite training loss
             out["learned_alpha"] = {"status": "failed", "error": str(exc)}
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_budget)
+        out["cpu_baseline"] = (cpu_baseline_c2(s, L, nu, args.cpu_budget) if args.workload == "c2"
+                               else cpu_baseline(args.cpu_budget))
     print(json.dumps(out), flush=True)
 
 
