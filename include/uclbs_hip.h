@@ -501,7 +501,10 @@ int uclbs_regrid_copied(void *ctx, int64_t *n);
  * every finer level. */
 int uclbs_clear_level(void *ctx, int lvl);
 /* Replaces ggm_calculator (GGM/ggm_calculator.f95:47-68: populate_q,
- * calculate_grad_mag, calculate_gradgrad_mag) on a one-box D3Q27 level: the
+ * calculate_grad_mag, calculate_gradgrad_mag) on a D3Q27 level (one box, or a
+ * refined level of many boxes, computed on a dense array over its stored
+ * nodes' bounding box; a refined level's ghost shell is first filled from the
+ * coarser level's current state): the
  * magnitude of the gradient of the gradient magnitude of a quantity
  * (1 density, 2 u, 3 v, 4 w, 7 velocity magnitude) from the current
  * populations, each derivative by derivative_windage_secondord

@@ -18,7 +18,11 @@ Restates the reference's gradient-of-gradient-magnitude tagging on one box:
                                         (reference component indices 2+a,
                                         16+a, 5+a, 19+a) deciding use
 A stencil node beyond a non-periodic face of the box is never usable and
-reads as 0 (the reference reads its AMReX ghost value there).
+reads as 0 (the reference reads its AMReX ghost value there).  A refined level
+of many boxes is restated on a dense array over its stored nodes' bounding
+box (boxes_dense): ghost-shell nodes first (every link usable), valid nodes
+over them (the level's link codes); positions no box stores are unfilled,
+never computed and never usable.
 
 windage1 / windage2 are the scalar functions as the reference writes them;
 gradmag is the vectorised form over a box, pinned against them node by node.
@@ -62,7 +66,7 @@ def windage2(q, ok, wind, scheme):
     return windage1((q[0], q[1], q[3]), (ok[0], ok[1], ok[3]), wind, scheme)
 
 
-def _stencil(shape, periodic, state, ax):
+def _stencil(shape, periodic, state, ax, filled=None):
     """Per node: neighbour index (or -1) and usability of (x+1, x+2, x-1, x-2)."""
     nx, ny, nz = shape
     n = nx * ny * nz
@@ -82,18 +86,22 @@ def _stencil(shape, periodic, state, ax):
         use = inb.copy()
         if state is not None:
             use &= state[comp] >= 0
+        if filled is not None:
+            use &= np.where(idx >= 0, filled[np.maximum(idx, 0)] != 0, False)
         nd.append(idx)
         ok.append(use)
     return nd, ok
 
 
-def gradmag(src, vel, shape, periodic=(0, 0, 0), state=None, scheme=CENTRAL, skip_null=True):
+def gradmag(src, vel, shape, periodic=(0, 0, 0), state=None, scheme=CENTRAL, skip_null=True,
+            filled=None):
     """|grad src| per node (calculate_grad_mag.f95:33-61 when skip_null,
-    calculate_gradgrad_mag.f95:37-63 otherwise); vel [3][n] the wind."""
+    calculate_gradgrad_mag.f95:37-63 otherwise); vel [3][n] the wind;
+    filled (optional): only filled positions are computed or usable."""
     src = np.asarray(src, np.float64)
     d2 = np.zeros(src.shape)
     for ax in range(3):
-        nd, ok = _stencil(shape, periodic, state, ax)
+        nd, ok = _stencil(shape, periodic, state, ax, filled)
         q = [src] + [np.where(i >= 0, src[np.maximum(i, 0)], 0.0) for i in nd]
         o1, o2, o3, o4 = ok
         w = vel[ax]
@@ -111,6 +119,8 @@ def gradmag(src, vel, shape, periodic=(0, 0, 0), state=None, scheme=CENTRAL, ski
         own = state[0]
         dead = (own < 0) & ~(skip_null & (own == -666))
         out = np.where(dead, 0.0, out)
+    if filled is not None:
+        out = np.where(filled != 0, out, 0.0)
     return out
 
 
@@ -144,3 +154,47 @@ def tags(ggm, limit, valid=None):
     """calculate_gradgrad_mag.f95:67-71: tag where ggm >= limit (valid nodes)."""
     t = ggm >= limit
     return t if valid is None else t & valid
+
+
+def boxes_dense(f, state, boxes, cx, cy, cz, which=1, scheme=CENTRAL):
+    """ggm of a refined level of many boxes, in its stored order: f [27][n]
+    and state [27][n] (or None) stored box after box, each with its one-node
+    ghost shell; boxes [(lo, hi)] their valid regions.  Returns (ggm, valid)
+    over the stored nodes (0 on ghost shells)."""
+    dlo = np.min([b[0] for b in boxes], axis=0) - 1
+    dhi = np.max([b[1] for b in boxes], axis=0) + 1
+    ext = dhi - dlo + 1
+    nd = int(np.prod(ext))
+    rho, vel = fields27(f, cx, cy, cz)
+    q = quantity(rho, vel, which)
+    qd = np.zeros(nd)
+    vd = np.zeros((3, nd))
+    filled = np.zeros(nd, np.int8)
+    sd = np.zeros((27, nd), np.int64) if state is not None else None
+    pos, off = [], 0
+    for lo, hi in boxes:
+        e = np.array(hi) - np.array(lo) + 3
+        z, y, x = np.mgrid[0:e[2], 0:e[1], 0:e[0]]
+        x, y, z = x.ravel(), y.ravel(), z.ravel()
+        shell = (x == 0) | (y == 0) | (z == 0) | (x == e[0] - 1) | (y == e[1] - 1) | (z == e[2] - 1)
+        g = np.stack([x + lo[0] - 1, y + lo[1] - 1, z + lo[2] - 1]) - dlo[:, None]
+        kd = g[0] + ext[0] * (g[1] + ext[1] * g[2])
+        pos.append((np.arange(off, off + x.size), kd, shell))
+        off += x.size
+    for pass_ in (0, 1):
+        for ks, kd, shell in pos:
+            m = shell if pass_ == 0 else ~shell
+            qd[kd[m]] = q[ks[m]]
+            vd[:, kd[m]] = vel[:, ks[m]]
+            filled[kd[m]] = 1 + pass_
+            if sd is not None:
+                sd[:, kd[m]] = 0 if pass_ == 0 else state[:, ks[m]]
+    shape = tuple(int(v) for v in ext)
+    gm = gradmag(qd, vd, shape, (0, 0, 0), sd, scheme, True, filled)
+    ggm = gradmag(gm, vd, shape, (0, 0, 0), sd, scheme, False, filled)
+    out = np.zeros(off)
+    valid = np.zeros(off, bool)
+    for ks, kd, shell in pos:
+        out[ks[~shell]] = ggm[kd[~shell]]
+        valid[ks[~shell]] = True
+    return out, valid

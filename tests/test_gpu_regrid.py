@@ -14,7 +14,11 @@ level on new boxes, and the tag -> cluster -> remake -> re-identify loop.
   the loop (Solver.refine_by_ggm), FP32 storage: level 1 created around the
     sphere from level 0's tags, links built, stepped; regridded twice as the
     flow develops, each time covering every tag, taking over the old nodes,
-    and stepping on with finite populations."""
+    and stepping on with finite populations
+  a refined level of many boxes: its GGM (dense over the boxes' stored
+    nodes, ghost shells filled from level 0 first) equals the same region as
+    one box bit for bit and the oracle's restatement to 1e-12 x max ggm;
+    level 2 made and regridded from level 1's tags (three levels)."""
 import numpy as np
 import pytest
 
@@ -200,3 +204,84 @@ def ctypes_copied(s):
     n = ctypes.c_int64()
     s._check(s.lib.uclbs_regrid_copied(s.ctx, ctypes.byref(n)))
     return n.value
+
+
+def test_ggm_on_many_boxes_equals_one_box_and_oracle(pkg, orc):
+    """The refined region around the sphere as one box and as 26 uneven boxes
+    (FP64, alpha tol 1e-12: the two runs hold the same bits): after 4 coarse
+    steps the GGM of level 1 (ghost shells filled from level 0 first) is the
+    same on both, bit for bit, and equals the oracle's dense restatement of
+    the many-box level (oracle/o_ggm.py boxes_dense) and its one-box form to
+    1e-12 x max ggm."""
+    from test_gpu_boxes import _tiling
+    fvalid = tuple(FHI[d] - FLO[d] + 1 for d in range(3))
+    fphys = tuple(0.5 * (FLO[d] - 1) for d in range(3))
+    tri, nrm = sphere(pkg)
+    boxes = _tiling(FLO, FHI, (5, 7, 9), 11)
+    cx, cy, cz, _, _ = orc.lattice27()
+    runs = []
+    for many in (False, True):
+        s = duct(pkg, "fp64")
+        s.build_links(0, tri, nrm)
+        if many:
+            s.define_level_boxes(1, boxes)
+            s.build_links(1, tri, nrm, origin=(0.0, 0.0, 0.0), dx=0.5)
+        else:
+            s.define_level(1, fvalid, periodic=(0, 0, 0), lo=FLO)
+            s.build_links(1, tri, nrm, origin=fphys, dx=0.5)
+        s.init_uniform(0, 1.0, (U, 0.0, 0.0))
+        s.init_from_coarse(1)
+        s.advance(4)
+        ggm, tags, cnt = s.tag_ggm(1, 7, 0, 1e-5)
+        f1, st1 = s.download_f(1), s.download_links(1).astype(np.int64)
+        if many:
+            ref, valid = o_ggm.boxes_dense(f1, st1, boxes, cx, cy, cz, 7, 0)
+            dense = s.stored_to_dense(1, ggm, fvalid[::-1], origin=FLO)
+        else:
+            fs = tuple(v + 2 for v in fvalid)
+            _, ref = o_ggm.ggm27(f1, fs, cx, cy, cz, 7, 0, state=st1)
+            ref = ref.reshape(fs[::-1])[1:-1, 1:-1, 1:-1].ravel()
+            ggm = ggm.reshape(fs[::-1])[1:-1, 1:-1, 1:-1].ravel()
+            dense = ggm.reshape(fvalid[::-1])
+            valid = np.ones(ggm.size, bool)
+        scale = float(np.abs(ref).max())
+        err = float(np.abs(ggm[valid] - ref[valid]).max()) / scale if many else \
+            float(np.abs(ggm - ref).max()) / scale
+        print(f"{'many boxes' if many else 'one box'}: GGM vs oracle max|d| / max {err:.2e}, "
+              f"{cnt} tags")
+        assert err < 1e-12 and cnt > 0
+        runs.append(dense)
+        s.close()
+    assert np.array_equal(runs[0], runs[1])
+
+
+def test_three_level_tag_cluster_remake(pkg):
+    """Level 1 from level 0's tags, then level 2 from the tags of level 1 (a
+    level of many boxes), link tables built on both, stepped; level 2 is
+    regridded again after more steps.  The new level-2 boxes nest in level
+    1 (their ghost stencils fall on level-1 valid nodes) and cover every
+    level-1 tag kept on nodes whose neighbours are all valid."""
+    tri, nrm = sphere(pkg)
+    s = duct(pkg, "fp32s")
+    s.build_links(0, tri, nrm)
+    s.init_uniform(0, 1.0, (U, 0.0, 0.0))
+    s.step(0, 20)
+    ggm0, _, _ = s.tag_ggm(0, 7, 0, 0.0)
+    b1 = s.refine_by_ggm(1, quantity=7, limit=float(np.quantile(ggm0[ggm0 > 0], 0.9)),
+                         blocking=2, buffer=1, max_size=16)
+    s.build_links(1, tri, nrm, origin=(0.0, 0.0, 0.0), dx=0.5)
+    s.advance(4)
+    history = []
+    for rnd in range(2):
+        ggm1, _, _ = s.tag_ggm(1, 7, 0, 0.0)
+        lim = float(np.quantile(ggm1[ggm1 > 0], 0.97))
+        had = 2 in s.shapes
+        b2 = s.refine_by_ggm(2, quantity=7, limit=lim, blocking=2, buffer=1, max_size=16)
+        assert b2
+        s.build_links(2, tri, nrm, origin=(0.0, 0.0, 0.0), dx=0.25)
+        s.advance(3)
+        f2 = s.download_f(2)
+        assert np.isfinite(f2).all() and np.isfinite(s.download_f(1)).all()
+        history.append((len(b1), len(b2), had))
+    print("three levels (level-1 boxes, level-2 boxes, level 2 regridded):", history)
+    s.close()

@@ -380,29 +380,74 @@ class Solver:
     def refine_by_ggm(self, lvl: int, quantity: int = 1, scheme: int = 0, limit: float = 0.04,
                       efficiency: float = 0.7, blocking: int = 2, buffer: int = 2,
                       max_size: int = 32):
-        """One regrid of level lvl (>= 1) from the GGM tags of the one-box
-        level lvl-1 (regrid_question_mark.f95:38-184 through AMReX's regrid):
-        tag, cluster within level lvl-1's valid nodes one node in from its
-        non-periodic faces (the new level's ghost shell must nest), refine the
-        boxes by 2, then regrid level lvl (define and fill it from level lvl-1
-        when it does not exist yet).  No tags: level lvl and finer are
-        cleared.  Returns the new boxes (level lvl's index space)."""
-        if lvl - 1 not in getattr(self, "places", {}):
-            raise UclbsError(f"refine_by_ggm: level {lvl - 1} must be a one-box level")
-        clo, cshape, cper = self.places[lvl - 1]
-        g = 2 if lvl - 1 > 0 else 0
+        """One regrid of level lvl (>= 1) from the GGM tags of level lvl-1
+        (regrid_question_mark.f95:38-184 through AMReX's regrid): tag, cluster
+        where the new level's ghost shell nests in level lvl-1's valid nodes,
+        refine the boxes by 2, then regrid level lvl (define and fill it from
+        level lvl-1 when it does not exist yet).  Level lvl-1 of one box:
+        clustered within its valid nodes one node in from its non-periodic
+        faces.  Level lvl-1 of many boxes (refined): tags kept on valid nodes
+        whose 26 neighbours are valid nodes too, and a clustered box that
+        leaves them is clustered again at efficiency 1 (boxes of tags only;
+        the buffer grows the tags before they are kept to those nodes).
+        No tags: level lvl and finer are cleared.  Returns the new boxes
+        (level lvl's index space)."""
+        places, mboxes = getattr(self, "places", {}), getattr(self, "boxes", {})
         _, tags, cnt = self.tag_ggm(lvl - 1, quantity, scheme, limit)
-        st = tags.reshape(tuple(v + g for v in cshape)[::-1])
-        if g:
-            st = st[1:-1, 1:-1, 1:-1]
-        inset = [0 if cper[d] else 1 for d in range(3)]
-        rlo = [clo[d] + inset[d] for d in range(3)]
-        rhi = [clo[d] + cshape[d] - 1 - inset[d] for d in range(3)]
-        region = np.ascontiguousarray(st[inset[2]:cshape[2] - inset[2], inset[1]:cshape[1] - inset[1],
-                                         inset[0]:cshape[0] - inset[0]])
-        boxes = cluster_tags(region, rlo, efficiency=efficiency, blocking=blocking, buffer=buffer,
-                             max_size=max_size) if region.any() else []
-        assert rhi == [rlo[d] + region.shape[2 - d] - 1 for d in range(3)]
+        if lvl - 1 in places:
+            clo, cshape, cper = places[lvl - 1]
+            g = 2 if lvl - 1 > 0 else 0
+            st = tags.reshape(tuple(v + g for v in cshape)[::-1])
+            if g:
+                st = st[1:-1, 1:-1, 1:-1]
+            inset = [0 if cper[d] else 1 for d in range(3)]
+            rlo = [clo[d] + inset[d] for d in range(3)]
+            region = np.ascontiguousarray(st[inset[2]:cshape[2] - inset[2],
+                                             inset[1]:cshape[1] - inset[1],
+                                             inset[0]:cshape[0] - inset[0]])
+            boxes = cluster_tags(region, rlo, efficiency=efficiency, blocking=blocking,
+                                 buffer=buffer, max_size=max_size) if region.any() else []
+        elif lvl - 1 in mboxes and lvl - 1 >= 1:
+            vlo = np.min([b[0] for b in mboxes[lvl - 1]], axis=0)
+            vhi = np.max([b[1] for b in mboxes[lvl - 1]], axis=0)
+            ext = vhi - vlo + 1
+            valid = np.zeros(tuple(ext[::-1]), bool)
+            t = np.zeros(tuple(ext[::-1]), np.uint8)
+            for off, e, lo in self.box_layout(lvl - 1):
+                blk = tags[off:off + e[0] * e[1] * e[2]].reshape(e[2], e[1], e[0])[1:-1, 1:-1, 1:-1]
+                z0, y0, x0 = lo[2] - vlo[2], lo[1] - vlo[1], lo[0] - vlo[0]
+                sl = (slice(z0, z0 + e[2] - 2), slice(y0, y0 + e[1] - 2), slice(x0, x0 + e[0] - 2))
+                valid[sl] = True
+                t[sl] = blk
+            inner = valid.copy()
+            pad = np.pad(valid, 1)
+            for dz in (0, 1, 2):
+                for dy in (0, 1, 2):
+                    for dx in (0, 1, 2):
+                        inner &= pad[dz:dz + ext[2], dy:dy + ext[1], dx:dx + ext[0]]
+            # the buffer grows the tags (a cube) before they are kept to `inner`
+            for ax in range(3):
+                grown = t.astype(bool)
+                for o in range(1, buffer + 1):
+                    lo_sl = [slice(None)] * 3
+                    hi_sl = [slice(None)] * 3
+                    lo_sl[2 - ax], hi_sl[2 - ax] = slice(o, None), slice(None, -o)
+                    grown[tuple(lo_sl)] |= t[tuple(hi_sl)].astype(bool)
+                    grown[tuple(hi_sl)] |= t[tuple(lo_sl)].astype(bool)
+                t = grown.astype(np.uint8)
+            t = np.ascontiguousarray(t * inner)
+            boxes = []
+            if t.any():
+                for lo, hi in cluster_tags(t, tuple(int(v) for v in vlo), efficiency=efficiency,
+                                           blocking=blocking, buffer=0, max_size=max_size):
+                    sl = tuple(slice(lo[d] - vlo[d], hi[d] - vlo[d] + 1) for d in (2, 1, 0))
+                    if inner[sl].all():
+                        boxes.append((lo, hi))
+                    else:
+                        boxes += cluster_tags(np.ascontiguousarray(t[sl]), lo, efficiency=1.0)
+        else:
+            raise UclbsError(f"refine_by_ggm: level {lvl - 1} is neither a one-box level nor a "
+                             f"refined level of many boxes")
         if not boxes:
             if lvl in self.shapes:
                 self.clear_level(lvl)

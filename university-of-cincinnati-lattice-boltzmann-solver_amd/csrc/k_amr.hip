@@ -142,7 +142,8 @@ __global__ __launch_bounds__(256) void k27_c2f_stencils(const void* __restrict__
                                                         void* __restrict__ ff_, unsigned fnn,
                                                         const C2FStencil* __restrict__ st,
                                                         unsigned nst, double theta,
-                                                        unsigned long long* __restrict__ stats) {
+                                                        unsigned long long* __restrict__ stats,
+                                                        unsigned extra_mask) {
   using T = typename std::conditional<kShifted, float, double>::type;
   const T* __restrict__ cprev = static_cast<const T*>(cprev_);
   const T* __restrict__ cnew = static_cast<const T*>(cnew_);
@@ -162,7 +163,7 @@ __global__ __launch_bounds__(256) void k27_c2f_stencils(const void* __restrict__
       f[i] += w * ((1.0 - theta) * a + theta * b);
     }
   }
-  repair_store27<kShifted>(f, static_cast<T*>(ff_), fnn, s.dst, stats, s.mask);
+  repair_store27<kShifted>(f, static_cast<T*>(ff_), fnn, s.dst, stats, s.mask | extra_mask);
 }
 
 // src and dst may be the same level (ghost copies): a pair's source is a valid
@@ -232,13 +233,16 @@ hipError_t launch_copy_pairs27(const void* src, unsigned nn_src, void* dst, unsi
 
 hipError_t launch_c2f_stencils27(const void* cprev, const void* cnew, unsigned nn_c, void* ff,
                                  unsigned nn_f, const C2FStencil* st, unsigned nst, double theta,
-                                 int shifted, unsigned long long* stats, hipStream_t s) {
+                                 int shifted, unsigned long long* stats, hipStream_t s,
+                                 int all_pops) {
   if (!nst) return hipSuccess;
   const unsigned g = (nst + 255) / 256;
   if (shifted)
-    k27_c2f_stencils<true><<<g, 256, 0, s>>>(cprev, cnew, nn_c, ff, nn_f, st, nst, theta, stats);
+    k27_c2f_stencils<true><<<g, 256, 0, s>>>(cprev, cnew, nn_c, ff, nn_f, st, nst, theta, stats,
+                                                             all_pops ? kAllPops27 : 0u);
   else
-    k27_c2f_stencils<false><<<g, 256, 0, s>>>(cprev, cnew, nn_c, ff, nn_f, st, nst, theta, stats);
+    k27_c2f_stencils<false><<<g, 256, 0, s>>>(cprev, cnew, nn_c, ff, nn_f, st, nst, theta, stats,
+                                                             all_pops ? kAllPops27 : 0u);
   return hipGetLastError();
 }
 

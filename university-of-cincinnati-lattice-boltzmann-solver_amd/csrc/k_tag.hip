@@ -83,8 +83,9 @@ __device__ long long nb_node(const Box& b, int x, int y, int z, int ax, int o) {
 }
 
 // stencil nodes along axis ax, (x, x+1, x+2, x-1, x-2), and which are usable
-__device__ void stencil(const Box& b, const int32_t* __restrict__ state, long long k, int x, int y,
-                        int z, int ax, long long nd[5], bool ok[5]) {
+__device__ void stencil(const Box& b, const int32_t* __restrict__ state,
+                        const uint8_t* __restrict__ filled, long long k, int x, int y, int z,
+                        int ax, long long nd[5], bool ok[5]) {
   nd[0] = k;
   ok[0] = true;
   const int off[4] = {1, 2, -1, -2};
@@ -96,6 +97,8 @@ __device__ void stencil(const Box& b, const int32_t* __restrict__ state, long lo
     const int dir[4] = {1 + ax, 15 + ax, 4 + ax, 18 + ax};
     for (int m = 0; m < 4; ++m) ok[m + 1] = ok[m + 1] && state[(size_t)dir[m] * b.n + k] >= 0;
   }
+  if (filled)
+    for (int m = 1; m < 5; ++m) ok[m] = ok[m] && filled[nd[m]] != 0;
 }
 
 template <bool kShifted>
@@ -128,14 +131,16 @@ __global__ __launch_bounds__(256) void k27_tag_fields(const void* __restrict__ f
 
 // |grad src| per node; 0 on nodes whose code is < 0 (skip_null: except -666,
 // as calculate_grad_mag.f95:37 has it)
+// (filled, when given: only filled positions are computed or usable)
 __global__ __launch_bounds__(256) void k27_tag_gradmag(const double* __restrict__ src,
                                                        const double* __restrict__ vel, Box b,
                                                        const int32_t* __restrict__ state,
+                                                       const uint8_t* __restrict__ filled,
                                                        int scheme, int skip_null,
                                                        double* __restrict__ out) {
   const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= b.n) return;
-  if (state && state[k] < 0 && !(skip_null && state[k] == -666)) {
+  if ((filled && !filled[k]) || (state && state[k] < 0 && !(skip_null && state[k] == -666))) {
     out[k] = 0.0;
     return;
   }
@@ -144,7 +149,7 @@ __global__ __launch_bounds__(256) void k27_tag_gradmag(const double* __restrict_
   for (int ax = 0; ax < 3; ++ax) {
     long long nd[5];
     bool ok[5];
-    stencil(b, state, k, x, y, z, ax, nd, ok);
+    stencil(b, state, filled, k, x, y, z, ax, nd, ok);
     double qv[5];
     for (int m = 0; m < 5; ++m) qv[m] = nd[m] >= 0 ? src[nd[m]] : 0.0;
     d[ax] = windage2(qv, ok, vel[(size_t)ax * b.n + k], scheme);
@@ -163,6 +168,72 @@ __global__ __launch_bounds__(256) void k27_tag_mark(const double* __restrict__ g
   tags[k] = valid && ggm[k] >= limit ? 1 : 0;
 }
 
+// A level of many boxes on a dense array over the bounding box of its stored
+// nodes: pass 0 places the ghost-shell nodes (filled 1, every link usable),
+// pass 1 the valid nodes over them (filled 2, the level's link codes).  A
+// position no box stores stays unfilled: never computed, never usable.
+__global__ __launch_bounds__(256) void k27_tag_scatter(
+    const double* __restrict__ q_s, const double* __restrict__ vel_s, long long ns,
+    const int32_t* __restrict__ state_s, const BoxDesc* __restrict__ boxes,
+    const int* __restrict__ blo, int3 dlo, Box d, int pass, double* __restrict__ q_d,
+    double* __restrict__ vel_d, uint8_t* __restrict__ filled, int32_t* __restrict__ state_d) {
+  const unsigned box = blockIdx.y;
+  const BoxDesc bx = boxes[box];
+  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= bx.n) return;
+  const unsigned nz = bx.n / (bx.nx * bx.ny);
+  const unsigned lx = t % bx.nx, ly = (t / bx.nx) % bx.ny, lz = t / (bx.nx * bx.ny);
+  const bool shell = lx == 0 || ly == 0 || lz == 0 || lx == bx.nx - 1 || ly == bx.ny - 1 ||
+                     lz == nz - 1;
+  if (shell != (pass == 0)) return;
+  const long long gx = blo[3 * box] - 1 + (int)lx - dlo.x, gy = blo[3 * box + 1] - 1 + (int)ly - dlo.y,
+                  gz = blo[3 * box + 2] - 1 + (int)lz - dlo.z;
+  const long long kd = gx + (long long)d.nx * (gy + (long long)d.ny * gz);
+  const long long ks = (long long)bx.off + t;
+  q_d[kd] = q_s[ks];
+  for (int c = 0; c < 3; ++c) vel_d[c * d.n + kd] = vel_s[c * ns + ks];
+  filled[kd] = pass == 0 ? 1 : 2;
+  if (state_d)
+    for (int i = 0; i < 27; ++i)
+      state_d[(size_t)i * d.n + kd] = pass == 0 || !state_s ? 0 : state_s[(size_t)i * ns + ks];
+}
+
+// dense ggm and tags back to the stored nodes (valid nodes; 0 on the shell)
+__global__ __launch_bounds__(256) void k27_tag_gather(
+    const double* __restrict__ ggm_d, const uint8_t* __restrict__ tag_d,
+    const BoxDesc* __restrict__ boxes, const int* __restrict__ blo, int3 dlo, Box d,
+    double* __restrict__ ggm_s, uint8_t* __restrict__ tag_s) {
+  const unsigned box = blockIdx.y;
+  const BoxDesc bx = boxes[box];
+  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= bx.n) return;
+  const unsigned nz = bx.n / (bx.nx * bx.ny);
+  const unsigned lx = t % bx.nx, ly = (t / bx.nx) % bx.ny, lz = t / (bx.nx * bx.ny);
+  const bool shell = lx == 0 || ly == 0 || lz == 0 || lx == bx.nx - 1 || ly == bx.ny - 1 ||
+                     lz == nz - 1;
+  const long long ks = (long long)bx.off + t;
+  if (shell) {
+    ggm_s[ks] = 0.0;
+    tag_s[ks] = 0;
+    return;
+  }
+  const long long gx = blo[3 * box] - 1 + (int)lx - dlo.x, gy = blo[3 * box + 1] - 1 + (int)ly - dlo.y,
+                  gz = blo[3 * box + 2] - 1 + (int)lz - dlo.z;
+  const long long kd = gx + (long long)d.nx * (gy + (long long)d.ny * gz);
+  ggm_s[ks] = ggm_d[kd];
+  tag_s[ks] = tag_d[kd];
+}
+
+// tags of the filled-2 (valid) positions of the dense array
+__global__ __launch_bounds__(256) void k27_tag_mark_dense(const double* __restrict__ ggm,
+                                                          const uint8_t* __restrict__ filled,
+                                                          long long n, double limit,
+                                                          uint8_t* __restrict__ tags) {
+  const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  tags[k] = filled[k] == 2 && ggm[k] >= limit ? 1 : 0;
+}
+
 unsigned tag_blocks(long long n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
@@ -178,10 +249,50 @@ hipError_t launch_d3q27_ggm(const Box& b, const void* f, int shifted, const int3
   const unsigned g = tag_blocks(b.n);
   if (shifted) k27_tag_fields<true><<<g, 256, 0, s>>>(f, b, quantity, q, vel);
   else k27_tag_fields<false><<<g, 256, 0, s>>>(f, b, quantity, q, vel);
-  k27_tag_gradmag<<<g, 256, 0, s>>>(q, vel, b, state, scheme, 1, gm);
-  k27_tag_gradmag<<<g, 256, 0, s>>>(gm, vel, b, state, scheme, 0, ggm);
+  k27_tag_gradmag<<<g, 256, 0, s>>>(q, vel, b, state, nullptr, scheme, 1, gm);
+  k27_tag_gradmag<<<g, 256, 0, s>>>(gm, vel, b, state, nullptr, scheme, 0, ggm);
   k27_tag_mark<<<g, 256, 0, s>>>(ggm, b, limit, make_int3(vlo[0], vlo[1], vlo[2]),
                                  make_int3(vhi[0], vhi[1], vhi[2]), tags);
+  return hipGetLastError();
+}
+
+// A refined level of many boxes: the fields on the stored nodes (stored work
+// 4 ns doubles), placed on the dense bounding box d (origin dlo in level
+// indices; dense work 6 nd doubles, filled and tags nd bytes each, state_d
+// 27 nd int32 when the level has a link table), the indicator there, back to
+// the stored nodes.  blo: [nbox][3] device copy of the boxes' valid lo.
+hipError_t launch_d3q27_ggm_boxes(const void* f, long long ns, int shifted, const int32_t* state,
+                                  const BoxDesc* boxes, const int* blo, unsigned nbox,
+                                  unsigned max_box_n, const int dlo[3], const Box& d, int quantity,
+                                  int scheme, double limit, double* work_s, double* work_d,
+                                  uint8_t* filled, uint8_t* tag_d, int32_t* state_d,
+                                  double* ggm_s, uint8_t* tag_s, hipStream_t s) {
+  Box bs{};
+  bs.nx = 1;
+  bs.ny = 1;
+  bs.nz = 1;
+  bs.n = ns;
+  double* q_s = work_s;
+  double* vel_s = work_s + ns;
+  double* q_d = work_d;
+  double* vel_d = work_d + d.n;
+  double* gm_d = work_d + 4 * d.n;
+  double* ggm_d = work_d + 5 * d.n;
+  const unsigned gs = tag_blocks(ns), gd = tag_blocks(d.n);
+  if (shifted) k27_tag_fields<true><<<gs, 256, 0, s>>>(f, bs, quantity, q_s, vel_s);
+  else k27_tag_fields<false><<<gs, 256, 0, s>>>(f, bs, quantity, q_s, vel_s);
+  hipError_t e = launch_fill(work_d, 4 * d.n, 0.0, s);
+  if (e == hipSuccess) e = hipMemsetAsync(filled, 0, (size_t)d.n, s);
+  if (e != hipSuccess) return e;
+  const dim3 grid(tag_blocks(max_box_n), nbox);
+  const int3 o = make_int3(dlo[0], dlo[1], dlo[2]);
+  for (int pass = 0; pass < 2; ++pass)
+    k27_tag_scatter<<<grid, 256, 0, s>>>(q_s, vel_s, ns, state, boxes, blo, o, d, pass, q_d, vel_d,
+                                         filled, state_d);
+  k27_tag_gradmag<<<gd, 256, 0, s>>>(q_d, vel_d, d, state_d, filled, scheme, 1, gm_d);
+  k27_tag_gradmag<<<gd, 256, 0, s>>>(gm_d, vel_d, d, state_d, filled, scheme, 0, ggm_d);
+  k27_tag_mark_dense<<<gd, 256, 0, s>>>(ggm_d, filled, d.n, limit, tag_d);
+  k27_tag_gather<<<grid, 256, 0, s>>>(ggm_d, tag_d, boxes, blo, o, d, ggm_s, tag_s);
   return hipGetLastError();
 }
 

@@ -205,7 +205,8 @@ hipError_t launch_copy_pairs27(const void* src, unsigned nn_src, void* dst, unsi
 // results replaced by the equilibrium of the interpolated moments
 hipError_t launch_c2f_stencils27(const void* cprev, const void* cnew, unsigned nn_c, void* ff,
                                  unsigned nn_f, const C2FStencil* st, unsigned nst, double theta,
-                                 int shifted, unsigned long long* stats, hipStream_t s);
+                                 int shifted, unsigned long long* stats, hipStream_t s,
+                                 int all_pops = 0);
 
 // ---- learned alpha -------------------------------------------------------------
 constexpr int kMlpParams = 9217;  // 12-64-64-64-1
@@ -344,6 +345,14 @@ hipError_t launch_d3q27_ggm(const Box& b, const void* f, int shifted, const int3
                             int quantity, int scheme, double limit, const int vlo[3],
                             const int vhi[3], double* work, double* ggm, uint8_t* tags,
                             hipStream_t s);
+// the same on a refined level of many boxes, through a dense array over its
+// stored nodes' bounding box d (origin dlo)
+hipError_t launch_d3q27_ggm_boxes(const void* f, long long ns, int shifted, const int32_t* state,
+                                  const BoxDesc* boxes, const int* blo, unsigned nbox,
+                                  unsigned max_box_n, const int dlo[3], const Box& d, int quantity,
+                                  int scheme, double limit, double* work_s, double* work_d,
+                                  uint8_t* filled, uint8_t* tag_d, int32_t* state_d,
+                                  double* ggm_s, uint8_t* tag_s, hipStream_t s);
 hipError_t launch_d3q39_probe(const void* f, const void* g, int64_t n, int fp32, int shifted,
                               const Ref39& rf, double cv, const uint32_t* nodes, unsigned m,
                               double* out, hipStream_t s);

@@ -2170,7 +2170,9 @@ static uclbs::LevelMap level_map(const Level& L) {
   return m;
 }
 
-static int interlevel_fill(Ctx* c, int lvl, double theta, bool all_nodes) {
+// all_pops: a ghost takes all 27 interpolated populations, not only those its
+// valid neighbours pull (for quantities computed on the ghost shell)
+static int interlevel_fill(Ctx* c, int lvl, double theta, bool all_nodes, bool all_pops = false) {
   Level* F;
   int rc = check_level(c, lvl, &F);
   if (rc) return rc;
@@ -2201,7 +2203,8 @@ static int interlevel_fill(Ctx* c, int lvl, double theta, bool all_nodes) {
     if (!all_nodes)
       return hip_rc(uclbs::launch_c2f_stencils27(C.f[1 - C.cur], C.f[C.cur], (unsigned)C.box.n,
                                                  F->f[F->cur], (unsigned)F->box.n, F->d_c2f,
-                                                 F->nc2f, theta, sh, c->d_stats, c->stream));
+                                                 F->nc2f, theta, sh, c->d_stats, c->stream,
+                                                 all_pops));
     // start-up: every valid node (and every ghost the coarse level reaches)
     std::vector<uclbs::C2FStencil> all;
     for (size_t b = 0; b < F->hbox.size(); ++b) {
@@ -2387,29 +2390,77 @@ int uclbs_tag_ggm(void* ctx, int lvl, int quantity, int scheme, double limit, do
   if (rc) return rc;
   if (c->cfg.lattice != UCLBS_LATTICE_D3Q27)
     return set_err(UCLBS_E_UNSUPPORTED, "tag_ggm: D3Q27 levels");
-  if (L->multibox)
-    return set_err(UCLBS_E_UNSUPPORTED, "tag_ggm: one-box levels (level %d has many boxes)", lvl);
+  if (L->multibox && !L->refined)
+    return set_err(UCLBS_E_UNSUPPORTED, "tag_ggm: a level 0 of many boxes (periodic) is not tagged");
   if (quantity != 1 && quantity != 2 && quantity != 3 && quantity != 4 && quantity != 7)
     return set_err(UCLBS_E_ARG, "tag_ggm: quantity %d (1 density, 2 u, 3 v, 4 w, 7 |u|)", quantity);
   if (scheme < 0 || scheme > 2)
     return set_err(UCLBS_E_ARG, "tag_ggm: scheme %d (0 central, 1 upwind, 2 downwind)", scheme);
   if ((rc = set_device(c))) return rc;
+  // a refined level's ghost shell holds last step's values; the stencils
+  // reach into it, so it is filled from the coarser level's current state
+  // first (the level's next step fills it again before stepping)
+  if (L->refined && (rc = interlevel_fill(c, lvl, 1.0, false, true))) return rc;
   const int64_t n = L->box.n;
-  int vlo[3], vhi[3];
-  for (int d = 0; d < 3; ++d) {
-    vlo[d] = L->lo[d] - L->origin[d];
-    vhi[d] = L->hi[d] - L->origin[d];
+  const int shifted = c->cfg.precision == UCLBS_PREC_FP32_SHIFTED;
+  double* work = nullptr;     // one box: 6 n doubles; many: 4 n stored + 6 nd dense + n ggm
+  uint8_t* dtag = nullptr;    // tags of the stored nodes
+  uint8_t* dense8 = nullptr;  // many boxes: filled and dense tags
+  int32_t* dstate = nullptr;  // many boxes with a link table: dense link codes
+  int* dblo = nullptr;
+  double* ggm_out = nullptr;
+  hipError_t e = hipSuccess;
+  if (!L->multibox) {
+    int vlo[3], vhi[3];
+    for (int d = 0; d < 3; ++d) {
+      vlo[d] = L->lo[d] - L->origin[d];
+      vhi[d] = L->hi[d] - L->origin[d];
+    }
+    e = hipMalloc(&work, (size_t)n * 6 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&dtag, (size_t)n);
+    if (e == hipSuccess)
+      e = uclbs::launch_d3q27_ggm(L->box, L->f[L->cur], shifted, L->state, quantity, scheme,
+                                  limit, vlo, vhi, work, work + 5 * n, dtag, c->stream);
+    ggm_out = work + 5 * n;
+  } else {
+    // the bounding box of the stored nodes (ghost shells included)
+    const size_t nbox = L->hbox.size();
+    int dlo[3], dhi[3];
+    unsigned maxn = 0;
+    for (int d = 0; d < 3; ++d) {
+      dlo[d] = L->blo[d] - 1;
+      dhi[d] = L->bhi[d] + 1;
+    }
+    for (size_t b = 0; b < nbox; ++b) {
+      maxn = std::max(maxn, L->hbox[b].n);
+      for (int d = 0; d < 3; ++d) {
+        dlo[d] = std::min(dlo[d], L->blo[3 * b + d] - 1);
+        dhi[d] = std::max(dhi[d], L->bhi[3 * b + d] + 1);
+      }
+    }
+    uclbs::Box db{};
+    db.nx = dhi[0] - dlo[0] + 1;
+    db.ny = dhi[1] - dlo[1] + 1;
+    db.nz = dhi[2] - dlo[2] + 1;
+    db.n = (int64_t)db.nx * db.ny * db.nz;
+    const size_t nd = (size_t)db.n;
+    e = hipMalloc(&work, ((size_t)n * 5 + nd * 6) * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&dtag, (size_t)n);
+    if (e == hipSuccess) e = hipMalloc(&dense8, 2 * nd);
+    if (e == hipSuccess && L->state) e = hipMalloc(&dstate, 27 * nd * sizeof(int32_t));
+    if (e == hipSuccess) e = hipMalloc(&dblo, 3 * nbox * sizeof(int));
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(dblo, L->blo.data(), 3 * nbox * sizeof(int), hipMemcpyHostToDevice,
+                         c->stream);
+    ggm_out = work + 4 * (size_t)n + 6 * nd;
+    if (e == hipSuccess)
+      e = uclbs::launch_d3q27_ggm_boxes(L->f[L->cur], n, shifted, L->state, L->d_boxes, dblo,
+                                        (unsigned)nbox, maxn, dlo, db, quantity, scheme, limit,
+                                        work, work + 4 * (size_t)n, dense8, dense8 + nd, dstate,
+                                        ggm_out, dtag, c->stream);
   }
-  double* work = nullptr;
-  uint8_t* dtag = nullptr;
-  hipError_t e = hipMalloc(&work, (size_t)n * 6 * sizeof(double));
-  if (e == hipSuccess) e = hipMalloc(&dtag, (size_t)n);
-  if (e == hipSuccess)
-    e = uclbs::launch_d3q27_ggm(L->box, L->f[L->cur], c->cfg.precision == UCLBS_PREC_FP32_SHIFTED,
-                                L->state, quantity, scheme, limit, vlo, vhi, work, work + 5 * n,
-                                dtag, c->stream);
   if (e == hipSuccess && ggm)
-    e = hipMemcpyAsync(ggm, work + 5 * n, (size_t)n * sizeof(double), hipMemcpyDeviceToHost,
+    e = hipMemcpyAsync(ggm, ggm_out, (size_t)n * sizeof(double), hipMemcpyDeviceToHost,
                        c->stream);
   std::vector<uint8_t> ht;
   if (e == hipSuccess && (tags || ntagged)) {
@@ -2417,8 +2468,8 @@ int uclbs_tag_ggm(void* ctx, int lvl, int quantity, int scheme, double limit, do
     e = hipMemcpyAsync(ht.data(), dtag, (size_t)n, hipMemcpyDeviceToHost, c->stream);
   }
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  if (work) (void)hipFree(work);
-  if (dtag) (void)hipFree(dtag);
+  for (void* p : {(void*)work, (void*)dtag, (void*)dense8, (void*)dstate, (void*)dblo})
+    if (p) (void)hipFree(p);
   if (e != hipSuccess) return set_err(UCLBS_E_HIP, "tag_ggm: %s", hipGetErrorString(e));
   if (tags) std::memcpy(tags, ht.data(), (size_t)n);
   if (ntagged) {
