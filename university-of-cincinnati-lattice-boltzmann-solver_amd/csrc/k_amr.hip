@@ -157,10 +157,16 @@ __global__ __launch_bounds__(256) void k27_c2f_stencils(const void* __restrict__
     const double w = (double)s.w[q];
     if (w == 0.0) continue;
     const unsigned k = s.src[q];
+    if (theta == 0.0) {
+      // (1 - 0) a + 0 b is a: the current state is not read
 #pragma unroll
-    for (int i = 0; i < 27; ++i) {
-      const double a = (double)cprev[(size_t)i * cnn + k], b = (double)cnew[(size_t)i * cnn + k];
-      f[i] += w * ((1.0 - theta) * a + theta * b);
+      for (int i = 0; i < 27; ++i) f[i] += w * (double)cprev[(size_t)i * cnn + k];
+    } else {
+#pragma unroll
+      for (int i = 0; i < 27; ++i) {
+        const double a = (double)cprev[(size_t)i * cnn + k], b = (double)cnew[(size_t)i * cnn + k];
+        f[i] += w * ((1.0 - theta) * a + theta * b);
+      }
     }
   }
   repair_store27<kShifted>(f, static_cast<T*>(ff_), fnn, s.dst, stats, s.mask | extra_mask);

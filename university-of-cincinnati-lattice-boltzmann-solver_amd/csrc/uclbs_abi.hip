@@ -1035,7 +1035,12 @@ static int finalize_links_boxes(Ctx* c, Level* L) {
           const int64_t src = valid_index(*L, G, nullptr);
           if (src >= 0 && hf[(size_t)src] != uclbs::kFlagInterior) {
             hf[k] = uclbs::kFlagGhost;
-            copies.push_back(make_uint3(k, (unsigned)src, uclbs::kAllPops27));
+            // a solid node's populations are read by no update (a fluid
+            // node's link to it is a wall link, rebuilt by the boundary
+            // kernel), so its ghosts are not refreshed each step; those of
+            // boundary nodes are
+            if (hf[(size_t)src] != uclbs::kFlagSolid)
+              copies.push_back(make_uint3(k, (unsigned)src, uclbs::kAllPops27));
           }
         }
   }
