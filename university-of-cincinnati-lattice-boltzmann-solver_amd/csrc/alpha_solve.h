@@ -138,6 +138,44 @@ __device__ __forceinline__ double log1p_dev(double x) {
   return de * ln2_hi + (de * ln2_lo + s * p) - c * __builtin_amdgcn_rcp(u);
 }
 
+// ln(1 + y) for y > -1 through a 64-entry table over the mantissa: u = 1 + y
+// = m 2^e with m in [1, 2), c_k = 1 + (k + 1/2)/64 the centre of m's 1/64
+// interval, inv_k = 1/c_k, r = m inv_k - 1 (|r| < 1/128, one FMA) and
+// ln u = e ln 2 + L_k + ln(1 + r), L_k = -ln(inv_k), ln(1 + r) by its series
+// to r^7 (|r|^8 / 8 < 2e-18).  The rounding of 1 + y is not corrected, so
+// the absolute error is up to 1.2e-16: in g = sum f ((1+y) ln(1+y) - y) it
+// stays below the acceptance |g| <= tol S2 (tol 1e-10) while S2 > 1e-5 rho,
+// as on the far nodes this serves; a node below would report no convergence,
+// not a wrong root.  14 FP64 operations and two table reads where log1p_dev
+// takes about 30.
+struct LogTable {
+  double inv[64], L[64];
+};
+// entry k of the table (one lane per entry)
+__device__ __forceinline__ void log_table_entry(LogTable& t, int k) {
+  const double inv = 1.0 / (1.0 + (k + 0.5) / 64.0);
+  t.inv[k] = inv;
+  t.L[k] = -log1p_dev(inv - 1.0);  // inv in (0.5, 1): inv - 1 is exact
+}
+__device__ __forceinline__ double log1p_tab(double y, const LogTable& t) {
+  const double u = 1.0 + y;
+  int e;
+  const double m = 2.0 * __builtin_frexp(u, &e);
+  const int k = (int)((m - 1.0) * 64.0);
+  const double r = fma(m, t.inv[k], -1.0);
+  double p = 1.0 / 7.0;
+  p = fma(p, r, -1.0 / 6.0);
+  p = fma(p, r, 1.0 / 5.0);
+  p = fma(p, r, -0.25);
+  p = fma(p, r, 1.0 / 3.0);
+  p = fma(p, r, -0.5);
+  p = fma(p, r, 1.0);
+  const double ln2_hi = 6.93147180369123816490e-01;
+  const double ln2_lo = 1.90821492927058770002e-10;
+  const double de = (double)(e - 1);
+  return (de * ln2_hi + t.L[k]) + (de * ln2_lo + p * r);
+}
+
 // phi(y) = (1 + y) ln(1 + y) - y, series for small |y|, direct otherwise
 __device__ __forceinline__ double phi1p(double y) {
   if (fabs(y) < 0.1) {
@@ -231,8 +269,9 @@ __device__ __forceinline__ AlphaOut newton_alpha(Eval&& ev, double S2, double am
 // or of the accepted root, as Newton counts its updates.
 // dx(i, d, x) yields D_i and x_i in FP64, fx(i) yields f_i; K = sum D ln(1+x).
 // kUnroll: unroll factor of the sweep over i (1 when dx / fx read memory).
-template <int Q, int N, int kUnroll, class DX, class FX>
-__device__ __forceinline__ AlphaOut recentred_alpha(DX&& dx, FX&& fx, double K, double S2,
+// lg(y): ln(1 + y) (log1p_dev or log1p_tab).
+template <int Q, int N, int kUnroll, class DX, class FX, class LG>
+__device__ __forceinline__ AlphaOut recentred_alpha(DX&& dx, FX&& fx, LG&& lg, double K, double S2,
                                                     double amax, double alpha0, double tol,
                                                     int maxit) {
   double lo = 1.0, hi = amax < 1e6 ? amax : 1e6;
@@ -241,9 +280,12 @@ __device__ __forceinline__ AlphaOut recentred_alpha(DX&& dx, FX&& fx, double K, 
   AlphaOut r{2.0, 0, ALPHA_NEWTON, true};
   for (int k = 0;; ++k) {
     double g = 0.0, gp = 0.0, zmax = 0.0;
-    double T[N + 1];
+    // the polynomial's coefficients only steer the step (g and g' stay FP64
+    // and decide acceptance), so they are summed in FP32; their rounding
+    // enters the acceptance bound below
+    float Tf[N + 1];
 #pragma unroll
-    for (int n = 2; n <= N; ++n) T[n] = 0.0;
+    for (int n = 2; n <= N; ++n) Tf[n] = 0.0f;
     bool edge = false;  // a at the positivity bound: some 1 + a x_i <= 0
 #pragma unroll kUnroll
     for (int i = 0; i < Q; ++i) {
@@ -253,16 +295,20 @@ __device__ __forceinline__ AlphaOut recentred_alpha(DX&& dx, FX&& fx, double K, 
       const double y = a * x;
       const double A = 1.0 + y;
       if (A <= 0.0) { g -= f * y; edge = true; continue; }  // limit of (1+y) ln(1+y) at 0
-      const double la = log1p_dev(y);
+      const double la = lg(y);
       g += f * (A * la - y);
       gp += d * la;
       const double z = x * drcp(A);
       zmax = fmax(zmax, fabs(z));
-      double t = d * z;
-      T[2] += t;
+      const float zf = (float)z;
+      float t = (float)d * zf;
+      Tf[2] += t;
 #pragma unroll
-      for (int n = 3; n <= N; ++n) { t *= z; T[n] += t; }
+      for (int n = 3; n <= N; ++n) { t *= zf; Tf[n] += t; }
     }
+    double T[N + 1];
+#pragma unroll
+    for (int n = 2; n <= N; ++n) T[n] = (double)Tf[n];
     g -= a * K;
     gp -= K;
     if (fabs(g) <= tol * S2) { r.alpha = a; r.iters = k; return r; }
@@ -304,6 +350,15 @@ __device__ __forceinline__ AlphaOut recentred_alpha(DX&& dx, FX&& fx, double K, 
 #pragma unroll
       for (int n = 0; n < N - 1; ++n) qp *= q;
       rem = T[2] * dd * dd * qp / (N * (N + 1.0) * (1.0 - q));
+      // the FP32 coefficients: relative error below 1e-6 (27 terms summed),
+      // times the size of each polynomial term at dd
+      double dn = dd * dd, terms = 0.0;
+#pragma unroll
+      for (int n = 2; n <= N; ++n) {
+        terms += fabs(T[n]) * dn / (n * (n - 1.0));
+        dn *= fabs(dd);
+      }
+      rem += 1e-6 * terms;
     }
     if (ok && fabs(p) + rem <= tol * S2) { r.alpha = a + dd; r.iters = k + 1; return r; }
     // next centre: the polynomial's step where it stays in the bracket and the

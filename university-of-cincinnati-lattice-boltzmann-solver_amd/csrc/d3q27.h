@@ -228,7 +228,7 @@ __device__ __forceinline__ AlphaOut collide27_far(float (&f)[27], double beta, c
                                                   double seed, float* __restrict__ sf,
                                                   float* __restrict__ sD, bool active,
                                                   float* __restrict__ fout, unsigned nn,
-                                                  unsigned node) {
+                                                  unsigned node, const LogTable& lt) {
   double rho_unused, u_unused[3];
   AlphaOut ao;
   float xmin = 0.0f;
@@ -259,14 +259,15 @@ __device__ __forceinline__ AlphaOut collide27_far(float (&f)[27], double beta, c
       x = d * drcp(W27[i] + (double)sf[i * kFarBlock]);
     };
     auto fx = [&](int i) __attribute__((always_inline)) { return W27[i] + (double)sf[i * kFarBlock]; };
+    auto lg = [&](double y) __attribute__((always_inline)) { return log1p_tab(y, lt); };
     double K = 0.0;
 #pragma unroll 3
     for (int i = 0; i < 27; ++i) {
       double d, x;
       dx(i, d, x);
-      K += d * log1p_dev(x);
+      K += d * lg(x);
     }
-    ao = recentred_alpha<27, kRecentreOrder, 3>(dx, fx, K, S2, amax, seed, ac.tol, ac.maxit);
+    ao = recentred_alpha<27, kRecentreOrder, 3>(dx, fx, lg, K, S2, amax, seed, ac.tol, ac.maxit);
   }
   const float ab = (float)(ao.alpha * beta);
   if (active) {

@@ -291,6 +291,9 @@ __global__ __launch_bounds__(kFarBlock) void k27_far(float* __restrict__ f_, dou
   // the next step's lengths start from zero (the previous far kernel read them)
   if (blockIdx.x == 0)
     for (unsigned s = threadIdx.x; s <= (unsigned)kFarShards; s += blockDim.x) a.defer_next[s] = 0u;
+  // the logarithm table, one entry per lane (kFarBlock = 64 entries)
+  __shared__ LogTable lt;
+  log_table_entry(lt, (int)threadIdx.x);
   // exclusive prefix of the shard lengths: lane l sums shards 5l .. 5l+4
   __shared__ unsigned pre[kFarShards + 2];
   {
@@ -344,7 +347,7 @@ __global__ __launch_bounds__(kFarBlock) void k27_far(float* __restrict__ f_, dou
     for (int i = 0; i < 27; ++i) f[i] = (f_ + (size_t)i * nn)[node];
     const double seed = (double)__uint_as_float(e.y);
     AlphaOut ao = collide27_far(f, a.beta, ac, seed, sf + threadIdx.x, sD + threadIdx.x, active,
-                                f_, nn, node);
+                                f_, nn, node, lt);
     if (counted && keep_alpha) alpha[node] = ao.alpha;
     wave_stats(stats, counted, ao.iters, ao.status, false, ao.logpath);
   }
