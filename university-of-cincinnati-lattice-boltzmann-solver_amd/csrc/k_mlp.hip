@@ -309,7 +309,18 @@ __global__ void k_normalise(float* __restrict__ y, unsigned long long n, const d
 // holds column l%16, the same k; D (16x16 FP32): lane l holds column l%16,
 // rows 4(l/16)..4(l/16)+3.
 typedef __attribute__((__vector_size__(4 * sizeof(short)))) short bf16x4;
+typedef __attribute__((__vector_size__(8 * sizeof(short)))) short bf16x8;
+typedef __bf16 bf16v8 __attribute__((ext_vector_type(8)));
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+__device__ __forceinline__ bf16x8 cat8(bf16x4 a, bf16x4 b) {
+  return bf16x8{a[0], a[1], a[2], a[3], b[0], b[1], b[2], b[3]};
+}
+// D += A B on v_mfma_f32_16x16x32_bf16 (two K-slots of eight per lane group)
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16v8, a),
+                                                 __builtin_bit_cast(bf16v8, b), c, 0, 0, 0);
+}
 
 __device__ __forceinline__ short bf16_bits(float x) {
   return (short)__bfloat16_as_ushort(__float2bfloat16(x));
@@ -377,17 +388,25 @@ __device__ __forceinline__ float mlp_wave(const float (&phi)[kIn], const MlpSmem
 #pragma unroll
   for (int j = 0; j < 16; ++j) feat[l * kWp1 + j] = bf16_bits(j < kIn ? phi[j] : 0.0f);
   __builtin_amdgcn_wave_barrier();
-  // A operands for this lane (same for every node group)
-  bf16x4 A1[4], A2[4][4], A3[4][4];
+  // A operands for this lane (same for every node group).  Layers 2 and 3 run
+  // on the K = 32 instruction: K-chunk c takes the previous layer's output
+  // tiles 2c and 2c + 1, whose rows this lane holds as 32c + g4 + (0..3) and
+  // 32c + 16 + g4 + (0..3); the weights take the same eight K indices, so the
+  // pairs line up whatever the instruction's K-slot order.
+  bf16x4 A1[4];
+  bf16x8 A2[4][2], A3[4][2];
   const short* w2 = sm.w + 64 * kWp1;
   const short* w3 = w2 + 64 * kWp;
 #pragma unroll
   for (int ot = 0; ot < 4; ++ot) {
     A1[ot] = *reinterpret_cast<const bf16x4*>(sm.w + (ot * 16 + r16) * kWp1 + g4);
 #pragma unroll
-    for (int kk = 0; kk < 4; ++kk) {
-      A2[ot][kk] = *reinterpret_cast<const bf16x4*>(w2 + (ot * 16 + r16) * kWp + kk * 16 + g4);
-      A3[ot][kk] = *reinterpret_cast<const bf16x4*>(w3 + (ot * 16 + r16) * kWp + kk * 16 + g4);
+    for (int c = 0; c < 2; ++c) {
+      const int row = (ot * 16 + r16) * kWp + 32 * c + g4;
+      A2[ot][c] = cat8(*reinterpret_cast<const bf16x4*>(w2 + row),
+                       *reinterpret_cast<const bf16x4*>(w2 + row + 16));
+      A3[ot][c] = cat8(*reinterpret_cast<const bf16x4*>(w3 + row),
+                       *reinterpret_cast<const bf16x4*>(w3 + row + 16));
     }
   }
   // biases and output weights of this lane's rows 4(l/16)..+3 of each output tile
@@ -403,21 +422,21 @@ __device__ __forceinline__ float mlp_wave(const float (&phi)[kIn], const MlpSmem
 #pragma unroll
     for (int ot = 0; ot < 4; ++ot)
       h1[ot] = pack_gelu(__builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A1[ot], x, bias(0, ot), 0, 0, 0));
+    const bf16x8 b1[2] = {cat8(h1[0], h1[1]), cat8(h1[2], h1[3])};
 #pragma unroll
     for (int ot = 0; ot < 4; ++ot) {
       f32x4 acc = bias(1, ot);
 #pragma unroll
-      for (int kk = 0; kk < 4; ++kk)
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A2[ot][kk], h1[kk], acc, 0, 0, 0);
+      for (int c = 0; c < 2; ++c) acc = mfma32(A2[ot][c], b1[c], acc);
       h2[ot] = pack_gelu(acc);
     }
+    const bf16x8 b2[2] = {cat8(h2[0], h2[1]), cat8(h2[2], h2[3])};
     float y = 0.0f;
 #pragma unroll
     for (int ot = 0; ot < 4; ++ot) {
       f32x4 acc = bias(2, ot);
 #pragma unroll
-      for (int kk = 0; kk < 4; ++kk)
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(A3[ot][kk], h2[kk], acc, 0, 0, 0);
+      for (int c = 0; c < 2; ++c) acc = mfma32(A3[ot][c], b2[c], acc);
       const f32x4 w4 = *reinterpret_cast<const f32x4*>(&sm.w4[ot * 16 + g4]);
 #pragma unroll
       for (int j = 0; j < 4; ++j) y = fmaf(gelu_fast(acc[j]), w4[j], y);
