@@ -381,11 +381,15 @@ int uclbs_mlp_forward(void *ctx, const float *X, int B, int use_mfma, float *yha
  * UCLBS_E_ARG.  Every call on one device shares that area, so calls that
  * overlap on one device (two host threads, two contexts) race. */
 int uclbs_mlp_forward_noalloc(void *ctx, const float *X, int B, float *yhat);
-/* device sample buffer for (features, y) training pairs */
+/* device sample buffer for (features, y) training pairs; capacity in
+ * 1 .. 2^31 - 1 pairs (about 130 bytes of device memory per pair, the
+ * ordering buffers included) */
 int uclbs_mlp_reserve(void *ctx, int64_t capacity);
 /* logs every stride-th interior node of the current state, advances the level
  * one plain step (alpha_mode must be UCLBS_ALPHA_NEWTON) and stores the
- * resulting alpha as the target; *n_samples = pairs held */
+ * resulting alpha as the target; *n_samples = pairs held.  Each call's pairs
+ * are held in node order, so identical runs hold identical training sets
+ * (pairs beyond the capacity are dropped) */
 int uclbs_mlp_collect(void *ctx, int lvl, int stride, int64_t *n_samples);
 int uclbs_mlp_download_samples(void *ctx, float *X, float *y, int64_t capacity, int64_t *n);
 /* node index (in the level's FAB order) of each held pair, in the order of

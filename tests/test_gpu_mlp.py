@@ -179,3 +179,29 @@ def test_learned_alpha_at_bench_schedule_duct(pkg):
         its[name] = st["newton_iters"] / st["node_updates"]
     print(f"Newton updates/node: plain {its['plain']:.3f}, learned {its['learned']:.3f}")
     assert its["learned"] < its["plain"]
+
+
+def test_collected_pairs_are_ordered_and_reproducible(pkg):
+    """Each collection's pairs sit in node order, whatever order the
+    wavefronts claimed their slots in, so two identical runs log bit-equal
+    training sets and train to bit-equal parameters."""
+    L = 24
+    runs = []
+    for _ in range(2):
+        s = _tgv(pkg, "newton", L)
+        s.step(0, 5)
+        s.mlp_reserve(3 * L ** 3)
+        n1 = s.mlp_collect(0, 1)
+        n2 = s.mlp_collect(0, 2)
+        X, y = s.mlp_samples(n2)
+        nodes = s.mlp_sample_nodes(n2)
+        for a, b in ((0, n1), (n1, n2)):
+            assert b - a > 1 and np.all(np.diff(nodes[a:b].astype(np.int64)) > 0)
+        s.mlp_init(7)
+        l0, l1 = s.mlp_train(50, 1024, 2e-3, 7)
+        runs.append((X, y, nodes, l0, l1, s.mlp_get_params()))
+    (X0, y0, n0, a0, b0, p0), (X1, y1, n1_, a1, b1, p1) = runs
+    assert np.array_equal(n0, n1_) and np.array_equal(X0, X1) and np.array_equal(y0, y1)
+    assert a0 == a1 and b0 == b1
+    for u, v in zip(p0, p1):
+        assert np.array_equal(np.asarray(u), np.asarray(v))

@@ -18,6 +18,7 @@
 #include "philox.h"
 
 #include <hip/hip_bf16.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 
@@ -157,6 +158,36 @@ __global__ __launch_bounds__(256) void k27_collect(const float* __restrict__ fin
     node_of[slot] = gid;
     m_of[slot] = (float)m;
   }
+}
+
+__global__ void k_iota(unsigned* __restrict__ v, unsigned n) {
+  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) v[t] = t;
+}
+
+// rows first + perm[k] -> staging row k, then back to rows first + k
+__global__ void k_order_gather(const unsigned* __restrict__ perm, unsigned long long first,
+                               unsigned n, const float* __restrict__ X, const float* __restrict__ m,
+                               float* __restrict__ X2, float* __restrict__ m2) {
+  const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const unsigned long long src = first + perm[k];
+#pragma unroll
+  for (int j = 0; j < kIn; ++j) X2[(size_t)k * kIn + j] = X[src * kIn + j];
+  m2[k] = m[src];
+}
+
+__global__ void k_order_scatter(unsigned long long first, unsigned n, const float* __restrict__ X2,
+                                const float* __restrict__ m2, const unsigned* __restrict__ key2,
+                                float* __restrict__ X, float* __restrict__ m,
+                                unsigned* __restrict__ node_of) {
+  const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const unsigned long long dst = first + k;
+#pragma unroll
+  for (int j = 0; j < kIn; ++j) X[dst * kIn + j] = X2[(size_t)k * kIn + j];
+  m[dst] = m2[k];
+  node_of[dst] = key2[k];
 }
 
 __global__ void k_collect_targets(const double* __restrict__ alpha, const unsigned* __restrict__ node_of,
@@ -524,6 +555,36 @@ hipError_t launch_mlp_collect(const Box& b, const float* fin, const uint8_t* fla
   const unsigned blocks = (unsigned)((b.n + 255) / 256);
   hipLaunchKernelGGL(k27_collect, dim3(blocks), dim3(256), 0, st, fin, b, flags, stride, phase,
                      s.X, s.node_of, s.m_of, s.count, (unsigned long long)s.cap);
+  return hipGetLastError();
+}
+
+hipError_t mlp_sort_bytes(int64_t cap, size_t* bytes) {
+  *bytes = 0;
+  return hipcub::DeviceRadixSort::SortPairs(nullptr, *bytes, (const unsigned*)nullptr,
+                                            (unsigned*)nullptr, (const unsigned*)nullptr,
+                                            (unsigned*)nullptr, (int)cap);
+}
+
+// The slots [first, last) filled by one collection, reordered by node: a
+// stable radix sort of (node, slot) and a permutation of the rows through the
+// staging buffer.  Nodes within one collection are distinct.
+hipError_t launch_mlp_order(MlpSamples& s, unsigned long long first, unsigned long long last,
+                            hipStream_t st) {
+  if (last <= first + 1) return hipSuccess;
+  if (last > (unsigned long long)s.cap || !s.sort_tmp) return hipErrorInvalidValue;
+  const unsigned n = (unsigned)(last - first);
+  unsigned* pin = s.perm;
+  unsigned* pout = s.perm + s.cap;
+  float* m2 = s.X2 + (size_t)s.cap * kIn;
+  const dim3 g((n + 255) / 256);
+  hipLaunchKernelGGL(k_iota, g, dim3(256), 0, st, pin, n);
+  size_t bytes = s.sort_bytes;
+  hipError_t e = hipcub::DeviceRadixSort::SortPairs(s.sort_tmp, bytes, s.node_of + first, s.key2,
+                                                    pin, pout, (int)n, 0, 32, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_order_gather, g, dim3(256), 0, st, pout, first, n, s.X, s.m_of, s.X2, m2);
+  hipLaunchKernelGGL(k_order_scatter, g, dim3(256), 0, st, first, n, s.X2, m2, s.key2, s.X,
+                     s.m_of, s.node_of);
   return hipGetLastError();
 }
 

@@ -794,7 +794,8 @@ int uclbs_destroy(void* ctx) {
   for (void* p : {(void*)c->mlp_p, (void*)c->mlp_g, (void*)c->mlp_m, (void*)c->mlp_v,
                   (void*)c->mlp_wbf, (void*)c->samples.X, (void*)c->samples.y,
                   (void*)c->samples.node_of, (void*)c->samples.m_of, (void*)c->samples.count,
-                  (void*)c->work_buf, (void*)c->batch_x, (void*)c->batch_idx, (void*)c->d_scalar})
+                  (void*)c->samples.key2, (void*)c->samples.perm, (void*)c->samples.X2,
+                  c->samples.sort_tmp, (void*)c->work_buf, (void*)c->batch_x, (void*)c->batch_idx, (void*)c->d_scalar})
     if (p) (void)hipFree(p);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -2863,11 +2864,12 @@ int uclbs_mlp_forward_noalloc(void* ctx, const float* X, int B, float* yhat) {
 
 int uclbs_mlp_reserve(void* ctx, int64_t capacity) {
   Ctx* c = static_cast<Ctx*>(ctx);
-  if (!c || capacity <= 0 || capacity > (1ll << 31)) return set_err(UCLBS_E_ARG, "bad capacity");
+  if (!c || capacity <= 0 || capacity > INT32_MAX) return set_err(UCLBS_E_ARG, "bad capacity");
   int rc = set_device(c);
   if (rc) return rc;
   uclbs::MlpSamples& m = c->samples;
-  for (void* p : {(void*)m.X, (void*)m.y, (void*)m.node_of, (void*)m.m_of, (void*)m.count})
+  for (void* p : {(void*)m.X, (void*)m.y, (void*)m.node_of, (void*)m.m_of, (void*)m.count,
+                  (void*)m.key2, (void*)m.perm, (void*)m.X2, m.sort_tmp})
     if (p) (void)hipFree(p);
   m = uclbs::MlpSamples{};
   HIP_TRY(hipMalloc(&m.X, (size_t)capacity * 12 * sizeof(float)));
@@ -2876,6 +2878,11 @@ int uclbs_mlp_reserve(void* ctx, int64_t capacity) {
   HIP_TRY(hipMalloc(&m.m_of, (size_t)capacity * sizeof(float)));
   HIP_TRY(hipMalloc(&m.count, sizeof(unsigned long long)));
   HIP_TRY(hipMemset(m.count, 0, sizeof(unsigned long long)));
+  HIP_TRY(hipMalloc(&m.key2, (size_t)capacity * sizeof(unsigned)));
+  HIP_TRY(hipMalloc(&m.perm, (size_t)capacity * 2 * sizeof(unsigned)));
+  HIP_TRY(hipMalloc(&m.X2, (size_t)capacity * 13 * sizeof(float)));
+  HIP_TRY(uclbs::mlp_sort_bytes(capacity, &m.sort_bytes));
+  HIP_TRY(hipMalloc(&m.sort_tmp, std::max<size_t>(m.sort_bytes, 256)));
   m.cap = capacity;
   c->samples_normalised = false;
   return UCLBS_OK;
@@ -2912,6 +2919,7 @@ int uclbs_mlp_collect(void* ctx, int lvl, int stride, int64_t* n_samples) {
   c->keep_alpha = keep;
   if (rc) return rc;
   if ((rc = sample_count(c, &last))) return rc;
+  HIP_TRY(uclbs::launch_mlp_order(c->samples, first, last, c->stream));
   HIP_TRY(uclbs::launch_mlp_targets(L->alpha, c->samples, first, last, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   *n_samples = (int64_t)last;
