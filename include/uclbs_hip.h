@@ -382,8 +382,8 @@ int uclbs_mlp_forward(void *ctx, const float *X, int B, int use_mfma, float *yha
  * overlap on one device (two host threads, two contexts) race. */
 int uclbs_mlp_forward_noalloc(void *ctx, const float *X, int B, float *yhat);
 /* device sample buffer for (features, y) training pairs; capacity in
- * 1 .. 2^31 - 1 pairs (about 130 bytes of device memory per pair, the
- * ordering buffers included) */
+ * 1 .. 2^31 - 1 pairs (60 bytes of device memory per pair; collecting
+ * adds 40 bytes per 256 nodes of the level) */
 int uclbs_mlp_reserve(void *ctx, int64_t capacity);
 /* logs every stride-th interior node of the current state, advances the level
  * one plain step (alpha_mode must be UCLBS_ALPHA_NEWTON) and stores the

@@ -18,7 +18,6 @@
 #include "philox.h"
 
 #include <hip/hip_bf16.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 
@@ -122,21 +121,20 @@ __device__ __forceinline__ void pull27(const float* __restrict__ fin, unsigned n
 }
 
 // Sample logging: every stride-th interior node with m above 1e-8 and finite
-// features appends (features, node, m) to the sample buffer; the target is filled after the
-// plain step has produced alpha (k_collect_targets).
-__global__ __launch_bounds__(256) void k27_collect(const float* __restrict__ fin, Box b,
-                                                   const uint8_t* __restrict__ flags, int stride,
-                                                   int phase, float* __restrict__ X,
-                                                   unsigned* __restrict__ node_of,
-                                                   float* __restrict__ m_of,
-                                                   unsigned long long* __restrict__ count,
-                                                   unsigned long long cap) {
-  const unsigned gid = blockIdx.x * blockDim.x + threadIdx.x;
-  const unsigned nn = (unsigned)b.n;
-  bool take = gid < nn && (int)(gid % (unsigned)stride) == phase;
+// features is a pair (features, node, m); the target is filled after the plain
+// step has produced alpha (k_collect_targets).  The pairs of one collection
+// land in node order: a counting pass records each wavefront's ballot of taken
+// nodes and each block's count, one block turns the counts into slot offsets
+// after the pairs already held, and a writing pass places each taken node at
+// its block's offset plus its rank in the block.
+__device__ __forceinline__ bool collect_take(const float* __restrict__ fin, const Box& b,
+                                             const uint8_t* __restrict__ flags, int stride,
+                                             int phase, unsigned gid, float (&phi)[kIn],
+                                             float& m) {
+  float f[27];
+  bool take = gid < (unsigned)b.n && (int)(gid % (unsigned)stride) == phase;
   if (take && flags) take = flags[gid] == kFlagInterior;
-  float f[27], phi[kIn];
-  float m = 0.0f;
+  m = 0.0f;
   if (take) {
     pull27(fin, gid, b, f);
     m = features27(f, phi);
@@ -145,49 +143,86 @@ __global__ __launch_bounds__(256) void k27_collect(const float* __restrict__ fin
 #pragma unroll
     for (int j = 0; j < kIn; ++j) take = take && __builtin_isfinite(phi[j]);
   }
+  return take;
+}
+
+__global__ __launch_bounds__(256) void k27_collect_count(const float* __restrict__ fin, Box b,
+                                                         const uint8_t* __restrict__ flags,
+                                                         int stride, int phase,
+                                                         unsigned long long* __restrict__ ballots,
+                                                         unsigned long long* __restrict__ blk) {
+  const unsigned gid = blockIdx.x * blockDim.x + threadIdx.x;
+  float phi[kIn], m;
+  const bool take = collect_take(fin, b, flags, stride, phase, gid, phi, m);
   const unsigned long long bm = __ballot(take);
-  if (!bm) return;
-  unsigned long long base = 0;
-  const int leader = __ffsll((long long)bm) - 1;
-  if ((int)__lane_id() == leader) base = atomicAdd(count, (unsigned long long)__popcll(bm));
-  base = __shfl(base, leader);
-  const unsigned long long slot = base + __popcll(bm & ((1ull << __lane_id()) - 1ull));
-  if (take && slot < cap) {
-#pragma unroll
-    for (int j = 0; j < kIn; ++j) X[slot * kIn + j] = phi[j];
-    node_of[slot] = gid;
-    m_of[slot] = (float)m;
+  __shared__ unsigned wc[4];
+  const int w = threadIdx.x / 64;
+  if (__lane_id() == 0) {
+    ballots[gid / 64] = bm;
+    wc[w] = (unsigned)__popcll(bm);
   }
+  __syncthreads();
+  if (threadIdx.x == 0) blk[blockIdx.x] = (unsigned long long)(wc[0] + wc[1] + wc[2] + wc[3]);
 }
 
-__global__ void k_iota(unsigned* __restrict__ v, unsigned n) {
-  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n) v[t] = t;
+// blk[i] <- count + sum of blk[0..i) (one block of 1024 threads); count <- the new total
+__global__ __launch_bounds__(1024) void k_scan_counts(unsigned long long* __restrict__ blk,
+                                                      unsigned nb,
+                                                      unsigned long long* __restrict__ count) {
+  __shared__ unsigned long long part[1024];
+  const unsigned t = threadIdx.x;
+  const unsigned chunk = (nb + 1023u) / 1024u;
+  const unsigned lo = min(nb, t * chunk), hi = min(nb, lo + chunk);
+  const unsigned long long base = *count;
+  unsigned long long own = 0;
+  for (unsigned i = lo; i < hi; ++i) own += blk[i];
+  part[t] = own;
+  __syncthreads();
+  for (unsigned off = 1; off < 1024u; off <<= 1) {
+    const unsigned long long v = t >= off ? part[t - off] : 0ull;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  unsigned long long run = base + part[t] - own;
+  for (unsigned i = lo; i < hi; ++i) {
+    const unsigned long long c = blk[i];
+    blk[i] = run;
+    run += c;
+  }
+  __syncthreads();
+  if (t == 0) *count = base + part[1023];
 }
 
-// rows first + perm[k] -> staging row k, then back to rows first + k
-__global__ void k_order_gather(const unsigned* __restrict__ perm, unsigned long long first,
-                               unsigned n, const float* __restrict__ X, const float* __restrict__ m,
-                               float* __restrict__ X2, float* __restrict__ m2) {
-  const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  const unsigned long long src = first + perm[k];
+__global__ __launch_bounds__(256) void k27_collect(const float* __restrict__ fin, Box b,
+                                                   const uint8_t* __restrict__ flags, int stride,
+                                                   int phase,
+                                                   const unsigned long long* __restrict__ ballots,
+                                                   const unsigned long long* __restrict__ blk,
+                                                   float* __restrict__ X,
+                                                   unsigned* __restrict__ node_of,
+                                                   float* __restrict__ m_of,
+                                                   unsigned long long cap) {
+  const unsigned gid = blockIdx.x * blockDim.x + threadIdx.x;
+  // the counting pass's decision, so the slots match its counts exactly
+  const unsigned long long bm = ballots[gid / 64];
+  const bool take = (bm >> __lane_id()) & 1ull;
+  __shared__ unsigned wc[4];
+  const int w = threadIdx.x / 64;
+  if (__lane_id() == 0) wc[w] = (unsigned)__popcll(bm);
+  __syncthreads();
+  if (!take) return;
+  unsigned before = 0;
+  for (int q = 0; q < w; ++q) before += wc[q];
+  const unsigned long long slot =
+      blk[blockIdx.x] + before + __popcll(bm & ((1ull << __lane_id()) - 1ull));
+  if (slot >= cap) return;
+  float phi[kIn], m;
+  (void)collect_take(fin, b, flags, stride, phase, gid, phi, m);
 #pragma unroll
-  for (int j = 0; j < kIn; ++j) X2[(size_t)k * kIn + j] = X[src * kIn + j];
-  m2[k] = m[src];
-}
-
-__global__ void k_order_scatter(unsigned long long first, unsigned n, const float* __restrict__ X2,
-                                const float* __restrict__ m2, const unsigned* __restrict__ key2,
-                                float* __restrict__ X, float* __restrict__ m,
-                                unsigned* __restrict__ node_of) {
-  const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  const unsigned long long dst = first + k;
-#pragma unroll
-  for (int j = 0; j < kIn; ++j) X[dst * kIn + j] = X2[(size_t)k * kIn + j];
-  m[dst] = m2[k];
-  node_of[dst] = key2[k];
+  for (int j = 0; j < kIn; ++j) X[slot * kIn + j] = phi[j];
+  node_of[slot] = gid;
+  m_of[slot] = (float)m;
 }
 
 __global__ void k_collect_targets(const double* __restrict__ alpha, const unsigned* __restrict__ node_of,
@@ -553,38 +588,12 @@ __global__ void k_pack_bf16(const float* __restrict__ P, short* __restrict__ Wbf
 hipError_t launch_mlp_collect(const Box& b, const float* fin, const uint8_t* flags, int stride,
                               int phase, MlpSamples& s, hipStream_t st) {
   const unsigned blocks = (unsigned)((b.n + 255) / 256);
+  if ((size_t)blocks > s.blk_n) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k27_collect_count, dim3(blocks), dim3(256), 0, st, fin, b, flags, stride,
+                     phase, s.ballots, s.blk);
+  hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(1024), 0, st, s.blk, blocks, s.count);
   hipLaunchKernelGGL(k27_collect, dim3(blocks), dim3(256), 0, st, fin, b, flags, stride, phase,
-                     s.X, s.node_of, s.m_of, s.count, (unsigned long long)s.cap);
-  return hipGetLastError();
-}
-
-hipError_t mlp_sort_bytes(int64_t cap, size_t* bytes) {
-  *bytes = 0;
-  return hipcub::DeviceRadixSort::SortPairs(nullptr, *bytes, (const unsigned*)nullptr,
-                                            (unsigned*)nullptr, (const unsigned*)nullptr,
-                                            (unsigned*)nullptr, (int)cap);
-}
-
-// The slots [first, last) filled by one collection, reordered by node: a
-// stable radix sort of (node, slot) and a permutation of the rows through the
-// staging buffer.  Nodes within one collection are distinct.
-hipError_t launch_mlp_order(MlpSamples& s, unsigned long long first, unsigned long long last,
-                            hipStream_t st) {
-  if (last <= first + 1) return hipSuccess;
-  if (last > (unsigned long long)s.cap || !s.sort_tmp) return hipErrorInvalidValue;
-  const unsigned n = (unsigned)(last - first);
-  unsigned* pin = s.perm;
-  unsigned* pout = s.perm + s.cap;
-  float* m2 = s.X2 + (size_t)s.cap * kIn;
-  const dim3 g((n + 255) / 256);
-  hipLaunchKernelGGL(k_iota, g, dim3(256), 0, st, pin, n);
-  size_t bytes = s.sort_bytes;
-  hipError_t e = hipcub::DeviceRadixSort::SortPairs(s.sort_tmp, bytes, s.node_of + first, s.key2,
-                                                    pin, pout, (int)n, 0, 32, st);
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(k_order_gather, g, dim3(256), 0, st, pout, first, n, s.X, s.m_of, s.X2, m2);
-  hipLaunchKernelGGL(k_order_scatter, g, dim3(256), 0, st, first, n, s.X2, m2, s.key2, s.X,
-                     s.m_of, s.node_of);
+                     s.ballots, s.blk, s.X, s.node_of, s.m_of, (unsigned long long)s.cap);
   return hipGetLastError();
 }
 

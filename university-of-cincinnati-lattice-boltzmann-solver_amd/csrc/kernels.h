@@ -217,13 +217,11 @@ struct MlpSamples {               // device-resident training pairs
   float* m_of = nullptr;          // its m
   unsigned long long* count = nullptr;
   int64_t cap = 0;
-  // ordering of each collected batch by node, so the training set does not
-  // depend on the order in which wavefronts claimed their slots
-  unsigned* key2 = nullptr;       // [cap] sorted nodes
-  unsigned* perm = nullptr;       // [2 cap] identity, then the sorting permutation
-  float* X2 = nullptr;            // [cap][12] features, then [cap] m: staging
-  void* sort_tmp = nullptr;       // radix-sort work space
-  size_t sort_bytes = 0;
+  // ordered placement of one collection (per wavefront ballots of taken
+  // nodes, per block counts then slot offsets), sized for blk_n blocks
+  unsigned long long* ballots = nullptr;  // [4 blk_n]
+  unsigned long long* blk = nullptr;      // [blk_n]
+  size_t blk_n = 0;
 };
 struct MlpWork {                  // activations of one training batch
   float *Z1, *A1, *Z2, *A2, *Z3, *A3, *Z4, *A4, *D1, *D2, *D3, *D4;
@@ -231,9 +229,6 @@ struct MlpWork {                  // activations of one training batch
 };
 hipError_t launch_mlp_collect(const Box& b, const float* fin, const uint8_t* flags, int stride,
                               int phase, MlpSamples& s, hipStream_t st);
-hipError_t mlp_sort_bytes(int64_t cap, size_t* bytes);
-hipError_t launch_mlp_order(MlpSamples& s, unsigned long long first, unsigned long long last,
-                            hipStream_t st);
 hipError_t launch_mlp_targets(const double* alpha, MlpSamples& s, unsigned long long first,
                               unsigned long long last, hipStream_t st);
 hipError_t launch_mlp_loss_grad(const float* P, const float* X, const float* y,

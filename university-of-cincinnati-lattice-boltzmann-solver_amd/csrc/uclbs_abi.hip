@@ -794,8 +794,7 @@ int uclbs_destroy(void* ctx) {
   for (void* p : {(void*)c->mlp_p, (void*)c->mlp_g, (void*)c->mlp_m, (void*)c->mlp_v,
                   (void*)c->mlp_wbf, (void*)c->samples.X, (void*)c->samples.y,
                   (void*)c->samples.node_of, (void*)c->samples.m_of, (void*)c->samples.count,
-                  (void*)c->samples.key2, (void*)c->samples.perm, (void*)c->samples.X2,
-                  c->samples.sort_tmp, (void*)c->work_buf, (void*)c->batch_x, (void*)c->batch_idx, (void*)c->d_scalar})
+                  (void*)c->samples.ballots, (void*)c->samples.blk, (void*)c->work_buf, (void*)c->batch_x, (void*)c->batch_idx, (void*)c->d_scalar})
     if (p) (void)hipFree(p);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -2869,7 +2868,7 @@ int uclbs_mlp_reserve(void* ctx, int64_t capacity) {
   if (rc) return rc;
   uclbs::MlpSamples& m = c->samples;
   for (void* p : {(void*)m.X, (void*)m.y, (void*)m.node_of, (void*)m.m_of, (void*)m.count,
-                  (void*)m.key2, (void*)m.perm, (void*)m.X2, m.sort_tmp})
+                  (void*)m.ballots, (void*)m.blk})
     if (p) (void)hipFree(p);
   m = uclbs::MlpSamples{};
   HIP_TRY(hipMalloc(&m.X, (size_t)capacity * 12 * sizeof(float)));
@@ -2878,11 +2877,6 @@ int uclbs_mlp_reserve(void* ctx, int64_t capacity) {
   HIP_TRY(hipMalloc(&m.m_of, (size_t)capacity * sizeof(float)));
   HIP_TRY(hipMalloc(&m.count, sizeof(unsigned long long)));
   HIP_TRY(hipMemset(m.count, 0, sizeof(unsigned long long)));
-  HIP_TRY(hipMalloc(&m.key2, (size_t)capacity * sizeof(unsigned)));
-  HIP_TRY(hipMalloc(&m.perm, (size_t)capacity * 2 * sizeof(unsigned)));
-  HIP_TRY(hipMalloc(&m.X2, (size_t)capacity * 13 * sizeof(float)));
-  HIP_TRY(uclbs::mlp_sort_bytes(capacity, &m.sort_bytes));
-  HIP_TRY(hipMalloc(&m.sort_tmp, std::max<size_t>(m.sort_bytes, 256)));
   m.cap = capacity;
   c->samples_normalised = false;
   return UCLBS_OK;
@@ -2907,6 +2901,18 @@ int uclbs_mlp_collect(void* ctx, int lvl, int stride, int64_t* n_samples) {
     return set_err(UCLBS_E_UNSUPPORTED, "samples come from the plain D3Q27 FP32-shifted solver");
   if (c->samples_normalised) return set_err(UCLBS_E_STATE, "samples already used for training");
   if ((rc = set_device(c))) return rc;
+  // ordering buffers for the level's launch (one entry per 256 nodes)
+  uclbs::MlpSamples& sm = c->samples;
+  const size_t nb = (size_t)((L->box.n + 255) / 256);
+  if (sm.blk_n < nb) {
+    if (sm.ballots) (void)hipFree(sm.ballots);
+    if (sm.blk) (void)hipFree(sm.blk);
+    sm.ballots = sm.blk = nullptr;
+    sm.blk_n = 0;
+    HIP_TRY(hipMalloc(&sm.ballots, nb * 4 * sizeof(unsigned long long)));
+    HIP_TRY(hipMalloc(&sm.blk, nb * sizeof(unsigned long long)));
+    sm.blk_n = nb;
+  }
   unsigned long long first = 0, last = 0;
   if ((rc = sample_count(c, &first))) return rc;
   HIP_TRY(uclbs::launch_mlp_collect(L->box, static_cast<const float*>(L->f[L->cur]),
@@ -2919,7 +2925,6 @@ int uclbs_mlp_collect(void* ctx, int lvl, int stride, int64_t* n_samples) {
   c->keep_alpha = keep;
   if (rc) return rc;
   if ((rc = sample_count(c, &last))) return rc;
-  HIP_TRY(uclbs::launch_mlp_order(c->samples, first, last, c->stream));
   HIP_TRY(uclbs::launch_mlp_targets(L->alpha, c->samples, first, last, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   *n_samples = (int64_t)last;
